@@ -1,0 +1,135 @@
+"""Golden-fixture cases: small systems that exercise every row of the solvent
+hot path (grid shift, binning order, cell sums, ghost particles, SRD ARBAXIS /
+ORTHAXIS, MPC-AT, MPC-AT+a, nematic LC with and without activity, thermostats,
+periodic and bounce-back planar walls, wall velocity, anchoring).
+
+Each case is recorded from the reference's own timestep() (oracle/_ref) by
+make_golden.py with injected sequential draws; tests replay the same initial
+state through the oracle and the GPU library.
+
+Wall dictionaries use the reference's bc field names (SRDclss.h:128-262);
+`domainWalls`-style periodic planes follow read.c:1441-1583.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def species(**kw):
+    sp = dict(mass=1.0, rfc=0.01, tumble=2.0, chihi=0.5, act=0.05, mfpot=10.0, damp=0.0, sigwidth=1.0,
+              sigpos=1.0, minactratio=0.0, vdist=4, len=0.007)
+    sp.update(kw)
+    return sp
+
+
+def pbc_walls(ax, L):
+    """the two periodic planes of axis `ax` (read.c:1459-1504, domainWalls == 1)"""
+    A1 = [0.0, 0.0, 0.0]
+    A1[ax] = 1.0
+    A2 = [0.0, 0.0, 0.0]
+    A2[ax] = -1.0
+    return [dict(A=A1, R=0.0, DN=float(L)), dict(A=A2, R=-float(L), DN=float(L))]
+
+
+def solid_walls(ax, L, **kw):
+    """the two no-slip bounce-back planes of axis `ax` (read.c:1514-1520, domainWalls == 2)"""
+    A1 = [0.0, 0.0, 0.0]
+    A1[ax] = 1.0
+    A2 = [0.0, 0.0, 0.0]
+    A2[ax] = -1.0
+    base = dict(DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1)
+    base.update(kw)
+    w1 = dict(A=A1, R=0.0, **base)
+    w2 = dict(A=A2, R=-float(L), **base)
+    return [w1, w2]
+
+
+def periodic(dim, L):
+    ws = []
+    for ax in range(dim):
+        ws += pbc_walls(ax, L[ax])
+    return ws
+
+
+def case(name, dim, L, ppc, rtech, steps, seed, lc=0, tstech=0, rotang=math.pi / 2, dt=0.1, kbt=1.0,
+         grav=(0, 0, 0), walls=None, sp=None, galinv=1, tau=0.5):
+    return dict(name=name, dim=dim, L=list(L) + [1] * (3 - len(L)), ppc=ppc, rtech=rtech, steps=steps,
+                seed=seed, lc=lc, tstech=tstech, rotang=rotang, dt=dt, kbt=kbt, grav=list(grav),
+                walls=walls if walls is not None else periodic(dim, L), species=[sp or species()],
+                galinv=galinv, tau=tau)
+
+
+CASES = [
+    # 2D SRD, ORTHAXIS, 90 degrees (the reference's CPU-runnable configuration, scaled down)
+    case("srd2d_orth", 2, (8, 8), 10, rtech=1, steps=4, seed=11),
+    # 2D SRD, ARBAXIS, gravity (acc_all before the cell sums), thermostat VSC
+    case("srd2d_arb_grav_vsc", 2, (10, 6), 8, rtech=0, steps=4, seed=12, grav=(0.01, -0.02, 0), tstech=1),
+    # 3D SRD, ARBAXIS, 130 degrees
+    case("srd3d_arb", 3, (6, 5, 4), 8, rtech=0, steps=3, seed=13, rotang=2.2689280275926285),
+    # 3D SRD, ORTHAXIS (keeps the reference's x-axis matrix entry -0.1*S)
+    case("srd3d_orth", 3, (5, 5, 5), 6, rtech=1, steps=3, seed=14),
+    # 3D MPC-AT with damping
+    case("at3d_damp", 3, (5, 4, 6), 7, rtech=2, steps=3, seed=15, sp=species(damp=0.1)),
+    # 3D MPC-AT+a channel: z bounce-back walls with ghost particles, gravity in x, Berendsen
+    case("rat3d_channel", 3, (6, 5, 6), 8, rtech=3, steps=4, seed=16, grav=(1e-3, 0, 0), dt=0.05, tstech=2,
+         walls=pbc_walls(0, 6) + pbc_walls(1, 5) + solid_walls(2, 6)),
+    # 2D MPC-AT+a periodic, two species of different mass
+    case("rat2d_2sp", 2, (9, 7), 9, rtech=3, steps=4, seed=17),
+    # 2D active nematic (DIPOLE_DIR_SUM), local S
+    case("lc2d_active", 2, (8, 8), 12, rtech=16, lc=1, steps=3, seed=18,
+         sp=species(mfpot=10.0, tumble=0.5, rfc=0.01, act=0.05)),
+    # 2D active nematic channel: y walls, homeotropic strong anchoring, gravity; DIPOLE_DIR_SIG_SUM
+    case("lc2d_channel", 2, (10, 6), 12, rtech=21, lc=1, steps=3, seed=19, grav=(5e-4, 0, 0),
+         sp=species(mfpot=10.0, tumble=0.5, rfc=0.01, act=0.03),
+         walls=pbc_walls(0, 10) + solid_walls(1, 6, MUN=1.0, MUT=0.0)),
+    # 2D passive nematic with global S (lc = 2)
+    case("lcg2d_passive", 2, (7, 7), 10, rtech=3, lc=2, steps=3, seed=20, sp=species(mfpot=5.0, tumble=1.5)),
+    # 3D nematic, strong field: Gaussian Maier-Saupe sampler
+    case("lc3d_gauss", 3, (4, 4, 4), 10, rtech=3, lc=1, steps=3, seed=21, sp=species(mfpot=12.0)),
+    # 3D nematic, weak field: exponential and Metropolis samplers; DIPOLE_DIR_AV
+    case("lc3d_weak", 3, (4, 4, 3), 10, rtech=17, lc=1, steps=3, seed=22, sp=species(mfpot=0.6)),
+    # 2D SRD Couette: bounce-back walls in y, the top one moving (wall velocity in velBC)
+    case("srd2d_couette", 2, (8, 6), 10, rtech=0, steps=4, seed=23,
+         walls=pbc_walls(0, 8) + [dict(A=[0, 1, 0], R=0.0, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1),
+                                  dict(A=[0, -1, 0], R=-6.0, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1,
+                                       V=[0.2, 0, 0])]),
+]
+
+# rat2d_2sp uses two species
+CASES[6]["species"] = [species(mass=1.0), species(mass=2.5)]
+
+
+def initial_state(c: dict):
+    """uniform positions, Maxwell-Boltzmann velocities, random orientations"""
+    rng = np.random.default_rng(1000 + c["seed"])
+    dim = c["dim"]
+    L = c["L"]
+    vol = int(np.prod(L[:dim]))
+    N = vol * c["ppc"]
+    Q = np.zeros((N, 3))
+    V = np.zeros((N, 3))
+    U = np.zeros((N, 3))
+    nsp = len(c["species"])
+    spid = (np.arange(N) % nsp).astype(np.int32)
+    for d in range(dim):
+        Q[:, d] = rng.uniform(0.0, L[d], N)
+    masses = np.array([s["mass"] for s in c["species"]])[spid]
+    V[:, :dim] = rng.normal(size=(N, dim)) * np.sqrt(c["kbt"] / masses)[:, None]
+    if dim == 3:
+        u = rng.normal(size=(N, 3))
+        U = u / np.linalg.norm(u, axis=1)[:, None]
+    else:
+        th = rng.uniform(0, 2 * np.pi, N)
+        U[:, 0] = np.cos(th)
+        U[:, 1] = np.sin(th)
+    avv = np.zeros(3)
+    avv[:dim] = V[:, :dim].mean(axis=0)
+    return Q, V, U, spid, avv
+
+
+def oracle_cfg(c: dict, gndnst: float, avv) -> dict:
+    return dict(dim=c["dim"], L=c["L"], dt=c["dt"], kbt=c["kbt"], rotang=c["rotang"], tau=c["tau"],
+                grav=c["grav"], avv=list(avv), gndnst=gndnst, rtech=c["rtech"], lc=c["lc"], tstech=c["tstech"],
+                galinv=c["galinv"], species=c["species"], walls=c["walls"])
