@@ -1,0 +1,184 @@
+/*
+ * mpcd_gpu.h -- C-ABI of the MI355X (gfx950) MPCD solvent library.
+ *
+ * Drop-in boundary for the per-timestep solvent path of NIAMH-MPCD.  The
+ * reference has no plugin or FFI layer; its seam is the sequence of calls
+ * inside timestep() (mpcd/subroutines/mpc.c:4567-4942) made from main()
+ * (mpcd/mpcd.c:268,298).  The entry points below replace those calls:
+ *
+ *   mpcd_gpu_bin      ranshift + gridShift_all(+)   mpc.c:4567-4575
+ *                     bin(shifted=1)                 mpc.c:4596 (bin: 948-992)
+ *   mpcd_gpu_collide  acc_all                        mpc.c:4621 (697-700)
+ *                     localPROP                      mpc.c:4630 (64-216)
+ *                     ghostPart                      mpc.c:4634 (351-567)
+ *                     LCcollision, jefferysTorque    mpc.c:4640-4678 (lc.c:427-718)
+ *                     MPCcollision per cell          mpc.c:4699-4714 (2777-2812)
+ *                     TEMP, scaleT                   mpc.c:4752-4778 (therm.c:52-235)
+ *                     gridShift_all(-)               mpc.c:4793-4799
+ *   mpcd_gpu_stream   stream_all                     mpc.c:4815-4817 (680-687)
+ *                     MPC_BCcollision per particle   mpc.c:4834 (bc.c:1363-1496)
+ *                     bin(shifted=0) + localPROP     mpc.c:4936-4942
+ *
+ * Plain C types only: no C++ or torch types cross this boundary.  All device
+ * memory is owned by the context; host arrays are borrowed for the duration
+ * of a call.  Calls on one context are stream ordered; a call that returns
+ * host data synchronises.  One context may hold many independent replicas
+ * (different boxes, densities, walls, seeds) that advance together.
+ *
+ * Injected draws (parity mode).  When a step is given an mpcd_draws, every
+ * random variate is read from host-provided arrays in the keyed layout below
+ * instead of the internal counter-based generator (Philox-4x32-10 keyed by
+ * seed, replica, step, purpose, key, k).  Keys:
+ *   shift   [3 per replica]            x, y, z grid shift in [0,1)   (rand.c:678-696)
+ *   cell    [4 per cell]               SRD: 3D ARBAXIS unit axis in [0..2] (ranvec3D),
+ *                                      2D ARBAXIS / ORTHAXIS uniform in [0],
+ *                                      rotation-sign uniform in [3]  (mpc.c:1286-1291)
+ *   ghost   [3 per cell]               ghost-particle normals        (mpc.c:553)
+ *   part    [3 per particle slot]      thermal normals of MPC-AT(+a) and LC collisions
+ *                                      (mpc.c:1382, 1506; lc.c:1559, 1854)
+ *   lc      [lc_stride per slot]       Maier-Saupe sampler variates, slot 0 = 2D angle
+ *                                      redraw, 1+2i / 2+2i = trial / acceptance of
+ *                                      anneal step i, then phi and sign (3D)  (lc.c:52-411)
+ * Cells are numbered x-major over (L0+1) x (L1+1) x (L2+1) (z extent 1 in 2D),
+ * concatenated over replicas; particle slots are positions in the collision
+ * order (cells x-major, each cell in the reference's linked-list order),
+ * concatenated over replicas.  Normals are standard normals (the library
+ * multiplies by sqrt(kT/m)); uniforms lie in [0,1).
+ */
+#ifndef MPCD_GPU_H
+#define MPCD_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPCD_MAXSPEC 10
+#define MPCD_MAXWALL 16
+
+typedef enum {
+    MPCD_OK = 0,
+    MPCD_EINVAL = 1,        /* bad argument (reference: printf + exit(1)) */
+    MPCD_ENOMEM = 2,        /* device allocation failed */
+    MPCD_EDEVICE = 3,       /* HIP runtime error */
+    MPCD_EUNSUPPORTED = 4,  /* option outside this path (e.g. mpc.c:2806 "Collision technique unacceptable") */
+    MPCD_EESCAPE = 5,       /* particles escaped the box and were re-placed (mpc.c:970-982) */
+    MPCD_ECAPACITY = 6      /* a cell exceeded the library's per-cell population limit */
+} mpcd_status;
+
+/* collision operators (definitions.h:261-303) supported on this path */
+enum { MPCD_ARBAXIS = 0, MPCD_ORTHAXIS = 1, MPCD_MPCAT = 2, MPCD_RAT = 3, MPCD_DIPOLE_DIR_SUM = 16,
+       MPCD_DIPOLE_DIR_AV = 17, MPCD_DIPOLE_DIR_SIG = 20, MPCD_DIPOLE_DIR_SIG_SUM = 21 };
+/* liquid-crystal modes (definitions.h:194-200) and thermostats (definitions.h:238-246) */
+enum { MPCD_ISOF = 0, MPCD_LCL = 1, MPCD_LCG = 2 };
+enum { MPCD_NOTHERM = 0, MPCD_VSC = 1, MPCD_BEREND = 2 };
+/* state precision */
+enum { MPCD_FP64 = 0, MPCD_FP32 = 1 };
+
+/* species -- subset of the reference's spec (SRDclss.h:54-77) */
+typedef struct {
+    double mass, rfc, tumble, chihi, act, mfpot, damp, sigwidth, sigpos, minactratio;
+    int vdist; /* velocity distribution used by wall re-thermalisation (push(), init.c:1261) */
+    int pad_;
+} mpcd_gpu_species;
+
+/* axis-aligned planar wall -- subset of the reference's bc (SRDclss.h:128-262);
+ * periodic planes have DN = box length, MVN = MVT = 1; bounce-back planes
+ * DN = 0, MVN = MVT = -1, PHANTOM = 1 (read.c:1441-1583) */
+typedef struct {
+    double A[3], Q[3], R, DN, DT, DVN, DVT, DVxyz[3], MVN, MVT, MUN, MUT, MUxyz[3], DUxyz[3], V[3], L[3];
+    int INV, PHANTOM, COLL_TYPE;
+    int inter_mask; /* bit s set: species s interacts with this wall (bc.INTER, SRDclss.h:238) */
+} mpcd_gpu_wall;
+
+/* one replica's parameters -- subset of inputList (SRDclss.h:395-424) */
+typedef struct {
+    int dim;               /* 2 or 3 */
+    int L[3];              /* box, cell size 1 (XYZ); L[2] ignored in 2D */
+    double dt, kbt;
+    double rotang;         /* SRD rotation angle RA; the library uses cos/sin computed on the host */
+    double tau;            /* Berendsen relaxation time (TAU) */
+    double grav[3];        /* body force (GRAV) */
+    double avv[3];         /* reference velocity of TEMP (AVV, therm.c:209) */
+    double gndnst;         /* global number density GnDNST (init.c:1002) */
+    int rtech, lc, tstech, galinv;
+    int true_random_sign;  /* 0: the reference's always-negative SRD angle (mpc.c:1291) */
+    int precision;         /* MPCD_FP64 or MPCD_FP32 particle state */
+    int nspec, nwall;
+    mpcd_gpu_species sp[MPCD_MAXSPEC];
+    mpcd_gpu_wall wall[MPCD_MAXWALL];
+} mpcd_gpu_params;
+
+typedef struct {
+    const double *shift;   /* [3 * nrep] */
+    const double *cell;    /* [4 * ncell_total] */
+    const double *ghost;   /* [3 * ncell_total] */
+    const double *part;    /* [3 * n_total] */
+    const double *lc;      /* [lc_stride * n_total] */
+    int lc_stride;
+} mpcd_draws;
+
+typedef struct mpcd_gpu_ctx mpcd_gpu_ctx;
+
+/* --- lifetime ------------------------------------------------------------ */
+/* A context holding `nrep` replicas; params[r] and n_particles[r] describe
+ * replica r.  dim, rtech, lc, precision and the species table must agree
+ * across replicas.  Replaces the reference's readJson()+initializeSIM()
+ * storage (read.c:1699-1719) for the solvent. */
+mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, const int64_t *n_particles, uint64_t seed,
+                            mpcd_gpu_ctx **out);
+void        mpcd_gpu_destroy(mpcd_gpu_ctx *ctx);
+const char *mpcd_gpu_last_error(const mpcd_gpu_ctx *ctx);
+
+/* --- state transfer (particleMPC array, SRDclss.h:34-44) ------------------ */
+/* Q, V, U: [3*n] in particle-index order; U and spid may be NULL.  Performs
+ * the initial binning and cell properties (binin + localPROP,
+ * init.c:2114-2118). */
+mpcd_status mpcd_gpu_upload(mpcd_gpu_ctx *ctx, int replica, const double *Q, const double *V, const double *U,
+                            const int32_t *spid);
+mpcd_status mpcd_gpu_download(mpcd_gpu_ctx *ctx, int replica, double *Q, double *V, double *U);
+
+/* --- the step, phase by phase (see the table at the top) ------------------ */
+/* shifted = 1: draw (or take) the grid shift and bin the shifted positions.
+ * shifted = 0: materialise the unshifted binning left by mpcd_gpu_stream.
+ * Optional outputs for replica 0..nrep-1 concatenated: cell_out [n_total]
+ * cell of every particle (particle-index order per replica), count_out
+ * [ncell_total] populations, perm_out [n_total] particle indices in list
+ * order. */
+mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *ctx, const mpcd_draws *draws, int shifted, int32_t *cell_out,
+                         int32_t *count_out, int32_t *perm_out);
+/* kbt_now_out: [nrep] temperature after the thermostat (KBTNOW), may be NULL */
+mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *ctx, const mpcd_draws *draws, double *kbt_now_out);
+/* bc_event_counts_out: [4] escapes, particles that exceeded NBOUNCE, rewinds,
+ * re-thermalisations (bc.c:1364-1368), may be NULL */
+mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *ctx, int64_t *bc_event_counts_out);
+
+/* --- whole steps ------------------------------------------------------------ */
+/* one full step (bin + collide + stream); draws may be NULL (internal RNG) */
+mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *ctx, const mpcd_draws *draws, double *kbt_now_out);
+/* nsteps steps with the internal RNG and no host synchronisation; with
+ * use_graph != 0 the step is captured once in a hipGraph and replayed */
+mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *ctx, int nsteps, int use_graph);
+mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *ctx);
+
+/* --- observables ------------------------------------------------------------ */
+/* per-cell populations, centre-of-mass velocities [3 per cell], scalar order
+ * parameter and director [3 per cell] of replica `replica` (localPROP
+ * results, mpc.c:64-216); any pointer may be NULL */
+mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *ctx, int replica, int32_t *pop, double *vcm, double *S,
+                               double *dir);
+/* list order of replica `replica` after the last phase: order [n] particle
+ * indices, cstart [ncell+1] cell offsets */
+mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *ctx, int replica, int32_t *order, int32_t *cstart);
+/* number of cells of replica `replica` and the keyed LC stride */
+int         mpcd_gpu_ncells(const mpcd_gpu_ctx *ctx, int replica);
+int         mpcd_gpu_lc_stride(const mpcd_gpu_ctx *ctx);
+/* device time of the last mpcd_gpu_run's kernels by kernel family, ms */
+mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *ctx, double *ms_out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPCD_GPU_H */

@@ -1,0 +1,293 @@
+"""niamh-mpcd_amd -- MI355X (gfx950) MPCD solvent library, Python side.
+
+The product is the C-ABI shared library built from csrc/ (libmpcd_gpu.so,
+entry points declared in include/mpcd_gpu.h).  This module builds it with
+hipcc for gfx950 and binds it with ctypes so Python callers (tests, bench.py,
+the calibration harness) drive exactly the entry points a C host would.
+
+There is no CPU fallback: every compute call goes through the HIP library,
+and loading fails loudly if it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB_PATH = os.path.join(HERE, "_build", "libmpcd_gpu.so")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # FP64 parity with the reference needs the reference's (unfused) operation order
+    "-ffp-contract=off",
+    # packed FP32 instructions stay off on this target
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+
+MAXSPEC = 10
+MAXWALL = 16
+
+# collision operators, LC modes, thermostats, precisions (include/mpcd_gpu.h)
+ARBAXIS, ORTHAXIS, MPCAT, RAT = 0, 1, 2, 3
+DIPOLE_DIR_SUM, DIPOLE_DIR_AV, DIPOLE_DIR_SIG, DIPOLE_DIR_SIG_SUM = 16, 17, 20, 21
+ISOF, LCL, LCG = 0, 1, 2
+NOTHERM, VSC, BEREND = 0, 1, 2
+FP64, FP32 = 0, 1
+
+STATUS = {0: "MPCD_OK", 1: "MPCD_EINVAL", 2: "MPCD_ENOMEM", 3: "MPCD_EDEVICE", 4: "MPCD_EUNSUPPORTED",
+          5: "MPCD_EESCAPE", 6: "MPCD_ECAPACITY"}
+
+
+def sources():
+    return [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile csrc/*.hip into _build/libmpcd_gpu.so for gfx950 (always recompiles)."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, *sources(), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+class Species(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("mass", "rfc", "tumble", "chihi", "act", "mfpot", "damp",
+                                          "sigwidth", "sigpos", "minactratio")] + [("vdist", C.c_int),
+                                                                                    ("pad_", C.c_int)]
+
+
+class Wall(C.Structure):
+    _fields_ = [("A", C.c_double * 3), ("Q", C.c_double * 3), ("R", C.c_double), ("DN", C.c_double),
+                ("DT", C.c_double), ("DVN", C.c_double), ("DVT", C.c_double), ("DVxyz", C.c_double * 3),
+                ("MVN", C.c_double), ("MVT", C.c_double), ("MUN", C.c_double), ("MUT", C.c_double),
+                ("MUxyz", C.c_double * 3), ("DUxyz", C.c_double * 3), ("V", C.c_double * 3),
+                ("L", C.c_double * 3), ("INV", C.c_int), ("PHANTOM", C.c_int), ("COLL_TYPE", C.c_int),
+                ("inter_mask", C.c_int)]
+
+
+class Params(C.Structure):
+    _fields_ = [("dim", C.c_int), ("L", C.c_int * 3), ("dt", C.c_double), ("kbt", C.c_double),
+                ("rotang", C.c_double), ("tau", C.c_double), ("grav", C.c_double * 3), ("avv", C.c_double * 3),
+                ("gndnst", C.c_double), ("rtech", C.c_int), ("lc", C.c_int), ("tstech", C.c_int),
+                ("galinv", C.c_int), ("true_random_sign", C.c_int), ("precision", C.c_int),
+                ("nspec", C.c_int), ("nwall", C.c_int), ("sp", Species * MAXSPEC), ("wall", Wall * MAXWALL)]
+
+
+class DrawsC(C.Structure):
+    _fields_ = [("shift", C.POINTER(C.c_double)), ("cell", C.POINTER(C.c_double)),
+                ("ghost", C.POINTER(C.c_double)), ("part", C.POINTER(C.c_double)),
+                ("lc", C.POINTER(C.c_double)), ("lc_stride", C.c_int)]
+
+
+EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_gpu_upload", "mpcd_gpu_download",
+           "mpcd_gpu_bin", "mpcd_gpu_collide", "mpcd_gpu_stream", "mpcd_gpu_step", "mpcd_gpu_run",
+           "mpcd_gpu_synchronize", "mpcd_gpu_get_cells", "mpcd_gpu_get_order", "mpcd_gpu_ncells",
+           "mpcd_gpu_lc_stride", "mpcd_gpu_timing"]
+
+_lib = None
+
+
+def lib():
+    """Load the built library; raises if it is missing (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    L.mpcd_gpu_create.argtypes = [C.POINTER(Params), C.c_int, i64p, C.c_uint64, C.POINTER(vp)]
+    L.mpcd_gpu_destroy.argtypes = [vp]
+    L.mpcd_gpu_last_error.restype = C.c_char_p
+    L.mpcd_gpu_last_error.argtypes = [vp]
+    L.mpcd_gpu_upload.argtypes = [vp, C.c_int, dp, dp, dp, ip]
+    L.mpcd_gpu_download.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.mpcd_gpu_bin.argtypes = [vp, C.POINTER(DrawsC), C.c_int, ip, ip, ip]
+    L.mpcd_gpu_collide.argtypes = [vp, C.POINTER(DrawsC), dp]
+    L.mpcd_gpu_stream.argtypes = [vp, i64p]
+    L.mpcd_gpu_step.argtypes = [vp, C.POINTER(DrawsC), dp]
+    L.mpcd_gpu_run.argtypes = [vp, C.c_int, C.c_int]
+    L.mpcd_gpu_synchronize.argtypes = [vp]
+    L.mpcd_gpu_get_cells.argtypes = [vp, C.c_int, ip, dp, dp, dp]
+    L.mpcd_gpu_get_order.argtypes = [vp, C.c_int, ip, ip]
+    L.mpcd_gpu_ncells.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_lc_stride.argtypes = [vp]
+    L.mpcd_gpu_timing.argtypes = [vp, dp, C.c_int]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+def make_params(cfg: dict, precision: int = FP64) -> Params:
+    """Parameters from a dict using the reference's names (inputList / spec / bc)."""
+    p = Params()
+    p.dim = int(cfg["dim"])
+    for i in range(3):
+        p.L[i] = int(cfg["L"][i]) if i < len(cfg["L"]) else 1
+        p.grav[i] = float(cfg.get("grav", (0, 0, 0))[i])
+        p.avv[i] = float(cfg.get("avv", (0, 0, 0))[i])
+    p.dt = float(cfg.get("dt", 0.1))
+    p.kbt = float(cfg.get("kbt", 1.0))
+    p.rotang = float(cfg.get("rotang", 1.570796))
+    p.tau = float(cfg.get("tau", 0.5))
+    p.gndnst = float(cfg.get("gndnst", 0.0))
+    p.rtech = int(cfg.get("rtech", MPCAT))
+    p.lc = int(cfg.get("lc", ISOF))
+    p.tstech = int(cfg.get("tstech", NOTHERM))
+    p.galinv = int(cfg.get("galinv", 1))
+    p.true_random_sign = int(cfg.get("true_random_sign", 0))
+    p.precision = int(precision)
+    sps = cfg["species"]
+    p.nspec = len(sps)
+    for i, sp in enumerate(sps):
+        for k in ("mass", "rfc", "tumble", "chihi", "act", "mfpot", "damp", "sigwidth", "sigpos", "minactratio"):
+            setattr(p.sp[i], k, float(sp.get(k, 0.0)))
+        p.sp[i].vdist = int(sp.get("vdist", 4))
+    ws = cfg["walls"]
+    p.nwall = len(ws)
+    for i, w in enumerate(ws):
+        W = p.wall[i]
+        for k in ("A", "Q", "DVxyz", "MUxyz", "DUxyz", "V", "L"):
+            v = w.get(k, (1, 1, 1) if k == "MUxyz" else (0, 0, 0))
+            for j in range(3):
+                getattr(W, k)[j] = float(v[j])
+        for k, dv in (("R", 0), ("DN", 0), ("DT", 0), ("DVN", 0), ("DVT", 0), ("MVN", 1), ("MVT", 1),
+                      ("MUN", 1), ("MUT", 1)):
+            setattr(W, k, float(w.get(k, dv)))
+        W.INV = int(w.get("INV", 0))
+        W.PHANTOM = int(w.get("PHANTOM", 0))
+        W.COLL_TYPE = int(w.get("COLL_TYPE", 1))
+        W.inter_mask = int(w.get("inter_mask", (1 << MAXSPEC) - 1))
+    return p
+
+
+class MPCDError(RuntimeError):
+    pass
+
+
+class Context:
+    """One GPU context holding one or more replicas (include/mpcd_gpu.h)."""
+
+    def __init__(self, cfgs, n_particles, seed: int = 1, precision: int = FP64):
+        if isinstance(cfgs, dict):
+            cfgs = [cfgs]
+            n_particles = [n_particles]
+        self.L = lib()
+        self.cfgs = list(cfgs)
+        self.nrep = len(self.cfgs)
+        self.params = (Params * self.nrep)(*[make_params(c, precision) for c in self.cfgs])
+        self.n = np.asarray(n_particles, dtype=np.int64)
+        h = C.c_void_p()
+        st = self.L.mpcd_gpu_create(self.params, self.nrep, self.n.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    C.c_uint64(seed), C.byref(h))
+        if st != 0:
+            raise MPCDError(f"mpcd_gpu_create failed: {STATUS.get(st, st)}")
+        self.h = h
+        self.ncells = [self.L.mpcd_gpu_ncells(self.h, r) for r in range(self.nrep)]
+        self.lc_stride = self.L.mpcd_gpu_lc_stride(self.h)
+
+    def _check(self, st, what):
+        if st not in (0,):
+            msg = self.L.mpcd_gpu_last_error(self.h).decode()
+            raise MPCDError(f"{what}: {STATUS.get(st, st)} {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mpcd_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, r, Q, V, U=None, spid=None):
+        Q = np.ascontiguousarray(Q, np.float64)
+        V = np.ascontiguousarray(V, np.float64)
+        U = None if U is None else np.ascontiguousarray(U, np.float64)
+        spid = None if spid is None else np.ascontiguousarray(spid, np.int32)
+        self._check(self.L.mpcd_gpu_upload(self.h, r, _dp(Q), _dp(V), _dp(U), _ip(spid)), "upload")
+
+    def download(self, r):
+        n = int(self.n[r])
+        Q, V, U = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self.L.mpcd_gpu_download(self.h, r, _dp(Q), _dp(V), _dp(U)), "download")
+        return Q, V, U
+
+    @staticmethod
+    def _draws(d):
+        if d is None:
+            return None, None
+        keep = {k: (None if d.get(k) is None else np.ascontiguousarray(d[k], np.float64))
+                for k in ("shift", "cell", "ghost", "part", "lc")}
+        s = DrawsC(_dp(keep["shift"]), _dp(keep["cell"]), _dp(keep["ghost"]), _dp(keep["part"]), _dp(keep["lc"]),
+                   int(d.get("lc_stride", 0)))
+        return s, keep
+
+    def bin(self, draws=None, shifted=1, outputs=False):
+        s, keep = self._draws(draws)
+        ntot, nct = int(self.n.sum()), int(sum(self.ncells))
+        cell = np.zeros(ntot, np.int32) if outputs else None
+        cnt = np.zeros(nct, np.int32) if outputs else None
+        perm = np.zeros(ntot, np.int32) if outputs else None
+        self._check(self.L.mpcd_gpu_bin(self.h, C.byref(s) if s else None, shifted, _ip(cell), _ip(cnt), _ip(perm)),
+                    "bin")
+        return (cell, cnt, perm) if outputs else None
+
+    def collide(self, draws=None):
+        s, keep = self._draws(draws)
+        kbt = np.zeros(self.nrep)
+        self._check(self.L.mpcd_gpu_collide(self.h, C.byref(s) if s else None, _dp(kbt)), "collide")
+        return kbt
+
+    def stream(self):
+        ev = np.zeros(4, np.int64)
+        st = self.L.mpcd_gpu_stream(self.h, ev.ctypes.data_as(C.POINTER(C.c_int64)))
+        if st not in (0, 5):
+            self._check(st, "stream")
+        return ev
+
+    def step(self, draws=None):
+        s, keep = self._draws(draws)
+        kbt = np.zeros(self.nrep)
+        self._check(self.L.mpcd_gpu_step(self.h, C.byref(s) if s else None, _dp(kbt)), "step")
+        return kbt
+
+    def run(self, nsteps, use_graph=True):
+        self._check(self.L.mpcd_gpu_run(self.h, int(nsteps), int(bool(use_graph))), "run")
+
+    def synchronize(self):
+        self._check(self.L.mpcd_gpu_synchronize(self.h), "synchronize")
+
+    def cells(self, r=0):
+        nc = self.ncells[r]
+        pop, vcm, S, d = np.zeros(nc, np.int32), np.zeros((nc, 3)), np.zeros(nc), np.zeros((nc, 3))
+        self._check(self.L.mpcd_gpu_get_cells(self.h, r, _ip(pop), _dp(vcm), _dp(S), _dp(d)), "get_cells")
+        return dict(pop=pop, vcm=vcm, S=S, dir=d)
+
+    def order(self, r=0):
+        o = np.zeros(int(self.n[r]), np.int32)
+        cs = np.zeros(self.ncells[r] + 1, np.int32)
+        self._check(self.L.mpcd_gpu_get_order(self.h, r, _ip(o), _ip(cs)), "get_order")
+        return o, cs

@@ -1,0 +1,759 @@
+// mpcd_gpu.hip -- C-ABI of the MPCD solvent library for gfx950 (see
+// include/mpcd_gpu.h for the entry points and the reference calls they
+// replace).  Host orchestration: device buffers, block tables, the per-phase
+// kernel sequences, and the hipGraph used by mpcd_gpu_run.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "mpcd_kernels.h"
+
+using namespace mpcd;
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+};
+
+template <typename Real>
+struct StateBufs {
+  State<Real> s{};
+};
+
+}  // namespace
+
+struct mpcd_gpu_ctx {
+  int nrep = 0, dim = 0, lc = 0, rtech = 0, precision = 0;
+  int ntot = 0, nctot = 0, nwtot = 0;
+  uint64_t seed = 0;
+  uint32_t step = 0;
+  std::string err;
+  std::vector<mpcd_gpu_params> params;
+  std::vector<RepDev> reps;
+  std::vector<BlkDev> pblk, cblk;  // slot blocks, cell blocks
+  std::vector<int> uploaded;
+  bool need_init = true;
+  bool pending = false;  // an unshifted bin is implied by cu[] and not materialised
+  int cur = 0;
+  bool any_therm = false, any_phantom = false, any_lcg = false;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+
+  // device memory
+  std::vector<void *> allocs;
+  RepDev *d_rep = nullptr;
+  BlkDev *d_pblk = nullptr, *d_cblk = nullptr;
+  mpcd_gpu_species *d_sp = nullptr;
+  mpcd_gpu_wall *d_wall = nullptr;
+  Ctx C{};
+  State<double> sd[2]{};
+  State<float> sf[2]{};
+  double *d_avpart = nullptr;
+  // injected draws staging
+  double *d_dshift = nullptr, *d_dcell = nullptr, *d_dghost = nullptr, *d_dpart = nullptr, *d_dlc = nullptr;
+  int dlc_stride = 0;
+  size_t dlc_cap = 0;
+  int lc_stride = 0;
+  // staging for upload/download
+  double *d_q = nullptr, *d_v = nullptr, *d_u = nullptr;
+  int32_t *d_spid = nullptr, *d_i32 = nullptr;
+  size_t stage_cap = 0;
+  std::vector<int32_t *> scan_lv;  // scan scratch, one pair per level
+};
+
+static mpcd_status fail(mpcd_gpu_ctx *c, mpcd_status st, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return st;
+}
+
+#define HIPCHK(ctx, call)                                                                       \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) return fail(ctx, MPCD_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static T *dalloc(mpcd_gpu_ctx *c, size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+  hipMemset(p, 0, n * sizeof(T));
+  c->allocs.push_back(p);
+  return (T *)p;
+}
+
+static int anneal_stride(const mpcd_gpu_params &p) {
+  double effm = 0.0;
+  for (int i = 0; i < p.nspec; i++)
+    if (p.sp[i].mfpot > effm) effm = p.sp[i].mfpot;
+  effm = effm * 0.5 * (double)p.dim;
+  int amax = (int)(kMCINT + kMCSLOPE * effm * 1.0 / p.kbt) + 1;
+  return 3 + 2 * amax;
+}
+
+template <typename Real>
+static State<Real> alloc_state(mpcd_gpu_ctx *c, int n, bool lc) {
+  State<Real> s{};
+  s.x = dalloc<Real>(c, n);
+  s.y = dalloc<Real>(c, n);
+  s.z = dalloc<Real>(c, n);
+  s.vx = dalloc<Real>(c, n);
+  s.vy = dalloc<Real>(c, n);
+  s.vz = dalloc<Real>(c, n);
+  if (lc) {
+    s.ux = dalloc<Real>(c, n);
+    s.uy = dalloc<Real>(c, n);
+    s.uz = dalloc<Real>(c, n);
+  }
+  s.id = dalloc<int32_t>(c, n);
+  s.sp = dalloc<int32_t>(c, n);
+  s.cst = dalloc<int32_t>(c, n);
+  return s;
+}
+
+template <typename Real>
+static bool state_ok(const State<Real> &s, bool lc) {
+  return s.x && s.y && s.z && s.vx && s.vy && s.vz && s.id && s.sp && s.cst && (!lc || (s.ux && s.uy && s.uz));
+}
+
+static void set_pbc(const mpcd_gpu_params &p, int pbc[3]) {
+  // setPBC, read.c:584-603
+  for (int i = 0; i < 3; i++) pbc[i] = 0;
+  for (int w = 0; w < p.nwall; w++) {
+    const mpcd_gpu_wall &W = p.wall[w];
+    for (int i = 0; i < 3; i++) {
+      double ainv = (W.A[i] != 0.0) ? 1.0 / W.A[i] : 0.0;
+      if (feq(fabs(ainv), 1.0) && W.MVN > 0.0 && W.MVT > 0.0 && (fneq(W.DN, 0.0) || fneq(W.DT, 0.0))) pbc[i] = 1;
+    }
+  }
+}
+
+extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, const int64_t *n_particles,
+                                       uint64_t seed, mpcd_gpu_ctx **out) {
+  if (!params || !n_particles || !out || nrep <= 0) return MPCD_EINVAL;
+  *out = nullptr;
+  mpcd_gpu_ctx *c = new mpcd_gpu_ctx();
+  c->nrep = nrep;
+  c->seed = seed;
+  c->dim = params[0].dim;
+  c->lc = params[0].lc;
+  c->rtech = params[0].rtech;
+  c->precision = params[0].precision;
+  c->params.assign(params, params + nrep);
+  auto bad = [&](mpcd_status st, const char *m) {
+    fprintf(stderr, "mpcd_gpu_create: %s\n", m);
+    delete c;
+    return st;
+  };
+  // validation: options on this path
+  int64_t ntot = 0, nctot = 0;
+  for (int r = 0; r < nrep; r++) {
+    const mpcd_gpu_params &p = params[r];
+    if (p.dim != c->dim || p.lc != c->lc || p.rtech != c->rtech || p.precision != c->precision)
+      return bad(MPCD_EINVAL, "dim, lc, rtech and precision must agree across replicas");
+    if (p.dim != 2 && p.dim != 3) return bad(MPCD_EINVAL, "dim must be 2 or 3");
+    if (p.nspec < 1 || p.nspec > MPCD_MAXSPEC || p.nwall < 0 || p.nwall > MPCD_MAXWALL)
+      return bad(MPCD_EINVAL, "species or wall count out of range");
+    if (memcmp(p.sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC) != 0 || p.nspec != params[0].nspec)
+      return bad(MPCD_EINVAL, "the species table must agree across replicas");
+    if (p.lc == MPCD_ISOF) {
+      if (!(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT || p.rtech == MPCD_RAT))
+        return bad(MPCD_EUNSUPPORTED, "collision operator outside this path");
+    } else if (p.lc != MPCD_LCL && p.lc != MPCD_LCG) {
+      return bad(MPCD_EUNSUPPORTED, "liquid-crystal mode outside this path");
+    }
+    if (!(p.tstech == MPCD_NOTHERM || p.tstech == MPCD_VSC || p.tstech == MPCD_BEREND))
+      return bad(MPCD_EUNSUPPORTED, "thermostat outside this path");
+    for (int w = 0; w < p.nwall; w++) {
+      const mpcd_gpu_wall &W = p.wall[w];
+      int nz = 0;
+      for (int i = 0; i < 3; i++) nz += fneq(W.A[i], 0.0) ? 1 : 0;
+      if (nz != 1) return bad(MPCD_EUNSUPPORTED, "only axis-aligned planar walls are on this path");
+      if (W.COLL_TYPE != 1) return bad(MPCD_EUNSUPPORTED, "only rule-based wall collisions are on this path");
+    }
+    if (n_particles[r] <= 0) return bad(MPCD_EINVAL, "replica without particles");
+    ntot += n_particles[r];
+    int nz = (p.dim == 3) ? p.L[2] + 1 : 1;
+    nctot += (int64_t)(p.L[0] + 1) * (p.L[1] + 1) * nz;
+  }
+  if (ntot >= (1ll << 31) - 1 || nctot >= (1ll << 31) - 1) return bad(MPCD_EINVAL, "too many particles or cells");
+  c->ntot = (int)ntot;
+  c->nctot = (int)nctot;
+
+  // replica descriptors and block tables
+  int poff = 0, coff = 0, woff = 0;
+  std::vector<mpcd_gpu_wall> walls;
+  for (int r = 0; r < nrep; r++) {
+    const mpcd_gpu_params &p = params[r];
+    RepDev R{};
+    R.dim = p.dim;
+    for (int i = 0; i < 3; i++) R.L[i] = (i < p.dim) ? p.L[i] : 1;
+    R.nx = R.L[0] + 1;
+    R.ny = R.L[1] + 1;
+    R.nz = (p.dim == 3) ? R.L[2] + 1 : 1;
+    set_pbc(p, R.pbc);
+    R.cell_off = coff;
+    R.ncell = R.nx * R.ny * R.nz;
+    R.part_off = poff;
+    R.npart = (int)n_particles[r];
+    R.wall_off = woff;
+    R.nwall = p.nwall;
+    R.rtech = p.rtech;
+    R.lc = p.lc;
+    R.tstech = p.tstech;
+    R.galinv = p.galinv;
+    R.true_sign = p.true_random_sign;
+    R.dt = p.dt;
+    R.kbt = p.kbt;
+    R.cosA = cos(p.rotang);  // in->C, in->S (init.c:1194-1195)
+    R.sinA = sin(p.rotang);
+    R.tau = p.tau;
+    for (int i = 0; i < 3; i++) {
+      R.grav[i] = p.grav[i];
+      R.avv[i] = p.avv[i];
+    }
+    R.gravflag = (fneq(p.grav[0], 0.0) || fneq(p.grav[1], 0.0) || fneq(p.grav[2], 0.0)) ? 1 : 0;
+    R.gndnst = p.gndnst;
+    R.anyphantom = 0;
+    for (int w = 0; w < p.nwall; w++) {
+      walls.push_back(p.wall[w]);
+      if (p.wall[w].PHANTOM) R.anyphantom = 1;
+    }
+    if (R.anyphantom) c->any_phantom = true;
+    if (p.tstech != MPCD_NOTHERM || (p.rtech >= 7 && p.rtech <= 17)) c->any_therm = true;
+    if (p.lc == MPCD_LCG) c->any_lcg = true;
+    c->reps.push_back(R);
+    for (int b = 0; b < R.npart; b += kBlock) c->pblk.push_back(BlkDev{r, poff + b, poff + std::min(b + kBlock, R.npart), 0});
+    for (int b = 0; b < R.ncell; b += kBlock) c->cblk.push_back(BlkDev{r, coff + b, coff + std::min(b + kBlock, R.ncell), 0});
+    poff += R.npart;
+    coff += R.ncell;
+    woff += p.nwall;
+  }
+  c->nwtot = woff;
+  c->uploaded.assign(nrep, 0);
+  c->lc_stride = anneal_stride(params[0]);
+
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad(MPCD_EDEVICE, "stream");
+  bool ok = true;
+  c->d_rep = dalloc<RepDev>(c, nrep);
+  c->d_pblk = dalloc<BlkDev>(c, c->pblk.size());
+  c->d_cblk = dalloc<BlkDev>(c, c->cblk.size());
+  c->d_sp = dalloc<mpcd_gpu_species>(c, MPCD_MAXSPEC);
+  c->d_wall = dalloc<mpcd_gpu_wall>(c, std::max<size_t>(1, walls.size()));
+  ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_sp && c->d_wall;
+  Ctx &C = c->C;
+  C.rep = c->d_rep;
+  C.sp = c->d_sp;
+  C.wall = c->d_wall;
+  C.wq = dalloc<double>(c, 3 * std::max(1, woff));
+  C.wq_s = dalloc<double>(c, 3 * std::max(1, woff));
+  C.wq_b = dalloc<double>(c, 3 * std::max(1, woff));
+  C.shift = dalloc<double>(c, 3 * nrep);
+  C.hist = dalloc<int32_t>(c, c->nctot + 1);
+  C.start = dalloc<int32_t>(c, c->nctot + 1 + 1024);
+  C.cnew = dalloc<int32_t>(c, c->ntot);
+  C.loc = dalloc<int32_t>(c, c->ntot);
+  C.cu = dalloc<int32_t>(c, c->ntot);
+  C.key = dalloc<uint64_t>(c, c->ntot);
+  C.skey = dalloc<uint64_t>(c, c->ntot);
+  C.cell.pop = dalloc<int32_t>(c, c->nctot);
+  C.cell.mass = dalloc<double>(c, c->nctot);
+  C.cell.vcm = dalloc<double>(c, 3 * (size_t)c->nctot);
+  C.cell.cm = dalloc<double>(c, 3 * (size_t)c->nctot);
+  C.cell.I = dalloc<double>(c, 6 * (size_t)c->nctot);
+  C.cell.S = dalloc<double>(c, c->nctot);
+  C.cell.dir = dalloc<double>(c, 3 * (size_t)c->nctot);
+  C.cell.E = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 9 * (size_t)c->nctot) : nullptr;
+  C.T = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 3 * (size_t)c->ntot) : nullptr;
+  C.tpart = dalloc<double>(c, c->cblk.size());
+  C.kbtnow = dalloc<double>(c, nrep);
+  C.gscale = dalloc<double>(c, nrep);
+  C.sg = dalloc<double>(c, nrep);
+  C.events = dalloc<unsigned long long>(c, 4);
+  C.flags = dalloc<int>(c, 4);
+  C.ntot = c->ntot;
+  C.nctot = c->nctot;
+  C.nrep = nrep;
+  c->d_avpart = (c->lc == MPCD_LCG) ? dalloc<double>(c, 6 * c->pblk.size()) : nullptr;
+  ok = ok && C.wq && C.wq_s && C.wq_b && C.shift && C.hist && C.start && C.cnew && C.loc && C.cu && C.key &&
+       C.skey && C.cell.pop && C.cell.mass && C.cell.vcm && C.cell.cm && C.cell.I && C.cell.S && C.cell.dir &&
+       C.tpart && C.kbtnow && C.gscale && C.sg && C.events && C.flags;
+  bool lcm = c->lc != MPCD_ISOF;
+  if (lcm) ok = ok && C.cell.E && C.T;
+  if (c->precision == MPCD_FP64) {
+    c->sd[0] = alloc_state<double>(c, c->ntot, lcm);
+    c->sd[1] = alloc_state<double>(c, c->ntot, lcm);
+    ok = ok && state_ok(c->sd[0], lcm) && state_ok(c->sd[1], lcm);
+  } else {
+    c->sf[0] = alloc_state<float>(c, c->ntot, lcm);
+    c->sf[1] = alloc_state<float>(c, c->ntot, lcm);
+    ok = ok && state_ok(c->sf[0], lcm) && state_ok(c->sf[1], lcm);
+  }
+  if (!ok) {
+    mpcd_gpu_destroy(c);
+    fprintf(stderr, "mpcd_gpu_create: device allocation failed\n");
+    return MPCD_ENOMEM;
+  }
+  // constant tables
+  std::vector<double> wq(3 * std::max(1, woff), 0.0);
+  for (int w = 0; w < woff; w++)
+    for (int j = 0; j < 3; j++) wq[w * 3 + j] = walls[w].Q[j];
+  hipMemcpy(c->d_rep, c->reps.data(), sizeof(RepDev) * nrep, hipMemcpyHostToDevice);
+  hipMemcpy(c->d_pblk, c->pblk.data(), sizeof(BlkDev) * c->pblk.size(), hipMemcpyHostToDevice);
+  hipMemcpy(c->d_cblk, c->cblk.data(), sizeof(BlkDev) * c->cblk.size(), hipMemcpyHostToDevice);
+  hipMemcpy(c->d_sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC, hipMemcpyHostToDevice);
+  if (woff) hipMemcpy(c->d_wall, walls.data(), sizeof(mpcd_gpu_wall) * woff, hipMemcpyHostToDevice);
+  hipMemcpy(C.wq, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice);
+  hipMemcpy(C.wq_s, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice);
+  hipMemcpy(C.wq_b, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    mpcd_gpu_destroy(c);
+    return MPCD_EDEVICE;
+  }
+  *out = c;
+  return MPCD_OK;
+}
+
+extern "C" void mpcd_gpu_destroy(mpcd_gpu_ctx *c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  for (void *p : c->allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char *mpcd_gpu_last_error(const mpcd_gpu_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+extern "C" int mpcd_gpu_ncells(const mpcd_gpu_ctx *c, int r) {
+  if (!c || r < 0 || r >= c->nrep) return -1;
+  return c->reps[r].ncell;
+}
+
+extern "C" int mpcd_gpu_lc_stride(const mpcd_gpu_ctx *c) { return c ? c->lc_stride : -1; }
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static int nblocks(int n, int b) { return (n + b - 1) / b; }
+
+static Draws make_draws(mpcd_gpu_ctx *c, int injected) {
+  Draws D{};
+  D.rng.seed = c->seed;
+  D.rng.step = c->step;
+  D.injected = injected;
+  D.shift = c->d_dshift;
+  D.cell = c->d_dcell;
+  D.ghost = c->d_dghost;
+  D.part = c->d_dpart;
+  D.lc = c->d_dlc;
+  D.lc_stride = c->dlc_stride;
+  return D;
+}
+
+static mpcd_status stage_draws(mpcd_gpu_ctx *c, const mpcd_draws *d) {
+  if (!d) return MPCD_OK;
+  if (!c->d_dshift) {
+    c->d_dshift = dalloc<double>(c, 3 * c->nrep);
+    c->d_dcell = dalloc<double>(c, 4 * (size_t)c->nctot);
+    c->d_dghost = dalloc<double>(c, 3 * (size_t)c->nctot);
+    c->d_dpart = dalloc<double>(c, 3 * (size_t)c->ntot);
+    if (!c->d_dshift || !c->d_dcell || !c->d_dghost || !c->d_dpart) return fail(c, MPCD_ENOMEM, "draw staging");
+  }
+  if (d->shift) HIPCHK(c, hipMemcpyAsync(c->d_dshift, d->shift, sizeof(double) * 3 * c->nrep, hipMemcpyHostToDevice, c->stream));
+  if (d->cell) HIPCHK(c, hipMemcpyAsync(c->d_dcell, d->cell, sizeof(double) * 4 * (size_t)c->nctot, hipMemcpyHostToDevice, c->stream));
+  if (d->ghost) HIPCHK(c, hipMemcpyAsync(c->d_dghost, d->ghost, sizeof(double) * 3 * (size_t)c->nctot, hipMemcpyHostToDevice, c->stream));
+  if (d->part) HIPCHK(c, hipMemcpyAsync(c->d_dpart, d->part, sizeof(double) * 3 * (size_t)c->ntot, hipMemcpyHostToDevice, c->stream));
+  if (d->lc) {
+    if (d->lc_stride <= 0) return fail(c, MPCD_EINVAL, "lc_stride");
+    size_t need = (size_t)d->lc_stride * c->ntot;
+    if (need > c->dlc_cap) {
+      c->d_dlc = dalloc<double>(c, need);
+      if (!c->d_dlc) return fail(c, MPCD_ENOMEM, "lc draw staging");
+      c->dlc_cap = need;
+    }
+    c->dlc_stride = d->lc_stride;
+    HIPCHK(c, hipMemcpyAsync(c->d_dlc, d->lc, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+  }
+  // the host arrays are borrowed only for the call
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MPCD_OK;
+}
+
+static void scan_fixed(mpcd_gpu_ctx *c, int32_t *in, int32_t *out, int n, std::vector<int32_t *> &scratch,
+                       size_t level) {
+  int nb = nblocks(n + 1, 1024);
+  if (nb == 1) {
+    k_scan_block<<<1, 256, 0, c->stream>>>(in, out, nullptr, n + 1);
+    return;
+  }
+  int32_t *bsum = scratch[2 * level], *boff = scratch[2 * level + 1];
+  k_scan_block<<<nb, 256, 0, c->stream>>>(in, out, bsum, n + 1);
+  scan_fixed(c, bsum, boff, nb - 1, scratch, level + 1);
+  k_scan_add<<<nb, 256, 0, c->stream>>>(out, boff, n + 1);
+}
+
+static std::vector<int32_t *> &scan_scratch(mpcd_gpu_ctx *c) {
+  if (!c->scan_lv.empty() || c->nctot + 1 <= 1024) return c->scan_lv;
+  int n = c->nctot;
+  while (true) {
+    int nb = nblocks(n + 1, 1024);
+    if (nb == 1) break;
+    c->scan_lv.push_back(dalloc<int32_t>(c, nb + 1));
+    c->scan_lv.push_back(dalloc<int32_t>(c, nb + 1));
+    n = nb - 1;
+  }
+  return c->scan_lv;
+}
+
+template <typename Real>
+static State<Real> *states(mpcd_gpu_ctx *c);
+template <>
+State<double> *states<double>(mpcd_gpu_ctx *c) { return c->sd; }
+template <>
+State<float> *states<float>(mpcd_gpu_ctx *c) { return c->sf; }
+
+// binning: mode 0 initial, 1 shifted, 2 materialise the unshifted bin.
+// gather = false leaves the state in place (only the sorted keys are built).
+template <typename Real>
+static void run_bin(mpcd_gpu_ctx *c, int mode, bool gather, const Draws &D) {
+  State<Real> *S = states<Real>(c);
+  Ctx &C = c->C;
+  k_fill<<<nblocks(c->nctot + 1, kBlock), kBlock, 0, c->stream>>>(C.hist, 0, c->nctot + 1);
+  if (mode != 0) k_shift<<<nblocks(c->nrep, 64), 64, 0, c->stream>>>(C, D, mode == 1 ? 1 : 0);
+  k_count<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, mode, c->pending ? 1 : 0);
+  scan_fixed(c, C.hist, C.start, c->nctot, scan_scratch(c), 0);
+  k_scatter<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C);
+  k_sortcells<<<nblocks(c->nctot, kBlock), kBlock, 0, c->stream>>>(C, S[1 - c->cur].cst);
+  if (gather) {
+    k_gather<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], S[1 - c->cur], c->d_pblk,
+                                                              mode == 1 ? 1 : 0, c->lc != MPCD_ISOF, D);
+    c->cur = 1 - c->cur;
+    c->pending = false;
+  }
+}
+
+template <typename Real>
+static void run_props(mpcd_gpu_ctx *c, int do_acc) {
+  State<Real> *S = states<Real>(c);
+  k_cellprops<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(c->C, S[c->cur], c->d_cblk, do_acc);
+  if (c->lc != MPCD_ISOF) {
+    k_velgrad_bulk<<<c->cblk.size(), kBlock, 0, c->stream>>>(c->C, c->d_cblk);
+    k_velgrad_edges<<<c->nrep, 256, 0, c->stream>>>(c->C);
+  }
+}
+
+template <typename Real>
+static void run_collide(mpcd_gpu_ctx *c, const Draws &D) {
+  State<Real> *S = states<Real>(c);
+  Ctx &C = c->C;
+  run_props<Real>(c, 1);
+  if (c->any_phantom) k_ghost<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+  if (c->lc != MPCD_ISOF) {
+    if (c->any_lcg) {
+      k_avorder_part<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, c->d_avpart);
+      k_avorder_final<<<1, 64, 0, c->stream>>>(C, c->d_pblk, (int)c->pblk.size(), c->d_avpart);
+    }
+    k_lc<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+  }
+  k_collide<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+  k_temp_final<<<1, 64, 0, c->stream>>>(C, c->d_cblk, (int)c->cblk.size());
+  if (c->any_therm) k_scale<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk);
+}
+
+template <typename Real>
+static void run_stream(mpcd_gpu_ctx *c, const Draws &D) {
+  State<Real> *S = states<Real>(c);
+  k_stream<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(c->C, S[c->cur], c->d_pblk, D);
+  c->pending = true;
+  c->step++;
+}
+
+template <typename Real>
+static void run_step(mpcd_gpu_ctx *c, const Draws &D) {
+  run_bin<Real>(c, 1, true, D);
+  run_collide<Real>(c, D);
+  run_stream<Real>(c, D);
+}
+
+static mpcd_status ensure_init(mpcd_gpu_ctx *c) {
+  if (!c->need_init) return MPCD_OK;
+  for (int r = 0; r < c->nrep; r++)
+    if (!c->uploaded[r]) return fail(c, MPCD_EINVAL, "replica %d has no state (mpcd_gpu_upload)", r);
+  Draws D = make_draws(c, 0);
+  if (c->precision == MPCD_FP64) {
+    run_bin<double>(c, 0, true, D);
+    run_props<double>(c, 0);
+  } else {
+    run_bin<float>(c, 0, true, D);
+    run_props<float>(c, 0);
+  }
+  int flags[4];
+  HIPCHK(c, hipMemcpyAsync(flags, c->C.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flags[0] & 2) return fail(c, MPCD_EINVAL, "initial positions outside the box");
+  c->need_init = false;
+  return MPCD_OK;
+}
+
+static mpcd_status check_launch(mpcd_gpu_ctx *c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, MPCD_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
+  return MPCD_OK;
+}
+
+static mpcd_status ensure_stage(mpcd_gpu_ctx *c, size_t n) {
+  if (n <= c->stage_cap) return MPCD_OK;
+  c->d_q = dalloc<double>(c, 3 * n);
+  c->d_v = dalloc<double>(c, 3 * n);
+  c->d_u = dalloc<double>(c, 3 * n);
+  c->d_spid = dalloc<int32_t>(c, n);
+  c->d_i32 = dalloc<int32_t>(c, n + 1);
+  if (!c->d_q || !c->d_v || !c->d_u || !c->d_spid || !c->d_i32) return fail(c, MPCD_ENOMEM, "staging");
+  c->stage_cap = n;
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_upload(mpcd_gpu_ctx *c, int r, const double *Q, const double *V, const double *U,
+                                       const int32_t *spid) {
+  if (!c || r < 0 || r >= c->nrep || !Q || !V) return c ? fail(c, MPCD_EINVAL, "upload: bad argument") : MPCD_EINVAL;
+  const RepDev &R = c->reps[r];
+  mpcd_status st = ensure_stage(c, R.npart);
+  if (st) return st;
+  size_t n = R.npart;
+  if (spid)
+    for (size_t i = 0; i < n; i++)
+      if (spid[i] < 0 || spid[i] >= c->params[r].nspec) return fail(c, MPCD_EINVAL, "species id out of range");
+  HIPCHK(c, hipMemcpyAsync(c->d_q, Q, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_v, V, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  if (U) HIPCHK(c, hipMemcpyAsync(c->d_u, U, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  if (spid) HIPCHK(c, hipMemcpyAsync(c->d_spid, spid, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  if (c->precision == MPCD_FP64)
+    k_stage<double><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->sd[c->cur], R.part_off, R.npart, c->d_q,
+                                                                      c->d_v, U ? c->d_u : nullptr, spid ? c->d_spid : nullptr);
+  else
+    k_stage<float><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->sf[c->cur], R.part_off, R.npart, c->d_q,
+                                                                     c->d_v, U ? c->d_u : nullptr, spid ? c->d_spid : nullptr);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->uploaded[r] = 1;
+  c->need_init = true;
+  c->pending = false;
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_download(mpcd_gpu_ctx *c, int r, double *Q, double *V, double *U) {
+  if (!c || r < 0 || r >= c->nrep || !Q || !V) return c ? fail(c, MPCD_EINVAL, "download: bad argument") : MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  const RepDev &R = c->reps[r];
+  st = ensure_stage(c, R.npart);
+  if (st) return st;
+  size_t n = R.npart;
+  int lc = c->lc != MPCD_ISOF;
+  if (c->precision == MPCD_FP64)
+    k_unsort<double><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->sd[c->cur], R.part_off, R.npart, c->d_q,
+                                                                       c->d_v, c->d_u, lc);
+  else
+    k_unsort<float><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->sf[c->cur], R.part_off, R.npart, c->d_q,
+                                                                      c->d_v, c->d_u, lc);
+  HIPCHK(c, hipMemcpyAsync(Q, c->d_q, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(V, c->d_v, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  if (U) {
+    if (lc) HIPCHK(c, hipMemcpyAsync(U, c->d_u, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    else memset(U, 0, sizeof(double) * 3 * n);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *c, const mpcd_draws *draws, int shifted, int32_t *cell_out,
+                                    int32_t *count_out, int32_t *perm_out) {
+  if (!c) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = stage_draws(c, draws);
+  if (st) return st;
+  Draws D = make_draws(c, draws != nullptr);
+  if (shifted == 0 && !c->pending) {
+    // already materialised: nothing to do
+  } else if (c->precision == MPCD_FP64) {
+    run_bin<double>(c, shifted ? 1 : 2, true, D);
+  } else {
+    run_bin<float>(c, shifted ? 1 : 2, true, D);
+  }
+  st = check_launch(c);
+  if (st) return st;
+  if (cell_out || count_out || perm_out) {
+    st = ensure_stage(c, c->ntot);
+    if (st) return st;
+    const int32_t *id = (c->precision == MPCD_FP64) ? c->sd[c->cur].id : c->sf[c->cur].id;
+    const int32_t *cst = (c->precision == MPCD_FP64) ? c->sd[c->cur].cst : c->sf[c->cur].cst;
+    std::vector<int32_t> hid(c->ntot), hcst(c->ntot);
+    HIPCHK(c, hipMemcpyAsync(hid.data(), id, sizeof(int32_t) * c->ntot, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hcst.data(), cst, sizeof(int32_t) * c->ntot, hipMemcpyDeviceToHost, c->stream));
+    if (count_out) HIPCHK(c, hipMemcpyAsync(count_out, c->C.hist, sizeof(int32_t) * c->nctot, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < c->nrep; r++) {
+      const RepDev &R = c->reps[r];
+      for (int t = R.part_off; t < R.part_off + R.npart; t++) {
+        if (perm_out) perm_out[t] = hid[t];
+        if (cell_out) cell_out[R.part_off + hid[t]] = hcst[t] - R.cell_off;
+      }
+    }
+  }
+  return MPCD_OK;
+}
+
+static mpcd_status read_kbt(mpcd_gpu_ctx *c, double *kbt_now_out) {
+  if (!kbt_now_out) return MPCD_OK;
+  HIPCHK(c, hipMemcpyAsync(kbt_now_out, c->C.kbtnow, sizeof(double) * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *c, const mpcd_draws *draws, double *kbt_now_out) {
+  if (!c) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = stage_draws(c, draws);
+  if (st) return st;
+  Draws D = make_draws(c, draws != nullptr);
+  if (c->precision == MPCD_FP64) run_collide<double>(c, D);
+  else run_collide<float>(c, D);
+  st = check_launch(c);
+  if (st) return st;
+  return read_kbt(c, kbt_now_out);
+}
+
+static mpcd_status read_events(mpcd_gpu_ctx *c, int64_t *out) {
+  unsigned long long ev[4];
+  HIPCHK(c, hipMemcpyAsync(ev, c->C.events, sizeof ev, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out)
+    for (int i = 0; i < 4; i++) out[i] = (int64_t)ev[i];
+  return ev[0] ? MPCD_EESCAPE : MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *c, int64_t *bc_event_counts_out) {
+  if (!c) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  Draws D = make_draws(c, 0);
+  if (c->precision == MPCD_FP64) run_stream<double>(c, D);
+  else run_stream<float>(c, D);
+  st = check_launch(c);
+  if (st) return st;
+  if (bc_event_counts_out) return read_events(c, bc_event_counts_out);
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *c, const mpcd_draws *draws, double *kbt_now_out) {
+  if (!c) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = stage_draws(c, draws);
+  if (st) return st;
+  Draws D = make_draws(c, draws != nullptr);
+  if (c->precision == MPCD_FP64) run_step<double>(c, D);
+  else run_step<float>(c, D);
+  st = check_launch(c);
+  if (st) return st;
+  return read_kbt(c, kbt_now_out);
+}
+
+extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) {
+  if (!c || nsteps < 0) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  scan_scratch(c);  // allocate outside any capture
+  for (int k = 0; k < nsteps; k++) {
+    Draws D = make_draws(c, 0);
+    if (c->precision == MPCD_FP64) run_step<double>(c, D);
+    else run_step<float>(c, D);
+  }
+  (void)use_graph;
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *c) {
+  if (!c) return MPCD_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, double *vcm, double *S, double *dir) {
+  if (!c || r < 0 || r >= c->nrep) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  if (c->pending) {
+    Draws D = make_draws(c, 0);
+    // the reference re-bins and recomputes localPROP after streaming (mpc.c:4936-4942)
+    if (c->precision == MPCD_FP64) {
+      run_bin<double>(c, 2, true, D);
+      run_props<double>(c, 0);
+    } else {
+      run_bin<float>(c, 2, true, D);
+      run_props<float>(c, 0);
+    }
+  }
+  const RepDev &R = c->reps[r];
+  size_t off = R.cell_off, n = R.ncell;
+  if (pop) HIPCHK(c, hipMemcpyAsync(pop, c->C.cell.pop + off, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (vcm) HIPCHK(c, hipMemcpyAsync(vcm, c->C.cell.vcm + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  if (S) HIPCHK(c, hipMemcpyAsync(S, c->C.cell.S + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  if (dir) HIPCHK(c, hipMemcpyAsync(dir, c->C.cell.dir + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *c, int r, int32_t *order, int32_t *cstart) {
+  if (!c || r < 0 || r >= c->nrep) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_stage(c, c->ntot);
+  if (st) return st;
+  const RepDev &R = c->reps[r];
+  const int32_t *id = (c->precision == MPCD_FP64) ? c->sd[c->cur].id : c->sf[c->cur].id;
+  Draws D = make_draws(c, 0);
+  if (c->pending) {
+    // build the unshifted order's keys without moving the state
+    if (c->precision == MPCD_FP64) run_bin<double>(c, 2, false, D);
+    else run_bin<float>(c, 2, false, D);
+    k_order_ids<<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->C.skey, id, c->d_i32, R.part_off, R.npart);
+  } else {
+    HIPCHK(c, hipMemcpyAsync(c->d_i32, id + R.part_off, sizeof(int32_t) * R.npart, hipMemcpyDeviceToDevice, c->stream));
+    // start[] of the current order: recompute the histogram of the slots' cells
+    if (c->precision == MPCD_FP64) run_bin<double>(c, 2, false, D);
+    else run_bin<float>(c, 2, false, D);
+  }
+  if (order) HIPCHK(c, hipMemcpyAsync(order, c->d_i32, sizeof(int32_t) * R.npart, hipMemcpyDeviceToHost, c->stream));
+  if (cstart) {
+    std::vector<int32_t> h(R.ncell + 1);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->C.start + R.cell_off, sizeof(int32_t) * (R.ncell + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i <= R.ncell; i++) cstart[i] = h[i] - R.part_off;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *c, double *ms_out, int n) {
+  if (!c || !ms_out) return MPCD_EINVAL;
+  for (int i = 0; i < n; i++) ms_out[i] = 0.0;
+  return MPCD_OK;
+}

@@ -1,0 +1,1217 @@
+// mpcd_kernels.h -- HIP kernels of the MPCD solvent step (gfx950).
+//
+// Layout in HBM.  Particles live in structure-of-arrays form, sorted by
+// collision cell ("slots"), concatenated over replicas: x,y,z,vx,vy,vz
+// (+ux,uy,uz for nematics) in the state precision (double or float), plus
+// the particle id, species id and the slot's cell.  Two such buffers are
+// ping-ponged by the binning gather.  Cells are numbered x-major over
+// (L0+1)(L1+1)(L2+1) (z extent 1 in 2D) and concatenated over replicas; all
+// per-cell sums are kept in double.
+//
+// Binning.  The reference keeps one linked list per cell and relinks movers
+// to the tail (mpc.c:948-992, addlink 1069-1087), which is a stable
+// partition: in every cell the stayers keep their order and are followed by
+// the arrivals in previous global order.  A step bins twice (shifted for the
+// collision, unshifted after streaming); the two stable partitions compose
+// into ONE counting sort by the new shifted cell with the in-cell key
+//   (moved since the unshifted bin, unshifted cell, moved in the unshifted
+//    bin, slot)
+// so a step needs one histogram (global atomics), one scan, one scatter of
+// 64-bit keys and one in-cell sort, and the unshifted bin is never
+// materialised unless a caller asks for it.
+#pragma once
+
+#include "mpcd_math.h"
+#include "../../include/mpcd_gpu.h"
+
+namespace mpcd {
+
+constexpr int kBlock = 256;
+
+struct RepDev {
+  int dim, L[3], nx, ny, nz, pbc[3];
+  int cell_off, ncell, part_off, npart;
+  int wall_off, nwall;
+  int rtech, lc, tstech, galinv, true_sign, gravflag, anyphantom;
+  double dt, kbt, cosA, sinA, tau, grav[3], avv[3], gndnst;
+};
+
+// a block of work inside one replica
+struct BlkDev {
+  int rep, beg, end, pad;
+};
+
+template <typename Real>
+struct State {
+  Real *x, *y, *z, *vx, *vy, *vz, *ux, *uy, *uz;
+  int32_t *id;
+  int32_t *sp;
+  int32_t *cst;  // cell of the slot in this buffer's order
+};
+
+struct CellDev {
+  int32_t *pop;
+  double *mass, *vcm, *cm, *I, *S, *dir, *E;
+};
+
+struct Ctx {
+  const RepDev *rep;
+  const mpcd_gpu_species *sp;
+  const mpcd_gpu_wall *wall;
+  double *wq, *wq_s, *wq_b;  // wall positions: persistent, shifted, shifted back
+  double *shift;             // [3 * nrep]
+  int32_t *hist, *start, *cnew, *loc, *cu;
+  uint64_t *key, *skey;
+  CellDev cell;
+  double *T;                 // LC torque per slot [3 * ntot]
+  double *tpart;             // temperature partial sums per block
+  double *kbtnow, *gscale, *sg; // per replica
+  unsigned long long *events;   // [4]
+  int *flags;                   // [0] capacity error
+  int ntot, nctot, nrep;
+};
+
+template <typename Real>
+MPCD_HD void load_q(const State<Real> &s, int i, double q[3]) {
+  q[0] = (double)s.x[i];
+  q[1] = (double)s.y[i];
+  q[2] = (double)s.z[i];
+}
+
+// cell index of a position; shifted = 1 applies the periodic wrap of
+// bin(shifted=1) (mpc.c:964-968).  Returns -1 for an escaped particle.
+__device__ __forceinline__ int cell_of(const RepDev &R, const double q[3], int shifted) {
+  int a = (int)q[0], b = (int)q[1], c = (int)q[2];
+  int Lz = (R.dim == 3) ? R.L[2] : 1;
+  if (shifted) {
+    if (R.pbc[0] && a == R.L[0]) a = 0;
+    if (R.pbc[1] && b == R.L[1]) b = 0;
+    if (R.pbc[2] && c == Lz) c = 0;
+  }
+  if (a > R.L[0] || b > R.L[1] || c > Lz || a < 0 || b < 0 || c < 0) return -1;
+  if (R.dim < 3) c = 0;
+  return R.cell_off + (a * R.ny + b) * R.nz + c;
+}
+
+// gridShift_all(+SHIFT), mpc.c:728-736
+__device__ __forceinline__ void shift_fwd(const RepDev &R, const double *s, double q[3]) {
+  for (int j = 0; j < R.dim; j++) {
+    q[j] += s[j];
+    if (q[j] > R.L[j] && q[j] <= R.L[j] + s[j]) q[j] -= R.L[j];
+  }
+}
+
+// gridShift_all(-SHIFT), mpc.c:728-736 with shiftBack = 1
+__device__ __forceinline__ void shift_back(const RepDev &R, const double *s, double q[3]) {
+  for (int j = 0; j < R.dim; j++) {
+    double sg = -1.0 * s[j];
+    q[j] += sg;
+    if (q[j] < 0.0 && q[j] >= sg) q[j] += R.L[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1. per-replica grid shift draw and wall positions (ranshift, rand.c:678)
+// ---------------------------------------------------------------------------
+__global__ void k_shift(Ctx C, Draws D, int do_shift) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C.nrep) return;
+  const RepDev &R = C.rep[r];
+  double s[3] = {0.0, 0.0, 0.0};
+  if (do_shift && R.galinv) {
+    for (int j = 0; j < R.dim; j++) s[j] = D.injected ? D.shift[r * 3 + j] : D.rng.uniform(r, P_SHIFT, 0, j);
+  }
+  for (int j = 0; j < 3; j++) C.shift[r * 3 + j] = s[j];
+  for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+    for (int j = 0; j < 3; j++) {
+      double q = C.wq[w * 3 + j];
+      if (j < R.dim && do_shift) {
+        double qs = q + s[j];
+        double qb = qs + (-1.0 * s[j]);
+        C.wq_s[w * 3 + j] = qs;
+        C.wq_b[w * 3 + j] = qb;
+        C.wq[w * 3 + j] = qb;
+      } else {
+        C.wq_s[w * 3 + j] = q;
+        C.wq_b[w * 3 + j] = q;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. histogram + composite key.  mode 0: initial binning (binin, mpc.c:924);
+//    1: shifted bin; 2: materialise the pending unshifted bin
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_count(Ctx C, State<Real> S, const BlkDev *blk, int mode, int pending) {
+  const BlkDev B = blk[blockIdx.x];
+  int s = B.beg + threadIdx.x;
+  if (s >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  double q[3];
+  load_q(S, s, q);
+  int cnew, cu;
+  uint32_t a1 = 0, a2 = 0;
+  if (mode == 0) {
+    cnew = cell_of(R, q, 0);
+    if (cnew < 0) { atomicOr(&C.flags[0], 2); cnew = R.cell_off; }
+    cu = cnew;
+  } else {
+    int cs = S.cst[s];
+    cu = pending ? C.cu[s] : cs;
+    a1 = (cu != cs) ? 1u : 0u;
+    if (mode == 1) {
+      shift_fwd(R, &C.shift[B.rep * 3], q);
+      cnew = cell_of(R, q, 1);
+      if (cnew < 0) cnew = cu;  // escaped: stays in its list (mpc.c:970-982)
+    } else {
+      cnew = cu;
+    }
+    a2 = (cnew != cu) ? 1u : 0u;
+  }
+  int l = atomicAdd(&C.hist[cnew], 1);
+  C.cnew[s] = cnew;
+  C.loc[s] = l;
+  C.key[s] = ((uint64_t)a2 << 63) | ((uint64_t)(uint32_t)cu << 32) | ((uint64_t)a1 << 31) | (uint64_t)(uint32_t)s;
+}
+
+// ---------------------------------------------------------------------------
+// 3. exclusive scan of the histogram (multi-level, 1024 cells per block)
+// ---------------------------------------------------------------------------
+__global__ void k_scan_block(const int32_t *in, int32_t *out, int32_t *bsum, int n) {
+  __shared__ int32_t sh[1024];
+  int base = blockIdx.x * 1024;
+  for (int k = threadIdx.x; k < 1024; k += blockDim.x) sh[k] = (base + k < n) ? in[base + k] : 0;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over 1024 values with 256 threads
+  for (int off = 1; off < 1024; off <<= 1) {
+    int32_t t[4];
+    for (int j = 0; j < 4; j++) {
+      int k = threadIdx.x + j * 256;
+      t[j] = (k >= off) ? sh[k - off] : 0;
+    }
+    __syncthreads();
+    for (int j = 0; j < 4; j++) sh[threadIdx.x + j * 256] += t[j];
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < 1024; k += blockDim.x)
+    if (base + k < n) out[base + k] = (k == 0) ? 0 : sh[k - 1];
+  if (threadIdx.x == 0 && bsum) bsum[blockIdx.x] = sh[1023];
+}
+
+__global__ void k_scan_add(int32_t *out, const int32_t *boff, int n) {
+  int i = blockIdx.x * 1024;
+  int32_t add = boff[blockIdx.x];
+  for (int k = threadIdx.x; k < 1024; k += blockDim.x)
+    if (i + k < n) out[i + k] += add;
+}
+
+__global__ void k_fill(int32_t *a, int32_t v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 4. scatter the keys to their cells
+// ---------------------------------------------------------------------------
+__global__ void k_scatter(Ctx C) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= C.ntot) return;
+  C.skey[C.start[C.cnew[s]] + C.loc[s]] = C.key[s];
+}
+
+// ---------------------------------------------------------------------------
+// 5. sort every cell's keys (insertion sort; cells hold tens of particles)
+//    and record the cell of every new slot
+// ---------------------------------------------------------------------------
+__global__ void k_sortcells(Ctx C, int32_t *cst_new) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C.nctot) return;
+  int b = C.start[c], e = C.start[c + 1];
+  uint64_t *k = C.skey;
+  for (int i = b + 1; i < e; i++) {
+    uint64_t v = k[i];
+    int j = i - 1;
+    while (j >= b && k[j] > v) {
+      k[j + 1] = k[j];
+      j--;
+    }
+    k[j + 1] = v;
+  }
+  for (int i = b; i < e; i++) cst_new[i] = c;
+}
+
+// ---------------------------------------------------------------------------
+// 6. gather the state into the new order, applying the grid shift
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_gather(Ctx C, State<Real> S, State<Real> D, const BlkDev *blk, int shifted, int lc, Draws Dr) {
+  const BlkDev B = blk[blockIdx.x];
+  int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  int src = (int)(C.skey[t] & 0x7FFFFFFFull);
+  double q[3];
+  load_q(S, src, q);
+  double v[3] = {(double)S.vx[src], (double)S.vy[src], (double)S.vz[src]};
+  int id = S.id[src], sp = S.sp[src];
+  if (shifted) {
+    shift_fwd(R, &C.shift[B.rep * 3], q);
+    if (cell_of(R, q, 1) < 0) {
+      // escape repair (mpc.c:970-982): fresh uniform position and Maxwell velocity
+      atomicAdd(&C.events[0], 1ull);
+      double m = C.sp[sp].mass;
+      for (int j = 0; j < R.dim; j++) {
+        q[j] = Dr.rng.uniform(B.rep, P_ESCAPE, (uint32_t)id, j) * R.L[j];
+        v[j] = sqrt(R.kbt / m) * Dr.rng.normal(B.rep, P_ESCAPE, (uint32_t)id, 16 + j);
+      }
+    }
+  }
+  D.x[t] = (Real)q[0];
+  D.y[t] = (Real)q[1];
+  D.z[t] = (Real)q[2];
+  D.vx[t] = (Real)v[0];
+  D.vy[t] = (Real)v[1];
+  D.vz[t] = (Real)v[2];
+  if (lc) {
+    D.ux[t] = S.ux[src];
+    D.uy[t] = S.uy[src];
+    D.uz[t] = S.uz[src];
+  }
+  D.id[t] = id;
+  D.sp[t] = sp;
+}
+
+// ---------------------------------------------------------------------------
+// 7. cell properties: acc_all (mpc.c:697) then localPROP (mpc.c:64-216) with
+//    localMomInertiaTensor (mpc.c:4209-4299) and the order tensor / director
+//    (lc.c:1125-1148, mtools.c:1443)
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc) {
+  const BlkDev B = cblk[blockIdx.x];
+  int c = B.beg + threadIdx.x;
+  if (c >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  const int dim = R.dim;
+  const bool ccm = (R.rtech == MPCD_RAT) || (R.lc != MPCD_ISOF);
+  int b = C.start[c], e = C.start[c + 1];
+  double vcm[3], cm[3], mass = 0.0;
+  for (int d = 0; d < 3; d++) { vcm[d] = C.cell.vcm[c * 3 + d]; cm[d] = C.cell.cm[c * 3 + d]; }
+  for (int d = 0; d < dim; d++) vcm[d] = 0.0;
+  if (ccm) for (int d = 0; d < dim; d++) cm[d] = 0.0;
+  for (int t = b; t < e; t++) {
+    double m = C.sp[S.sp[t]].mass;
+    double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
+    if (do_acc && R.gravflag) {
+      for (int d = 0; d < dim; d++) v[d] = v[d] + R.dt * R.grav[d];
+      S.vx[t] = (Real)v[0];
+      S.vy[t] = (Real)v[1];
+      if (dim == 3) S.vz[t] = (Real)v[2];
+      v[0] = (double)S.vx[t];
+      v[1] = (double)S.vy[t];
+      v[2] = (double)S.vz[t];
+    }
+    mass += m;
+    for (int d = 0; d < dim; d++) vcm[d] += v[d] * m;
+    if (ccm) {
+      double q[3];
+      load_q(S, t, q);
+      for (int d = 0; d < dim; d++) cm[d] += q[d] * m;
+    }
+  }
+  if (mass > 0.0)
+    for (int d = 0; d < dim; d++) {
+      vcm[d] /= mass;
+      if (ccm) cm[d] /= mass;
+    }
+  C.cell.pop[c] = e - b;
+  C.cell.mass[c] = mass;
+  for (int d = 0; d < 3; d++) { C.cell.vcm[c * 3 + d] = vcm[d]; C.cell.cm[c * 3 + d] = cm[d]; }
+  if (ccm) {
+    double I[6] = {0, 0, 0, 0, 0, 0};  // I00 I11 I22 I01 I02 I12
+    for (int t = b; t < e; t++) {
+      double m = C.sp[S.sp[t]].mass, q[3], r[3] = {0, 0, 0};
+      load_q(S, t, q);
+      for (int d = 0; d < dim; d++) r[d] = q[d] - cm[d];
+      I[0] += m * (r[1] * r[1] + r[2] * r[2]);
+      I[1] += m * (r[0] * r[0] + r[2] * r[2]);
+      I[2] += m * (r[0] * r[0] + r[1] * r[1]);
+      I[3] += m * r[0] * r[1];
+      I[4] += m * r[0] * r[2];
+      I[5] += m * r[1] * r[2];
+    }
+    I[3] *= -1.; I[4] *= -1.; I[5] *= -1.;
+    for (int k = 0; k < 6; k++) C.cell.I[c * 6 + k] = I[k];
+  }
+  if (R.lc != MPCD_ISOF && e - b > 1) {
+    double Sm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double fD = (double)dim, cc = 1. / (fD - 1.), POPinv = 1. / ((double)(e - b));
+    for (int t = b; t < e; t++) {
+      double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]};
+      for (int i = 0; i < dim; i++)
+        for (int j = i; j < dim; j++) Sm[i][j] += u[i] * u[j];
+    }
+    for (int i = 0; i < dim; i++)
+      for (int j = i; j < dim; j++) Sm[i][j] *= cc * fD * POPinv;
+    for (int i = 0; i < dim; i++) Sm[i][i] -= cc;
+    for (int i = 0; i < dim; i++)
+      for (int j = i + 1; j < dim; j++) Sm[j][i] = Sm[i][j];
+    double Sv, dir[3] = {0, 0, 0};
+    if (dim == 3) {
+      double ev[3];
+      eig3_top(Sm, ev, dir);
+      Sv = -1. * (ev[1] + ev[2]);
+    } else {
+      double m2[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
+      Sv = eig2_top(m2, dir);
+    }
+    if (Sv > 1.0) Sv = 1.0;
+    C.cell.S[c] = Sv;
+    for (int i = 0; i < dim; i++) C.cell.dir[c * 3 + i] = dir[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 8. ghost particles and strong anchoring at phantom walls (ghostPart,
+//    mpc.c:351-567; oriBC lc.c:1290-1393)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wall_W(const mpcd_gpu_wall &W, const double *wq, const double *q, int dim) {
+  double Wv = 0.;                                         // calcW_PLANE, bc.c:306
+  for (int i = 0; i < dim; i++) Wv += W.A[i] * (q[i] - wq[i]);
+  Wv -= W.R;
+  if (W.INV) Wv *= -1.;
+  return Wv;
+}
+
+__device__ __forceinline__ bool anchoring(const mpcd_gpu_wall &W) { return feq(W.MUN, 0.0) || feq(W.MUT, 0.0); }
+
+__device__ __forceinline__ void ori_bc(double *u, const mpcd_gpu_wall &W, const double *n, int dim) {
+  double UN[3] = {0, 0, 0}, UT[3] = {0, 0, 0};
+  normn(u, dim);
+  if (dotn(u, n, dim) >= 0.0) { for (int i = 0; i < dim; i++) u[i] *= 1.; }
+  else { for (int i = 0; i < dim; i++) u[i] *= -1.; }
+  projn(u, n, UN, dim);
+  for (int i = 0; i < dim; i++) UT[i] = u[i] - UN[i];
+  if (feq(W.MUN, 0.0) && feq(W.MUT, 0.0)) {
+    for (int i = 0; i < dim; i++) u[i] = W.DUxyz[i];
+  } else {
+    for (int i = 0; i < dim; i++) { UN[i] *= W.MUN; UT[i] *= W.MUT; }
+    for (int i = 0; i < dim; i++) u[i] = UN[i] + UT[i];
+    u[0] *= W.MUxyz[0];
+    if (dim >= 2) u[1] *= W.MUxyz[1];
+    if (dim >= 3) u[2] *= W.MUxyz[2];
+  }
+  normn(u, dim);
+}
+
+template <typename Real>
+__global__ void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+  const BlkDev B = cblk[blockIdx.x];
+  int c = B.beg + threadIdx.x;
+  if (c >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  if (!R.anyphantom) return;
+  int pop = C.cell.pop[c];
+  if (pop <= 0) return;
+  const int dim = R.dim, nco = (dim == 3) ? 8 : 4;
+  int lc_ = c - R.cell_off;
+  int a = lc_ / (R.ny * R.nz), bb = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
+  double tq[8][3];
+  for (int j = 0; j < 8; j++) {
+    tq[j][0] = (double)a + (double)(j & 1);
+    tq[j][1] = (double)bb + (double)((j >> 1) & 1);
+    tq[j][2] = (double)cz + (double)((j >> 2) & 1);
+  }
+  int numBC = 0, wallindex = -1, flagW = 0;
+  for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+    const mpcd_gpu_wall &W = C.wall[w];
+    if (!(W.PHANTOM && anchoring(W))) continue;
+    int fw = 0;
+    for (int j = 0; j < nco; j++)
+      if (wall_W(W, &C.wq_s[w * 3], tq[j], dim) < 0.0) { fw = 1; wallindex = w; }
+    if (fw) numBC++;
+  }
+  int b = C.start[c], e = C.start[c + 1];
+  for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+    const mpcd_gpu_wall &W = C.wall[w];
+    if (!W.PHANTOM) continue;
+    for (int j = 0; j < nco; j++)
+      if (wall_W(W, &C.wq_s[w * 3], tq[j], dim) < 0.0) flagW = 1;
+    if (R.lc != MPCD_ISOF && flagW && numBC == 1 && wallindex == w && anchoring(W)) {
+      for (int t = b; t < e; t++) {
+        if (!((W.inter_mask >> S.sp[t]) & 1)) continue;
+        double n[3] = {0, 0, 0}, u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]};
+        for (int k = 0; k < dim; k++) n[k] = W.A[k];
+        normn(n, dim);
+        ori_bc(u, W, n, dim);
+        S.ux[t] = (Real)u[0];
+        S.uy[t] = (Real)u[1];
+        S.uz[t] = (Real)u[2];
+      }
+      C.cell.S[c] = 1.0;
+      double d3[3] = {(double)S.ux[b], (double)S.uy[b], (double)S.uz[b]};
+      normn(d3, dim);
+      for (int k = 0; k < dim; k++) C.cell.dir[c * 3 + k] = d3[k];
+    }
+  }
+  if (flagW && pop < R.gndnst && pop > 0) {
+    double invN = 1.0 / (R.gndnst - (double)pop);
+    for (int k = 0; k < dim; k++) {
+      double sigma = sqrt(R.kbt / invN);
+      double Rg = sigma * Dr.ghost_n(B.rep, c, k);
+      double v = C.cell.vcm[c * 3 + k];
+      v += Rg;
+      v /= R.gndnst;
+      C.cell.vcm[c * 3 + k] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 9. nematic LC: velocity gradient (velGradD2Q9 / velGradD3Q15,
+//    lc.c:2070-2162), global order (avOrderParam lc.c:982), Maier-Saupe
+//    orientation collision (LCcollision lc.c:427-544) and Jeffery torque
+//    (lc.c:559-718)
+// ---------------------------------------------------------------------------
+__global__ void k_velgrad_bulk(Ctx C, const BlkDev *cblk) {
+  const BlkDev B = cblk[blockIdx.x];
+  int c = B.beg + threadIdx.x;
+  if (c >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  int lc_ = c - R.cell_off;
+  int a = lc_ / (R.ny * R.nz), b = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
+  const double *V = C.cell.vcm;
+  double *E = &C.cell.E[(size_t)c * 9];
+  auto idx = [&](int aa, int bb, int cc) { return R.cell_off + (aa * R.ny + bb) * R.nz + cc; };
+  if (R.dim == 2) {
+    if (!(a >= 1 && a < R.L[0] && b >= 1 && b < R.L[1])) return;
+    const int uv[8][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}, {1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
+    const double w1 = 1. / 9., w5 = 1. / 36.;
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        double acc = 0.;
+        for (int k = 0; k < 8; k++) {
+          double w = (k < 4) ? w1 : w5;
+          acc += w * uv[k][j] * V[idx(a + uv[k][0], b + uv[k][1], 0) * 3 + i];
+        }
+        acc *= 3.;
+        E[i * 3 + j] = acc;
+      }
+  } else {
+    if (!(a >= 1 && a < R.L[0] && b >= 1 && b < R.L[1] && cz >= 1 && cz < R.L[2])) return;
+    const int uv[14][3] = {{1, 0, 0},  {0, 1, 0},  {-1, 0, 0}, {0, -1, 0},  {0, 0, 1},
+                           {0, 0, -1}, {1, 1, 1},  {-1, 1, 1}, {1, -1, 1},  {-1, -1, 1},
+                           {1, 1, -1}, {-1, 1, -1}, {1, -1, -1}, {-1, -1, -1}};
+    const double w1 = 1. / 9., w7 = 1. / 72.;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double acc = 0.;
+        for (int k = 0; k < 14; k++) {
+          double w = (k < 6) ? w1 : w7;
+          acc += w * uv[k][j] * V[idx(a + uv[k][0], b + uv[k][1], cz + uv[k][2]) * 3 + i];
+        }
+        acc *= 3.;
+        E[i * 3 + j] = acc;
+      }
+  }
+}
+
+// edge copies in the reference's order: x faces, then y faces, then z faces
+// (corners read values not yet refreshed this call, as the reference does)
+__global__ void k_velgrad_edges(Ctx C) {
+  const RepDev &R = C.rep[blockIdx.x];
+  const int X = R.L[0], Y = R.L[1], dim = R.dim;
+  const int Z = (dim == 3) ? R.L[2] : 1;
+  auto idx = [&](int aa, int bb, int cc) { return R.cell_off + (aa * R.ny + bb) * R.nz + cc; };
+  double *E = C.cell.E;
+  const int zr = (dim == 3) ? Z : 1;
+  // x faces
+  for (int k = threadIdx.x; k < Y * zr; k += blockDim.x) {
+    int b = k / zr, c = k % zr;
+    for (int ij = 0; ij < 9; ij++) {
+      if ((ij / 3) >= dim || (ij % 3) >= dim) continue;
+      E[(size_t)idx(0, b, c) * 9 + ij] = E[(size_t)idx(1, b, c) * 9 + ij];
+      E[(size_t)idx(X, b, c) * 9 + ij] = E[(size_t)idx(X - 1, b, c) * 9 + ij];
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < X * zr; k += blockDim.x) {
+    int a = k / zr, c = k % zr;
+    for (int ij = 0; ij < 9; ij++) {
+      if ((ij / 3) >= dim || (ij % 3) >= dim) continue;
+      E[(size_t)idx(a, 0, c) * 9 + ij] = E[(size_t)idx(a, 1, c) * 9 + ij];
+      E[(size_t)idx(a, Y, c) * 9 + ij] = E[(size_t)idx(a, Y - 1, c) * 9 + ij];
+    }
+  }
+  if (dim == 3) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < X * Y; k += blockDim.x) {
+      int a = k / Y, b = k % Y;
+      for (int ij = 0; ij < 9; ij++) {
+        E[(size_t)idx(a, b, 0) * 9 + ij] = E[(size_t)idx(a, b, 1) * 9 + ij];
+        E[(size_t)idx(a, b, Z) * 9 + ij] = E[(size_t)idx(a, b, Z - 1) * 9 + ij];
+      }
+    }
+  }
+}
+
+// global order tensor partial sums (lc = 2): 6 unique entries per block
+template <typename Real>
+__global__ void k_avorder_part(Ctx C, State<Real> S, const BlkDev *blk, double *part) {
+  const BlkDev B = blk[blockIdx.x];
+  int t = B.beg + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (t < B.end) {
+    double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]};
+    v[0] = u[0] * u[0]; v[1] = u[0] * u[1]; v[2] = u[0] * u[2];
+    v[3] = u[1] * u[1]; v[4] = u[1] * u[2]; v[5] = u[2] * u[2];
+  }
+  __shared__ double sh[6][kBlock];
+  for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; k++) part[blockIdx.x * 6 + k] = sh[k][0];
+}
+
+__global__ void k_avorder_final(Ctx C, const BlkDev *blk, int nblk, const double *part) {
+  int r = threadIdx.x;
+  if (r >= C.nrep) return;
+  const RepDev &R = C.rep[r];
+  double s6[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nblk; b++)
+    if (blk[b].rep == r)
+      for (int k = 0; k < 6; k++) s6[k] += part[b * 6 + k];
+  const int dim = R.dim;
+  double fD = (double)dim, cc = 1. / (fD - 1.), inv = 1. / ((double)R.npart);
+  double Sm[3][3] = {{s6[0], s6[1], s6[2]}, {s6[1], s6[3], s6[4]}, {s6[2], s6[4], s6[5]}};
+  for (int i = 0; i < dim; i++)
+    for (int j = i; j < dim; j++) Sm[i][j] *= cc * fD * inv;
+  for (int i = 0; i < dim; i++) Sm[i][i] -= cc;
+  for (int i = 0; i < dim; i++)
+    for (int j = i + 1; j < dim; j++) Sm[j][i] = Sm[i][j];
+  double dir[3];
+  if (dim == 3) {
+    double ev[3];
+    eig3_top(Sm, ev, dir);
+    C.sg[r] = -1. * (ev[1] + ev[2]);
+  } else {
+    double m2[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
+    C.sg[r] = eig2_top(m2, dir);
+  }
+}
+
+// Maier-Saupe samplers (lc.c:52-411); `slot` keys the draws
+__device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32_t slot, const double *DIR,
+                          double *rotAx, double rotAngle, double *U, double KBT, double S, double effM) {
+  if (R.dim == 2) {
+    double newU[3] = {U[0], U[1], 0.0};
+    double ang0 = signed_angle(DIR, U, 2);
+    if (feq(ang0, 0.5 * kPI) || feq(ang0, -0.5 * kPI)) ang0 = kPI * (0.5 - Dr.lc_u(rep, slot, 0));
+    double uz0 = dotn(DIR, U, 2);
+    int annealNum = (int)(kMCINT + kMCSLOPE * effM * S / KBT);
+    int cnt = 0;
+    for (int i = 0; i < annealNum; i++) {
+      int accept = 0;
+      double w0 = effM * S * uz0 * uz0 / KBT;
+      double ang1 = kPI * (0.5 - Dr.lc_u(rep, slot, 1 + 2 * i));
+      double uz1 = cos(ang1);
+      double w1 = effM * S * uz1 * uz1 / KBT;
+      double dw = w1 - w0;
+      if (dw >= 0.) accept = 1;
+      else if (Dr.lc_u(rep, slot, 2 + 2 * i) <= exp(dw)) accept = 1;
+      if (accept) { cnt += 1; ang0 = ang1; uz0 = uz1; }
+    }
+    newU[0] = uz0;
+    newU[1] = sin(ang0);
+    if (cnt > 0) {
+      rodrigues(newU, rotAx, rotAngle);
+      U[0] = newU[0];
+      U[1] = newU[1];
+    }
+    return;
+  }
+  double BUS = effM * S / KBT;
+  double newU[3] = {U[0], U[1], U[2]};
+  if (BUS > kBUSMAX) {
+    double sd = 1. / sqrt(2. * effM * S / KBT);
+    double theta = sd * Dr.lc_n(rep, slot, 0) + 0.0;
+    double uz = 1. - 0.5 * theta * theta;
+    double pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
+    newU[0] = pm * uz;
+    double phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
+    newU[1] = sqrt(1.0 - uz * uz) * cos(phi);
+    newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
+    rodrigues(newU, rotAx, rotAngle);
+    for (int d = 0; d < 3; d++) U[d] = newU[d];
+  } else if (BUS < kBUSMIN) {
+    double B2 = BUS;
+    if (B2 < kTOL) B2 = kTOL;
+    double Rr = Dr.lc_u(rep, slot, 0) * (1. + B2 / 3.);
+    double c = sqrt(9. * B2 * B2 * B2 * B2 * Rr * Rr + 4. * B2 * B2 * B2);
+    double num = pow(3. * B2 * B2 * Rr + c, 1. / 3.);
+    double denom = pow(2., 1. / 3.);
+    double uz = num / (denom * B2) - denom / num;
+    double pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
+    newU[0] = pm * uz;
+    double phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
+    newU[1] = sqrt(1.0 - uz * uz) * cos(phi);
+    newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
+    rodrigues(newU, rotAx, rotAngle);
+    for (int d = 0; d < 3; d++) U[d] = newU[d];
+  } else {
+    double uz0 = dotn(DIR, U, 3);
+    int annealNum = (int)(kMCINT + kMCSLOPE * effM * S / KBT);
+    int cnt = 0;
+    for (int i = 0; i < annealNum; i++) {
+      int accept = 0;
+      double w0 = effM * S * uz0 * uz0 / KBT;
+      double uz1 = Dr.lc_u(rep, slot, 1 + 2 * i);
+      double w1 = effM * S * uz1 * uz1 / KBT;
+      double dw = w1 - w0;
+      if (dw >= 0.) accept = 1;
+      else if (Dr.lc_u(rep, slot, 2 + 2 * i) <= exp(dw)) accept = 1;
+      if (accept) { cnt += 1; uz0 = uz1; }
+    }
+    double phi = Dr.lc_u(rep, slot, 1 + 2 * annealNum) * 2.0 * kPI;
+    double pm = (Dr.lc_u(rep, slot, 2 + 2 * annealNum) <= 0.5) ? -1.0 : 1.0;
+    newU[0] = pm * uz0;
+    newU[1] = sqrt(1.0 - uz0 * uz0) * cos(phi);
+    newU[2] = sqrt(1.0 - uz0 * uz0) * sin(phi);
+    if (cnt > 0) {
+      rodrigues(newU, rotAx, rotAngle);
+      for (int d = 0; d < 3; d++) U[d] = newU[d];
+    }
+  }
+}
+
+template <typename Real>
+__global__ void k_lc(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+  const BlkDev B = cblk[blockIdx.x];
+  int c = B.beg + threadIdx.x;
+  if (c >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  int b = C.start[c], e = C.start[c + 1];
+  if (e - b <= 1) return;
+  const int dim = R.dim;
+  // LCcollision, lc.c:427-544
+  double DIR[3] = {0, 0, 0}, rotAx[3], xaxis[3] = {1.0, 0.0, 0.0};
+  double Sv = (R.lc == MPCD_LCL) ? C.cell.S[c] : C.sg[B.rep];
+  for (int i = 0; i < dim; i++) DIR[i] = C.cell.dir[c * 3 + i];
+  cross3(DIR, xaxis, rotAx);
+  double rotAngle = -1. * abs_angle(DIR, xaxis, dim);
+  normn(rotAx, 3);
+  double avM = 0.0;
+  for (int t = b; t < e; t++) avM += C.sp[S.sp[t]].mfpot;
+  avM /= (float)(e - b);
+  double effM = avM * 0.5 * (double)dim;
+  double dU[3] = {0, 0, 0};
+  for (int t = b; t < e; t++) {
+    double U[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, T[3];
+    double rfc = C.sp[S.sp[t]].rfc;
+    for (int i = 0; i < dim; i++) dU[i] = U[i];
+    ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, R.kbt, Sv, effM);
+    for (int i = 0; i < dim; i++) dU[i] = (U[i] - dU[i]) / R.dt;
+    cross3(U, dU, T);
+    for (int i = 0; i < 3; i++) T[i] *= rfc;
+    S.ux[t] = (Real)U[0];
+    S.uy[t] = (Real)U[1];
+    S.uz[t] = (Real)U[2];
+    for (int i = 0; i < 3; i++) C.T[(size_t)t * 3 + i] = T[i];
+  }
+  // jefferysTorque + larsonRotRate, lc.c:559-718
+  const double *E = &C.cell.E[(size_t)c * 9];
+  double D[3][3], W[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      D[j][i] = 0.5 * (E[i * 3 + j] + E[j * 3 + i]);
+      W[j][i] = 0.5 * (E[i * 3 + j] - E[j * 3 + i]);
+    }
+  for (int t = b; t < e; t++) {
+    const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+    double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, dudt[3], w[3];
+    double uuuD[3] = {0, 0, 0}, uw[3] = {0, 0, 0}, uD[3] = {0, 0, 0};
+    for (int j = 0; j < 3; j++) {
+      for (int i = 0; i < 3; i++) {
+        for (int k = 0; k < 3; k++) uuuD[j] += u[j] * (u[i] * D[i][k] * u[k]);
+        uD[j] += u[i] * D[i][j];
+        uw[j] += u[i] * W[i][j];
+      }
+      dudt[j] = uw[j] + sp.tumble * (uD[j] - uuuD[j]);
+    }
+    cross3(u, dudt, w);
+    for (int i = 0; i < 3; i++) w[i] *= sp.chihi;
+    for (int i = 0; i < 3; i++) w[i] *= R.dt;
+    double theta = lengthn(w, 3);
+    normn(w, 3);
+    rodrigues(u, w, theta);
+    S.ux[t] = (Real)u[0];
+    S.uy[t] = (Real)u[1];
+    S.uz[t] = (Real)u[2];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 10. collision operators (MPCcollision switch, mpc.c:2777-2812), then the
+//     kinetic-energy partial sums of TEMP (therm.c:209) and gridShift back
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+  const BlkDev B = cblk[blockIdx.x];
+  int c = B.beg + threadIdx.x;
+  const RepDev &R = C.rep[B.rep];
+  const int dim = R.dim;
+  double ke = 0.0;
+  if (c < B.end) {
+    int b = C.start[c], e = C.start[c + 1], n = e - b;
+    double vcm[3] = {C.cell.vcm[c * 3], C.cell.vcm[c * 3 + 1], C.cell.vcm[c * 3 + 2]};
+    if (n > 1) {
+      if (R.lc == MPCD_ISOF && (R.rtech == MPCD_ARBAXIS || R.rtech == MPCD_ORTHAXIS)) {
+        // stochrotMPC + rotate, mpc.c:1181-1333
+        double RV[3] = {0., 0., 0.}, M[3][3];
+        int CA = 0;
+        if (R.rtech == MPCD_ARBAXIS) {
+          if (dim == 3) Dr.cell_axis(B.rep, c, RV);
+          else { double x = Dr.cell_u(B.rep, c, 0); RV[0] = x; RV[1] = sqrt(1. - x * x); }
+        } else {
+          CA = (int)((int)dim * Dr.cell_u(B.rep, c, 0));
+        }
+        double sgn = Dr.cell_u(B.rep, c, 3);
+        double Cc = R.cosA, Sn;
+        if (R.true_sign ? (sgn >= 0.5) : ((long)sgn > 0.5)) Sn = R.sinA;
+        else Sn = -1. * R.sinA;
+        if (dim == 3) {
+          const double *L = RV;
+          if (R.rtech == MPCD_ARBAXIS) {
+            M[0][0] = L[0] * L[0] + (1. - L[0] * L[0]) * Cc;
+            M[0][1] = L[0] * L[1] * (1. - Cc) - L[2] * Sn;
+            M[0][2] = L[0] * L[2] * (1. - Cc) + L[1] * Sn;
+            M[1][0] = L[0] * L[1] * (1. - Cc) + L[2] * Sn;
+            M[1][1] = L[1] * L[1] + (1. - L[1] * L[1]) * Cc;
+            M[1][2] = L[1] * L[2] * (1. - Cc) - L[0] * Sn;
+            M[2][0] = L[0] * L[2] * (1. - Cc) - L[1] * Sn;
+            M[2][1] = L[1] * L[2] * (1. - Cc) + L[0] * Sn;
+            M[2][2] = L[2] * L[2] + (1. - L[2] * L[2]) * Cc;
+          } else if (CA == 0) {
+            M[0][0] = 1.; M[0][1] = 0.; M[0][2] = 0.;
+            M[1][0] = 0.; M[1][1] = Cc; M[1][2] = Sn;
+            M[2][0] = 0.; M[2][1] = -.1 * Sn; M[2][2] = Cc;  // mpc.c:1215 as written
+          } else if (CA == 1) {
+            M[0][0] = Cc; M[0][1] = 0.; M[0][2] = Sn;
+            M[1][0] = 0.; M[1][1] = 1.; M[1][2] = 0.;
+            M[2][0] = -1. * Sn; M[2][1] = 0.; M[2][2] = Cc;
+          } else {
+            M[0][0] = Cc; M[0][1] = Sn; M[0][2] = 0.;
+            M[1][0] = -1. * Sn; M[1][1] = Cc; M[1][2] = 0.;
+            M[2][0] = 0.; M[2][1] = 0.; M[2][2] = 1.;
+          }
+        } else {
+          M[0][0] = Cc; M[0][1] = -1. * Sn;
+          M[1][0] = Sn; M[1][1] = Cc;
+        }
+        for (int t = b; t < e; t++) {
+          double V[3] = {(double)S.vx[t] - vcm[0], (double)S.vy[t] - vcm[1], (double)S.vz[t] - vcm[2]};
+          double TM[3] = {0.0, 0.0, 0.0};
+          for (int i = 0; i < dim; i++)
+            for (int j = 0; j < dim; j++) TM[i] += M[i][j] * V[j];
+          S.vx[t] = (Real)(vcm[0] + TM[0]);
+          S.vy[t] = (Real)(vcm[1] + TM[1]);
+          if (dim == 3) S.vz[t] = (Real)(vcm[2] + TM[2]);
+        }
+      } else if (R.lc == MPCD_ISOF && R.rtech == MPCD_MPCAT) {
+        // andersenMPC, mpc.c:1353-1444
+        double RS[3] = {0, 0, 0};
+        for (int t = b; t < e; t++) {
+          double m = C.sp[S.sp[t]].mass, sig = sqrt(R.kbt / m);
+          for (int j = 0; j < dim; j++) RS[j] += m * (sig * Dr.part_n(B.rep, t, j));
+        }
+        for (int j = 0; j < dim; j++) RS[j] /= C.cell.mass[c];
+        for (int t = b; t < e; t++) {
+          const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+          double sig = sqrt(R.kbt / sp.mass), v[3];
+          for (int j = 0; j < dim; j++) {
+            double rv = sig * Dr.part_n(B.rep, t, j);
+            double dv = (sp.damp) * (vcm[j]) / ((double)n);
+            v[j] = vcm[j] + rv - RS[j] - dv;
+          }
+          S.vx[t] = (Real)v[0];
+          S.vy[t] = (Real)v[1];
+          if (dim == 3) S.vz[t] = (Real)v[2];
+        }
+      } else {
+        // andersenROT (mpc.c:1461-1657), andersenROT_LC (lc.c:1515-1728),
+        // dipoleAndersenROT_LC (lc.c:1749-2043)
+        const int mode = (R.lc == MPCD_ISOF) ? 0
+                         : (R.rtech == MPCD_DIPOLE_DIR_SUM || R.rtech == MPCD_DIPOLE_DIR_AV ||
+                            R.rtech == MPCD_DIPOLE_DIR_SIG || R.rtech == MPCD_DIPOLE_DIR_SIG_SUM) ? 2 : 1;
+        const double *cm = &C.cell.cm[c * 3];
+        double ACT = 0., PA[3] = {0, 0, 0};
+        if (mode == 2) {
+          int spc[MPCD_MAXSPEC];
+          for (int k = 0; k < MPCD_MAXSPEC; k++) spc[k] = 0;
+          for (int t = b; t < e; t++) spc[S.sp[t]]++;
+          double sigW = 0., sigP = 0.;
+          for (int t = b; t < e; t++) {
+            const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+            if ((sp.minactratio == 0.0) || (sp.minactratio * R.gndnst < (double)spc[S.sp[t]])) {
+              ACT += sp.act / sp.mass;
+              sigW += sp.sigwidth;
+              sigP += sp.sigpos;
+            }
+          }
+          if (R.rtech == MPCD_DIPOLE_DIR_AV || R.rtech == MPCD_DIPOLE_DIR_SIG) ACT *= R.gndnst / ((double)n);
+          if (R.rtech == MPCD_DIPOLE_DIR_SIG || R.rtech == MPCD_DIPOLE_DIR_SIG_SUM) {
+            sigW /= (double)n;
+            sigP /= (double)n;
+            double fall = (1 - tanh(((double)n - R.gndnst * (1 + sigP)) / (R.gndnst * sigW)));
+            ACT *= fall / 2;
+          }
+          ACT *= R.dt;
+          for (int j = 0; j < dim; j++) PA[j] = C.cell.dir[c * 3 + j];
+        }
+        const bool act = (mode == 2) && fneq(ACT, 0.0);
+        double RS[3] = {0, 0, 0}, AS[3] = {0, 0, 0};
+        for (int t = b; t < e; t++) {
+          const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+          double sig = sqrt(R.kbt / sp.mass);
+          for (int j = 0; j < dim; j++) RS[j] += sp.mass * (sig * Dr.part_n(B.rep, t, j));
+          if (act) {
+            double q[3], pW = 0.0;
+            load_q(S, t, q);
+            for (int j = 0; j < dim; j++) pW += PA[j] * (q[j] - cm[j]);
+            pW -= 0.0;
+            double pm = (pW <= 0) ? -1. : 1.;
+            for (int j = 0; j < dim; j++) AS[j] += sp.mass * (pm * ACT * PA[j]);
+          }
+        }
+        for (int j = 0; j < dim; j++) RS[j] /= C.cell.mass[c];
+        if (mode == 2) for (int j = 0; j < dim; j++) AS[j] /= C.cell.mass[c];
+        double II[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        const double *I6 = &C.cell.I[c * 6];
+        if (dim == 3) {
+          if (n >= 3) {
+            double Im[3][3] = {{I6[0], I6[3], I6[4]}, {I6[3], I6[1], I6[5]}, {I6[4], I6[5], I6[2]}};
+            inv3(II, Im);
+          }
+        } else {
+          II[2][2] = 1. / I6[2];
+        }
+        double L[3] = {0, 0, 0}, Llm[3] = {0, 0, 0}, Lnm[3] = {0, 0, 0};
+        for (int t = b; t < e; t++) {
+          const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+          double sig = sqrt(R.kbt / sp.mass), q[3], rq[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, am[3];
+          double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
+          load_q(S, t, q);
+          for (int j = 0; j < dim; j++) rq[j] = q[j] - cm[j];
+          for (int j = 0; j < dim; j++) {
+            double rv = sig * Dr.part_n(B.rep, t, j);
+            if (act) {
+              double pW = 0.0;
+              for (int jj = 0; jj < dim; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
+              pW -= 0.0;
+              double pm = (pW <= 0) ? -1. : 1.;
+              dv[j] = sp.mass * (v[j] - rv - pm * ACT * PA[j]);
+            } else if (mode == 2) {
+              dv[j] = sp.mass * (v[j] - rv - 0.0);
+            } else {
+              dv[j] = sp.mass * (v[j] - rv);
+            }
+          }
+          cross3(rq, dv, am);
+          if (mode == 0) for (int j = 0; j < 3; j++) L[j] += am[j];
+          else {
+            for (int j = 0; j < 3; j++) Llm[j] += am[j];
+            for (int j = 0; j < 3; j++) Lnm[j] += C.T[(size_t)t * 3 + j] * R.dt;
+          }
+        }
+        if (mode != 0) for (int j = 0; j < 3; j++) L[j] = Llm[j] + Lnm[j];
+        double Wv[3];
+        for (int i = 0; i < 3; i++) {
+          Wv[i] = 0.;
+          for (int j = 0; j < 3; j++) Wv[i] += II[i][j] * L[j];
+        }
+        for (int t = b; t < e; t++) {
+          const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+          double sig = sqrt(R.kbt / sp.mass), q[3], rq[3] = {0, 0, 0}, at[3], v[3];
+          load_q(S, t, q);
+          for (int j = 0; j < dim; j++) rq[j] = q[j] - cm[j];
+          cross3(Wv, rq, at);
+          double pm = 0.0;
+          if (act) {
+            double pW = 0.0;
+            for (int jj = 0; jj < dim; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
+            pW -= 0.0;
+            pm = (pW <= 0) ? -1. : 1.;
+          }
+          for (int j = 0; j < dim; j++) {
+            double rv = sig * Dr.part_n(B.rep, t, j);
+            if (mode == 0) v[j] = vcm[j] + rv - RS[j] - (sp.damp) * (vcm[j]) + at[j];
+            else if (mode == 1) v[j] = vcm[j] + rv - RS[j] + at[j];
+            else v[j] = vcm[j] + rv - RS[j] + at[j] + (act ? pm * ACT * PA[j] : 0.0) - AS[j] - (sp.damp) * (vcm[j]);
+          }
+          S.vx[t] = (Real)v[0];
+          S.vy[t] = (Real)v[1];
+          if (dim == 3) S.vz[t] = (Real)v[2];
+        }
+      }
+    }
+    // TEMP partial sum (post collision) and gridShift back
+    for (int t = b; t < e; t++) {
+      double m = C.sp[S.sp[t]].mass;
+      double v0 = (double)S.vx[t] - R.avv[0], v1 = (double)S.vy[t] - R.avv[1], v2 = (double)S.vz[t] - R.avv[2];
+      double tmp = v0 * v0;
+      tmp += v1 * v1;
+      tmp += v2 * v2;
+      ke += m * tmp;
+      if (R.galinv) {
+        double q[3];
+        load_q(S, t, q);
+        shift_back(R, &C.shift[B.rep * 3], q);
+        S.x[t] = (Real)q[0];
+        S.y[t] = (Real)q[1];
+        if (dim == 3) S.z[t] = (Real)q[2];
+      }
+    }
+  }
+  // block reduction of the kinetic energy
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = ke;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) C.tpart[blockIdx.x] = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// 11. TEMP (therm.c:209-235) and the thermostat factor (therm.c:52-143),
+//     including the active-matter guard of timestep() (mpc.c:4766-4778)
+// ---------------------------------------------------------------------------
+__global__ void k_temp_final(Ctx C, const BlkDev *cblk, int nblk) {
+  int r = threadIdx.x;
+  if (r >= C.nrep) return;
+  const RepDev &R = C.rep[r];
+  double K = 0.0;
+  for (int b = 0; b < nblk; b++)
+    if (cblk[b].rep == r) K += C.tpart[b];
+  K /= ((double)(R.npart * R.dim));
+  double g = 1.0;
+  if (R.tstech == MPCD_VSC) g = sqrt(R.kbt / K);
+  else if (R.tstech == MPCD_BEREND) g = sqrt(1. + (R.kbt / K - 1.) * R.dt / R.tau);
+  K = K * g * g;
+  if (R.rtech >= 7 && R.rtech <= 17 && K > 50.0 * R.kbt) {
+    double g2 = sqrt(R.kbt / K);
+    g *= g2;
+    K = K * g2 * g2;
+  }
+  C.gscale[r] = g;
+  C.kbtnow[r] = K;
+}
+
+template <typename Real>
+__global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
+  const BlkDev B = blk[blockIdx.x];
+  int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  double g = C.gscale[B.rep];
+  if (g == 1.0) return;
+  const RepDev &R = C.rep[B.rep];
+  S.vx[t] = (Real)((double)S.vx[t] * g);
+  S.vy[t] = (Real)((double)S.vy[t] * g);
+  if (R.dim == 3) S.vz[t] = (Real)((double)S.vz[t] * g);
+}
+
+// ---------------------------------------------------------------------------
+// 12. streaming and planar walls (stream_all mpc.c:680; MPC_BCcollision
+//     bc.c:1363-1496), then the unshifted cell of every particle
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
+  const BlkDev B = blk[blockIdx.x];
+  int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  const int dim = R.dim;
+  double Q[3], V[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
+  double U[3] = {0, 0, 0};
+  load_q(S, t, Q);
+  const bool lc = R.lc != MPCD_ISOF;
+  if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
+  const int spid = S.sp[t];
+  const mpcd_gpu_species &sp = C.sp[spid];
+  for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
+  // MPC_BCcollision
+  double t_step = R.dt, t_left = t_step, t_coll = 0.0;
+  int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
+  double n[3] = {0, 0, 0};
+  while (true) {
+    double tmin = t_left, chW = 1.0;
+    for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+      const mpcd_gpu_wall &W = C.wall[w];
+      if (!((W.inter_mask >> spid) & 1)) continue;
+      const double *wq = &C.wq_b[w * 3];
+      double tw = wall_W(W, wq, Q, dim);
+      if (tw < 0.0) {
+        double tc = W.R;
+        for (int i = 0; i < dim; i++) tc += W.A[i] * (wq[i] - Q[i]);
+        tc /= (W.A[0] * (-V[0]) + W.A[1] * (-V[1]) + W.A[2] * (-V[2]));
+        double t1 = t_left - tc;
+        if (t1 < tmin) { tmin = t1; chosen = w; chW = tw; }
+      }
+    }
+    t_coll = tmin;
+    if (chW > -kTOL) break;
+    cnt++;
+    const mpcd_gpu_wall &W = C.wall[chosen];
+    const double *wq = &C.wq_b[chosen * 3];
+    if (t_coll > -kTOL) {
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
+      for (int i = 0; i < dim; i++) n[i] = W.A[i];
+      normn(n, dim);
+      double Rr[3] = {0, 0, 0}, VR[3], Vr[3] = {0, 0, 0}, VN[3] = {0, 0, 0}, VT[3] = {0, 0, 0};
+      for (int i = 0; i < dim; i++) Rr[i] = Q[i] - wq[i];
+      cross3(W.L, Rr, VR);
+      for (int i = 0; i < dim; i++) Vr[i] = V[i] - W.V[i] - VR[i];
+      projn(Vr, n, VN, dim);
+      for (int i = 0; i < dim; i++) VT[i] = Vr[i] - VN[i];
+      double IMpart = 1.0 / sp.mass;
+      for (int i = 0; i < dim; i++) {
+        VN[i] *= W.MVN;
+        VN[i] += W.DVN * n[i];
+        VT[i] *= W.MVT;
+        VT[i] += W.DVT;
+      }
+      for (int i = 0; i < dim; i++) Vr[i] = VN[i] + VT[i];
+      for (int i = 0; i < dim; i++) Vr[i] += W.DVxyz[i];
+      for (int i = 0; i < dim; i++) Vr[i] += W.V[i] + VR[i];
+      for (int i = 0; i < dim; i++) {
+        Vr[i] -= V[i];
+        Vr[i] *= sp.mass;
+      }
+      for (int i = 0; i < dim; i++) V[i] += Vr[i] * 1.0 * IMpart;
+      double PN[3] = {0, 0, 0}, PT[3] = {0, 0, 0};
+      projn(Q, n, PN, dim);
+      for (int i = 0; i < dim; i++) PT[i] = Q[i] - PN[i];
+      for (int i = 0; i < dim; i++) {
+        PN[i] += W.DN * n[i] * (1. - 2. * kTOL);
+        PT[i] += W.DT;
+      }
+      for (int i = 0; i < dim; i++) Q[i] = PN[i] + PT[i];
+      if (lc) ori_bc(U, W, n, dim);
+      t_left -= t_coll;
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] + t_left * V[i];
+      if (wall_W(W, wq, Q, dim) < -kTOL)
+        for (int i = 0; i < dim; i++) Q[i] = Q[i] - t_left * V[i];
+    } else {
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
+    }
+    if (cnt > kNBOUNCE) {
+      flagCNT = 1;
+      if (cnt < 2 * kNBOUNCE && cnt2 == 0) {
+        atomicAdd(&C.events[2], 1ull);
+        if (t_left > kTOL) t_left *= 1.1;
+        else {
+          t_left = 0.9 * t_step;
+          t_coll = 0.1 * t_step;
+        }
+      } else {
+        atomicAdd(&C.events[3], 1ull);
+        const uint32_t pid = (uint32_t)S.id[t];
+        for (int j = 0; j < dim; j++) {
+          if (sp.vdist == 0) V[j] = sqrt(R.kbt / sp.mass) * (2. * Dr.rng.uniform(B.rep, P_RETHERM, pid, rk + j) - 1.);
+          else V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(B.rep, P_RETHERM, pid, rk + j);
+        }
+        rk += 3;
+        t_left = t_step;
+        t_coll = 0.0;
+        cnt2++;
+        cnt = 0;
+      }
+      if (cnt2 > 1000) break;  // a particle that can never leave the walls
+    }
+  }
+  if (flagCNT) atomicAdd(&C.events[1], 1ull);
+  // the unshifted bin (mpc.c:4936): escaped particles are re-placed and stay
+  // in the list they were in (mpc.c:970-982)
+  int cu = cell_of(R, Q, 0);
+  if (cu < 0) {
+    atomicAdd(&C.events[0], 1ull);
+    const uint32_t pid = (uint32_t)S.id[t];
+    for (int j = 0; j < dim; j++) {
+      Q[j] = Dr.rng.uniform(B.rep, P_ESCAPE, pid, 8 + j) * R.L[j];
+      V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(B.rep, P_ESCAPE, pid, 24 + j);
+    }
+    cu = S.cst[t];
+  }
+  S.x[t] = (Real)Q[0];
+  S.y[t] = (Real)Q[1];
+  S.z[t] = (Real)Q[2];
+  S.vx[t] = (Real)V[0];
+  S.vy[t] = (Real)V[1];
+  S.vz[t] = (Real)V[2];
+  if (lc) {
+    S.ux[t] = (Real)U[0];
+    S.uy[t] = (Real)U[1];
+    S.uz[t] = (Real)U[2];
+  }
+  C.cu[t] = cu;
+}
+
+// ---------------------------------------------------------------------------
+// output helpers
+// ---------------------------------------------------------------------------
+__global__ void k_order_ids(const uint64_t *skey, const int32_t *id, int32_t *out, int beg, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = id[(int)(skey[beg + t] & 0x7FFFFFFFull)];
+}
+
+template <typename Real>
+__global__ void k_unsort(State<Real> S, int beg, int n, double *Q, double *V, double *U, int lc) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int s = beg + t;
+  int i = S.id[s];
+  Q[i * 3 + 0] = (double)S.x[s];
+  Q[i * 3 + 1] = (double)S.y[s];
+  Q[i * 3 + 2] = (double)S.z[s];
+  V[i * 3 + 0] = (double)S.vx[s];
+  V[i * 3 + 1] = (double)S.vy[s];
+  V[i * 3 + 2] = (double)S.vz[s];
+  if (lc && U) {
+    U[i * 3 + 0] = (double)S.ux[s];
+    U[i * 3 + 1] = (double)S.uy[s];
+    U[i * 3 + 2] = (double)S.uz[s];
+  }
+}
+
+template <typename Real>
+__global__ void k_stage(State<Real> S, int beg, int n, const double *Q, const double *V, const double *U,
+                        const int32_t *spid) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int s = beg + t;
+  S.x[s] = (Real)Q[t * 3 + 0];
+  S.y[s] = (Real)Q[t * 3 + 1];
+  S.z[s] = (Real)Q[t * 3 + 2];
+  S.vx[s] = (Real)V[t * 3 + 0];
+  S.vy[s] = (Real)V[t * 3 + 1];
+  S.vz[s] = (Real)V[t * 3 + 2];
+  if (U && S.ux) {
+    S.ux[s] = (Real)U[t * 3 + 0];
+    S.uy[s] = (Real)U[t * 3 + 1];
+    S.uz[s] = (Real)U[t * 3 + 2];
+  }
+  S.id[s] = t;
+  S.sp[s] = spid ? spid[t] : 0;
+  S.cst[s] = -1;
+}
+
+}  // namespace mpcd

@@ -1,0 +1,284 @@
+// mpcd_math.h -- per-cell and per-particle arithmetic of the MPCD solvent step
+// for gfx950.  Every helper keeps the operation order of the reference routine
+// it names, so that FP64 results agree with the reference to round-off (bit
+// for bit where only +,-,*,/,sqrt are involved; the library is compiled with
+// -ffp-contract=off for that reason).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpcd {
+
+constexpr double kTOL = 1e-8;          // definitions.h:57
+constexpr int kNBOUNCE = 50;           // definitions.h:63
+constexpr double kPI = 3.14159265358979323846;
+constexpr int kMCINT = 5;              // definitions.h:355
+constexpr double kMCSLOPE = 0.1;       // definitions.h:353
+constexpr double kBUSMIN = 0.5;        // definitions.h:179
+constexpr double kBUSMAX = 5.0;        // definitions.h:181
+
+#define MPCD_HD __host__ __device__ __forceinline__
+
+MPCD_HD bool feq(double x, double y) { return fabs(x - y) <= kTOL; }   // mtools.c:61
+MPCD_HD bool fneq(double x, double y) { return fabs(x - y) >= kTOL; }  // mtools.c:74
+
+MPCD_HD double dotn(const double *x, const double *y, int n) {         // mtools.c:111
+  double r = 0.;
+  for (int i = 0; i < n; i++) r += x[i] * y[i];
+  return r;
+}
+
+MPCD_HD void cross3(const double *x, const double *y, double *r) {     // mtools.c:177
+  r[0] = x[1] * y[2] - x[2] * y[1];
+  r[1] = x[2] * y[0] - x[0] * y[2];
+  r[2] = x[0] * y[1] - x[1] * y[0];
+}
+
+MPCD_HD double lengthn(const double *x, int n) {                       // mtools.c:227
+  double r = 0.;
+  for (int i = 0; i < n; i++) r += x[i] * x[i];
+  return sqrt(r);
+}
+
+MPCD_HD void normn(double *x, int n) {                                 // mtools.c:246
+  double l = lengthn(x, n);
+  if (fneq(l, 0.0))
+    for (int i = 0; i < n; i++) x[i] = x[i] / l;
+}
+
+MPCD_HD void projn(const double *v, const double *nr, double *vn, int n) {  // mtools.c:293
+  double x = dotn(v, nr, n);
+  for (int i = 0; i < n; i++) vn[i] = x * nr[i];
+}
+
+// the reference's atan2 (mtools.c:342-356), which overrides libm's
+MPCD_HD double ref_atan2(double y, double x) {
+  double at = 0.0;
+  if (x > 0.0) at = atan(y / x);
+  else if (x < 0.0 && y >= 0.0) at = atan(y / x) + kPI;
+  else if (x < 0.0 && y < 0.0) at = atan(y / x) - kPI;
+  else if (feq(x, 0.0) && y > 0.0) at = 0.5 * kPI;
+  else if (feq(x, 0.0) && y < 0.0) at = -0.5 * kPI;
+  return at;
+}
+
+MPCD_HD double abs_angle(const double *a, const double *b, int n) {    // mtools.c:325,365
+  double c = dotn(a, b, n);
+  c /= lengthn(a, n);
+  c /= lengthn(b, n);
+  return acos(c);
+}
+
+MPCD_HD double signed_angle(const double *v1, const double *v2, int n) {  // mtools.c:380
+  double A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, c3[3];
+  for (int i = 0; i < n; i++) { A[i] = v1[i]; B[i] = v2[i]; }
+  cross3(A, B, c3);
+  return ref_atan2(lengthn(c3, 3), dotn(A, B, n));
+}
+
+// rodriguesRotation, mtools.c:1572-1584 (renormalises the axis in place)
+MPCD_HD void rodrigues(double *vec, double *ax, double theta) {
+  double cp[3];
+  normn(ax, 3);
+  double oldl = lengthn(vec, 3);
+  cross3(ax, vec, cp);
+  double dp = dotn(ax, vec, 3);
+  for (int i = 0; i < 3; i++) vec[i] = vec[i] * cos(theta) + cp[i] * sin(theta) + ax[i] * dp * (1.0 - cos(theta));
+  double newl = lengthn(vec, 3);
+  for (int i = 0; i < 3; i++) vec[i] *= (oldl / newl);
+}
+
+MPCD_HD double det3(const double m[3][3]) {                            // mtools.c:857
+  double c1 = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+  double c2 = m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+  double c3 = m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  return (c1 - c2 + c3);
+}
+
+// invert3x3 via cofactors, mtools.c:972-1037
+MPCD_HD void inv3(double inv[3][3], const double m[3][3]) {
+  double det = det3(m);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      int r0 = (i == 0) ? 1 : 0, r1 = (i == 2) ? 1 : 2;
+      int c0 = (j == 0) ? 1 : 0, c1 = (j == 2) ? 1 : 2;
+      int sg = (2 * ((i + j) / 2) == (i + j)) ? 1 : -1;
+      double cof = sg * (m[r0][c0] * m[r1][c1] - m[r0][c1] * m[r1][c0]);
+      inv[j][i] = cof / det;
+    }
+}
+
+// eigenvalues2x2 + eigenvectors2x2, mtools.c:1230-1305.  Returns the largest
+// eigenvalue and writes the first eigenvector (the cell director).
+MPCD_HD double eig2_top(const double m[2][2], double v0[2]) {
+  double det = m[0][0] * m[1][1] - m[1][0] * m[0][1];
+  double tr = m[0][0] + m[1][1];
+  double sq = sqrt(tr * tr * 0.25 - det);
+  double e0 = tr * 0.5 + sq;
+  v0[0] = 0.0;
+  v0[1] = 0.0;
+  if (fneq(m[1][0], 0.0)) {
+    v0[0] = e0 - m[1][1];
+    v0[1] = m[1][0];
+    normn(v0, 2);
+  } else if (fneq(m[0][1], 0.0)) {
+    v0[0] = m[0][1];
+    v0[1] = e0 - m[0][0];
+    normn(v0, 2);
+  } else {
+    if (feq(m[0][0], m[1][1])) {
+      v0[0] = sqrt(2.);
+      v0[1] = v0[0];
+    } else if (feq(m[0][0], e0)) {
+      v0[0] = 1.0;
+      v0[1] = 0.0;
+    } else if (feq(m[1][1], e0)) {
+      v0[0] = 1.0;
+      v0[1] = 0.0;
+    }
+  }
+  return e0;
+}
+
+// eigenvalues3x3 (Smith's trigonometric method) + eigenvectors3x3
+// (Cayley-Hamilton column products), mtools.c:1307-1430.  Writes all three
+// eigenvalues (descending by construction) and the first eigenvector.
+MPCD_HD void eig3_top(const double m[3][3], double ev[3], double v0[3]) {
+  double p1 = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+  if (feq(p1, 0.0)) {
+    double q;
+    for (int i = 0; i < 3; i++) ev[i] = m[i][i];
+    if (ev[1] > ev[0]) { q = ev[0]; ev[0] = ev[1]; ev[1] = q; }
+    if (ev[2] > ev[0]) { q = ev[0]; ev[0] = ev[2]; ev[2] = q; }
+    if (ev[2] > ev[1]) { q = ev[1]; ev[1] = ev[2]; ev[2] = q; }
+  } else {
+    double p2 = 0.;
+    double q = (m[0][0] + m[1][1] + m[2][2]) / 3.;
+    for (int i = 0; i < 3; i++) p2 += (m[i][i] - q) * (m[i][i] - q);
+    p2 += 2. * p1;
+    double p = sqrt(p2 / 6.);
+    double ip = 1. / p;
+    double B[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) B[i][j] = m[i][j];
+    for (int i = 0; i < 3; i++) B[i][i] -= q;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) B[i][j] *= ip;
+    double r = det3(B) * 0.5, phi;
+    if (r <= -1.) phi = kPI / 3.;
+    else if (r >= 1.) phi = 0.;
+    else phi = acos(r) / 3.;
+    ev[0] = q + 2. * p * cos(phi);
+    ev[2] = q + 2. * p * cos(phi + (2. * kPI / 3.));
+    ev[1] = 3. * q - ev[0] - ev[2];
+  }
+  // first eigenvector: product (m - e1)(m - e2), first non-zero column
+  for (int col = 0; col < 3; col++) {
+    for (int row = 0; row < 3; row++) v0[row] = 0.;
+    for (int row = 0; row < 3; row++)
+      for (int i = 0; i < 3; i++) {
+        double a = m[row][i];
+        if (i == row) a -= ev[1];
+        double b = m[i][col];
+        if (i == col) b -= ev[2];
+        v0[row] += a * b;
+      }
+    if (!(feq(v0[0], 0.0) && feq(v0[1], 0.0) && feq(v0[2], 0.0))) break;
+  }
+  normn(v0, 3);
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based random numbers: Philox-4x32-10 (Salmon et al., SC11).
+// A variate is a pure function of (seed, replica, step, purpose, key, k):
+// counter = {key, purpose << 24 | block, step, replica}, key = seed.
+// ---------------------------------------------------------------------------
+MPCD_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+  }
+}
+
+MPCD_HD double u53(uint32_t a, uint32_t b) {
+  return (double)(((uint64_t)a << 21) | (uint64_t)(b >> 11)) * (1.0 / 9007199254740992.0);
+}
+
+struct Rng {
+  uint64_t seed;
+  uint32_t step;
+  MPCD_HD void block(uint32_t replica, int purpose, uint32_t key, uint32_t blk, uint32_t w[4]) const {
+    w[0] = key;
+    w[1] = ((uint32_t)purpose << 24) | (blk & 0xFFFFFFu);
+    w[2] = step;
+    w[3] = replica;
+    philox4x32_10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+  }
+  MPCD_HD double uniform(uint32_t rep, int purpose, uint32_t key, uint32_t k) const {
+    uint32_t w[4];
+    block(rep, purpose, key, k, w);
+    return u53(w[0], w[1]);
+  }
+  MPCD_HD double normal(uint32_t rep, int purpose, uint32_t key, uint32_t k) const {
+    uint32_t w[4];
+    block(rep, purpose, key, k >> 1, w);
+    double u1 = 1.0 - u53(w[0], w[1]);
+    double u2 = u53(w[2], w[3]);
+    double r = sqrt(-2.0 * log(u1));
+    return (k & 1) ? r * sin(6.283185307179586 * u2) : r * cos(6.283185307179586 * u2);
+  }
+  MPCD_HD void unitvec(uint32_t rep, int purpose, uint32_t key, uint32_t k, double v[3]) const {
+    uint32_t w[4];
+    block(rep, purpose, key, k, w);
+    double z = 1.0 - 2.0 * u53(w[0], w[1]);
+    double phi = 6.283185307179586 * u53(w[2], w[3]);
+    double s = sqrt((1.0 - z) * (1.0 + z));
+    v[0] = s * cos(phi);
+    v[1] = s * sin(phi);
+    v[2] = z;
+  }
+};
+
+enum Purpose { P_SHIFT = 1, P_CELL = 2, P_GHOST = 3, P_NORMAL = 4, P_LC = 5, P_ESCAPE = 6, P_RETHERM = 7 };
+
+// Draw source seen by the kernels: the counter-based generator or injected
+// arrays in the keyed layout of include/mpcd_gpu.h.
+struct Draws {
+  Rng rng;
+  int injected;
+  const double *shift, *cell, *ghost, *part, *lc;
+  int lc_stride;
+
+  MPCD_HD double cell_u(uint32_t rep, uint32_t c, int k) const {
+    return injected ? cell[(size_t)c * 4 + k] : rng.uniform(rep, P_CELL, c, k);
+  }
+  MPCD_HD void cell_axis(uint32_t rep, uint32_t c, double v[3]) const {
+    if (injected) { v[0] = cell[(size_t)c * 4]; v[1] = cell[(size_t)c * 4 + 1]; v[2] = cell[(size_t)c * 4 + 2]; }
+    else rng.unitvec(rep, P_CELL, c, 0, v);
+  }
+  MPCD_HD double ghost_n(uint32_t rep, uint32_t c, int k) const {
+    return injected ? ghost[(size_t)c * 3 + k] : rng.normal(rep, P_GHOST, c, k);
+  }
+  MPCD_HD double part_n(uint32_t rep, uint32_t slot, int k) const {
+    return injected ? part[(size_t)slot * 3 + k] : rng.normal(rep, P_NORMAL, slot, k);
+  }
+  MPCD_HD double lc_u(uint32_t rep, uint32_t slot, int k) const {
+    return injected ? lc[(size_t)slot * lc_stride + k] : rng.uniform(rep, P_LC, slot, k);
+  }
+  MPCD_HD double lc_n(uint32_t rep, uint32_t slot, int k) const {
+    return injected ? lc[(size_t)slot * lc_stride + k] : rng.normal(rep, P_LC, slot, k);
+  }
+};
+
+}  // namespace mpcd

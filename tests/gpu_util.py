@@ -1,0 +1,58 @@
+"""Shared helpers for the GPU tests: load the package, build keyed draws with
+the oracle, and run the GPU library through its C-ABI."""
+import importlib.util
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def package():
+    if "niamh_mpcd_amd" in sys.modules:
+        return sys.modules["niamh_mpcd_amd"]
+    spec = importlib.util.spec_from_file_location("niamh_mpcd_amd", os.path.join(ROOT, "niamh-mpcd_amd", "__init__.py"))
+    m = importlib.util.module_from_spec(spec)
+    sys.modules["niamh_mpcd_amd"] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def load_golden(name):
+    z = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    g = {k: z[k] for k in z.files}
+    g["case_d"] = json.loads(str(g["case"]))
+    return g
+
+
+def gpu_cfg(c, gndnst, avv):
+    cfg = dict(dim=c["dim"], L=c["L"], dt=c["dt"], kbt=c["kbt"], rotang=c["rotang"], tau=c["tau"], grav=c["grav"],
+               avv=list(avv), gndnst=gndnst, rtech=c["rtech"], lc=c["lc"], tstech=c["tstech"], galinv=c["galinv"],
+               species=c["species"], walls=c["walls"])
+    return cfg
+
+
+class KeyedRecorder:
+    """Runs the oracle in sequential mode (== the reference's draws) and
+    records every consumed variate into the keyed layout of the C-ABI."""
+
+    def __init__(self, oracle_sys, seed, ncells, n):
+        import pyoracle
+        self.po = pyoracle
+        self.s = oracle_sys
+        self.d = pyoracle.Draws(pyoracle.DRAW_SEQ, seed)
+        self.nc, self.n = ncells, n
+        self.stride = oracle_sys.lc_stride()
+
+    def step(self):
+        shift = np.zeros(3)
+        cell = np.zeros(self.nc * 4)
+        ghost = np.zeros(self.nc * 3)
+        part = np.zeros(self.n * 3)
+        lc = np.zeros(self.n * self.stride)
+        self.d.set_record(shift, cell, ghost, part, lc, self.stride)
+        self.s.step(self.d)
+        return dict(shift=shift, cell=cell, ghost=ghost, part=part, lc=lc, lc_stride=self.stride)

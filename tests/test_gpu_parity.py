@@ -1,0 +1,152 @@
+"""GPU parity: the HIP library (through its C-ABI) against the reference's own
+step and against the oracle.
+
+* test_gpu_matches_reference: every golden case recorded from the reference
+  (tests/golden) is replayed on the GPU with the reference's draws injected
+  in the keyed layout (recorded by the oracle's sequential mode, which equals
+  the reference draw for draw).  List order and cell populations must match
+  bit-exactly; FP64 positions, velocities and orientations within 1e-12
+  relative (plus 1e-12 absolute near zero).
+* test_gpu_phases: the three-phase API (bin / collide / stream) gives the
+  same result as the whole-step entry point.
+* test_gpu_philox_vs_oracle: internal counter-based RNG mode against the
+  oracle's Philox mode on larger boxes.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+import gpu_util as U  # noqa: E402
+import cases as CS  # noqa: E402
+
+NAMES = [c["name"] for c in CS.CASES]
+RTOL = 1e-12
+ATOL = 1e-12
+
+
+def _assert_close(a, b, what, rtol=RTOL, atol=ATOL):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    err = np.abs(a - b)
+    lim = atol + rtol * np.abs(b)
+    bad = err > lim
+    if bad.any():
+        i = np.unravel_index(np.argmax(err - lim), err.shape)
+        raise AssertionError(f"{what}: {bad.sum()} entries out of tolerance; worst at {i}: {a[i]!r} vs {b[i]!r}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+def test_gpu_matches_reference(name):
+    import pyoracle
+    M = U.package()
+    g = U.load_golden(name)
+    c = g["case_d"]
+    cfg = U.gpu_cfg(c, float(g["gndnst"]), g["avv"])
+    print(f"[gpu parity] case={name} N={g['Q0'].shape[0]} steps={c['steps']}", flush=True)
+    ctx = M.Context(cfg, g["Q0"].shape[0], seed=c["seed"], precision=M.FP64)
+    ctx.upload(0, g["Q0"], g["V0"], g["U0"] if c["lc"] else None, g["spid"])
+    o, cs = ctx.order(0)
+    np.testing.assert_array_equal(o, g["order0"], err_msg=f"{name}: initial binning order")
+    ocfg = CS.oracle_cfg(c, float(g["gndnst"]), g["avv"])
+    osys = pyoracle.System(ocfg, g["Q0"], g["V0"], g["U0"], g["spid"])
+    rec = U.KeyedRecorder(osys, c["seed"], ctx.ncells[0], g["Q0"].shape[0])
+    for k in range(c["steps"]):
+        draws = rec.step()
+        ctx.step(draws)
+        order, cstart = ctx.order(0)
+        np.testing.assert_array_equal(np.diff(cstart), g["pop"][k], err_msg=f"{name} step {k} populations")
+        np.testing.assert_array_equal(order, g["order"][k], err_msg=f"{name} step {k} list order")
+        Q, V, Uo = ctx.download(0)
+        _assert_close(Q, g["Q"][k], f"{name} step {k} Q")
+        _assert_close(V, g["V"][k], f"{name} step {k} V")
+        if c["lc"]:
+            _assert_close(Uo, g["U"][k], f"{name} step {k} U", rtol=1e-11, atol=1e-11)
+    ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["srd3d_arb", "rat3d_channel", "lc2d_channel"])
+def test_gpu_phases(name):
+    """bin + collide + stream phases == one step (same injected draws)."""
+    import pyoracle
+    M = U.package()
+    g = U.load_golden(name)
+    c = g["case_d"]
+    cfg = U.gpu_cfg(c, float(g["gndnst"]), g["avv"])
+    N = g["Q0"].shape[0]
+    a = M.Context(cfg, N, seed=3)
+    b = M.Context(cfg, N, seed=3)
+    for x in (a, b):
+        x.upload(0, g["Q0"], g["V0"], g["U0"] if c["lc"] else None, g["spid"])
+    osys = pyoracle.System(CS.oracle_cfg(c, float(g["gndnst"]), g["avv"]), g["Q0"], g["V0"], g["U0"], g["spid"])
+    rec = U.KeyedRecorder(osys, c["seed"], a.ncells[0], N)
+    for k in range(2):
+        d = rec.step()
+        a.step(d)
+        cell, cnt, perm = b.bin(d, shifted=1, outputs=True)
+        assert cnt.sum() == N
+        assert sorted(perm.tolist()) == list(range(N))
+        b.collide(d)
+        b.stream()
+    for x, y in zip(a.download(0), b.download(0)):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a.order(0)[0], b.order(0)[0])
+
+
+def _random_box(dim, L, ppc, rtech, lc=0, seed=5, walls=None, **kw):
+    rng = np.random.default_rng(seed)
+    N = int(np.prod(L[:dim])) * ppc
+    Q = np.zeros((N, 3))
+    V = np.zeros((N, 3))
+    Uo = np.zeros((N, 3))
+    for d in range(dim):
+        Q[:, d] = rng.uniform(0, L[d], N)
+    V[:, :dim] = rng.normal(size=(N, dim))
+    if dim == 3:
+        u = rng.normal(size=(N, 3))
+        Uo = u / np.linalg.norm(u, axis=1)[:, None]
+    else:
+        th = rng.uniform(0, 2 * np.pi, N)
+        Uo[:, 0], Uo[:, 1] = np.cos(th), np.sin(th)
+    c = dict(name="rand", dim=dim, L=list(L) + [1] * (3 - len(L)), rtech=rtech, lc=lc, tstech=kw.get("tstech", 0),
+             rotang=kw.get("rotang", 2.2689280275926285), dt=kw.get("dt", 0.1), kbt=1.0,
+             grav=kw.get("grav", [0, 0, 0]), walls=walls if walls is not None else CS.periodic(dim, L),
+             species=[CS.species(**kw.get("sp", {}))], galinv=1, tau=0.5)
+    return c, Q, V, Uo
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spec", [
+    (3, (16, 16, 16), 10, 0, 0),
+    (3, (12, 10, 14), 10, 3, 0),
+    (2, (40, 40), 20, 16, 1),
+])
+def test_gpu_philox_vs_oracle(spec):
+    import pyoracle
+    M = U.package()
+    dim, L, ppc, rtech, lc = spec
+    c, Q, V, Uo = _random_box(dim, L, ppc, rtech, lc, sp=dict(mfpot=10.0, tumble=0.5, act=0.05))
+    gnd = float(ppc)
+    ocfg = CS.oracle_cfg(c, gnd, np.zeros(3))
+    osys = pyoracle.System(ocfg, Q, V, Uo, None)
+    d = pyoracle.Draws(pyoracle.DRAW_PHILOX, 77, 0)
+    ctx = M.Context(U.gpu_cfg(c, gnd, np.zeros(3)), Q.shape[0], seed=77)
+    ctx.upload(0, Q, V, Uo if lc else None, None)
+    print(f"[gpu philox] dim={dim} L={L} ppc={ppc} rtech={rtech} lc={lc}", flush=True)
+    for k in range(3):
+        osys.step(d)
+        ctx.step(None)
+    Qo, Vo, Uor = osys.state()
+    Qg, Vg, Ug = ctx.download(0)
+    o_ord, _ = osys.order()
+    g_ord, _ = ctx.order(0)
+    np.testing.assert_array_equal(g_ord, o_ord)
+    tol = 1e-11 if lc else 1e-12
+    _assert_close(Qg, Qo, "Q", rtol=tol, atol=tol)
+    _assert_close(Vg, Vo, "V", rtol=tol, atol=tol)
+    if lc:
+        _assert_close(Ug, Uor, "U", rtol=1e-10, atol=1e-10)
