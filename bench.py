@@ -1,0 +1,257 @@
+"""Benchmark: MPCD particle-steps/s on one MI355X, FP32-state batched and
+plain FP64, with the HBM roofline of the dominant kernel and the reference's
+CPU step timed on this host.
+
+Workload (both forms): 3D SRD, periodic 64^3 cells x 10 particles per cell
+(2 621 440 particles per replica), ARBAXIS rotation 130 degrees, dt 0.1,
+kT 1, random grid shift every step, no thermostat.
+  * headline `value`: FP32 particle state with FP64 cell sums, B independent
+    replicas (default 8) advancing together in one context;
+  * `fp64_plain`: FP64 state, one replica.
+A step is one full MPCD step (shift, bin, cell sums, collision, shift back,
+stream, walls) over every particle.  Inputs are resident in HBM before the
+timed region; the timed region is bracketed by device synchronisation.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas B]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "MPCD particle-steps/s on 1 MI355X (plain FP64 vs FP32 batched) + % HBM peak"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per particle-step of the whole step: one read and one
+# write of r and v (3D)
+B_ALG = {"fp64": 96.0, "fp32": 48.0}
+PREC = {"fp64": 8, "fp32": 4}
+
+
+def fam_bytes(fam, form):
+    """algorithmic bytes per particle that one launch of a kernel family must
+    move (each input read once, each output written once); p = bytes per real"""
+    p = PREC[form]
+    return {"count": 3 * p + 24,        # read r, slot cell, unshifted cell; write cell, rank, 64-bit key
+            "scatter": 24,              # read cell, rank, key; write key
+            "sort": 20,                 # read + write keys; write slot cells
+            "gather": 12 * p + 24,      # read key, r, v, id, species; write r, v, id, species
+            "props": 6 * p + 4,         # read r, v, species
+            "collide": 12 * p + 4,      # read r, v, species; write r, v
+            "stream": 12 * p + 8,       # read r, v, species; write r, v, unshifted cell
+            # fused: read key, r, v, id; write r, v, id, slot cell, next cell, rank, key
+            "fused": 12 * p + 44,
+            "ghost": 0, "lc": 6 * p + 4, "temp": 0, "scan": 0}[fam]
+ROTANG_130 = 2.2689280275926285
+
+
+def package():
+    spec = importlib.util.spec_from_file_location("niamh_mpcd_amd", os.path.join(ROOT, "niamh-mpcd_amd", "__init__.py"))
+    m = importlib.util.module_from_spec(spec)
+    sys.modules["niamh_mpcd_amd"] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def periodic_walls(L):
+    ws = []
+    for ax in range(3):
+        a1 = [0.0, 0.0, 0.0]
+        a1[ax] = 1.0
+        a2 = [0.0, 0.0, 0.0]
+        a2[ax] = -1.0
+        ws += [dict(A=a1, R=0.0, DN=float(L[ax])), dict(A=a2, R=-float(L[ax]), DN=float(L[ax]))]
+    return ws
+
+
+def srd_cfg(L, ppc):
+    sp = dict(mass=1.0, rfc=0.01, tumble=2.0, chihi=0.5, act=0.05, mfpot=10.0, damp=0.0, sigwidth=1.0, sigpos=1.0,
+              minactratio=0.0, vdist=4)
+    return dict(dim=3, L=list(L), dt=0.1, kbt=1.0, rotang=ROTANG_130, tau=0.5, grav=[0, 0, 0], avv=[0, 0, 0],
+                gndnst=float(ppc), rtech=0, lc=0, tstech=0, galinv=1, species=[sp], walls=periodic_walls(L))
+
+
+def initial_state(L, ppc, seed):
+    rng = np.random.default_rng(seed)
+    N = int(np.prod(L)) * ppc
+    Q = np.empty((N, 3))
+    for d in range(3):
+        Q[:, d] = rng.uniform(0.0, L[d], N)
+    V = rng.normal(size=(N, 3))
+    V -= V.mean(axis=0)
+    return Q, V
+
+
+def timed_run(M, ctx, steps, warmup, n_total):
+    ctx.run(warmup, use_graph=False)
+    ctx.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    ctx.run(steps, use_graph=False)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    timing = ctx.timing()
+    ctx.profile(False)
+    sec = t1 - t0
+    return dict(sec=sec, ms_per_step=1e3 * sec / steps, value=n_total * steps / sec, timing=timing)
+
+
+def roofline(timing, n_per_launch, form):
+    fam, (ms, cnt) = max(timing.items(), key=lambda kv: kv[1][0])
+    avg_s = (ms / max(cnt, 1)) * 1e-3
+    bytes_alg = fam_bytes(fam, form) * n_per_launch
+    achieved = bytes_alg / avg_s / 1e9
+    return dict(bound="hbm", kernel_family=fam, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(fam, form), avg_launch_ms=ms / max(cnt, 1),
+                alg_bytes_per_launch=bytes_alg)
+
+
+def measured_traffic(fam, form):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any"""
+    path = os.path.join(ROOT, "profiles", f"traffic_{fam}_{form}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(L=(64, 64, 64), ppc=10, steps=3):
+    """the reference's own C step (oracle/_ref/ref_harness, built from the
+    reference's sources) timed on one host core on a bounded sample of the
+    same workload; falls back to the oracle port when the build is absent"""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    Q, V = initial_state(L, ppc, seed=99)
+    N = Q.shape[0]
+    if os.path.exists(harness):
+        with tempfile.TemporaryDirectory() as d:
+            case = os.path.join(d, "case.txt")
+            state = os.path.join(d, "state.bin")
+            lines = ["dim 3", "L {} {} {}".format(*L), f"N {N}", f"steps {steps}", "rtech 0", "lc 0", "tstech 0",
+                     "galinv 1", "dt 0.1", "kbt 1.0", f"rotang {ROTANG_130!r}", "tau 0.5", "grav 0 0 0",
+                     "avv 0 0 0", "seed 5", "species 1 0.01 2 0.5 0.05 10 0 1 1 0 4 0.007"]
+            for w in periodic_walls(L):
+                v = list(w["A"]) + [0, 0, 0] + [w["R"], w["DN"], 0, 0, 0] + [0, 0, 0] + [1, 1, 1, 1] + \
+                    [1, 1, 1] + [0, 0, 0] + [0, 0, 0] + [0, 0, 1]
+                lines.append("wall " + " ".join(repr(float(x)) for x in v))
+            open(case, "w").write("\n".join(lines) + "\n")
+            with open(state, "wb") as f:
+                f.write(np.hstack([Q, V, np.zeros_like(Q)]).astype("<f8").tobytes())
+                f.write(np.zeros(N, "<i4").tobytes())
+            out = subprocess.run([harness, case, state, os.path.join(d, "o.bin"), "timing"], capture_output=True,
+                                 text=True, check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            r = json.loads(out.stdout.strip().splitlines()[-1])
+            sec = r["seconds"]
+        kind = "reference"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        import cases as CS
+        import pyoracle
+        cfg = srd_cfg(L, ppc)
+        cfg["walls"] = CS.periodic(3, L)
+        s = pyoracle.System(cfg, Q, V, np.zeros_like(Q), None)
+        d = pyoracle.Draws(pyoracle.DRAW_PHILOX, 5)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step(d)
+        sec = time.perf_counter() - t0
+        kind = "port"
+    return dict(value=N * steps / sec, unit="particle-steps/s", cores=1, kind=kind,
+                sample=f"3D SRD {L[0]}x{L[1]}x{L[2]} cells x {ppc} ppc ({N} particles), {steps} steps, one core")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--replicas", type=int, default=8)
+    ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--ppc", type=int, default=10)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "replicas shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    M = package()
+    L = (args.L,) * 3
+    cfg = srd_cfg(L, args.ppc)
+    result = {}
+    if args.only in ("fp32", "both"):
+        B = args.replicas
+        n1 = int(np.prod(L)) * args.ppc
+        ctx = M.Context([cfg] * B, [n1] * B, seed=2024, precision=M.FP32)
+        for r in range(B):
+            Q, V = initial_state(L, args.ppc, seed=100 + r)
+            ctx.upload(r, Q, V)
+        print(f"[bench] fp32 batched: {B} x {n1} particles, warmup {args.warmup}, steps {args.steps}",
+              file=sys.stderr, flush=True)
+        r32 = timed_run(M, ctx, args.steps, args.warmup, B * n1)
+        result["fp32"] = (r32, B * n1, B)
+        ctx.close()
+    if args.only in ("fp64", "both"):
+        n1 = int(np.prod(L)) * args.ppc
+        ctx = M.Context(cfg, n1, seed=2025, precision=M.FP64)
+        Q, V = initial_state(L, args.ppc, seed=7)
+        ctx.upload(0, Q, V)
+        k64 = args.fp64_steps or args.steps
+        print(f"[bench] fp64 plain: {n1} particles, warmup {args.warmup}, steps {k64}", file=sys.stderr, flush=True)
+        r64 = timed_run(M, ctx, k64, args.warmup, n1)
+        result["fp64"] = (r64, n1, 1)
+        ctx.close()
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    head_key = "fp32" if "fp32" in result else "fp64"
+    r, n, B = result[head_key]
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps if head_key == "fp32" else (args.fp64_steps or args.steps),
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state / f64 cell sums" if head_key == "fp32" else "f64",
+        "data": "synthetic (uniform positions, Maxwell-Boltzmann velocities)",
+        "config": {
+            "workload": f"3D SRD {L[0]}^3 cells x {args.ppc} ppc, ARBAXIS 130 deg, dt 0.1, kT 1, random grid shift, "
+                        f"periodic; {'FP32 state / FP64 cell sums, ' + str(B) + ' replicas' if head_key == 'fp32' else 'FP64, 1 replica'}",
+            "replicas": B, "particles": n, "collision": "SRD ARBAXIS", "precision": head_key,
+        },
+        "roofline": roofline(r["timing"], n, head_key),
+        "cpu_baseline": cpu,
+        "kernels_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in r["timing"].items() if v[1]},
+        "step_hbm_frac": r["value"] * B_ALG[head_key] / 1e9 / HBM_PEAK_GBS,
+    }
+    if "fp64" in result and head_key != "fp64":
+        r64, n64, _ = result["fp64"]
+        line["fp64_plain"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
+                              "roofline": roofline(r64["timing"], n64, "fp64"),
+                              "step_hbm_frac": r64["value"] * B_ALG["fp64"] / 1e9 / HBM_PEAK_GBS}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

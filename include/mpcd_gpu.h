@@ -174,8 +174,15 @@ mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *ctx, int replica, int32_t *order, i
 /* number of cells of replica `replica` and the keyed LC stride */
 int         mpcd_gpu_ncells(const mpcd_gpu_ctx *ctx, int replica);
 int         mpcd_gpu_lc_stride(const mpcd_gpu_ctx *ctx);
-/* device time of the last mpcd_gpu_run's kernels by kernel family, ms */
-mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *ctx, double *ms_out, int n);
+/* kernel timing with HIP events recorded on the library's stream around
+ * every kernel family while enabled (enable != 0 also clears the records) */
+mpcd_status mpcd_gpu_profile(mpcd_gpu_ctx *ctx, int enable);
+/* out[f] = total device ms of family f, out[12 + f] = number of timed
+ * launches, for the families 0 count (histogram + keys), 1 scan, 2 scatter,
+ * 3 in-cell sort, 4 gather, 5 cell properties, 6 ghost particles, 7 LC,
+ * 8 collision, 9 temperature/thermostat, 10 stream + walls, 11 fused
+ * gather + collision + stream + next binning; n >= 24 */
+mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *ctx, double *out, int n);
 
 #ifdef __cplusplus
 }

@@ -95,7 +95,7 @@ class DrawsC(C.Structure):
 EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_gpu_upload", "mpcd_gpu_download",
            "mpcd_gpu_bin", "mpcd_gpu_collide", "mpcd_gpu_stream", "mpcd_gpu_step", "mpcd_gpu_run",
            "mpcd_gpu_synchronize", "mpcd_gpu_get_cells", "mpcd_gpu_get_order", "mpcd_gpu_ncells",
-           "mpcd_gpu_lc_stride", "mpcd_gpu_timing"]
+           "mpcd_gpu_lc_stride", "mpcd_gpu_profile", "mpcd_gpu_timing"]
 
 _lib = None
 
@@ -125,6 +125,7 @@ def lib():
     L.mpcd_gpu_get_order.argtypes = [vp, C.c_int, ip, ip]
     L.mpcd_gpu_ncells.argtypes = [vp, C.c_int]
     L.mpcd_gpu_lc_stride.argtypes = [vp]
+    L.mpcd_gpu_profile.argtypes = [vp, C.c_int]
     L.mpcd_gpu_timing.argtypes = [vp, dp, C.c_int]
     _lib = L
     return L
@@ -285,6 +286,19 @@ class Context:
         pop, vcm, S, d = np.zeros(nc, np.int32), np.zeros((nc, 3)), np.zeros(nc), np.zeros((nc, 3))
         self._check(self.L.mpcd_gpu_get_cells(self.h, r, _ip(pop), _dp(vcm), _dp(S), _dp(d)), "get_cells")
         return dict(pop=pop, vcm=vcm, S=S, dir=d)
+
+    FAMILIES = ["count", "scan", "scatter", "sort", "gather", "props", "ghost", "lc", "collide", "temp", "stream",
+                "fused"]
+
+    def profile(self, enable=True):
+        self._check(self.L.mpcd_gpu_profile(self.h, int(bool(enable))), "profile")
+
+    def timing(self):
+        """{family: (total_ms, launches)} of the kernels timed since profile(True)"""
+        F = len(self.FAMILIES)
+        out = np.zeros(2 * F)
+        self._check(self.L.mpcd_gpu_timing(self.h, _dp(out), 2 * F), "timing")
+        return {f: (out[i], int(out[F + i])) for i, f in enumerate(self.FAMILIES)}
 
     def order(self, r=0):
         o = np.zeros(int(self.n[r]), np.int32)

@@ -1,0 +1,550 @@
+// mpcd_fused.h -- the fused, LDS-tiled step kernel for the isotropic
+// collision operators (SRD, MPC-AT, MPC-AT+a).
+//
+// One workgroup owns a tile of consecutive cells.  It
+//   1. loads the tile's sorted 64-bit keys and ranks them inside each cell
+//      (the composite key reproduces the reference's linked-list order),
+//   2. gathers the particles' r, v (and id, species) from the previous
+//      buffer into LDS, applying the grid shift and the body force,
+//   3. runs one thread per cell over its particles in list order for the
+//      FP64 cell sums (mass, centre-of-mass velocity and position, inertia
+//      tensor, random-velocity sums, angular momentum) -- the reference's own
+//      summation order, so FP64 results match it to round-off,
+//   4. applies the collision, the kinetic-energy partial sum, the shift
+//      back, streaming and the planar walls with one thread per particle,
+//   5. writes the new state coalesced, and -- when the next step's grid
+//      shift is known -- the next step's cell, in-cell rank and key plus its
+//      histogram (global atomics), so the next step starts at the scan.
+// Tiles larger than the LDS capacity are processed in cell-aligned chunks.
+#pragma once
+
+#include "mpcd_kernels.h"
+
+namespace mpcd {
+
+constexpr int kTileThreads = 256;
+constexpr int kTileCap = 640;     // particles staged in LDS at once
+constexpr int kTileCells = 64;    // cells per tile at most
+
+struct TileDev {
+  int rep, c0, c1, pad;
+};
+
+enum { OP_SRD = 0, OP_AT = 1, OP_RAT = 2 };
+
+// per-particle stream + walls + unshifted cell, shared by the fused kernel
+// and the split (thermostatted) path.  Returns the unshifted cell (or the
+// slot's cell for an escaped particle).
+__device__ __forceinline__ int stream_walls(const Ctx &C, const RepDev &R, const double *wq_b, const Draws &Dr,
+                                            int rep, int spid, int pid, int cell_this, double Q[3], double V[3],
+                                            double U[3], bool lc) {
+  const int dim = R.dim;
+  const mpcd_gpu_species &sp = C.sp[spid];
+  for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
+  double t_step = R.dt, t_left = t_step, t_coll = 0.0;
+  int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
+  double n[3] = {0, 0, 0};
+  while (R.nwall > 0) {
+    double tmin = t_left, chW = 1.0;
+    for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+      const mpcd_gpu_wall &W = C.wall[w];
+      if (!((W.inter_mask >> spid) & 1)) continue;
+      const double *wq = &wq_b[w * 3];
+      double tw = wall_W(W, wq, Q, dim);
+      if (tw < 0.0) {
+        double tc = W.R;
+        for (int i = 0; i < dim; i++) tc += W.A[i] * (wq[i] - Q[i]);
+        tc /= (W.A[0] * (-V[0]) + W.A[1] * (-V[1]) + W.A[2] * (-V[2]));
+        double t1 = t_left - tc;
+        if (t1 < tmin) { tmin = t1; chosen = w; chW = tw; }
+      }
+    }
+    t_coll = tmin;
+    if (chW > -kTOL) break;
+    cnt++;
+    const mpcd_gpu_wall &W = C.wall[chosen];
+    const double *wq = &wq_b[chosen * 3];
+    if (t_coll > -kTOL) {
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
+      for (int i = 0; i < dim; i++) n[i] = W.A[i];
+      normn(n, dim);
+      double Rr[3] = {0, 0, 0}, VR[3], Vr[3] = {0, 0, 0}, VN[3] = {0, 0, 0}, VT[3] = {0, 0, 0};
+      for (int i = 0; i < dim; i++) Rr[i] = Q[i] - wq[i];
+      cross3(W.L, Rr, VR);
+      for (int i = 0; i < dim; i++) Vr[i] = V[i] - W.V[i] - VR[i];
+      projn(Vr, n, VN, dim);
+      for (int i = 0; i < dim; i++) VT[i] = Vr[i] - VN[i];
+      double IMpart = 1.0 / sp.mass;
+      for (int i = 0; i < dim; i++) {
+        VN[i] *= W.MVN;
+        VN[i] += W.DVN * n[i];
+        VT[i] *= W.MVT;
+        VT[i] += W.DVT;
+      }
+      for (int i = 0; i < dim; i++) Vr[i] = VN[i] + VT[i];
+      for (int i = 0; i < dim; i++) Vr[i] += W.DVxyz[i];
+      for (int i = 0; i < dim; i++) Vr[i] += W.V[i] + VR[i];
+      for (int i = 0; i < dim; i++) {
+        Vr[i] -= V[i];
+        Vr[i] *= sp.mass;
+      }
+      for (int i = 0; i < dim; i++) V[i] += Vr[i] * 1.0 * IMpart;
+      double PN[3] = {0, 0, 0}, PT[3] = {0, 0, 0};
+      projn(Q, n, PN, dim);
+      for (int i = 0; i < dim; i++) PT[i] = Q[i] - PN[i];
+      for (int i = 0; i < dim; i++) {
+        PN[i] += W.DN * n[i] * (1. - 2. * kTOL);
+        PT[i] += W.DT;
+      }
+      for (int i = 0; i < dim; i++) Q[i] = PN[i] + PT[i];
+      if (lc) ori_bc(U, W, n, dim);
+      t_left -= t_coll;
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] + t_left * V[i];
+      if (wall_W(W, wq, Q, dim) < -kTOL)
+        for (int i = 0; i < dim; i++) Q[i] = Q[i] - t_left * V[i];
+    } else {
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
+    }
+    if (cnt > kNBOUNCE) {
+      flagCNT = 1;
+      if (cnt < 2 * kNBOUNCE && cnt2 == 0) {
+        atomicAdd(&C.events[2], 1ull);
+        if (t_left > kTOL) t_left *= 1.1;
+        else {
+          t_left = 0.9 * t_step;
+          t_coll = 0.1 * t_step;
+        }
+      } else {
+        atomicAdd(&C.events[3], 1ull);
+        for (int j = 0; j < dim; j++) {
+          if (sp.vdist == 0) V[j] = sqrt(R.kbt / sp.mass) * (2. * Dr.rng.uniform(rep, P_RETHERM, pid, rk + j) - 1.);
+          else V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(rep, P_RETHERM, pid, rk + j);
+        }
+        rk += 3;
+        t_left = t_step;
+        t_coll = 0.0;
+        cnt2++;
+        cnt = 0;
+      }
+      if (cnt2 > 1000) break;
+    }
+  }
+  if (flagCNT) atomicAdd(&C.events[1], 1ull);
+  int cu = cell_of(R, Q, 0);
+  if (cu < 0) {
+    atomicAdd(&C.events[0], 1ull);
+    for (int j = 0; j < dim; j++) {
+      Q[j] = Dr.rng.uniform(rep, P_ESCAPE, pid, 8 + j) * R.L[j];
+      V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(rep, P_ESCAPE, pid, 24 + j);
+    }
+    cu = cell_this;
+  }
+  return cu;
+}
+
+// next-step binning of one particle: histogram + rank + composite key
+__device__ __forceinline__ void next_bin(const Ctx &C, const RepDev &R, const double *shift_next, int32_t *hist_next,
+                                         int slot, int cell_this, int cu, const double Q[3]) {
+  double q[3] = {Q[0], Q[1], Q[2]};
+  shift_fwd(R, shift_next, q);
+  int cn = cell_of(R, q, 1);
+  if (cn < 0) cn = cu;
+  uint32_t a1 = (cu != cell_this) ? 1u : 0u, a2 = (cn != cu) ? 1u : 0u;
+  int l = atomicAdd(&hist_next[cn], 1);
+  C.cnew[slot] = cn;
+  C.loc[slot] = l;
+  C.key[slot] = ((uint64_t)a2 << 63) | ((uint64_t)(uint32_t)cu << 32) | ((uint64_t)a1 << 31) | (uint64_t)(uint32_t)slot;
+}
+
+template <typename Real, int OP, bool ONESP>
+__global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, State<Real> Sn, const TileDev *tiles,
+                                                        Draws Dr, const double *shift_cur, const double *wq_b,
+                                                        const double *shift_next, int32_t *hist_next, int do_next,
+                                                        int do_stream) {
+  const TileDev T = tiles[blockIdx.x];
+  const RepDev &R = C.rep[T.rep];
+  const int dim = R.dim, tid = threadIdx.x;
+  const double *shc = &shift_cur[T.rep * 3];
+
+  __shared__ int32_t s_start[kTileCells + 1];
+  __shared__ int32_t s_ce;
+  __shared__ uint64_t s_key[kTileCap];
+  __shared__ int32_t s_src[kTileCap];
+  __shared__ uint8_t s_cell[kTileCap];
+  __shared__ uint8_t s_sp[ONESP ? 1 : kTileCap];
+  __shared__ Real s_q[3][kTileCap];
+  __shared__ Real s_v[3][kTileCap];
+  __shared__ double s_rv[OP == OP_SRD ? 1 : 3][OP == OP_SRD ? 1 : kTileCap];
+  __shared__ double s_cd[kTileCells][12];  // per cell: vcm[3], then SRD matrix[9] or RS[3], W[3], CM[3]
+  __shared__ double s_ke[kTileThreads];
+
+  double ke = 0.0;
+  int cs = T.c0;
+  while (cs < T.c1) {
+    // cell-aligned chunk that fits the LDS capacity: the cell offsets of the
+    // rest of the tile are loaded in parallel and the chunk end is the last
+    // offset within capacity of the first (offsets are non-decreasing)
+    const int nrest = T.c1 - cs;
+    if (tid == 0) s_ce = cs + 1;
+    for (int k = tid; k <= nrest; k += kTileThreads) s_start[k] = C.start[cs + k];
+    __syncthreads();
+    for (int k = 2 + tid; k <= nrest; k += kTileThreads)
+      if (s_start[k] - s_start[0] <= kTileCap) atomicMax(&s_ce, cs + k);
+    __syncthreads();
+    const int ce = s_ce, ncl = ce - cs;
+    const int base = s_start[0];
+    int n = s_start[ncl] - base;
+    if (n > kTileCap) {
+      if (tid == 0) atomicOr(&C.flags[0], 1);  // one cell above capacity
+      n = kTileCap;                            // never overrun LDS
+    }
+    __syncthreads();
+    for (int k = tid; k <= ncl; k += kTileThreads) {
+      int v = s_start[k] - base;
+      s_start[k] = v > n ? n : v;
+    }
+    for (int e = tid; e < n; e += kTileThreads) s_key[e] = C.skey[base + e];
+    __syncthreads();
+    // rank inside the cell: the key order is the reference's list order
+    for (int e = tid; e < n; e += kTileThreads) {
+      int lo = 0, hi = ncl;  // last k with s_start[k] <= e
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_start[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      const uint64_t ke_ = s_key[e];
+      int r = 0;
+      for (int f = s_start[lo]; f < s_start[lo + 1]; f++) r += (s_key[f] < ke_) ? 1 : 0;
+      const int pos = s_start[lo] + r;
+      s_src[pos] = (int)(ke_ & 0x7FFFFFFFull);
+      s_cell[pos] = (uint8_t)lo;
+    }
+    __syncthreads();
+    // gather with grid shift and body force
+    for (int p = tid; p < n; p += kTileThreads) {
+      const int src = s_src[p];
+      double q[3];
+      load_q(So, src, q);
+      double v[3] = {(double)So.vx[src], (double)So.vy[src], (double)So.vz[src]};
+      if (R.galinv) shift_fwd(R, shc, q);
+      const int spid = ONESP ? 0 : So.sp[src];
+      if (cell_of(R, q, 1) < 0) {
+        atomicAdd(&C.events[0], 1ull);
+        const int id = So.id[src];
+        double m = C.sp[spid].mass;
+        for (int j = 0; j < dim; j++) {
+          q[j] = Dr.rng.uniform(T.rep, P_ESCAPE, (uint32_t)id, j) * R.L[j];
+          v[j] = sqrt(R.kbt / m) * Dr.rng.normal(T.rep, P_ESCAPE, (uint32_t)id, 16 + j);
+        }
+      }
+      s_q[0][p] = (Real)q[0];
+      s_q[1][p] = (Real)q[1];
+      s_q[2][p] = (Real)q[2];
+      if (R.gravflag) {
+        // acc_all (mpc.c:697), in the state precision
+        Real vv[3] = {(Real)v[0], (Real)v[1], (Real)v[2]};
+        for (int d = 0; d < dim; d++) vv[d] = (Real)((double)vv[d] + R.dt * R.grav[d]);
+        s_v[0][p] = vv[0];
+        s_v[1][p] = vv[1];
+        s_v[2][p] = vv[2];
+      } else {
+        s_v[0][p] = (Real)v[0];
+        s_v[1][p] = (Real)v[1];
+        s_v[2][p] = (Real)v[2];
+      }
+      Sn.id[base + p] = So.id[src];
+      if (!ONESP) {
+        Sn.sp[base + p] = spid;
+        s_sp[p] = (uint8_t)spid;
+      }
+    }
+    __syncthreads();
+    // one thread per cell: sums in list order and the collision's cell terms
+    for (int k = tid; k < ncl; k += kTileThreads) {
+      const int c = cs + k;
+      const int b = s_start[k], e = s_start[k + 1], np_ = e - b;
+      double mass = 0.0, vcm[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
+      for (int p = b; p < e; p++) {
+        const double m = C.sp[ONESP ? 0 : s_sp[p]].mass;
+        mass += m;
+        for (int d = 0; d < dim; d++) vcm[d] += (double)s_v[d][p] * m;
+        if (OP == OP_RAT)
+          for (int d = 0; d < dim; d++) cm[d] += (double)s_q[d][p] * m;
+      }
+      if (mass > 0.0)
+        for (int d = 0; d < dim; d++) {
+          vcm[d] /= mass;
+          if (OP == OP_RAT) cm[d] /= mass;
+        }
+      // ghost particles at phantom walls (ghostPart, mpc.c:351-567)
+      if (R.anyphantom && np_ > 0) {
+        int lc_ = c - R.cell_off;
+        int a = lc_ / (R.ny * R.nz), bb = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
+        const int nco = (dim == 3) ? 8 : 4;
+        int flagW = 0;
+        for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+          const mpcd_gpu_wall &W = C.wall[w];
+          if (!W.PHANTOM) continue;
+          for (int j = 0; j < nco; j++) {
+            double tq[3] = {(double)a + (double)(j & 1), (double)bb + (double)((j >> 1) & 1),
+                            (double)cz + (double)((j >> 2) & 1)};
+            if (wall_W(W, &C.wq_s[w * 3], tq, dim) < 0.0) flagW = 1;
+          }
+        }
+        if (flagW && np_ < R.gndnst) {
+          double invN = 1.0 / (R.gndnst - (double)np_);
+          for (int d = 0; d < dim; d++) {
+            double sigma = sqrt(R.kbt / invN);
+            vcm[d] += sigma * Dr.ghost_n(T.rep, c, d);
+            vcm[d] /= R.gndnst;
+          }
+        }
+      }
+      double *cd = s_cd[k];
+      cd[0] = vcm[0];
+      cd[1] = vcm[1];
+      cd[2] = vcm[2];
+      if (np_ <= 1) continue;
+      if (OP == OP_SRD) {
+        double RV[3] = {0., 0., 0.};
+        int CA = 0;
+        if (R.rtech == MPCD_ARBAXIS) {
+          if (dim == 3) Dr.cell_axis(T.rep, c, RV);
+          else { double x = Dr.cell_u(T.rep, c, 0); RV[0] = x; RV[1] = sqrt(1. - x * x); }
+        } else {
+          CA = (int)((int)dim * Dr.cell_u(T.rep, c, 0));
+        }
+        double sgn = Dr.cell_u(T.rep, c, 3);
+        double Cc = R.cosA, Sn_;
+        if (R.true_sign ? (sgn >= 0.5) : ((long)sgn > 0.5)) Sn_ = R.sinA;
+        else Sn_ = -1. * R.sinA;
+        double *M = cd + 3;
+        if (dim == 3) {
+          const double *L = RV;
+          if (R.rtech == MPCD_ARBAXIS) {
+            M[0] = L[0] * L[0] + (1. - L[0] * L[0]) * Cc;
+            M[1] = L[0] * L[1] * (1. - Cc) - L[2] * Sn_;
+            M[2] = L[0] * L[2] * (1. - Cc) + L[1] * Sn_;
+            M[3] = L[0] * L[1] * (1. - Cc) + L[2] * Sn_;
+            M[4] = L[1] * L[1] + (1. - L[1] * L[1]) * Cc;
+            M[5] = L[1] * L[2] * (1. - Cc) - L[0] * Sn_;
+            M[6] = L[0] * L[2] * (1. - Cc) - L[1] * Sn_;
+            M[7] = L[1] * L[2] * (1. - Cc) + L[0] * Sn_;
+            M[8] = L[2] * L[2] + (1. - L[2] * L[2]) * Cc;
+          } else if (CA == 0) {
+            M[0] = 1.; M[1] = 0.; M[2] = 0.; M[3] = 0.; M[4] = Cc; M[5] = Sn_; M[6] = 0.; M[7] = -.1 * Sn_; M[8] = Cc;
+          } else if (CA == 1) {
+            M[0] = Cc; M[1] = 0.; M[2] = Sn_; M[3] = 0.; M[4] = 1.; M[5] = 0.; M[6] = -1. * Sn_; M[7] = 0.; M[8] = Cc;
+          } else {
+            M[0] = Cc; M[1] = Sn_; M[2] = 0.; M[3] = -1. * Sn_; M[4] = Cc; M[5] = 0.; M[6] = 0.; M[7] = 0.; M[8] = 1.;
+          }
+        } else {
+          M[0] = Cc; M[1] = -1. * Sn_; M[2] = 0.;
+          M[3] = Sn_; M[4] = Cc; M[5] = 0.;
+          M[6] = 0.; M[7] = 0.; M[8] = 0.;
+        }
+      } else {
+        // thermal draws in list order and their mass-weighted sum
+        double RS[3] = {0, 0, 0};
+        for (int p = b; p < e; p++) {
+          const double m = C.sp[ONESP ? 0 : s_sp[p]].mass, sig = sqrt(R.kbt / m);
+          for (int j = 0; j < dim; j++) {
+            double rv = sig * Dr.part_n(T.rep, base + p, j);
+            s_rv[j][p] = rv;
+            RS[j] += m * rv;
+          }
+        }
+        for (int j = 0; j < dim; j++) RS[j] /= mass;
+        cd[3] = RS[0];
+        cd[4] = RS[1];
+        cd[5] = RS[2];
+        if (OP == OP_RAT) {
+          // inertia about the centre of mass (mpc.c:4209-4299) and angular momentum (mpc.c:1554-1618)
+          double I[6] = {0, 0, 0, 0, 0, 0};
+          for (int p = b; p < e; p++) {
+            const double m = C.sp[ONESP ? 0 : s_sp[p]].mass;
+            double r[3] = {0, 0, 0};
+            for (int d = 0; d < dim; d++) r[d] = (double)s_q[d][p] - cm[d];
+            I[0] += m * (r[1] * r[1] + r[2] * r[2]);
+            I[1] += m * (r[0] * r[0] + r[2] * r[2]);
+            I[2] += m * (r[0] * r[0] + r[1] * r[1]);
+            I[3] += m * r[0] * r[1];
+            I[4] += m * r[0] * r[2];
+            I[5] += m * r[1] * r[2];
+          }
+          I[3] *= -1.; I[4] *= -1.; I[5] *= -1.;
+          double II[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+          if (dim == 3) {
+            if (np_ >= 3) {
+              double Im[3][3] = {{I[0], I[3], I[4]}, {I[3], I[1], I[5]}, {I[4], I[5], I[2]}};
+              inv3(II, Im);
+            }
+          } else {
+            II[2][2] = 1. / I[2];
+          }
+          double L[3] = {0, 0, 0};
+          for (int p = b; p < e; p++) {
+            const double m = C.sp[ONESP ? 0 : s_sp[p]].mass;
+            double rq[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, am[3];
+            for (int j = 0; j < dim; j++) {
+              rq[j] = (double)s_q[j][p] - cm[j];
+              dv[j] = m * ((double)s_v[j][p] - s_rv[j][p]);
+            }
+            cross3(rq, dv, am);
+            for (int j = 0; j < 3; j++) L[j] += am[j];
+          }
+          for (int i = 0; i < 3; i++) {
+            double w = 0.;
+            for (int j = 0; j < 3; j++) w += II[i][j] * L[j];
+            cd[6 + i] = w;
+            cd[9 + i] = cm[i];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // one thread per particle: collision, kinetic energy, shift back, stream, walls, next bin
+    for (int p = tid; p < n; p += kTileThreads) {
+      const int k = s_cell[p], c = cs + k;
+      const int np_ = s_start[k + 1] - s_start[k];
+      const double *cd = s_cd[k];
+      const int spid = ONESP ? 0 : s_sp[p];
+      const mpcd_gpu_species &sp = C.sp[spid];
+      double v[3] = {(double)s_v[0][p], (double)s_v[1][p], (double)s_v[2][p]};
+      double q[3] = {(double)s_q[0][p], (double)s_q[1][p], (double)s_q[2][p]};
+      if (np_ > 1) {
+        const double vcm[3] = {cd[0], cd[1], cd[2]};
+        if (OP == OP_SRD) {
+          const double *M = cd + 3;
+          double Vr[3] = {v[0] - vcm[0], v[1] - vcm[1], v[2] - vcm[2]};
+          double TM[3] = {0.0, 0.0, 0.0};
+          for (int i = 0; i < dim; i++)
+            for (int j = 0; j < dim; j++) TM[i] += M[i * 3 + j] * Vr[j];
+          for (int d = 0; d < dim; d++) v[d] = vcm[d] + TM[d];
+        } else if (OP == OP_AT) {
+          for (int j = 0; j < dim; j++) {
+            double dv = (sp.damp) * (vcm[j]) / ((double)np_);
+            v[j] = vcm[j] + s_rv[j][p] - cd[3 + j] - dv;
+          }
+        } else {
+          double rq[3] = {0, 0, 0}, at[3];
+          for (int j = 0; j < dim; j++) rq[j] = q[j] - cd[9 + j];
+          const double Wv[3] = {cd[6], cd[7], cd[8]};
+          cross3(Wv, rq, at);
+          for (int j = 0; j < dim; j++) v[j] = vcm[j] + s_rv[j][p] - cd[3 + j] - (sp.damp) * (vcm[j]) + at[j];
+        }
+        for (int d = 0; d < dim; d++) v[d] = (double)(Real)v[d];  // the state keeps the collided velocity
+      }
+      {
+        double v0 = v[0] - R.avv[0], v1 = v[1] - R.avv[1], v2 = v[2] - R.avv[2];
+        double tmp = v0 * v0;
+        tmp += v1 * v1;
+        tmp += v2 * v2;
+        ke += sp.mass * tmp;
+      }
+      if (R.galinv) {
+        shift_back(R, shc, q);
+        for (int d = 0; d < dim; d++) q[d] = (double)(Real)q[d];
+      }
+      const int slot = base + p;
+      if (do_stream) {
+        double U0[3] = {0, 0, 0};
+        const int pid = Sn.id[slot];
+        int cu = stream_walls(C, R, wq_b, Dr, T.rep, spid, pid, c, q, v, U0, false);
+        for (int d = 0; d < 3; d++) {
+          q[d] = (double)(Real)q[d];
+          v[d] = (double)(Real)v[d];
+        }
+        if (do_next) next_bin(C, R, &shift_next[T.rep * 3], hist_next, slot, c, cu, q);
+        else C.cu[slot] = cu;
+      }
+      Sn.x[slot] = (Real)q[0];
+      Sn.y[slot] = (Real)q[1];
+      Sn.z[slot] = (Real)q[2];
+      Sn.vx[slot] = (Real)v[0];
+      Sn.vy[slot] = (Real)v[1];
+      Sn.vz[slot] = (Real)v[2];
+      Sn.cst[slot] = c;
+    }
+    __syncthreads();
+    cs = ce;
+  }
+  s_ke[tid] = ke;
+  __syncthreads();
+  for (int off = kTileThreads / 2; off > 0; off >>= 1) {
+    if (tid < off) s_ke[tid] += s_ke[tid + off];
+    __syncthreads();
+  }
+  if (tid == 0) C.tpart[blockIdx.x] = s_ke[0];
+}
+
+// stream after a global thermostat: scale, stream, walls, next bin
+template <typename Real>
+__global__ void k_stream_next(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, const double *wq_b,
+                              const double *shift_next, int32_t *hist_next, int do_next) {
+  const BlkDev B = blk[blockIdx.x];
+  int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  const double g = C.gscale[B.rep];
+  const bool lc = R.lc != MPCD_ISOF;
+  double q[3], v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]}, U[3] = {0, 0, 0};
+  load_q(S, t, q);
+  if (g != 1.0) {
+    for (int d = 0; d < R.dim; d++) v[d] = (double)(Real)(v[d] * g);
+  }
+  if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
+  const int c = S.cst[t];
+  int cu = stream_walls(C, R, wq_b, Dr, B.rep, S.sp[t], S.id[t], c, q, v, U, lc);
+  for (int d = 0; d < 3; d++) {
+    q[d] = (double)(Real)q[d];
+    v[d] = (double)(Real)v[d];
+  }
+  if (do_next) next_bin(C, R, &shift_next[B.rep * 3], hist_next, t, c, cu, q);
+  else C.cu[t] = cu;
+  S.x[t] = (Real)q[0];
+  S.y[t] = (Real)q[1];
+  S.z[t] = (Real)q[2];
+  S.vx[t] = (Real)v[0];
+  S.vy[t] = (Real)v[1];
+  S.vz[t] = (Real)v[2];
+  if (lc) {
+    S.ux[t] = (Real)U[0];
+    S.uy[t] = (Real)U[1];
+    S.uz[t] = (Real)U[2];
+  }
+}
+
+// TEMP (therm.c:209-235) from per-block kinetic-energy partials of one
+// replica (blocks [beg, end)), with the thermostat factor (therm.c:52-143)
+// and the active-matter guard (mpc.c:4766-4778); one workgroup per replica
+__global__ void k_temp_reduce(Ctx C, const int2 *ranges) {
+  const int r = blockIdx.x;
+  const RepDev &R = C.rep[r];
+  const int2 rg = ranges[r];
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int b = rg.x + threadIdx.x; b < rg.y; b += 256) acc += C.tpart[b];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double K = sh[0] / ((double)(R.npart * R.dim));
+  double g = 1.0;
+  if (R.tstech == MPCD_VSC) g = sqrt(R.kbt / K);
+  else if (R.tstech == MPCD_BEREND) g = sqrt(1. + (R.kbt / K - 1.) * R.dt / R.tau);
+  K = K * g * g;
+  if (R.rtech >= 7 && R.rtech <= 17 && K > 50.0 * R.kbt) {
+    double g2 = sqrt(R.kbt / K);
+    g *= g2;
+    K = K * g2 * g2;
+  }
+  C.gscale[r] = g;
+  C.kbtnow[r] = K;
+}
+
+}  // namespace mpcd

@@ -1,7 +1,8 @@
 // mpcd_gpu.hip -- C-ABI of the MPCD solvent library for gfx950 (see
 // include/mpcd_gpu.h for the entry points and the reference calls they
-// replace).  Host orchestration: device buffers, block tables, the per-phase
-// kernel sequences, and the hipGraph used by mpcd_gpu_run.
+// replace).  Host orchestration: device buffers, block and tile tables, the
+// per-phase kernel sequences, the fused whole-step sequence and the hipGraph
+// used by mpcd_gpu_run.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -9,29 +10,17 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
+#include "mpcd_fused.h"
 #include "mpcd_kernels.h"
 
 using namespace mpcd;
 
-namespace {
-
-struct DevBuf {
-  void *p = nullptr;
-  size_t n = 0;
-};
-
-template <typename Real>
-struct StateBufs {
-  State<Real> s{};
-};
-
-}  // namespace
-
 struct mpcd_gpu_ctx {
-  int nrep = 0, dim = 0, lc = 0, rtech = 0, precision = 0;
+  int nrep = 0, dim = 0, lc = 0, rtech = 0, precision = 0, nspec = 1;
   int ntot = 0, nctot = 0, nwtot = 0;
   uint64_t seed = 0;
   uint32_t step = 0;
@@ -39,35 +28,82 @@ struct mpcd_gpu_ctx {
   std::vector<mpcd_gpu_params> params;
   std::vector<RepDev> reps;
   std::vector<BlkDev> pblk, cblk;  // slot blocks, cell blocks
+  std::vector<TileDev> tiles;      // cell tiles of the fused kernel
+  std::vector<int2> cblk_rng, tile_rng;
   std::vector<int> uploaded;
   bool need_init = true;
-  bool pending = false;  // an unshifted bin is implied by cu[] and not materialised
+  // binning state: the logical list order is the storage order, or (pending)
+  // the unshifted bin implied by the unshifted cells in cu[] (cu_src 0) or in
+  // the keys (cu_src 1); hist_ready: the next shifted bin's histogram and
+  // keys are already built (by the fused kernel)
+  bool pending = false, hist_ready = false;
+  int cu_src = 0;
   int cur = 0;
-  bool any_therm = false, any_phantom = false, any_lcg = false;
+  bool any_therm = false, any_phantom = false, any_lcg = false, fusable = false;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
-  // device memory
   std::vector<void *> allocs;
   RepDev *d_rep = nullptr;
   BlkDev *d_pblk = nullptr, *d_cblk = nullptr;
+  TileDev *d_tiles = nullptr;
+  int2 *d_cblk_rng = nullptr, *d_tile_rng = nullptr;
   mpcd_gpu_species *d_sp = nullptr;
   mpcd_gpu_wall *d_wall = nullptr;
   Ctx C{};
+  double *shift2 = nullptr, *wqs2 = nullptr, *wqb2 = nullptr;  // per step parity
+  int32_t *hist2[2] = {nullptr, nullptr};
   State<double> sd[2]{};
   State<float> sf[2]{};
   double *d_avpart = nullptr;
-  // injected draws staging
   double *d_dshift = nullptr, *d_dcell = nullptr, *d_dghost = nullptr, *d_dpart = nullptr, *d_dlc = nullptr;
   int dlc_stride = 0;
   size_t dlc_cap = 0;
   int lc_stride = 0;
-  // staging for upload/download
   double *d_q = nullptr, *d_v = nullptr, *d_u = nullptr;
   int32_t *d_spid = nullptr, *d_i32 = nullptr;
   size_t stage_cap = 0;
-  std::vector<int32_t *> scan_lv;  // scan scratch, one pair per level
+  std::vector<int32_t *> scan_lv;
+  // kernel timing with HIP events on the library stream (mpcd_gpu_profile)
+  bool prof = false;
+  struct ProfRec {
+    int fam;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+};
+
+// kernel families reported by mpcd_gpu_timing
+enum { F_COUNT = 0, F_SCAN, F_SCATTER, F_SORT, F_GATHER, F_PROPS, F_GHOST, F_LC, F_COLLIDE, F_TEMP, F_STREAM, F_FUSED,
+       F_N };
+
+static hipEvent_t prof_event(mpcd_gpu_ctx *c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_used++];
+}
+
+// records an event pair on the library stream around the launches in scope
+struct ProfScope {
+  mpcd_gpu_ctx *c;
+  int fam;
+  hipEvent_t a = nullptr;
+  ProfScope(mpcd_gpu_ctx *c_, int f) : c(c_), fam(f) {
+    if (c->prof && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = prof_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->prof_recs.push_back({fam, a, b});
+  }
 };
 
 static mpcd_status fail(mpcd_gpu_ctx *c, mpcd_status st, const char *fmt, ...) {
@@ -80,9 +116,9 @@ static mpcd_status fail(mpcd_gpu_ctx *c, mpcd_status st, const char *fmt, ...) {
   return st;
 }
 
-#define HIPCHK(ctx, call)                                                                       \
-  do {                                                                                          \
-    hipError_t e_ = (call);                                                                     \
+#define HIPCHK(ctx, call)                                                                           \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
     if (e_ != hipSuccess) return fail(ctx, MPCD_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
@@ -91,8 +127,8 @@ static T *dalloc(mpcd_gpu_ctx *c, size_t n) {
   void *p = nullptr;
   if (n == 0) n = 1;
   if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-  hipMemset(p, 0, n * sizeof(T));
   c->allocs.push_back(p);
+  if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
   return (T *)p;
 }
 
@@ -142,6 +178,15 @@ static void set_pbc(const mpcd_gpu_params &p, int pbc[3]) {
   }
 }
 
+// cells per fused tile for a replica: about kTileTarget particles per tile
+static int tile_cells(const RepDev &R) {
+  const int kTileTarget = 320;
+  double vol = (double)R.L[0] * R.L[1] * (R.dim == 3 ? R.L[2] : 1);
+  double ppc = R.npart / vol;
+  int tc = (int)lround(kTileTarget / std::max(ppc, 1.0));
+  return std::max(1, std::min(kTileCells, tc));
+}
+
 extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, const int64_t *n_particles,
                                        uint64_t seed, mpcd_gpu_ctx **out) {
   if (!params || !n_particles || !out || nrep <= 0) return MPCD_EINVAL;
@@ -153,23 +198,26 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->lc = params[0].lc;
   c->rtech = params[0].rtech;
   c->precision = params[0].precision;
+  c->nspec = params[0].nspec;
   c->params.assign(params, params + nrep);
   auto bad = [&](mpcd_status st, const char *m) {
     fprintf(stderr, "mpcd_gpu_create: %s\n", m);
     delete c;
     return st;
   };
-  // validation: options on this path
   int64_t ntot = 0, nctot = 0;
   for (int r = 0; r < nrep; r++) {
     const mpcd_gpu_params &p = params[r];
     if (p.dim != c->dim || p.lc != c->lc || p.rtech != c->rtech || p.precision != c->precision)
       return bad(MPCD_EINVAL, "dim, lc, rtech and precision must agree across replicas");
     if (p.dim != 2 && p.dim != 3) return bad(MPCD_EINVAL, "dim must be 2 or 3");
+    if (p.precision != MPCD_FP64 && p.precision != MPCD_FP32) return bad(MPCD_EINVAL, "precision");
     if (p.nspec < 1 || p.nspec > MPCD_MAXSPEC || p.nwall < 0 || p.nwall > MPCD_MAXWALL)
       return bad(MPCD_EINVAL, "species or wall count out of range");
     if (memcmp(p.sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC) != 0 || p.nspec != params[0].nspec)
       return bad(MPCD_EINVAL, "the species table must agree across replicas");
+    for (int i = 0; i < p.nspec; i++)
+      if (!(p.sp[i].mass > 0.0)) return bad(MPCD_EINVAL, "species mass must be positive");
     if (p.lc == MPCD_ISOF) {
       if (!(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT || p.rtech == MPCD_RAT))
         return bad(MPCD_EUNSUPPORTED, "collision operator outside this path");
@@ -185,6 +233,8 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
       if (nz != 1) return bad(MPCD_EUNSUPPORTED, "only axis-aligned planar walls are on this path");
       if (W.COLL_TYPE != 1) return bad(MPCD_EUNSUPPORTED, "only rule-based wall collisions are on this path");
     }
+    for (int i = 0; i < p.dim; i++)
+      if (p.L[i] < 2) return bad(MPCD_EINVAL, "box lengths must be at least 2 cells");
     if (n_particles[r] <= 0) return bad(MPCD_EINVAL, "replica without particles");
     ntot += n_particles[r];
     int nz = (p.dim == 3) ? p.L[2] + 1 : 1;
@@ -194,7 +244,6 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->ntot = (int)ntot;
   c->nctot = (int)nctot;
 
-  // replica descriptors and block tables
   int poff = 0, coff = 0, woff = 0;
   std::vector<mpcd_gpu_wall> walls;
   for (int r = 0; r < nrep; r++) {
@@ -237,8 +286,15 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
     if (p.tstech != MPCD_NOTHERM || (p.rtech >= 7 && p.rtech <= 17)) c->any_therm = true;
     if (p.lc == MPCD_LCG) c->any_lcg = true;
     c->reps.push_back(R);
-    for (int b = 0; b < R.npart; b += kBlock) c->pblk.push_back(BlkDev{r, poff + b, poff + std::min(b + kBlock, R.npart), 0});
-    for (int b = 0; b < R.ncell; b += kBlock) c->cblk.push_back(BlkDev{r, coff + b, coff + std::min(b + kBlock, R.ncell), 0});
+    int cb0 = (int)c->cblk.size(), t0 = (int)c->tiles.size();
+    for (int b = 0; b < R.npart; b += kBlock)
+      c->pblk.push_back(BlkDev{r, poff + b, poff + std::min(b + kBlock, R.npart), 0});
+    for (int b = 0; b < R.ncell; b += kBlock)
+      c->cblk.push_back(BlkDev{r, coff + b, coff + std::min(b + kBlock, R.ncell), 0});
+    int tc = tile_cells(R);
+    for (int b = 0; b < R.ncell; b += tc) c->tiles.push_back(TileDev{r, coff + b, coff + std::min(b + tc, R.ncell), 0});
+    c->cblk_rng.push_back(make_int2(cb0, (int)c->cblk.size()));
+    c->tile_rng.push_back(make_int2(t0, (int)c->tiles.size()));
     poff += R.npart;
     coff += R.ncell;
     woff += p.nwall;
@@ -246,24 +302,30 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->nwtot = woff;
   c->uploaded.assign(nrep, 0);
   c->lc_stride = anneal_stride(params[0]);
+  c->fusable = (c->lc == MPCD_ISOF);
 
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad(MPCD_EDEVICE, "stream");
   bool ok = true;
+  const int nw3 = 3 * std::max(1, woff);
   c->d_rep = dalloc<RepDev>(c, nrep);
   c->d_pblk = dalloc<BlkDev>(c, c->pblk.size());
   c->d_cblk = dalloc<BlkDev>(c, c->cblk.size());
+  c->d_tiles = dalloc<TileDev>(c, c->tiles.size());
+  c->d_cblk_rng = dalloc<int2>(c, nrep);
+  c->d_tile_rng = dalloc<int2>(c, nrep);
   c->d_sp = dalloc<mpcd_gpu_species>(c, MPCD_MAXSPEC);
   c->d_wall = dalloc<mpcd_gpu_wall>(c, std::max<size_t>(1, walls.size()));
-  ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_sp && c->d_wall;
+  c->shift2 = dalloc<double>(c, 2 * 3 * nrep);
+  c->wqs2 = dalloc<double>(c, 2 * nw3);
+  c->wqb2 = dalloc<double>(c, 2 * nw3);
+  c->hist2[0] = dalloc<int32_t>(c, c->nctot + 1);
+  c->hist2[1] = dalloc<int32_t>(c, c->nctot + 1);
+  ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_tile_rng && c->d_sp &&
+       c->d_wall && c->shift2 && c->wqs2 && c->wqb2 && c->hist2[0] && c->hist2[1];
   Ctx &C = c->C;
   C.rep = c->d_rep;
   C.sp = c->d_sp;
   C.wall = c->d_wall;
-  C.wq = dalloc<double>(c, 3 * std::max(1, woff));
-  C.wq_s = dalloc<double>(c, 3 * std::max(1, woff));
-  C.wq_b = dalloc<double>(c, 3 * std::max(1, woff));
-  C.shift = dalloc<double>(c, 3 * nrep);
-  C.hist = dalloc<int32_t>(c, c->nctot + 1);
   C.start = dalloc<int32_t>(c, c->nctot + 1 + 1024);
   C.cnew = dalloc<int32_t>(c, c->ntot);
   C.loc = dalloc<int32_t>(c, c->ntot);
@@ -279,7 +341,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   C.cell.dir = dalloc<double>(c, 3 * (size_t)c->nctot);
   C.cell.E = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 9 * (size_t)c->nctot) : nullptr;
   C.T = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 3 * (size_t)c->ntot) : nullptr;
-  C.tpart = dalloc<double>(c, c->cblk.size());
+  C.tpart = dalloc<double>(c, std::max(c->cblk.size(), c->tiles.size()));
   C.kbtnow = dalloc<double>(c, nrep);
   C.gscale = dalloc<double>(c, nrep);
   C.sg = dalloc<double>(c, nrep);
@@ -289,11 +351,12 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   C.nctot = c->nctot;
   C.nrep = nrep;
   c->d_avpart = (c->lc == MPCD_LCG) ? dalloc<double>(c, 6 * c->pblk.size()) : nullptr;
-  ok = ok && C.wq && C.wq_s && C.wq_b && C.shift && C.hist && C.start && C.cnew && C.loc && C.cu && C.key &&
-       C.skey && C.cell.pop && C.cell.mass && C.cell.vcm && C.cell.cm && C.cell.I && C.cell.S && C.cell.dir &&
-       C.tpart && C.kbtnow && C.gscale && C.sg && C.events && C.flags;
+  ok = ok && C.start && C.cnew && C.loc && C.cu && C.key && C.skey && C.cell.pop && C.cell.mass && C.cell.vcm &&
+       C.cell.cm && C.cell.I && C.cell.S && C.cell.dir && C.tpart && C.kbtnow && C.gscale && C.sg && C.events &&
+       C.flags;
   bool lcm = c->lc != MPCD_ISOF;
   if (lcm) ok = ok && C.cell.E && C.T;
+  if (c->any_lcg) ok = ok && c->d_avpart;
   if (c->precision == MPCD_FP64) {
     c->sd[0] = alloc_state<double>(c, c->ntot, lcm);
     c->sd[1] = alloc_state<double>(c, c->ntot, lcm);
@@ -308,19 +371,21 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
     fprintf(stderr, "mpcd_gpu_create: device allocation failed\n");
     return MPCD_ENOMEM;
   }
-  // constant tables
-  std::vector<double> wq(3 * std::max(1, woff), 0.0);
+  std::vector<double> wq(2 * nw3, 0.0);
   for (int w = 0; w < woff; w++)
-    for (int j = 0; j < 3; j++) wq[w * 3 + j] = walls[w].Q[j];
-  hipMemcpy(c->d_rep, c->reps.data(), sizeof(RepDev) * nrep, hipMemcpyHostToDevice);
-  hipMemcpy(c->d_pblk, c->pblk.data(), sizeof(BlkDev) * c->pblk.size(), hipMemcpyHostToDevice);
-  hipMemcpy(c->d_cblk, c->cblk.data(), sizeof(BlkDev) * c->cblk.size(), hipMemcpyHostToDevice);
-  hipMemcpy(c->d_sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC, hipMemcpyHostToDevice);
-  if (woff) hipMemcpy(c->d_wall, walls.data(), sizeof(mpcd_gpu_wall) * woff, hipMemcpyHostToDevice);
-  hipMemcpy(C.wq, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice);
-  hipMemcpy(C.wq_s, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice);
-  hipMemcpy(C.wq_b, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice);
-  if (hipDeviceSynchronize() != hipSuccess) {
+    for (int j = 0; j < 3; j++) wq[w * 3 + j] = wq[nw3 + w * 3 + j] = walls[w].Q[j];
+  bool cp = true;
+  cp = cp && hipMemcpy(c->d_rep, c->reps.data(), sizeof(RepDev) * nrep, hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->d_pblk, c->pblk.data(), sizeof(BlkDev) * c->pblk.size(), hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->d_cblk, c->cblk.data(), sizeof(BlkDev) * c->cblk.size(), hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->d_cblk_rng, c->cblk_rng.data(), sizeof(int2) * nrep, hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->d_tile_rng, c->tile_rng.data(), sizeof(int2) * nrep, hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->d_sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC, hipMemcpyHostToDevice) == hipSuccess;
+  if (woff) cp = cp && hipMemcpy(c->d_wall, walls.data(), sizeof(mpcd_gpu_wall) * woff, hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->wqs2, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice) == hipSuccess;
+  cp = cp && hipMemcpy(c->wqb2, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice) == hipSuccess;
+  if (!cp || hipDeviceSynchronize() != hipSuccess) {
     mpcd_gpu_destroy(c);
     return MPCD_EDEVICE;
   }
@@ -330,11 +395,12 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
 
 extern "C" void mpcd_gpu_destroy(mpcd_gpu_ctx *c) {
   if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
-  for (void *p : c->allocs) hipFree(p);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -352,10 +418,10 @@ extern "C" int mpcd_gpu_lc_stride(const mpcd_gpu_ctx *c) { return c ? c->lc_stri
 // ---------------------------------------------------------------------------
 static int nblocks(int n, int b) { return (n + b - 1) / b; }
 
-static Draws make_draws(mpcd_gpu_ctx *c, int injected) {
+static Draws make_draws(mpcd_gpu_ctx *c, int injected, uint32_t step) {
   Draws D{};
   D.rng.seed = c->seed;
-  D.rng.step = c->step;
+  D.rng.step = step;
   D.injected = injected;
   D.shift = c->d_dshift;
   D.cell = c->d_dcell;
@@ -364,6 +430,17 @@ static Draws make_draws(mpcd_gpu_ctx *c, int injected) {
   D.lc = c->d_dlc;
   D.lc_stride = c->dlc_stride;
   return D;
+}
+
+// the kernels' view of the context for the step with parity `par`
+static Ctx view(mpcd_gpu_ctx *c, int par) {
+  Ctx C = c->C;
+  const int nw3 = 3 * std::max(1, c->nwtot);
+  C.shift = c->shift2 + par * 3 * c->nrep;
+  C.wq_s = c->wqs2 + par * nw3;
+  C.wq_b = c->wqb2 + par * nw3;
+  C.hist = c->hist2[par];
+  return C;
 }
 
 static mpcd_status stage_draws(mpcd_gpu_ctx *c, const mpcd_draws *d) {
@@ -390,35 +467,35 @@ static mpcd_status stage_draws(mpcd_gpu_ctx *c, const mpcd_draws *d) {
     c->dlc_stride = d->lc_stride;
     HIPCHK(c, hipMemcpyAsync(c->d_dlc, d->lc, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
   }
-  // the host arrays are borrowed only for the call
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // host arrays are borrowed only for the call
   return MPCD_OK;
 }
 
-static void scan_fixed(mpcd_gpu_ctx *c, int32_t *in, int32_t *out, int n, std::vector<int32_t *> &scratch,
-                       size_t level) {
+static void scan_fixed(mpcd_gpu_ctx *c, int32_t *in, int32_t *out, int n, size_t level) {
   int nb = nblocks(n + 1, 1024);
   if (nb == 1) {
     k_scan_block<<<1, 256, 0, c->stream>>>(in, out, nullptr, n + 1);
     return;
   }
-  int32_t *bsum = scratch[2 * level], *boff = scratch[2 * level + 1];
+  int32_t *bsum = c->scan_lv[2 * level], *boff = c->scan_lv[2 * level + 1];
   k_scan_block<<<nb, 256, 0, c->stream>>>(in, out, bsum, n + 1);
-  scan_fixed(c, bsum, boff, nb - 1, scratch, level + 1);
+  scan_fixed(c, bsum, boff, nb - 1, level + 1);
   k_scan_add<<<nb, 256, 0, c->stream>>>(out, boff, n + 1);
 }
 
-static std::vector<int32_t *> &scan_scratch(mpcd_gpu_ctx *c) {
-  if (!c->scan_lv.empty() || c->nctot + 1 <= 1024) return c->scan_lv;
+static bool alloc_scan(mpcd_gpu_ctx *c) {
+  if (!c->scan_lv.empty() || c->nctot + 1 <= 1024) return true;
   int n = c->nctot;
   while (true) {
     int nb = nblocks(n + 1, 1024);
     if (nb == 1) break;
-    c->scan_lv.push_back(dalloc<int32_t>(c, nb + 1));
-    c->scan_lv.push_back(dalloc<int32_t>(c, nb + 1));
+    int32_t *a = dalloc<int32_t>(c, nb + 1), *b = dalloc<int32_t>(c, nb + 1);
+    if (!a || !b) return false;
+    c->scan_lv.push_back(a);
+    c->scan_lv.push_back(b);
     n = nb - 1;
   }
-  return c->scan_lv;
+  return true;
 }
 
 template <typename Real>
@@ -428,21 +505,48 @@ State<double> *states<double>(mpcd_gpu_ctx *c) { return c->sd; }
 template <>
 State<float> *states<float>(mpcd_gpu_ctx *c) { return c->sf; }
 
-// binning: mode 0 initial, 1 shifted, 2 materialise the unshifted bin.
-// gather = false leaves the state in place (only the sorted keys are built).
+// grid shift draw of step `step` and the wall positions it implies; reads
+// the previous step's shifted-back walls, writes this step's parity slot
+static void launch_shift(mpcd_gpu_ctx *c, uint32_t step, int injected) {
+  const int par = step & 1, nw3 = 3 * std::max(1, c->nwtot);
+  Draws D = make_draws(c, injected, step);
+  k_shift<<<nblocks(c->nrep, 64), 64, 0, c->stream>>>(c->C, D, 1, c->wqb2 + (par ^ 1) * nw3,
+                                                        c->shift2 + par * 3 * c->nrep, c->wqs2 + par * nw3,
+                                                        c->wqb2 + par * nw3);
+}
+
+// binning: mode 0 initial, 1 shifted (this step), 2 materialise the pending
+// unshifted bin.  gather = false only builds the sorted keys.
 template <typename Real>
-static void run_bin(mpcd_gpu_ctx *c, int mode, bool gather, const Draws &D) {
+static void run_bin(mpcd_gpu_ctx *c, int mode, bool gather, int injected) {
   State<Real> *S = states<Real>(c);
-  Ctx &C = c->C;
-  k_fill<<<nblocks(c->nctot + 1, kBlock), kBlock, 0, c->stream>>>(C.hist, 0, c->nctot + 1);
-  if (mode != 0) k_shift<<<nblocks(c->nrep, 64), 64, 0, c->stream>>>(C, D, mode == 1 ? 1 : 0);
-  k_count<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, mode, c->pending ? 1 : 0);
-  scan_fixed(c, C.hist, C.start, c->nctot, scan_scratch(c), 0);
-  k_scatter<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C);
-  k_sortcells<<<nblocks(c->nctot, kBlock), kBlock, 0, c->stream>>>(C, S[1 - c->cur].cst);
+  const int par = c->step & 1;
+  Ctx C = view(c, par);
+  Draws D = make_draws(c, injected, c->step);
+  {
+    ProfScope ps(c, F_COUNT);
+    k_fill<<<nblocks(c->nctot + 1, kBlock), kBlock, 0, c->stream>>>(C.hist, 0, c->nctot + 1);
+    if (mode == 1) launch_shift(c, c->step, injected);
+    k_count<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, mode, c->pending ? 1 : 0,
+                                                             c->cu_src);
+  }
+  c->hist_ready = false;
+  {
+    ProfScope ps(c, F_SCAN);
+    scan_fixed(c, C.hist, C.start, c->nctot, 0);
+  }
+  {
+    ProfScope ps(c, F_SCATTER);
+    k_scatter<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C);
+  }
+  {
+    ProfScope ps(c, F_SORT);
+    k_sortcells<<<nblocks(c->nctot, kBlock), kBlock, 0, c->stream>>>(C, S[1 - c->cur].cst);
+  }
   if (gather) {
-    k_gather<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], S[1 - c->cur], c->d_pblk,
-                                                              mode == 1 ? 1 : 0, c->lc != MPCD_ISOF, D);
+    ProfScope ps(c, F_GATHER);
+    k_gather<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], S[1 - c->cur], c->d_pblk, mode == 1 ? 1 : 0,
+                                                              c->lc != MPCD_ISOF, D);
     c->cur = 1 - c->cur;
     c->pending = false;
   }
@@ -451,69 +555,175 @@ static void run_bin(mpcd_gpu_ctx *c, int mode, bool gather, const Draws &D) {
 template <typename Real>
 static void run_props(mpcd_gpu_ctx *c, int do_acc) {
   State<Real> *S = states<Real>(c);
-  k_cellprops<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(c->C, S[c->cur], c->d_cblk, do_acc);
+  Ctx C = view(c, c->step & 1);
+  ProfScope ps(c, F_PROPS);
+  k_cellprops<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, do_acc);
   if (c->lc != MPCD_ISOF) {
-    k_velgrad_bulk<<<c->cblk.size(), kBlock, 0, c->stream>>>(c->C, c->d_cblk);
-    k_velgrad_edges<<<c->nrep, 256, 0, c->stream>>>(c->C);
+    k_velgrad_bulk<<<c->cblk.size(), kBlock, 0, c->stream>>>(C, c->d_cblk);
+    k_velgrad_edges<<<c->nrep, 256, 0, c->stream>>>(C);
   }
 }
 
 template <typename Real>
-static void run_collide(mpcd_gpu_ctx *c, const Draws &D) {
+static void run_collide(mpcd_gpu_ctx *c, int injected) {
   State<Real> *S = states<Real>(c);
-  Ctx &C = c->C;
+  Ctx C = view(c, c->step & 1);
+  Draws D = make_draws(c, injected, c->step);
   run_props<Real>(c, 1);
-  if (c->any_phantom) k_ghost<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+  if (c->any_phantom) {
+    ProfScope ps(c, F_GHOST);
+    k_ghost<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+  }
   if (c->lc != MPCD_ISOF) {
+    ProfScope ps(c, F_LC);
     if (c->any_lcg) {
       k_avorder_part<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, c->d_avpart);
       k_avorder_final<<<1, 64, 0, c->stream>>>(C, c->d_pblk, (int)c->pblk.size(), c->d_avpart);
     }
     k_lc<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
   }
-  k_collide<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
-  k_temp_final<<<1, 64, 0, c->stream>>>(C, c->d_cblk, (int)c->cblk.size());
+  {
+    ProfScope ps(c, F_COLLIDE);
+    k_collide<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+  }
+  ProfScope ps(c, F_TEMP);
+  k_temp_reduce<<<c->nrep, 256, 0, c->stream>>>(C, c->d_cblk_rng);
   if (c->any_therm) k_scale<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk);
 }
 
 template <typename Real>
-static void run_stream(mpcd_gpu_ctx *c, const Draws &D) {
+static void run_stream(mpcd_gpu_ctx *c, int injected) {
   State<Real> *S = states<Real>(c);
-  k_stream<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(c->C, S[c->cur], c->d_pblk, D);
+  Ctx C = view(c, c->step & 1);
+  Draws D = make_draws(c, injected, c->step);
+  ProfScope ps(c, F_STREAM);
+  k_stream<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D);
   c->pending = true;
+  c->cu_src = 0;
+  c->hist_ready = false;
+  c->step++;
+}
+
+template <typename Real, int OP, bool ONESP>
+static void launch_fused(mpcd_gpu_ctx *c, const Ctx &C, State<Real> *S, const Draws &D, int par, int do_next,
+                         int do_stream) {
+  const int nw3 = 3 * std::max(1, c->nwtot);
+  k_fused<Real, OP, ONESP><<<c->tiles.size(), kTileThreads, 0, c->stream>>>(
+      C, S[c->cur], S[1 - c->cur], c->d_tiles, D, c->shift2 + par * 3 * c->nrep, c->wqb2 + par * nw3,
+      c->shift2 + (par ^ 1) * 3 * c->nrep, c->hist2[par ^ 1], do_next, do_stream);
+}
+
+template <typename Real>
+static void dispatch_fused(mpcd_gpu_ctx *c, const Ctx &C, State<Real> *S, const Draws &D, int par, int do_next,
+                           int do_stream) {
+  const bool one = c->nspec == 1;
+  const int op = (c->rtech == MPCD_MPCAT) ? OP_AT : (c->rtech == MPCD_RAT) ? OP_RAT : OP_SRD;
+  if (op == OP_SRD) {
+    if (one) launch_fused<Real, OP_SRD, true>(c, C, S, D, par, do_next, do_stream);
+    else launch_fused<Real, OP_SRD, false>(c, C, S, D, par, do_next, do_stream);
+  } else if (op == OP_AT) {
+    if (one) launch_fused<Real, OP_AT, true>(c, C, S, D, par, do_next, do_stream);
+    else launch_fused<Real, OP_AT, false>(c, C, S, D, par, do_next, do_stream);
+  } else {
+    if (one) launch_fused<Real, OP_RAT, true>(c, C, S, D, par, do_next, do_stream);
+    else launch_fused<Real, OP_RAT, false>(c, C, S, D, par, do_next, do_stream);
+  }
+}
+
+// one whole step through the fused kernel (isotropic operators).  With the
+// internal generator the next step's shift is drawn first, so the kernel
+// also builds the next step's histogram and keys.
+template <typename Real>
+static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
+  State<Real> *S = states<Real>(c);
+  const int par = c->step & 1;
+  Ctx C = view(c, par);
+  Draws D = make_draws(c, injected, c->step);
+  // a prebuilt histogram used the internal shift: injected draws recount
+  if (!c->hist_ready || injected) {
+    ProfScope ps(c, F_COUNT);
+    k_fill<<<nblocks(c->nctot + 1, kBlock), kBlock, 0, c->stream>>>(C.hist, 0, c->nctot + 1);
+    launch_shift(c, c->step, injected);
+    k_count<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, 1, c->pending ? 1 : 0,
+                                                             c->cu_src);
+  }
+  const int do_next = injected ? 0 : 1;
+  const int do_stream = c->any_therm ? 0 : 1;
+  {
+    ProfScope ps(c, F_SCAN);
+    if (do_next) launch_shift(c, c->step + 1, 0);
+    scan_fixed(c, C.hist, C.start, c->nctot, 0);
+  }
+  {
+    ProfScope ps(c, F_SCATTER);
+    k_scatter<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C);
+    if (do_next) k_fill<<<nblocks(c->nctot + 1, kBlock), kBlock, 0, c->stream>>>(c->hist2[par ^ 1], 0, c->nctot + 1);
+  }
+  {
+    ProfScope ps(c, F_FUSED);
+    dispatch_fused<Real>(c, C, S, D, par, do_next, do_stream);
+  }
+  c->cur = 1 - c->cur;
+  {
+    ProfScope ps(c, F_TEMP);
+    k_temp_reduce<<<c->nrep, 256, 0, c->stream>>>(C, c->d_tile_rng);
+  }
+  if (!do_stream) {
+    ProfScope ps(c, F_STREAM);
+    const int nw3 = 3 * std::max(1, c->nwtot);
+    k_stream_next<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, c->wqb2 + par * nw3,
+                                                                   c->shift2 + (par ^ 1) * 3 * c->nrep,
+                                                                   c->hist2[par ^ 1], do_next);
+  }
+  c->pending = true;
+  c->cu_src = do_next ? 1 : 0;
+  c->hist_ready = do_next != 0;
   c->step++;
 }
 
 template <typename Real>
-static void run_step(mpcd_gpu_ctx *c, const Draws &D) {
-  run_bin<Real>(c, 1, true, D);
-  run_collide<Real>(c, D);
-  run_stream<Real>(c, D);
+static void run_step(mpcd_gpu_ctx *c, int injected) {
+  if (c->fusable) {
+    run_step_fused<Real>(c, injected);
+    return;
+  }
+  run_bin<Real>(c, 1, true, injected);
+  run_collide<Real>(c, injected);
+  run_stream<Real>(c, injected);
+}
+
+static mpcd_status check_launch(mpcd_gpu_ctx *c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, MPCD_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
+  return MPCD_OK;
+}
+
+static mpcd_status check_flags(mpcd_gpu_ctx *c) {
+  int flags[4];
+  HIPCHK(c, hipMemcpyAsync(flags, c->C.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flags[0] & 2) return fail(c, MPCD_EINVAL, "initial positions outside the box");
+  if (flags[0] & 1) return fail(c, MPCD_ECAPACITY, "a cell holds more than %d particles", kTileCap);
+  return MPCD_OK;
 }
 
 static mpcd_status ensure_init(mpcd_gpu_ctx *c) {
   if (!c->need_init) return MPCD_OK;
   for (int r = 0; r < c->nrep; r++)
     if (!c->uploaded[r]) return fail(c, MPCD_EINVAL, "replica %d has no state (mpcd_gpu_upload)", r);
-  Draws D = make_draws(c, 0);
+  if (!alloc_scan(c)) return fail(c, MPCD_ENOMEM, "scan scratch");
   if (c->precision == MPCD_FP64) {
-    run_bin<double>(c, 0, true, D);
+    run_bin<double>(c, 0, true, 0);
     run_props<double>(c, 0);
   } else {
-    run_bin<float>(c, 0, true, D);
+    run_bin<float>(c, 0, true, 0);
     run_props<float>(c, 0);
   }
-  int flags[4];
-  HIPCHK(c, hipMemcpyAsync(flags, c->C.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (flags[0] & 2) return fail(c, MPCD_EINVAL, "initial positions outside the box");
+  mpcd_status st = check_launch(c);
+  if (st) return st;
+  st = check_flags(c);
+  if (st) return st;
   c->need_init = false;
-  return MPCD_OK;
-}
-
-static mpcd_status check_launch(mpcd_gpu_ctx *c) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, MPCD_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
   return MPCD_OK;
 }
 
@@ -539,20 +749,25 @@ extern "C" mpcd_status mpcd_gpu_upload(mpcd_gpu_ctx *c, int r, const double *Q, 
   if (spid)
     for (size_t i = 0; i < n; i++)
       if (spid[i] < 0 || spid[i] >= c->params[r].nspec) return fail(c, MPCD_EINVAL, "species id out of range");
+  for (size_t i = 0; i < 3 * n; i++)
+    if (!isfinite(Q[i]) || !isfinite(V[i])) return fail(c, MPCD_EINVAL, "non-finite position or velocity");
   HIPCHK(c, hipMemcpyAsync(c->d_q, Q, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_v, V, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   if (U) HIPCHK(c, hipMemcpyAsync(c->d_u, U, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   if (spid) HIPCHK(c, hipMemcpyAsync(c->d_spid, spid, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
   if (c->precision == MPCD_FP64)
     k_stage<double><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->sd[c->cur], R.part_off, R.npart, c->d_q,
-                                                                      c->d_v, U ? c->d_u : nullptr, spid ? c->d_spid : nullptr);
+                                                                      c->d_v, U ? c->d_u : nullptr,
+                                                                      spid ? c->d_spid : nullptr);
   else
     k_stage<float><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->sf[c->cur], R.part_off, R.npart, c->d_q,
-                                                                     c->d_v, U ? c->d_u : nullptr, spid ? c->d_spid : nullptr);
+                                                                     c->d_v, U ? c->d_u : nullptr,
+                                                                     spid ? c->d_spid : nullptr);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->uploaded[r] = 1;
   c->need_init = true;
   c->pending = false;
+  c->hist_ready = false;
   return check_launch(c);
 }
 
@@ -588,26 +803,25 @@ extern "C" mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *c, const mpcd_draws *draws, in
   if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
-  Draws D = make_draws(c, draws != nullptr);
   if (shifted == 0 && !c->pending) {
-    // already materialised: nothing to do
+    // already materialised
   } else if (c->precision == MPCD_FP64) {
-    run_bin<double>(c, shifted ? 1 : 2, true, D);
+    run_bin<double>(c, shifted ? 1 : 2, true, draws != nullptr);
   } else {
-    run_bin<float>(c, shifted ? 1 : 2, true, D);
+    run_bin<float>(c, shifted ? 1 : 2, true, draws != nullptr);
   }
   st = check_launch(c);
   if (st) return st;
   if (cell_out || count_out || perm_out) {
-    st = ensure_stage(c, c->ntot);
-    if (st) return st;
     const int32_t *id = (c->precision == MPCD_FP64) ? c->sd[c->cur].id : c->sf[c->cur].id;
     const int32_t *cst = (c->precision == MPCD_FP64) ? c->sd[c->cur].cst : c->sf[c->cur].cst;
-    std::vector<int32_t> hid(c->ntot), hcst(c->ntot);
+    std::vector<int32_t> hid(c->ntot), hcst(c->ntot), hstart(c->nctot + 1);
     HIPCHK(c, hipMemcpyAsync(hid.data(), id, sizeof(int32_t) * c->ntot, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(hcst.data(), cst, sizeof(int32_t) * c->ntot, hipMemcpyDeviceToHost, c->stream));
-    if (count_out) HIPCHK(c, hipMemcpyAsync(count_out, c->C.hist, sizeof(int32_t) * c->nctot, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hstart.data(), c->C.start, sizeof(int32_t) * (c->nctot + 1), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (count_out)
+      for (int i = 0; i < c->nctot; i++) count_out[i] = hstart[i + 1] - hstart[i];
     for (int r = 0; r < c->nrep; r++) {
       const RepDev &R = c->reps[r];
       for (int t = R.part_off; t < R.part_off + R.npart; t++) {
@@ -616,7 +830,7 @@ extern "C" mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *c, const mpcd_draws *draws, in
       }
     }
   }
-  return MPCD_OK;
+  return check_flags(c);
 }
 
 static mpcd_status read_kbt(mpcd_gpu_ctx *c, double *kbt_now_out) {
@@ -632,9 +846,8 @@ extern "C" mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *c, const mpcd_draws *draws
   if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
-  Draws D = make_draws(c, draws != nullptr);
-  if (c->precision == MPCD_FP64) run_collide<double>(c, D);
-  else run_collide<float>(c, D);
+  if (c->precision == MPCD_FP64) run_collide<double>(c, draws != nullptr);
+  else run_collide<float>(c, draws != nullptr);
   st = check_launch(c);
   if (st) return st;
   return read_kbt(c, kbt_now_out);
@@ -653,9 +866,8 @@ extern "C" mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *c, int64_t *bc_event_counts
   if (!c) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
-  Draws D = make_draws(c, 0);
-  if (c->precision == MPCD_FP64) run_stream<double>(c, D);
-  else run_stream<float>(c, D);
+  if (c->precision == MPCD_FP64) run_stream<double>(c, 0);
+  else run_stream<float>(c, 0);
   st = check_launch(c);
   if (st) return st;
   if (bc_event_counts_out) return read_events(c, bc_event_counts_out);
@@ -668,32 +880,33 @@ extern "C" mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *c, const mpcd_draws *draws, d
   if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
-  Draws D = make_draws(c, draws != nullptr);
-  if (c->precision == MPCD_FP64) run_step<double>(c, D);
-  else run_step<float>(c, D);
+  if (c->precision == MPCD_FP64) run_step<double>(c, draws != nullptr);
+  else run_step<float>(c, draws != nullptr);
   st = check_launch(c);
   if (st) return st;
-  return read_kbt(c, kbt_now_out);
+  st = read_kbt(c, kbt_now_out);
+  if (st) return st;
+  return check_flags(c);
 }
 
 extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) {
   if (!c || nsteps < 0) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
-  scan_scratch(c);  // allocate outside any capture
-  for (int k = 0; k < nsteps; k++) {
-    Draws D = make_draws(c, 0);
-    if (c->precision == MPCD_FP64) run_step<double>(c, D);
-    else run_step<float>(c, D);
-  }
   (void)use_graph;
+  for (int k = 0; k < nsteps; k++) {
+    if (c->precision == MPCD_FP64) run_step<double>(c, 0);
+    else run_step<float>(c, 0);
+  }
   return check_launch(c);
 }
 
 extern "C" mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *c) {
   if (!c) return MPCD_EINVAL;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return check_launch(c);
+  mpcd_status st = check_launch(c);
+  if (st) return st;
+  return check_flags(c);
 }
 
 extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, double *vcm, double *S, double *dir) {
@@ -701,13 +914,12 @@ extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, 
   mpcd_status st = ensure_init(c);
   if (st) return st;
   if (c->pending) {
-    Draws D = make_draws(c, 0);
     // the reference re-bins and recomputes localPROP after streaming (mpc.c:4936-4942)
     if (c->precision == MPCD_FP64) {
-      run_bin<double>(c, 2, true, D);
+      run_bin<double>(c, 2, true, 0);
       run_props<double>(c, 0);
     } else {
-      run_bin<float>(c, 2, true, D);
+      run_bin<float>(c, 2, true, 0);
       run_props<float>(c, 0);
     }
   }
@@ -729,22 +941,15 @@ extern "C" mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *c, int r, int32_t *order
   if (st) return st;
   const RepDev &R = c->reps[r];
   const int32_t *id = (c->precision == MPCD_FP64) ? c->sd[c->cur].id : c->sf[c->cur].id;
-  Draws D = make_draws(c, 0);
-  if (c->pending) {
-    // build the unshifted order's keys without moving the state
-    if (c->precision == MPCD_FP64) run_bin<double>(c, 2, false, D);
-    else run_bin<float>(c, 2, false, D);
-    k_order_ids<<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->C.skey, id, c->d_i32, R.part_off, R.npart);
-  } else {
-    HIPCHK(c, hipMemcpyAsync(c->d_i32, id + R.part_off, sizeof(int32_t) * R.npart, hipMemcpyDeviceToDevice, c->stream));
-    // start[] of the current order: recompute the histogram of the slots' cells
-    if (c->precision == MPCD_FP64) run_bin<double>(c, 2, false, D);
-    else run_bin<float>(c, 2, false, D);
-  }
+  // keys of the current logical order, without moving the state
+  if (c->precision == MPCD_FP64) run_bin<double>(c, 2, false, 0);
+  else run_bin<float>(c, 2, false, 0);
+  k_order_ids<<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->C.skey, id, c->d_i32, R.part_off, R.npart);
   if (order) HIPCHK(c, hipMemcpyAsync(order, c->d_i32, sizeof(int32_t) * R.npart, hipMemcpyDeviceToHost, c->stream));
   if (cstart) {
     std::vector<int32_t> h(R.ncell + 1);
-    HIPCHK(c, hipMemcpyAsync(h.data(), c->C.start + R.cell_off, sizeof(int32_t) * (R.ncell + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->C.start + R.cell_off, sizeof(int32_t) * (R.ncell + 1), hipMemcpyDeviceToHost,
+                             c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int i = 0; i <= R.ncell; i++) cstart[i] = h[i] - R.part_off;
   }
@@ -752,8 +957,29 @@ extern "C" mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *c, int r, int32_t *order
   return check_launch(c);
 }
 
-extern "C" mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *c, double *ms_out, int n) {
-  if (!c || !ms_out) return MPCD_EINVAL;
-  for (int i = 0; i < n; i++) ms_out[i] = 0.0;
+extern "C" mpcd_status mpcd_gpu_profile(mpcd_gpu_ctx *c, int enable) {
+  if (!c) return MPCD_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->prof = enable != 0;
+  c->prof_recs.clear();
+  c->ev_used = 0;
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *c, double *out, int n) {
+  if (!c || !out) return MPCD_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> tot(F_N, 0.0), cnt(F_N, 0.0);
+  for (auto &r : c->prof_recs) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, r.a, r.b));
+    tot[r.fam] += ms;
+    cnt[r.fam] += 1.0;
+  }
+  for (int i = 0; i < n; i++) out[i] = 0.0;
+  for (int f = 0; f < F_N; f++) {
+    if (f < n) out[f] = tot[f];
+    if (F_N + f < n) out[F_N + f] = cnt[f];
+  }
   return MPCD_OK;
 }

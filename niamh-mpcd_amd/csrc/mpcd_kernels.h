@@ -58,7 +58,7 @@ struct Ctx {
   const RepDev *rep;
   const mpcd_gpu_species *sp;
   const mpcd_gpu_wall *wall;
-  double *wq, *wq_s, *wq_b;  // wall positions: persistent, shifted, shifted back
+  double *wq_s, *wq_b;       // wall positions of this step: shifted, shifted back
   double *shift;             // [3 * nrep]
   int32_t *hist, *start, *cnew, *loc, *cu;
   uint64_t *key, *skey;
@@ -113,7 +113,10 @@ __device__ __forceinline__ void shift_back(const RepDev &R, const double *s, dou
 // ---------------------------------------------------------------------------
 // 1. per-replica grid shift draw and wall positions (ranshift, rand.c:678)
 // ---------------------------------------------------------------------------
-__global__ void k_shift(Ctx C, Draws D, int do_shift) {
+__global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, double *shift_out, double *wqs_out,
+                        double *wqb_out) {
+  // reads the previous step's shifted-back wall positions and writes this
+  // step's shift and wall positions, so a repeated launch is idempotent
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C.nrep) return;
   const RepDev &R = C.rep[r];
@@ -121,19 +124,17 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift) {
   if (do_shift && R.galinv) {
     for (int j = 0; j < R.dim; j++) s[j] = D.injected ? D.shift[r * 3 + j] : D.rng.uniform(r, P_SHIFT, 0, j);
   }
-  for (int j = 0; j < 3; j++) C.shift[r * 3 + j] = s[j];
+  for (int j = 0; j < 3; j++) shift_out[r * 3 + j] = s[j];
   for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
     for (int j = 0; j < 3; j++) {
-      double q = C.wq[w * 3 + j];
+      double q = wq_prev_b[w * 3 + j];
       if (j < R.dim && do_shift) {
         double qs = q + s[j];
-        double qb = qs + (-1.0 * s[j]);
-        C.wq_s[w * 3 + j] = qs;
-        C.wq_b[w * 3 + j] = qb;
-        C.wq[w * 3 + j] = qb;
+        wqs_out[w * 3 + j] = qs;
+        wqb_out[w * 3 + j] = qs + (-1.0 * s[j]);
       } else {
-        C.wq_s[w * 3 + j] = q;
-        C.wq_b[w * 3 + j] = q;
+        wqs_out[w * 3 + j] = q;
+        wqb_out[w * 3 + j] = q;
       }
     }
   }
@@ -144,7 +145,7 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift) {
 //    1: shifted bin; 2: materialise the pending unshifted bin
 // ---------------------------------------------------------------------------
 template <typename Real>
-__global__ void k_count(Ctx C, State<Real> S, const BlkDev *blk, int mode, int pending) {
+__global__ void k_count(Ctx C, State<Real> S, const BlkDev *blk, int mode, int pending, int cu_src) {
   const BlkDev B = blk[blockIdx.x];
   int s = B.beg + threadIdx.x;
   if (s >= B.end) return;
@@ -159,7 +160,9 @@ __global__ void k_count(Ctx C, State<Real> S, const BlkDev *blk, int mode, int p
     cu = cnew;
   } else {
     int cs = S.cst[s];
-    cu = pending ? C.cu[s] : cs;
+    // the unshifted cell left by streaming: in cu[] or in the key the fused
+    // kernel wrote for this slot (bits 32..62)
+    cu = pending ? (cu_src ? (int)((C.key[s] >> 32) & 0x7FFFFFFFull) : C.cu[s]) : cs;
     a1 = (cu != cs) ? 1u : 0u;
     if (mode == 1) {
       shift_fwd(R, &C.shift[B.rep * 3], q);
@@ -991,30 +994,9 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
 }
 
 // ---------------------------------------------------------------------------
-// 11. TEMP (therm.c:209-235) and the thermostat factor (therm.c:52-143),
-//     including the active-matter guard of timestep() (mpc.c:4766-4778)
+// 11. thermostat velocity scaling (scaleT, therm.c:52-143); the factor comes
+//     from k_temp_reduce
 // ---------------------------------------------------------------------------
-__global__ void k_temp_final(Ctx C, const BlkDev *cblk, int nblk) {
-  int r = threadIdx.x;
-  if (r >= C.nrep) return;
-  const RepDev &R = C.rep[r];
-  double K = 0.0;
-  for (int b = 0; b < nblk; b++)
-    if (cblk[b].rep == r) K += C.tpart[b];
-  K /= ((double)(R.npart * R.dim));
-  double g = 1.0;
-  if (R.tstech == MPCD_VSC) g = sqrt(R.kbt / K);
-  else if (R.tstech == MPCD_BEREND) g = sqrt(1. + (R.kbt / K - 1.) * R.dt / R.tau);
-  K = K * g * g;
-  if (R.rtech >= 7 && R.rtech <= 17 && K > 50.0 * R.kbt) {
-    double g2 = sqrt(R.kbt / K);
-    g *= g2;
-    K = K * g2 * g2;
-  }
-  C.gscale[r] = g;
-  C.kbtnow[r] = K;
-}
-
 template <typename Real>
 __global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
   const BlkDev B = blk[blockIdx.x];
