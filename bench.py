@@ -176,6 +176,19 @@ def cpu_baseline(L=(64, 64, 64), ppc=10, steps=3):
                 sample=f"3D SRD {L[0]}x{L[1]}x{L[2]} cells x {ppc} ppc ({N} particles), {steps} steps, one core")
 
 
+def copy_probe(mib):
+    """a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
+    calibrates the counters of the step kernels"""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,7 +200,12 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "replicas shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)

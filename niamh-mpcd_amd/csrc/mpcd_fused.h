@@ -1,17 +1,20 @@
 // mpcd_fused.h -- the fused, LDS-tiled step kernel for the isotropic
 // collision operators (SRD, MPC-AT, MPC-AT+a).
 //
-// One workgroup owns a tile of consecutive cells.  It
+// One wavefront owns a tile of consecutive cells.  It
 //   1. loads the tile's sorted 64-bit keys and ranks them inside each cell
 //      (the composite key reproduces the reference's linked-list order),
 //   2. gathers the particles' r, v (and id, species) from the previous
-//      buffer into LDS, applying the grid shift and the body force,
-//   3. runs one thread per cell over its particles in list order for the
-//      FP64 cell sums (mass, centre-of-mass velocity and position, inertia
+//      buffer into LDS, one lane per particle, applying the grid shift and
+//      the body force and drawing the thermal velocities of MPC-AT(+a),
+//   3. runs one lane per cell over its particles in list order for the FP64
+//      cell sums (mass, centre-of-mass velocity and position, inertia
 //      tensor, random-velocity sums, angular momentum) -- the reference's own
 //      summation order, so FP64 results match it to round-off,
 //   4. applies the collision, the kinetic-energy partial sum, the shift
-//      back, streaming and the planar walls with one thread per particle,
+//      back, streaming and the planar walls with one lane per particle; a
+//      particle strictly inside the step's wall-free box (computed with the
+//      wall positions) skips the wall search, which would find nothing,
 //   5. writes the new state coalesced, and -- when the next step's grid
 //      shift is known -- the next step's cell, in-cell rank and key plus its
 //      histogram (global atomics), so the next step starts at the scan.
@@ -22,36 +25,73 @@
 
 namespace mpcd {
 
-constexpr int kTileThreads = 256;
-constexpr int kTileCap = 640;     // particles staged in LDS at once
-constexpr int kTileCells = 64;    // cells per tile at most
+constexpr int kTileThreads = 64;  // one wavefront per tile: no cross-wave barriers
+constexpr int kTileCells = 32;    // cells per tile at most
+
+enum { OP_SRD = 0, OP_AT = 1, OP_RAT = 2 };
+
+// particles staged in LDS at once (sized for about a dozen workgroups per CU)
+__host__ __device__ constexpr int tile_cap(bool fp64, int op) { return (fp64 || op != OP_SRD) ? 192 : 256; }
 
 struct TileDev {
   int rep, c0, c1, pad;
 };
 
-enum { OP_SRD = 0, OP_AT = 1, OP_RAT = 2 };
+// cell_of / shift_fwd / shift_back with the dimension known at compile time
+// (same arithmetic as the runtime-dimension versions)
+template <int DIM>
+__device__ __forceinline__ int cell_of_d(const RepDev &R, const double q[3], int shifted) {
+  int a = (int)q[0], b = (int)q[1], c = (int)q[2];
+  const int Lz = (DIM == 3) ? R.L[2] : 1;
+  if (shifted) {
+    if (R.pbc[0] && a == R.L[0]) a = 0;
+    if (R.pbc[1] && b == R.L[1]) b = 0;
+    if (R.pbc[2] && c == Lz) c = 0;
+  }
+  if (a > R.L[0] || b > R.L[1] || c > Lz || a < 0 || b < 0 || c < 0) return -1;
+  if (DIM < 3) c = 0;
+  return R.cell_off + (a * R.ny + b) * R.nz + c;
+}
 
-// per-particle stream + walls + unshifted cell, shared by the fused kernel
-// and the split (thermostatted) path.  Returns the unshifted cell (or the
-// slot's cell for an escaped particle).
-__device__ __forceinline__ int stream_walls(const Ctx &C, const RepDev &R, const double *wq_b, const Draws &Dr,
-                                            int rep, int spid, int pid, int cell_this, double Q[3], double V[3],
-                                            double U[3], bool lc) {
-  const int dim = R.dim;
-  const mpcd_gpu_species &sp = C.sp[spid];
-  for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
+template <int DIM>
+__device__ __forceinline__ void shift_fwd_d(const RepDev &R, const double *s, double q[3]) {
+#pragma unroll
+  for (int j = 0; j < DIM; j++) {
+    q[j] += s[j];
+    if (q[j] > R.L[j] && q[j] <= R.L[j] + s[j]) q[j] -= R.L[j];
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ void shift_back_d(const RepDev &R, const double *s, double q[3]) {
+#pragma unroll
+  for (int j = 0; j < DIM; j++) {
+    double sg = -1.0 * s[j];
+    q[j] += sg;
+    if (q[j] < 0.0 && q[j] >= sg) q[j] += R.L[j];
+  }
+}
+
+// MPC_BCcollision for planar walls (bc.c:1363-1496) on an already drifted
+// particle: search the walls, bounce, repeat until no wall is crossed
+template <int DIM>
+__device__ __forceinline__ void wall_loop(const Ctx &C, const RepDev &R, const mpcd_gpu_species &sp,
+                                       const mpcd_gpu_wall *__restrict__ walls, const WallHot *__restrict__ wh,
+                                       const double *__restrict__ wq_b, const Draws &Dr, int rep, int spid, int pid,
+                                       double Q[3], double V[3], double U[3], bool lc) {
+  const int dim = DIM;
   double t_step = R.dt, t_left = t_step, t_coll = 0.0;
   int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
   double n[3] = {0, 0, 0};
   while (R.nwall > 0) {
     double tmin = t_left, chW = 1.0;
     for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
-      const mpcd_gpu_wall &W = C.wall[w];
-      if (!((W.inter_mask >> spid) & 1)) continue;
-      const double *wq = &wq_b[w * 3];
-      double tw = wall_W(W, wq, Q, dim);
+      const WallHot &H = wh[w];
+      if (!((H.mask >> spid) & 1)) continue;
+      double tw = wall_W_hot(H, Q, H.wqb);
       if (tw < 0.0) {
+        const mpcd_gpu_wall &W = walls[w];
+        const double *wq = &wq_b[w * 3];
         double tc = W.R;
         for (int i = 0; i < dim; i++) tc += W.A[i] * (wq[i] - Q[i]);
         tc /= (W.A[0] * (-V[0]) + W.A[1] * (-V[1]) + W.A[2] * (-V[2]));
@@ -62,7 +102,7 @@ __device__ __forceinline__ int stream_walls(const Ctx &C, const RepDev &R, const
     t_coll = tmin;
     if (chW > -kTOL) break;
     cnt++;
-    const mpcd_gpu_wall &W = C.wall[chosen];
+    const mpcd_gpu_wall &W = walls[chosen];
     const double *wq = &wq_b[chosen * 3];
     if (t_coll > -kTOL) {
       for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
@@ -126,14 +166,22 @@ __device__ __forceinline__ int stream_walls(const Ctx &C, const RepDev &R, const
         cnt2++;
         cnt = 0;
       }
-      if (cnt2 > 1000) break;
+      if (cnt2 > 1000) break;  // a particle that can never leave the walls
     }
   }
   if (flagCNT) atomicAdd(&C.events[1], 1ull);
-  int cu = cell_of(R, Q, 0);
+}
+
+// the unshifted bin after streaming (mpc.c:4936): escaped particles are
+// re-placed and stay in the list they were in (mpc.c:970-982)
+template <int DIM>
+__device__ __forceinline__ int unshifted_cell(const Ctx &C, const RepDev &R, const mpcd_gpu_species &sp,
+                                              const Draws &Dr, int rep, int pid, int cell_this, double Q[3],
+                                              double V[3]) {
+  int cu = cell_of_d<DIM>(R, Q, 0);
   if (cu < 0) {
     atomicAdd(&C.events[0], 1ull);
-    for (int j = 0; j < dim; j++) {
+    for (int j = 0; j < DIM; j++) {
       Q[j] = Dr.rng.uniform(rep, P_ESCAPE, pid, 8 + j) * R.L[j];
       V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(rep, P_ESCAPE, pid, 24 + j);
     }
@@ -142,29 +190,66 @@ __device__ __forceinline__ int stream_walls(const Ctx &C, const RepDev &R, const
   return cu;
 }
 
-// next-step binning of one particle: histogram + rank + composite key
-__device__ __forceinline__ void next_bin(const Ctx &C, const RepDev &R, const double *shift_next, int32_t *hist_next,
-                                         int slot, int cell_this, int cu, const double Q[3]) {
-  double q[3] = {Q[0], Q[1], Q[2]};
-  shift_fwd(R, shift_next, q);
-  int cn = cell_of(R, q, 1);
-  if (cn < 0) cn = cu;
-  uint32_t a1 = (cu != cell_this) ? 1u : 0u, a2 = (cn != cu) ? 1u : 0u;
-  int l = atomicAdd(&hist_next[cn], 1);
-  C.cnew[slot] = cn;
-  C.loc[slot] = l;
-  C.key[slot] = ((uint64_t)a2 << 63) | ((uint64_t)(uint32_t)cu << 32) | ((uint64_t)a1 << 31) | (uint64_t)(uint32_t)slot;
+// stream_all (mpc.c:680) + walls + unshifted cell of one particle.  `box`
+// holds the step's wall-free box [lo0, hi0, lo1, hi1, lo2, hi2]: a drifted
+// particle strictly inside it crosses no wall, so the search is skipped.
+template <int DIM>
+__device__ __forceinline__ int stream_walls(const Ctx &C, const RepDev &R, const mpcd_gpu_species *__restrict__ sps,
+                                            const mpcd_gpu_wall *__restrict__ walls,
+                                            const WallHot *__restrict__ wh, const double *__restrict__ wq_b,
+                                            const double *__restrict__ box, const Draws &Dr, int rep, int spid,
+                                            int pid, int cell_this, double Q[3], double V[3], double U[3], bool lc) {
+  const mpcd_gpu_species &sp = sps[spid];
+#pragma unroll
+  for (int i = 0; i < DIM; i++) Q[i] = Q[i] + R.dt * V[i];
+  bool inside = true;
+#pragma unroll
+  for (int i = 0; i < DIM; i++) inside = inside && (Q[i] > box[2 * i]) && (Q[i] < box[2 * i + 1]);
+  if (!inside) wall_loop<DIM>(C, R, sp, walls, wh, wq_b, Dr, rep, spid, pid, Q, V, U, lc);
+  return unshifted_cell<DIM>(C, R, sp, Dr, rep, pid, cell_this, Q, V);
 }
 
-template <typename Real, int OP, bool ONESP>
-__global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, State<Real> Sn, const TileDev *tiles,
-                                                        Draws Dr, const double *shift_cur, const double *wq_b,
-                                                        const double *shift_next, int32_t *hist_next, int do_next,
-                                                        int do_stream) {
+// next-step shifted cell of a streamed particle (bin(shifted=1), mpc.c:948);
+// an escaped particle stays in its list
+template <int DIM>
+__device__ __forceinline__ int next_cell(const RepDev &R, const double *shift_next, int cu, const double Q[3]) {
+  double q[3] = {Q[0], Q[1], Q[2]};
+  shift_fwd_d<DIM>(R, shift_next, q);
+  int cn = cell_of_d<DIM>(R, q, 1);
+  return cn < 0 ? cu : cn;
+}
+
+__device__ __forceinline__ uint64_t bin_key(int cn, int cu, int cell_this, int slot) {
+  uint32_t a1 = (cu != cell_this) ? 1u : 0u, a2 = (cn != cu) ? 1u : 0u;
+  return ((uint64_t)a2 << 63) | ((uint64_t)(uint32_t)cu << 32) | ((uint64_t)a1 << 31) | (uint64_t)(uint32_t)slot;
+}
+
+// next-step binning of one particle: histogram + rank + composite key
+template <int DIM>
+__device__ __forceinline__ void next_bin(const Ctx &C, const RepDev &R, const double *shift_next, int32_t *hist_next,
+                                         int slot, int cell_this, int cu, const double Q[3]) {
+  const int cn = next_cell<DIM>(R, shift_next, cu, Q);
+  const int l = atomicAdd(&hist_next[cn], 1);
+  C.cnew[slot] = cn;
+  C.loc[slot] = l;
+  C.key[slot] = bin_key(cn, cu, cell_this, slot);
+}
+
+template <typename Real, int OP, bool ONESP, int DIM>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_fused(
+    Ctx C, State<Real> So, State<Real> Sn, const TileDev *__restrict__ tiles, const RepDev *__restrict__ reps,
+    const mpcd_gpu_species *__restrict__ sps, const mpcd_gpu_wall *__restrict__ walls,
+    const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr,
+    const double *__restrict__ shift_cur, const double *__restrict__ wq_b, const double *__restrict__ shift_next,
+    int32_t *hist_next, int do_next, int do_stream) {
+  // the uniform tables come in through restrict-qualified arguments so that
+  // their loads can be scalar
+  constexpr int kTileCap = tile_cap(sizeof(Real) == 8, OP);
   const TileDev T = tiles[blockIdx.x];
-  const RepDev &R = C.rep[T.rep];
-  const int dim = R.dim, tid = threadIdx.x;
+  const RepDev &R = reps[T.rep];
+  const int tid = threadIdx.x;
   const double *shc = &shift_cur[T.rep * 3];
+  const double *box = &boxes[T.rep * 6];
 
   __shared__ int32_t s_start[kTileCells + 1];
   __shared__ int32_t s_ce;
@@ -172,6 +257,7 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
   __shared__ int32_t s_src[kTileCap];
   __shared__ uint8_t s_cell[kTileCap];
   __shared__ uint8_t s_sp[ONESP ? 1 : kTileCap];
+  __shared__ int32_t s_id[kTileCap];
   __shared__ Real s_q[3][kTileCap];
   __shared__ Real s_v[3][kTileCap];
   __shared__ double s_rv[OP == OP_SRD ? 1 : 3][OP == OP_SRD ? 1 : kTileCap];
@@ -221,21 +307,21 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
       s_cell[pos] = (uint8_t)lo;
     }
     __syncthreads();
-    // gather with grid shift and body force
+    // gather with grid shift and body force; thermal draws of MPC-AT(+a)
     for (int p = tid; p < n; p += kTileThreads) {
       const int src = s_src[p];
       double q[3];
       load_q(So, src, q);
       double v[3] = {(double)So.vx[src], (double)So.vy[src], (double)So.vz[src]};
-      if (R.galinv) shift_fwd(R, shc, q);
+      const int pid = So.id[src];
       const int spid = ONESP ? 0 : So.sp[src];
-      if (cell_of(R, q, 1) < 0) {
+      if (R.galinv) shift_fwd_d<DIM>(R, shc, q);
+      if (cell_of_d<DIM>(R, q, 1) < 0) {
         atomicAdd(&C.events[0], 1ull);
-        const int id = So.id[src];
-        double m = C.sp[spid].mass;
-        for (int j = 0; j < dim; j++) {
-          q[j] = Dr.rng.uniform(T.rep, P_ESCAPE, (uint32_t)id, j) * R.L[j];
-          v[j] = sqrt(R.kbt / m) * Dr.rng.normal(T.rep, P_ESCAPE, (uint32_t)id, 16 + j);
+        double m = sps[spid].mass;
+        for (int j = 0; j < DIM; j++) {
+          q[j] = Dr.rng.uniform(T.rep, P_ESCAPE, (uint32_t)pid, j) * R.L[j];
+          v[j] = sqrt(R.kbt / m) * Dr.rng.normal(T.rep, P_ESCAPE, (uint32_t)pid, 16 + j);
         }
       }
       s_q[0][p] = (Real)q[0];
@@ -244,7 +330,7 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
       if (R.gravflag) {
         // acc_all (mpc.c:697), in the state precision
         Real vv[3] = {(Real)v[0], (Real)v[1], (Real)v[2]};
-        for (int d = 0; d < dim; d++) vv[d] = (Real)((double)vv[d] + R.dt * R.grav[d]);
+        for (int d = 0; d < DIM; d++) vv[d] = (Real)((double)vv[d] + R.dt * R.grav[d]);
         s_v[0][p] = vv[0];
         s_v[1][p] = vv[1];
         s_v[2][p] = vv[2];
@@ -253,27 +339,33 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
         s_v[1][p] = (Real)v[1];
         s_v[2][p] = (Real)v[2];
       }
-      Sn.id[base + p] = So.id[src];
+      Sn.id[base + p] = pid;
+      s_id[p] = pid;
       if (!ONESP) {
         Sn.sp[base + p] = spid;
         s_sp[p] = (uint8_t)spid;
       }
+      if (OP != OP_SRD) {
+        // thermal velocities of the collision (mpc.c:1382, 1506), keyed by slot
+        const double sig = sqrt(R.kbt / sps[spid].mass);
+        for (int j = 0; j < DIM; j++) s_rv[j][p] = sig * Dr.part_n(T.rep, base + p, j);
+      }
     }
     __syncthreads();
-    // one thread per cell: sums in list order and the collision's cell terms
+    // one lane per cell: sums in list order and the collision's cell terms
     for (int k = tid; k < ncl; k += kTileThreads) {
       const int c = cs + k;
       const int b = s_start[k], e = s_start[k + 1], np_ = e - b;
       double mass = 0.0, vcm[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
       for (int p = b; p < e; p++) {
-        const double m = C.sp[ONESP ? 0 : s_sp[p]].mass;
+        const double m = sps[ONESP ? 0 : s_sp[p]].mass;
         mass += m;
-        for (int d = 0; d < dim; d++) vcm[d] += (double)s_v[d][p] * m;
+        for (int d = 0; d < DIM; d++) vcm[d] += (double)s_v[d][p] * m;
         if (OP == OP_RAT)
-          for (int d = 0; d < dim; d++) cm[d] += (double)s_q[d][p] * m;
+          for (int d = 0; d < DIM; d++) cm[d] += (double)s_q[d][p] * m;
       }
       if (mass > 0.0)
-        for (int d = 0; d < dim; d++) {
+        for (int d = 0; d < DIM; d++) {
           vcm[d] /= mass;
           if (OP == OP_RAT) cm[d] /= mass;
         }
@@ -281,20 +373,20 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
       if (R.anyphantom && np_ > 0) {
         int lc_ = c - R.cell_off;
         int a = lc_ / (R.ny * R.nz), bb = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
-        const int nco = (dim == 3) ? 8 : 4;
+        const int nco = (DIM == 3) ? 8 : 4;
         int flagW = 0;
         for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
-          const mpcd_gpu_wall &W = C.wall[w];
-          if (!W.PHANTOM) continue;
+          const WallHot &H = wh[w];
+          if (!H.phantom) continue;
           for (int j = 0; j < nco; j++) {
             double tq[3] = {(double)a + (double)(j & 1), (double)bb + (double)((j >> 1) & 1),
                             (double)cz + (double)((j >> 2) & 1)};
-            if (wall_W(W, &C.wq_s[w * 3], tq, dim) < 0.0) flagW = 1;
+            if (wall_W_hot(H, tq, H.wqs) < 0.0) flagW = 1;
           }
         }
         if (flagW && np_ < R.gndnst) {
           double invN = 1.0 / (R.gndnst - (double)np_);
-          for (int d = 0; d < dim; d++) {
+          for (int d = 0; d < DIM; d++) {
             double sigma = sqrt(R.kbt / invN);
             vcm[d] += sigma * Dr.ghost_n(T.rep, c, d);
             vcm[d] /= R.gndnst;
@@ -310,17 +402,17 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
         double RV[3] = {0., 0., 0.};
         int CA = 0;
         if (R.rtech == MPCD_ARBAXIS) {
-          if (dim == 3) Dr.cell_axis(T.rep, c, RV);
+          if (DIM == 3) Dr.cell_axis(T.rep, c, RV);
           else { double x = Dr.cell_u(T.rep, c, 0); RV[0] = x; RV[1] = sqrt(1. - x * x); }
         } else {
-          CA = (int)((int)dim * Dr.cell_u(T.rep, c, 0));
+          CA = (int)((int)DIM * Dr.cell_u(T.rep, c, 0));
         }
         double sgn = Dr.cell_u(T.rep, c, 3);
         double Cc = R.cosA, Sn_;
         if (R.true_sign ? (sgn >= 0.5) : ((long)sgn > 0.5)) Sn_ = R.sinA;
         else Sn_ = -1. * R.sinA;
         double *M = cd + 3;
-        if (dim == 3) {
+        if (DIM == 3) {
           const double *L = RV;
           if (R.rtech == MPCD_ARBAXIS) {
             M[0] = L[0] * L[0] + (1. - L[0] * L[0]) * Cc;
@@ -345,17 +437,13 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
           M[6] = 0.; M[7] = 0.; M[8] = 0.;
         }
       } else {
-        // thermal draws in list order and their mass-weighted sum
+        // mass-weighted sum of the thermal draws, in list order
         double RS[3] = {0, 0, 0};
         for (int p = b; p < e; p++) {
-          const double m = C.sp[ONESP ? 0 : s_sp[p]].mass, sig = sqrt(R.kbt / m);
-          for (int j = 0; j < dim; j++) {
-            double rv = sig * Dr.part_n(T.rep, base + p, j);
-            s_rv[j][p] = rv;
-            RS[j] += m * rv;
-          }
+          const double m = sps[ONESP ? 0 : s_sp[p]].mass;
+          for (int j = 0; j < DIM; j++) RS[j] += m * s_rv[j][p];
         }
-        for (int j = 0; j < dim; j++) RS[j] /= mass;
+        for (int j = 0; j < DIM; j++) RS[j] /= mass;
         cd[3] = RS[0];
         cd[4] = RS[1];
         cd[5] = RS[2];
@@ -363,9 +451,9 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
           // inertia about the centre of mass (mpc.c:4209-4299) and angular momentum (mpc.c:1554-1618)
           double I[6] = {0, 0, 0, 0, 0, 0};
           for (int p = b; p < e; p++) {
-            const double m = C.sp[ONESP ? 0 : s_sp[p]].mass;
+            const double m = sps[ONESP ? 0 : s_sp[p]].mass;
             double r[3] = {0, 0, 0};
-            for (int d = 0; d < dim; d++) r[d] = (double)s_q[d][p] - cm[d];
+            for (int d = 0; d < DIM; d++) r[d] = (double)s_q[d][p] - cm[d];
             I[0] += m * (r[1] * r[1] + r[2] * r[2]);
             I[1] += m * (r[0] * r[0] + r[2] * r[2]);
             I[2] += m * (r[0] * r[0] + r[1] * r[1]);
@@ -375,7 +463,7 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
           }
           I[3] *= -1.; I[4] *= -1.; I[5] *= -1.;
           double II[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-          if (dim == 3) {
+          if (DIM == 3) {
             if (np_ >= 3) {
               double Im[3][3] = {{I[0], I[3], I[4]}, {I[3], I[1], I[5]}, {I[4], I[5], I[2]}};
               inv3(II, Im);
@@ -385,9 +473,9 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
           }
           double L[3] = {0, 0, 0};
           for (int p = b; p < e; p++) {
-            const double m = C.sp[ONESP ? 0 : s_sp[p]].mass;
+            const double m = sps[ONESP ? 0 : s_sp[p]].mass;
             double rq[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, am[3];
-            for (int j = 0; j < dim; j++) {
+            for (int j = 0; j < DIM; j++) {
               rq[j] = (double)s_q[j][p] - cm[j];
               dv[j] = m * ((double)s_v[j][p] - s_rv[j][p]);
             }
@@ -404,13 +492,14 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
       }
     }
     __syncthreads();
-    // one thread per particle: collision, kinetic energy, shift back, stream, walls, next bin
+    // one lane per particle: collision, kinetic energy, shift back, stream,
+    // walls, next bin
     for (int p = tid; p < n; p += kTileThreads) {
       const int k = s_cell[p], c = cs + k;
       const int np_ = s_start[k + 1] - s_start[k];
       const double *cd = s_cd[k];
       const int spid = ONESP ? 0 : s_sp[p];
-      const mpcd_gpu_species &sp = C.sp[spid];
+      const mpcd_gpu_species &sp = sps[spid];
       double v[3] = {(double)s_v[0][p], (double)s_v[1][p], (double)s_v[2][p]};
       double q[3] = {(double)s_q[0][p], (double)s_q[1][p], (double)s_q[2][p]};
       if (np_ > 1) {
@@ -419,22 +508,29 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
           const double *M = cd + 3;
           double Vr[3] = {v[0] - vcm[0], v[1] - vcm[1], v[2] - vcm[2]};
           double TM[3] = {0.0, 0.0, 0.0};
-          for (int i = 0; i < dim; i++)
-            for (int j = 0; j < dim; j++) TM[i] += M[i * 3 + j] * Vr[j];
-          for (int d = 0; d < dim; d++) v[d] = vcm[d] + TM[d];
+#pragma unroll
+          for (int i = 0; i < DIM; i++)
+#pragma unroll
+            for (int j = 0; j < DIM; j++) TM[i] += M[i * 3 + j] * Vr[j];
+#pragma unroll
+          for (int d = 0; d < DIM; d++) v[d] = vcm[d] + TM[d];
         } else if (OP == OP_AT) {
-          for (int j = 0; j < dim; j++) {
+#pragma unroll
+          for (int j = 0; j < DIM; j++) {
             double dv = (sp.damp) * (vcm[j]) / ((double)np_);
             v[j] = vcm[j] + s_rv[j][p] - cd[3 + j] - dv;
           }
         } else {
           double rq[3] = {0, 0, 0}, at[3];
-          for (int j = 0; j < dim; j++) rq[j] = q[j] - cd[9 + j];
+#pragma unroll
+          for (int j = 0; j < DIM; j++) rq[j] = q[j] - cd[9 + j];
           const double Wv[3] = {cd[6], cd[7], cd[8]};
           cross3(Wv, rq, at);
-          for (int j = 0; j < dim; j++) v[j] = vcm[j] + s_rv[j][p] - cd[3 + j] - (sp.damp) * (vcm[j]) + at[j];
+#pragma unroll
+          for (int j = 0; j < DIM; j++) v[j] = vcm[j] + s_rv[j][p] - cd[3 + j] - (sp.damp) * (vcm[j]) + at[j];
         }
-        for (int d = 0; d < dim; d++) v[d] = (double)(Real)v[d];  // the state keeps the collided velocity
+#pragma unroll
+        for (int d = 0; d < DIM; d++) v[d] = (double)(Real)v[d];  // the state keeps the collided velocity
       }
       {
         double v0 = v[0] - R.avv[0], v1 = v[1] - R.avv[1], v2 = v[2] - R.avv[2];
@@ -444,19 +540,21 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
         ke += sp.mass * tmp;
       }
       if (R.galinv) {
-        shift_back(R, shc, q);
-        for (int d = 0; d < dim; d++) q[d] = (double)(Real)q[d];
+        shift_back_d<DIM>(R, shc, q);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) q[d] = (double)(Real)q[d];
       }
       const int slot = base + p;
       if (do_stream) {
         double U0[3] = {0, 0, 0};
-        const int pid = Sn.id[slot];
-        int cu = stream_walls(C, R, wq_b, Dr, T.rep, spid, pid, c, q, v, U0, false);
+        const int cu = stream_walls<DIM>(C, R, sps, walls, wh, wq_b, box, Dr, T.rep, spid, s_id[p], c, q, v, U0,
+                                         false);
+#pragma unroll
         for (int d = 0; d < 3; d++) {
           q[d] = (double)(Real)q[d];
           v[d] = (double)(Real)v[d];
         }
-        if (do_next) next_bin(C, R, &shift_next[T.rep * 3], hist_next, slot, c, cu, q);
+        if (do_next) next_bin<DIM>(C, R, &shift_next[T.rep * 3], hist_next, slot, c, cu, q);
         else C.cu[slot] = cu;
       }
       Sn.x[slot] = (Real)q[0];
@@ -480,28 +578,32 @@ __global__ __launch_bounds__(kTileThreads) void k_fused(Ctx C, State<Real> So, S
 }
 
 // stream after a global thermostat: scale, stream, walls, next bin
-template <typename Real>
-__global__ void k_stream_next(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, const double *wq_b,
-                              const double *shift_next, int32_t *hist_next, int do_next) {
+template <typename Real, int DIM>
+__global__ void k_stream_next(Ctx C, State<Real> S, const BlkDev *__restrict__ blk, const RepDev *__restrict__ reps,
+                              const mpcd_gpu_species *__restrict__ sps, const mpcd_gpu_wall *__restrict__ walls,
+                              const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr,
+                              const double *__restrict__ wq_b, const double *__restrict__ shift_next,
+                              int32_t *hist_next, int do_next) {
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
   if (t >= B.end) return;
-  const RepDev &R = C.rep[B.rep];
+  const RepDev &R = reps[B.rep];
   const double g = C.gscale[B.rep];
   const bool lc = R.lc != MPCD_ISOF;
   double q[3], v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]}, U[3] = {0, 0, 0};
   load_q(S, t, q);
   if (g != 1.0) {
-    for (int d = 0; d < R.dim; d++) v[d] = (double)(Real)(v[d] * g);
+    for (int d = 0; d < DIM; d++) v[d] = (double)(Real)(v[d] * g);
   }
   if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
   const int c = S.cst[t];
-  int cu = stream_walls(C, R, wq_b, Dr, B.rep, S.sp[t], S.id[t], c, q, v, U, lc);
+  int cu = stream_walls<DIM>(C, R, sps, walls, wh, wq_b, &boxes[B.rep * 6], Dr, B.rep, S.sp[t], S.id[t], c, q, v, U,
+                             lc);
   for (int d = 0; d < 3; d++) {
     q[d] = (double)(Real)q[d];
     v[d] = (double)(Real)v[d];
   }
-  if (do_next) next_bin(C, R, &shift_next[B.rep * 3], hist_next, t, c, cu, q);
+  if (do_next) next_bin<DIM>(C, R, &shift_next[B.rep * 3], hist_next, t, c, cu, q);
   else C.cu[t] = cu;
   S.x[t] = (Real)q[0];
   S.y[t] = (Real)q[1];

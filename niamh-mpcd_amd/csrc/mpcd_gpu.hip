@@ -53,6 +53,8 @@ struct mpcd_gpu_ctx {
   mpcd_gpu_wall *d_wall = nullptr;
   Ctx C{};
   double *shift2 = nullptr, *wqs2 = nullptr, *wqb2 = nullptr;  // per step parity
+  WallHot *wh2 = nullptr;                                         // per step parity
+  double *box2 = nullptr;                                         // per step parity: wall-free boxes
   int32_t *hist2[2] = {nullptr, nullptr};
   State<double> sd[2]{};
   State<float> sf[2]{};
@@ -75,6 +77,10 @@ struct mpcd_gpu_ctx {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
 };
+
+// mean particles per cell above which the split path is used (cells must
+// fit the fused kernel's LDS capacity; populations are Poisson distributed)
+constexpr double kFusedMaxPPC = 40.0;
 
 // kernel families reported by mpcd_gpu_timing
 enum { F_COUNT = 0, F_SCAN, F_SCATTER, F_SORT, F_GATHER, F_PROPS, F_GHOST, F_LC, F_COLLIDE, F_TEMP, F_STREAM, F_FUSED,
@@ -178,12 +184,17 @@ static void set_pbc(const mpcd_gpu_params &p, int pbc[3]) {
   }
 }
 
-// cells per fused tile for a replica: about kTileTarget particles per tile
-static int tile_cells(const RepDev &R) {
-  const int kTileTarget = 320;
+static double mean_ppc(const RepDev &R) {
   double vol = (double)R.L[0] * R.L[1] * (R.dim == 3 ? R.L[2] : 1);
-  double ppc = R.npart / vol;
-  int tc = (int)lround(kTileTarget / std::max(ppc, 1.0));
+  return R.npart / vol;
+}
+
+static int fused_op(int rtech) { return (rtech == MPCD_MPCAT) ? OP_AT : (rtech == MPCD_RAT) ? OP_RAT : OP_SRD; }
+
+// cells per fused tile for a replica: tiles hold a little under half the LDS
+// capacity on average, so a chunk rarely needs splitting
+static int tile_cells(const RepDev &R, int cap) {
+  int tc = (int)lround(0.45 * cap / std::max(mean_ppc(R), 1.0));
   return std::max(1, std::min(kTileCells, tc));
 }
 
@@ -291,7 +302,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
       c->pblk.push_back(BlkDev{r, poff + b, poff + std::min(b + kBlock, R.npart), 0});
     for (int b = 0; b < R.ncell; b += kBlock)
       c->cblk.push_back(BlkDev{r, coff + b, coff + std::min(b + kBlock, R.ncell), 0});
-    int tc = tile_cells(R);
+    int tc = tile_cells(R, tile_cap(p.precision == MPCD_FP64, fused_op(p.rtech)));
     for (int b = 0; b < R.ncell; b += tc) c->tiles.push_back(TileDev{r, coff + b, coff + std::min(b + tc, R.ncell), 0});
     c->cblk_rng.push_back(make_int2(cb0, (int)c->cblk.size()));
     c->tile_rng.push_back(make_int2(t0, (int)c->tiles.size()));
@@ -302,7 +313,10 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->nwtot = woff;
   c->uploaded.assign(nrep, 0);
   c->lc_stride = anneal_stride(params[0]);
+  // the fused kernel stages a cell in LDS: dense systems take the split path
   c->fusable = (c->lc == MPCD_ISOF);
+  for (const RepDev &R : c->reps)
+    if (mean_ppc(R) > kFusedMaxPPC) c->fusable = false;
 
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad(MPCD_EDEVICE, "stream");
   bool ok = true;
@@ -318,10 +332,12 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->shift2 = dalloc<double>(c, 2 * 3 * nrep);
   c->wqs2 = dalloc<double>(c, 2 * nw3);
   c->wqb2 = dalloc<double>(c, 2 * nw3);
+  c->wh2 = dalloc<WallHot>(c, 2 * std::max(1, woff));
+  c->box2 = dalloc<double>(c, 2 * 6 * nrep);
   c->hist2[0] = dalloc<int32_t>(c, c->nctot + 1);
   c->hist2[1] = dalloc<int32_t>(c, c->nctot + 1);
   ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_tile_rng && c->d_sp &&
-       c->d_wall && c->shift2 && c->wqs2 && c->wqb2 && c->hist2[0] && c->hist2[1];
+       c->d_wall && c->shift2 && c->wqs2 && c->wqb2 && c->wh2 && c->box2 && c->hist2[0] && c->hist2[1];
   Ctx &C = c->C;
   C.rep = c->d_rep;
   C.sp = c->d_sp;
@@ -512,7 +528,8 @@ static void launch_shift(mpcd_gpu_ctx *c, uint32_t step, int injected) {
   Draws D = make_draws(c, injected, step);
   k_shift<<<nblocks(c->nrep, 64), 64, 0, c->stream>>>(c->C, D, 1, c->wqb2 + (par ^ 1) * nw3,
                                                         c->shift2 + par * 3 * c->nrep, c->wqs2 + par * nw3,
-                                                        c->wqb2 + par * nw3);
+                                                        c->wqb2 + par * nw3, c->wh2 + par * std::max(1, c->nwtot),
+                                                        c->box2 + par * 6 * c->nrep);
 }
 
 // binning: mode 0 initial, 1 shifted (this step), 2 materialise the pending
@@ -604,29 +621,37 @@ static void run_stream(mpcd_gpu_ctx *c, int injected) {
   c->step++;
 }
 
-template <typename Real, int OP, bool ONESP>
+template <typename Real, int OP, bool ONESP, int DIM>
 static void launch_fused(mpcd_gpu_ctx *c, const Ctx &C, State<Real> *S, const Draws &D, int par, int do_next,
                          int do_stream) {
   const int nw3 = 3 * std::max(1, c->nwtot);
-  k_fused<Real, OP, ONESP><<<c->tiles.size(), kTileThreads, 0, c->stream>>>(
-      C, S[c->cur], S[1 - c->cur], c->d_tiles, D, c->shift2 + par * 3 * c->nrep, c->wqb2 + par * nw3,
+  k_fused<Real, OP, ONESP, DIM><<<c->tiles.size(), kTileThreads, 0, c->stream>>>(
+      C, S[c->cur], S[1 - c->cur], c->d_tiles, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * std::max(1, c->nwtot),
+      c->box2 + par * 6 * c->nrep, D, c->shift2 + par * 3 * c->nrep, c->wqb2 + par * nw3,
       c->shift2 + (par ^ 1) * 3 * c->nrep, c->hist2[par ^ 1], do_next, do_stream);
+}
+
+template <typename Real, int OP, bool ONESP>
+static void launch_fused_dim(mpcd_gpu_ctx *c, const Ctx &C, State<Real> *S, const Draws &D, int par, int do_next,
+                             int do_stream) {
+  if (c->dim == 3) launch_fused<Real, OP, ONESP, 3>(c, C, S, D, par, do_next, do_stream);
+  else launch_fused<Real, OP, ONESP, 2>(c, C, S, D, par, do_next, do_stream);
 }
 
 template <typename Real>
 static void dispatch_fused(mpcd_gpu_ctx *c, const Ctx &C, State<Real> *S, const Draws &D, int par, int do_next,
                            int do_stream) {
   const bool one = c->nspec == 1;
-  const int op = (c->rtech == MPCD_MPCAT) ? OP_AT : (c->rtech == MPCD_RAT) ? OP_RAT : OP_SRD;
+  const int op = fused_op(c->rtech);
   if (op == OP_SRD) {
-    if (one) launch_fused<Real, OP_SRD, true>(c, C, S, D, par, do_next, do_stream);
-    else launch_fused<Real, OP_SRD, false>(c, C, S, D, par, do_next, do_stream);
+    if (one) launch_fused_dim<Real, OP_SRD, true>(c, C, S, D, par, do_next, do_stream);
+    else launch_fused_dim<Real, OP_SRD, false>(c, C, S, D, par, do_next, do_stream);
   } else if (op == OP_AT) {
-    if (one) launch_fused<Real, OP_AT, true>(c, C, S, D, par, do_next, do_stream);
-    else launch_fused<Real, OP_AT, false>(c, C, S, D, par, do_next, do_stream);
+    if (one) launch_fused_dim<Real, OP_AT, true>(c, C, S, D, par, do_next, do_stream);
+    else launch_fused_dim<Real, OP_AT, false>(c, C, S, D, par, do_next, do_stream);
   } else {
-    if (one) launch_fused<Real, OP_RAT, true>(c, C, S, D, par, do_next, do_stream);
-    else launch_fused<Real, OP_RAT, false>(c, C, S, D, par, do_next, do_stream);
+    if (one) launch_fused_dim<Real, OP_RAT, true>(c, C, S, D, par, do_next, do_stream);
+    else launch_fused_dim<Real, OP_RAT, false>(c, C, S, D, par, do_next, do_stream);
   }
 }
 
@@ -671,9 +696,15 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
   if (!do_stream) {
     ProfScope ps(c, F_STREAM);
     const int nw3 = 3 * std::max(1, c->nwtot);
-    k_stream_next<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, c->wqb2 + par * nw3,
-                                                                   c->shift2 + (par ^ 1) * 3 * c->nrep,
-                                                                   c->hist2[par ^ 1], do_next);
+    const int nwm = std::max(1, c->nwtot);
+    if (c->dim == 3)
+      k_stream_next<Real, 3><<<c->pblk.size(), kBlock, 0, c->stream>>>(
+          C, S[c->cur], c->d_pblk, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
+          c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, c->hist2[par ^ 1], do_next);
+    else
+      k_stream_next<Real, 2><<<c->pblk.size(), kBlock, 0, c->stream>>>(
+          C, S[c->cur], c->d_pblk, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
+          c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, c->hist2[par ^ 1], do_next);
   }
   c->pending = true;
   c->cu_src = do_next ? 1 : 0;
@@ -703,7 +734,9 @@ static mpcd_status check_flags(mpcd_gpu_ctx *c) {
   HIPCHK(c, hipMemcpyAsync(flags, c->C.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (flags[0] & 2) return fail(c, MPCD_EINVAL, "initial positions outside the box");
-  if (flags[0] & 1) return fail(c, MPCD_ECAPACITY, "a cell holds more than %d particles", kTileCap);
+  if (flags[0] & 1)
+    return fail(c, MPCD_ECAPACITY, "a cell holds more than %d particles (fused-kernel LDS capacity)",
+                tile_cap(c->precision == MPCD_FP64, fused_op(c->rtech)));
   return MPCD_OK;
 }
 

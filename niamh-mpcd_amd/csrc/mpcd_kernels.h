@@ -71,6 +71,24 @@ struct Ctx {
   int ntot, nctot, nrep;
 };
 
+// hot fields of an axis-aligned planar wall for the per-particle wall test:
+// the normal's one nonzero component A (0 when its axis is not a box axis),
+// the offset R and the wall's shifted-back / shifted position along the axis
+struct WallHot {
+  double A, R, wqb, wqs;
+  int axis, inv, mask, phantom;
+};
+
+// calcW_PLANE (bc.c:306) for an axis-aligned wall: the other axes add exact
+// zeros to the reference's sum, so the sign tests agree with wall_W()
+__device__ __forceinline__ double wall_W_hot(const WallHot &H, const double *q, double wq) {
+  const double qa = (H.axis == 0) ? q[0] : (H.axis == 1) ? q[1] : q[2];
+  double Wv = H.A * (qa - wq);
+  Wv -= H.R;
+  if (H.inv) Wv *= -1.;
+  return Wv;
+}
+
 template <typename Real>
 MPCD_HD void load_q(const State<Real> &s, int i, double q[3]) {
   q[0] = (double)s.x[i];
@@ -114,7 +132,7 @@ __device__ __forceinline__ void shift_back(const RepDev &R, const double *s, dou
 // 1. per-replica grid shift draw and wall positions (ranshift, rand.c:678)
 // ---------------------------------------------------------------------------
 __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, double *shift_out, double *wqs_out,
-                        double *wqb_out) {
+                        double *wqb_out, WallHot *wh_out, double *box_out) {
   // reads the previous step's shifted-back wall positions and writes this
   // step's shift and wall positions, so a repeated launch is idempotent
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,6 +143,9 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, d
     for (int j = 0; j < R.dim; j++) s[j] = D.injected ? D.shift[r * 3 + j] : D.rng.uniform(r, P_SHIFT, 0, j);
   }
   for (int j = 0; j < 3; j++) shift_out[r * 3 + j] = s[j];
+  double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
+  double Lmax = 0.0;
+  for (int j = 0; j < R.dim; j++) Lmax = fmax(Lmax, (double)R.L[j]);
   for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
     for (int j = 0; j < 3; j++) {
       double q = wq_prev_b[w * 3 + j];
@@ -137,6 +158,35 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, d
         wqb_out[w * 3 + j] = q;
       }
     }
+    const mpcd_gpu_wall &W = C.wall[w];
+    int k = 0;
+    for (int j = 0; j < 3; j++)
+      if (W.A[j] != 0.0) k = j;
+    WallHot H;
+    H.A = (k < R.dim) ? W.A[k] : 0.0;
+    H.R = W.R;
+    H.wqb = wqb_out[w * 3 + k];
+    H.wqs = wqs_out[w * 3 + k];
+    H.axis = k;
+    H.inv = W.INV;
+    H.mask = W.inter_mask;
+    H.phantom = W.PHANTOM;
+    wh_out[w] = H;
+    // wall-free box: a point strictly inside is on the positive side of
+    // every wall by far more than the rounding of calcW_PLANE
+    if (H.A == 0.0) {
+      const bool safe = H.inv ? (H.R > 0.0) : (H.R < 0.0);  // calcW is the constant -R (or R)
+      if (!safe) lo[0] = INFINITY;
+    } else {
+      const double b = H.wqb + H.R / H.A;
+      const double m = 1e-9 * (1.0 + fabs(H.wqb) + fabs(H.R / H.A) + Lmax);
+      if ((H.A > 0.0) == (H.inv != 0)) hi[k] = fmin(hi[k], b - m);
+      else lo[k] = fmax(lo[k], b + m);
+    }
+  }
+  for (int j = 0; j < 3; j++) {
+    box_out[r * 6 + 2 * j] = lo[j];
+    box_out[r * 6 + 2 * j + 1] = hi[j];
   }
 }
 
