@@ -156,7 +156,8 @@ __device__ __forceinline__ void wall_loop(const Ctx &C, const RepDev &R, const m
         }
       } else {
         atomicAdd(&C.events[3], 1ull);
-        for (int j = 0; j < dim; j++) {
+#pragma unroll
+        for (int j = 0; j < DIM; j++) {
           if (sp.vdist == 0) V[j] = sqrt(R.kbt / sp.mass) * (2. * Dr.rng.uniform(rep, P_RETHERM, pid, rk + j) - 1.);
           else V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(rep, P_RETHERM, pid, rk + j);
         }
@@ -181,6 +182,7 @@ __device__ __forceinline__ int unshifted_cell(const Ctx &C, const RepDev &R, con
   int cu = cell_of_d<DIM>(R, Q, 0);
   if (cu < 0) {
     atomicAdd(&C.events[0], 1ull);
+#pragma unroll
     for (int j = 0; j < DIM; j++) {
       Q[j] = Dr.rng.uniform(rep, P_ESCAPE, pid, 8 + j) * R.L[j];
       V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(rep, P_ESCAPE, pid, 24 + j);
@@ -319,6 +321,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
       if (cell_of_d<DIM>(R, q, 1) < 0) {
         atomicAdd(&C.events[0], 1ull);
         double m = sps[spid].mass;
+#pragma unroll
         for (int j = 0; j < DIM; j++) {
           q[j] = Dr.rng.uniform(T.rep, P_ESCAPE, (uint32_t)pid, j) * R.L[j];
           v[j] = sqrt(R.kbt / m) * Dr.rng.normal(T.rep, P_ESCAPE, (uint32_t)pid, 16 + j);
@@ -399,15 +402,18 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
       cd[2] = vcm[2];
       if (np_ <= 1) continue;
       if (OP == OP_SRD) {
+        // the cell's draws in the keyed layout: injected, or written by
+        // k_celldraws for this step
+        const double *cdr = &Dr.cell[(size_t)c * 4];
         double RV[3] = {0., 0., 0.};
         int CA = 0;
         if (R.rtech == MPCD_ARBAXIS) {
-          if (DIM == 3) Dr.cell_axis(T.rep, c, RV);
-          else { double x = Dr.cell_u(T.rep, c, 0); RV[0] = x; RV[1] = sqrt(1. - x * x); }
+          if (DIM == 3) { RV[0] = cdr[0]; RV[1] = cdr[1]; RV[2] = cdr[2]; }
+          else { double x = cdr[0]; RV[0] = x; RV[1] = sqrt(1. - x * x); }
         } else {
-          CA = (int)((int)DIM * Dr.cell_u(T.rep, c, 0));
+          CA = (int)((int)DIM * cdr[0]);
         }
-        double sgn = Dr.cell_u(T.rep, c, 3);
+        double sgn = cdr[3];
         double Cc = R.cosA, Sn_;
         if (R.true_sign ? (sgn >= 0.5) : ((long)sgn > 0.5)) Sn_ = R.sinA;
         else Sn_ = -1. * R.sinA;
@@ -577,6 +583,27 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
   if (tid == 0) C.tpart[blockIdx.x] = s_ke[0];
 }
 
+// the SRD cells' rotation draws of one step in the keyed layout of injected
+// draws (cell [4 per cell]: ARBAXIS 3D unit axis in [0..2], otherwise a
+// uniform in [0]; the rotation-sign uniform in [3]), one lane per cell
+__global__ void k_celldraws(Ctx C, const BlkDev *__restrict__ cblk, Draws D, double *out) {
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  if (c >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  double *o = &out[(size_t)c * 4];
+  if (R.rtech == MPCD_ARBAXIS && R.dim == 3) {
+    double v[3];
+    D.rng.unitvec(B.rep, P_CELL, c, 0, v);
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = v[2];
+  } else {
+    o[0] = D.rng.uniform(B.rep, P_CELL, c, 0);
+  }
+  o[3] = D.rng.uniform(B.rep, P_CELL, c, 3);
+}
+
 // stream after a global thermostat: scale, stream, walls, next bin
 template <typename Real, int DIM>
 __global__ void k_stream_next(Ctx C, State<Real> S, const BlkDev *__restrict__ blk, const RepDev *__restrict__ reps,
@@ -621,16 +648,18 @@ __global__ void k_stream_next(Ctx C, State<Real> S, const BlkDev *__restrict__ b
 // TEMP (therm.c:209-235) from per-block kinetic-energy partials of one
 // replica (blocks [beg, end)), with the thermostat factor (therm.c:52-143)
 // and the active-matter guard (mpc.c:4766-4778); one workgroup per replica
-__global__ void k_temp_reduce(Ctx C, const int2 *ranges) {
+constexpr int kTempThreads = 1024;
+
+__global__ __launch_bounds__(kTempThreads) void k_temp_reduce(Ctx C, const int2 *ranges) {
   const int r = blockIdx.x;
   const RepDev &R = C.rep[r];
   const int2 rg = ranges[r];
-  __shared__ double sh[256];
+  __shared__ double sh[kTempThreads];
   double acc = 0.0;
-  for (int b = rg.x + threadIdx.x; b < rg.y; b += 256) acc += C.tpart[b];
+  for (int b = rg.x + threadIdx.x; b < rg.y; b += kTempThreads) acc += C.tpart[b];
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
+  for (int off = kTempThreads / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
     __syncthreads();
   }

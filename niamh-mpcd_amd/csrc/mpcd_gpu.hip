@@ -55,6 +55,7 @@ struct mpcd_gpu_ctx {
   double *shift2 = nullptr, *wqs2 = nullptr, *wqb2 = nullptr;  // per step parity
   WallHot *wh2 = nullptr;                                         // per step parity
   double *box2 = nullptr;                                         // per step parity: wall-free boxes
+  double *d_celldraw = nullptr;  // SRD cell draws of the current step (fused path, internal generator)
   int32_t *hist2[2] = {nullptr, nullptr};
   State<double> sd[2]{};
   State<float> sf[2]{};
@@ -334,6 +335,10 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->wqb2 = dalloc<double>(c, 2 * nw3);
   c->wh2 = dalloc<WallHot>(c, 2 * std::max(1, woff));
   c->box2 = dalloc<double>(c, 2 * 6 * nrep);
+  if (c->fusable && fused_op(c->rtech) == OP_SRD) {
+    c->d_celldraw = dalloc<double>(c, 4 * (size_t)c->nctot);
+    ok = ok && c->d_celldraw;
+  }
   c->hist2[0] = dalloc<int32_t>(c, c->nctot + 1);
   c->hist2[1] = dalloc<int32_t>(c, c->nctot + 1);
   ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_tile_rng && c->d_sp &&
@@ -604,7 +609,7 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
     k_collide<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
   }
   ProfScope ps(c, F_TEMP);
-  k_temp_reduce<<<c->nrep, 256, 0, c->stream>>>(C, c->d_cblk_rng);
+  k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(C, c->d_cblk_rng);
   if (c->any_therm) k_scale<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk);
 }
 
@@ -679,6 +684,12 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
     if (do_next) launch_shift(c, c->step + 1, 0);
     scan_fixed(c, C.hist, C.start, c->nctot, 0);
   }
+  if (c->d_celldraw && !injected) {
+    // the fused kernel reads the SRD cell draws from the keyed array
+    ProfScope ps(c, F_COLLIDE);
+    k_celldraws<<<c->cblk.size(), kBlock, 0, c->stream>>>(C, c->d_cblk, D, c->d_celldraw);
+    D.cell = c->d_celldraw;
+  }
   {
     ProfScope ps(c, F_SCATTER);
     k_scatter<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C);
@@ -691,7 +702,7 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
   c->cur = 1 - c->cur;
   {
     ProfScope ps(c, F_TEMP);
-    k_temp_reduce<<<c->nrep, 256, 0, c->stream>>>(C, c->d_tile_rng);
+    k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(C, c->d_tile_rng);
   }
   if (!do_stream) {
     ProfScope ps(c, F_STREAM);

@@ -82,7 +82,11 @@ struct WallHot {
 // calcW_PLANE (bc.c:306) for an axis-aligned wall: the other axes add exact
 // zeros to the reference's sum, so the sign tests agree with wall_W()
 __device__ __forceinline__ double wall_W_hot(const WallHot &H, const double *q, double wq) {
-  const double qa = (H.axis == 0) ? q[0] : (H.axis == 1) ? q[1] : q[2];
+  // select values, not addresses: a dynamic index would move q to scratch
+  const double q0 = q[0], q1 = q[1], q2 = q[2];
+  double qa = q2;
+  if (H.axis == 0) qa = q0;
+  if (H.axis == 1) qa = q1;
   double Wv = H.A * (qa - wq);
   Wv -= H.R;
   if (H.inv) Wv *= -1.;
