@@ -135,7 +135,9 @@ static T *dalloc(mpcd_gpu_ctx *c, size_t n) {
   if (n == 0) n = 1;
   if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
   c->allocs.push_back(p);
-  if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
+  // zeroed on the library stream: the stream is non-blocking, so a memset on
+  // the null stream would not be ordered before the library's own kernels
+  if (hipMemsetAsync(p, 0, n * sizeof(T), c->stream) != hipSuccess) return nullptr;
   return (T *)p;
 }
 
@@ -396,17 +398,21 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   for (int w = 0; w < woff; w++)
     for (int j = 0; j < 3; j++) wq[w * 3 + j] = wq[nw3 + w * 3 + j] = walls[w].Q[j];
   bool cp = true;
-  cp = cp && hipMemcpy(c->d_rep, c->reps.data(), sizeof(RepDev) * nrep, hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->d_pblk, c->pblk.data(), sizeof(BlkDev) * c->pblk.size(), hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->d_cblk, c->cblk.data(), sizeof(BlkDev) * c->cblk.size(), hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->d_cblk_rng, c->cblk_rng.data(), sizeof(int2) * nrep, hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->d_tile_rng, c->tile_rng.data(), sizeof(int2) * nrep, hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->d_sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC, hipMemcpyHostToDevice) == hipSuccess;
-  if (woff) cp = cp && hipMemcpy(c->d_wall, walls.data(), sizeof(mpcd_gpu_wall) * woff, hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->wqs2, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice) == hipSuccess;
-  cp = cp && hipMemcpy(c->wqb2, wq.data(), sizeof(double) * wq.size(), hipMemcpyHostToDevice) == hipSuccess;
-  if (!cp || hipDeviceSynchronize() != hipSuccess) {
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    cp = cp && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+  };
+  up(c->d_rep, c->reps.data(), sizeof(RepDev) * nrep);
+  up(c->d_pblk, c->pblk.data(), sizeof(BlkDev) * c->pblk.size());
+  up(c->d_cblk, c->cblk.data(), sizeof(BlkDev) * c->cblk.size());
+  up(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size());
+  up(c->d_cblk_rng, c->cblk_rng.data(), sizeof(int2) * nrep);
+  up(c->d_tile_rng, c->tile_rng.data(), sizeof(int2) * nrep);
+  up(c->d_sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC);
+  if (woff) up(c->d_wall, walls.data(), sizeof(mpcd_gpu_wall) * woff);
+  up(c->wqs2, wq.data(), sizeof(double) * wq.size());
+  up(c->wqb2, wq.data(), sizeof(double) * wq.size());
+  // host tables are borrowed: wait for the copies (and the zeroing) here
+  if (!cp || hipStreamSynchronize(c->stream) != hipSuccess) {
     mpcd_gpu_destroy(c);
     return MPCD_EDEVICE;
   }
