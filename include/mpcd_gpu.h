@@ -158,9 +158,20 @@ mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *ctx, int64_t *bc_event_counts_out);
 /* one full step (bin + collide + stream); draws may be NULL (internal RNG) */
 mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *ctx, const mpcd_draws *draws, double *kbt_now_out);
 /* nsteps steps with the internal RNG and no host synchronisation; with
- * use_graph != 0 the step is captured once in a hipGraph and replayed */
+ * use_graph != 0 (fused path) pairs of steps are recorded once in a hipGraph
+ * and replayed, the step number being read from device memory */
 mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *ctx, int nsteps, int use_graph);
 mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *ctx);
+
+/* Batched-form tuning (no counterpart in the reference, which has no tiles):
+ * runs steps_per_trial real steps (plus one untimed step) for each candidate
+ * tile fill of the fused kernel, keeps the fastest and returns its ms per
+ * step (0 when the context uses the split path).  The trial steps advance the
+ * simulation like mpcd_gpu_run.  Invalidates a graph recorded by
+ * mpcd_gpu_run. */
+mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *ctx, int steps_per_trial, double *ms_per_step_out);
+/* the tile fill in use: mean cell-tile population / LDS capacity */
+double      mpcd_gpu_tile_fill(const mpcd_gpu_ctx *ctx);
 
 /* --- observables ------------------------------------------------------------ */
 /* per-cell populations, centre-of-mass velocities [3 per cell], scalar order

@@ -95,7 +95,8 @@ class DrawsC(C.Structure):
 EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_gpu_upload", "mpcd_gpu_download",
            "mpcd_gpu_bin", "mpcd_gpu_collide", "mpcd_gpu_stream", "mpcd_gpu_step", "mpcd_gpu_run",
            "mpcd_gpu_synchronize", "mpcd_gpu_get_cells", "mpcd_gpu_get_order", "mpcd_gpu_ncells",
-           "mpcd_gpu_lc_stride", "mpcd_gpu_profile", "mpcd_gpu_timing"]
+           "mpcd_gpu_lc_stride", "mpcd_gpu_profile", "mpcd_gpu_timing", "mpcd_gpu_autotune",
+           "mpcd_gpu_tile_fill"]
 
 _lib = None
 
@@ -127,6 +128,9 @@ def lib():
     L.mpcd_gpu_lc_stride.argtypes = [vp]
     L.mpcd_gpu_profile.argtypes = [vp, C.c_int]
     L.mpcd_gpu_timing.argtypes = [vp, dp, C.c_int]
+    L.mpcd_gpu_autotune.argtypes = [vp, C.c_int, dp]
+    L.mpcd_gpu_tile_fill.argtypes = [vp]
+    L.mpcd_gpu_tile_fill.restype = C.c_double
     _lib = L
     return L
 
@@ -280,6 +284,15 @@ class Context:
 
     def synchronize(self):
         self._check(self.L.mpcd_gpu_synchronize(self.h), "synchronize")
+
+    def autotune(self, steps_per_trial=5):
+        """pick the fused kernel's tile fill by timing real steps; returns ms per step"""
+        ms = C.c_double(0.0)
+        self._check(self.L.mpcd_gpu_autotune(self.h, int(steps_per_trial), C.byref(ms)), "autotune")
+        return ms.value
+
+    def tile_fill(self):
+        return self.L.mpcd_gpu_tile_fill(self.h)
 
     def cells(self, r=0):
         nc = self.ncells[r]

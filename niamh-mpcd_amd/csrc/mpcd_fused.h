@@ -244,6 +244,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
     const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr,
     const double *__restrict__ shift_cur, const double *__restrict__ wq_b, const double *__restrict__ shift_next,
     int32_t *hist_next, int do_next, int do_stream) {
+  Dr.resolve();
   // the uniform tables come in through restrict-qualified arguments so that
   // their loads can be scalar
   constexpr int kTileCap = tile_cap(sizeof(Real) == 8, OP);
@@ -587,6 +588,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
 // draws (cell [4 per cell]: ARBAXIS 3D unit axis in [0..2], otherwise a
 // uniform in [0]; the rotation-sign uniform in [3]), one lane per cell
 __global__ void k_celldraws(Ctx C, const BlkDev *__restrict__ cblk, Draws D, double *out) {
+  D.resolve();
   const BlkDev B = cblk[blockIdx.x];
   const int c = B.beg + threadIdx.x;
   if (c >= B.end) return;
@@ -611,6 +613,7 @@ __global__ void k_stream_next(Ctx C, State<Real> S, const BlkDev *__restrict__ b
                               const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr,
                               const double *__restrict__ wq_b, const double *__restrict__ shift_next,
                               int32_t *hist_next, int do_next) {
+  Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
   if (t >= B.end) return;

@@ -43,6 +43,13 @@ struct mpcd_gpu_ctx {
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // mpcd_gpu_run graphs: two fused steps captured from an even step with the
+  // ping-pong buffer `graph_cur` current; the step number is read from d_step
+  bool capturing = false;
+  uint32_t cap_base = 0;
+  int graph_cur = -1;
+  uint32_t *d_step = nullptr;
+  double tile_fill = 0.45;  // mean tile population as a fraction of the LDS capacity (autotuned)
 
   std::vector<void *> allocs;
   RepDev *d_rep = nullptr;
@@ -102,7 +109,7 @@ struct ProfScope {
   int fam;
   hipEvent_t a = nullptr;
   ProfScope(mpcd_gpu_ctx *c_, int f) : c(c_), fam(f) {
-    if (c->prof && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
+    if (c->prof && !c->capturing && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
   }
   ~ProfScope() {
     if (!a) return;
@@ -194,11 +201,26 @@ static double mean_ppc(const RepDev &R) {
 
 static int fused_op(int rtech) { return (rtech == MPCD_MPCAT) ? OP_AT : (rtech == MPCD_RAT) ? OP_RAT : OP_SRD; }
 
-// cells per fused tile for a replica: tiles hold a little under half the LDS
-// capacity on average, so a chunk rarely needs splitting
-static int tile_cells(const RepDev &R, int cap) {
-  int tc = (int)lround(0.45 * cap / std::max(mean_ppc(R), 1.0));
+// cells per fused tile for a replica: the mean tile population is `fill`
+// times the LDS capacity (0.45 by default), so a chunk rarely needs splitting
+static int tile_cells(const RepDev &R, int cap, double fill) {
+  int tc = (int)lround(fill * cap / std::max(mean_ppc(R), 1.0));
   return std::max(1, std::min(kTileCells, tc));
+}
+
+// the fused kernel's tile table for a given fill fraction (host side)
+static void build_tiles(mpcd_gpu_ctx *c, double fill) {
+  c->tiles.clear();
+  c->tile_rng.clear();
+  const int cap = tile_cap(c->precision == MPCD_FP64, fused_op(c->rtech));
+  for (int r = 0; r < c->nrep; r++) {
+    const RepDev &R = c->reps[r];
+    const int t0 = (int)c->tiles.size(), tc = tile_cells(R, cap, fill);
+    for (int b = 0; b < R.ncell; b += tc)
+      c->tiles.push_back(TileDev{r, R.cell_off + b, R.cell_off + std::min(b + tc, R.ncell), 0});
+    c->tile_rng.push_back(make_int2(t0, (int)c->tiles.size()));
+  }
+  c->tile_fill = fill;
 }
 
 extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, const int64_t *n_particles,
@@ -300,20 +322,18 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
     if (p.tstech != MPCD_NOTHERM || (p.rtech >= 7 && p.rtech <= 17)) c->any_therm = true;
     if (p.lc == MPCD_LCG) c->any_lcg = true;
     c->reps.push_back(R);
-    int cb0 = (int)c->cblk.size(), t0 = (int)c->tiles.size();
+    int cb0 = (int)c->cblk.size();
     for (int b = 0; b < R.npart; b += kBlock)
       c->pblk.push_back(BlkDev{r, poff + b, poff + std::min(b + kBlock, R.npart), 0});
     for (int b = 0; b < R.ncell; b += kBlock)
       c->cblk.push_back(BlkDev{r, coff + b, coff + std::min(b + kBlock, R.ncell), 0});
-    int tc = tile_cells(R, tile_cap(p.precision == MPCD_FP64, fused_op(p.rtech)));
-    for (int b = 0; b < R.ncell; b += tc) c->tiles.push_back(TileDev{r, coff + b, coff + std::min(b + tc, R.ncell), 0});
     c->cblk_rng.push_back(make_int2(cb0, (int)c->cblk.size()));
-    c->tile_rng.push_back(make_int2(t0, (int)c->tiles.size()));
     poff += R.npart;
     coff += R.ncell;
     woff += p.nwall;
   }
   c->nwtot = woff;
+  build_tiles(c, c->tile_fill);
   c->uploaded.assign(nrep, 0);
   c->lc_stride = anneal_stride(params[0]);
   // the fused kernel stages a cell in LDS: dense systems take the split path
@@ -327,7 +347,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->d_rep = dalloc<RepDev>(c, nrep);
   c->d_pblk = dalloc<BlkDev>(c, c->pblk.size());
   c->d_cblk = dalloc<BlkDev>(c, c->cblk.size());
-  c->d_tiles = dalloc<TileDev>(c, c->tiles.size());
+  c->d_tiles = dalloc<TileDev>(c, c->nctot);  // room for the finest tiling
   c->d_cblk_rng = dalloc<int2>(c, nrep);
   c->d_tile_rng = dalloc<int2>(c, nrep);
   c->d_sp = dalloc<mpcd_gpu_species>(c, MPCD_MAXSPEC);
@@ -337,6 +357,8 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->wqb2 = dalloc<double>(c, 2 * nw3);
   c->wh2 = dalloc<WallHot>(c, 2 * std::max(1, woff));
   c->box2 = dalloc<double>(c, 2 * 6 * nrep);
+  c->d_step = dalloc<uint32_t>(c, 1);
+  ok = ok && c->d_step;
   if (c->fusable && fused_op(c->rtech) == OP_SRD) {
     c->d_celldraw = dalloc<double>(c, 4 * (size_t)c->nctot);
     ok = ok && c->d_celldraw;
@@ -364,7 +386,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   C.cell.dir = dalloc<double>(c, 3 * (size_t)c->nctot);
   C.cell.E = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 9 * (size_t)c->nctot) : nullptr;
   C.T = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 3 * (size_t)c->ntot) : nullptr;
-  C.tpart = dalloc<double>(c, std::max(c->cblk.size(), c->tiles.size()));
+  C.tpart = dalloc<double>(c, std::max<size_t>(c->cblk.size(), c->nctot));
   C.kbtnow = dalloc<double>(c, nrep);
   C.gscale = dalloc<double>(c, nrep);
   C.sg = dalloc<double>(c, nrep);
@@ -456,6 +478,10 @@ static Draws make_draws(mpcd_gpu_ctx *c, int injected, uint32_t step) {
   D.part = c->d_dpart;
   D.lc = c->d_dlc;
   D.lc_stride = c->dlc_stride;
+  if (c->capturing) {
+    D.step_dev = c->d_step;
+    D.step_off = step - c->cap_base;
+  }
   return D;
 }
 
@@ -939,17 +965,120 @@ extern "C" mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *c, const mpcd_draws *draws, d
   return check_flags(c);
 }
 
+// records two fused steps (from an even step) and the advance of the device
+// step counter into c->gexec; nothing runs while recording
+static mpcd_status capture_pair(mpcd_gpu_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  const uint32_t step0 = c->step;
+  const int cur0 = c->cur;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  c->cap_base = step0;
+  for (int k = 0; k < 2; k++) {
+    if (c->precision == MPCD_FP64) run_step_fused<double>(c, 0);
+    else run_step_fused<float>(c, 0);
+  }
+  k_addu32<<<1, 1, 0, c->stream>>>(c->d_step, 2u);
+  c->capturing = false;
+  hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+  // the recorded steps have not run: restore the host-side step state
+  c->step = step0;
+  c->cur = cur0;
+  if (e != hipSuccess) return fail(c, MPCD_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->graph_cur = cur0;
+  return MPCD_OK;
+}
+
 extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) {
   if (!c || nsteps < 0) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
-  (void)use_graph;
-  for (int k = 0; k < nsteps; k++) {
+  auto eager = [&]() {
     if (c->precision == MPCD_FP64) run_step<double>(c, 0);
     else run_step<float>(c, 0);
+  };
+  if (use_graph && c->fusable) {
+    // graphs start from an even step whose histogram the previous step built
+    while (nsteps > 0 && (!c->hist_ready || (c->step & 1))) {
+      eager();
+      nsteps--;
+    }
+    const int pairs = nsteps / 2;
+    if (pairs > 0) {
+      if (!c->gexec || c->graph_cur != c->cur) {
+        st = check_launch(c);
+        if (st) return st;
+        st = capture_pair(c);
+        if (st) return st;
+      }
+      k_setu32<<<1, 1, 0, c->stream>>>(c->d_step, c->step);
+      for (int k = 0; k < pairs; k++) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      c->step += 2u * pairs;
+      nsteps -= 2 * pairs;
+    }
   }
+  for (int k = 0; k < nsteps; k++) eager();
   return check_launch(c);
 }
+
+extern "C" mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *c, int steps_per_trial, double *ms_per_step_out) {
+  if (!c || steps_per_trial < 1) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  if (!c->fusable) {
+    if (ms_per_step_out) *ms_per_step_out = 0.0;
+    return MPCD_OK;  // the split path has no tile parameter
+  }
+  const double fills[] = {0.25, 0.35, 0.45, 0.6, 0.8};
+  double best = 1e300, best_fill = c->tile_fill;
+  hipEvent_t a, b;
+  HIPCHK(c, hipEventCreate(&a));
+  HIPCHK(c, hipEventCreate(&b));
+  for (double f : fills) {
+    build_tiles(c, f);
+    HIPCHK(c, hipMemcpyAsync(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_tile_rng, c->tile_rng.data(), sizeof(int2) * c->nrep, hipMemcpyHostToDevice,
+                             c->stream));
+    if (c->precision == MPCD_FP64) run_step<double>(c, 0);  // untimed: settles the histogram state
+    else run_step<float>(c, 0);
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    for (int k = 0; k < steps_per_trial; k++) {
+      if (c->precision == MPCD_FP64) run_step<double>(c, 0);
+      else run_step<float>(c, 0);
+    }
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+    if (ms / steps_per_trial < best) {
+      best = ms / steps_per_trial;
+      best_fill = f;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  build_tiles(c, best_fill);
+  HIPCHK(c, hipMemcpyAsync(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tile_rng, c->tile_rng.data(), sizeof(int2) * c->nrep, hipMemcpyHostToDevice,
+                           c->stream));
+  // the tile count is part of a recorded graph
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  if (ms_per_step_out) *ms_per_step_out = best;
+  st = check_launch(c);
+  if (st) return st;
+  return check_flags(c);
+}
+
+extern "C" double mpcd_gpu_tile_fill(const mpcd_gpu_ctx *c) { return c ? c->tile_fill : 0.0; }
 
 extern "C" mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *c) {
   if (!c) return MPCD_EINVAL;

@@ -137,6 +137,7 @@ __device__ __forceinline__ void shift_back(const RepDev &R, const double *s, dou
 // ---------------------------------------------------------------------------
 __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, double *shift_out, double *wqs_out,
                         double *wqb_out, WallHot *wh_out, double *box_out) {
+  D.resolve();
   // reads the previous step's shifted-back wall positions and writes this
   // step's shift and wall positions, so a repeated launch is idempotent
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,6 +265,9 @@ __global__ void k_scan_add(int32_t *out, const int32_t *boff, int n) {
     if (i + k < n) out[i + k] += add;
 }
 
+__global__ void k_setu32(uint32_t *a, uint32_t v) { *a = v; }
+__global__ void k_addu32(uint32_t *a, uint32_t v) { *a += v; }
+
 __global__ void k_fill(int32_t *a, int32_t v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -304,6 +308,7 @@ __global__ void k_sortcells(Ctx C, int32_t *cst_new) {
 // ---------------------------------------------------------------------------
 template <typename Real>
 __global__ void k_gather(Ctx C, State<Real> S, State<Real> D, const BlkDev *blk, int shifted, int lc, Draws Dr) {
+  Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
   if (t >= B.end) return;
@@ -465,6 +470,7 @@ __device__ __forceinline__ void ori_bc(double *u, const mpcd_gpu_wall &W, const 
 
 template <typename Real>
 __global__ void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+  Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   int c = B.beg + threadIdx.x;
   if (c >= B.end) return;
@@ -750,6 +756,7 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
 
 template <typename Real>
 __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+  Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   int c = B.beg + threadIdx.x;
   if (c >= B.end) return;
@@ -820,6 +827,7 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
 // ---------------------------------------------------------------------------
 template <typename Real>
 __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+  Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   int c = B.beg + threadIdx.x;
   const RepDev &R = C.rep[B.rep];
@@ -1070,6 +1078,7 @@ __global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
 // ---------------------------------------------------------------------------
 template <typename Real>
 __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
+  Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
   if (t >= B.end) return;

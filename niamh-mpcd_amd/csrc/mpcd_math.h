@@ -259,6 +259,14 @@ struct Draws {
   int injected;
   const double *shift, *cell, *ghost, *part, *lc;
   int lc_stride;
+  // graph replays: the step number lives in device memory (step_dev) and a
+  // launch is step_off steps after it; eager launches carry rng.step
+  const uint32_t *step_dev;
+  uint32_t step_off;
+
+  __device__ void resolve() {
+    if (step_dev) rng.step = *step_dev + step_off;
+  }
 
   MPCD_HD double cell_u(uint32_t rep, uint32_t c, int k) const {
     return injected ? cell[(size_t)c * 4 + k] : rng.uniform(rep, P_CELL, c, k);
