@@ -56,7 +56,9 @@ ROTANG_130 = 2.2689280275926285
 
 
 def package():
-    spec = importlib.util.spec_from_file_location("niamh_mpcd_amd", os.path.join(ROOT, "niamh-mpcd_amd", "__init__.py"))
+    spec = importlib.util.spec_from_file_location(
+        "niamh_mpcd_amd", os.path.join(ROOT, "niamh-mpcd_amd", "__init__.py"),
+        submodule_search_locations=[os.path.join(ROOT, "niamh-mpcd_amd")])
     m = importlib.util.module_from_spec(spec)
     sys.modules["niamh_mpcd_amd"] = m
     spec.loader.exec_module(m)
@@ -90,6 +92,17 @@ def initial_state(L, ppc, seed):
     V = rng.normal(size=(N, 3))
     V -= V.mean(axis=0)
     return Q, V
+
+
+def graph_run(ctx, steps, n_total):
+    """the same steps replayed as hipGraphs (pairs of fused steps), no profiling"""
+    ctx.run(2, use_graph=True)  # records the graph
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(steps, use_graph=True)
+    ctx.synchronize()
+    sec = time.perf_counter() - t0
+    return dict(value=n_total * steps / sec, ms_per_step=1e3 * sec / steps)
 
 
 def timed_run(M, ctx, steps, warmup, n_total):
@@ -199,6 +212,7 @@ def main():
     ap.add_argument("--ppc", type=int, default=10)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-validate", action="store_true", help="skip the FP32-vs-FP64 observable report")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
@@ -221,9 +235,12 @@ def main():
         for r in range(B):
             Q, V = initial_state(L, args.ppc, seed=100 + r)
             ctx.upload(r, Q, V)
-        print(f"[bench] fp32 batched: {B} x {n1} particles, warmup {args.warmup}, steps {args.steps}",
-              file=sys.stderr, flush=True)
+        tuned_ms = ctx.autotune(3)
+        print(f"[bench] fp32 batched: {B} x {n1} particles, tile fill {ctx.tile_fill():.2f} ({tuned_ms:.3f} ms/step "
+              f"in the autotuner), warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
         r32 = timed_run(M, ctx, args.steps, args.warmup, B * n1)
+        r32["graph"] = graph_run(ctx, args.steps, B * n1)
+        r32["tile_fill"] = ctx.tile_fill()
         result["fp32"] = (r32, B * n1, B)
         ctx.close()
     if args.only in ("fp64", "both"):
@@ -232,11 +249,20 @@ def main():
         Q, V = initial_state(L, args.ppc, seed=7)
         ctx.upload(0, Q, V)
         k64 = args.fp64_steps or args.steps
-        print(f"[bench] fp64 plain: {n1} particles, warmup {args.warmup}, steps {k64}", file=sys.stderr, flush=True)
+        ctx.autotune(3)
+        print(f"[bench] fp64 plain: {n1} particles, tile fill {ctx.tile_fill():.2f}, warmup {args.warmup}, "
+              f"steps {k64}", file=sys.stderr, flush=True)
         r64 = timed_run(M, ctx, k64, args.warmup, n1)
+        r64["graph"] = graph_run(ctx, k64, n1)
+        r64["tile_fill"] = ctx.tile_fill()
         result["fp64"] = (r64, n1, 1)
         ctx.close()
 
+    validation = None
+    if not args.no_validate and args.only == "both":
+        import importlib
+        print("[bench] FP32-state observables vs FP64", file=sys.stderr, flush=True)
+        validation = importlib.import_module("niamh_mpcd_amd.validate").fp32_report(M, quick=True)
     cpu = None if args.no_cpu else cpu_baseline()
     head_key = "fp32" if "fp32" in result else "fp64"
     r, n, B = result[head_key]
@@ -261,11 +287,15 @@ def main():
         "roofline": roofline(r["timing"], n, head_key),
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in r["timing"].items() if v[1]},
+        # the same workload replayed as hipGraphs of two fused steps (no per-kernel event timing)
+        "hipgraph": r["graph"],
+        "tile_fill": r["tile_fill"],
         "step_hbm_frac": r["value"] * B_ALG[head_key] / 1e9 / HBM_PEAK_GBS,
+        "fp32_validation": validation,
     }
     if "fp64" in result and head_key != "fp64":
         r64, n64, _ = result["fp64"]
-        line["fp64_plain"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
+        line["fp64_plain"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"], "hipgraph": r64["graph"],
                               "roofline": roofline(r64["timing"], n64, "fp64"),
                               "step_hbm_frac": r64["value"] * B_ALG["fp64"] / 1e9 / HBM_PEAK_GBS}
     print(json.dumps(line))

@@ -14,7 +14,9 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 def package():
     if "niamh_mpcd_amd" in sys.modules:
         return sys.modules["niamh_mpcd_amd"]
-    spec = importlib.util.spec_from_file_location("niamh_mpcd_amd", os.path.join(ROOT, "niamh-mpcd_amd", "__init__.py"))
+    spec = importlib.util.spec_from_file_location(
+        "niamh_mpcd_amd", os.path.join(ROOT, "niamh-mpcd_amd", "__init__.py"),
+        submodule_search_locations=[os.path.join(ROOT, "niamh-mpcd_amd")])
     m = importlib.util.module_from_spec(spec)
     sys.modules["niamh_mpcd_amd"] = m
     spec.loader.exec_module(m)
