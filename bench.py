@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true", help="skip the FP32-vs-FP64 observable report")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
@@ -263,6 +265,21 @@ def main():
         import importlib
         print("[bench] FP32-state observables vs FP64", file=sys.stderr, flush=True)
         validation = importlib.import_module("niamh_mpcd_amd.validate").fp32_report(M, quick=True)
+    ensemble = None
+    if not args.no_ensemble and args.only == "both":
+        import importlib
+        print("[bench] 64-replica MPC-AT+a channel ensemble", file=sys.stderr, flush=True)
+        e = importlib.import_module("niamh_mpcd_amd.batched").run(M, M.FP32)
+        by_ppc = {}
+        for x in e["per_replica"]:
+            by_ppc.setdefault(x["ppc"], []).append(x["nu"])
+        ensemble = {k: e[k] for k in ("particles", "replicas", "value", "ms_per_step", "steps", "samples")}
+        ensemble.update(
+            workload="64 replicas of 3D MPC-AT+a channels: L^3 with L ~ U{16..48}, ppc ~ U{5..20}, dt 0.05, kT 1, "
+                     "bounce-back phantom walls in z, g = 1e-3 along x, FP32 state, autotuned tiles, hipGraph replays",
+            unit="particle-steps/s",
+            viscosity_by_ppc={str(k): dict(mean=float(np.mean(v)), n=len(v)) for k, v in sorted(by_ppc.items())},
+            per_replica=[[x["L"], x["ppc"], round(x["nu"], 4)] for x in e["per_replica"]])
     cpu = None if args.no_cpu else cpu_baseline()
     head_key = "fp32" if "fp32" in result else "fp64"
     r, n, B = result[head_key]
@@ -292,6 +309,7 @@ def main():
         "tile_fill": r["tile_fill"],
         "step_hbm_frac": r["value"] * B_ALG[head_key] / 1e9 / HBM_PEAK_GBS,
         "fp32_validation": validation,
+        "ensemble_mpcat_a": ensemble,
     }
     if "fp64" in result and head_key != "fp64":
         r64, n64, _ = result["fp64"]
