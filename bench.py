@@ -189,6 +189,57 @@ def cpu_baseline(L=(64, 64, 64), ppc=10, steps=3):
                 sample=f"3D SRD {L[0]}x{L[1]}x{L[2]} cells x {ppc} ppc ({N} particles), {steps} steps, one core")
 
 
+def adjoint_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, angle=2.0, h=1e-9):
+    """gradient of the time-averaged centreline velocity of a 2D SRD channel
+    (bounce-back phantom walls in y, body force g along x) with respect to g
+    and the rotation angle: adjoint (checkpoint every 20 steps) against central
+    finite differences with common random numbers"""
+    import importlib
+    H = importlib.import_module("niamh_mpcd_amd.adjoint")
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    rng = np.random.default_rng(3)
+    N = Lx * Ly * ppc
+    Q = np.zeros((N, 3))
+    Q[:, 0] = rng.uniform(0, Lx, N)
+    Q[:, 1] = rng.uniform(0, Ly, N)
+    V = np.zeros((N, 3))
+    V[:, :2] = rng.normal(size=(N, 2))
+    V -= V.mean(axis=0)
+
+    def cfg(gg, aa):
+        sp = dict(mass=1.0, vdist=4)
+        ws = [dict(A=[1.0, 0, 0], R=0.0, DN=float(Lx)), dict(A=[-1.0, 0, 0], R=-float(Lx), DN=float(Lx)),
+              dict(A=[0, 1.0, 0], R=0.0, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1),
+              dict(A=[0, -1.0, 0], R=-float(Ly), DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1)]
+        return dict(dim=2, L=[Lx, Ly, 1], dt=0.1, kbt=1.0, rotang=aa, tau=0.5, grav=[gg, 0.0, 0.0],
+                    avv=[0.0, 0.0, 0.0], gndnst=float(ppc), rtech=0, lc=0, tstech=0, galinv=1, species=[sp],
+                    walls=ws)
+
+    band = (Ly // 2 - 1, Ly // 2 + 1)
+    t0 = time.perf_counter()
+    run = H.AdjointRun(M, cfg(g, angle), Q, V, K, 11, band, ckpt=20)
+    J = run.forward()
+    t1 = time.perf_counter()
+    dg, da = run.backward()
+    t2 = time.perf_counter()
+    run.close()
+
+    def J_of(gg, aa):
+        r = H.AdjointRun(M, cfg(gg, aa), Q, V, K, 11, band, ckpt=20)
+        j = r.forward()
+        r.close()
+        return j
+
+    fd_g = (J_of(g + h, angle) - J_of(g - h, angle)) / (2 * h)
+    fd_a = (J_of(g, angle + h) - J_of(g, angle - h)) / (2 * h)
+    return dict(workload=f"2D SRD channel {Lx}x{Ly} x {ppc} ppc, phantom bounce-back walls in y, g={g} along x, "
+                         f"rotation angle {angle}, K={K} steps, checkpoint every 20, FP64; "
+                         f"J = time-averaged mean v_x of the cells {band[0]}..{band[1] - 1} across the channel",
+                J=J, dJ_dg=float(dg[0]), dJ_dangle=da, fd_dJ_dg=fd_g, fd_dJ_dangle=fd_a, fd_h=h,
+                rel_err_g=abs(dg[0] - fd_g) / abs(fd_g), rel_err_angle=abs(da - fd_a) / abs(fd_a),
+                forward_s=t1 - t0, backward_s=t2 - t1)
+
+
 def copy_probe(mib):
     """a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
     calibrates the counters of the step kernels"""
@@ -213,6 +264,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true", help="skip the FP32-vs-FP64 observable report")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the pathwise-adjoint check (2D SRD channel, gradient vs finite differences)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
@@ -280,6 +333,10 @@ def main():
             unit="particle-steps/s",
             viscosity_by_ppc={str(k): dict(mean=float(np.mean(v)), n=len(v)) for k, v in sorted(by_ppc.items())},
             per_replica=[[x["L"], x["ppc"], round(x["nu"], 4)] for x in e["per_replica"]])
+    adjoint = None
+    if not args.no_adjoint and args.only == "both":
+        print("[bench] pathwise adjoint vs finite differences", file=sys.stderr, flush=True)
+        adjoint = adjoint_check(M)
     cpu = None if args.no_cpu else cpu_baseline()
     head_key = "fp32" if "fp32" in result else "fp64"
     r, n, B = result[head_key]
@@ -310,6 +367,7 @@ def main():
         "step_hbm_frac": r["value"] * B_ALG[head_key] / 1e9 / HBM_PEAK_GBS,
         "fp32_validation": validation,
         "ensemble_mpcat_a": ensemble,
+        "adjoint": adjoint,
     }
     if "fp64" in result and head_key != "fp64":
         r64, n64, _ = result["fp64"]

@@ -173,6 +173,28 @@ mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *ctx, int steps_per_trial, double *ms
 /* the tile fill in use: mean cell-tile population / LDS capacity */
 double      mpcd_gpu_tile_fill(const mpcd_gpu_ctx *ctx);
 
+/* --- checkpoints and the adjoint ------------------------------------------ */
+/* Save / restore everything that carries over between steps (particle state in
+ * list order, pending binning, wall positions, step number) in device memory
+ * slot `slot`; a restored run repeats the saved one bit for bit.  No
+ * counterpart in the reference (its checkpoints are files, pout.c:2325). */
+mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *ctx, int slot);
+mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *ctx, int slot);
+/* Pathwise adjoint of streaming plus collision (SRD and MPC-AT, tsTech 0,
+ * resting walls) with cell assignments, draws and wall events held fixed.
+ * mpcd_gpu_record(ctx, k) starts a tape of the next k steps (the steps run on
+ * the unfused path while recording; k = 0 stops recording).
+ * mpcd_gpu_step_backward pops the last taped step: adj_in [3 * n_total]
+ * (particle-index order, replicas concatenated; may be NULL) is dJ/dv of the
+ * observable at the end of that step and is added to the carried lambda
+ * = dJ/dv; adj_out [3 * n_total] receives lambda at the start of the step
+ * (may be NULL); dJ_dtheta [4 * nrep] receives the running sums of
+ * dJ/dGRAV[0..2] and dJ/dRA per replica since mpcd_gpu_adjoint_reset (may be
+ * NULL).  The north star's adjoint entry point; the reference has none. */
+mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *ctx, int max_steps);
+mpcd_status mpcd_gpu_adjoint_reset(mpcd_gpu_ctx *ctx);
+mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *ctx, const double *adj_in, double *adj_out, double *dJ_dtheta);
+
 /* --- observables ------------------------------------------------------------ */
 /* per-cell populations, centre-of-mass velocities [3 per cell], scalar order
  * parameter and director [3 per cell] of replica `replica` (localPROP

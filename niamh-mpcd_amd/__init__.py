@@ -96,7 +96,8 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_bin", "mpcd_gpu_collide", "mpcd_gpu_stream", "mpcd_gpu_step", "mpcd_gpu_run",
            "mpcd_gpu_synchronize", "mpcd_gpu_get_cells", "mpcd_gpu_get_order", "mpcd_gpu_ncells",
            "mpcd_gpu_lc_stride", "mpcd_gpu_profile", "mpcd_gpu_timing", "mpcd_gpu_autotune",
-           "mpcd_gpu_tile_fill"]
+           "mpcd_gpu_tile_fill", "mpcd_gpu_checkpoint", "mpcd_gpu_restore", "mpcd_gpu_record",
+           "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward"]
 
 _lib = None
 
@@ -131,6 +132,11 @@ def lib():
     L.mpcd_gpu_autotune.argtypes = [vp, C.c_int, dp]
     L.mpcd_gpu_tile_fill.argtypes = [vp]
     L.mpcd_gpu_tile_fill.restype = C.c_double
+    L.mpcd_gpu_checkpoint.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_restore.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_record.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_adjoint_reset.argtypes = [vp]
+    L.mpcd_gpu_step_backward.argtypes = [vp, dp, dp, dp]
     _lib = L
     return L
 
@@ -293,6 +299,30 @@ class Context:
 
     def tile_fill(self):
         return self.L.mpcd_gpu_tile_fill(self.h)
+
+    # --- checkpoints and the adjoint -------------------------------------------------------------------
+    def checkpoint(self, slot):
+        self._check(self.L.mpcd_gpu_checkpoint(self.h, int(slot)), "checkpoint")
+
+    def restore(self, slot):
+        self._check(self.L.mpcd_gpu_restore(self.h, int(slot)), "restore")
+
+    def record(self, max_steps):
+        self._check(self.L.mpcd_gpu_record(self.h, int(max_steps)), "record")
+
+    def adjoint_reset(self):
+        self._check(self.L.mpcd_gpu_adjoint_reset(self.h), "adjoint_reset")
+
+    def step_backward(self, adj_in=None, want_lambda=False):
+        """pops one taped step; returns (lambda or None, dJ/dtheta [nrep, 4])"""
+        n = int(self.n.sum())
+        a = None if adj_in is None else np.ascontiguousarray(adj_in, np.float64).reshape(-1)
+        if a is not None and a.size != 3 * n:
+            raise ValueError("adj_in must hold 3 values per particle")
+        lam = np.zeros(3 * n) if want_lambda else None
+        th = np.zeros(4 * self.nrep)
+        self._check(self.L.mpcd_gpu_step_backward(self.h, _dp(a), _dp(lam), _dp(th)), "step_backward")
+        return (None if lam is None else lam.reshape(n, 3)), th.reshape(self.nrep, 4)
 
     def cells(self, r=0):
         nc = self.ncells[r]

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "mpcd_adjoint.h"
 #include "mpcd_fused.h"
 #include "mpcd_kernels.h"
 
@@ -50,6 +51,22 @@ struct mpcd_gpu_ctx {
   int graph_cur = -1;
   uint32_t *d_step = nullptr;
   double tile_fill = 0.45;  // mean tile population as a fraction of the LDS capacity (autotuned)
+  // adjoint: a tape of recorded steps (split path), lambda = dJ/dv per particle
+  bool recording = false;
+  int tape_cap = 0, tape_len = 0;
+  int32_t *tp_id = nullptr, *tp_sp = nullptr, *tp_start = nullptr;
+  double *tp_v1 = nullptr, *tp_vcm = nullptr, *tp_a = nullptr, *tp_mass = nullptr, *tp_M = nullptr, *tp_dM = nullptr;
+  float *tp_wmul = nullptr;
+  double *lam = nullptr, *lam_s = nullptr, *lam1_s = nullptr, *adj_dev = nullptr, *dtheta = nullptr;
+  // device checkpoints: (destination, source, bytes) copies plus the host step state
+  struct Checkpoint {
+    std::vector<void *> bufs;
+    uint32_t step;
+    int cur;
+    bool pending, hist_ready, need_init;
+    int cu_src;
+  };
+  std::vector<Checkpoint> ckpts;
 
   std::vector<void *> allocs;
   RepDev *d_rep = nullptr;
@@ -485,6 +502,22 @@ static Draws make_draws(mpcd_gpu_ctx *c, int injected, uint32_t step) {
   return D;
 }
 
+static TapeDev tape_at(mpcd_gpu_ctx *c, int k) {
+  TapeDev T;
+  const size_t n = c->ntot, nc = c->nctot;
+  T.id = c->tp_id + k * n;
+  T.sp = c->tp_sp + k * n;
+  T.start = c->tp_start + k * (nc + 1);
+  T.v1 = c->tp_v1 + k * 3 * n;
+  T.vcm = c->tp_vcm + k * 3 * nc;
+  T.a = c->tp_a + k * nc;
+  T.mass = c->tp_mass + k * nc;
+  T.M = c->tp_M + k * 9 * nc;
+  T.dM = c->tp_dM + k * 9 * nc;
+  T.wmul = c->tp_wmul + k * 3 * n;
+  return T;
+}
+
 // the kernels' view of the context for the step with parity `par`
 static Ctx view(mpcd_gpu_ctx *c, int par) {
   Ctx C = c->C;
@@ -628,6 +661,8 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
     ProfScope ps(c, F_GHOST);
     k_ghost<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
   }
+  if (c->recording && c->tape_len < c->tape_cap)
+    k_tape_cells<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, tape_at(c, c->tape_len));
   if (c->lc != MPCD_ISOF) {
     ProfScope ps(c, F_LC);
     if (c->any_lcg) {
@@ -651,7 +686,10 @@ static void run_stream(mpcd_gpu_ctx *c, int injected) {
   Ctx C = view(c, c->step & 1);
   Draws D = make_draws(c, injected, c->step);
   ProfScope ps(c, F_STREAM);
-  k_stream<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D);
+  const bool tape = c->recording && c->tape_len < c->tape_cap;
+  k_stream<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D,
+                                                             tape ? tape_at(c, c->tape_len).wmul : nullptr);
+  if (tape) c->tape_len++;
   c->pending = true;
   c->cu_src = 0;
   c->hist_ready = false;
@@ -757,7 +795,7 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
 
 template <typename Real>
 static void run_step(mpcd_gpu_ctx *c, int injected) {
-  if (c->fusable) {
+  if (c->fusable && !c->recording) {
     run_step_fused<Real>(c, injected);
     return;
   }
@@ -1001,7 +1039,7 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
     if (c->precision == MPCD_FP64) run_step<double>(c, 0);
     else run_step<float>(c, 0);
   };
-  if (use_graph && c->fusable) {
+  if (use_graph && c->fusable && !c->recording) {
     // graphs start from an even step whose histogram the previous step built
     while (nsteps > 0 && (!c->hist_ready || (c->step & 1))) {
       eager();
@@ -1079,6 +1117,146 @@ extern "C" mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *c, int steps_per_trial, d
 }
 
 extern "C" double mpcd_gpu_tile_fill(const mpcd_gpu_ctx *c) { return c ? c->tile_fill : 0.0; }
+
+// ---------------------------------------------------------------------------
+// checkpoints and the adjoint
+// ---------------------------------------------------------------------------
+// every device buffer whose contents carry over from one step to the next
+static void state_buffers(mpcd_gpu_ctx *c, std::vector<std::pair<void *, size_t>> &v) {
+  const size_t n = c->ntot, nc = c->nctot, nw3 = 3 * std::max(1, c->nwtot);
+  const size_t rs = (c->precision == MPCD_FP64) ? sizeof(double) : sizeof(float);
+  const bool lc = c->lc != MPCD_ISOF;
+  auto add_state = [&](auto &S) {
+    v.push_back({S.x, n * rs}); v.push_back({S.y, n * rs}); v.push_back({S.z, n * rs});
+    v.push_back({S.vx, n * rs}); v.push_back({S.vy, n * rs}); v.push_back({S.vz, n * rs});
+    if (lc) { v.push_back({S.ux, n * rs}); v.push_back({S.uy, n * rs}); v.push_back({S.uz, n * rs}); }
+    v.push_back({S.id, n * 4}); v.push_back({S.sp, n * 4}); v.push_back({S.cst, n * 4});
+  };
+  if (c->precision == MPCD_FP64) add_state(c->sd[c->cur]);
+  else add_state(c->sf[c->cur]);
+  v.push_back({c->C.key, n * 8}); v.push_back({c->C.cnew, n * 4}); v.push_back({c->C.loc, n * 4});
+  v.push_back({c->C.cu, n * 4});
+  v.push_back({c->hist2[0], (nc + 1) * 4}); v.push_back({c->hist2[1], (nc + 1) * 4});
+  v.push_back({c->shift2, 2 * 3 * c->nrep * sizeof(double)});
+  v.push_back({c->wqs2, 2 * nw3 * sizeof(double)}); v.push_back({c->wqb2, 2 * nw3 * sizeof(double)});
+  v.push_back({c->wh2, 2 * std::max(1, c->nwtot) * sizeof(WallHot)});
+  v.push_back({c->box2, 2 * 6 * c->nrep * sizeof(double)});
+  v.push_back({c->C.kbtnow, c->nrep * sizeof(double)}); v.push_back({c->C.gscale, c->nrep * sizeof(double)});
+  v.push_back({c->C.cell.S, nc * sizeof(double)}); v.push_back({c->C.cell.dir, 3 * nc * sizeof(double)});
+}
+
+extern "C" mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *c, int slot) {
+  if (!c || slot < 0) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  std::vector<std::pair<void *, size_t>> v;
+  state_buffers(c, v);
+  if ((int)c->ckpts.size() <= slot) c->ckpts.resize(slot + 1);
+  mpcd_gpu_ctx::Checkpoint &k = c->ckpts[slot];
+  if (k.bufs.empty()) {
+    for (auto &b : v) {
+      void *p = dalloc<uint8_t>(c, b.second);
+      if (!p) return fail(c, MPCD_ENOMEM, "checkpoint");
+      k.bufs.push_back(p);
+    }
+  }
+  for (size_t i = 0; i < v.size(); i++)
+    HIPCHK(c, hipMemcpyAsync(k.bufs[i], v[i].first, v[i].second, hipMemcpyDeviceToDevice, c->stream));
+  k.step = c->step;
+  k.cur = c->cur;
+  k.pending = c->pending;
+  k.hist_ready = c->hist_ready;
+  k.need_init = c->need_init;
+  k.cu_src = c->cu_src;
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *c, int slot) {
+  if (!c || slot < 0 || slot >= (int)c->ckpts.size() || c->ckpts[slot].bufs.empty())
+    return c ? fail(c, MPCD_EINVAL, "restore: no checkpoint in slot %d", slot) : MPCD_EINVAL;
+  mpcd_gpu_ctx::Checkpoint &k = c->ckpts[slot];
+  c->cur = k.cur;  // the saved buffer set is the one that was current
+  std::vector<std::pair<void *, size_t>> v;
+  state_buffers(c, v);
+  for (size_t i = 0; i < v.size(); i++)
+    HIPCHK(c, hipMemcpyAsync(v[i].first, k.bufs[i], v[i].second, hipMemcpyDeviceToDevice, c->stream));
+  c->step = k.step;
+  c->pending = k.pending;
+  c->hist_ready = k.hist_ready;
+  c->need_init = k.need_init;
+  c->cu_src = k.cu_src;
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *c, int max_steps) {
+  if (!c || max_steps < 0) return MPCD_EINVAL;
+  if (max_steps == 0) {
+    c->recording = false;
+    return MPCD_OK;
+  }
+  for (const mpcd_gpu_params &p : c->params) {
+    if (p.lc != MPCD_ISOF || !(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT))
+      return fail(c, MPCD_EUNSUPPORTED, "the adjoint covers SRD and MPC-AT");
+    if (p.tstech != MPCD_NOTHERM) return fail(c, MPCD_EUNSUPPORTED, "the adjoint needs tsTech = 0");
+    for (int w = 0; w < p.nwall; w++)
+      for (int j = 0; j < 3; j++)
+        if (p.wall[w].L[j] != 0.0 || p.wall[w].V[j] != 0.0)
+          return fail(c, MPCD_EUNSUPPORTED, "the adjoint needs resting walls");
+  }
+  if (max_steps > c->tape_cap) {
+    const size_t n = c->ntot, nc = c->nctot, k = max_steps;
+    c->tp_id = dalloc<int32_t>(c, k * n);
+    c->tp_sp = dalloc<int32_t>(c, k * n);
+    c->tp_start = dalloc<int32_t>(c, k * (nc + 1));
+    c->tp_v1 = dalloc<double>(c, k * 3 * n);
+    c->tp_vcm = dalloc<double>(c, k * 3 * nc);
+    c->tp_a = dalloc<double>(c, k * nc);
+    c->tp_mass = dalloc<double>(c, k * nc);
+    c->tp_M = dalloc<double>(c, k * 9 * nc);
+    c->tp_dM = dalloc<double>(c, k * 9 * nc);
+    c->tp_wmul = dalloc<float>(c, k * 3 * n);
+    if (!c->lam) {
+      c->lam = dalloc<double>(c, 3 * n);
+      c->lam_s = dalloc<double>(c, 3 * n);
+      c->lam1_s = dalloc<double>(c, 3 * n);
+      c->adj_dev = dalloc<double>(c, 3 * n);
+      c->dtheta = dalloc<double>(c, 4 * c->nrep);
+    }
+    if (!c->tp_id || !c->tp_sp || !c->tp_start || !c->tp_v1 || !c->tp_vcm || !c->tp_a || !c->tp_mass || !c->tp_M ||
+        !c->tp_dM || !c->tp_wmul || !c->lam || !c->lam_s || !c->lam1_s || !c->adj_dev || !c->dtheta)
+      return fail(c, MPCD_ENOMEM, "adjoint tape");
+    c->tape_cap = max_steps;
+  }
+  c->tape_len = 0;
+  c->recording = true;
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_adjoint_reset(mpcd_gpu_ctx *c) {
+  if (!c) return MPCD_EINVAL;
+  if (!c->lam) return fail(c, MPCD_EINVAL, "no adjoint state (mpcd_gpu_record first)");
+  HIPCHK(c, hipMemsetAsync(c->lam, 0, sizeof(double) * 3 * c->ntot, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->dtheta, 0, sizeof(double) * 4 * c->nrep, c->stream));
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *c, const double *adj_in, double *adj_out,
+                                              double *dJ_dtheta) {
+  if (!c) return MPCD_EINVAL;
+  if (c->tape_len <= 0 || !c->lam) return fail(c, MPCD_EINVAL, "step_backward: the tape is empty");
+  const TapeDev T = tape_at(c, --c->tape_len);
+  const size_t n = c->ntot;
+  if (adj_in) HIPCHK(c, hipMemcpyAsync(c->adj_dev, adj_in, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  k_adj_gather<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(T, c->lam, adj_in ? c->adj_dev : nullptr, c->lam_s,
+                                                                    c->ntot);
+  k_adj_cells<<<c->cblk.size(), kBlock, 0, c->stream>>>(c->C, T, c->d_cblk, c->lam_s, c->lam1_s, c->dtheta);
+  k_adj_scatter<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(T, c->lam1_s, c->lam, c->ntot);
+  if (adj_out) HIPCHK(c, hipMemcpyAsync(adj_out, c->lam, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  if (dJ_dtheta)
+    HIPCHK(c, hipMemcpyAsync(dJ_dtheta, c->dtheta, sizeof(double) * 4 * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
 
 extern "C" mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *c) {
   if (!c) return MPCD_EINVAL;

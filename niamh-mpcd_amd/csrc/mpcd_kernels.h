@@ -1077,7 +1077,7 @@ __global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
 //     bc.c:1363-1496), then the unshifted cell of every particle
 // ---------------------------------------------------------------------------
 template <typename Real>
-__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
+__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul) {
   Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
@@ -1096,6 +1096,9 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
   double t_step = R.dt, t_left = t_step, t_coll = 0.0;
   int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
   double n[3] = {0, 0, 0};
+  // adjoint tape: the product of the velocity factors of the bounces (MVN on
+  // the wall's axis, MVT on the others); NaN when the velocity is redrawn
+  float mul[3] = {1.f, 1.f, 1.f};
   while (true) {
     double tmin = t_left, chW = 1.0;
     for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
@@ -1141,6 +1144,7 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
         Vr[i] *= sp.mass;
       }
       for (int i = 0; i < dim; i++) V[i] += Vr[i] * 1.0 * IMpart;
+      for (int i = 0; i < 3; i++) mul[i] *= (float)((W.A[i] != 0.0) ? W.MVN : W.MVT);
       double PN[3] = {0, 0, 0}, PT[3] = {0, 0, 0};
       projn(Q, n, PN, dim);
       for (int i = 0; i < dim; i++) PT[i] = Q[i] - PN[i];
@@ -1168,6 +1172,7 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
         }
       } else {
         atomicAdd(&C.events[3], 1ull);
+        mul[0] = mul[1] = mul[2] = __builtin_nanf("");
         const uint32_t pid = (uint32_t)S.id[t];
         for (int j = 0; j < dim; j++) {
           if (sp.vdist == 0) V[j] = sqrt(R.kbt / sp.mass) * (2. * Dr.rng.uniform(B.rep, P_RETHERM, pid, rk + j) - 1.);
@@ -1188,6 +1193,7 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
   int cu = cell_of(R, Q, 0);
   if (cu < 0) {
     atomicAdd(&C.events[0], 1ull);
+    mul[0] = mul[1] = mul[2] = __builtin_nanf("");
     const uint32_t pid = (uint32_t)S.id[t];
     for (int j = 0; j < dim; j++) {
       Q[j] = Dr.rng.uniform(B.rep, P_ESCAPE, pid, 8 + j) * R.L[j];
@@ -1207,6 +1213,8 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
     S.uz[t] = (Real)U[2];
   }
   C.cu[t] = cu;
+  if (wmul)
+    for (int i = 0; i < 3; i++) wmul[3 * t + i] = mul[i];
 }
 
 // ---------------------------------------------------------------------------
