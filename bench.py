@@ -189,11 +189,15 @@ def cpu_baseline(L=(64, 64, 64), ppc=10, steps=3):
                 sample=f"3D SRD {L[0]}x{L[1]}x{L[2]} cells x {ppc} ppc ({N} particles), {steps} steps, one core")
 
 
-def adjoint_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, angle=2.0, h=1e-9):
+def adjoint_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, angle=2.0, h=1e-11):
     """gradient of the time-averaged centreline velocity of a 2D SRD channel
     (bounce-back phantom walls in y, body force g along x) with respect to g
     and the rotation angle: adjoint (checkpoint every 20 steps) against central
-    finite differences with common random numbers"""
+    finite differences with common random numbers.  The pathwise gradient
+    holds cell assignments fixed, so h must be small enough that no particle
+    changes cell between the two runs (at this size h = 1e-9 already moves a
+    few dozen particles across cell boundaries over 200 steps); 1e-11 is
+    below that and above FP64 round-off of the observable."""
     import importlib
     H = importlib.import_module("niamh_mpcd_amd.adjoint")
     sys.path.insert(0, os.path.join(ROOT, "tests"))
