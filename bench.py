@@ -51,6 +51,8 @@ def fam_bytes(fam, form):
             "stream": 12 * p + 8,       # read r, v, species; write r, v, unshifted cell
             # fused: read key, r, v, id; write r, v, id, slot cell, next cell, rank, key
             "fused": 12 * p + 44,
+            # bucketed step: read r, v, id, key from the bucket; write r, v, id, key to the next bucket
+            "bucket": 12 * p + 24,
             "ghost": 0, "lc": 6 * p + 4, "temp": 0, "scan": 0}[fam]
 ROTANG_130 = 2.2689280275926285
 

@@ -157,20 +157,38 @@ mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *ctx, int64_t *bc_event_counts_out);
 /* --- whole steps ------------------------------------------------------------ */
 /* one full step (bin + collide + stream); draws may be NULL (internal RNG) */
 mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *ctx, const mpcd_draws *draws, double *kbt_now_out);
-/* nsteps steps with the internal RNG and no host synchronisation; with
- * use_graph != 0 (fused path) pairs of steps are recorded once in a hipGraph
- * and replayed, the step number being read from device memory */
+/* nsteps steps with the internal RNG; with use_graph != 0 (fused path) pairs
+ * of steps are recorded once in a hipGraph and replayed, the step number
+ * being read from device memory.  For the isotropic operators without a
+ * thermostat the run uses the bucketed layout (mpcd_gpu_set_layout): the
+ * state is moved into per-tile buckets at the start and back into the
+ * cell-sorted order at the end (one host synchronisation per call); results
+ * are bit-identical to the cell-sorted layout.  A bucket overflow makes the
+ * run finish on the cell-sorted path from the last complete step (the
+ * boundary-event counters then include the discarded steps' events). */
 mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *ctx, int nsteps, int use_graph);
+/* Layout of mpcd_gpu_run (no counterpart in the reference): bucketed != 0
+ * uses per-tile buckets (the default), 0 the cell-sorted arrays; fill > 0
+ * sets the mean bucket population as a fraction of the bucket capacity
+ * (default 0.6); capacity > 0 lowers the usable slots per bucket below the
+ * kernel's maximum (0 restores it).  A fill near 1 or a low capacity makes
+ * overflows, and the fallback, frequent. */
+mpcd_status mpcd_gpu_set_layout(mpcd_gpu_ctx *ctx, int bucketed, double fill, int capacity);
+/* out[4]: bucketed steps completed, bucket overflows (runs finished on the
+ * cell-sorted path), usable slots per bucket, buckets */
+mpcd_status mpcd_gpu_layout_stats(const mpcd_gpu_ctx *ctx, int64_t *out);
 mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *ctx);
 
 /* Batched-form tuning (no counterpart in the reference, which has no tiles):
  * runs steps_per_trial real steps (plus one untimed step) for each candidate
- * tile fill of the fused kernel, keeps the fastest and returns its ms per
- * step (0 when the context uses the split path).  The trial steps advance the
- * simulation like mpcd_gpu_run.  Invalidates a graph recorded by
+ * fill of the layout mpcd_gpu_run uses (bucket fill of the bucketed layout,
+ * else tile fill of the fused kernel), keeps the fastest and returns its ms
+ * per step (0 when the context uses the split path).  The trial steps advance
+ * the simulation like mpcd_gpu_run.  Invalidates a graph recorded by
  * mpcd_gpu_run. */
 mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *ctx, int steps_per_trial, double *ms_per_step_out);
-/* the tile fill in use: mean cell-tile population / LDS capacity */
+/* the fill in use by mpcd_gpu_run: mean bucket population / bucket capacity
+ * (bucketed layout) or mean tile population / LDS capacity (fused kernel) */
 double      mpcd_gpu_tile_fill(const mpcd_gpu_ctx *ctx);
 
 /* --- checkpoints and the adjoint ------------------------------------------ */
@@ -210,11 +228,11 @@ int         mpcd_gpu_lc_stride(const mpcd_gpu_ctx *ctx);
 /* kernel timing with HIP events recorded on the library's stream around
  * every kernel family while enabled (enable != 0 also clears the records) */
 mpcd_status mpcd_gpu_profile(mpcd_gpu_ctx *ctx, int enable);
-/* out[f] = total device ms of family f, out[12 + f] = number of timed
+/* out[f] = total device ms of family f, out[13 + f] = number of timed
  * launches, for the families 0 count (histogram + keys), 1 scan, 2 scatter,
  * 3 in-cell sort, 4 gather, 5 cell properties, 6 ghost particles, 7 LC,
  * 8 collision, 9 temperature/thermostat, 10 stream + walls, 11 fused
- * gather + collision + stream + next binning; n >= 24 */
+ * gather + collision + stream + next binning, 12 bucketed step; n >= 26 */
 mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *ctx, double *out, int n);
 
 #ifdef __cplusplus

@@ -97,7 +97,8 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_synchronize", "mpcd_gpu_get_cells", "mpcd_gpu_get_order", "mpcd_gpu_ncells",
            "mpcd_gpu_lc_stride", "mpcd_gpu_profile", "mpcd_gpu_timing", "mpcd_gpu_autotune",
            "mpcd_gpu_tile_fill", "mpcd_gpu_checkpoint", "mpcd_gpu_restore", "mpcd_gpu_record",
-           "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward"]
+           "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
+           "mpcd_gpu_layout_stats"]
 
 _lib = None
 
@@ -132,6 +133,8 @@ def lib():
     L.mpcd_gpu_autotune.argtypes = [vp, C.c_int, dp]
     L.mpcd_gpu_tile_fill.argtypes = [vp]
     L.mpcd_gpu_tile_fill.restype = C.c_double
+    L.mpcd_gpu_set_layout.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    L.mpcd_gpu_layout_stats.argtypes = [vp, C.POINTER(C.c_int64)]
     L.mpcd_gpu_checkpoint.argtypes = [vp, C.c_int]
     L.mpcd_gpu_restore.argtypes = [vp, C.c_int]
     L.mpcd_gpu_record.argtypes = [vp, C.c_int]
@@ -292,13 +295,26 @@ class Context:
         self._check(self.L.mpcd_gpu_synchronize(self.h), "synchronize")
 
     def autotune(self, steps_per_trial=5):
-        """pick the fused kernel's tile fill by timing real steps; returns ms per step"""
+        """pick the fill of run()'s layout by timing real steps; returns ms per step"""
         ms = C.c_double(0.0)
         self._check(self.L.mpcd_gpu_autotune(self.h, int(steps_per_trial), C.byref(ms)), "autotune")
         return ms.value
 
     def tile_fill(self):
         return self.L.mpcd_gpu_tile_fill(self.h)
+
+    def set_layout(self, bucketed=True, fill=0.0, capacity=0):
+        """layout of run(): per-tile buckets (default) or the cell-sorted arrays;
+        fill > 0 sets the mean bucket population / bucket capacity, capacity > 0
+        lowers the usable slots per bucket"""
+        self._check(self.L.mpcd_gpu_set_layout(self.h, 1 if bucketed else 0, float(fill), int(capacity)),
+                    "set_layout")
+
+    def layout_stats(self):
+        """(bucketed steps, bucket overflows, usable slots per bucket, buckets)"""
+        out = (C.c_int64 * 4)()
+        self._check(self.L.mpcd_gpu_layout_stats(self.h, out), "layout_stats")
+        return tuple(int(x) for x in out)
 
     # --- checkpoints and the adjoint -------------------------------------------------------------------
     def checkpoint(self, slot):
@@ -331,7 +347,7 @@ class Context:
         return dict(pop=pop, vcm=vcm, S=S, dir=d)
 
     FAMILIES = ["count", "scan", "scatter", "sort", "gather", "props", "ghost", "lc", "collide", "temp", "stream",
-                "fused"]
+                "fused", "bucket"]
 
     def profile(self, enable=True):
         self._check(self.L.mpcd_gpu_profile(self.h, int(bool(enable))), "profile")

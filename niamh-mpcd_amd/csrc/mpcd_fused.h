@@ -237,6 +237,194 @@ __device__ __forceinline__ void next_bin(const Ctx &C, const RepDev &R, const do
   C.key[slot] = bin_key(cn, cu, cell_this, slot);
 }
 
+// the terms of one cell whose particles sit at positions [b, e) of the LDS
+// arrays (component d of particle p at [d * stride + p]): FP64 sums in list
+// order (localPROP, mpc.c:64-216), ghost particles at phantom walls, and the
+// collision's cell data cd[12]: vcm[3], then the SRD matrix[9] or the
+// thermal-draw sum RS[3], W[3] (angular velocity) and CM[3]
+template <typename Real, int OP, bool ONESP, int DIM>
+__device__ __forceinline__ void cell_terms(const RepDev &R, const mpcd_gpu_species *__restrict__ sps,
+                                           const WallHot *__restrict__ wh, const Draws &Dr, int rep, int c, int b,
+                                           int e, const Real *sv, const Real *sq, const double *srv,
+                                           const uint8_t *ssp, int stride, double *cd,
+                                           const double *cdr_pre = nullptr) {
+  const int np_ = e - b;
+  double mass = 0.0, vcm[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
+  // the loads of several particles are in flight at once; the sums keep
+  // their sequential order
+#pragma unroll 4
+  for (int p = b; p < e; p++) {
+    const double m = sps[ONESP ? 0 : ssp[p]].mass;
+    mass += m;
+    for (int d = 0; d < DIM; d++) vcm[d] += (double)sv[d * stride + p] * m;
+    if (OP == OP_RAT)
+      for (int d = 0; d < DIM; d++) cm[d] += (double)sq[d * stride + p] * m;
+  }
+  if (mass > 0.0)
+    for (int d = 0; d < DIM; d++) {
+      vcm[d] /= mass;
+      if (OP == OP_RAT) cm[d] /= mass;
+    }
+  // ghost particles at phantom walls (ghostPart, mpc.c:351-567)
+  if (R.anyphantom && np_ > 0) {
+    int lc_ = c - R.cell_off;
+    int a = lc_ / (R.ny * R.nz), bb = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
+    const int nco = (DIM == 3) ? 8 : 4;
+    int flagW = 0;
+    for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+      const WallHot &H = wh[w];
+      if (!H.phantom) continue;
+      for (int j = 0; j < nco; j++) {
+        double tq[3] = {(double)a + (double)(j & 1), (double)bb + (double)((j >> 1) & 1),
+                        (double)cz + (double)((j >> 2) & 1)};
+        if (wall_W_hot(H, tq, H.wqs) < 0.0) flagW = 1;
+      }
+    }
+    if (flagW && np_ < R.gndnst) {
+      double invN = 1.0 / (R.gndnst - (double)np_);
+      for (int d = 0; d < DIM; d++) {
+        double sigma = sqrt(R.kbt / invN);
+        vcm[d] += sigma * Dr.ghost_n(rep, c, d);
+        vcm[d] /= R.gndnst;
+      }
+    }
+  }
+  cd[0] = vcm[0];
+  cd[1] = vcm[1];
+  cd[2] = vcm[2];
+  if (np_ <= 1) return;
+  if (OP == OP_SRD) {
+    // the cell's draws in the keyed layout: injected, or written by
+    // k_celldraws for this step (possibly already staged by the caller)
+    const double *cdr = cdr_pre ? cdr_pre : &Dr.cell[(size_t)c * 4];
+    double RV[3] = {0., 0., 0.};
+    int CA = 0;
+    if (R.rtech == MPCD_ARBAXIS) {
+      if (DIM == 3) { RV[0] = cdr[0]; RV[1] = cdr[1]; RV[2] = cdr[2]; }
+      else { double x = cdr[0]; RV[0] = x; RV[1] = sqrt(1. - x * x); }
+    } else {
+      CA = (int)((int)DIM * cdr[0]);
+    }
+    double sgn = cdr[3];
+    double Cc = R.cosA, Sn_;
+    if (R.true_sign ? (sgn >= 0.5) : ((long)sgn > 0.5)) Sn_ = R.sinA;
+    else Sn_ = -1. * R.sinA;
+    double *M = cd + 3;
+    if (DIM == 3) {
+      const double *L = RV;
+      if (R.rtech == MPCD_ARBAXIS) {
+        M[0] = L[0] * L[0] + (1. - L[0] * L[0]) * Cc;
+        M[1] = L[0] * L[1] * (1. - Cc) - L[2] * Sn_;
+        M[2] = L[0] * L[2] * (1. - Cc) + L[1] * Sn_;
+        M[3] = L[0] * L[1] * (1. - Cc) + L[2] * Sn_;
+        M[4] = L[1] * L[1] + (1. - L[1] * L[1]) * Cc;
+        M[5] = L[1] * L[2] * (1. - Cc) - L[0] * Sn_;
+        M[6] = L[0] * L[2] * (1. - Cc) - L[1] * Sn_;
+        M[7] = L[1] * L[2] * (1. - Cc) + L[0] * Sn_;
+        M[8] = L[2] * L[2] + (1. - L[2] * L[2]) * Cc;
+      } else if (CA == 0) {
+        M[0] = 1.; M[1] = 0.; M[2] = 0.; M[3] = 0.; M[4] = Cc; M[5] = Sn_; M[6] = 0.; M[7] = -.1 * Sn_; M[8] = Cc;
+      } else if (CA == 1) {
+        M[0] = Cc; M[1] = 0.; M[2] = Sn_; M[3] = 0.; M[4] = 1.; M[5] = 0.; M[6] = -1. * Sn_; M[7] = 0.; M[8] = Cc;
+      } else {
+        M[0] = Cc; M[1] = Sn_; M[2] = 0.; M[3] = -1. * Sn_; M[4] = Cc; M[5] = 0.; M[6] = 0.; M[7] = 0.; M[8] = 1.;
+      }
+    } else {
+      M[0] = Cc; M[1] = -1. * Sn_; M[2] = 0.;
+      M[3] = Sn_; M[4] = Cc; M[5] = 0.;
+      M[6] = 0.; M[7] = 0.; M[8] = 0.;
+    }
+  } else {
+    // mass-weighted sum of the thermal draws, in list order
+    double RS[3] = {0, 0, 0};
+    for (int p = b; p < e; p++) {
+      const double m = sps[ONESP ? 0 : ssp[p]].mass;
+      for (int j = 0; j < DIM; j++) RS[j] += m * srv[j * stride + p];
+    }
+    for (int j = 0; j < DIM; j++) RS[j] /= mass;
+    cd[3] = RS[0];
+    cd[4] = RS[1];
+    cd[5] = RS[2];
+    if (OP == OP_RAT) {
+      // inertia about the centre of mass (mpc.c:4209-4299) and angular momentum (mpc.c:1554-1618)
+      double I[6] = {0, 0, 0, 0, 0, 0};
+      for (int p = b; p < e; p++) {
+        const double m = sps[ONESP ? 0 : ssp[p]].mass;
+        double r[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; d++) r[d] = (double)sq[d * stride + p] - cm[d];
+        I[0] += m * (r[1] * r[1] + r[2] * r[2]);
+        I[1] += m * (r[0] * r[0] + r[2] * r[2]);
+        I[2] += m * (r[0] * r[0] + r[1] * r[1]);
+        I[3] += m * r[0] * r[1];
+        I[4] += m * r[0] * r[2];
+        I[5] += m * r[1] * r[2];
+      }
+      I[3] *= -1.; I[4] *= -1.; I[5] *= -1.;
+      double II[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      if (DIM == 3) {
+        if (np_ >= 3) {
+          double Im[3][3] = {{I[0], I[3], I[4]}, {I[3], I[1], I[5]}, {I[4], I[5], I[2]}};
+          inv3(II, Im);
+        }
+      } else {
+        II[2][2] = 1. / I[2];
+      }
+      double L[3] = {0, 0, 0};
+      for (int p = b; p < e; p++) {
+        const double m = sps[ONESP ? 0 : ssp[p]].mass;
+        double rq[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, am[3];
+        for (int j = 0; j < DIM; j++) {
+          rq[j] = (double)sq[j * stride + p] - cm[j];
+          dv[j] = m * ((double)sv[j * stride + p] - srv[j * stride + p]);
+        }
+        cross3(rq, dv, am);
+        for (int j = 0; j < 3; j++) L[j] += am[j];
+      }
+      for (int i = 0; i < 3; i++) {
+        double w = 0.;
+        for (int j = 0; j < 3; j++) w += II[i][j] * L[j];
+        cd[6 + i] = w;
+        cd[9 + i] = cm[i];
+      }
+    }
+  }
+}
+
+// the collision of one particle of a cell with more than one particle,
+// given the cell data of cell_terms and, for MPC-AT(+a), its thermal draws rv
+template <typename Real, int OP, int DIM>
+__device__ __forceinline__ void collide_one(const mpcd_gpu_species &sp, const double *cd, int np_, const double rv[3],
+                                            const double q[3], double v[3]) {
+  const double vcm[3] = {cd[0], cd[1], cd[2]};
+  if (OP == OP_SRD) {
+    const double *M = cd + 3;
+    double Vr[3] = {v[0] - vcm[0], v[1] - vcm[1], v[2] - vcm[2]};
+    double TM[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) TM[i] += M[i * 3 + j] * Vr[j];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) v[d] = vcm[d] + TM[d];
+  } else if (OP == OP_AT) {
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+      double dv = (sp.damp) * (vcm[j]) / ((double)np_);
+      v[j] = vcm[j] + rv[j] - cd[3 + j] - dv;
+    }
+  } else {
+    double rq[3] = {0, 0, 0}, at[3];
+#pragma unroll
+    for (int j = 0; j < DIM; j++) rq[j] = q[j] - cd[9 + j];
+    const double Wv[3] = {cd[6], cd[7], cd[8]};
+    cross3(Wv, rq, at);
+#pragma unroll
+    for (int j = 0; j < DIM; j++) v[j] = vcm[j] + rv[j] - cd[3 + j] - (sp.damp) * (vcm[j]) + at[j];
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) v[d] = (double)(Real)v[d];  // the state keeps the collided velocity
+}
+
 template <typename Real, int OP, bool ONESP, int DIM>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_fused(
     Ctx C, State<Real> So, State<Real> Sn, const TileDev *__restrict__ tiles, const RepDev *__restrict__ reps,
@@ -284,7 +472,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
     const int base = s_start[0];
     int n = s_start[ncl] - base;
     if (n > kTileCap) {
-      if (tid == 0) atomicOr(&C.flags[0], 1);  // one cell above capacity
+      if (tid == 0) atomicOr(&C.flags[0], kFlagCellCap);  // one cell above capacity
       n = kTileCap;                            // never overrun LDS
     }
     __syncthreads();
@@ -357,147 +545,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
     }
     __syncthreads();
     // one lane per cell: sums in list order and the collision's cell terms
-    for (int k = tid; k < ncl; k += kTileThreads) {
-      const int c = cs + k;
-      const int b = s_start[k], e = s_start[k + 1], np_ = e - b;
-      double mass = 0.0, vcm[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
-      for (int p = b; p < e; p++) {
-        const double m = sps[ONESP ? 0 : s_sp[p]].mass;
-        mass += m;
-        for (int d = 0; d < DIM; d++) vcm[d] += (double)s_v[d][p] * m;
-        if (OP == OP_RAT)
-          for (int d = 0; d < DIM; d++) cm[d] += (double)s_q[d][p] * m;
-      }
-      if (mass > 0.0)
-        for (int d = 0; d < DIM; d++) {
-          vcm[d] /= mass;
-          if (OP == OP_RAT) cm[d] /= mass;
-        }
-      // ghost particles at phantom walls (ghostPart, mpc.c:351-567)
-      if (R.anyphantom && np_ > 0) {
-        int lc_ = c - R.cell_off;
-        int a = lc_ / (R.ny * R.nz), bb = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
-        const int nco = (DIM == 3) ? 8 : 4;
-        int flagW = 0;
-        for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
-          const WallHot &H = wh[w];
-          if (!H.phantom) continue;
-          for (int j = 0; j < nco; j++) {
-            double tq[3] = {(double)a + (double)(j & 1), (double)bb + (double)((j >> 1) & 1),
-                            (double)cz + (double)((j >> 2) & 1)};
-            if (wall_W_hot(H, tq, H.wqs) < 0.0) flagW = 1;
-          }
-        }
-        if (flagW && np_ < R.gndnst) {
-          double invN = 1.0 / (R.gndnst - (double)np_);
-          for (int d = 0; d < DIM; d++) {
-            double sigma = sqrt(R.kbt / invN);
-            vcm[d] += sigma * Dr.ghost_n(T.rep, c, d);
-            vcm[d] /= R.gndnst;
-          }
-        }
-      }
-      double *cd = s_cd[k];
-      cd[0] = vcm[0];
-      cd[1] = vcm[1];
-      cd[2] = vcm[2];
-      if (np_ <= 1) continue;
-      if (OP == OP_SRD) {
-        // the cell's draws in the keyed layout: injected, or written by
-        // k_celldraws for this step
-        const double *cdr = &Dr.cell[(size_t)c * 4];
-        double RV[3] = {0., 0., 0.};
-        int CA = 0;
-        if (R.rtech == MPCD_ARBAXIS) {
-          if (DIM == 3) { RV[0] = cdr[0]; RV[1] = cdr[1]; RV[2] = cdr[2]; }
-          else { double x = cdr[0]; RV[0] = x; RV[1] = sqrt(1. - x * x); }
-        } else {
-          CA = (int)((int)DIM * cdr[0]);
-        }
-        double sgn = cdr[3];
-        double Cc = R.cosA, Sn_;
-        if (R.true_sign ? (sgn >= 0.5) : ((long)sgn > 0.5)) Sn_ = R.sinA;
-        else Sn_ = -1. * R.sinA;
-        double *M = cd + 3;
-        if (DIM == 3) {
-          const double *L = RV;
-          if (R.rtech == MPCD_ARBAXIS) {
-            M[0] = L[0] * L[0] + (1. - L[0] * L[0]) * Cc;
-            M[1] = L[0] * L[1] * (1. - Cc) - L[2] * Sn_;
-            M[2] = L[0] * L[2] * (1. - Cc) + L[1] * Sn_;
-            M[3] = L[0] * L[1] * (1. - Cc) + L[2] * Sn_;
-            M[4] = L[1] * L[1] + (1. - L[1] * L[1]) * Cc;
-            M[5] = L[1] * L[2] * (1. - Cc) - L[0] * Sn_;
-            M[6] = L[0] * L[2] * (1. - Cc) - L[1] * Sn_;
-            M[7] = L[1] * L[2] * (1. - Cc) + L[0] * Sn_;
-            M[8] = L[2] * L[2] + (1. - L[2] * L[2]) * Cc;
-          } else if (CA == 0) {
-            M[0] = 1.; M[1] = 0.; M[2] = 0.; M[3] = 0.; M[4] = Cc; M[5] = Sn_; M[6] = 0.; M[7] = -.1 * Sn_; M[8] = Cc;
-          } else if (CA == 1) {
-            M[0] = Cc; M[1] = 0.; M[2] = Sn_; M[3] = 0.; M[4] = 1.; M[5] = 0.; M[6] = -1. * Sn_; M[7] = 0.; M[8] = Cc;
-          } else {
-            M[0] = Cc; M[1] = Sn_; M[2] = 0.; M[3] = -1. * Sn_; M[4] = Cc; M[5] = 0.; M[6] = 0.; M[7] = 0.; M[8] = 1.;
-          }
-        } else {
-          M[0] = Cc; M[1] = -1. * Sn_; M[2] = 0.;
-          M[3] = Sn_; M[4] = Cc; M[5] = 0.;
-          M[6] = 0.; M[7] = 0.; M[8] = 0.;
-        }
-      } else {
-        // mass-weighted sum of the thermal draws, in list order
-        double RS[3] = {0, 0, 0};
-        for (int p = b; p < e; p++) {
-          const double m = sps[ONESP ? 0 : s_sp[p]].mass;
-          for (int j = 0; j < DIM; j++) RS[j] += m * s_rv[j][p];
-        }
-        for (int j = 0; j < DIM; j++) RS[j] /= mass;
-        cd[3] = RS[0];
-        cd[4] = RS[1];
-        cd[5] = RS[2];
-        if (OP == OP_RAT) {
-          // inertia about the centre of mass (mpc.c:4209-4299) and angular momentum (mpc.c:1554-1618)
-          double I[6] = {0, 0, 0, 0, 0, 0};
-          for (int p = b; p < e; p++) {
-            const double m = sps[ONESP ? 0 : s_sp[p]].mass;
-            double r[3] = {0, 0, 0};
-            for (int d = 0; d < DIM; d++) r[d] = (double)s_q[d][p] - cm[d];
-            I[0] += m * (r[1] * r[1] + r[2] * r[2]);
-            I[1] += m * (r[0] * r[0] + r[2] * r[2]);
-            I[2] += m * (r[0] * r[0] + r[1] * r[1]);
-            I[3] += m * r[0] * r[1];
-            I[4] += m * r[0] * r[2];
-            I[5] += m * r[1] * r[2];
-          }
-          I[3] *= -1.; I[4] *= -1.; I[5] *= -1.;
-          double II[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-          if (DIM == 3) {
-            if (np_ >= 3) {
-              double Im[3][3] = {{I[0], I[3], I[4]}, {I[3], I[1], I[5]}, {I[4], I[5], I[2]}};
-              inv3(II, Im);
-            }
-          } else {
-            II[2][2] = 1. / I[2];
-          }
-          double L[3] = {0, 0, 0};
-          for (int p = b; p < e; p++) {
-            const double m = sps[ONESP ? 0 : s_sp[p]].mass;
-            double rq[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, am[3];
-            for (int j = 0; j < DIM; j++) {
-              rq[j] = (double)s_q[j][p] - cm[j];
-              dv[j] = m * ((double)s_v[j][p] - s_rv[j][p]);
-            }
-            cross3(rq, dv, am);
-            for (int j = 0; j < 3; j++) L[j] += am[j];
-          }
-          for (int i = 0; i < 3; i++) {
-            double w = 0.;
-            for (int j = 0; j < 3; j++) w += II[i][j] * L[j];
-            cd[6 + i] = w;
-            cd[9 + i] = cm[i];
-          }
-        }
-      }
-    }
+    for (int k = tid; k < ncl; k += kTileThreads)
+      cell_terms<Real, OP, ONESP, DIM>(R, sps, wh, Dr, T.rep, cs + k, s_start[k], s_start[k + 1], &s_v[0][0],
+                                       &s_q[0][0], OP == OP_SRD ? nullptr : &s_rv[0][0], ONESP ? nullptr : s_sp,
+                                       kTileCap, s_cd[k]);
     __syncthreads();
     // one lane per particle: collision, kinetic energy, shift back, stream,
     // walls, next bin
@@ -510,34 +561,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
       double v[3] = {(double)s_v[0][p], (double)s_v[1][p], (double)s_v[2][p]};
       double q[3] = {(double)s_q[0][p], (double)s_q[1][p], (double)s_q[2][p]};
       if (np_ > 1) {
-        const double vcm[3] = {cd[0], cd[1], cd[2]};
-        if (OP == OP_SRD) {
-          const double *M = cd + 3;
-          double Vr[3] = {v[0] - vcm[0], v[1] - vcm[1], v[2] - vcm[2]};
-          double TM[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-          for (int i = 0; i < DIM; i++)
-#pragma unroll
-            for (int j = 0; j < DIM; j++) TM[i] += M[i * 3 + j] * Vr[j];
-#pragma unroll
-          for (int d = 0; d < DIM; d++) v[d] = vcm[d] + TM[d];
-        } else if (OP == OP_AT) {
-#pragma unroll
-          for (int j = 0; j < DIM; j++) {
-            double dv = (sp.damp) * (vcm[j]) / ((double)np_);
-            v[j] = vcm[j] + s_rv[j][p] - cd[3 + j] - dv;
-          }
-        } else {
-          double rq[3] = {0, 0, 0}, at[3];
-#pragma unroll
-          for (int j = 0; j < DIM; j++) rq[j] = q[j] - cd[9 + j];
-          const double Wv[3] = {cd[6], cd[7], cd[8]};
-          cross3(Wv, rq, at);
-#pragma unroll
-          for (int j = 0; j < DIM; j++) v[j] = vcm[j] + s_rv[j][p] - cd[3 + j] - (sp.damp) * (vcm[j]) + at[j];
-        }
-#pragma unroll
-        for (int d = 0; d < DIM; d++) v[d] = (double)(Real)v[d];  // the state keeps the collided velocity
+        double rv[3] = {0., 0., 0.};
+        if (OP != OP_SRD)
+          for (int j = 0; j < DIM; j++) rv[j] = s_rv[j][p];
+        collide_one<Real, OP, DIM>(sp, cd, np_, rv, q, v);
       }
       {
         double v0 = v[0] - R.avv[0], v1 = v[1] - R.avv[1], v2 = v[2] - R.avv[2];

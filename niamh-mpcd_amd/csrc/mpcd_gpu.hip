@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "mpcd_adjoint.h"
+#include "mpcd_bucket.h"
 #include "mpcd_fused.h"
 #include "mpcd_kernels.h"
 
@@ -51,6 +52,30 @@ struct mpcd_gpu_ctx {
   int graph_cur = -1;
   uint32_t *d_step = nullptr;
   double tile_fill = 0.45;  // mean tile population as a fraction of the LDS capacity (autotuned)
+  // the bucketed layout of mpcd_gpu_run (isotropic operators, no thermostat):
+  // tiles, their buckets of bkt_cap slots, the two bucketed states and the
+  // per-bucket counts; bcur is the buffer holding the coming step
+  bool bkt_on = true, bkt_dirty = true;
+  bool in_bucket = false;  // the state at rest is in the bucketed layout
+  double bkt_fill = 0.6;  // mean bucket population as a fraction of its capacity (autotuned)
+  int nbkt = 0, bcur = 0, bkt_caplim = 0;  // bkt_caplim: usable slots per bucket (0: all)
+  int64_t bkt_steps = 0, bkt_overflows = 0;
+  size_t bkt_slots = 0;
+  uint32_t bkt_step0 = 0;
+  std::vector<TileDev> bkts;
+  std::vector<BucketRep> brep;
+  std::vector<int2> bkt_rng;
+  TileDev *d_bkts = nullptr;
+  BucketRep *d_brep = nullptr;
+  int2 *d_bkt_rng = nullptr;
+  BState<double> bd[2]{};
+  BState<float> bf[2]{};
+  WallList<double> wld{};
+  WallList<float> wlf{};
+  int32_t *bcnt[2] = {nullptr, nullptr}, *bbase = nullptr, *bstart[2] = {nullptr, nullptr};
+  hipGraph_t bgraph = nullptr;
+  hipGraphExec_t bgexec = nullptr;
+  int bgraph_cur = -1;
   // adjoint: a tape of recorded steps (split path), lambda = dJ/dv per particle
   bool recording = false;
   int tape_cap = 0, tape_len = 0;
@@ -109,7 +134,7 @@ constexpr double kFusedMaxPPC = 40.0;
 
 // kernel families reported by mpcd_gpu_timing
 enum { F_COUNT = 0, F_SCAN, F_SCATTER, F_SORT, F_GATHER, F_PROPS, F_GHOST, F_LC, F_COLLIDE, F_TEMP, F_STREAM, F_FUSED,
-       F_N };
+       F_BUCKET, F_N };
 
 static hipEvent_t prof_event(mpcd_gpu_ctx *c) {
   if (c->ev_used == c->ev_pool.size()) {
@@ -152,6 +177,10 @@ static mpcd_status fail(mpcd_gpu_ctx *c, mpcd_status st, const char *fmt, ...) {
     hipError_t e_ = (call);                                                                         \
     if (e_ != hipSuccess) return fail(ctx, MPCD_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
+
+// the cell-sorted state for the entry points that read or change it (the
+// state at rest may be in the bucketed layout after mpcd_gpu_run)
+static mpcd_status ensure_dense(mpcd_gpu_ctx *c);
 
 template <typename T>
 static T *dalloc(mpcd_gpu_ctx *c, size_t n) {
@@ -464,6 +493,8 @@ extern "C" void mpcd_gpu_destroy(mpcd_gpu_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->bgexec) (void)hipGraphExecDestroy(c->bgexec);
+  if (c->bgraph) (void)hipGraphDestroy(c->bgraph);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -856,6 +887,10 @@ static mpcd_status ensure_stage(mpcd_gpu_ctx *c, size_t n) {
 extern "C" mpcd_status mpcd_gpu_upload(mpcd_gpu_ctx *c, int r, const double *Q, const double *V, const double *U,
                                        const int32_t *spid) {
   if (!c || r < 0 || r >= c->nrep || !Q || !V) return c ? fail(c, MPCD_EINVAL, "upload: bad argument") : MPCD_EINVAL;
+  {
+    mpcd_status st0 = ensure_dense(c);
+    if (st0) return st0;
+  }
   const RepDev &R = c->reps[r];
   mpcd_status st = ensure_stage(c, R.npart);
   if (st) return st;
@@ -889,6 +924,8 @@ extern "C" mpcd_status mpcd_gpu_download(mpcd_gpu_ctx *c, int r, double *Q, doub
   if (!c || r < 0 || r >= c->nrep || !Q || !V) return c ? fail(c, MPCD_EINVAL, "download: bad argument") : MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
   const RepDev &R = c->reps[r];
   st = ensure_stage(c, R.npart);
   if (st) return st;
@@ -914,6 +951,8 @@ extern "C" mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *c, const mpcd_draws *draws, in
                                     int32_t *count_out, int32_t *perm_out) {
   if (!c) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_dense(c);
   if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
@@ -958,6 +997,8 @@ extern "C" mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *c, const mpcd_draws *draws
   if (!c) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
   if (c->precision == MPCD_FP64) run_collide<double>(c, draws != nullptr);
@@ -980,6 +1021,8 @@ extern "C" mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *c, int64_t *bc_event_counts
   if (!c) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
   if (c->precision == MPCD_FP64) run_stream<double>(c, 0);
   else run_stream<float>(c, 0);
   st = check_launch(c);
@@ -991,6 +1034,8 @@ extern "C" mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *c, int64_t *bc_event_counts
 extern "C" mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *c, const mpcd_draws *draws, double *kbt_now_out) {
   if (!c) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_dense(c);
   if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
@@ -1031,6 +1076,393 @@ static mpcd_status capture_pair(mpcd_gpu_ctx *c) {
   return MPCD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// the bucketed layout (mpcd_bucket.h)
+// ---------------------------------------------------------------------------
+template <typename Real>
+static BState<Real> *bstates(mpcd_gpu_ctx *c);
+template <>
+BState<double> *bstates<double>(mpcd_gpu_ctx *c) { return c->bd; }
+template <>
+BState<float> *bstates<float>(mpcd_gpu_ctx *c) { return c->bf; }
+template <typename Real>
+static WallList<Real> &wall_list(mpcd_gpu_ctx *c);
+template <>
+WallList<double> &wall_list<double>(mpcd_gpu_ctx *c) { return c->wld; }
+template <>
+WallList<float> &wall_list<float>(mpcd_gpu_ctx *c) { return c->wlf; }
+
+template <typename Real>
+static bool alloc_wall_list(mpcd_gpu_ctx *c, WallList<Real> &w) {
+  const size_t n = c->ntot;
+  w.x = dalloc<Real>(c, n);
+  w.y = dalloc<Real>(c, n);
+  w.z = dalloc<Real>(c, n);
+  w.vx = dalloc<Real>(c, n);
+  w.vy = dalloc<Real>(c, n);
+  w.vz = dalloc<Real>(c, n);
+  w.meta = dalloc<int4>(c, n);
+  w.cnt = dalloc<int32_t>(c, 1);
+  return w.x && w.y && w.z && w.vx && w.vy && w.vz && w.meta && w.cnt;
+}
+
+static bool bucket_usable(const mpcd_gpu_ctx *c) {
+  return c->bkt_on && c->fusable && !c->recording && !c->any_therm;
+}
+
+static void drop_bucket_graph(mpcd_gpu_ctx *c) {
+  if (c->bgexec) (void)hipGraphExecDestroy(c->bgexec);
+  if (c->bgraph) (void)hipGraphDestroy(c->bgraph);
+  c->bgexec = nullptr;
+  c->bgraph = nullptr;
+}
+
+// tiles of consecutive cells per replica whose mean population is `fill`
+// times the bucket capacity
+static void build_buckets(mpcd_gpu_ctx *c, double fill) {
+  c->bkts.clear();
+  c->brep.clear();
+  c->bkt_rng.clear();
+  const int cap = bkt_cap(fused_op(c->rtech));
+  for (int r = 0; r < c->nrep; r++) {
+    const RepDev &R = c->reps[r];
+    int tc = (int)floor(fill * cap / std::max(mean_ppc(R), 1e-9));
+    tc = std::max(1, std::min(kBktMaxCells, tc));
+    const int b0 = (int)c->bkts.size();
+    for (int b = 0; b < R.ncell; b += tc)
+      c->bkts.push_back(TileDev{r, R.cell_off + b, R.cell_off + std::min(b + tc, R.ncell), 0});
+    c->brep.push_back(BucketRep{b0, tc, 1.0 / tc});
+    c->bkt_rng.push_back(make_int2(b0, (int)c->bkts.size()));
+  }
+  c->nbkt = (int)c->bkts.size();
+  c->bkt_fill = fill;
+  c->bkt_dirty = true;
+  drop_bucket_graph(c);
+}
+
+static void dfree(mpcd_gpu_ctx *c, void *p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+template <typename Real>
+static bool alloc_bstate(mpcd_gpu_ctx *c, BState<Real> &b, size_t n) {
+  void *old[] = {b.x, b.y, b.z, b.vx, b.vy, b.vz, b.id, b.sp, b.key};
+  for (void *p : old) dfree(c, p);
+  b.x = dalloc<Real>(c, n);
+  b.y = dalloc<Real>(c, n);
+  b.z = dalloc<Real>(c, n);
+  b.vx = dalloc<Real>(c, n);
+  b.vy = dalloc<Real>(c, n);
+  b.vz = dalloc<Real>(c, n);
+  b.id = dalloc<int32_t>(c, n);
+  b.sp = dalloc<int32_t>(c, n);
+  b.key = dalloc<uint64_t>(c, n);
+  return b.x && b.y && b.z && b.vx && b.vy && b.vz && b.id && b.sp && b.key;
+}
+
+// device tables and buffers of the current bucket layout
+static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
+  if (c->bkts.empty()) build_buckets(c, c->bkt_fill);
+  if (!c->d_bkts) {
+    const size_t nc1 = (size_t)c->nctot + 1;
+    c->d_bkts = dalloc<TileDev>(c, c->nctot);
+    c->d_brep = dalloc<BucketRep>(c, c->nrep);
+    c->d_bkt_rng = dalloc<int2>(c, c->nrep);
+    c->bcnt[0] = dalloc<int32_t>(c, nc1);
+    c->bcnt[1] = dalloc<int32_t>(c, nc1);
+    c->bbase = dalloc<int32_t>(c, nc1);
+    c->bstart[0] = dalloc<int32_t>(c, nc1);
+    c->bstart[1] = dalloc<int32_t>(c, nc1);
+    if (!c->d_bkts || !c->d_brep || !c->d_bkt_rng || !c->bcnt[0] || !c->bcnt[1] || !c->bbase || !c->bstart[0] ||
+        !c->bstart[1])
+      return fail(c, MPCD_ENOMEM, "bucket tables");
+    if (!alloc_scan(c)) return fail(c, MPCD_ENOMEM, "scan scratch");
+    // the wall list holds at most every particle once per step
+    if (!(c->precision == MPCD_FP64 ? alloc_wall_list<double>(c, c->wld) : alloc_wall_list<float>(c, c->wlf)))
+      return fail(c, MPCD_ENOMEM, "wall list");
+  }
+  const int cap = bkt_cap(fused_op(c->rtech));
+  const size_t slots = (size_t)c->nbkt * cap;
+  if (slots > c->bkt_slots) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    bool ok = true;
+    for (int k = 0; k < 2; k++)
+      ok = ok && (c->precision == MPCD_FP64 ? alloc_bstate<double>(c, c->bd[k], slots)
+                                            : alloc_bstate<float>(c, c->bf[k], slots));
+    if (!ok) {
+      c->bkt_slots = 0;
+      return fail(c, MPCD_ENOMEM, "bucketed state (%zu slots)", slots);
+    }
+    c->bkt_slots = slots;
+    drop_bucket_graph(c);
+  }
+  if (c->bkt_dirty) {
+    HIPCHK(c, hipMemcpyAsync(c->d_bkts, c->bkts.data(), sizeof(TileDev) * c->nbkt, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_brep, c->brep.data(), sizeof(BucketRep) * c->nrep, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_bkt_rng, c->bkt_rng.data(), sizeof(int2) * c->nrep, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->bkt_dirty = false;
+  }
+  for (int k = 0; k < 2; k++) {
+    c->bd[k].cnt = c->bcnt[k];
+    c->bf[k].cnt = c->bcnt[k];
+  }
+  return MPCD_OK;
+}
+
+static int bkt_limit(const mpcd_gpu_ctx *c) {
+  const int cap = bkt_cap(fused_op(c->rtech));
+  return (c->bkt_caplim > 0 && c->bkt_caplim < cap) ? c->bkt_caplim : cap;
+}
+
+template <typename Real, int OP, bool ONESP, int DIM>
+static void launch_bucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  BState<Real> *B = bstates<Real>(c);
+  const int nw3 = 3 * std::max(1, c->nwtot), nwm = std::max(1, c->nwtot);
+  k_bucket<Real, OP, ONESP, DIM, bkt_ppt(OP)><<<c->nbkt, kBktThreads, 0, c->stream>>>(
+      C, B[c->bcur], B[1 - c->bcur], c->d_bkts, c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
+      c->box2 + par * 6 * c->nrep, D, c->shift2 + par * 3 * c->nrep, c->wqb2 + par * nw3,
+      c->shift2 + (par ^ 1) * 3 * c->nrep, c->bbase, c->bstart[par], bkt_limit(c), wall_list<Real>(c));
+}
+
+template <typename Real, int OP, bool ONESP>
+static void launch_bucket_dim(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  if (c->dim == 3) launch_bucket<Real, OP, ONESP, 3>(c, C, D, par);
+  else launch_bucket<Real, OP, ONESP, 2>(c, C, D, par);
+}
+
+template <typename Real>
+static void dispatch_bucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  const bool one = c->nspec == 1;
+  const int op = fused_op(c->rtech);
+  if (op == OP_SRD) {
+    if (one) launch_bucket_dim<Real, OP_SRD, true>(c, C, D, par);
+    else launch_bucket_dim<Real, OP_SRD, false>(c, C, D, par);
+  } else if (op == OP_AT) {
+    if (one) launch_bucket_dim<Real, OP_AT, true>(c, C, D, par);
+    else launch_bucket_dim<Real, OP_AT, false>(c, C, D, par);
+  } else {
+    if (one) launch_bucket_dim<Real, OP_RAT, true>(c, C, D, par);
+    else launch_bucket_dim<Real, OP_RAT, false>(c, C, D, par);
+  }
+}
+
+// one step in the bucketed layout (internal generator)
+template <typename Real>
+static void run_step_bucket(mpcd_gpu_ctx *c) {
+  const int par = c->step & 1;
+  Ctx C = view(c, par);
+  Draws D = make_draws(c, 0, c->step);
+  {
+    // next step's shift, bucket-count block sums, zeroed output counts and
+    // this step's SRD cell draws in one launch
+    ProfScope ps(c, F_SCAN);
+    const int p1 = (c->step + 1) & 1, nw3 = 3 * std::max(1, c->nwtot);
+    const int nbs = nblocks(c->nbkt + 1, kBlock);
+    const int ncb = c->d_celldraw ? (int)c->cblk.size() : 0;
+    Draws D1 = make_draws(c, 0, c->step + 1);
+    k_bucket_prep<<<nbs + 1 + ncb, kBlock, 0, c->stream>>>(
+        C, D1, c->wqb2 + (p1 ^ 1) * nw3, c->shift2 + p1 * 3 * c->nrep, c->wqs2 + p1 * nw3, c->wqb2 + p1 * nw3,
+        c->wh2 + p1 * std::max(1, c->nwtot), c->box2 + p1 * 6 * c->nrep, c->bcnt[c->bcur], c->bbase,
+        c->bcnt[1 - c->bcur], wall_list<Real>(c).cnt, c->nbkt, nbs, D, c->d_cblk, c->d_celldraw);
+    if (c->d_celldraw) D.cell = c->d_celldraw;
+  }
+  {
+    ProfScope ps(c, F_BUCKET);
+    dispatch_bucket<Real>(c, C, D, par);
+  }
+  {
+    ProfScope ps(c, F_STREAM);
+    BState<Real> *B = bstates<Real>(c);
+    const int nw3 = 3 * std::max(1, c->nwtot), nwm = std::max(1, c->nwtot);
+    const int cap = bkt_cap(fused_op(c->rtech)), nb = std::min(2048, nblocks(c->ntot, kBlock));
+    if (c->dim == 3)
+      k_bucket_walls<Real, 3><<<nb, kBlock, 0, c->stream>>>(
+          C, wall_list<Real>(c), B[1 - c->bcur], c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
+          c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c));
+    else
+      k_bucket_walls<Real, 2><<<nb, kBlock, 0, c->stream>>>(
+          C, wall_list<Real>(c), B[1 - c->bcur], c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
+          c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c));
+  }
+  c->bcur ^= 1;
+  c->step++;
+}
+
+// dense -> bucketed, from a dense state whose next shifted bin is built
+template <typename Real>
+static mpcd_status bucket_enter(mpcd_gpu_ctx *c) {
+  mpcd_status st = ensure_buckets(c);
+  if (st) return st;
+  State<Real> *S = states<Real>(c);
+  BState<Real> *B = bstates<Real>(c);
+  Ctx C = view(c, c->step & 1);
+  k_fill<<<nblocks(c->nbkt + 1, kBlock), kBlock, 0, c->stream>>>(c->bcnt[0], 0, c->nbkt + 1);
+  k_fill<<<1, 1, 0, c->stream>>>(c->C.flags + 1, 0x7FFFFFFF, 1);
+  k_bucketize<Real><<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C, S[c->cur], B[0], c->d_brep,
+                                                                          bkt_cap(fused_op(c->rtech)), bkt_limit(c));
+  c->bcur = 0;
+  c->bkt_step0 = c->step;
+  return check_launch(c);
+}
+
+// bucketed -> dense at step s from bucket buffer bsrc (the buffer holding
+// step s's particles), with the pending shifted bin of step s
+template <typename Real>
+static void bucket_densify(mpcd_gpu_ctx *c, uint32_t s, int bsrc) {
+  Ctx C = view(c, s & 1);
+  State<Real> *S = states<Real>(c);
+  BState<Real> *B = bstates<Real>(c);
+  const int cap = bkt_cap(fused_op(c->rtech)), one = c->nspec == 1 ? 1 : 0;
+  const double *sh = c->shift2 + (s & 1) * 3 * c->nrep;
+  k_fill<<<nblocks(c->nctot + 1, kBlock), kBlock, 0, c->stream>>>(C.hist, 0, c->nctot + 1);
+  if (c->dim == 3)
+    k_densify<Real, 3><<<c->nbkt, kBlock, 0, c->stream>>>(C, B[bsrc], S[c->cur], c->d_bkts, cap, one, sh,
+                                                           c->bstart[(s - 1) & 1]);
+  else
+    k_densify<Real, 2><<<c->nbkt, kBlock, 0, c->stream>>>(C, B[bsrc], S[c->cur], c->d_bkts, cap, one, sh,
+                                                           c->bstart[(s - 1) & 1]);
+  c->pending = true;
+  c->cu_src = 1;
+  c->hist_ready = true;
+}
+
+// the end of a bucketed run: the temperature of its last step and the
+// overflow check.  After a bucket overflow the state is the one before the
+// first overflowing step -- the input buffer of that step, untouched since
+// every later bucketed kernel was a no-op -- in the cell-sorted layout, and
+// *done counts the steps completed since run_step0.  Without an overflow the
+// state stays bucketed.
+template <typename Real>
+static mpcd_status bucket_finish(mpcd_gpu_ctx *c, uint32_t run_step0, int *done) {
+  const uint32_t s0 = c->bkt_step0;
+  if (c->step != run_step0) {
+    ProfScope ps(c, F_TEMP);
+    k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(view(c, (c->step - 1) & 1), c->d_bkt_rng);
+  }
+  int fl[2];
+  HIPCHK(c, hipMemcpyAsync(fl, c->C.flags, sizeof fl, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (fl[0] & kFlagBadCell) return fail(c, MPCD_EDEVICE, "bucketed step: a particle outside its bucket's tile");
+  *done = (int)(c->step - run_step0);
+  if (!(fl[0] & kFlagPoison)) {
+    c->bkt_steps += *done;
+    return MPCD_OK;
+  }
+  const uint32_t s = (fl[1] < (int)s0) ? s0 : (uint32_t)fl[1];
+  c->bkt_overflows++;
+  int clr[2] = {fl[0] & ~kFlagPoison, 0x7FFFFFFF};
+  HIPCHK(c, hipMemcpyAsync(c->C.flags, clr, sizeof clr, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // s == s0: the cell-sorted state the bucketed layout started from is intact
+  if (s != s0) bucket_densify<Real>(c, s, (int)((s - s0) & 1u));
+  c->in_bucket = false;
+  *done = (int)(s - run_step0);
+  c->bkt_steps += *done;
+  c->step = s;
+  return check_launch(c);
+}
+
+// records two bucketed steps (from an even step) and the advance of the
+// device step counter into c->bgexec; nothing runs while recording
+template <typename Real>
+static mpcd_status capture_bucket_pair(mpcd_gpu_ctx *c) {
+  drop_bucket_graph(c);
+  const uint32_t step0 = c->step;
+  const int bcur0 = c->bcur;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  c->cap_base = step0;
+  for (int k = 0; k < 2; k++) run_step_bucket<Real>(c);
+  k_addu32<<<1, 1, 0, c->stream>>>(c->d_step, 2u);
+  c->capturing = false;
+  hipError_t e = hipStreamEndCapture(c->stream, &c->bgraph);
+  c->step = step0;
+  c->bcur = bcur0;
+  if (e != hipSuccess) return fail(c, MPCD_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  HIPCHK(c, hipGraphInstantiate(&c->bgexec, c->bgraph, nullptr, nullptr, 0));
+  c->bgraph_cur = bcur0;
+  return MPCD_OK;
+}
+
+// nsteps steps in the bucketed layout; *done < nsteps after a bucket overflow
+template <typename Real>
+static mpcd_status bucket_run(mpcd_gpu_ctx *c, int nsteps, int use_graph, int *done) {
+  *done = 0;
+  mpcd_status st;
+  if (!c->in_bucket) {
+    st = bucket_enter<Real>(c);
+    if (st) return st;
+    c->in_bucket = true;
+  }
+  const uint32_t run_step0 = c->step;
+  int left = nsteps;
+  if (use_graph) {
+    while (left > 0 && (c->step & 1)) {
+      run_step_bucket<Real>(c);
+      left--;
+    }
+    const int pairs = left / 2;
+    if (pairs > 0) {
+      if (!c->bgexec || c->bgraph_cur != c->bcur) {
+        st = check_launch(c);
+        if (st) return st;
+        st = capture_bucket_pair<Real>(c);
+        if (st) return st;
+      }
+      k_setu32<<<1, 1, 0, c->stream>>>(c->d_step, c->step);
+      for (int k = 0; k < pairs; k++) HIPCHK(c, hipGraphLaunch(c->bgexec, c->stream));
+      c->step += 2u * pairs;
+      left -= 2 * pairs;
+    }
+  }
+  for (; left > 0; left--) run_step_bucket<Real>(c);
+  st = check_launch(c);
+  if (st) return st;
+  return bucket_finish<Real>(c, run_step0, done);
+}
+
+static mpcd_status ensure_dense(mpcd_gpu_ctx *c) {
+  if (!c->in_bucket) return MPCD_OK;
+  const uint32_t s = c->step;
+  const int bsrc = (int)((s - c->bkt_step0) & 1u);
+  if (s != c->bkt_step0) {
+    if (c->precision == MPCD_FP64) bucket_densify<double>(c, s, bsrc);
+    else bucket_densify<float>(c, s, bsrc);
+  }
+  c->in_bucket = false;
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_set_layout(mpcd_gpu_ctx *c, int bucketed, double fill, int capacity) {
+  if (!c) return MPCD_EINVAL;
+  mpcd_status st = ensure_dense(c);
+  if (st) return st;
+  if (fill > 0.0) {
+    if (fill > 4.0) return fail(c, MPCD_EINVAL, "set_layout: fill %g above 4", fill);
+    build_buckets(c, fill);
+  }
+  if (capacity < 0) return fail(c, MPCD_EINVAL, "set_layout: capacity %d", capacity);
+  c->bkt_caplim = capacity;
+  c->bkt_on = bucketed != 0;
+  drop_bucket_graph(c);
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_layout_stats(const mpcd_gpu_ctx *c, int64_t *out) {
+  if (!c || !out) return MPCD_EINVAL;
+  out[0] = c->bkt_steps;
+  out[1] = c->bkt_overflows;
+  out[2] = bkt_limit(c);
+  out[3] = c->nbkt;
+  return MPCD_OK;
+}
+
 extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) {
   if (!c || nsteps < 0) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
@@ -1039,6 +1471,24 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
     if (c->precision == MPCD_FP64) run_step<double>(c, 0);
     else run_step<float>(c, 0);
   };
+  if (bucket_usable(c) && nsteps > 0) {
+    // the bucketed layout starts from a cell-sorted state whose next bin is built
+    while (!c->in_bucket && nsteps > 0 && !c->hist_ready) {
+      eager();
+      nsteps--;
+    }
+    if (nsteps > 0) {
+      int done = 0;
+      st = c->precision == MPCD_FP64 ? bucket_run<double>(c, nsteps, use_graph, &done)
+                                     : bucket_run<float>(c, nsteps, use_graph, &done);
+      if (st) return st;
+      nsteps -= done;  // after a bucket overflow the dense path finishes the run
+    }
+  }
+  if (nsteps > 0) {
+    st = ensure_dense(c);
+    if (st) return st;
+  }
   if (use_graph && c->fusable && !c->recording) {
     // graphs start from an even step whose histogram the previous step built
     while (nsteps > 0 && (!c->hist_ready || (c->step & 1))) {
@@ -1063,13 +1513,69 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
   return check_launch(c);
 }
 
+// the bucket fill of the bucketed layout: candidates in increasing order, so
+// the first allocates the most slots; a trial whose buckets overflow loses
+template <typename Real>
+static mpcd_status autotune_buckets(mpcd_gpu_ctx *c, int steps, double *ms_out) {
+  const double fills[] = {0.4, 0.5, 0.6, 0.7, 0.75};
+  double best = 1e300, best_fill = c->bkt_fill;
+  hipEvent_t a, b;
+  HIPCHK(c, hipEventCreate(&a));
+  HIPCHK(c, hipEventCreate(&b));
+  mpcd_status st = MPCD_OK;
+  for (double f : fills) {
+    // 1 + steps steps per candidate: the untimed one is a cell-sorted step
+    // when the next bin is not built yet
+    build_buckets(c, f);
+    int untimed = 0;
+    if (!c->hist_ready) run_step<Real>(c, 0);
+    else untimed = 1;
+    st = bucket_enter<Real>(c);
+    if (st) break;
+    c->in_bucket = true;
+    const uint32_t s0 = c->step;
+    if (untimed) run_step_bucket<Real>(c);
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    for (int k = 0; k < steps; k++) run_step_bucket<Real>(c);
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    int done = 0;
+    st = bucket_finish<Real>(c, s0, &done);
+    if (st) break;
+    st = ensure_dense(c);  // the next candidate has another layout
+    if (st) break;
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+    if (done == steps + untimed && ms / steps < best) {
+      best = ms / steps;
+      best_fill = f;
+    }
+    for (int k = done; k < steps + untimed; k++) run_step<Real>(c, 0);  // after an overflow
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (st) return st;
+  build_buckets(c, best_fill);
+  if (ms_out) *ms_out = best;
+  return MPCD_OK;
+}
+
 extern "C" mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *c, int steps_per_trial, double *ms_per_step_out) {
   if (!c || steps_per_trial < 1) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
   if (!c->fusable) {
     if (ms_per_step_out) *ms_per_step_out = 0.0;
     return MPCD_OK;  // the split path has no tile parameter
+  }
+  if (bucket_usable(c)) {
+    st = c->precision == MPCD_FP64 ? autotune_buckets<double>(c, steps_per_trial, ms_per_step_out)
+                                   : autotune_buckets<float>(c, steps_per_trial, ms_per_step_out);
+    if (st) return st;
+    st = check_launch(c);
+    if (st) return st;
+    return check_flags(c);
   }
   const double fills[] = {0.25, 0.35, 0.45, 0.6, 0.8};
   double best = 1e300, best_fill = c->tile_fill;
@@ -1116,7 +1622,10 @@ extern "C" mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *c, int steps_per_trial, d
   return check_flags(c);
 }
 
-extern "C" double mpcd_gpu_tile_fill(const mpcd_gpu_ctx *c) { return c ? c->tile_fill : 0.0; }
+extern "C" double mpcd_gpu_tile_fill(const mpcd_gpu_ctx *c) {
+  if (!c) return 0.0;
+  return bucket_usable(c) ? c->bkt_fill : c->tile_fill;
+}
 
 // ---------------------------------------------------------------------------
 // checkpoints and the adjoint
@@ -1149,6 +1658,8 @@ extern "C" mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *c, int slot) {
   if (!c || slot < 0) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
   std::vector<std::pair<void *, size_t>> v;
   state_buffers(c, v);
   if ((int)c->ckpts.size() <= slot) c->ckpts.resize(slot + 1);
@@ -1174,6 +1685,7 @@ extern "C" mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *c, int slot) {
 extern "C" mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *c, int slot) {
   if (!c || slot < 0 || slot >= (int)c->ckpts.size() || c->ckpts[slot].bufs.empty())
     return c ? fail(c, MPCD_EINVAL, "restore: no checkpoint in slot %d", slot) : MPCD_EINVAL;
+  c->in_bucket = false;  // the restored state is cell-sorted
   mpcd_gpu_ctx::Checkpoint &k = c->ckpts[slot];
   c->cur = k.cur;  // the saved buffer set is the one that was current
   std::vector<std::pair<void *, size_t>> v;
@@ -1190,6 +1702,10 @@ extern "C" mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *c, int slot) {
 
 extern "C" mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *c, int max_steps) {
   if (!c || max_steps < 0) return MPCD_EINVAL;
+  {
+    mpcd_status st0 = ensure_dense(c);
+    if (st0) return st0;
+  }
   if (max_steps == 0) {
     c->recording = false;
     return MPCD_OK;
@@ -1270,6 +1786,8 @@ extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, 
   if (!c || r < 0 || r >= c->nrep) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
   if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
   if (c->pending) {
     // the reference re-bins and recomputes localPROP after streaming (mpc.c:4936-4942)
     if (c->precision == MPCD_FP64) {
@@ -1293,6 +1811,8 @@ extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, 
 extern "C" mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *c, int r, int32_t *order, int32_t *cstart) {
   if (!c || r < 0 || r >= c->nrep) return MPCD_EINVAL;
   mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_dense(c);
   if (st) return st;
   st = ensure_stage(c, c->ntot);
   if (st) return st;

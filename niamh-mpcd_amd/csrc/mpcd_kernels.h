@@ -67,9 +67,15 @@ struct Ctx {
   double *tpart;             // temperature partial sums per block
   double *kbtnow, *gscale, *sg; // per replica
   unsigned long long *events;   // [4]
-  int *flags;                   // [0] capacity error
+  int *flags;                   // [0] error bits (kFlag*), [1] first overflowing step of a bucketed run
   int ntot, nctot, nrep;
 };
+
+// bits of Ctx::flags[0]
+constexpr int kFlagCellCap = 1;  // a cell above the fused kernel's LDS capacity
+constexpr int kFlagOutside = 2;  // initial positions outside the box
+constexpr int kFlagPoison = 4;   // a bucket overflowed: the bucketed steps from then on are no-ops
+constexpr int kFlagBadCell = 8;  // a particle outside its bucket's tile (internal error)
 
 // hot fields of an axis-aligned planar wall for the per-particle wall test:
 // the normal's one nonzero component A (0 when its axis is not a box axis),
@@ -135,13 +141,12 @@ __device__ __forceinline__ void shift_back(const RepDev &R, const double *s, dou
 // ---------------------------------------------------------------------------
 // 1. per-replica grid shift draw and wall positions (ranshift, rand.c:678)
 // ---------------------------------------------------------------------------
-__global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, double *shift_out, double *wqs_out,
-                        double *wqb_out, WallHot *wh_out, double *box_out) {
-  D.resolve();
-  // reads the previous step's shifted-back wall positions and writes this
-  // step's shift and wall positions, so a repeated launch is idempotent
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= C.nrep) return;
+// the grid shift of replica r and the wall positions it implies: reads the
+// previous step's shifted-back wall positions and writes this step's shift
+// and wall positions, so a repeated launch is idempotent
+__device__ __forceinline__ void shift_replica(const Ctx &C, const Draws &D, int r, int do_shift,
+                                              const double *wq_prev_b, double *shift_out, double *wqs_out,
+                                              double *wqb_out, WallHot *wh_out, double *box_out) {
   const RepDev &R = C.rep[r];
   double s[3] = {0.0, 0.0, 0.0};
   if (do_shift && R.galinv) {
@@ -152,16 +157,21 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, d
   double Lmax = 0.0;
   for (int j = 0; j < R.dim; j++) Lmax = fmax(Lmax, (double)R.L[j]);
   for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+    // the positions stay in registers (no store and reload through memory)
+    double qs3[3], qb3[3];
+#pragma unroll
     for (int j = 0; j < 3; j++) {
       double q = wq_prev_b[w * 3 + j];
       if (j < R.dim && do_shift) {
         double qs = q + s[j];
-        wqs_out[w * 3 + j] = qs;
-        wqb_out[w * 3 + j] = qs + (-1.0 * s[j]);
+        qs3[j] = qs;
+        qb3[j] = qs + (-1.0 * s[j]);
       } else {
-        wqs_out[w * 3 + j] = q;
-        wqb_out[w * 3 + j] = q;
+        qs3[j] = q;
+        qb3[j] = q;
       }
+      wqs_out[w * 3 + j] = qs3[j];
+      wqb_out[w * 3 + j] = qb3[j];
     }
     const mpcd_gpu_wall &W = C.wall[w];
     int k = 0;
@@ -170,8 +180,8 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, d
     WallHot H;
     H.A = (k < R.dim) ? W.A[k] : 0.0;
     H.R = W.R;
-    H.wqb = wqb_out[w * 3 + k];
-    H.wqs = wqs_out[w * 3 + k];
+    H.wqb = (k == 0) ? qb3[0] : (k == 1) ? qb3[1] : qb3[2];
+    H.wqs = (k == 0) ? qs3[0] : (k == 1) ? qs3[1] : qs3[2];
     H.axis = k;
     H.inv = W.INV;
     H.mask = W.inter_mask;
@@ -195,6 +205,16 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, d
   }
 }
 
+__global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, double *shift_out, double *wqs_out,
+                        double *wqb_out, WallHot *wh_out, double *box_out) {
+  D.resolve();
+  // after a bucket overflow the shifts of the last complete state are kept
+  if (C.flags[0] & kFlagPoison) return;
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C.nrep) return;
+  shift_replica(C, D, r, do_shift, wq_prev_b, shift_out, wqs_out, wqb_out, wh_out, box_out);
+}
+
 // ---------------------------------------------------------------------------
 // 2. histogram + composite key.  mode 0: initial binning (binin, mpc.c:924);
 //    1: shifted bin; 2: materialise the pending unshifted bin
@@ -211,7 +231,7 @@ __global__ void k_count(Ctx C, State<Real> S, const BlkDev *blk, int mode, int p
   uint32_t a1 = 0, a2 = 0;
   if (mode == 0) {
     cnew = cell_of(R, q, 0);
-    if (cnew < 0) { atomicOr(&C.flags[0], 2); cnew = R.cell_off; }
+    if (cnew < 0) { atomicOr(&C.flags[0], kFlagOutside); cnew = R.cell_off; }
     cu = cnew;
   } else {
     int cs = S.cst[s];
