@@ -46,15 +46,75 @@ struct BucketRep {
   double inv_tc;
 };
 
-// the bucketed state: `cap` slots per bucket (of which the first cap_lim,
-// at most cap, may be filled), `cnt` particles in each
+// The bucketed state: `cap` slots per bucket (of which the first cap_lim,
+// at most cap, may be filled), `cnt` particles in each.  A slot is one
+// record packed into vector words, so a particle costs three or four memory
+// instructions each way:
+//   FP32: a = (x, y, z, vx), b = (vy, vz, id, key bits 0..31),
+//         c = key bits 32..63 (36 B), plus sp when there are several species;
+//   FP64: a = (x, y), b = (z, vx), c = (vy, vz), d = (id, key bits 0..31,
+//         key bits 32..63, species) (64 B).
 template <typename Real>
-struct BState {
-  Real *x, *y, *z, *vx, *vy, *vz;
-  int32_t *id, *sp;
-  uint64_t *key;
+struct BState;
+template <>
+struct BState<float> {
+  float4 *a, *b;
+  uint32_t *c;
+  int32_t *sp;
   int32_t *cnt;
 };
+template <>
+struct BState<double> {
+  double2 *a, *b, *c;
+  int4 *d;
+  int32_t *cnt;
+};
+
+__device__ __forceinline__ void rec_load(const BState<float> &B, size_t o, bool onesp, float q[3], float v[3], int &id,
+                                         int &sp, uint64_t &key) {
+  const float4 a = B.a[o], b = B.b[o];
+  const uint32_t hi = B.c[o];
+  q[0] = a.x;
+  q[1] = a.y;
+  q[2] = a.z;
+  v[0] = a.w;
+  v[1] = b.x;
+  v[2] = b.y;
+  id = __float_as_int(b.z);
+  key = ((uint64_t)hi << 32) | (uint64_t)(uint32_t)__float_as_int(b.w);
+  sp = onesp ? 0 : B.sp[o];
+}
+
+__device__ __forceinline__ void rec_store(const BState<float> &B, size_t o, bool onesp, const float q[3],
+                                          const float v[3], int id, int sp, uint64_t key) {
+  B.a[o] = make_float4(q[0], q[1], q[2], v[0]);
+  B.b[o] = make_float4(v[1], v[2], __int_as_float(id), __int_as_float((int)(uint32_t)key));
+  B.c[o] = (uint32_t)(key >> 32);
+  if (!onesp) B.sp[o] = sp;
+}
+
+__device__ __forceinline__ void rec_load(const BState<double> &B, size_t o, bool, double q[3], double v[3], int &id,
+                                         int &sp, uint64_t &key) {
+  const double2 a = B.a[o], b = B.b[o], c = B.c[o];
+  const int4 d = B.d[o];
+  q[0] = a.x;
+  q[1] = a.y;
+  q[2] = b.x;
+  v[0] = b.y;
+  v[1] = c.x;
+  v[2] = c.y;
+  id = d.x;
+  key = ((uint64_t)(uint32_t)d.z << 32) | (uint64_t)(uint32_t)d.y;
+  sp = d.w;
+}
+
+__device__ __forceinline__ void rec_store(const BState<double> &B, size_t o, bool, const double q[3], const double v[3],
+                                          int id, int sp, uint64_t key) {
+  B.a[o] = make_double2(q[0], q[1]);
+  B.b[o] = make_double2(q[2], v[0]);
+  B.c[o] = make_double2(v[1], v[2]);
+  B.d[o] = make_int4(id, (int)(uint32_t)key, (int)(uint32_t)(key >> 32), sp);
+}
 
 // (cell - cell_off) / tc: an FP64 estimate (off by at most one for offsets
 // below 2^31) and one correction step
@@ -205,11 +265,13 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
 #pragma unroll
     for (int d = 0; d < 3; d++) rq[j][d] = rv[j][d] = (Real)0;
     if (i < n) {
-      double q[3] = {(double)I.x[off + i], (double)I.y[off + i], (double)I.z[off + i]};
-      double v[3] = {(double)I.vx[off + i], (double)I.vy[off + i], (double)I.vz[off + i]};
-      const int pid = I.id[off + i];
-      const int spid = ONESP ? 0 : I.sp[off + i];
-      const uint64_t key = I.key[off + i];
+      Real q_[3], v_[3];
+      int pid, spid;
+      uint64_t key;
+      rec_load(I, off + i, ONESP, q_, v_, pid, spid, key);
+      if (ONESP) spid = 0;
+      double q[3] = {(double)q_[0], (double)q_[1], (double)q_[2]};
+      double v[3] = {(double)v_[0], (double)v_[1], (double)v_[2]};
       if (R.galinv) shift_fwd_bf<DIM>(Ld, shc, q);
       int cn = cell_bf<DIM>(Li, pbc, cell_off, ny, nz, q, true);
       if (cn < 0) {
@@ -412,15 +474,8 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
       continue;
     }
     const size_t o = (size_t)b2 * CAP + p2;
-    O.x[o] = rq[j][0];
-    O.y[o] = rq[j][1];
-    O.z[o] = rq[j][2];
-    O.vx[o] = rv[j][0];
-    O.vy[o] = rv[j][1];
-    O.vz[o] = rv[j][2];
-    O.id[o] = rid[j];
-    if (!ONESP) O.sp[o] = rsp[j];
-    O.key[o] = rkey[j];
+    const Real q3[3] = {rq[j][0], rq[j][1], rq[j][2]}, v3[3] = {rv[j][0], rv[j][1], rv[j][2]};
+    rec_store(O, o, ONESP, q3, v3, rid[j], rsp[j], rkey[j]);
   }
   // kinetic-energy partial of the tile
 #pragma unroll
@@ -523,15 +578,7 @@ __global__ void k_bucket_walls(Ctx C, WallList<Real> WL, BState<Real> O, const B
       continue;
     }
     const size_t o = (size_t)b2 * cap + p2;
-    O.x[o] = rq[0];
-    O.y[o] = rq[1];
-    O.z[o] = rq[2];
-    O.vx[o] = rv[0];
-    O.vy[o] = rv[1];
-    O.vz[o] = rv[2];
-    O.id[o] = pid;
-    O.sp[o] = spid;
-    O.key[o] = bin_key(cn, cu, c, slot);
+    rec_store(O, o, false, rq, rv, pid, spid, bin_key(cn, cu, c, slot));
   }
 }
 
@@ -554,15 +601,8 @@ __global__ void k_bucketize(Ctx C, State<Real> S, BState<Real> O, const BucketRe
     return;
   }
   const size_t o = (size_t)b * cap + p;
-  O.x[o] = S.x[s];
-  O.y[o] = S.y[s];
-  O.z[o] = S.z[s];
-  O.vx[o] = S.vx[s];
-  O.vy[o] = S.vy[s];
-  O.vz[o] = S.vz[s];
-  O.id[o] = S.id[s];
-  O.sp[o] = S.sp[s];
-  O.key[o] = C.key[s];
+  const Real q[3] = {S.x[s], S.y[s], S.z[s]}, v[3] = {S.vx[s], S.vy[s], S.vz[s]};
+  rec_store(O, o, false, q, v, S.id[s], S.sp[s], C.key[s]);
 }
 
 // bucketed -> dense: each particle goes back to its slot of the last step's
@@ -577,18 +617,21 @@ __global__ void k_densify(Ctx C, BState<Real> I, State<Real> S, const TileDev *_
   const int n = min(I.cnt[blockIdx.x], cap);
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const size_t o = (size_t)blockIdx.x * cap + i;
-    const uint64_t key = I.key[o];
+    Real q_[3], v_[3];
+    int id, sp;
+    uint64_t key;
+    rec_load(I, o, onesp != 0, q_, v_, id, sp, key);
     const int slot = (int)(key & 0x7FFFFFFFull);
     const int cu = (int)((key >> 32) & 0x7FFFFFFFull);
-    const double q[3] = {(double)I.x[o], (double)I.y[o], (double)I.z[o]};
-    S.x[slot] = I.x[o];
-    S.y[slot] = I.y[o];
-    S.z[slot] = I.z[o];
-    S.vx[slot] = I.vx[o];
-    S.vy[slot] = I.vy[o];
-    S.vz[slot] = I.vz[o];
-    S.id[slot] = I.id[o];
-    S.sp[slot] = onesp ? 0 : I.sp[o];
+    const double q[3] = {(double)q_[0], (double)q_[1], (double)q_[2]};
+    S.x[slot] = q_[0];
+    S.y[slot] = q_[1];
+    S.z[slot] = q_[2];
+    S.vx[slot] = v_[0];
+    S.vy[slot] = v_[1];
+    S.vz[slot] = v_[2];
+    S.id[slot] = id;
+    S.sp[slot] = onesp ? 0 : sp;
     const int cn = next_cell<DIM>(R, &shift[T.rep * 3], cu, q);
     C.cnew[slot] = cn;
     C.loc[slot] = atomicAdd(&C.hist[cn], 1);

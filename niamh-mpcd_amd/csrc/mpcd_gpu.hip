@@ -1147,20 +1147,24 @@ static void dfree(mpcd_gpu_ctx *c, void *p) {
   (void)hipFree(p);
 }
 
-template <typename Real>
-static bool alloc_bstate(mpcd_gpu_ctx *c, BState<Real> &b, size_t n) {
-  void *old[] = {b.x, b.y, b.z, b.vx, b.vy, b.vz, b.id, b.sp, b.key};
+static bool alloc_bstate(mpcd_gpu_ctx *c, BState<float> &b, size_t n) {
+  void *old[] = {b.a, b.b, b.c, b.sp};
   for (void *p : old) dfree(c, p);
-  b.x = dalloc<Real>(c, n);
-  b.y = dalloc<Real>(c, n);
-  b.z = dalloc<Real>(c, n);
-  b.vx = dalloc<Real>(c, n);
-  b.vy = dalloc<Real>(c, n);
-  b.vz = dalloc<Real>(c, n);
-  b.id = dalloc<int32_t>(c, n);
+  b.a = dalloc<float4>(c, n);
+  b.b = dalloc<float4>(c, n);
+  b.c = dalloc<uint32_t>(c, n);
   b.sp = dalloc<int32_t>(c, n);
-  b.key = dalloc<uint64_t>(c, n);
-  return b.x && b.y && b.z && b.vx && b.vy && b.vz && b.id && b.sp && b.key;
+  return b.a && b.b && b.c && b.sp;
+}
+
+static bool alloc_bstate(mpcd_gpu_ctx *c, BState<double> &b, size_t n) {
+  void *old[] = {b.a, b.b, b.c, b.d};
+  for (void *p : old) dfree(c, p);
+  b.a = dalloc<double2>(c, n);
+  b.b = dalloc<double2>(c, n);
+  b.c = dalloc<double2>(c, n);
+  b.d = dalloc<int4>(c, n);
+  return b.a && b.b && b.c && b.d;
 }
 
 // device tables and buffers of the current bucket layout
@@ -1190,8 +1194,7 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     bool ok = true;
     for (int k = 0; k < 2; k++)
-      ok = ok && (c->precision == MPCD_FP64 ? alloc_bstate<double>(c, c->bd[k], slots)
-                                            : alloc_bstate<float>(c, c->bf[k], slots));
+      ok = ok && (c->precision == MPCD_FP64 ? alloc_bstate(c, c->bd[k], slots) : alloc_bstate(c, c->bf[k], slots));
     if (!ok) {
       c->bkt_slots = 0;
       return fail(c, MPCD_ENOMEM, "bucketed state (%zu slots)", slots);
