@@ -174,7 +174,7 @@ __device__ __forceinline__ int cell_bf(const int Li[3], const int pbc[3], int ce
   return bad ? -1 : cell_off + (a * ny + b) * nz + c;
 }
 
-template <typename Real, int OP, bool ONESP, int DIM, int PPT>
+template <typename Real, int OP, bool ONESP, int DIM, int PPT, bool GHOST>
 __global__ __launch_bounds__(kBktThreads) void k_bucket(
     Ctx C, BState<Real> I, BState<Real> O, const TileDev *__restrict__ bkts, const BucketRep *__restrict__ brep,
     const RepDev *__restrict__ reps, const mpcd_gpu_species *__restrict__ sps, const mpcd_gpu_wall *__restrict__ walls,
@@ -194,7 +194,6 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   __shared__ int32_t s_hkey[kBktHash], s_hcnt[kBktHash], s_hbase[kBktHash];
   __shared__ int32_t s_nwl, s_wlbase;  // wall-list entries of the workgroup, their first index
   __shared__ double s_ke[NT / 64];
-  __shared__ double s_cdr[OP == OP_SRD ? kBktMaxCells : 1][4];
 
   // the first loads are independent: issue them together.  The slot base
   // of the bucket is the sum of the counts of all earlier buckets: the
@@ -241,15 +240,6 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   for (int k = 0; k < 6; k++) box[k] = boxes[T.rep * 6 + k];
   const double dt = R.dt;
 
-  // the SRD draws of the tile's cells, loaded while the bucket is read
-  double cdr[4] = {0., 0., 0., 0.};
-  if (OP == OP_SRD && tid < ncl) {
-    const double *g = &Dr.cell[(size_t)(T.c0 + tid) * 4];
-    cdr[0] = g[0];
-    cdr[1] = g[1];
-    cdr[2] = g[2];
-    cdr[3] = g[3];
-  }
   // 1. the bucket: grid shift, escape repair, body force, cell in the tile
   Real rq[PPT][3], rv[PPT][3];
   int rid[PPT], rsp[PPT], rk[PPT], ra[PPT];
@@ -308,9 +298,6 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
       }
     }
   }
-  if (OP == OP_SRD && tid < ncl)
-#pragma unroll
-    for (int k = 0; k < 4; k++) s_cdr[tid][k] = cdr[k];
   __syncthreads();
   // 2. exclusive scan of the cell counts (one wavefront, two cells a lane)
   if (tid < 64) {
@@ -358,9 +345,9 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   // 3. one lane per cell: sums in list order and the collision's cell terms;
   //    the cell starts in the dense slot numbering
   for (int k = tid; k < ncl; k += NT) {
-    cell_terms<Real, OP, ONESP, DIM>(R, sps, wh, Dr, T.rep, T.c0 + k, s_cnt[k], s_cnt[k + 1], &s_v[0][0],
+    cell_terms<Real, OP, ONESP, DIM, GHOST>(R, sps, wh, Dr, T.rep, T.c0 + k, s_cnt[k], s_cnt[k + 1], &s_v[0][0],
                                      &s_q[0][0], OP == OP_SRD ? nullptr : &s_rv[0][0], ONESP ? nullptr : s_sp, CAP,
-                                     s_cd[k], OP == OP_SRD ? s_cdr[k] : nullptr);
+                                     s_cd[k]);
     start_out[T.c0 + k] = gbase + s_cnt[k];
   }
   if (tid == 0) start_out[T.c1] = gbase + s_cnt[ncl];

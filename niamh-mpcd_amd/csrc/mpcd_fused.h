@@ -241,8 +241,9 @@ __device__ __forceinline__ void next_bin(const Ctx &C, const RepDev &R, const do
 // arrays (component d of particle p at [d * stride + p]): FP64 sums in list
 // order (localPROP, mpc.c:64-216), ghost particles at phantom walls, and the
 // collision's cell data cd[12]: vcm[3], then the SRD matrix[9] or the
-// thermal-draw sum RS[3], W[3] (angular velocity) and CM[3]
-template <typename Real, int OP, bool ONESP, int DIM>
+// thermal-draw sum RS[3], W[3] (angular velocity) and CM[3].  GHOST = false
+// leaves out the ghost particles (no replica has phantom walls).
+template <typename Real, int OP, bool ONESP, int DIM, bool GHOST = true>
 __device__ __forceinline__ void cell_terms(const RepDev &R, const mpcd_gpu_species *__restrict__ sps,
                                            const WallHot *__restrict__ wh, const Draws &Dr, int rep, int c, int b,
                                            int e, const Real *sv, const Real *sq, const double *srv,
@@ -266,7 +267,7 @@ __device__ __forceinline__ void cell_terms(const RepDev &R, const mpcd_gpu_speci
       if (OP == OP_RAT) cm[d] /= mass;
     }
   // ghost particles at phantom walls (ghostPart, mpc.c:351-567)
-  if (R.anyphantom && np_ > 0) {
+  if (GHOST && R.anyphantom && np_ > 0) {
     int lc_ = c - R.cell_off;
     int a = lc_ / (R.ny * R.nz), bb = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
     const int nco = (DIM == 3) ? 8 : 4;

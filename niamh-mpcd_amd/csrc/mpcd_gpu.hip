@@ -1223,14 +1223,22 @@ static int bkt_limit(const mpcd_gpu_ctx *c) {
   return (c->bkt_caplim > 0 && c->bkt_caplim < cap) ? c->bkt_caplim : cap;
 }
 
-template <typename Real, int OP, bool ONESP, int DIM>
-static void launch_bucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+template <typename Real, int OP, bool ONESP, int DIM, bool GHOST>
+static void launch_bucket_g(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
   BState<Real> *B = bstates<Real>(c);
   const int nw3 = 3 * std::max(1, c->nwtot), nwm = std::max(1, c->nwtot);
-  k_bucket<Real, OP, ONESP, DIM, bkt_ppt(OP)><<<c->nbkt, kBktThreads, 0, c->stream>>>(
+  k_bucket<Real, OP, ONESP, DIM, bkt_ppt(OP), GHOST><<<c->nbkt, kBktThreads, 0, c->stream>>>(
       C, B[c->bcur], B[1 - c->bcur], c->d_bkts, c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
       c->box2 + par * 6 * c->nrep, D, c->shift2 + par * 3 * c->nrep, c->wqb2 + par * nw3,
       c->shift2 + (par ^ 1) * 3 * c->nrep, c->bbase, c->bstart[par], bkt_limit(c), wall_list<Real>(c));
+}
+
+// the ghost-particle code only where a replica has phantom walls (it sets
+// the kernel's register count)
+template <typename Real, int OP, bool ONESP, int DIM>
+static void launch_bucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  if (c->any_phantom) launch_bucket_g<Real, OP, ONESP, DIM, true>(c, C, D, par);
+  else launch_bucket_g<Real, OP, ONESP, DIM, false>(c, C, D, par);
 }
 
 template <typename Real, int OP, bool ONESP>
