@@ -531,15 +531,36 @@ __global__ __launch_bounds__(kBlock) void k_bucket_prep(
 // the wall list of a bucketed step: stream with the planar walls
 // (MPC_BCcollision, bc.c:1363-1496), unshifted cell with escape repair, next
 // cell and key, one global atomic per particle on its destination bucket
+// walls staged in LDS by k_bucket_walls (the wall search reads every wall
+// of the replica on each pass)
+constexpr int kWallStage = 32;
+
 template <typename Real, int DIM>
-__global__ void k_bucket_walls(Ctx C, WallList<Real> WL, BState<Real> O, const BucketRep *__restrict__ brep,
-                               const RepDev *__restrict__ reps, const mpcd_gpu_species *__restrict__ sps,
-                               const mpcd_gpu_wall *__restrict__ walls, const WallHot *__restrict__ wh,
-                               const double *__restrict__ boxes, Draws Dr, const double *__restrict__ wq_b,
-                               const double *__restrict__ shift_next, int cap, int cap_lim) {
+__global__ __launch_bounds__(kBlock) void k_bucket_walls(
+    Ctx C, WallList<Real> WL, BState<Real> O, const BucketRep *__restrict__ brep, const RepDev *__restrict__ reps,
+    const mpcd_gpu_species *__restrict__ sps, const mpcd_gpu_wall *__restrict__ walls,
+    const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr, const double *__restrict__ wq_b,
+    const double *__restrict__ shift_next, int cap, int cap_lim, int nwall) {
   if (C.flags[0] & kFlagPoison) return;
-  Dr.resolve();
   const int n = *WL.cnt;
+  if ((int)(blockIdx.x * blockDim.x) >= n) return;  // the whole block has no entry
+  __shared__ WallHot s_wh[kWallStage];
+  __shared__ mpcd_gpu_wall s_wall[kWallStage];
+  __shared__ double s_wqb[3 * kWallStage];
+  if (nwall <= kWallStage) {
+    for (int w = threadIdx.x; w < nwall; w += blockDim.x) {
+      s_wh[w] = wh[w];
+      s_wall[w] = walls[w];
+      s_wqb[3 * w] = wq_b[3 * w];
+      s_wqb[3 * w + 1] = wq_b[3 * w + 1];
+      s_wqb[3 * w + 2] = wq_b[3 * w + 2];
+    }
+    __syncthreads();
+    wh = s_wh;
+    walls = s_wall;
+    wq_b = s_wqb;
+  }
+  Dr.resolve();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int4 m = WL.meta[i];
     const int pid = m.x, c = m.y, slot = m.z, rep = m.w >> 8, spid = m.w & 255;

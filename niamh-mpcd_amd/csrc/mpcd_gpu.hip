@@ -1295,11 +1295,13 @@ static void run_step_bucket(mpcd_gpu_ctx *c) {
     if (c->dim == 3)
       k_bucket_walls<Real, 3><<<nb, kBlock, 0, c->stream>>>(
           C, wall_list<Real>(c), B[1 - c->bcur], c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
-          c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c));
+          c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c),
+          c->nwtot);
     else
       k_bucket_walls<Real, 2><<<nb, kBlock, 0, c->stream>>>(
           C, wall_list<Real>(c), B[1 - c->bcur], c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
-          c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c));
+          c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c),
+          c->nwtot);
   }
   c->bcur ^= 1;
   c->step++;
