@@ -275,6 +275,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--fill", type=float, default=0.0,
+                    help="fix the bucket fill of the run layout instead of autotuning it (reproducible profiles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -296,7 +298,11 @@ def main():
         for r in range(B):
             Q, V = initial_state(L, args.ppc, seed=100 + r)
             ctx.upload(r, Q, V)
-        tuned_ms = ctx.autotune(3)
+        if args.fill > 0:
+            ctx.set_layout(True, fill=args.fill)
+            tuned_ms = 0.0
+        else:
+            tuned_ms = ctx.autotune(3)
         print(f"[bench] fp32 batched: {B} x {n1} particles, tile fill {ctx.tile_fill():.2f} ({tuned_ms:.3f} ms/step "
               f"in the autotuner), warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
         r32 = timed_run(M, ctx, args.steps, args.warmup, B * n1)
@@ -310,7 +316,10 @@ def main():
         Q, V = initial_state(L, args.ppc, seed=7)
         ctx.upload(0, Q, V)
         k64 = args.fp64_steps or args.steps
-        ctx.autotune(3)
+        if args.fill > 0:
+            ctx.set_layout(True, fill=args.fill)
+        else:
+            ctx.autotune(3)
         print(f"[bench] fp64 plain: {n1} particles, tile fill {ctx.tile_fill():.2f}, warmup {args.warmup}, "
               f"steps {k64}", file=sys.stderr, flush=True)
         r64 = timed_run(M, ctx, k64, args.warmup, n1)
