@@ -246,6 +246,42 @@ def adjoint_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, angle=2.0, h=1e-11):
                 forward_s=t1 - t0, backward_s=t2 - t1)
 
 
+def lc_throughput(M, L=256, ppc=20, steps=40, warmup=5):
+    """the active-nematic configuration of the scope table (2D 256^2 x 20 ppc,
+    lc = 1, DIPOLE_DIR_SUM, U = 10, tumble 0.5, rfc 0.01, activity 0.05,
+    random isotropic orientations): particle-steps/s of the split path in FP64
+    and in the FP32-state form, with the per-kernel times"""
+    import importlib
+    V_ = importlib.import_module("niamh_mpcd_amd.validate")
+    out = {}
+    for name, prec in (("fp64", M.FP64), ("fp32", M.FP32)):
+        cfg = dict(dim=2, L=[L, L, 1], dt=0.1, kbt=1.0, rotang=ROTANG_130, tau=0.5, grav=[0, 0, 0], avv=[0, 0, 0],
+                   gndnst=float(ppc), rtech=16, lc=1, tstech=0, galinv=1,
+                   species=[V_.species(mfpot=10.0, tumble=0.5, rfc=0.01, act=0.05)], walls=V_.periodic_walls(2, (L, L)))
+        N = L * L * ppc
+        ctx = M.Context(cfg, N, seed=31, precision=prec)
+        rng = np.random.default_rng(31)
+        Q = np.zeros((N, 3))
+        Q[:, 0] = rng.uniform(0, L, N)
+        Q[:, 1] = rng.uniform(0, L, N)
+        V = np.zeros((N, 3))
+        V[:, :2] = rng.normal(size=(N, 2))
+        V -= V.mean(axis=0)
+        th = rng.uniform(0, 2 * math.pi, N)
+        U = np.zeros((N, 3))
+        U[:, 0] = np.cos(th)
+        U[:, 1] = np.sin(th)
+        ctx.upload(0, Q, V, U)
+        r = timed_run(M, ctx, steps, warmup, N)
+        ctx.close()
+        out[name] = dict(value=r["value"], ms_per_step=r["ms_per_step"],
+                         kernels_ms_per_step={k: v[0] / steps for k, v in r["timing"].items() if v[1]})
+    out["workload"] = (f"2D active nematic {L}x{L} x {ppc} ppc (lc=1, collision 16 DIPOLE_DIR_SUM, U=10, tumble 0.5, "
+                       f"rfc 0.01, activity 0.05), {steps} timed steps after {warmup}; split path")
+    out["unit"] = "particle-steps/s"
+    return out
+
+
 def copy_probe(mib):
     """a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
     calibrates the counters of the step kernels"""
@@ -274,6 +310,7 @@ def main():
                     help="skip the pathwise-adjoint check (2D SRD channel, gradient vs finite differences)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
+    ap.add_argument("--no-lc", action="store_true", help="skip the 2D active-nematic throughput line")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--fill", type=float, default=0.0,
                     help="fix the bucket fill of the run layout instead of autotuning it (reproducible profiles)")
@@ -348,6 +385,10 @@ def main():
             unit="particle-steps/s",
             viscosity_by_ppc={str(k): dict(mean=float(np.mean(v)), n=len(v)) for k, v in sorted(by_ppc.items())},
             per_replica=[[x["L"], x["ppc"], round(x["nu"], 4)] for x in e["per_replica"]])
+    lc = None
+    if not args.no_lc and args.only == "both":
+        print("[bench] 2D active nematic (LC) throughput", file=sys.stderr, flush=True)
+        lc = lc_throughput(M)
     adjoint = None
     if not args.no_adjoint and args.only == "both":
         print("[bench] pathwise adjoint vs finite differences", file=sys.stderr, flush=True)
@@ -383,6 +424,7 @@ def main():
         "fp32_validation": validation,
         "ensemble_mpcat_a": ensemble,
         "adjoint": adjoint,
+        "lc_active_nematic": lc,
     }
     if "fp64" in result and head_key != "fp64":
         r64, n64, _ = result["fp64"]

@@ -700,7 +700,7 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
       k_avorder_part<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, c->d_avpart);
       k_avorder_final<<<1, 64, 0, c->stream>>>(C, c->d_pblk, (int)c->pblk.size(), c->d_avpart);
     }
-    k_lc<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+    k_lc<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D);
   }
   {
     ProfScope ps(c, F_COLLIDE);

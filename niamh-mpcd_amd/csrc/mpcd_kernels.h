@@ -774,17 +774,23 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
   }
 }
 
+// LCcollision (lc.c:427-544) and jefferysTorque + larsonRotRate
+// (lc.c:559-718) with one lane per particle: the cell's director, order,
+// mean field and velocity gradient are recomputed by each lane with the
+// per-cell arithmetic of the reference (the mean field summed over the cell
+// in list order), and the Metropolis draws are keyed by slot, so the result
+// does not depend on the mapping
 template <typename Real>
-__global__ void k_lc(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+__global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr) {
   Dr.resolve();
-  const BlkDev B = cblk[blockIdx.x];
-  int c = B.beg + threadIdx.x;
-  if (c >= B.end) return;
+  const BlkDev B = pblk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
   const RepDev &R = C.rep[B.rep];
-  int b = C.start[c], e = C.start[c + 1];
+  const int c = S.cst[t];
+  const int b = C.start[c], e = C.start[c + 1];
   if (e - b <= 1) return;
   const int dim = R.dim;
-  // LCcollision, lc.c:427-544
   double DIR[3] = {0, 0, 0}, rotAx[3], xaxis[3] = {1.0, 0.0, 0.0};
   double Sv = (R.lc == MPCD_LCL) ? C.cell.S[c] : C.sg[B.rep];
   for (int i = 0; i < dim; i++) DIR[i] = C.cell.dir[c * 3 + i];
@@ -792,24 +798,22 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
   double rotAngle = -1. * abs_angle(DIR, xaxis, dim);
   normn(rotAx, 3);
   double avM = 0.0;
-  for (int t = b; t < e; t++) avM += C.sp[S.sp[t]].mfpot;
+  for (int t2 = b; t2 < e; t2++) avM += C.sp[S.sp[t2]].mfpot;
   avM /= (float)(e - b);
   double effM = avM * 0.5 * (double)dim;
-  double dU[3] = {0, 0, 0};
-  for (int t = b; t < e; t++) {
-    double U[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, T[3];
-    double rfc = C.sp[S.sp[t]].rfc;
+  const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+  {
+    double U[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, T[3], dU[3] = {0, 0, 0};
     for (int i = 0; i < dim; i++) dU[i] = U[i];
     ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, R.kbt, Sv, effM);
     for (int i = 0; i < dim; i++) dU[i] = (U[i] - dU[i]) / R.dt;
     cross3(U, dU, T);
-    for (int i = 0; i < 3; i++) T[i] *= rfc;
+    for (int i = 0; i < 3; i++) T[i] *= sp.rfc;
     S.ux[t] = (Real)U[0];
     S.uy[t] = (Real)U[1];
     S.uz[t] = (Real)U[2];
     for (int i = 0; i < 3; i++) C.T[(size_t)t * 3 + i] = T[i];
   }
-  // jefferysTorque + larsonRotRate, lc.c:559-718
   const double *E = &C.cell.E[(size_t)c * 9];
   double D[3][3], W[3][3];
   for (int i = 0; i < 3; i++)
@@ -817,28 +821,25 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
       D[j][i] = 0.5 * (E[i * 3 + j] + E[j * 3 + i]);
       W[j][i] = 0.5 * (E[i * 3 + j] - E[j * 3 + i]);
     }
-  for (int t = b; t < e; t++) {
-    const mpcd_gpu_species &sp = C.sp[S.sp[t]];
-    double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, dudt[3], w[3];
-    double uuuD[3] = {0, 0, 0}, uw[3] = {0, 0, 0}, uD[3] = {0, 0, 0};
-    for (int j = 0; j < 3; j++) {
-      for (int i = 0; i < 3; i++) {
-        for (int k = 0; k < 3; k++) uuuD[j] += u[j] * (u[i] * D[i][k] * u[k]);
-        uD[j] += u[i] * D[i][j];
-        uw[j] += u[i] * W[i][j];
-      }
-      dudt[j] = uw[j] + sp.tumble * (uD[j] - uuuD[j]);
+  double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, dudt[3], w[3];
+  double uuuD[3] = {0, 0, 0}, uw[3] = {0, 0, 0}, uD[3] = {0, 0, 0};
+  for (int j = 0; j < 3; j++) {
+    for (int i = 0; i < 3; i++) {
+      for (int k = 0; k < 3; k++) uuuD[j] += u[j] * (u[i] * D[i][k] * u[k]);
+      uD[j] += u[i] * D[i][j];
+      uw[j] += u[i] * W[i][j];
     }
-    cross3(u, dudt, w);
-    for (int i = 0; i < 3; i++) w[i] *= sp.chihi;
-    for (int i = 0; i < 3; i++) w[i] *= R.dt;
-    double theta = lengthn(w, 3);
-    normn(w, 3);
-    rodrigues(u, w, theta);
-    S.ux[t] = (Real)u[0];
-    S.uy[t] = (Real)u[1];
-    S.uz[t] = (Real)u[2];
+    dudt[j] = uw[j] + sp.tumble * (uD[j] - uuuD[j]);
   }
+  cross3(u, dudt, w);
+  for (int i = 0; i < 3; i++) w[i] *= sp.chihi;
+  for (int i = 0; i < 3; i++) w[i] *= R.dt;
+  double theta = lengthn(w, 3);
+  normn(w, 3);
+  rodrigues(u, w, theta);
+  S.ux[t] = (Real)u[0];
+  S.uy[t] = (Real)u[1];
+  S.uz[t] = (Real)u[2];
 }
 
 // ---------------------------------------------------------------------------
