@@ -60,7 +60,25 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
+    build_host(verbose)
     return LIB_PATH
+
+
+HOST_PATH = os.path.join(HERE, "_build", "mpcd_host")
+
+
+def build_host(verbose: bool = False) -> str:
+    """Compile the C host (host/*.c) against the built library with gcc."""
+    cc = shutil.which("gcc") or "cc"
+    host = os.path.join(HERE, "host")
+    cmd = [cc, "-O2", "-std=c11", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-I", INCLUDE,
+           os.path.join(host, "mpcd_host.c"), os.path.join(host, "json.c"),
+           "-L", os.path.dirname(LIB_PATH), "-lmpcd_gpu", "-Wl,-rpath,$ORIGIN", "-lm", "-o", HOST_PATH + ".tmp"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(HOST_PATH + ".tmp", HOST_PATH)
+    return HOST_PATH
 
 
 class Species(C.Structure):

@@ -1,0 +1,410 @@
+/*
+ * mpcd_host.c -- a C host for the MI355X solvent step: reads the reference's
+ * input.json, sets up the particles and runs the reference's warm-up and
+ * temporal loops (mpcd/mpcd.c:255-300) through the C-ABI of mpcd_gpu.h.
+ *
+ *   mpcd_host input.json [-o DIR] [--fp32] [--replicas N] [--every K] [--steps N] [--seed S] [--no-graph]
+ *
+ * JSON keys and defaults follow readJson (mpcd/subroutines/read.c:973-1804)
+ * for the options of this path: domain, kbt, dt, simSteps, warmUp, galInv,
+ * tsTech, rTech/collOp, lc, tau, rotAng, grav, seed, domainWalls, species
+ * (mass, pop/dens, vDist, rfc, tumble, shearSusc, act, mfpot, sigWidth,
+ * sigPos, minActRatio, damp) and planar BC entries (aInv, Q, P, R, DN, DT,
+ * DVN, DVT, DVxyz, MVN, MVT, MUN, MUT, MUxyz, DUxyz, V, L, inv, phantom,
+ * colType, interSRD, homeotropic, planar).  Options outside the path (MD,
+ * swimmers, checkpoints, curved or wavy walls) stop with a message, as the
+ * reference stops on an unknown option.
+ *
+ * Output: DIR/solvent.dat with one line per output step: the temporal
+ * loop's step counter (runtime, 0..simSteps, mpcd.c:291), the
+ * temperature TEMP (therm.c:209-235, about the initial mean velocity AVV),
+ * the kinetic energy and the total momentum; a timing line on stdout.
+ * Like the reference, each loop runs its step count plus one
+ * (mpcd.c:264, 294).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include "json.h"
+#include "mpcd_gpu.h"
+
+/* splitmix64: the host's initial-state generator */
+static uint64_t rng_state = 0x9E3779B97F4A7C15ull;
+static uint64_t rng_next(void) {
+    uint64_t z = (rng_state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double rng_u(void) { return (double)(rng_next() >> 11) * (1.0 / 9007199254740992.0); }
+static double rng_n(void) {
+    double u1 = 1.0 - rng_u(), u2 = rng_u();
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+static void die(const char *fmt, const char *arg) {
+    fprintf(stderr, "mpcd_host: ");
+    fprintf(stderr, fmt, arg);
+    fprintf(stderr, "\n");
+    exit(1);
+}
+
+static void check(mpcd_gpu_ctx *ctx, mpcd_status st, const char *what) {
+    if (st == MPCD_OK) return;
+    fprintf(stderr, "mpcd_host: %s failed (status %d): %s\n", what, (int)st, ctx ? mpcd_gpu_last_error(ctx) : "");
+    exit(1);
+}
+
+/* keys this host rejects: they select parts of the reference outside the solvent path */
+static void reject_unsupported(const json *J) {
+    const char *strs[] = {"checkpointIn", "mdIn"};
+    for (int k = 0; k < 2; k++) {
+        const char *v = json_str(J, strs[k], "");
+        if (v[0]) die("\"%s\" (outside the solvent path) is not supported", strs[k]);
+    }
+    const char *ints[] = {"noHI", "incomp", "multiphase"};
+    for (int k = 0; k < 3; k++)
+        if (json_int(J, ints[k], 0)) die("\"%s\" is not supported", ints[k]);
+    const json *sw = json_get(J, "swimmers");
+    if (!sw) sw = json_get(J, "swimmer");
+    if (sw && sw->type == JSON_ARR && sw->n > 0) die("%s are not supported", "swimmers");
+}
+
+/* BC entry of the JSON (read.c:1136-1440) as a planar wall */
+static void read_wall(const json *o, int nspec, mpcd_gpu_wall *w, int index) {
+    char msg[64];
+    memset(w, 0, sizeof *w);
+    double ainv[3], P[4] = {1, 1, 1, 1}, v3[3];
+    if (json_nums(o, "aInv", ainv, 3) != 3) {
+        snprintf(msg, sizeof msg, "%d", index);
+        die("BC %s: aInv must have 3 components", msg);
+    }
+    for (int j = 0; j < 3; j++) w->A[j] = (fabs(ainv[j]) >= 1e-8) ? 1.0 / ainv[j] : 0.0;
+    if (json_nums(o, "P", P, 4) != 4) {
+        snprintf(msg, sizeof msg, "%d", index);
+        die("BC %s: P must have 4 elements", msg);
+    }
+    double B[3] = {0, 0, 0};
+    json_nums(o, "wavy", B, 3);
+    int naxis = 0;
+    for (int j = 0; j < 3; j++) naxis += fabs(w->A[j]) >= 1e-8;
+    const int planar = fabs(B[0]) <= 1e-8 && fabs(P[0] - 1) <= 1e-8 && fabs(P[1] - 1) <= 1e-8 &&
+                       fabs(P[2] - 1) <= 1e-8 && naxis == 1;
+    if (!planar) {
+        snprintf(msg, sizeof msg, "%d", index);
+        die("BC %s is not an axis-aligned plane (curved and wavy walls are outside this path)", msg);
+    }
+    if (json_nums(o, "Q", v3, 3) == 3) memcpy(w->Q, v3, sizeof v3);
+    if (json_nums(o, "V", v3, 3) == 3) memcpy(w->V, v3, sizeof v3);
+    if (json_nums(o, "L", v3, 3) == 3) memcpy(w->L, v3, sizeof v3);
+    if (json_nums(o, "DVxyz", v3, 3) == 3) memcpy(w->DVxyz, v3, sizeof v3);
+    if (json_nums(o, "MUxyz", v3, 3) == 3) memcpy(w->MUxyz, v3, sizeof v3);
+    if (json_nums(o, "DUxyz", v3, 3) == 3) memcpy(w->DUxyz, v3, sizeof v3);
+    w->R = json_num(o, "R", 2.0);
+    w->DN = json_num(o, "DN", 1.0);
+    w->DT = json_num(o, "DT", 0.0);
+    w->DVN = json_num(o, "DVN", 0.0);
+    w->DVT = json_num(o, "DVT", 0.0);
+    w->MVN = json_num(o, "MVN", 1.0);
+    w->MVT = json_num(o, "MVT", 1.0);
+    w->MUN = json_num(o, "MUN", 1.0);
+    w->MUT = json_num(o, "MUT", 1.0);
+    w->INV = json_int(o, "inv", 0);
+    w->PHANTOM = json_int(o, "phantom", 0);
+    w->COLL_TYPE = json_int(o, "colType", 1);
+    if (json_int(o, "homeotropic", 0) == 1) {
+        w->MUN = 1;
+        w->MUT = 0;
+    } else if (json_int(o, "planar", 0) == 1) {
+        w->MUN = 0;
+        w->MUT = 1;
+    }
+    double inter[MPCD_MAXSPEC];
+    const int ni = json_nums(o, "interSRD", inter, MPCD_MAXSPEC);
+    w->inter_mask = 0;
+    for (int s = 0; s < nspec; s++)
+        if (ni < 0 || (s < ni && inter[s] != 0.0)) w->inter_mask |= 1 << s;
+}
+
+/* domainWalls override (read.c:1441-1583): periodic (1) or solid (2) box walls */
+static int domain_walls(int dim, const int L[3], int mode, int nspec, mpcd_gpu_wall *out) {
+    int n = 0;
+    for (int ax = 0; ax < dim; ax++)
+        for (int side = 0; side < 2; side++) {
+            mpcd_gpu_wall *w = &out[n++];
+            memset(w, 0, sizeof *w);
+            w->A[ax] = side ? -1.0 : 1.0;
+            w->Q[ax] = side ? (double)L[ax] : 0.0;
+            w->R = 0.0;
+            w->MUN = w->MUT = 1.0;
+            w->COLL_TYPE = 1;
+            w->inter_mask = (1 << nspec) - 1;
+            if (mode == 1) {
+                w->DN = (double)L[ax];
+                w->MVN = w->MVT = 1.0;
+            } else {
+                w->DN = 0.0;
+                w->MVN = w->MVT = -1.0;
+                w->PHANTOM = 1;
+            }
+        }
+    return n;
+}
+
+int main(int argc, char **argv) {
+    const char *input = NULL, *outdir = ".";
+    int fp32 = 0, nrep = 1, every = 0, steps_override = -1, use_graph = 1, dump = 0;
+    long seed_override = -1;
+    for (int a = 1; a < argc; a++) {
+        if (!strcmp(argv[a], "-o") && a + 1 < argc) outdir = argv[++a];
+        else if (!strcmp(argv[a], "--fp32")) fp32 = 1;
+        else if (!strcmp(argv[a], "--replicas") && a + 1 < argc) nrep = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "--every") && a + 1 < argc) every = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "--steps") && a + 1 < argc) steps_override = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "--seed") && a + 1 < argc) seed_override = atol(argv[++a]);
+        else if (!strcmp(argv[a], "--no-graph")) use_graph = 0;
+        else if (!strcmp(argv[a], "--dump-params")) dump = 1;
+        else if (argv[a][0] != '-' && !input) input = argv[a];
+        else die("unknown argument %s", argv[a]);
+    }
+    if (!input) {
+        fprintf(stderr,
+                "usage: mpcd_host input.json [-o DIR] [--fp32] [--replicas N] [--every K] [--steps N] [--seed S] "
+                "[--no-graph]\n");
+        return 2;
+    }
+    if (nrep < 1) die("%s", "--replicas must be at least 1");
+    char err[256];
+    json *J = json_load(input, err, sizeof err);
+    if (!J) die("%s", err);
+    if (J->type != JSON_OBJ) die("%s", "the input is not a JSON object");
+    reject_unsupported(J);
+
+    /* simulation-scale inputs (read.c:1023-1114) */
+    mpcd_gpu_params p;
+    memset(&p, 0, sizeof p);
+    double dom[3] = {0, 0, 0};
+    const int dim = json_nums(J, "domain", dom, 3);
+    if (dim != 2 && dim != 3) die("%s", "domain must have 2 or 3 entries");
+    p.dim = dim;
+    for (int d = 0; d < 3; d++) p.L[d] = d < dim ? (int)dom[d] : 1;
+    p.kbt = json_num(J, "kbt", 1.0);
+    p.dt = json_num(J, "dt", 0.1);
+    int sim_steps = json_int(J, "simSteps", 2000);
+    const int warm = json_int(J, "warmUp", 0);
+    if (steps_override >= 0) sim_steps = steps_override;
+    p.galinv = json_int(J, "galInv", 1);
+    p.tstech = json_int(J, "tsTech", 0);
+    /* getJObjIntMultiple (cJson.c:265-279): each tag that differs from the
+     * default overrides, so collOp wins over rTech when both are set */
+    p.rtech = 2;
+    const char *optags[2] = {"rTech", "collOp"};
+    for (int k = 0; k < 2; k++) {
+        const int v = json_int(J, optags[k], 2);
+        if (v != 2) p.rtech = v;
+    }
+    p.lc = json_int(J, "lc", 0);
+    p.tau = json_num(J, "tau", 0.5);
+    p.rotang = json_num(J, "rotAng", 1.570796);
+    double g[3] = {0, 0, 0};
+    json_nums(J, "grav", g, 3);
+    memcpy(p.grav, g, sizeof g);
+    p.precision = fp32 ? MPCD_FP32 : MPCD_FP64;
+    long seed = json_int(J, "seed", 0);
+    if (seed_override >= 0) seed = seed_override;
+    if (seed == 0) seed = (long)time(NULL);
+    rng_state ^= (uint64_t)seed * 0xD1B54A32D192ED03ull;
+
+    /* species (read.c:1598-1680) */
+    const json *spa = json_get(J, "species");
+    p.nspec = (spa && spa->type == JSON_ARR && spa->n > 0) ? spa->n : 1;
+    if (p.nspec > MPCD_MAXSPEC) die("%s", "too many species");
+    const double vol = (double)p.L[0] * p.L[1] * (dim == 3 ? p.L[2] : 1);
+    int64_t pop[MPCD_MAXSPEC], ntot = 0;
+    int vdist[MPCD_MAXSPEC];
+    for (int s = 0; s < p.nspec; s++) {
+        const json *o = (spa && spa->type == JSON_ARR && spa->n > 0) ? &spa->items[s] : NULL;
+        mpcd_gpu_species *sp = &p.sp[s];
+        sp->mass = json_num(o, "mass", 1.0);
+        const double dens = json_num(o, "dens", -1.0);
+        pop[s] = dens < 0 ? (int64_t)json_int(o, "pop", (int)(vol * 20)) : (int64_t)(vol * dens);
+        vdist[s] = json_int(o, "vDist", 0);
+        sp->vdist = vdist[s];
+        sp->rfc = json_num(o, "rfc", 0.01);
+        sp->tumble = json_num(o, "tumble", 2.0);
+        sp->chihi = json_num(o, "shearSusc", 0.5);
+        sp->act = json_num(o, "act", 0.05);
+        sp->mfpot = json_num(o, "mfpot", 10.0);
+        sp->sigwidth = json_num(o, "sigWidth", 1.0);
+        sp->sigpos = json_num(o, "sigPos", sp->sigwidth);
+        sp->minactratio = json_num(o, "minActRatio", 0.0);
+        sp->damp = json_num(o, "damp", 0.0);
+        ntot += pop[s];
+    }
+    if (ntot <= 0) die("%s", "no particles");
+
+    /* boundaries: the BC list, or the domainWalls override */
+    const json *bca = json_get(J, "BC");
+    const int dwalls = json_int(J, "domainWalls", 0);
+    if (bca && bca->type == JSON_ARR && bca->n > 0 && dwalls == 0) {
+        if (bca->n > MPCD_MAXWALL) die("%s", "too many BC entries");
+        for (int w = 0; w < bca->n; w++) read_wall(&bca->items[w], p.nspec, &p.wall[w], w);
+        p.nwall = bca->n;
+    } else {
+        p.nwall = domain_walls(dim, p.L, dwalls == 2 ? 2 : 1, p.nspec, p.wall);
+    }
+
+    if (dump) {
+        /* the parameters as read, without touching the GPU */
+        printf("{\"dim\": %d, \"L\": [%d, %d, %d], \"dt\": %.17g, \"kbt\": %.17g, \"rotang\": %.17g, "
+               "\"tau\": %.17g, \"grav\": [%.17g, %.17g, %.17g], \"rtech\": %d, \"lc\": %d, \"tstech\": %d, "
+               "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"seed\": %ld, \"species\": [",
+               p.dim, p.L[0], p.L[1], p.L[2], p.dt, p.kbt, p.rotang, p.tau, p.grav[0], p.grav[1], p.grav[2], p.rtech,
+               p.lc, p.tstech, p.galinv, sim_steps, warm, seed);
+        for (int s = 0; s < p.nspec; s++) {
+            const mpcd_gpu_species *sp = &p.sp[s];
+            printf("%s{\"pop\": %lld, \"mass\": %.17g, \"vdist\": %d, \"rfc\": %.17g, \"tumble\": %.17g, "
+                   "\"chihi\": %.17g, \"act\": %.17g, \"mfpot\": %.17g, \"sigwidth\": %.17g, \"sigpos\": %.17g, "
+                   "\"minactratio\": %.17g, \"damp\": %.17g}",
+                   s ? ", " : "", (long long)pop[s], sp->mass, sp->vdist, sp->rfc, sp->tumble, sp->chihi, sp->act,
+                   sp->mfpot, sp->sigwidth, sp->sigpos, sp->minactratio, sp->damp);
+        }
+        printf("], \"walls\": [");
+        for (int w = 0; w < p.nwall; w++) {
+            const mpcd_gpu_wall *W = &p.wall[w];
+            printf("%s{\"A\": [%.17g, %.17g, %.17g], \"Q\": [%.17g, %.17g, %.17g], \"R\": %.17g, \"DN\": %.17g, "
+                   "\"MVN\": %.17g, \"MVT\": %.17g, \"MUN\": %.17g, \"MUT\": %.17g, \"V\": [%.17g, %.17g, %.17g], "
+                   "\"INV\": %d, \"PHANTOM\": %d, \"inter_mask\": %d}",
+                   w ? ", " : "", W->A[0], W->A[1], W->A[2], W->Q[0], W->Q[1], W->Q[2], W->R, W->DN, W->MVN, W->MVT,
+                   W->MUN, W->MUT, W->V[0], W->V[1], W->V[2], W->INV, W->PHANTOM, W->inter_mask);
+        }
+        printf("]}\n");
+        json_free(J);
+        free(J);
+        return 0;
+    }
+
+    /* initial state: uniform positions, velocities by vDist (4 Gaussian,
+     * otherwise random direction at the thermal speed), zero net momentum
+     * (galileantrans, mtools.c:673), random orientations */
+    const size_t n = (size_t)ntot;
+    const size_t nall = n * (size_t)nrep;
+    double *Q = (double *)calloc(3 * nall, sizeof(double));
+    double *V = (double *)calloc(3 * nall, sizeof(double));
+    double *U = (double *)calloc(3 * nall, sizeof(double));
+    int32_t *spid = (int32_t *)calloc(n, sizeof(int32_t));
+    int *ccount = (int *)calloc((size_t)p.L[0] * p.L[1] * p.L[2], sizeof(int));
+    mpcd_gpu_params *pr = (mpcd_gpu_params *)calloc((size_t)nrep, sizeof(mpcd_gpu_params));
+    int64_t *nr = (int64_t *)calloc((size_t)nrep, sizeof(int64_t));
+    if (!Q || !V || !U || !spid || !ccount || !pr || !nr) die("%s", "out of host memory");
+    for (int r = 0; r < nrep; r++) {
+        double *Qr = Q + 3 * n * r, *Vr = V + 3 * n * r, *Ur = U + 3 * n * r;
+        size_t i = 0;
+        double mom[3] = {0, 0, 0}, mtot = 0.0;
+        memset(ccount, 0, sizeof(int) * (size_t)p.L[0] * p.L[1] * p.L[2]);
+        for (int s = 0; s < p.nspec; s++)
+            for (int64_t k = 0; k < pop[s]; k++, i++) {
+                const double m = p.sp[s].mass, sig = sqrt(p.kbt / m);
+                spid[i] = s;
+                for (int d = 0; d < 3; d++) Qr[3 * i + d] = d < dim ? rng_u() * p.L[d] : 0.0;
+                if (vdist[s] == 4) {
+                    for (int d = 0; d < dim; d++) Vr[3 * i + d] = sig * rng_n();
+                } else {
+                    double u[3] = {0, 0, 0}, nn = 0.0;
+                    for (int d = 0; d < dim; d++) {
+                        u[d] = rng_n();
+                        nn += u[d] * u[d];
+                    }
+                    nn = sqrt(nn);
+                    for (int d = 0; d < dim; d++) Vr[3 * i + d] = sqrt((double)dim) * sig * u[d] / nn;
+                }
+                double o[3] = {0, 0, 0}, on = 0.0;
+                for (int d = 0; d < dim; d++) {
+                    o[d] = rng_n();
+                    on += o[d] * o[d];
+                }
+                on = sqrt(on);
+                for (int d = 0; d < dim; d++) Ur[3 * i + d] = o[d] / on;
+                for (int d = 0; d < 3; d++) mom[d] += m * Vr[3 * i + d];
+                mtot += m;
+                const int a = (int)Qr[3 * i], b = (int)Qr[3 * i + 1], c = dim == 3 ? (int)Qr[3 * i + 2] : 0;
+                ccount[(a * p.L[1] + b) * p.L[2] + c]++;
+            }
+        for (size_t k = 0; k < n; k++)
+            for (int d = 0; d < 3; d++) Vr[3 * k + d] -= mom[d] / mtot;
+        /* GnDNST: mean population of the occupied cells (globalDensities, init.c:995-1018) */
+        double occ = 0.0;
+        for (int k = 0; k < p.L[0] * p.L[1] * p.L[2]; k++) occ += ccount[k] > 0;
+        pr[r] = p;
+        pr[r].gndnst = (double)n / occ;
+        nr[r] = ntot;
+    }
+    mpcd_gpu_ctx *ctx = NULL;
+    check(NULL, mpcd_gpu_create(pr, nrep, nr, (uint64_t)seed, &ctx), "mpcd_gpu_create");
+    for (int r = 0; r < nrep; r++)
+        check(ctx, mpcd_gpu_upload(ctx, r, Q + 3 * n * r, V + 3 * n * r, p.lc ? U + 3 * n * r : NULL,
+                                   p.nspec > 1 ? spid : NULL),
+              "mpcd_gpu_upload");
+
+    /* the reference's loops: warmUp + 1 and simSteps + 1 steps, outputs in between */
+    char path[4096];
+    mkdir(outdir, 0755);
+    snprintf(path, sizeof path, "%s/solvent.dat", outdir);
+    FILE *fo = fopen(path, "w");
+    if (!fo) die("cannot write %s", path);
+    fprintf(fo, "# step  KBT(replica 0)  KE  Px  Py  Pz   (%s state, %d replica%s of %lld particles)\n",
+            fp32 ? "FP32" : "FP64", nrep, nrep > 1 ? "s" : "", (long long)ntot);
+    const int total = sim_steps + 1;
+    if (every <= 0) every = total > 10 ? total / 10 : 1;
+    if (warm > 0) check(ctx, mpcd_gpu_run(ctx, warm + 1, use_graph), "warm-up");
+    check(ctx, mpcd_gpu_synchronize(ctx), "synchronize");
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    double busy = 0.0;
+    for (int done = 0; done < total;) {
+        const int k = (total - done) < every ? (total - done) : every;
+        struct timespec a, b;
+        clock_gettime(CLOCK_MONOTONIC, &a);
+        check(ctx, mpcd_gpu_run(ctx, k, use_graph), "mpcd_gpu_run");
+        check(ctx, mpcd_gpu_synchronize(ctx), "synchronize");
+        clock_gettime(CLOCK_MONOTONIC, &b);
+        busy += (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
+        done += k;
+        check(ctx, mpcd_gpu_download(ctx, 0, Q, V, NULL), "mpcd_gpu_download");
+        double ke = 0.0, P[3] = {0, 0, 0};
+        for (size_t i = 0; i < n; i++) {
+            const double m = p.sp[p.nspec > 1 ? spid[i] : 0].mass;
+            double v2 = 0.0;
+            for (int d = 0; d < 3; d++) {
+                v2 += (V[3 * i + d] - p.avv[d]) * (V[3 * i + d] - p.avv[d]);
+                P[d] += m * V[3 * i + d];
+            }
+            ke += m * v2;
+        }
+        fprintf(fo, "%d %.12g %.12g %.6e %.6e %.6e\n", done - 1, ke / (dim * (double)n), 0.5 * ke, P[0], P[1],
+                P[2]);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    fclose(fo);
+    printf("{\"steps\": %d, \"replicas\": %d, \"particles_per_replica\": %lld, \"precision\": \"%s\", "
+           "\"wall_s\": %.3f, \"step_s\": %.3f, \"particle_steps_per_s\": %.4g, \"output\": \"%s\"}\n",
+           total, nrep, (long long)ntot, fp32 ? "fp32" : "fp64",
+           (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec), busy,
+           (double)total * (double)ntot * nrep / busy, path);
+    mpcd_gpu_destroy(ctx);
+    json_free(J);
+    free(J);
+    free(Q);
+    free(V);
+    free(U);
+    free(spid);
+    free(ccount);
+    free(pr);
+    free(nr);
+    return 0;
+}

@@ -1,0 +1,162 @@
+"""The C host (niamh-mpcd_amd/host): reading of the reference's input.json.
+
+CPU: `mpcd_host --dump-params` prints the parameters it read without
+touching the GPU.  The checks cover the reference's defaults
+(read.c:1058-1114, 1153-1440, 1609-1680), the aInv -> A inversion, the
+domainWalls override (read.c:1441-1583), the anchoring overrides, species
+interaction masks, densities, repeated "comment" keys and the options this
+path rejects.  GPU: the host runs a periodic box, whose momentum the body force sets in
+closed form, and a channel between bounce-back walls."""
+import json
+import os
+import subprocess
+
+import pytest
+
+import gpu_util as U
+
+
+def host():
+    M = U.package()
+    path = os.path.join(os.path.dirname(M.LIB_PATH), "mpcd_host")
+    if not os.path.exists(path):
+        pytest.skip("C host not built (run __graft_entry__.build())")
+    return path
+
+
+def run_dump(tmp_path, cfg, text=None):
+    f = tmp_path / "input.json"
+    f.write_text(text if text is not None else json.dumps(cfg))
+    r = subprocess.run([host(), str(f), "--dump-params"], capture_output=True, text=True)
+    return r
+
+
+def dump(tmp_path, cfg=None, text=None):
+    r = run_dump(tmp_path, cfg, text)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def test_defaults_and_duplicate_keys(tmp_path):
+    text = '''{
+        "comment": "repeated comment keys, as in the reference's sample inputs",
+        "comment": "the first of each key is used",
+        "domain": [16, 12, 8],
+        "species": [{"comment": "x", "pop": 1000}]
+    }'''
+    d = dump(tmp_path, text=text)
+    assert d["dim"] == 3 and d["L"] == [16, 12, 8]
+    assert (d["kbt"], d["dt"], d["tau"], d["rotang"]) == (1.0, 0.1, 0.5, 1.570796)
+    assert (d["rtech"], d["lc"], d["tstech"], d["galinv"], d["simSteps"], d["warmUp"]) == (2, 0, 0, 1, 2000, 0)
+    sp = d["species"][0]
+    assert sp["pop"] == 1000 and sp["mass"] == 1.0 and sp["vdist"] == 0
+    assert (sp["rfc"], sp["tumble"], sp["chihi"], sp["act"], sp["mfpot"]) == (0.01, 2.0, 0.5, 0.05, 10.0)
+    assert (sp["sigwidth"], sp["sigpos"], sp["minactratio"], sp["damp"]) == (1.0, 1.0, 0.0, 0.0)
+    # no BC list: periodic box walls (domainWalls override)
+    assert len(d["walls"]) == 6
+    for k, w in enumerate(d["walls"]):
+        ax, side = divmod(k, 2)
+        assert w["A"][ax] == (-1.0 if side else 1.0) and w["DN"] == [16, 12, 8][ax]
+        assert w["MVN"] == 1.0 and w["MVT"] == 1.0 and w["PHANTOM"] == 0
+
+
+def test_density_solid_domain_walls_and_collop(tmp_path):
+    d = dump(tmp_path, dict(domain=[10, 6], rTech=0, collOp=3, domainWalls=2, seed=5,
+                            species=[dict(dens=7.0, mass=2.0, vDist=4)]))
+    assert d["dim"] == 2 and d["rtech"] == 3  # both set: the later tag, collOp, wins (cJson.c:265-279)
+    assert d["seed"] == 5 and d["species"][0]["pop"] == 420 and d["species"][0]["mass"] == 2.0
+    assert len(d["walls"]) == 4
+    for w in d["walls"]:
+        assert w["DN"] == 0.0 and w["MVN"] == -1.0 and w["MVT"] == -1.0 and w["PHANTOM"] == 1
+
+
+def test_bc_entries(tmp_path):
+    bc = [dict(aInv=[0, 0.5, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], R=0, DN=0, MVN=-1, MVT=-1, phantom=1,
+               homeotropic=1, V=[0.2, 0, 0], interSRD=[1, 0]),
+          dict(aInv=[0, -1, 0], Q=[0, 9, 0], P=[1, 1, 1, 1], planar=1, inv=1)]
+    d = dump(tmp_path, dict(domain=[9, 9], BC=bc, species=[dict(pop=10), dict(pop=20)]))
+    w0, w1 = d["walls"]
+    assert w0["A"] == [0.0, 2.0, 0.0] and w0["MUN"] == 1.0 and w0["MUT"] == 0.0 and w0["V"] == [0.2, 0.0, 0.0]
+    assert w0["PHANTOM"] == 1 and w0["inter_mask"] == 1
+    # absent keys take the reference's defaults: R 2, DN 1, MVN = MVT = 1
+    assert w1["A"] == [0.0, -1.0, 0.0] and w1["R"] == 2.0 and w1["DN"] == 1.0 and w1["MVN"] == 1.0
+    assert w1["MUN"] == 0.0 and w1["MUT"] == 1.0 and w1["INV"] == 1 and w1["inter_mask"] == 3
+
+
+def test_rtech_alone(tmp_path):
+    assert dump(tmp_path, dict(domain=[8, 8], rTech=1))["rtech"] == 1
+    assert dump(tmp_path, dict(domain=[8, 8], collOp=0))["rtech"] == 0
+
+
+@pytest.mark.parametrize("cfg,msg", [
+    (dict(domain=[8, 8], mdIn="md.inp"), "mdIn"),
+    (dict(domain=[8, 8], checkpointIn="c.dat"), "checkpointIn"),
+    (dict(domain=[8, 8], BC=[dict(aInv=[0.25, 0.25, 0], Q=[4, 4, 0], P=[2, 2, 1, 1], R=4)]), "not an axis-aligned"),
+    (dict(domain=[8, 8], BC=[dict(aInv=[1, 0, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], wavy=[0.5, 1, 0])]),
+     "not an axis-aligned"),
+    (dict(domain=[8]), "domain"),
+])
+def test_rejected_options(tmp_path, cfg, msg):
+    r = run_dump(tmp_path, cfg)
+    assert r.returncode != 0 and msg in r.stderr
+
+
+def test_malformed_json(tmp_path):
+    r = run_dump(tmp_path, None, text='{"domain": [8, 8], "kbt": }')
+    assert r.returncode != 0 and "JSON line 1" in r.stderr
+
+
+def run_host(tmp_path, cfg, *args):
+    f = tmp_path / "input.json"
+    f.write_text(json.dumps(cfg))
+    out = tmp_path / "out"
+    r = subprocess.run([host(), str(f), "-o", str(out), *args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    rows = [list(map(float, l.split())) for l in open(out / "solvent.dat") if not l.startswith("#")]
+    return info, rows
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [(), ("--replicas", "2", "--no-graph")])
+def test_host_periodic_box_momentum(tmp_path, extra):
+    """SRD in a periodic box conserves momentum, so the body force alone sets
+    it: P_x = N m g dt (steps done) from the zero initial momentum."""
+    cfg = {"domain": [30, 20], "simSteps": 199, "warmUp": 9, "rTech": 0, "grav": [0.0005, 0, 0], "seed": 3,
+           "species": [{"dens": 10, "vDist": 4}]}
+    print(f"[host] periodic 2D box 30x20x10, 10 + 200 steps {extra}", flush=True)
+    info, rows = run_host(tmp_path, cfg, "--every", "50", *extra)
+    assert info["steps"] == 200 and info["particles_per_replica"] == 6000
+    assert [int(x[0]) for x in rows] == [49, 99, 149, 199]
+    for step, kbt, ke, px, py, pz in rows:
+        assert 0.9 < kbt < 1.1
+        expect = 6000 * 1.0 * 0.0005 * 0.1 * (10 + step + 1)  # warm-up: 9 + 1 steps
+        assert abs(px - expect) < 2e-6 * expect, (step, px, expect)
+        assert abs(py) < 1e-6 and pz == 0.0
+
+
+@pytest.mark.gpu
+def test_host_runs_a_channel(tmp_path):
+    """A channel between bounce-back phantom walls: every wall hit reverses a
+    particle's velocity, so the total momentum random-walks by far more than
+    the body force adds in 200 steps; the checks are the step count, the
+    temperature (SRD and bounce-back keep the kinetic energy) and the plane."""
+    cfg = {
+        "domain": [30, 20], "simSteps": 199, "collOp": 0, "grav": [0.0005, 0, 0], "seed": 3,
+        "species": [{"dens": 10, "vDist": 4}],
+        "BC": [
+            {"aInv": [1, 0, 0], "Q": [0, 0, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 30},
+            {"aInv": [-1, 0, 0], "Q": [30, 0, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 30},
+            {"aInv": [0, 1, 0], "Q": [0, 0, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 0, "MVN": -1, "MVT": -1,
+             "phantom": 1},
+            {"aInv": [0, -1, 0], "Q": [0, 20, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 0, "MVN": -1, "MVT": -1,
+             "phantom": 1},
+        ],
+    }
+    print("[host] 2D channel 30x20x10, 200 steps", flush=True)
+    info, rows = run_host(tmp_path, cfg, "--every", "50")
+    assert info["steps"] == 200 and info["particles_per_replica"] == 6000
+    assert [int(x[0]) for x in rows] == [49, 99, 149, 199]
+    for step, kbt, ke, px, py, pz in rows:
+        assert 0.9 < kbt < 1.1
+        assert pz == 0.0
