@@ -322,7 +322,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "replicas shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+              "replicas would shard across GPUs with no communication", file=sys.stderr)
         sys.exit(2)
     M = package()
     L = (args.L,) * 3

@@ -208,10 +208,26 @@ mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *ctx, int slot);
  * = dJ/dv; adj_out [3 * n_total] receives lambda at the start of the step
  * (may be NULL); dJ_dtheta [4 * nrep] receives the running sums of
  * dJ/dGRAV[0..2] and dJ/dRA per replica since mpcd_gpu_adjoint_reset (may be
- * NULL).  The north star's adjoint entry point; the reference has none. */
+ * NULL).  No counterpart in the reference. */
 mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *ctx, int max_steps);
 mpcd_status mpcd_gpu_adjoint_reset(mpcd_gpu_ctx *ctx);
 mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *ctx, const double *adj_in, double *adj_out, double *dJ_dtheta);
+/* Forward-mode tangent of the whole step (every operator of the unfused path:
+ * SRD, MPC-AT, MPC-AT+a, the nematic LC collision with Jeffery rotation and
+ * dipole activity, ghost particles and anchoring at phantom walls, planar
+ * walls; FP64 state, tsTech 0, lc 0 or 1) along one direction in parameter
+ * space: dgrav [3] = d(GRAV)/d(theta) (may be NULL: zero) and dact [nspec] =
+ * d(ACT of species k)/d(theta) (may be NULL: zero).  Cell assignments, draws,
+ * wall events and every other discrete decision (Metropolis acceptances,
+ * activity side, clamps) are those of the run itself, so the tangent is the
+ * pathwise derivative; the values the steps compute are the FP64 values bit
+ * for bit.  on != 0 starts it from zero for the current state (a state
+ * uploaded or restored later restarts it from zero), on = 0 stops it.  While
+ * it is on, steps run on the unfused path without graphs.
+ * mpcd_gpu_tangent_download returns d(Q, V, U)/d(theta) of replica `replica`
+ * in particle-index order (dU may be NULL).  No counterpart in the reference. */
+mpcd_status mpcd_gpu_tangent(mpcd_gpu_ctx *ctx, int on, const double *dgrav, const double *dact);
+mpcd_status mpcd_gpu_tangent_download(mpcd_gpu_ctx *ctx, int replica, double *dQ, double *dV, double *dU);
 
 /* --- observables ------------------------------------------------------------ */
 /* per-cell populations, centre-of-mass velocities [3 per cell], scalar order

@@ -116,7 +116,7 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_lc_stride", "mpcd_gpu_profile", "mpcd_gpu_timing", "mpcd_gpu_autotune",
            "mpcd_gpu_tile_fill", "mpcd_gpu_checkpoint", "mpcd_gpu_restore", "mpcd_gpu_record",
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
-           "mpcd_gpu_layout_stats"]
+           "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download"]
 
 _lib = None
 
@@ -158,6 +158,8 @@ def lib():
     L.mpcd_gpu_record.argtypes = [vp, C.c_int]
     L.mpcd_gpu_adjoint_reset.argtypes = [vp]
     L.mpcd_gpu_step_backward.argtypes = [vp, dp, dp, dp]
+    L.mpcd_gpu_tangent.argtypes = [vp, C.c_int, dp, dp]
+    L.mpcd_gpu_tangent_download.argtypes = [vp, C.c_int, dp, dp, dp]
     _lib = L
     return L
 
@@ -357,6 +359,19 @@ class Context:
         th = np.zeros(4 * self.nrep)
         self._check(self.L.mpcd_gpu_step_backward(self.h, _dp(a), _dp(lam), _dp(th)), "step_backward")
         return (None if lam is None else lam.reshape(n, 3)), th.reshape(self.nrep, 4)
+
+    def tangent(self, on=True, dgrav=None, dact=None):
+        """forward-mode tangent along (d GRAV, d ACT per species)/d theta; on=False stops it"""
+        g = None if dgrav is None else np.ascontiguousarray(dgrav, np.float64).reshape(3)
+        a = None if dact is None else np.ascontiguousarray(dact, np.float64).reshape(-1)
+        self._check(self.L.mpcd_gpu_tangent(self.h, 1 if on else 0, _dp(g), _dp(a)), "tangent")
+
+    def tangent_download(self, r=0):
+        """d(Q, V, U)/d theta of replica r in particle-index order"""
+        n = int(self.n[r])
+        dQ, dV, dU = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self.L.mpcd_gpu_tangent_download(self.h, r, _dp(dQ), _dp(dV), _dp(dU)), "tangent_download")
+        return dQ, dV, dU
 
     def cells(self, r=0):
         nc = self.ncells[r]

@@ -14,6 +14,13 @@ those of the recorded run (pathwise gradient).
 
 `CentrelineVelocity` wraps the two passes as a torch.autograd.Function of the
 parameter vector (g_x, RA).
+
+The forward-mode tangent (mpcd_gpu_tangent) gives the same pathwise
+derivative for every operator of the step, including the nematic LC
+collision with activity: `tangent_value_and_grad` runs the K steps once per
+direction in parameter space (body force, species activity) and averages the
+band's tangent velocity beside its velocity.  `ActiveChannel` wraps it as a
+torch.autograd.Function of (g_x, activity).
 """
 from __future__ import annotations
 
@@ -80,6 +87,35 @@ def value_and_grad(M, cfg, Q0, V0, K, seed, band, **kw):
     return J, dg, da
 
 
+def tangent_value_and_grad(M, cfg, Q0, V0, K, seed, band, directions, U0=None, spid=None, axis=1):
+    """J and dJ/dtheta_k for each direction k, a dict(dgrav=[3], dact=[nspec]):
+    one forward run of K steps per direction, the band and the draws of the
+    run itself (pathwise derivative)"""
+    N = int(Q0.shape[0])
+    J, grads = None, []
+    for d in directions:
+        ctx = M.Context(cfg, N, seed=seed, precision=M.FP64)
+        try:
+            ctx.upload(0, Q0, V0, U=U0, spid=spid)
+            ctx.tangent(True, dgrav=d.get("dgrav"), dact=d.get("dact"))
+            j = dj = 0.0
+            for _ in range(K):
+                ctx.step()
+                Q, V, _ = ctx.download(0)
+                _, dV, _ = ctx.tangent_download(0)
+                y = np.trunc(Q[:, axis])
+                m = (y >= band[0]) & (y < band[1])
+                j += V[m, 0].mean() / K
+                dj += dV[m, 0].mean() / K
+        finally:
+            ctx.close()
+        if J is not None and j != J:
+            raise RuntimeError("tangent runs of one problem disagree on J")
+        J = j
+        grads.append(dj)
+    return J, grads
+
+
 def _torch_function():
     import torch
 
@@ -106,3 +142,31 @@ def _torch_function():
 def centreline_velocity(theta, problem):
     """torch entry point: J = centreline_velocity(theta, problem); J.backward()"""
     return _torch_function().apply(theta, problem)
+
+
+def _torch_active():
+    import torch
+
+    class ActiveChannel(torch.autograd.Function):
+        """J(theta) with theta = (g_x, activity of every species); problem(g, act)
+        returns (M, cfg, Q0, V0, U0, K, seed, band)"""
+
+        @staticmethod
+        def forward(ctx, theta, problem):
+            M, cfg, Q0, V0, U0, K, seed, band = problem(float(theta[0]), float(theta[1]))
+            nsp = len(cfg["species"])
+            J, (dg, da) = tangent_value_and_grad(M, cfg, Q0, V0, K, seed, band,
+                                                 [dict(dgrav=[1.0, 0.0, 0.0]), dict(dact=[1.0] * nsp)], U0=U0)
+            ctx.grad = theta.new_tensor([dg, da])
+            return theta.new_tensor(J)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            return gJ * ctx.grad, None
+
+    return ActiveChannel
+
+
+def active_channel_velocity(theta, problem):
+    """torch entry point: J = active_channel_velocity(theta, problem); J.backward()"""
+    return _torch_active().apply(theta, problem)

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mpcd_adjoint.h"
@@ -83,6 +84,14 @@ struct mpcd_gpu_ctx {
   double *tp_v1 = nullptr, *tp_vcm = nullptr, *tp_a = nullptr, *tp_mass = nullptr, *tp_M = nullptr, *tp_dM = nullptr;
   float *tp_wmul = nullptr;
   double *lam = nullptr, *lam_s = nullptr, *lam1_s = nullptr, *adj_dev = nullptr, *dtheta = nullptr;
+  // forward-mode tangent (mpcd_gpu_tangent): 9 slot arrays (q, v, u) per
+  // ping-pong buffer, 25 doubles per cell, the torque tangent per slot, and
+  // the direction in parameter space
+  bool tan_on = false;
+  double *tan_part[2] = {nullptr, nullptr};
+  double *tan_cell = nullptr, *tan_T = nullptr;
+  double tan_dgrav[3] = {0.0, 0.0, 0.0};
+  double tan_dact[MPCD_MAXSPEC] = {};
   // device checkpoints: (destination, source, bytes) copies plus the host step state
   struct Checkpoint {
     std::vector<void *> bufs;
@@ -181,6 +190,7 @@ static mpcd_status fail(mpcd_gpu_ctx *c, mpcd_status st, const char *fmt, ...) {
 // the cell-sorted state for the entry points that read or change it (the
 // state at rest may be in the bucketed layout after mpcd_gpu_run)
 static mpcd_status ensure_dense(mpcd_gpu_ctx *c);
+static mpcd_status zero_tangent(mpcd_gpu_ctx *c);
 
 template <typename T>
 static T *dalloc(mpcd_gpu_ctx *c, size_t n) {
@@ -549,6 +559,28 @@ static TapeDev tape_at(mpcd_gpu_ctx *c, int k) {
   return T;
 }
 
+// the kernels' view of the tangent arrays of ping-pong buffer `buf`
+static TanDev tan_view(mpcd_gpu_ctx *c, int buf) {
+  TanDev T{};
+  if (!c->tan_part[0]) return T;
+  const size_t n = c->ntot, nc = c->nctot;
+  for (int k = 0; k < 3; k++) {
+    T.q[k] = c->tan_part[buf] + k * n;
+    T.v[k] = c->tan_part[buf] + (3 + k) * n;
+    T.u[k] = c->tan_part[buf] + (6 + k) * n;
+  }
+  T.vcm = c->tan_cell;
+  T.cm = c->tan_cell + 3 * nc;
+  T.I = c->tan_cell + 6 * nc;
+  T.S = c->tan_cell + 12 * nc;
+  T.dir = c->tan_cell + 13 * nc;
+  T.E = c->tan_cell + 16 * nc;
+  T.T = c->tan_T;
+  for (int k = 0; k < 3; k++) T.dgrav[k] = c->tan_dgrav[k];
+  for (int k = 0; k < MPCD_MAXSPEC; k++) T.dact[k] = c->tan_dact[k];
+  return T;
+}
+
 // the kernels' view of the context for the step with parity `par`
 static Ctx view(mpcd_gpu_ctx *c, int par) {
   Ctx C = c->C;
@@ -665,6 +697,10 @@ static void run_bin(mpcd_gpu_ctx *c, int mode, bool gather, int injected) {
     ProfScope ps(c, F_GATHER);
     k_gather<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], S[1 - c->cur], c->d_pblk, mode == 1 ? 1 : 0,
                                                               c->lc != MPCD_ISOF, D);
+    if (c->tan_on)
+      k_gather_tan<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], tan_view(c, c->cur),
+                                                                    tan_view(c, 1 - c->cur), c->d_pblk,
+                                                                    mode == 1 ? 1 : 0, c->lc != MPCD_ISOF);
     c->cur = 1 - c->cur;
     c->pending = false;
   }
@@ -675,10 +711,25 @@ static void run_props(mpcd_gpu_ctx *c, int do_acc) {
   State<Real> *S = states<Real>(c);
   Ctx C = view(c, c->step & 1);
   ProfScope ps(c, F_PROPS);
-  k_cellprops<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, do_acc);
+  const bool tan = std::is_same<Real, double>::value && c->tan_on;
+  const TanDev T = tan ? tan_view(c, c->cur) : TanDev{};
+  if constexpr (std::is_same<Real, double>::value)
+    if (tan) {
+      k_cellprops<Real, Dual><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, do_acc, T);
+    }
+  if (!tan) k_cellprops<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, do_acc, T);
   if (c->lc != MPCD_ISOF) {
     k_velgrad_bulk<<<c->cblk.size(), kBlock, 0, c->stream>>>(C, c->d_cblk);
     k_velgrad_edges<<<c->nrep, 256, 0, c->stream>>>(C);
+    if (tan) {
+      // the velocity gradient is linear in the cell velocities: the same
+      // stencil maps their tangents to its tangent
+      Ctx Ct = C;
+      Ct.cell.vcm = T.vcm;
+      Ct.cell.E = T.E;
+      k_velgrad_bulk<<<c->cblk.size(), kBlock, 0, c->stream>>>(Ct, c->d_cblk);
+      k_velgrad_edges<<<c->nrep, 256, 0, c->stream>>>(Ct);
+    }
   }
 }
 
@@ -688,9 +739,15 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
   Ctx C = view(c, c->step & 1);
   Draws D = make_draws(c, injected, c->step);
   run_props<Real>(c, 1);
+  const bool tan = std::is_same<Real, double>::value && c->tan_on;
+  const TanDev T = tan ? tan_view(c, c->cur) : TanDev{};
   if (c->any_phantom) {
     ProfScope ps(c, F_GHOST);
-    k_ghost<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+    if constexpr (std::is_same<Real, double>::value)
+      if (tan) {
+        k_ghost<Real, Dual><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
+      }
+    if (!tan) k_ghost<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
   }
   if (c->recording && c->tape_len < c->tape_cap)
     k_tape_cells<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, tape_at(c, c->tape_len));
@@ -700,11 +757,19 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
       k_avorder_part<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, c->d_avpart);
       k_avorder_final<<<1, 64, 0, c->stream>>>(C, c->d_pblk, (int)c->pblk.size(), c->d_avpart);
     }
-    k_lc<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D);
+    if constexpr (std::is_same<Real, double>::value)
+      if (tan) {
+        k_lc<Real, Dual><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, T);
+      }
+    if (!tan) k_lc<Real, double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, T);
   }
   {
     ProfScope ps(c, F_COLLIDE);
-    k_collide<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D);
+    if constexpr (std::is_same<Real, double>::value)
+      if (tan) {
+        k_collide<Real, Dual><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
+      }
+    if (!tan) k_collide<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
   }
   ProfScope ps(c, F_TEMP);
   k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(C, c->d_cblk_rng);
@@ -718,8 +783,14 @@ static void run_stream(mpcd_gpu_ctx *c, int injected) {
   Draws D = make_draws(c, injected, c->step);
   ProfScope ps(c, F_STREAM);
   const bool tape = c->recording && c->tape_len < c->tape_cap;
-  k_stream<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D,
-                                                             tape ? tape_at(c, c->tape_len).wmul : nullptr);
+  float *wmul = tape ? tape_at(c, c->tape_len).wmul : nullptr;
+  const bool tan = std::is_same<Real, double>::value && c->tan_on;
+  if constexpr (std::is_same<Real, double>::value)
+    if (tan) {
+      k_stream<Real, Dual><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, wmul,
+                                                                     tan_view(c, c->cur));
+    }
+  if (!tan) k_stream<Real, double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, wmul, TanDev{});
   if (tape) c->tape_len++;
   c->pending = true;
   c->cu_src = 0;
@@ -826,7 +897,7 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
 
 template <typename Real>
 static void run_step(mpcd_gpu_ctx *c, int injected) {
-  if (c->fusable && !c->recording) {
+  if (c->fusable && !c->recording && !c->tan_on) {
     run_step_fused<Real>(c, injected);
     return;
   }
@@ -912,6 +983,10 @@ extern "C" mpcd_status mpcd_gpu_upload(mpcd_gpu_ctx *c, int r, const double *Q, 
     k_stage<float><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->sf[c->cur], R.part_off, R.npart, c->d_q,
                                                                      c->d_v, U ? c->d_u : nullptr,
                                                                      spid ? c->d_spid : nullptr);
+  {
+    mpcd_status zs = zero_tangent(c);  // a new state has no derivative yet
+    if (zs) return zs;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->uploaded[r] = 1;
   c->need_init = true;
@@ -1107,7 +1182,7 @@ static bool alloc_wall_list(mpcd_gpu_ctx *c, WallList<Real> &w) {
 }
 
 static bool bucket_usable(const mpcd_gpu_ctx *c) {
-  return c->bkt_on && c->fusable && !c->recording && !c->any_therm;
+  return c->bkt_on && c->fusable && !c->recording && !c->tan_on && !c->any_therm;
 }
 
 static void drop_bucket_graph(mpcd_gpu_ctx *c) {
@@ -1502,7 +1577,7 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
     st = ensure_dense(c);
     if (st) return st;
   }
-  if (use_graph && c->fusable && !c->recording) {
+  if (use_graph && c->fusable && !c->recording && !c->tan_on) {
     // graphs start from an even step whose histogram the previous step built
     while (nsteps > 0 && (!c->hist_ready || (c->step & 1))) {
       eager();
@@ -1710,6 +1785,8 @@ extern "C" mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *c, int slot) {
   c->hist_ready = k.hist_ready;
   c->need_init = k.need_init;
   c->cu_src = k.cu_src;
+  mpcd_status st = zero_tangent(c);  // checkpoints hold no tangent: it restarts from zero
+  if (st) return st;
   return check_launch(c);
 }
 
@@ -1723,6 +1800,7 @@ extern "C" mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *c, int max_steps) {
     c->recording = false;
     return MPCD_OK;
   }
+  if (c->tan_on) return fail(c, MPCD_EINVAL, "the tangent and the adjoint tape cannot run together");
   for (const mpcd_gpu_params &p : c->params) {
     if (p.lc != MPCD_ISOF || !(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT))
       return fail(c, MPCD_EUNSUPPORTED, "the adjoint covers SRD and MPC-AT");
@@ -1783,6 +1861,79 @@ extern "C" mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *c, const double *adj
   if (adj_out) HIPCHK(c, hipMemcpyAsync(adj_out, c->lam, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   if (dJ_dtheta)
     HIPCHK(c, hipMemcpyAsync(dJ_dtheta, c->dtheta, sizeof(double) * 4 * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+static mpcd_status zero_tangent(mpcd_gpu_ctx *c) {
+  if (!c->tan_part[0]) return MPCD_OK;
+  const size_t n = c->ntot, nc = c->nctot;
+  for (int b = 0; b < 2; b++) HIPCHK(c, hipMemsetAsync(c->tan_part[b], 0, sizeof(double) * 9 * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->tan_cell, 0, sizeof(double) * 25 * nc, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->tan_T, 0, sizeof(double) * 3 * n, c->stream));
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_tangent(mpcd_gpu_ctx *c, int on, const double *dgrav, const double *dact) {
+  if (!c) return MPCD_EINVAL;
+  if (!on) {
+    c->tan_on = false;
+    return MPCD_OK;
+  }
+  if (c->precision != MPCD_FP64) return fail(c, MPCD_EUNSUPPORTED, "the tangent needs the FP64 state");
+  if (c->recording) return fail(c, MPCD_EINVAL, "the tangent and the adjoint tape cannot run together");
+  for (const mpcd_gpu_params &p : c->params) {
+    if (p.tstech != MPCD_NOTHERM) return fail(c, MPCD_EUNSUPPORTED, "the tangent needs tsTech = 0");
+    if (p.lc == MPCD_LCG) return fail(c, MPCD_EUNSUPPORTED, "the tangent needs the local order parameter (lc 1)");
+  }
+  mpcd_status st = ensure_dense(c);
+  if (st) return st;
+  if (!c->tan_part[0]) {
+    const size_t n = c->ntot, nc = c->nctot;
+    c->tan_part[0] = dalloc<double>(c, 9 * n);
+    c->tan_part[1] = dalloc<double>(c, 9 * n);
+    c->tan_cell = dalloc<double>(c, 25 * nc);
+    c->tan_T = dalloc<double>(c, 3 * n);
+    if (!c->tan_part[0] || !c->tan_part[1] || !c->tan_cell || !c->tan_T) {
+      c->tan_part[0] = c->tan_part[1] = nullptr;
+      return fail(c, MPCD_ENOMEM, "tangent arrays");
+    }
+  } else {
+    st = zero_tangent(c);
+    if (st) return st;
+  }
+  for (int k = 0; k < 3; k++) c->tan_dgrav[k] = dgrav ? dgrav[k] : 0.0;
+  for (int k = 0; k < MPCD_MAXSPEC; k++) c->tan_dact[k] = (dact && k < c->params[0].nspec) ? dact[k] : 0.0;
+  c->tan_on = true;
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_tangent_download(mpcd_gpu_ctx *c, int r, double *dQ, double *dV, double *dU) {
+  if (!c || r < 0 || r >= c->nrep || !dQ || !dV)
+    return c ? fail(c, MPCD_EINVAL, "tangent_download: bad argument") : MPCD_EINVAL;
+  if (!c->tan_part[0]) return fail(c, MPCD_EINVAL, "no tangent (mpcd_gpu_tangent first)");
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  const RepDev &R = c->reps[r];
+  st = ensure_stage(c, R.npart);
+  if (st) return st;
+  // the tangent arrays in the layout of a state, with the slots' particle ids
+  const TanDev T = tan_view(c, c->cur);
+  State<double> V{};
+  V.x = T.q[0]; V.y = T.q[1]; V.z = T.q[2];
+  V.vx = T.v[0]; V.vy = T.v[1]; V.vz = T.v[2];
+  V.ux = T.u[0]; V.uy = T.u[1]; V.uz = T.u[2];
+  V.id = c->sd[c->cur].id;
+  const int lc = c->lc != MPCD_ISOF;
+  const size_t n = R.npart;
+  k_unsort<double><<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(V, R.part_off, R.npart, c->d_q, c->d_v, c->d_u,
+                                                                     lc);
+  HIPCHK(c, hipMemcpyAsync(dQ, c->d_q, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dV, c->d_v, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  if (dU) {
+    if (lc) HIPCHK(c, hipMemcpyAsync(dU, c->d_u, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    else memset(dU, 0, sizeof(double) * 3 * n);
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c);
 }

@@ -22,9 +22,12 @@
 #pragma once
 
 #include "mpcd_math.h"
+#include "mpcd_dual.h"
 #include "../../include/mpcd_gpu.h"
 
 namespace mpcd {
+
+using mpcd_ad::Dual;
 
 constexpr int kBlock = 256;
 
@@ -71,6 +74,42 @@ struct Ctx {
   int ntot, nctot, nrep;
 };
 
+// Tangent of the FP64 state along one direction in parameter space (forward
+// mode, mpcd_gpu_tangent): per slot of the current buffer the derivatives of
+// position, velocity and orientation; per cell those of the cell properties;
+// and the direction itself (derivatives of the body force and of each
+// species' activity).
+struct TanDev {
+  double *q[3], *v[3], *u[3];
+  double *vcm, *cm, *I, *S, *dir, *E;  // per cell: 3, 3, 6, 1, 3, 9
+  double *T;                           // LC torque per slot [3 * ntot]
+  double dgrav[3];
+  double dact[MPCD_MAXSPEC];
+};
+
+// loads and stores of state values and their tangents: TA<double> touches
+// the values only, TA<Dual> the values and the tangent arrays
+template <typename F>
+struct TA;
+template <>
+struct TA<double> {
+  template <typename R>
+  static MPCD_HD double ld(const R *a, const double *, size_t i) { return (double)a[i]; }
+  template <typename R>
+  static MPCD_HD void st(R *a, double *, size_t i, double x) { a[i] = (R)x; }
+  static MPCD_HD double par(double x, double) { return x; }
+  static MPCD_HD bool moves(double) { return false; }
+};
+template <>
+struct TA<Dual> {
+  template <typename R>
+  static MPCD_HD Dual ld(const R *a, const double *d, size_t i) { return Dual((double)a[i], d[i]); }
+  template <typename R>
+  static MPCD_HD void st(R *a, double *d, size_t i, const Dual &x) { a[i] = (R)x.v; d[i] = x.d; }
+  static MPCD_HD Dual par(double x, double dx) { return Dual(x, dx); }
+  static MPCD_HD bool moves(const Dual &x) { return x.d != 0.0; }
+};
+
 // bits of Ctx::flags[0]
 constexpr int kFlagCellCap = 1;  // a cell above the fused kernel's LDS capacity
 constexpr int kFlagOutside = 2;  // initial positions outside the box
@@ -106,10 +145,18 @@ MPCD_HD void load_q(const State<Real> &s, int i, double q[3]) {
   q[2] = (double)s.z[i];
 }
 
+template <typename F, typename Real>
+__device__ __forceinline__ void load_qt(const State<Real> &s, const TanDev &T, int i, F q[3]) {
+  q[0] = TA<F>::ld(s.x, T.q[0], i);
+  q[1] = TA<F>::ld(s.y, T.q[1], i);
+  q[2] = TA<F>::ld(s.z, T.q[2], i);
+}
+
 // cell index of a position; shifted = 1 applies the periodic wrap of
 // bin(shifted=1) (mpc.c:964-968).  Returns -1 for an escaped particle.
-__device__ __forceinline__ int cell_of(const RepDev &R, const double q[3], int shifted) {
-  int a = (int)q[0], b = (int)q[1], c = (int)q[2];
+template <typename F>
+__device__ __forceinline__ int cell_of(const RepDev &R, const F q[3], int shifted) {
+  int a = (int)val(q[0]), b = (int)val(q[1]), c = (int)val(q[2]);
   int Lz = (R.dim == 3) ? R.L[2] : 1;
   if (shifted) {
     if (R.pbc[0] && a == R.L[0]) a = 0;
@@ -130,7 +177,8 @@ __device__ __forceinline__ void shift_fwd(const RepDev &R, const double *s, doub
 }
 
 // gridShift_all(-SHIFT), mpc.c:728-736 with shiftBack = 1
-__device__ __forceinline__ void shift_back(const RepDev &R, const double *s, double q[3]) {
+template <typename F>
+__device__ __forceinline__ void shift_back(const RepDev &R, const double *s, F q[3]) {
   for (int j = 0; j < R.dim; j++) {
     double sg = -1.0 * s[j];
     q[j] += sg;
@@ -365,13 +413,37 @@ __global__ void k_gather(Ctx C, State<Real> S, State<Real> D, const BlkDev *blk,
   D.sp[t] = sp;
 }
 
+// the tangent's share of the gather: the same permutation; a particle the
+// escape repair re-places gets fresh draws, which carry no derivative
+template <typename Real>
+__global__ void k_gather_tan(Ctx C, State<Real> S, TanDev Ts, TanDev Td, const BlkDev *blk, int shifted, int lc) {
+  const BlkDev B = blk[blockIdx.x];
+  int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  int src = (int)(C.skey[t] & 0x7FFFFFFFull);
+  bool fresh = false;
+  if (shifted) {
+    double q[3];
+    load_q(S, src, q);
+    shift_fwd(R, &C.shift[B.rep * 3], q);
+    fresh = cell_of(R, q, 1) < 0;
+  }
+  for (int k = 0; k < 3; k++) {
+    Td.q[k][t] = fresh ? 0.0 : Ts.q[k][src];
+    Td.v[k][t] = fresh ? 0.0 : Ts.v[k][src];
+    if (lc) Td.u[k][t] = Ts.u[k][src];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 7. cell properties: acc_all (mpc.c:697) then localPROP (mpc.c:64-216) with
 //    localMomInertiaTensor (mpc.c:4209-4299) and the order tensor / director
 //    (lc.c:1125-1148, mtools.c:1443)
 // ---------------------------------------------------------------------------
-template <typename Real>
-__global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc) {
+template <typename Real, typename F>
+__global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc, TanDev T) {
+  using A = TA<F>;
   const BlkDev B = cblk[blockIdx.x];
   int c = B.beg + threadIdx.x;
   if (c >= B.end) return;
@@ -379,27 +451,35 @@ __global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc
   const int dim = R.dim;
   const bool ccm = (R.rtech == MPCD_RAT) || (R.lc != MPCD_ISOF);
   int b = C.start[c], e = C.start[c + 1];
-  double vcm[3], cm[3], mass = 0.0;
-  for (int d = 0; d < 3; d++) { vcm[d] = C.cell.vcm[c * 3 + d]; cm[d] = C.cell.cm[c * 3 + d]; }
+  F vcm[3], cm[3];
+  double mass = 0.0;
+  for (int d = 0; d < 3; d++) {
+    vcm[d] = A::ld(C.cell.vcm, T.vcm, c * 3 + d);
+    cm[d] = A::ld(C.cell.cm, T.cm, c * 3 + d);
+  }
   for (int d = 0; d < dim; d++) vcm[d] = 0.0;
   if (ccm) for (int d = 0; d < dim; d++) cm[d] = 0.0;
+  F grav[3];
+  for (int d = 0; d < 3; d++) grav[d] = A::par(R.grav[d], T.dgrav[d]);
+  // a tangent along g applies acc_all even where the force itself is zero
+  const bool acc = do_acc && (R.gravflag || A::moves(grav[0]) || A::moves(grav[1]) || A::moves(grav[2]));
   for (int t = b; t < e; t++) {
     double m = C.sp[S.sp[t]].mass;
-    double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
-    if (do_acc && R.gravflag) {
-      for (int d = 0; d < dim; d++) v[d] = v[d] + R.dt * R.grav[d];
-      S.vx[t] = (Real)v[0];
-      S.vy[t] = (Real)v[1];
-      if (dim == 3) S.vz[t] = (Real)v[2];
-      v[0] = (double)S.vx[t];
-      v[1] = (double)S.vy[t];
-      v[2] = (double)S.vz[t];
+    F v[3] = {A::ld(S.vx, T.v[0], t), A::ld(S.vy, T.v[1], t), A::ld(S.vz, T.v[2], t)};
+    if (acc) {
+      for (int d = 0; d < dim; d++) v[d] = v[d] + R.dt * grav[d];
+      A::st(S.vx, T.v[0], t, v[0]);
+      A::st(S.vy, T.v[1], t, v[1]);
+      if (dim == 3) A::st(S.vz, T.v[2], t, v[2]);
+      v[0] = A::ld(S.vx, T.v[0], t);
+      v[1] = A::ld(S.vy, T.v[1], t);
+      v[2] = A::ld(S.vz, T.v[2], t);
     }
     mass += m;
     for (int d = 0; d < dim; d++) vcm[d] += v[d] * m;
     if (ccm) {
-      double q[3];
-      load_q(S, t, q);
+      F q[3];
+      load_qt(S, T, t, q);
       for (int d = 0; d < dim; d++) cm[d] += q[d] * m;
     }
   }
@@ -410,12 +490,16 @@ __global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc
     }
   C.cell.pop[c] = e - b;
   C.cell.mass[c] = mass;
-  for (int d = 0; d < 3; d++) { C.cell.vcm[c * 3 + d] = vcm[d]; C.cell.cm[c * 3 + d] = cm[d]; }
+  for (int d = 0; d < 3; d++) {
+    A::st(C.cell.vcm, T.vcm, c * 3 + d, vcm[d]);
+    A::st(C.cell.cm, T.cm, c * 3 + d, cm[d]);
+  }
   if (ccm) {
-    double I[6] = {0, 0, 0, 0, 0, 0};  // I00 I11 I22 I01 I02 I12
+    F I[6] = {0., 0., 0., 0., 0., 0.};  // I00 I11 I22 I01 I02 I12
     for (int t = b; t < e; t++) {
-      double m = C.sp[S.sp[t]].mass, q[3], r[3] = {0, 0, 0};
-      load_q(S, t, q);
+      double m = C.sp[S.sp[t]].mass;
+      F q[3], r[3] = {0., 0., 0.};
+      load_qt(S, T, t, q);
       for (int d = 0; d < dim; d++) r[d] = q[d] - cm[d];
       I[0] += m * (r[1] * r[1] + r[2] * r[2]);
       I[1] += m * (r[0] * r[0] + r[2] * r[2]);
@@ -425,13 +509,13 @@ __global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc
       I[5] += m * r[1] * r[2];
     }
     I[3] *= -1.; I[4] *= -1.; I[5] *= -1.;
-    for (int k = 0; k < 6; k++) C.cell.I[c * 6 + k] = I[k];
+    for (int k = 0; k < 6; k++) A::st(C.cell.I, T.I, c * 6 + k, I[k]);
   }
   if (R.lc != MPCD_ISOF && e - b > 1) {
-    double Sm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    F Sm[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
     double fD = (double)dim, cc = 1. / (fD - 1.), POPinv = 1. / ((double)(e - b));
     for (int t = b; t < e; t++) {
-      double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]};
+      F u[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)};
       for (int i = 0; i < dim; i++)
         for (int j = i; j < dim; j++) Sm[i][j] += u[i] * u[j];
     }
@@ -440,18 +524,18 @@ __global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc
     for (int i = 0; i < dim; i++) Sm[i][i] -= cc;
     for (int i = 0; i < dim; i++)
       for (int j = i + 1; j < dim; j++) Sm[j][i] = Sm[i][j];
-    double Sv, dir[3] = {0, 0, 0};
+    F Sv, dir[3] = {0., 0., 0.};
     if (dim == 3) {
-      double ev[3];
+      F ev[3];
       eig3_top(Sm, ev, dir);
       Sv = -1. * (ev[1] + ev[2]);
     } else {
-      double m2[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
+      F m2[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
       Sv = eig2_top(m2, dir);
     }
     if (Sv > 1.0) Sv = 1.0;
-    C.cell.S[c] = Sv;
-    for (int i = 0; i < dim; i++) C.cell.dir[c * 3 + i] = dir[i];
+    A::st(C.cell.S, T.S, c, Sv);
+    for (int i = 0; i < dim; i++) A::st(C.cell.dir, T.dir, c * 3 + i, dir[i]);
   }
 }
 
@@ -459,8 +543,9 @@ __global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc
 // 8. ghost particles and strong anchoring at phantom walls (ghostPart,
 //    mpc.c:351-567; oriBC lc.c:1290-1393)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wall_W(const mpcd_gpu_wall &W, const double *wq, const double *q, int dim) {
-  double Wv = 0.;                                         // calcW_PLANE, bc.c:306
+template <typename F>
+__device__ __forceinline__ F wall_W(const mpcd_gpu_wall &W, const double *wq, const F *q, int dim) {
+  F Wv = 0.;                                              // calcW_PLANE, bc.c:306
   for (int i = 0; i < dim; i++) Wv += W.A[i] * (q[i] - wq[i]);
   Wv -= W.R;
   if (W.INV) Wv *= -1.;
@@ -469,8 +554,9 @@ __device__ __forceinline__ double wall_W(const mpcd_gpu_wall &W, const double *w
 
 __device__ __forceinline__ bool anchoring(const mpcd_gpu_wall &W) { return feq(W.MUN, 0.0) || feq(W.MUT, 0.0); }
 
-__device__ __forceinline__ void ori_bc(double *u, const mpcd_gpu_wall &W, const double *n, int dim) {
-  double UN[3] = {0, 0, 0}, UT[3] = {0, 0, 0};
+template <typename F>
+__device__ __forceinline__ void ori_bc(F *u, const mpcd_gpu_wall &W, const F *n, int dim) {
+  F UN[3] = {0., 0., 0.}, UT[3] = {0., 0., 0.};
   normn(u, dim);
   if (dotn(u, n, dim) >= 0.0) { for (int i = 0; i < dim; i++) u[i] *= 1.; }
   else { for (int i = 0; i < dim; i++) u[i] *= -1.; }
@@ -488,8 +574,9 @@ __device__ __forceinline__ void ori_bc(double *u, const mpcd_gpu_wall &W, const 
   normn(u, dim);
 }
 
-template <typename Real>
-__global__ void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+template <typename Real, typename F>
+__global__ void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+  using A = TA<F>;
   Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   int c = B.beg + threadIdx.x;
@@ -525,18 +612,18 @@ __global__ void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
     if (R.lc != MPCD_ISOF && flagW && numBC == 1 && wallindex == w && anchoring(W)) {
       for (int t = b; t < e; t++) {
         if (!((W.inter_mask >> S.sp[t]) & 1)) continue;
-        double n[3] = {0, 0, 0}, u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]};
+        F n[3] = {0., 0., 0.}, u[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)};
         for (int k = 0; k < dim; k++) n[k] = W.A[k];
         normn(n, dim);
         ori_bc(u, W, n, dim);
-        S.ux[t] = (Real)u[0];
-        S.uy[t] = (Real)u[1];
-        S.uz[t] = (Real)u[2];
+        A::st(S.ux, T.u[0], t, u[0]);
+        A::st(S.uy, T.u[1], t, u[1]);
+        A::st(S.uz, T.u[2], t, u[2]);
       }
-      C.cell.S[c] = 1.0;
-      double d3[3] = {(double)S.ux[b], (double)S.uy[b], (double)S.uz[b]};
+      A::st(C.cell.S, T.S, c, F(1.0));
+      F d3[3] = {A::ld(S.ux, T.u[0], b), A::ld(S.uy, T.u[1], b), A::ld(S.uz, T.u[2], b)};
       normn(d3, dim);
-      for (int k = 0; k < dim; k++) C.cell.dir[c * 3 + k] = d3[k];
+      for (int k = 0; k < dim; k++) A::st(C.cell.dir, T.dir, c * 3 + k, d3[k]);
     }
   }
   if (flagW && pop < R.gndnst && pop > 0) {
@@ -544,10 +631,10 @@ __global__ void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
     for (int k = 0; k < dim; k++) {
       double sigma = sqrt(R.kbt / invN);
       double Rg = sigma * Dr.ghost_n(B.rep, c, k);
-      double v = C.cell.vcm[c * 3 + k];
+      F v = A::ld(C.cell.vcm, T.vcm, c * 3 + k);
       v += Rg;
       v /= R.gndnst;
-      C.cell.vcm[c * 3 + k] = v;
+      A::st(C.cell.vcm, T.vcm, c * 3 + k, v);
     }
   }
 }
@@ -691,22 +778,23 @@ __global__ void k_avorder_final(Ctx C, const BlkDev *blk, int nblk, const double
 }
 
 // Maier-Saupe samplers (lc.c:52-411); `slot` keys the draws
-__device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32_t slot, const double *DIR,
-                          double *rotAx, double rotAngle, double *U, double KBT, double S, double effM) {
+template <typename F>
+__device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32_t slot, const F *DIR, F *rotAx,
+                          const F &rotAngle, F *U, double KBT, const F &S, double effM) {
   if (R.dim == 2) {
-    double newU[3] = {U[0], U[1], 0.0};
-    double ang0 = signed_angle(DIR, U, 2);
+    F newU[3] = {U[0], U[1], 0.0};
+    F ang0 = signed_angle(DIR, U, 2);
     if (feq(ang0, 0.5 * kPI) || feq(ang0, -0.5 * kPI)) ang0 = kPI * (0.5 - Dr.lc_u(rep, slot, 0));
-    double uz0 = dotn(DIR, U, 2);
-    int annealNum = (int)(kMCINT + kMCSLOPE * effM * S / KBT);
+    F uz0 = dotn(DIR, U, 2);
+    int annealNum = (int)val(kMCINT + kMCSLOPE * effM * S / KBT);
     int cnt = 0;
     for (int i = 0; i < annealNum; i++) {
       int accept = 0;
-      double w0 = effM * S * uz0 * uz0 / KBT;
+      F w0 = effM * S * uz0 * uz0 / KBT;
       double ang1 = kPI * (0.5 - Dr.lc_u(rep, slot, 1 + 2 * i));
       double uz1 = cos(ang1);
-      double w1 = effM * S * uz1 * uz1 / KBT;
-      double dw = w1 - w0;
+      F w1 = effM * S * uz1 * uz1 / KBT;
+      F dw = w1 - w0;
       if (dw >= 0.) accept = 1;
       else if (Dr.lc_u(rep, slot, 2 + 2 * i) <= exp(dw)) accept = 1;
       if (accept) { cnt += 1; ang0 = ang1; uz0 = uz1; }
@@ -720,12 +808,12 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     }
     return;
   }
-  double BUS = effM * S / KBT;
-  double newU[3] = {U[0], U[1], U[2]};
+  F BUS = effM * S / KBT;
+  F newU[3] = {U[0], U[1], U[2]};
   if (BUS > kBUSMAX) {
-    double sd = 1. / sqrt(2. * effM * S / KBT);
-    double theta = sd * Dr.lc_n(rep, slot, 0) + 0.0;
-    double uz = 1. - 0.5 * theta * theta;
+    F sd = 1. / sqrt(2. * effM * S / KBT);
+    F theta = sd * Dr.lc_n(rep, slot, 0) + 0.0;
+    F uz = 1. - 0.5 * theta * theta;
     double pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
     newU[0] = pm * uz;
     double phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
@@ -734,13 +822,13 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     rodrigues(newU, rotAx, rotAngle);
     for (int d = 0; d < 3; d++) U[d] = newU[d];
   } else if (BUS < kBUSMIN) {
-    double B2 = BUS;
+    F B2 = BUS;
     if (B2 < kTOL) B2 = kTOL;
-    double Rr = Dr.lc_u(rep, slot, 0) * (1. + B2 / 3.);
-    double c = sqrt(9. * B2 * B2 * B2 * B2 * Rr * Rr + 4. * B2 * B2 * B2);
-    double num = pow(3. * B2 * B2 * Rr + c, 1. / 3.);
+    F Rr = Dr.lc_u(rep, slot, 0) * (1. + B2 / 3.);
+    F c = sqrt(9. * B2 * B2 * B2 * B2 * Rr * Rr + 4. * B2 * B2 * B2);
+    F num = pow(3. * B2 * B2 * Rr + c, 1. / 3.);
     double denom = pow(2., 1. / 3.);
-    double uz = num / (denom * B2) - denom / num;
+    F uz = num / (denom * B2) - denom / num;
     double pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
     newU[0] = pm * uz;
     double phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
@@ -749,15 +837,15 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     rodrigues(newU, rotAx, rotAngle);
     for (int d = 0; d < 3; d++) U[d] = newU[d];
   } else {
-    double uz0 = dotn(DIR, U, 3);
-    int annealNum = (int)(kMCINT + kMCSLOPE * effM * S / KBT);
+    F uz0 = dotn(DIR, U, 3);
+    int annealNum = (int)val(kMCINT + kMCSLOPE * effM * S / KBT);
     int cnt = 0;
     for (int i = 0; i < annealNum; i++) {
       int accept = 0;
-      double w0 = effM * S * uz0 * uz0 / KBT;
+      F w0 = effM * S * uz0 * uz0 / KBT;
       double uz1 = Dr.lc_u(rep, slot, 1 + 2 * i);
-      double w1 = effM * S * uz1 * uz1 / KBT;
-      double dw = w1 - w0;
+      F w1 = effM * S * uz1 * uz1 / KBT;
+      F dw = w1 - w0;
       if (dw >= 0.) accept = 1;
       else if (Dr.lc_u(rep, slot, 2 + 2 * i) <= exp(dw)) accept = 1;
       if (accept) { cnt += 1; uz0 = uz1; }
@@ -780,8 +868,9 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
 // per-cell arithmetic of the reference (the mean field summed over the cell
 // in list order), and the Metropolis draws are keyed by slot, so the result
 // does not depend on the mapping
-template <typename Real>
-__global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr) {
+template <typename Real, typename F>
+__global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev T) {
+  using A = TA<F>;
   Dr.resolve();
   const BlkDev B = pblk[blockIdx.x];
   const int t = B.beg + threadIdx.x;
@@ -791,11 +880,11 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr) {
   const int b = C.start[c], e = C.start[c + 1];
   if (e - b <= 1) return;
   const int dim = R.dim;
-  double DIR[3] = {0, 0, 0}, rotAx[3], xaxis[3] = {1.0, 0.0, 0.0};
-  double Sv = (R.lc == MPCD_LCL) ? C.cell.S[c] : C.sg[B.rep];
-  for (int i = 0; i < dim; i++) DIR[i] = C.cell.dir[c * 3 + i];
+  F DIR[3] = {0., 0., 0.}, rotAx[3], xaxis[3] = {1.0, 0.0, 0.0};
+  F Sv = (R.lc == MPCD_LCL) ? A::ld(C.cell.S, T.S, c) : F(C.sg[B.rep]);
+  for (int i = 0; i < dim; i++) DIR[i] = A::ld(C.cell.dir, T.dir, c * 3 + i);
   cross3(DIR, xaxis, rotAx);
-  double rotAngle = -1. * abs_angle(DIR, xaxis, dim);
+  F rotAngle = -1. * abs_angle(DIR, xaxis, dim);
   normn(rotAx, 3);
   double avM = 0.0;
   for (int t2 = b; t2 < e; t2++) avM += C.sp[S.sp[t2]].mfpot;
@@ -803,26 +892,26 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr) {
   double effM = avM * 0.5 * (double)dim;
   const mpcd_gpu_species &sp = C.sp[S.sp[t]];
   {
-    double U[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, T[3], dU[3] = {0, 0, 0};
+    F U[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)}, Tq[3], dU[3] = {0., 0., 0.};
     for (int i = 0; i < dim; i++) dU[i] = U[i];
     ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, R.kbt, Sv, effM);
     for (int i = 0; i < dim; i++) dU[i] = (U[i] - dU[i]) / R.dt;
-    cross3(U, dU, T);
-    for (int i = 0; i < 3; i++) T[i] *= sp.rfc;
-    S.ux[t] = (Real)U[0];
-    S.uy[t] = (Real)U[1];
-    S.uz[t] = (Real)U[2];
-    for (int i = 0; i < 3; i++) C.T[(size_t)t * 3 + i] = T[i];
+    cross3(U, dU, Tq);
+    for (int i = 0; i < 3; i++) Tq[i] *= sp.rfc;
+    A::st(S.ux, T.u[0], t, U[0]);
+    A::st(S.uy, T.u[1], t, U[1]);
+    A::st(S.uz, T.u[2], t, U[2]);
+    for (int i = 0; i < 3; i++) A::st(C.T, T.T, (size_t)t * 3 + i, Tq[i]);
   }
-  const double *E = &C.cell.E[(size_t)c * 9];
-  double D[3][3], W[3][3];
+  F D[3][3], W[3][3];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
-      D[j][i] = 0.5 * (E[i * 3 + j] + E[j * 3 + i]);
-      W[j][i] = 0.5 * (E[i * 3 + j] - E[j * 3 + i]);
+      F Eij = A::ld(C.cell.E, T.E, (size_t)c * 9 + i * 3 + j), Eji = A::ld(C.cell.E, T.E, (size_t)c * 9 + j * 3 + i);
+      D[j][i] = 0.5 * (Eij + Eji);
+      W[j][i] = 0.5 * (Eij - Eji);
     }
-  double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]}, dudt[3], w[3];
-  double uuuD[3] = {0, 0, 0}, uw[3] = {0, 0, 0}, uD[3] = {0, 0, 0};
+  F u[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)}, dudt[3], w[3];
+  F uuuD[3] = {0., 0., 0.}, uw[3] = {0., 0., 0.}, uD[3] = {0., 0., 0.};
   for (int j = 0; j < 3; j++) {
     for (int i = 0; i < 3; i++) {
       for (int k = 0; k < 3; k++) uuuD[j] += u[j] * (u[i] * D[i][k] * u[k]);
@@ -834,20 +923,21 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr) {
   cross3(u, dudt, w);
   for (int i = 0; i < 3; i++) w[i] *= sp.chihi;
   for (int i = 0; i < 3; i++) w[i] *= R.dt;
-  double theta = lengthn(w, 3);
+  F theta = lengthn(w, 3);
   normn(w, 3);
   rodrigues(u, w, theta);
-  S.ux[t] = (Real)u[0];
-  S.uy[t] = (Real)u[1];
-  S.uz[t] = (Real)u[2];
+  A::st(S.ux, T.u[0], t, u[0]);
+  A::st(S.uy, T.u[1], t, u[1]);
+  A::st(S.uz, T.u[2], t, u[2]);
 }
 
 // ---------------------------------------------------------------------------
 // 10. collision operators (MPCcollision switch, mpc.c:2777-2812), then the
 //     kinetic-energy partial sums of TEMP (therm.c:209) and gridShift back
 // ---------------------------------------------------------------------------
-template <typename Real>
-__global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
+template <typename Real, typename F>
+__global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+  using A = TA<F>;
   Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   int c = B.beg + threadIdx.x;
@@ -856,7 +946,8 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
   double ke = 0.0;
   if (c < B.end) {
     int b = C.start[c], e = C.start[c + 1], n = e - b;
-    double vcm[3] = {C.cell.vcm[c * 3], C.cell.vcm[c * 3 + 1], C.cell.vcm[c * 3 + 2]};
+    F vcm[3] = {A::ld(C.cell.vcm, T.vcm, c * 3), A::ld(C.cell.vcm, T.vcm, c * 3 + 1),
+                A::ld(C.cell.vcm, T.vcm, c * 3 + 2)};
     if (n > 1) {
       if (R.lc == MPCD_ISOF && (R.rtech == MPCD_ARBAXIS || R.rtech == MPCD_ORTHAXIS)) {
         // stochrotMPC + rotate, mpc.c:1181-1333
@@ -902,13 +993,13 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
           M[1][0] = Sn; M[1][1] = Cc;
         }
         for (int t = b; t < e; t++) {
-          double V[3] = {(double)S.vx[t] - vcm[0], (double)S.vy[t] - vcm[1], (double)S.vz[t] - vcm[2]};
-          double TM[3] = {0.0, 0.0, 0.0};
+          F V[3] = {A::ld(S.vx, T.v[0], t) - vcm[0], A::ld(S.vy, T.v[1], t) - vcm[1], A::ld(S.vz, T.v[2], t) - vcm[2]};
+          F TM[3] = {0.0, 0.0, 0.0};
           for (int i = 0; i < dim; i++)
             for (int j = 0; j < dim; j++) TM[i] += M[i][j] * V[j];
-          S.vx[t] = (Real)(vcm[0] + TM[0]);
-          S.vy[t] = (Real)(vcm[1] + TM[1]);
-          if (dim == 3) S.vz[t] = (Real)(vcm[2] + TM[2]);
+          A::st(S.vx, T.v[0], t, vcm[0] + TM[0]);
+          A::st(S.vy, T.v[1], t, vcm[1] + TM[1]);
+          if (dim == 3) A::st(S.vz, T.v[2], t, vcm[2] + TM[2]);
         }
       } else if (R.lc == MPCD_ISOF && R.rtech == MPCD_MPCAT) {
         // andersenMPC, mpc.c:1353-1444
@@ -920,15 +1011,16 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
         for (int j = 0; j < dim; j++) RS[j] /= C.cell.mass[c];
         for (int t = b; t < e; t++) {
           const mpcd_gpu_species &sp = C.sp[S.sp[t]];
-          double sig = sqrt(R.kbt / sp.mass), v[3];
+          double sig = sqrt(R.kbt / sp.mass);
+          F v[3];
           for (int j = 0; j < dim; j++) {
             double rv = sig * Dr.part_n(B.rep, t, j);
-            double dv = (sp.damp) * (vcm[j]) / ((double)n);
+            F dv = (sp.damp) * (vcm[j]) / ((double)n);
             v[j] = vcm[j] + rv - RS[j] - dv;
           }
-          S.vx[t] = (Real)v[0];
-          S.vy[t] = (Real)v[1];
-          if (dim == 3) S.vz[t] = (Real)v[2];
+          A::st(S.vx, T.v[0], t, v[0]);
+          A::st(S.vy, T.v[1], t, v[1]);
+          if (dim == 3) A::st(S.vz, T.v[2], t, v[2]);
         }
       } else {
         // andersenROT (mpc.c:1461-1657), andersenROT_LC (lc.c:1515-1728),
@@ -936,8 +1028,8 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
         const int mode = (R.lc == MPCD_ISOF) ? 0
                          : (R.rtech == MPCD_DIPOLE_DIR_SUM || R.rtech == MPCD_DIPOLE_DIR_AV ||
                             R.rtech == MPCD_DIPOLE_DIR_SIG || R.rtech == MPCD_DIPOLE_DIR_SIG_SUM) ? 2 : 1;
-        const double *cm = &C.cell.cm[c * 3];
-        double ACT = 0., PA[3] = {0, 0, 0};
+        F cm[3] = {A::ld(C.cell.cm, T.cm, c * 3), A::ld(C.cell.cm, T.cm, c * 3 + 1), A::ld(C.cell.cm, T.cm, c * 3 + 2)};
+        F ACT = 0., PA[3] = {0., 0., 0.};
         if (mode == 2) {
           int spc[MPCD_MAXSPEC];
           for (int k = 0; k < MPCD_MAXSPEC; k++) spc[k] = 0;
@@ -946,7 +1038,7 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
           for (int t = b; t < e; t++) {
             const mpcd_gpu_species &sp = C.sp[S.sp[t]];
             if ((sp.minactratio == 0.0) || (sp.minactratio * R.gndnst < (double)spc[S.sp[t]])) {
-              ACT += sp.act / sp.mass;
+              ACT += A::par(sp.act, T.dact[S.sp[t]]) / sp.mass;
               sigW += sp.sigwidth;
               sigP += sp.sigpos;
             }
@@ -959,17 +1051,20 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
             ACT *= fall / 2;
           }
           ACT *= R.dt;
-          for (int j = 0; j < dim; j++) PA[j] = C.cell.dir[c * 3 + j];
+          for (int j = 0; j < dim; j++) PA[j] = A::ld(C.cell.dir, T.dir, c * 3 + j);
         }
-        const bool act = (mode == 2) && fneq(ACT, 0.0);
-        double RS[3] = {0, 0, 0}, AS[3] = {0, 0, 0};
+        // a tangent along the activity keeps the active terms where the
+        // activity itself is zero (they add exact zeros to the values)
+        const bool act = (mode == 2) && (fneq(ACT, 0.0) || A::moves(ACT));
+        double RS[3] = {0, 0, 0};
+        F AS[3] = {0., 0., 0.};
         for (int t = b; t < e; t++) {
           const mpcd_gpu_species &sp = C.sp[S.sp[t]];
           double sig = sqrt(R.kbt / sp.mass);
           for (int j = 0; j < dim; j++) RS[j] += sp.mass * (sig * Dr.part_n(B.rep, t, j));
           if (act) {
-            double q[3], pW = 0.0;
-            load_q(S, t, q);
+            F q[3], pW = 0.0;
+            load_qt(S, T, t, q);
             for (int j = 0; j < dim; j++) pW += PA[j] * (q[j] - cm[j]);
             pW -= 0.0;
             double pm = (pW <= 0) ? -1. : 1.;
@@ -978,27 +1073,29 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
         }
         for (int j = 0; j < dim; j++) RS[j] /= C.cell.mass[c];
         if (mode == 2) for (int j = 0; j < dim; j++) AS[j] /= C.cell.mass[c];
-        double II[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        const double *I6 = &C.cell.I[c * 6];
+        F II[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+        F I6[6];
+        for (int k = 0; k < 6; k++) I6[k] = A::ld(C.cell.I, T.I, c * 6 + k);
         if (dim == 3) {
           if (n >= 3) {
-            double Im[3][3] = {{I6[0], I6[3], I6[4]}, {I6[3], I6[1], I6[5]}, {I6[4], I6[5], I6[2]}};
+            F Im[3][3] = {{I6[0], I6[3], I6[4]}, {I6[3], I6[1], I6[5]}, {I6[4], I6[5], I6[2]}};
             inv3(II, Im);
           }
         } else {
           II[2][2] = 1. / I6[2];
         }
-        double L[3] = {0, 0, 0}, Llm[3] = {0, 0, 0}, Lnm[3] = {0, 0, 0};
+        F L[3] = {0., 0., 0.}, Llm[3] = {0., 0., 0.}, Lnm[3] = {0., 0., 0.};
         for (int t = b; t < e; t++) {
           const mpcd_gpu_species &sp = C.sp[S.sp[t]];
-          double sig = sqrt(R.kbt / sp.mass), q[3], rq[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, am[3];
-          double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
-          load_q(S, t, q);
+          double sig = sqrt(R.kbt / sp.mass);
+          F q[3], rq[3] = {0., 0., 0.}, dv[3] = {0., 0., 0.}, am[3];
+          F v[3] = {A::ld(S.vx, T.v[0], t), A::ld(S.vy, T.v[1], t), A::ld(S.vz, T.v[2], t)};
+          load_qt(S, T, t, q);
           for (int j = 0; j < dim; j++) rq[j] = q[j] - cm[j];
           for (int j = 0; j < dim; j++) {
             double rv = sig * Dr.part_n(B.rep, t, j);
             if (act) {
-              double pW = 0.0;
+              F pW = 0.0;
               for (int jj = 0; jj < dim; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
               pW -= 0.0;
               double pm = (pW <= 0) ? -1. : 1.;
@@ -1013,24 +1110,25 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
           if (mode == 0) for (int j = 0; j < 3; j++) L[j] += am[j];
           else {
             for (int j = 0; j < 3; j++) Llm[j] += am[j];
-            for (int j = 0; j < 3; j++) Lnm[j] += C.T[(size_t)t * 3 + j] * R.dt;
+            for (int j = 0; j < 3; j++) Lnm[j] += A::ld(C.T, T.T, (size_t)t * 3 + j) * R.dt;
           }
         }
         if (mode != 0) for (int j = 0; j < 3; j++) L[j] = Llm[j] + Lnm[j];
-        double Wv[3];
+        F Wv[3];
         for (int i = 0; i < 3; i++) {
           Wv[i] = 0.;
           for (int j = 0; j < 3; j++) Wv[i] += II[i][j] * L[j];
         }
         for (int t = b; t < e; t++) {
           const mpcd_gpu_species &sp = C.sp[S.sp[t]];
-          double sig = sqrt(R.kbt / sp.mass), q[3], rq[3] = {0, 0, 0}, at[3], v[3];
-          load_q(S, t, q);
+          double sig = sqrt(R.kbt / sp.mass);
+          F q[3], rq[3] = {0., 0., 0.}, at[3], v[3];
+          load_qt(S, T, t, q);
           for (int j = 0; j < dim; j++) rq[j] = q[j] - cm[j];
           cross3(Wv, rq, at);
           double pm = 0.0;
           if (act) {
-            double pW = 0.0;
+            F pW = 0.0;
             for (int jj = 0; jj < dim; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
             pW -= 0.0;
             pm = (pW <= 0) ? -1. : 1.;
@@ -1039,11 +1137,11 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
             double rv = sig * Dr.part_n(B.rep, t, j);
             if (mode == 0) v[j] = vcm[j] + rv - RS[j] - (sp.damp) * (vcm[j]) + at[j];
             else if (mode == 1) v[j] = vcm[j] + rv - RS[j] + at[j];
-            else v[j] = vcm[j] + rv - RS[j] + at[j] + (act ? pm * ACT * PA[j] : 0.0) - AS[j] - (sp.damp) * (vcm[j]);
+            else v[j] = vcm[j] + rv - RS[j] + at[j] + (act ? pm * ACT * PA[j] : F(0.0)) - AS[j] - (sp.damp) * (vcm[j]);
           }
-          S.vx[t] = (Real)v[0];
-          S.vy[t] = (Real)v[1];
-          if (dim == 3) S.vz[t] = (Real)v[2];
+          A::st(S.vx, T.v[0], t, v[0]);
+          A::st(S.vy, T.v[1], t, v[1]);
+          if (dim == 3) A::st(S.vz, T.v[2], t, v[2]);
         }
       }
     }
@@ -1056,12 +1154,12 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr) {
       tmp += v2 * v2;
       ke += m * tmp;
       if (R.galinv) {
-        double q[3];
-        load_q(S, t, q);
+        F q[3];
+        load_qt(S, T, t, q);
         shift_back(R, &C.shift[B.rep * 3], q);
-        S.x[t] = (Real)q[0];
-        S.y[t] = (Real)q[1];
-        if (dim == 3) S.z[t] = (Real)q[2];
+        A::st(S.x, T.q[0], t, q[0]);
+        A::st(S.y, T.q[1], t, q[1]);
+        if (dim == 3) A::st(S.z, T.q[2], t, q[2]);
       }
     }
   }
@@ -1097,41 +1195,42 @@ __global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
 // 12. streaming and planar walls (stream_all mpc.c:680; MPC_BCcollision
 //     bc.c:1363-1496), then the unshifted cell of every particle
 // ---------------------------------------------------------------------------
-template <typename Real>
-__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul) {
+template <typename Real, typename F>
+__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul, TanDev T) {
+  using A = TA<F>;
   Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
   if (t >= B.end) return;
   const RepDev &R = C.rep[B.rep];
   const int dim = R.dim;
-  double Q[3], V[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
-  double U[3] = {0, 0, 0};
-  load_q(S, t, Q);
+  F Q[3], V[3] = {A::ld(S.vx, T.v[0], t), A::ld(S.vy, T.v[1], t), A::ld(S.vz, T.v[2], t)};
+  F U[3] = {0., 0., 0.};
+  load_qt(S, T, t, Q);
   const bool lc = R.lc != MPCD_ISOF;
-  if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
+  if (lc) { U[0] = A::ld(S.ux, T.u[0], t); U[1] = A::ld(S.uy, T.u[1], t); U[2] = A::ld(S.uz, T.u[2], t); }
   const int spid = S.sp[t];
   const mpcd_gpu_species &sp = C.sp[spid];
   for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
   // MPC_BCcollision
-  double t_step = R.dt, t_left = t_step, t_coll = 0.0;
+  F t_step = R.dt, t_left = t_step, t_coll = 0.0;
   int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
-  double n[3] = {0, 0, 0};
+  F n[3] = {0., 0., 0.};
   // adjoint tape: the product of the velocity factors of the bounces (MVN on
   // the wall's axis, MVT on the others); NaN when the velocity is redrawn
   float mul[3] = {1.f, 1.f, 1.f};
   while (true) {
-    double tmin = t_left, chW = 1.0;
+    F tmin = t_left, chW = 1.0;
     for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
       const mpcd_gpu_wall &W = C.wall[w];
       if (!((W.inter_mask >> spid) & 1)) continue;
       const double *wq = &C.wq_b[w * 3];
-      double tw = wall_W(W, wq, Q, dim);
+      F tw = wall_W(W, wq, Q, dim);
       if (tw < 0.0) {
-        double tc = W.R;
+        F tc = W.R;
         for (int i = 0; i < dim; i++) tc += W.A[i] * (wq[i] - Q[i]);
         tc /= (W.A[0] * (-V[0]) + W.A[1] * (-V[1]) + W.A[2] * (-V[2]));
-        double t1 = t_left - tc;
+        F t1 = t_left - tc;
         if (t1 < tmin) { tmin = t1; chosen = w; chW = tw; }
       }
     }
@@ -1144,9 +1243,10 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, floa
       for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
       for (int i = 0; i < dim; i++) n[i] = W.A[i];
       normn(n, dim);
-      double Rr[3] = {0, 0, 0}, VR[3], Vr[3] = {0, 0, 0}, VN[3] = {0, 0, 0}, VT[3] = {0, 0, 0};
+      F Rr[3] = {0., 0., 0.}, VR[3], Vr[3] = {0., 0., 0.}, VN[3] = {0., 0., 0.}, VT[3] = {0., 0., 0.};
+      F WL[3] = {W.L[0], W.L[1], W.L[2]};
       for (int i = 0; i < dim; i++) Rr[i] = Q[i] - wq[i];
-      cross3(W.L, Rr, VR);
+      cross3(WL, Rr, VR);
       for (int i = 0; i < dim; i++) Vr[i] = V[i] - W.V[i] - VR[i];
       projn(Vr, n, VN, dim);
       for (int i = 0; i < dim; i++) VT[i] = Vr[i] - VN[i];
@@ -1166,7 +1266,7 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, floa
       }
       for (int i = 0; i < dim; i++) V[i] += Vr[i] * 1.0 * IMpart;
       for (int i = 0; i < 3; i++) mul[i] *= (float)((W.A[i] != 0.0) ? W.MVN : W.MVT);
-      double PN[3] = {0, 0, 0}, PT[3] = {0, 0, 0};
+      F PN[3] = {0., 0., 0.}, PT[3] = {0., 0., 0.};
       projn(Q, n, PN, dim);
       for (int i = 0; i < dim; i++) PT[i] = Q[i] - PN[i];
       for (int i = 0; i < dim; i++) {
@@ -1222,16 +1322,16 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, floa
     }
     cu = S.cst[t];
   }
-  S.x[t] = (Real)Q[0];
-  S.y[t] = (Real)Q[1];
-  S.z[t] = (Real)Q[2];
-  S.vx[t] = (Real)V[0];
-  S.vy[t] = (Real)V[1];
-  S.vz[t] = (Real)V[2];
+  A::st(S.x, T.q[0], t, Q[0]);
+  A::st(S.y, T.q[1], t, Q[1]);
+  A::st(S.z, T.q[2], t, Q[2]);
+  A::st(S.vx, T.v[0], t, V[0]);
+  A::st(S.vy, T.v[1], t, V[1]);
+  A::st(S.vz, T.v[2], t, V[2]);
   if (lc) {
-    S.ux[t] = (Real)U[0];
-    S.uy[t] = (Real)U[1];
-    S.uz[t] = (Real)U[2];
+    A::st(S.ux, T.u[0], t, U[0]);
+    A::st(S.uy, T.u[1], t, U[1]);
+    A::st(S.uz, T.u[2], t, U[2]);
   }
   C.cu[t] = cu;
   if (wmul)

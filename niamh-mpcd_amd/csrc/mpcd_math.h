@@ -23,38 +23,48 @@ constexpr double kBUSMAX = 5.0;        // definitions.h:181
 MPCD_HD bool feq(double x, double y) { return fabs(x - y) <= kTOL; }   // mtools.c:61
 MPCD_HD bool fneq(double x, double y) { return fabs(x - y) >= kTOL; }  // mtools.c:74
 
-MPCD_HD double dotn(const double *x, const double *y, int n) {         // mtools.c:111
-  double r = 0.;
+// The geometric helpers are templates on the arithmetic type F: double for
+// the step, mpcd_ad::Dual (mpcd_dual.h) for its tangent.
+MPCD_HD double val(double x) { return x; }
+
+template <typename F>
+MPCD_HD F dotn(const F *x, const F *y, int n) {                        // mtools.c:111
+  F r = 0.;
   for (int i = 0; i < n; i++) r += x[i] * y[i];
   return r;
 }
 
-MPCD_HD void cross3(const double *x, const double *y, double *r) {     // mtools.c:177
+template <typename F>
+MPCD_HD void cross3(const F *x, const F *y, F *r) {                    // mtools.c:177
   r[0] = x[1] * y[2] - x[2] * y[1];
   r[1] = x[2] * y[0] - x[0] * y[2];
   r[2] = x[0] * y[1] - x[1] * y[0];
 }
 
-MPCD_HD double lengthn(const double *x, int n) {                       // mtools.c:227
-  double r = 0.;
+template <typename F>
+MPCD_HD F lengthn(const F *x, int n) {                                 // mtools.c:227
+  F r = 0.;
   for (int i = 0; i < n; i++) r += x[i] * x[i];
   return sqrt(r);
 }
 
-MPCD_HD void normn(double *x, int n) {                                 // mtools.c:246
-  double l = lengthn(x, n);
+template <typename F>
+MPCD_HD void normn(F *x, int n) {                                      // mtools.c:246
+  F l = lengthn(x, n);
   if (fneq(l, 0.0))
     for (int i = 0; i < n; i++) x[i] = x[i] / l;
 }
 
-MPCD_HD void projn(const double *v, const double *nr, double *vn, int n) {  // mtools.c:293
-  double x = dotn(v, nr, n);
+template <typename F>
+MPCD_HD void projn(const F *v, const F *nr, F *vn, int n) {            // mtools.c:293
+  F x = dotn(v, nr, n);
   for (int i = 0; i < n; i++) vn[i] = x * nr[i];
 }
 
 // the reference's atan2 (mtools.c:342-356), which overrides libm's
-MPCD_HD double ref_atan2(double y, double x) {
-  double at = 0.0;
+template <typename F>
+MPCD_HD F ref_atan2(const F &y, const F &x) {
+  F at = 0.0;
   if (x > 0.0) at = atan(y / x);
   else if (x < 0.0 && y >= 0.0) at = atan(y / x) + kPI;
   else if (x < 0.0 && y < 0.0) at = atan(y / x) - kPI;
@@ -63,59 +73,65 @@ MPCD_HD double ref_atan2(double y, double x) {
   return at;
 }
 
-MPCD_HD double abs_angle(const double *a, const double *b, int n) {    // mtools.c:325,365
-  double c = dotn(a, b, n);
+template <typename F>
+MPCD_HD F abs_angle(const F *a, const F *b, int n) {                   // mtools.c:325,365
+  F c = dotn(a, b, n);
   c /= lengthn(a, n);
   c /= lengthn(b, n);
   return acos(c);
 }
 
-MPCD_HD double signed_angle(const double *v1, const double *v2, int n) {  // mtools.c:380
-  double A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, c3[3];
+template <typename F>
+MPCD_HD F signed_angle(const F *v1, const F *v2, int n) {              // mtools.c:380
+  F A[3] = {0., 0., 0.}, B[3] = {0., 0., 0.}, c3[3];
   for (int i = 0; i < n; i++) { A[i] = v1[i]; B[i] = v2[i]; }
   cross3(A, B, c3);
   return ref_atan2(lengthn(c3, 3), dotn(A, B, n));
 }
 
 // rodriguesRotation, mtools.c:1572-1584 (renormalises the axis in place)
-MPCD_HD void rodrigues(double *vec, double *ax, double theta) {
-  double cp[3];
+template <typename F>
+MPCD_HD void rodrigues(F *vec, F *ax, const F &theta) {
+  F cp[3];
   normn(ax, 3);
-  double oldl = lengthn(vec, 3);
+  F oldl = lengthn(vec, 3);
   cross3(ax, vec, cp);
-  double dp = dotn(ax, vec, 3);
+  F dp = dotn(ax, vec, 3);
   for (int i = 0; i < 3; i++) vec[i] = vec[i] * cos(theta) + cp[i] * sin(theta) + ax[i] * dp * (1.0 - cos(theta));
-  double newl = lengthn(vec, 3);
+  F newl = lengthn(vec, 3);
   for (int i = 0; i < 3; i++) vec[i] *= (oldl / newl);
 }
 
-MPCD_HD double det3(const double m[3][3]) {                            // mtools.c:857
-  double c1 = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
-  double c2 = m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
-  double c3 = m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+template <typename F>
+MPCD_HD F det3(const F m[3][3]) {                                      // mtools.c:857
+  F c1 = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+  F c2 = m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+  F c3 = m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
   return (c1 - c2 + c3);
 }
 
 // invert3x3 via cofactors, mtools.c:972-1037
-MPCD_HD void inv3(double inv[3][3], const double m[3][3]) {
-  double det = det3(m);
+template <typename F>
+MPCD_HD void inv3(F inv[3][3], const F m[3][3]) {
+  F det = det3(m);
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
       int r0 = (i == 0) ? 1 : 0, r1 = (i == 2) ? 1 : 2;
       int c0 = (j == 0) ? 1 : 0, c1 = (j == 2) ? 1 : 2;
       int sg = (2 * ((i + j) / 2) == (i + j)) ? 1 : -1;
-      double cof = sg * (m[r0][c0] * m[r1][c1] - m[r0][c1] * m[r1][c0]);
+      F cof = (double)sg * (m[r0][c0] * m[r1][c1] - m[r0][c1] * m[r1][c0]);
       inv[j][i] = cof / det;
     }
 }
 
 // eigenvalues2x2 + eigenvectors2x2, mtools.c:1230-1305.  Returns the largest
 // eigenvalue and writes the first eigenvector (the cell director).
-MPCD_HD double eig2_top(const double m[2][2], double v0[2]) {
-  double det = m[0][0] * m[1][1] - m[1][0] * m[0][1];
-  double tr = m[0][0] + m[1][1];
-  double sq = sqrt(tr * tr * 0.25 - det);
-  double e0 = tr * 0.5 + sq;
+template <typename F>
+MPCD_HD F eig2_top(const F m[2][2], F v0[2]) {
+  F det = m[0][0] * m[1][1] - m[1][0] * m[0][1];
+  F tr = m[0][0] + m[1][1];
+  F sq = sqrt(tr * tr * 0.25 - det);
+  F e0 = tr * 0.5 + sq;
   v0[0] = 0.0;
   v0[1] = 0.0;
   if (fneq(m[1][0], 0.0)) {
@@ -144,28 +160,29 @@ MPCD_HD double eig2_top(const double m[2][2], double v0[2]) {
 // eigenvalues3x3 (Smith's trigonometric method) + eigenvectors3x3
 // (Cayley-Hamilton column products), mtools.c:1307-1430.  Writes all three
 // eigenvalues (descending by construction) and the first eigenvector.
-MPCD_HD void eig3_top(const double m[3][3], double ev[3], double v0[3]) {
-  double p1 = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+template <typename F>
+MPCD_HD void eig3_top(const F m[3][3], F ev[3], F v0[3]) {
+  F p1 = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
   if (feq(p1, 0.0)) {
-    double q;
+    F q;
     for (int i = 0; i < 3; i++) ev[i] = m[i][i];
     if (ev[1] > ev[0]) { q = ev[0]; ev[0] = ev[1]; ev[1] = q; }
     if (ev[2] > ev[0]) { q = ev[0]; ev[0] = ev[2]; ev[2] = q; }
     if (ev[2] > ev[1]) { q = ev[1]; ev[1] = ev[2]; ev[2] = q; }
   } else {
-    double p2 = 0.;
-    double q = (m[0][0] + m[1][1] + m[2][2]) / 3.;
+    F p2 = 0.;
+    F q = (m[0][0] + m[1][1] + m[2][2]) / 3.;
     for (int i = 0; i < 3; i++) p2 += (m[i][i] - q) * (m[i][i] - q);
     p2 += 2. * p1;
-    double p = sqrt(p2 / 6.);
-    double ip = 1. / p;
-    double B[3][3];
+    F p = sqrt(p2 / 6.);
+    F ip = 1. / p;
+    F B[3][3];
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) B[i][j] = m[i][j];
     for (int i = 0; i < 3; i++) B[i][i] -= q;
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) B[i][j] *= ip;
-    double r = det3(B) * 0.5, phi;
+    F r = det3(B) * 0.5, phi;
     if (r <= -1.) phi = kPI / 3.;
     else if (r >= 1.) phi = 0.;
     else phi = acos(r) / 3.;
@@ -178,9 +195,9 @@ MPCD_HD void eig3_top(const double m[3][3], double ev[3], double v0[3]) {
     for (int row = 0; row < 3; row++) v0[row] = 0.;
     for (int row = 0; row < 3; row++)
       for (int i = 0; i < 3; i++) {
-        double a = m[row][i];
+        F a = m[row][i];
         if (i == row) a -= ev[1];
-        double b = m[i][col];
+        F b = m[i][col];
         if (i == col) b -= ev[2];
         v0[row] += a * b;
       }
