@@ -191,6 +191,81 @@ def cpu_baseline(L=(64, 64, 64), ppc=10, steps=3):
                 sample=f"3D SRD {L[0]}x{L[1]}x{L[2]} cells x {ppc} ppc ({N} particles), {steps} steps, one core")
 
 
+def tangent_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, act=0.03, hg=(1e-11,), ha=(1e-13,)):
+    """gradient of the time-averaged centreline velocity of a 2D active nematic
+    channel (lc 1, dipole activity DIPOLE_DIR_SUM, U = 10, tumbling 0.5,
+    rfc 0.01; bounce-back phantom walls in y with homeotropic strong anchoring;
+    body force g along x) with respect to g and the activity, by the
+    forward-mode tangent (one run per direction) against central finite
+    differences with common random numbers at the steps hg, ha.
+    The pathwise derivative holds every discrete decision fixed.  A difference
+    step perturbs the orientations by h times their tangents, which grow to
+    O(100) over 200 steps, and Metropolis acceptances (about 2e8 comparisons at
+    128 x 64 x 20) flip once h exceeds about 1e-12; below about 1e-13 the
+    FP64 round-off of the band average dominates.  At that size finite
+    differences therefore resolve the derivative to about 1 %; a smaller box
+    has a clean window and checks it to 1e-3."""
+    import importlib
+    H = importlib.import_module("niamh_mpcd_amd.adjoint")
+    rng = np.random.default_rng(4)
+    N = Lx * Ly * ppc
+    Q = np.zeros((N, 3))
+    Q[:, 0] = rng.uniform(0, Lx, N)
+    Q[:, 1] = rng.uniform(0, Ly, N)
+    V = np.zeros((N, 3))
+    V[:, :2] = rng.normal(size=(N, 2))
+    V -= V.mean(axis=0)
+    th = rng.uniform(0, 2 * np.pi, N)
+    U = np.zeros((N, 3))
+    U[:, 0], U[:, 1] = np.cos(th), np.sin(th)
+
+    def cfg(gg, aa):
+        sp = dict(mass=1.0, rfc=0.01, tumble=0.5, chihi=0.5, act=aa, mfpot=10.0, damp=0.0, sigwidth=1.0,
+                  sigpos=1.0, minactratio=0.0, vdist=4)
+        anch = dict(DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1, MUN=1.0, MUT=0.0)
+        ws = [dict(A=[1.0, 0, 0], R=0.0, DN=float(Lx)), dict(A=[-1.0, 0, 0], R=-float(Lx), DN=float(Lx)),
+              dict(A=[0, 1.0, 0], R=0.0, **anch), dict(A=[0, -1.0, 0], R=-float(Ly), **anch)]
+        return dict(dim=2, L=[Lx, Ly, 1], dt=0.1, kbt=1.0, rotang=1.570796, tau=0.5, grav=[gg, 0.0, 0.0],
+                    avv=[0.0, 0.0, 0.0], gndnst=float(ppc), rtech=16, lc=1, tstech=0, galinv=1, species=[sp],
+                    walls=ws)
+
+    band = (Ly // 2 - 1, Ly // 2 + 1)
+    t0 = time.perf_counter()
+    J, (dg, da) = H.tangent_value_and_grad(M, cfg(g, act), Q, V, K, 13, band,
+                                           [dict(dgrav=[1.0, 0.0, 0.0]), dict(dact=[1.0])], U0=U)
+    t1 = time.perf_counter()
+
+    def J_of(gg, aa):
+        ctx = M.Context(cfg(gg, aa), N, seed=13, precision=M.FP64)
+        ctx.upload(0, Q, V, U=U)
+        j = 0.0
+        for _ in range(K):
+            ctx.step()
+            Qs, Vs, _ = ctx.download(0)
+            y = np.trunc(Qs[:, 1])
+            m = (y >= band[0]) & (y < band[1])
+            j += Vs[m, 0].mean() / K
+        ctx.close()
+        return j
+
+    t2 = time.perf_counter()
+    j0 = J_of(g, act)
+    t3 = time.perf_counter()
+    fd_g = {h: (J_of(g + h, act) - J_of(g - h, act)) / (2 * h) for h in hg}
+    fd_a = {h: (J_of(g, act + h) - J_of(g, act - h)) / (2 * h) for h in ha}
+    rel = lambda a, b: abs(a - b) / max(abs(b), 1e-300)
+    return dict(
+        workload=f"2D active nematic channel {Lx}x{Ly} cells x {ppc} ppc ({N} particles), lc 1, DIPOLE_DIR_SUM, "
+                 f"U 10, tumbling 0.5, rfc 0.01, activity {act}, g {g}, homeotropic strong anchoring at bounce-back "
+                 f"phantom walls, K {K} steps, J = time-averaged x-velocity of the centreline band y in [{band[0]}, "
+                 f"{band[1]})",
+        J=J, J_plain=j0, dJ_dg=dg, fd_dg={str(h): v for h, v in fd_g.items()},
+        rel_err_g={str(h): rel(dg, v) for h, v in fd_g.items()},
+        dJ_dact=da, fd_dact={str(h): v for h, v in fd_a.items()},
+        rel_err_act={str(h): rel(da, v) for h, v in fd_a.items()},
+        tangent_s_per_direction=(t1 - t0) / 2, plain_s=t3 - t2)
+
+
 def adjoint_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, angle=2.0, h=1e-11):
     """gradient of the time-averaged centreline velocity of a 2D SRD channel
     (bounce-back phantom walls in y, body force g along x) with respect to g
@@ -307,7 +382,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true", help="skip the FP32-vs-FP64 observable report")
     ap.add_argument("--no-adjoint", action="store_true",
-                    help="skip the pathwise-adjoint check (2D SRD channel, gradient vs finite differences)")
+                    help="skip the pathwise gradient checks (adjoint of a 2D SRD channel, tangent of an active "
+                         "nematic channel; gradients vs finite differences)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
     ap.add_argument("--no-lc", action="store_true", help="skip the 2D active-nematic throughput line")
@@ -393,6 +469,12 @@ def main():
     if not args.no_adjoint and args.only == "both":
         print("[bench] pathwise adjoint vs finite differences", file=sys.stderr, flush=True)
         adjoint = adjoint_check(M)
+    tangent = None
+    if not args.no_adjoint and args.only == "both":
+        print("[bench] forward-mode tangent of the active nematic channel vs finite differences", file=sys.stderr,
+              flush=True)
+        tangent = dict(c5=tangent_check(M, hg=(1e-11, 1e-12), ha=(1e-12, 3e-13, 1e-13)),
+                       box_32x16=tangent_check(M, Lx=32, Ly=16, K=100, hg=(1e-10,), ha=(1e-10,)))
     cpu = None if args.no_cpu else cpu_baseline()
     head_key = "fp32" if "fp32" in result else "fp64"
     r, n, B = result[head_key]
@@ -424,6 +506,7 @@ def main():
         "fp32_validation": validation,
         "ensemble_mpcat_a": ensemble,
         "adjoint": adjoint,
+        "tangent_active_nematic": tangent,
         "lc_active_nematic": lc,
     }
     if "fp64" in result and head_key != "fp64":
