@@ -33,10 +33,6 @@ enum { OP_SRD = 0, OP_AT = 1, OP_RAT = 2 };
 // particles staged in LDS at once (sized for about a dozen workgroups per CU)
 __host__ __device__ constexpr int tile_cap(bool fp64, int op) { return (fp64 || op != OP_SRD) ? 192 : 256; }
 
-struct TileDev {
-  int rep, c0, c1, pad;
-};
-
 // cell_of / shift_fwd / shift_back with the dimension known at compile time
 // (same arithmetic as the runtime-dimension versions)
 template <int DIM>

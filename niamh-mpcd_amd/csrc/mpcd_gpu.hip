@@ -32,6 +32,10 @@ struct mpcd_gpu_ctx {
   std::vector<RepDev> reps;
   std::vector<BlkDev> pblk, cblk;  // slot blocks, cell blocks
   std::vector<TileDev> tiles;      // cell tiles of the fused kernel
+  std::vector<TileDev> lct;        // cell tiles of the nematic collision kernel
+  std::vector<int2> lct_rng;
+  TileDev *d_lct = nullptr;
+  int2 *d_lct_rng = nullptr;
   std::vector<int2> cblk_rng, tile_rng;
   std::vector<int> uploaded;
   bool need_init = true;
@@ -390,6 +394,15 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   }
   c->nwtot = woff;
   build_tiles(c, c->tile_fill);
+  // nematic collision tiles: about half the kernel's LDS capacity on average
+  for (int r = 0; r < nrep; r++) {
+    const RepDev &R = c->reps[r];
+    const int t0 = (int)c->lct.size();
+    const int tc = std::max(1, std::min(kLcTileCells, (int)lround(0.5 * kLcCap / std::max(mean_ppc(R), 1.0))));
+    for (int b = 0; b < R.ncell; b += tc)
+      c->lct.push_back(TileDev{r, R.cell_off + b, R.cell_off + std::min(b + tc, R.ncell), 0});
+    c->lct_rng.push_back(make_int2(t0, (int)c->lct.size()));
+  }
   c->uploaded.assign(nrep, 0);
   c->lc_stride = anneal_stride(params[0]);
   // the fused kernel stages a cell in LDS: dense systems take the split path
@@ -406,6 +419,8 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->d_tiles = dalloc<TileDev>(c, c->nctot);  // room for the finest tiling
   c->d_cblk_rng = dalloc<int2>(c, nrep);
   c->d_tile_rng = dalloc<int2>(c, nrep);
+  c->d_lct = dalloc<TileDev>(c, c->lct.size());
+  c->d_lct_rng = dalloc<int2>(c, nrep);
   c->d_sp = dalloc<mpcd_gpu_species>(c, MPCD_MAXSPEC);
   c->d_wall = dalloc<mpcd_gpu_wall>(c, std::max<size_t>(1, walls.size()));
   c->shift2 = dalloc<double>(c, 2 * 3 * nrep);
@@ -422,6 +437,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->hist2[0] = dalloc<int32_t>(c, c->nctot + 1);
   c->hist2[1] = dalloc<int32_t>(c, c->nctot + 1);
   ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_tile_rng && c->d_sp &&
+       c->d_lct && c->d_lct_rng &&
        c->d_wall && c->shift2 && c->wqs2 && c->wqb2 && c->wh2 && c->box2 && c->hist2[0] && c->hist2[1];
   Ctx &C = c->C;
   C.rep = c->d_rep;
@@ -485,6 +501,8 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   up(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size());
   up(c->d_cblk_rng, c->cblk_rng.data(), sizeof(int2) * nrep);
   up(c->d_tile_rng, c->tile_rng.data(), sizeof(int2) * nrep);
+  up(c->d_lct, c->lct.data(), sizeof(TileDev) * c->lct.size());
+  up(c->d_lct_rng, c->lct_rng.data(), sizeof(int2) * nrep);
   up(c->d_sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC);
   if (woff) up(c->d_wall, walls.data(), sizeof(mpcd_gpu_wall) * woff);
   up(c->wqs2, wq.data(), sizeof(double) * wq.size());
@@ -717,7 +735,12 @@ static void run_props(mpcd_gpu_ctx *c, int do_acc) {
     if (tan) {
       k_cellprops<Real, Dual><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, do_acc, T);
     }
-  if (!tan) k_cellprops<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, do_acc, T);
+  if (!tan) {
+    if (c->dim == 3)
+      k_cellprops_t<Real, 3><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, do_acc);
+    else
+      k_cellprops_t<Real, 2><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, do_acc);
+  }
   if (c->lc != MPCD_ISOF) {
     k_velgrad_bulk<<<c->cblk.size(), kBlock, 0, c->stream>>>(C, c->d_cblk);
     k_velgrad_edges<<<c->nrep, 256, 0, c->stream>>>(C);
@@ -741,6 +764,9 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
   run_props<Real>(c, 1);
   const bool tan = std::is_same<Real, double>::value && c->tan_on;
   const TanDev T = tan ? tan_view(c, c->cur) : TanDev{};
+  // the nematic operators run on cell tiles staged in LDS (the tangent keeps
+  // the lane-per-cell kernel, whose values are the same)
+  const bool lc_tiled = c->lc != MPCD_ISOF && !tan;
   if (c->any_phantom) {
     ProfScope ps(c, F_GHOST);
     if constexpr (std::is_same<Real, double>::value)
@@ -769,10 +795,18 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
       if (tan) {
         k_collide<Real, Dual><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
       }
-    if (!tan) k_collide<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
+    if (!tan) {
+      if (lc_tiled)
+        if (c->dim == 3)
+          k_collide_lc<Real, 3><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, D);
+        else
+          k_collide_lc<Real, 2><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, D);
+      else
+        k_collide<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
+    }
   }
   ProfScope ps(c, F_TEMP);
-  k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(C, c->d_cblk_rng);
+  k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(C, lc_tiled ? c->d_lct_rng : c->d_cblk_rng);
   if (c->any_therm) k_scale<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk);
 }
 

@@ -44,6 +44,14 @@ struct BlkDev {
   int rep, beg, end, pad;
 };
 
+// a tile of consecutive cells [c0, c1) of one replica
+struct TileDev {
+  int rep, c0, c1, pad;
+};
+
+// tiles of the unfused path's cell kernels: slots staged in LDS per tile
+constexpr int kLcThreads = 256, kLcPPT = 2, kLcCap = kLcThreads * kLcPPT, kLcTileCells = 64;
+
 template <typename Real>
 struct State {
   Real *x, *y, *z, *vx, *vy, *vz, *ux, *uy, *uz;
@@ -442,12 +450,9 @@ __global__ void k_gather_tan(Ctx C, State<Real> S, TanDev Ts, TanDev Td, const B
 //    (lc.c:1125-1148, mtools.c:1443)
 // ---------------------------------------------------------------------------
 template <typename Real, typename F>
-__global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc, TanDev T) {
+__device__ void cellprops_cell(const Ctx &C, const State<Real> &S, int rep, int c, int do_acc, const TanDev &T) {
   using A = TA<F>;
-  const BlkDev B = cblk[blockIdx.x];
-  int c = B.beg + threadIdx.x;
-  if (c >= B.end) return;
-  const RepDev &R = C.rep[B.rep];
+  const RepDev &R = C.rep[rep];
   const int dim = R.dim;
   const bool ccm = (R.rtech == MPCD_RAT) || (R.lc != MPCD_ISOF);
   int b = C.start[c], e = C.start[c + 1];
@@ -536,6 +541,128 @@ __global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc
     if (Sv > 1.0) Sv = 1.0;
     A::st(C.cell.S, T.S, c, Sv);
     for (int i = 0; i < dim; i++) A::st(C.cell.dir, T.dir, c * 3 + i, dir[i]);
+  }
+}
+
+template <typename Real, typename F>
+__global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc, TanDev T) {
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  if (c < B.end) cellprops_cell<Real, F>(C, S, B.rep, c, do_acc, T);
+}
+
+// order tensor, eigen-decomposition and the S <= 1 clamp of one cell
+// (lc.c:1125-1148, mpc.c:189-209) from its sums
+template <int DIM>
+__device__ __forceinline__ void order_of_cell(Ctx &C, int c, double Sm[3][3], int n) {
+  double fD = (double)DIM, cc = 1. / (fD - 1.), POPinv = 1. / ((double)n);
+  for (int i = 0; i < DIM; i++)
+    for (int j = i; j < DIM; j++) Sm[i][j] *= cc * fD * POPinv;
+  for (int i = 0; i < DIM; i++) Sm[i][i] -= cc;
+  for (int i = 0; i < DIM; i++)
+    for (int j = i + 1; j < DIM; j++) Sm[j][i] = Sm[i][j];
+  double Sv, dir[3] = {0., 0., 0.};
+  if (DIM == 3) {
+    double ev[3];
+    eig3_top(Sm, ev, dir);
+    Sv = -1. * (ev[1] + ev[2]);
+  } else {
+    double m2[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
+    Sv = eig2_top(m2, dir);
+  }
+  if (Sv > 1.0) Sv = 1.0;
+  C.cell.S[c] = Sv;
+  for (int i = 0; i < DIM; i++) C.cell.dir[c * 3 + i] = dir[i];
+}
+
+// localPROP on a tile of consecutive cells (the nematic and unfused paths):
+// one lane per slot applies the body force and stages v, r, u and the mass
+// in LDS with coalesced loads; one lane per cell forms the sums in list
+// order with the arithmetic of cellprops_cell, whose values it gives bit for
+// bit.  A tile above the capacity falls back to cellprops_cell.
+template <typename Real, int DIM>
+__global__ __launch_bounds__(kLcThreads) void k_cellprops_t(Ctx C, State<Real> S, const TileDev *tiles, int do_acc) {
+  constexpr int NT = kLcThreads;
+  __shared__ double s_v[DIM][kLcCap], s_q[DIM][kLcCap], s_u[DIM][kLcCap], s_m[kLcCap];
+  const TileDev Tl = tiles[blockIdx.x];
+  const RepDev &R = C.rep[Tl.rep];
+  const int tid = threadIdx.x;
+  const int b0 = C.start[Tl.c0], n = C.start[Tl.c1] - b0, ncl = Tl.c1 - Tl.c0;
+  if (n > kLcCap) {
+    const TanDev T0{};
+    for (int k = tid; k < ncl; k += NT) cellprops_cell<Real, double>(C, S, Tl.rep, Tl.c0 + k, do_acc, T0);
+    return;
+  }
+  const bool ccm = (R.rtech == MPCD_RAT) || (R.lc != MPCD_ISOF), lc = R.lc != MPCD_ISOF;
+  const bool acc = do_acc && R.gravflag;
+  for (int i = tid; i < n; i += NT) {
+    const int t = b0 + i;
+    s_m[i] = C.sp[S.sp[t]].mass;
+    double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
+    if (acc) {
+      for (int d = 0; d < DIM; d++) v[d] = v[d] + R.dt * R.grav[d];
+      S.vx[t] = (Real)v[0];
+      S.vy[t] = (Real)v[1];
+      if (DIM == 3) S.vz[t] = (Real)v[2];
+      v[0] = (double)S.vx[t];
+      v[1] = (double)S.vy[t];
+      v[2] = (double)S.vz[t];
+    }
+    for (int d = 0; d < DIM; d++) s_v[d][i] = v[d];
+    if (ccm) {
+      double q[3];
+      load_q(S, t, q);
+      for (int d = 0; d < DIM; d++) s_q[d][i] = q[d];
+    }
+    if (lc) {
+      const Real *u[3] = {S.ux, S.uy, S.uz};
+      for (int d = 0; d < DIM; d++) s_u[d][i] = (double)u[d][t];
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < ncl; k += NT) {
+    const int c = Tl.c0 + k, ib = C.start[c] - b0, ie = C.start[c + 1] - b0;
+    double vcm[3], cm[3], mass = 0.0;
+    for (int d = 0; d < 3; d++) { vcm[d] = C.cell.vcm[c * 3 + d]; cm[d] = C.cell.cm[c * 3 + d]; }
+    for (int d = 0; d < DIM; d++) vcm[d] = 0.0;
+    if (ccm) for (int d = 0; d < DIM; d++) cm[d] = 0.0;
+    for (int i = ib; i < ie; i++) {
+      const double m = s_m[i];
+      mass += m;
+      for (int d = 0; d < DIM; d++) vcm[d] += s_v[d][i] * m;
+      if (ccm) for (int d = 0; d < DIM; d++) cm[d] += s_q[d][i] * m;
+    }
+    if (mass > 0.0)
+      for (int d = 0; d < DIM; d++) {
+        vcm[d] /= mass;
+        if (ccm) cm[d] /= mass;
+      }
+    C.cell.pop[c] = ie - ib;
+    C.cell.mass[c] = mass;
+    for (int d = 0; d < 3; d++) { C.cell.vcm[c * 3 + d] = vcm[d]; C.cell.cm[c * 3 + d] = cm[d]; }
+    if (ccm) {
+      double I[6] = {0., 0., 0., 0., 0., 0.};
+      for (int i = ib; i < ie; i++) {
+        const double m = s_m[i];
+        double r[3] = {0., 0., 0.};
+        for (int d = 0; d < DIM; d++) r[d] = s_q[d][i] - cm[d];
+        I[0] += m * (r[1] * r[1] + r[2] * r[2]);
+        I[1] += m * (r[0] * r[0] + r[2] * r[2]);
+        I[2] += m * (r[0] * r[0] + r[1] * r[1]);
+        I[3] += m * r[0] * r[1];
+        I[4] += m * r[0] * r[2];
+        I[5] += m * r[1] * r[2];
+      }
+      I[3] *= -1.; I[4] *= -1.; I[5] *= -1.;
+      for (int m = 0; m < 6; m++) C.cell.I[c * 6 + m] = I[m];
+    }
+    if (lc && ie - ib > 1) {
+      double Sm[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+      for (int i = ib; i < ie; i++)
+        for (int a = 0; a < DIM; a++)
+          for (int b = a; b < DIM; b++) Sm[a][b] += s_u[a][i] * s_u[b][i];
+      order_of_cell<DIM>(C, c, Sm, ie - ib);
+    }
   }
 }
 
@@ -935,16 +1062,60 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev 
 // 10. collision operators (MPCcollision switch, mpc.c:2777-2812), then the
 //     kinetic-energy partial sums of TEMP (therm.c:209) and gridShift back
 // ---------------------------------------------------------------------------
+// dipole activity of the cell holding slots [b, e) (dipoleAndersenROT_LC,
+// lc.c:1811-1841): ACT_i / m_i summed in list order over the particles that
+// pass MINACTRATIO, normalised (AV, SIG) and cut off (SIG, SIG_SUM), times dt.
+// spof(t) is the species of slot t; the species populations of the cell are
+// counted only for a species with a MINACTRATIO
+template <typename F, typename SpOf>
+__device__ __forceinline__ F cell_activity(const mpcd_gpu_species *sps, SpOf spof, const RepDev &R, int b, int e,
+                                           const TanDev &T) {
+  const int n = e - b;
+  F ACT = 0.;
+  double sigW = 0., sigP = 0.;
+  for (int t = b; t < e; t++) {
+    const int k = spof(t);
+    const mpcd_gpu_species &sp = sps[k];
+    bool pass = (sp.minactratio == 0.0);
+    if (!pass) {
+      int pop = 0;
+      for (int t2 = b; t2 < e; t2++) pop += (spof(t2) == k) ? 1 : 0;
+      pass = sp.minactratio * R.gndnst < (double)pop;
+    }
+    if (pass) {
+      ACT += TA<F>::par(sp.act, T.dact[k]) / sp.mass;
+      sigW += sp.sigwidth;
+      sigP += sp.sigpos;
+    }
+  }
+  if (R.rtech == MPCD_DIPOLE_DIR_AV || R.rtech == MPCD_DIPOLE_DIR_SIG) ACT *= R.gndnst / ((double)n);
+  if (R.rtech == MPCD_DIPOLE_DIR_SIG || R.rtech == MPCD_DIPOLE_DIR_SIG_SUM) {
+    sigW /= (double)n;
+    sigP /= (double)n;
+    double fall = (1 - tanh(((double)n - R.gndnst * (1 + sigP)) / (R.gndnst * sigW)));
+    ACT *= fall / 2;
+  }
+  ACT *= R.dt;
+  return ACT;
+}
+
+__device__ __forceinline__ int lc_mode(const RepDev &R) {
+  return (R.lc == MPCD_ISOF) ? 0
+         : (R.rtech == MPCD_DIPOLE_DIR_SUM || R.rtech == MPCD_DIPOLE_DIR_AV || R.rtech == MPCD_DIPOLE_DIR_SIG ||
+            R.rtech == MPCD_DIPOLE_DIR_SIG_SUM)
+             ? 2
+             : 1;
+}
+
+// the collision of cell c of replica `rep` and its kinetic-energy sum (one
+// lane per cell)
 template <typename Real, typename F>
-__global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+__device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int c, const Draws &Dr, const TanDev &T) {
   using A = TA<F>;
-  Dr.resolve();
-  const BlkDev B = cblk[blockIdx.x];
-  int c = B.beg + threadIdx.x;
-  const RepDev &R = C.rep[B.rep];
+  const RepDev &R = C.rep[rep];
   const int dim = R.dim;
   double ke = 0.0;
-  if (c < B.end) {
+  {
     int b = C.start[c], e = C.start[c + 1], n = e - b;
     F vcm[3] = {A::ld(C.cell.vcm, T.vcm, c * 3), A::ld(C.cell.vcm, T.vcm, c * 3 + 1),
                 A::ld(C.cell.vcm, T.vcm, c * 3 + 2)};
@@ -954,12 +1125,12 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
         double RV[3] = {0., 0., 0.}, M[3][3];
         int CA = 0;
         if (R.rtech == MPCD_ARBAXIS) {
-          if (dim == 3) Dr.cell_axis(B.rep, c, RV);
-          else { double x = Dr.cell_u(B.rep, c, 0); RV[0] = x; RV[1] = sqrt(1. - x * x); }
+          if (dim == 3) Dr.cell_axis(rep, c, RV);
+          else { double x = Dr.cell_u(rep, c, 0); RV[0] = x; RV[1] = sqrt(1. - x * x); }
         } else {
-          CA = (int)((int)dim * Dr.cell_u(B.rep, c, 0));
+          CA = (int)((int)dim * Dr.cell_u(rep, c, 0));
         }
-        double sgn = Dr.cell_u(B.rep, c, 3);
+        double sgn = Dr.cell_u(rep, c, 3);
         double Cc = R.cosA, Sn;
         if (R.true_sign ? (sgn >= 0.5) : ((long)sgn > 0.5)) Sn = R.sinA;
         else Sn = -1. * R.sinA;
@@ -1006,7 +1177,7 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
         double RS[3] = {0, 0, 0};
         for (int t = b; t < e; t++) {
           double m = C.sp[S.sp[t]].mass, sig = sqrt(R.kbt / m);
-          for (int j = 0; j < dim; j++) RS[j] += m * (sig * Dr.part_n(B.rep, t, j));
+          for (int j = 0; j < dim; j++) RS[j] += m * (sig * Dr.part_n(rep, t, j));
         }
         for (int j = 0; j < dim; j++) RS[j] /= C.cell.mass[c];
         for (int t = b; t < e; t++) {
@@ -1014,7 +1185,7 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
           double sig = sqrt(R.kbt / sp.mass);
           F v[3];
           for (int j = 0; j < dim; j++) {
-            double rv = sig * Dr.part_n(B.rep, t, j);
+            double rv = sig * Dr.part_n(rep, t, j);
             F dv = (sp.damp) * (vcm[j]) / ((double)n);
             v[j] = vcm[j] + rv - RS[j] - dv;
           }
@@ -1025,32 +1196,11 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
       } else {
         // andersenROT (mpc.c:1461-1657), andersenROT_LC (lc.c:1515-1728),
         // dipoleAndersenROT_LC (lc.c:1749-2043)
-        const int mode = (R.lc == MPCD_ISOF) ? 0
-                         : (R.rtech == MPCD_DIPOLE_DIR_SUM || R.rtech == MPCD_DIPOLE_DIR_AV ||
-                            R.rtech == MPCD_DIPOLE_DIR_SIG || R.rtech == MPCD_DIPOLE_DIR_SIG_SUM) ? 2 : 1;
+        const int mode = lc_mode(R);
         F cm[3] = {A::ld(C.cell.cm, T.cm, c * 3), A::ld(C.cell.cm, T.cm, c * 3 + 1), A::ld(C.cell.cm, T.cm, c * 3 + 2)};
         F ACT = 0., PA[3] = {0., 0., 0.};
         if (mode == 2) {
-          int spc[MPCD_MAXSPEC];
-          for (int k = 0; k < MPCD_MAXSPEC; k++) spc[k] = 0;
-          for (int t = b; t < e; t++) spc[S.sp[t]]++;
-          double sigW = 0., sigP = 0.;
-          for (int t = b; t < e; t++) {
-            const mpcd_gpu_species &sp = C.sp[S.sp[t]];
-            if ((sp.minactratio == 0.0) || (sp.minactratio * R.gndnst < (double)spc[S.sp[t]])) {
-              ACT += A::par(sp.act, T.dact[S.sp[t]]) / sp.mass;
-              sigW += sp.sigwidth;
-              sigP += sp.sigpos;
-            }
-          }
-          if (R.rtech == MPCD_DIPOLE_DIR_AV || R.rtech == MPCD_DIPOLE_DIR_SIG) ACT *= R.gndnst / ((double)n);
-          if (R.rtech == MPCD_DIPOLE_DIR_SIG || R.rtech == MPCD_DIPOLE_DIR_SIG_SUM) {
-            sigW /= (double)n;
-            sigP /= (double)n;
-            double fall = (1 - tanh(((double)n - R.gndnst * (1 + sigP)) / (R.gndnst * sigW)));
-            ACT *= fall / 2;
-          }
-          ACT *= R.dt;
+          ACT = cell_activity<F>(C.sp, [&](int t) { return (int)S.sp[t]; }, R, b, e, T);
           for (int j = 0; j < dim; j++) PA[j] = A::ld(C.cell.dir, T.dir, c * 3 + j);
         }
         // a tangent along the activity keeps the active terms where the
@@ -1061,7 +1211,7 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
         for (int t = b; t < e; t++) {
           const mpcd_gpu_species &sp = C.sp[S.sp[t]];
           double sig = sqrt(R.kbt / sp.mass);
-          for (int j = 0; j < dim; j++) RS[j] += sp.mass * (sig * Dr.part_n(B.rep, t, j));
+          for (int j = 0; j < dim; j++) RS[j] += sp.mass * (sig * Dr.part_n(rep, t, j));
           if (act) {
             F q[3], pW = 0.0;
             load_qt(S, T, t, q);
@@ -1093,7 +1243,7 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
           load_qt(S, T, t, q);
           for (int j = 0; j < dim; j++) rq[j] = q[j] - cm[j];
           for (int j = 0; j < dim; j++) {
-            double rv = sig * Dr.part_n(B.rep, t, j);
+            double rv = sig * Dr.part_n(rep, t, j);
             if (act) {
               F pW = 0.0;
               for (int jj = 0; jj < dim; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
@@ -1134,7 +1284,7 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
             pm = (pW <= 0) ? -1. : 1.;
           }
           for (int j = 0; j < dim; j++) {
-            double rv = sig * Dr.part_n(B.rep, t, j);
+            double rv = sig * Dr.part_n(rep, t, j);
             if (mode == 0) v[j] = vcm[j] + rv - RS[j] - (sp.damp) * (vcm[j]) + at[j];
             else if (mode == 1) v[j] = vcm[j] + rv - RS[j] + at[j];
             else v[j] = vcm[j] + rv - RS[j] + at[j] + (act ? pm * ACT * PA[j] : F(0.0)) - AS[j] - (sp.damp) * (vcm[j]);
@@ -1156,13 +1306,22 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
       if (R.galinv) {
         F q[3];
         load_qt(S, T, t, q);
-        shift_back(R, &C.shift[B.rep * 3], q);
+        shift_back(R, &C.shift[rep * 3], q);
         A::st(S.x, T.q[0], t, q[0]);
         A::st(S.y, T.q[1], t, q[1]);
         if (dim == 3) A::st(S.z, T.q[2], t, q[2]);
       }
     }
   }
+  return ke;
+}
+
+template <typename Real, typename F>
+__global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+  Dr.resolve();
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  const double ke = (c < B.end) ? collide_cell<Real, F>(C, S, B.rep, c, Dr, T) : 0.0;
   // block reduction of the kinetic energy
   __shared__ double sh[kBlock];
   sh[threadIdx.x] = ke;
@@ -1172,6 +1331,216 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
     __syncthreads();
   }
   if (threadIdx.x == 0) C.tpart[blockIdx.x] = sh[0];
+}
+
+// The nematic collision operators (andersenROT_LC, lc.c:1515-1728, and
+// dipoleAndersenROT_LC, lc.c:1749-2043) on a tile of consecutive cells whose
+// slots fit LDS.  One lane per slot draws its thermal velocity once and
+// computes its terms of the cell sums (momentum of the draws, active
+// momentum, angular momentum, torque) with the expressions of collide_cell;
+// one lane per cell adds the terms in list order and forms W = I^-1 L; one
+// lane per slot applies the update.  The values are collide_cell's bit for
+// bit; a tile above the capacity falls back to collide_cell.
+
+template <typename Real, int DIM>
+__global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S, const TileDev *tiles, Draws Dr) {
+  constexpr int NT = kLcThreads, NTERM = (DIM == 3) ? 12 : 6;
+  // terms per slot: 3D RS[3] AS[3] L[3] T dt[3]; 2D RS[2] AS[2] L_z (T dt)_z
+  __shared__ double s_term[NTERM][kLcCap];
+  __shared__ double s_act[kLcTileCells];
+  __shared__ int s_actf[kLcTileCells];
+  __shared__ double s_cell[kLcTileCells][9];  // RS, AS, W
+  __shared__ double s_ke[NT / 64];
+  __shared__ mpcd_gpu_species s_spc[MPCD_MAXSPEC];
+  __shared__ int32_t s_sp[kLcCap];
+  Dr.resolve();
+  const TileDev Tl = tiles[blockIdx.x];
+  const RepDev &R = C.rep[Tl.rep];
+  const int tid = threadIdx.x;
+  const int b0 = C.start[Tl.c0], n = C.start[Tl.c1] - b0, ncl = Tl.c1 - Tl.c0;
+  const int mode = lc_mode(R);
+  const TanDev T0{};
+  double ke = 0.0;
+  if (n > kLcCap) {
+    for (int k = tid; k < ncl; k += NT) ke += collide_cell<Real, double>(C, S, Tl.rep, Tl.c0 + k, Dr, T0);
+  } else {
+    // the species table and the tile's species ids in LDS
+    for (int k = tid; k < MPCD_MAXSPEC; k += NT) s_spc[k] = C.sp[k];
+    for (int i = tid; i < n; i += NT) s_sp[i] = S.sp[b0 + i];
+    __syncthreads();
+    // 0. one lane per cell: the activity
+    for (int k = tid; k < ncl; k += NT) {
+      const int c = Tl.c0 + k, b = C.start[c], e = C.start[c + 1];
+      double ACT = 0.;
+      if (mode == 2 && e - b > 1) ACT = cell_activity<double>(s_spc, [&](int t) { return s_sp[t - b0]; }, R, b, e, T0);
+      s_act[k] = ACT;
+      s_actf[k] = (mode == 2) && fneq(ACT, 0.0);
+    }
+    __syncthreads();
+    // 1. one lane per slot: draws and the terms of the cell sums
+    double rvk[kLcPPT][3], rqk[kLcPPT][3], pmk[kLcPPT];
+    int ck[kLcPPT];
+#pragma unroll
+    for (int j = 0; j < kLcPPT; j++) {
+      const int i = tid + j * NT;
+      ck[j] = -1;
+      pmk[j] = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) rvk[j][d] = rqk[j][d] = 0.0;
+      if (i >= n) continue;
+      const int t = b0 + i, c = S.cst[t], k = c - Tl.c0;
+      ck[j] = c;
+      if (C.start[c + 1] - C.start[c] <= 1) continue;
+      const mpcd_gpu_species &sp = s_spc[s_sp[i]];
+      const double sig = sqrt(R.kbt / sp.mass);
+      const double ACT = s_act[k];
+      const bool act = s_actf[k] != 0;
+      double cm[3] = {C.cell.cm[c * 3], C.cell.cm[c * 3 + 1], C.cell.cm[c * 3 + 2]}, PA[3] = {0., 0., 0.};
+      if (mode == 2)
+        for (int d = 0; d < DIM; d++) PA[d] = C.cell.dir[c * 3 + d];
+      double q[3], rq[3] = {0., 0., 0.}, dv[3] = {0., 0., 0.}, am[3];
+      const double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
+      load_q(S, t, q);
+      for (int d = 0; d < DIM; d++) rq[d] = q[d] - cm[d];
+      double pm = 0.0;
+      if (act) {
+        double pW = 0.0;
+        for (int jj = 0; jj < DIM; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
+        pW -= 0.0;
+        pm = (pW <= 0) ? -1. : 1.;
+      }
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const double rv = sig * Dr.part_n(Tl.rep, t, d);
+        rvk[j][d] = rv;
+        s_term[d][i] = sp.mass * rv;
+        s_term[DIM + d][i] = act ? sp.mass * (pm * ACT * PA[d]) : 0.0;
+        if (act) dv[d] = sp.mass * (v[d] - rv - pm * ACT * PA[d]);
+        else if (mode == 2) dv[d] = sp.mass * (v[d] - rv - 0.0);
+        else dv[d] = sp.mass * (v[d] - rv);
+      }
+      cross3(rq, dv, am);
+      if (DIM == 3) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          s_term[6 + d][i] = am[d];
+          s_term[9 + d][i] = C.T[(size_t)t * 3 + d] * R.dt;
+        }
+      } else {
+        s_term[4][i] = am[2];
+        s_term[5][i] = C.T[(size_t)t * 3 + 2] * R.dt;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; d++) rqk[j][d] = rq[d];
+      pmk[j] = pm;
+    }
+    __syncthreads();
+    // 2. one lane per cell: the sums in list order and W = I^-1 L
+    for (int k = tid; k < ncl; k += NT) {
+      const int c = Tl.c0 + k, ib = C.start[c] - b0, ie = C.start[c + 1] - b0, nn = ie - ib;
+      if (nn <= 1) continue;
+      const bool act = s_actf[k] != 0;
+      double RS[3] = {0, 0, 0}, AS[3] = {0., 0., 0.};
+      for (int i = ib; i < ie; i++) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) RS[d] += s_term[d][i];
+        if (act)
+#pragma unroll
+          for (int d = 0; d < DIM; d++) AS[d] += s_term[DIM + d][i];
+      }
+      for (int d = 0; d < DIM; d++) RS[d] /= C.cell.mass[c];
+      if (mode == 2)
+        for (int d = 0; d < DIM; d++) AS[d] /= C.cell.mass[c];
+      double II[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+      double I6[6];
+      for (int m = 0; m < 6; m++) I6[m] = C.cell.I[c * 6 + m];
+      if (DIM == 3) {
+        if (nn >= 3) {
+          double Im[3][3] = {{I6[0], I6[3], I6[4]}, {I6[3], I6[1], I6[5]}, {I6[4], I6[5], I6[2]}};
+          inv3(II, Im);
+        }
+      } else {
+        II[2][2] = 1. / I6[2];
+      }
+      double Llm[3] = {0., 0., 0.}, Lnm[3] = {0., 0., 0.}, L[3];
+      for (int i = ib; i < ie; i++) {
+        if (DIM == 3) {
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            Llm[d] += s_term[6 + d][i];
+            Lnm[d] += s_term[9 + d][i];
+          }
+        } else {
+          Llm[2] += s_term[4][i];
+          Lnm[2] += s_term[5][i];
+        }
+      }
+      for (int d = 0; d < 3; d++) L[d] = Llm[d] + Lnm[d];
+      double Wv[3];
+      for (int a = 0; a < 3; a++) {
+        Wv[a] = 0.;
+        for (int b = 0; b < 3; b++) Wv[a] += II[a][b] * L[b];
+      }
+      for (int d = 0; d < 3; d++) {
+        s_cell[k][d] = RS[d];
+        s_cell[k][3 + d] = AS[d];
+        s_cell[k][6 + d] = Wv[d];
+      }
+    }
+    __syncthreads();
+    // 3. one lane per slot: the update, the kinetic energy and gridShift back
+#pragma unroll
+    for (int j = 0; j < kLcPPT; j++) {
+      const int c = ck[j];
+      if (c < 0) continue;
+      const int t = b0 + tid + j * NT, k = c - Tl.c0;
+      const mpcd_gpu_species &sp = s_spc[s_sp[tid + j * NT]];
+      if (C.start[c + 1] - C.start[c] > 1) {
+        const double vcm[3] = {C.cell.vcm[c * 3], C.cell.vcm[c * 3 + 1], C.cell.vcm[c * 3 + 2]};
+        const double Wv[3] = {s_cell[k][6], s_cell[k][7], s_cell[k][8]};
+        double at[3], v[3];
+        cross3(Wv, rqk[j], at);
+        const bool act = s_actf[k] != 0;
+        const double ACT = s_act[k];
+        for (int d = 0; d < DIM; d++) {
+          const double rv = rvk[j][d], RSd = s_cell[k][d];
+          if (mode == 1) {
+            v[d] = vcm[d] + rv - RSd + at[d];
+          } else {
+            const double PAd = C.cell.dir[c * 3 + d];
+            v[d] = vcm[d] + rv - RSd + at[d] + (act ? pmk[j] * ACT * PAd : 0.0) - s_cell[k][3 + d] -
+                   (sp.damp) * (vcm[d]);
+          }
+        }
+        S.vx[t] = (Real)v[0];
+        S.vy[t] = (Real)v[1];
+        if (DIM == 3) S.vz[t] = (Real)v[2];
+      }
+      const double m = sp.mass;
+      double v0 = (double)S.vx[t] - R.avv[0], v1 = (double)S.vy[t] - R.avv[1], v2 = (double)S.vz[t] - R.avv[2];
+      double tmp = v0 * v0;
+      tmp += v1 * v1;
+      tmp += v2 * v2;
+      ke += m * tmp;
+      if (R.galinv) {
+        double q[3];
+        load_q(S, t, q);
+        shift_back(R, &C.shift[Tl.rep * 3], q);
+        S.x[t] = (Real)q[0];
+        S.y[t] = (Real)q[1];
+        if (DIM == 3) S.z[t] = (Real)q[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) ke += __shfl_xor(ke, o, 64);
+  if ((tid & 63) == 0) s_ke[tid >> 6] = ke;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; w++) t += s_ke[w];
+    C.tpart[blockIdx.x] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------
