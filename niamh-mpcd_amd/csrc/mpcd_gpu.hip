@@ -709,7 +709,7 @@ static void run_bin(mpcd_gpu_ctx *c, int mode, bool gather, int injected) {
   }
   {
     ProfScope ps(c, F_SORT);
-    k_sortcells<<<nblocks(c->nctot, kBlock), kBlock, 0, c->stream>>>(C, S[1 - c->cur].cst);
+    k_sortcells_t<<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[1 - c->cur].cst, c->d_lct);
   }
   if (gather) {
     ProfScope ps(c, F_GATHER);

@@ -362,21 +362,46 @@ __global__ void k_scatter(Ctx C) {
 // 5. sort every cell's keys (insertion sort; cells hold tens of particles)
 //    and record the cell of every new slot
 // ---------------------------------------------------------------------------
-__global__ void k_sortcells(Ctx C, int32_t *cst_new) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C.nctot) return;
-  int b = C.start[c], e = C.start[c + 1];
-  uint64_t *k = C.skey;
-  for (int i = b + 1; i < e; i++) {
-    uint64_t v = k[i];
-    int j = i - 1;
-    while (j >= b && k[j] > v) {
-      k[j + 1] = k[j];
-      j--;
+// One tile of consecutive cells per workgroup: the tile's keys (one
+// contiguous range) are staged in LDS and every key's position in its cell
+// is its rank there (the keys are distinct).  A tile above the capacity
+// falls back to an insertion sort, one lane per cell.
+__global__ __launch_bounds__(kLcThreads) void k_sortcells_t(Ctx C, int32_t *cst_new, const TileDev *tiles) {
+  constexpr int NT = kLcThreads;
+  __shared__ uint64_t s_k[kLcCap];
+  __shared__ int16_t s_c[kLcCap];
+  const TileDev Tl = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int b0 = C.start[Tl.c0], n = C.start[Tl.c1] - b0, ncl = Tl.c1 - Tl.c0;
+  if (n > kLcCap) {
+    for (int k = tid; k < ncl; k += NT) {
+      const int c = Tl.c0 + k, b = C.start[c], e = C.start[c + 1];
+      uint64_t *kk = C.skey;
+      for (int i = b + 1; i < e; i++) {
+        uint64_t v = kk[i];
+        int j = i - 1;
+        while (j >= b && kk[j] > v) {
+          kk[j + 1] = kk[j];
+          j--;
+        }
+        kk[j + 1] = v;
+      }
+      for (int i = b; i < e; i++) cst_new[i] = c;
     }
-    k[j + 1] = v;
+    return;
   }
-  for (int i = b; i < e; i++) cst_new[i] = c;
+  for (int i = tid; i < n; i += NT) s_k[i] = C.skey[b0 + i];
+  for (int k = tid; k < ncl; k += NT)
+    for (int i = C.start[Tl.c0 + k] - b0; i < C.start[Tl.c0 + k + 1] - b0; i++) s_c[i] = (int16_t)k;
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    const int k = s_c[i], c = Tl.c0 + k, ib = C.start[c] - b0, ie = C.start[c + 1] - b0;
+    const uint64_t v = s_k[i];
+    int r = 0;
+    for (int j = ib; j < ie; j++) r += (s_k[j] < v) ? 1 : 0;
+    C.skey[b0 + ib + r] = v;
+    cst_new[b0 + i] = c;
+  }
 }
 
 // ---------------------------------------------------------------------------
