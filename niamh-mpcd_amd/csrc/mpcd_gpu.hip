@@ -818,13 +818,15 @@ static void run_stream(mpcd_gpu_ctx *c, int injected) {
   ProfScope ps(c, F_STREAM);
   const bool tape = c->recording && c->tape_len < c->tape_cap;
   float *wmul = tape ? tape_at(c, c->tape_len).wmul : nullptr;
+  const double *box = c->box2 + (c->step & 1) * 6 * c->nrep;  // this step's wall-free boxes
   const bool tan = std::is_same<Real, double>::value && c->tan_on;
   if constexpr (std::is_same<Real, double>::value)
     if (tan) {
       k_stream<Real, Dual><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, wmul,
-                                                                     tan_view(c, c->cur));
+                                                                     tan_view(c, c->cur), box);
     }
-  if (!tan) k_stream<Real, double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, wmul, TanDev{});
+  if (!tan)
+    k_stream<Real, double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, wmul, TanDev{}, box);
   if (tape) c->tape_len++;
   c->pending = true;
   c->cu_src = 0;

@@ -1590,7 +1590,8 @@ __global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
 //     bc.c:1363-1496), then the unshifted cell of every particle
 // ---------------------------------------------------------------------------
 template <typename Real, typename F>
-__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul, TanDev T) {
+__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul, TanDev T,
+                         const double *boxes) {
   using A = TA<F>;
   Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
@@ -1606,6 +1607,11 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, floa
   const int spid = S.sp[t];
   const mpcd_gpu_species &sp = C.sp[spid];
   for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
+  // a particle strictly inside the step's wall-free box crosses no wall: the
+  // search below would end at once
+  const double *box = &boxes[B.rep * 6];
+  bool inside = true;
+  for (int i = 0; i < dim; i++) inside = inside && (Q[i] > box[2 * i]) && (Q[i] < box[2 * i + 1]);
   // MPC_BCcollision
   F t_step = R.dt, t_left = t_step, t_coll = 0.0;
   int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
@@ -1613,7 +1619,7 @@ __global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, floa
   // adjoint tape: the product of the velocity factors of the bounces (MVN on
   // the wall's axis, MVT on the others); NaN when the velocity is redrawn
   float mul[3] = {1.f, 1.f, 1.f};
-  while (true) {
+  while (!inside) {
     F tmin = t_left, chW = 1.0;
     for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
       const mpcd_gpu_wall &W = C.wall[w];
