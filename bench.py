@@ -387,7 +387,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
     ap.add_argument("--no-lc", action="store_true", help="skip the 2D active-nematic throughput line")
-    ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--only", choices=["fp32", "fp64", "both", "lc"], default="both",
+                    help="lc: only the 2D active nematic throughput (its JSON instead of the bench line)")
     ap.add_argument("--fill", type=float, default=0.0,
                     help="fix the bucket fill of the run layout instead of autotuning it (reproducible profiles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -401,6 +402,9 @@ def main():
               "replicas would shard across GPUs with no communication", file=sys.stderr)
         sys.exit(2)
     M = package()
+    if args.only == "lc":
+        print(json.dumps(lc_throughput(M, steps=args.steps, warmup=args.warmup)))
+        return
     L = (args.L,) * 3
     cfg = srd_cfg(L, args.ppc)
     result = {}
