@@ -457,6 +457,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   C.cell.S = dalloc<double>(c, c->nctot);
   C.cell.dir = dalloc<double>(c, 3 * (size_t)c->nctot);
   C.cell.E = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 9 * (size_t)c->nctot) : nullptr;
+  C.cell.avm = (c->lc != MPCD_ISOF) ? dalloc<double>(c, (size_t)c->nctot) : nullptr;
   C.T = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 3 * (size_t)c->ntot) : nullptr;
   C.tpart = dalloc<double>(c, std::max<size_t>(c->cblk.size(), c->nctot));
   C.kbtnow = dalloc<double>(c, nrep);
@@ -472,7 +473,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
        C.cell.cm && C.cell.I && C.cell.S && C.cell.dir && C.tpart && C.kbtnow && C.gscale && C.sg && C.events &&
        C.flags;
   bool lcm = c->lc != MPCD_ISOF;
-  if (lcm) ok = ok && C.cell.E && C.T;
+  if (lcm) ok = ok && C.cell.E && C.cell.avm && C.T;
   if (c->any_lcg) ok = ok && c->d_avpart;
   if (c->precision == MPCD_FP64) {
     c->sd[0] = alloc_state<double>(c, c->ntot, lcm);

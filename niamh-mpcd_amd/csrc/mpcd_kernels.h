@@ -63,6 +63,7 @@ struct State {
 struct CellDev {
   int32_t *pop;
   double *mass, *vcm, *cm, *I, *S, *dir, *E;
+  double *avm;  // nematic cells: mean mean-field potential (lc.c:495), cells of two or more
 };
 
 struct Ctx {
@@ -566,6 +567,10 @@ __device__ void cellprops_cell(const Ctx &C, const State<Real> &S, int rep, int 
     if (Sv > 1.0) Sv = 1.0;
     A::st(C.cell.S, T.S, c, Sv);
     for (int i = 0; i < dim; i++) A::st(C.cell.dir, T.dir, c * 3 + i, dir[i]);
+    double avM = 0.0;
+    for (int t = b; t < e; t++) avM += C.sp[S.sp[t]].mfpot;
+    avM /= (float)(e - b);
+    C.cell.avm[c] = avM;
   }
 }
 
@@ -608,7 +613,7 @@ __device__ __forceinline__ void order_of_cell(Ctx &C, int c, double Sm[3][3], in
 template <typename Real, int DIM>
 __global__ __launch_bounds__(kLcThreads) void k_cellprops_t(Ctx C, State<Real> S, const TileDev *tiles, int do_acc) {
   constexpr int NT = kLcThreads;
-  __shared__ double s_v[DIM][kLcCap], s_q[DIM][kLcCap], s_u[DIM][kLcCap], s_m[kLcCap];
+  __shared__ double s_v[DIM][kLcCap], s_q[DIM][kLcCap], s_u[DIM][kLcCap], s_m[kLcCap], s_mf[kLcCap];
   const TileDev Tl = tiles[blockIdx.x];
   const RepDev &R = C.rep[Tl.rep];
   const int tid = threadIdx.x;
@@ -622,7 +627,9 @@ __global__ __launch_bounds__(kLcThreads) void k_cellprops_t(Ctx C, State<Real> S
   const bool acc = do_acc && R.gravflag;
   for (int i = tid; i < n; i += NT) {
     const int t = b0 + i;
-    s_m[i] = C.sp[S.sp[t]].mass;
+    const mpcd_gpu_species &sp = C.sp[S.sp[t]];
+    s_m[i] = sp.mass;
+    s_mf[i] = sp.mfpot;
     double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
     if (acc) {
       for (int d = 0; d < DIM; d++) v[d] = v[d] + R.dt * R.grav[d];
@@ -687,6 +694,10 @@ __global__ __launch_bounds__(kLcThreads) void k_cellprops_t(Ctx C, State<Real> S
         for (int a = 0; a < DIM; a++)
           for (int b = a; b < DIM; b++) Sm[a][b] += s_u[a][i] * s_u[b][i];
       order_of_cell<DIM>(C, c, Sm, ie - ib);
+      double avM = 0.0;
+      for (int i = ib; i < ie; i++) avM += s_mf[i];
+      avM /= (float)(ie - ib);
+      C.cell.avm[c] = avM;
     }
   }
 }
@@ -1038,9 +1049,7 @@ __global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev 
   cross3(DIR, xaxis, rotAx);
   F rotAngle = -1. * abs_angle(DIR, xaxis, dim);
   normn(rotAx, 3);
-  double avM = 0.0;
-  for (int t2 = b; t2 < e; t2++) avM += C.sp[S.sp[t2]].mfpot;
-  avM /= (float)(e - b);
+  const double avM = C.cell.avm[c];  // summed in list order by localPROP
   double effM = avM * 0.5 * (double)dim;
   const mpcd_gpu_species &sp = C.sp[S.sp[t]];
   {
