@@ -64,7 +64,7 @@ __device__ __forceinline__ void srd_matrix(int dim, int rtech, int CA, const dou
 // of the step and before its collision: record the cell's linear map and
 // its particles' pre-collision velocities
 template <typename Real>
-__global__ void k_tape_cells(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TapeDev T) {
+__global__ __launch_bounds__(kBlock) void k_tape_cells(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TapeDev T) {
   Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   const int c = B.beg + threadIdx.x;
@@ -139,7 +139,7 @@ __global__ void k_adj_gather(TapeDev T, const double *lam, const double *adj_in,
 }
 
 // one lane per cell: the transposed collision, the parameter derivatives
-__global__ void k_adj_cells(Ctx C, TapeDev T, const BlkDev *cblk, const double *lam_s, double *lam1_s,
+__global__ __launch_bounds__(kBlock) void k_adj_cells(Ctx C, TapeDev T, const BlkDev *cblk, const double *lam_s, double *lam1_s,
                             double *dtheta) {
   const BlkDev B = cblk[blockIdx.x];
   const int c = B.beg + threadIdx.x;

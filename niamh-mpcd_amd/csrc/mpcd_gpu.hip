@@ -36,6 +36,7 @@ struct mpcd_gpu_ctx {
   std::vector<int2> lct_rng;
   TileDev *d_lct = nullptr;
   int2 *d_lct_rng = nullptr;
+  int32_t *d_lct_over = nullptr;  // [1 + tiles]: count, then the tiles above k_collide_lc's capacity
   std::vector<int2> cblk_rng, tile_rng;
   std::vector<int> uploaded;
   bool need_init = true;
@@ -421,6 +422,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->d_tile_rng = dalloc<int2>(c, nrep);
   c->d_lct = dalloc<TileDev>(c, c->lct.size());
   c->d_lct_rng = dalloc<int2>(c, nrep);
+  c->d_lct_over = dalloc<int32_t>(c, c->lct.size() + 1);
   c->d_sp = dalloc<mpcd_gpu_species>(c, MPCD_MAXSPEC);
   c->d_wall = dalloc<mpcd_gpu_wall>(c, std::max<size_t>(1, walls.size()));
   c->shift2 = dalloc<double>(c, 2 * 3 * nrep);
@@ -437,7 +439,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->hist2[0] = dalloc<int32_t>(c, c->nctot + 1);
   c->hist2[1] = dalloc<int32_t>(c, c->nctot + 1);
   ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_tile_rng && c->d_sp &&
-       c->d_lct && c->d_lct_rng &&
+       c->d_lct && c->d_lct_rng && c->d_lct_over &&
        c->d_wall && c->shift2 && c->wqs2 && c->wqb2 && c->wh2 && c->box2 && c->hist2[0] && c->hist2[1];
   Ctx &C = c->C;
   C.rep = c->d_rep;
@@ -797,12 +799,15 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
         k_collide<Real, Dual><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
       }
     if (!tan) {
-      if (lc_tiled)
+      if (lc_tiled) {
         if (c->dim == 3)
-          k_collide_lc<Real, 3><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, D);
+          k_collide_lc<Real, 3><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, D,
+                                                                             c->d_lct_over);
         else
-          k_collide_lc<Real, 2><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, D);
-      else
+          k_collide_lc<Real, 2><<<c->lct.size(), kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, D,
+                                                                             c->d_lct_over);
+        k_collide_lc_over<Real><<<1, kLcThreads, 0, c->stream>>>(C, S[c->cur], c->d_lct, D, c->d_lct_over);
+      } else
         k_collide<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
     }
   }

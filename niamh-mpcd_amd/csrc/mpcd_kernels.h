@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kLcThreads) void k_sortcells_t(Ctx C, int32_t *cst_
 // 6. gather the state into the new order, applying the grid shift
 // ---------------------------------------------------------------------------
 template <typename Real>
-__global__ void k_gather(Ctx C, State<Real> S, State<Real> D, const BlkDev *blk, int shifted, int lc, Draws Dr) {
+__global__ __launch_bounds__(kBlock) void k_gather(Ctx C, State<Real> S, State<Real> D, const BlkDev *blk, int shifted, int lc, Draws Dr) {
   Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
@@ -575,7 +575,7 @@ __device__ void cellprops_cell(const Ctx &C, const State<Real> &S, int rep, int 
 }
 
 template <typename Real, typename F>
-__global__ void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc, TanDev T) {
+__global__ __launch_bounds__(kBlock) void k_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc, TanDev T) {
   const BlkDev B = cblk[blockIdx.x];
   const int c = B.beg + threadIdx.x;
   if (c < B.end) cellprops_cell<Real, F>(C, S, B.rep, c, do_acc, T);
@@ -738,7 +738,7 @@ __device__ __forceinline__ void ori_bc(F *u, const mpcd_gpu_wall &W, const F *n,
 }
 
 template <typename Real, typename F>
-__global__ void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+__global__ __launch_bounds__(kBlock) void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
   using A = TA<F>;
   Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
@@ -1032,7 +1032,7 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
 // in list order), and the Metropolis draws are keyed by slot, so the result
 // does not depend on the mapping
 template <typename Real, typename F>
-__global__ void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev T) {
+__global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev T) {
   using A = TA<F>;
   Dr.resolve();
   const BlkDev B = pblk[blockIdx.x];
@@ -1351,7 +1351,7 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
 }
 
 template <typename Real, typename F>
-__global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+__global__ __launch_bounds__(kBlock) void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
   Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   const int c = B.beg + threadIdx.x;
@@ -1374,10 +1374,11 @@ __global__ void k_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, Ta
 // momentum, angular momentum, torque) with the expressions of collide_cell;
 // one lane per cell adds the terms in list order and forms W = I^-1 L; one
 // lane per slot applies the update.  The values are collide_cell's bit for
-// bit; a tile above the capacity falls back to collide_cell.
+// bit; a tile above the capacity is left to k_collide_lc_over.
 
 template <typename Real, int DIM>
-__global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S, const TileDev *tiles, Draws Dr) {
+__global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S, const TileDev *tiles, Draws Dr,
+                                                           int32_t *over) {
   constexpr int NT = kLcThreads, NTERM = (DIM == 3) ? 12 : 6;
   // terms per slot: 3D RS[3] AS[3] L[3] T dt[3]; 2D RS[2] AS[2] L_z (T dt)_z
   __shared__ double s_term[NTERM][kLcCap];
@@ -1396,7 +1397,8 @@ __global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S,
   const TanDev T0{};
   double ke = 0.0;
   if (n > kLcCap) {
-    for (int k = tid; k < ncl; k += NT) ke += collide_cell<Real, double>(C, S, Tl.rep, Tl.c0 + k, Dr, T0);
+    // k_collide_lc_over takes the tile (and writes its kinetic energy)
+    if (tid == 0) over[1 + atomicAdd(&over[0], 1)] = blockIdx.x;
   } else {
     // the species table and the tile's species ids in LDS
     for (int k = tid; k < MPCD_MAXSPEC; k += NT) s_spc[k] = C.sp[k];
@@ -1577,6 +1579,35 @@ __global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S,
   }
 }
 
+// the tiles k_collide_lc found above its capacity, one lane per cell (one
+// workgroup, so it can reset the list for the next step when done)
+template <typename Real>
+__global__ __launch_bounds__(kLcThreads) void k_collide_lc_over(Ctx C, State<Real> S, const TileDev *tiles, Draws Dr,
+                                                                int32_t *over) {
+  constexpr int NT = kLcThreads;
+  __shared__ double s_ke[NT / 64];
+  Dr.resolve();
+  const int tid = threadIdx.x, nover = over[0];
+  const TanDev T0{};
+  for (int j = 0; j < nover; j++) {
+    const int tile = over[1 + j];
+    const TileDev Tl = tiles[tile];
+    double ke = 0.0;
+    for (int k = tid; k < Tl.c1 - Tl.c0; k += NT) ke += collide_cell<Real, double>(C, S, Tl.rep, Tl.c0 + k, Dr, T0);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ke += __shfl_xor(ke, o, 64);
+    if ((tid & 63) == 0) s_ke[tid >> 6] = ke;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < NT / 64; w++) t += s_ke[w];
+      C.tpart[tile] = t;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) over[0] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // 11. thermostat velocity scaling (scaleT, therm.c:52-143); the factor comes
 //     from k_temp_reduce
@@ -1599,7 +1630,7 @@ __global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
 //     bc.c:1363-1496), then the unshifted cell of every particle
 // ---------------------------------------------------------------------------
 template <typename Real, typename F>
-__global__ void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul, TanDev T,
+__global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul, TanDev T,
                          const double *boxes) {
   using A = TA<F>;
   Dr.resolve();
