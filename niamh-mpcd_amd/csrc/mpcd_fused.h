@@ -632,7 +632,7 @@ __global__ void k_celldraws(Ctx C, const BlkDev *__restrict__ cblk, Draws D, dou
 
 // stream after a global thermostat: scale, stream, walls, next bin
 template <typename Real, int DIM>
-__global__ void k_stream_next(Ctx C, State<Real> S, const BlkDev *__restrict__ blk, const RepDev *__restrict__ reps,
+__global__ __launch_bounds__(kBlock) void k_stream_next(Ctx C, State<Real> S, const BlkDev *__restrict__ blk, const RepDev *__restrict__ reps,
                               const mpcd_gpu_species *__restrict__ sps, const mpcd_gpu_wall *__restrict__ walls,
                               const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr,
                               const double *__restrict__ wq_b, const double *__restrict__ shift_next,

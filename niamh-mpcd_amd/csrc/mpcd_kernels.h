@@ -276,6 +276,27 @@ __global__ void k_shift(Ctx C, Draws D, int do_shift, const double *wq_prev_b, d
 // 2. histogram + composite key.  mode 0: initial binning (binin, mpc.c:924);
 //    1: shifted bin; 2: materialise the pending unshifted bin
 // ---------------------------------------------------------------------------
+// hist[cell]++ for every active lane, returning each lane's old count: one
+// atomic per distinct cell of the wavefront (sorted slots put most of a
+// wavefront's particles in a few cells).  The counts a cell's lanes get are
+// distinct, which is all the scatter needs: the keys are sorted in the cell.
+__device__ __forceinline__ int hist_add(int32_t *hist, int cell) {
+  const int lane = (int)__lane_id();
+  uint64_t pending = __ballot(1);
+  int loc = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lead_cell = __shfl(cell, leader, 64);
+    const uint64_t peers = __ballot(cell == lead_cell) & pending;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&hist[lead_cell], __popcll(peers));
+    base = __shfl(base, leader, 64);
+    if ((peers >> lane) & 1ull) loc = base + __popcll(peers & ((1ull << lane) - 1ull));
+    pending &= ~peers;
+  }
+  return loc;
+}
+
 template <typename Real>
 __global__ void k_count(Ctx C, State<Real> S, const BlkDev *blk, int mode, int pending, int cu_src) {
   const BlkDev B = blk[blockIdx.x];
@@ -305,7 +326,7 @@ __global__ void k_count(Ctx C, State<Real> S, const BlkDev *blk, int mode, int p
     }
     a2 = (cnew != cu) ? 1u : 0u;
   }
-  int l = atomicAdd(&C.hist[cnew], 1);
+  int l = hist_add(C.hist, cnew);
   C.cnew[s] = cnew;
   C.loc[s] = l;
   C.key[s] = ((uint64_t)a2 << 63) | ((uint64_t)(uint32_t)cu << 32) | ((uint64_t)a1 << 31) | (uint64_t)(uint32_t)s;
