@@ -602,7 +602,7 @@ __global__ void k_bucketize(Ctx C, State<Real> S, BState<Real> O, const BucketRe
   int r = 0;
   while (r + 1 < C.nrep && C.rep[r + 1].cell_off <= cn) r++;
   const int b = bucket_of(brep[r], C.rep[r], cn);
-  const int p = atomicAdd(&O.cnt[b], 1);
+  const int p = hist_add(O.cnt, b);  // consecutive slots share a few buckets
   if (p >= cap_lim) {
     atomicOr(&C.flags[0], kFlagPoison);
     atomicMin(&C.flags[1], -1);
@@ -642,7 +642,7 @@ __global__ void k_densify(Ctx C, BState<Real> I, State<Real> S, const TileDev *_
     S.sp[slot] = onesp ? 0 : sp;
     const int cn = next_cell<DIM>(R, &shift[T.rep * 3], cu, q);
     C.cnew[slot] = cn;
-    C.loc[slot] = atomicAdd(&C.hist[cn], 1);
+    C.loc[slot] = hist_add(C.hist, cn);
     C.key[slot] = key;
     // last cell of the replica whose start is <= slot
     int lo = R.cell_off, hi = R.cell_off + R.ncell;
