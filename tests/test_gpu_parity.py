@@ -124,6 +124,10 @@ def _random_box(dim, L, ppc, rtech, lc=0, seed=5, walls=None, **kw):
     (3, (16, 16, 16), 10, 0, 0),
     (3, (12, 10, 14), 10, 3, 0),
     (2, (40, 40), 20, 16, 1),
+    # 600 particles per cell: every cell tile of the unfused kernels is above
+    # their LDS capacity, so the lane-per-cell fallbacks run
+    (2, (3, 3), 600, 16, 1),
+    (3, (3, 3, 3), 600, 3, 0),
 ])
 def test_gpu_philox_vs_oracle(spec):
     import pyoracle
