@@ -138,6 +138,18 @@ const char *mpcd_gpu_last_error(const mpcd_gpu_ctx *ctx);
 mpcd_status mpcd_gpu_upload(mpcd_gpu_ctx *ctx, int replica, const double *Q, const double *V, const double *U,
                             const int32_t *spid);
 mpcd_status mpcd_gpu_download(mpcd_gpu_ctx *ctx, int replica, double *Q, double *V, double *U);
+/* The same from / into an array of structures such as the reference's
+ * particleMPC (SRDclss.h:34-44, 128 B records): particle i's field is at
+ * (char *)base + i * stride + off.  off_q, off_v: double[3]; off_u: double[3]
+ * or -1 (none); off_spid: int32 or -1 (none).  A download writes Q, V and
+ * (off_u >= 0) U only, leaving every other byte of the records as it was.
+ * From C: mpcd_gpu_upload_aos(ctx, 0, P, sizeof(particleMPC),
+ * offsetof(particleMPC, Q), offsetof(particleMPC, V), offsetof(particleMPC, U),
+ * offsetof(particleMPC, SPID)). */
+mpcd_status mpcd_gpu_upload_aos(mpcd_gpu_ctx *ctx, int replica, const void *base, int64_t stride, int64_t off_q,
+                                int64_t off_v, int64_t off_u, int64_t off_spid);
+mpcd_status mpcd_gpu_download_aos(mpcd_gpu_ctx *ctx, int replica, void *base, int64_t stride, int64_t off_q,
+                                  int64_t off_v, int64_t off_u);
 
 /* --- the step, phase by phase (see the table at the top) ------------------ */
 /* shifted = 1: draw (or take) the grid shift and bin the shifted positions.

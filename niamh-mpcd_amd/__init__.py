@@ -116,7 +116,8 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_lc_stride", "mpcd_gpu_profile", "mpcd_gpu_timing", "mpcd_gpu_autotune",
            "mpcd_gpu_tile_fill", "mpcd_gpu_checkpoint", "mpcd_gpu_restore", "mpcd_gpu_record",
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
-           "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download"]
+           "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
+           "mpcd_gpu_download_aos"]
 
 _lib = None
 
@@ -159,6 +160,9 @@ def lib():
     L.mpcd_gpu_adjoint_reset.argtypes = [vp]
     L.mpcd_gpu_step_backward.argtypes = [vp, dp, dp, dp]
     L.mpcd_gpu_tangent.argtypes = [vp, C.c_int, dp, dp]
+    L.mpcd_gpu_upload_aos.argtypes = [vp, C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                      C.c_int64]
+    L.mpcd_gpu_download_aos.argtypes = [vp, C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64]
     L.mpcd_gpu_tangent_download.argtypes = [vp, C.c_int, dp, dp, dp]
     _lib = L
     return L
@@ -262,6 +266,23 @@ class Context:
         U = None if U is None else np.ascontiguousarray(U, np.float64)
         spid = None if spid is None else np.ascontiguousarray(spid, np.int32)
         self._check(self.L.mpcd_gpu_upload(self.h, r, _dp(Q), _dp(V), _dp(U), _ip(spid)), "upload")
+
+    def upload_aos(self, r, rec, q="Q", v="V", u=None, spid=None):
+        """upload from a numpy structured array of particle records (e.g. the
+        reference's particleMPC layout); q, v, u name double[3] fields, spid an int32 field"""
+        rec = np.ascontiguousarray(rec)
+        f = rec.dtype.fields
+        off = lambda name: -1 if name is None else f[name][1]
+        self._check(self.L.mpcd_gpu_upload_aos(self.h, r, rec.ctypes.data, rec.dtype.itemsize, off(q), off(v), off(u),
+                                               off(spid)), "upload_aos")
+
+    def download_aos(self, r, rec, q="Q", v="V", u=None):
+        """write Q, V (and U) into a contiguous numpy structured array in place"""
+        assert rec.flags["C_CONTIGUOUS"]
+        f = rec.dtype.fields
+        off = lambda name: -1 if name is None else f[name][1]
+        self._check(self.L.mpcd_gpu_download_aos(self.h, r, rec.ctypes.data, rec.dtype.itemsize, off(q), off(v),
+                                                 off(u)), "download_aos")
 
     def download(self, r):
         n = int(self.n[r])

@@ -1064,6 +1064,50 @@ extern "C" mpcd_status mpcd_gpu_download(mpcd_gpu_ctx *c, int r, double *Q, doub
   return check_launch(c);
 }
 
+// the records of an array of structures (the reference's particleMPC): base
+// + i * stride + offset holds particle i's field
+static bool aos_fields_ok(int64_t stride, int64_t off, int64_t bytes) { return off < 0 || off + bytes <= stride; }
+
+extern "C" mpcd_status mpcd_gpu_upload_aos(mpcd_gpu_ctx *c, int r, const void *base, int64_t stride, int64_t off_q,
+                                           int64_t off_v, int64_t off_u, int64_t off_spid) {
+  if (!c || r < 0 || r >= c->nrep || !base || stride <= 0 || off_q < 0 || off_v < 0 ||
+      !aos_fields_ok(stride, off_q, 24) || !aos_fields_ok(stride, off_v, 24) || !aos_fields_ok(stride, off_u, 24) ||
+      !aos_fields_ok(stride, off_spid, 4))
+    return c ? fail(c, MPCD_EINVAL, "upload_aos: bad argument") : MPCD_EINVAL;
+  const size_t n = c->reps[r].npart;
+  std::vector<double> Q(3 * n), V(3 * n), U(off_u >= 0 ? 3 * n : 0);
+  std::vector<int32_t> sp(off_spid >= 0 ? n : 0);
+  const char *b = (const char *)base;
+  for (size_t i = 0; i < n; i++) {
+    const char *rec = b + i * (size_t)stride;
+    memcpy(&Q[3 * i], rec + off_q, 24);
+    memcpy(&V[3 * i], rec + off_v, 24);
+    if (off_u >= 0) memcpy(&U[3 * i], rec + off_u, 24);
+    if (off_spid >= 0) memcpy(&sp[i], rec + off_spid, 4);
+  }
+  return mpcd_gpu_upload(c, r, Q.data(), V.data(), off_u >= 0 ? U.data() : nullptr,
+                         off_spid >= 0 ? sp.data() : nullptr);
+}
+
+extern "C" mpcd_status mpcd_gpu_download_aos(mpcd_gpu_ctx *c, int r, void *base, int64_t stride, int64_t off_q,
+                                             int64_t off_v, int64_t off_u) {
+  if (!c || r < 0 || r >= c->nrep || !base || stride <= 0 || off_q < 0 || off_v < 0 ||
+      !aos_fields_ok(stride, off_q, 24) || !aos_fields_ok(stride, off_v, 24) || !aos_fields_ok(stride, off_u, 24))
+    return c ? fail(c, MPCD_EINVAL, "download_aos: bad argument") : MPCD_EINVAL;
+  const size_t n = c->reps[r].npart;
+  std::vector<double> Q(3 * n), V(3 * n), U(3 * n);
+  mpcd_status st = mpcd_gpu_download(c, r, Q.data(), V.data(), U.data());
+  if (st) return st;
+  char *b = (char *)base;
+  for (size_t i = 0; i < n; i++) {
+    char *rec = b + i * (size_t)stride;
+    memcpy(rec + off_q, &Q[3 * i], 24);
+    memcpy(rec + off_v, &V[3 * i], 24);
+    if (off_u >= 0) memcpy(rec + off_u, &U[3 * i], 24);
+  }
+  return MPCD_OK;
+}
+
 extern "C" mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *c, const mpcd_draws *draws, int shifted, int32_t *cell_out,
                                     int32_t *count_out, int32_t *perm_out) {
   if (!c) return MPCD_EINVAL;
