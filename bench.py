@@ -146,7 +146,7 @@ def measured_traffic(fam, form):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(L=(64, 64, 64), ppc=10, steps=3):
+def cpu_baseline(L=(64, 64, 64), ppc=10, steps=4):
     """the reference's own C step (oracle/_ref/ref_harness, built from the
     reference's sources) timed on one host core on a bounded sample of the
     same workload; falls back to the oracle port when the build is absent"""
