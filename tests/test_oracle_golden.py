@@ -74,7 +74,7 @@ def test_oracle_matches_reference(name):
 
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_bitexact_on_this_host(name):
-    """With the image's glibc (the fixtures' recording host and the GPU box share it) the match is exact."""
+    """With the glibc the fixtures were recorded against (the same image runs every test) the match is exact."""
     g = load(name)
     c, s = oracle_for(g)
     d = pyoracle.Draws(pyoracle.DRAW_SEQ, c["seed"])
