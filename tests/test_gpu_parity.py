@@ -154,3 +154,37 @@ def test_gpu_philox_vs_oracle(spec):
     _assert_close(Vg, Vo, "V", rtol=tol, atol=tol)
     if lc:
         _assert_close(Ug, Uor, "U", rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rtech", [20, 21, 17])
+def test_gpu_active_species_vs_oracle(rtech):
+    """two species, one active with a MINACTRATIO near its mean share (about
+    half of the cells count enough of it to be active), in a channel with homeotropic anchoring
+    at bounce-back phantom walls: the GPU's internal draws against the oracle's
+    (the same Philox variates)"""
+    import pyoracle
+    M = U.package()
+    L, ppc = (14, 10), 12
+    walls = CS.pbc_walls(0, L[0]) + CS.solid_walls(1, L[1], MUN=1.0, MUT=0.0)
+    c, Q, V, Uo = _random_box(2, L, ppc, rtech, 1, seed=17, walls=walls, grav=[5e-4, 0, 0])
+    c["species"] = [CS.species(mfpot=10.0, tumble=0.5, act=0.05, minactratio=0.65, sigwidth=0.8, sigpos=0.9),
+                    CS.species(mass=1.5, mfpot=6.0, tumble=0.8, act=0.02)]
+    spid = (np.arange(Q.shape[0]) % 3 == 0).astype(np.int32)  # one particle in three of species 1
+    gnd = float(ppc)
+    osys = pyoracle.System(CS.oracle_cfg(c, gnd, np.zeros(3)), Q, V, Uo, spid)
+    d = pyoracle.Draws(pyoracle.DRAW_PHILOX, 91, 0)
+    ctx = M.Context(U.gpu_cfg(c, gnd, np.zeros(3)), Q.shape[0], seed=91)
+    ctx.upload(0, Q, V, Uo, spid)
+    print(f"[gpu philox] active species rtech={rtech}", flush=True)
+    for _ in range(3):
+        osys.step(d)
+        ctx.step(None)
+    Qo, Vo, Uor = osys.state()
+    Qg, Vg, Ug = ctx.download(0)
+    o_ord, _ = osys.order()
+    g_ord, _ = ctx.order(0)
+    np.testing.assert_array_equal(g_ord, o_ord)
+    _assert_close(Qg, Qo, "Q", rtol=1e-11, atol=1e-11)
+    _assert_close(Vg, Vo, "V", rtol=1e-11, atol=1e-11)
+    _assert_close(Ug, Uor, "U", rtol=1e-10, atol=1e-10)
