@@ -790,7 +790,9 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
       if (tan) {
         k_lc<Real, Dual><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, T);
       }
-    if (!tan) k_lc<Real, double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, T);
+    // the orientation step computes in the state's precision
+    using FL = typename std::conditional<std::is_same<Real, float>::value, float, double>::type;
+    if (!tan) k_lc<Real, FL><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, T);
   }
   {
     ProfScope ps(c, F_COLLIDE);

@@ -109,6 +109,16 @@ struct TA<double> {
   static MPCD_HD double par(double x, double) { return x; }
   static MPCD_HD bool moves(double) { return false; }
 };
+// FP32 arithmetic on the FP32 state (the orientation step of k_lc): values only
+template <>
+struct TA<float> {
+  template <typename R>
+  static MPCD_HD float ld(const R *a, const double *, size_t i) { return (float)a[i]; }
+  template <typename R>
+  static MPCD_HD void st(R *a, double *, size_t i, float x) { a[i] = (R)x; }
+  static MPCD_HD float par(double x, double) { return (float)x; }
+  static MPCD_HD bool moves(float) { return false; }
+};
 template <>
 struct TA<Dual> {
   template <typename R>
@@ -964,7 +974,7 @@ __global__ void k_avorder_final(Ctx C, const BlkDev *blk, int nblk, const double
 // Maier-Saupe samplers (lc.c:52-411); `slot` keys the draws
 template <typename F>
 __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32_t slot, const F *DIR, F *rotAx,
-                          const F &rotAngle, F *U, double KBT, const F &S, double effM) {
+                          const F &rotAngle, F *U, const F &KBT, const F &S, const F &effM) {
   if (R.dim == 2) {
     F newU[3] = {U[0], U[1], 0.0};
     F ang0 = signed_angle(DIR, U, 2);
@@ -975,8 +985,8 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     for (int i = 0; i < annealNum; i++) {
       int accept = 0;
       F w0 = effM * S * uz0 * uz0 / KBT;
-      double ang1 = kPI * (0.5 - Dr.lc_u(rep, slot, 1 + 2 * i));
-      double uz1 = cos(ang1);
+      F ang1 = kPI * (0.5 - Dr.lc_u(rep, slot, 1 + 2 * i));
+      F uz1 = cos(ang1);
       F w1 = effM * S * uz1 * uz1 / KBT;
       F dw = w1 - w0;
       if (dw >= 0.) accept = 1;
@@ -998,9 +1008,9 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     F sd = 1. / sqrt(2. * effM * S / KBT);
     F theta = sd * Dr.lc_n(rep, slot, 0) + 0.0;
     F uz = 1. - 0.5 * theta * theta;
-    double pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
+    F pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
     newU[0] = pm * uz;
-    double phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
+    F phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
     newU[1] = sqrt(1.0 - uz * uz) * cos(phi);
     newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
     rodrigues(newU, rotAx, rotAngle);
@@ -1011,11 +1021,11 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     F Rr = Dr.lc_u(rep, slot, 0) * (1. + B2 / 3.);
     F c = sqrt(9. * B2 * B2 * B2 * B2 * Rr * Rr + 4. * B2 * B2 * B2);
     F num = pow(3. * B2 * B2 * Rr + c, 1. / 3.);
-    double denom = pow(2., 1. / 3.);
+    F denom = pow(2., 1. / 3.);
     F uz = num / (denom * B2) - denom / num;
-    double pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
+    F pm = (Dr.lc_u(rep, slot, 1) <= 0.5) ? -1.0 : 1.0;
     newU[0] = pm * uz;
-    double phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
+    F phi = Dr.lc_u(rep, slot, 2) * 2.0 * kPI;
     newU[1] = sqrt(1.0 - uz * uz) * cos(phi);
     newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
     rodrigues(newU, rotAx, rotAngle);
@@ -1027,15 +1037,15 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     for (int i = 0; i < annealNum; i++) {
       int accept = 0;
       F w0 = effM * S * uz0 * uz0 / KBT;
-      double uz1 = Dr.lc_u(rep, slot, 1 + 2 * i);
+      F uz1 = Dr.lc_u(rep, slot, 1 + 2 * i);
       F w1 = effM * S * uz1 * uz1 / KBT;
       F dw = w1 - w0;
       if (dw >= 0.) accept = 1;
       else if (Dr.lc_u(rep, slot, 2 + 2 * i) <= exp(dw)) accept = 1;
       if (accept) { cnt += 1; uz0 = uz1; }
     }
-    double phi = Dr.lc_u(rep, slot, 1 + 2 * annealNum) * 2.0 * kPI;
-    double pm = (Dr.lc_u(rep, slot, 2 + 2 * annealNum) <= 0.5) ? -1.0 : 1.0;
+    F phi = Dr.lc_u(rep, slot, 1 + 2 * annealNum) * 2.0 * kPI;
+    F pm = (Dr.lc_u(rep, slot, 2 + 2 * annealNum) <= 0.5) ? -1.0 : 1.0;
     newU[0] = pm * uz0;
     newU[1] = sqrt(1.0 - uz0 * uz0) * cos(phi);
     newU[2] = sqrt(1.0 - uz0 * uz0) * sin(phi);
@@ -1076,7 +1086,7 @@ __global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDe
   {
     F U[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)}, Tq[3], dU[3] = {0., 0., 0.};
     for (int i = 0; i < dim; i++) dU[i] = U[i];
-    ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, R.kbt, Sv, effM);
+    ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, F(R.kbt), Sv, F(effM));
     for (int i = 0; i < dim; i++) dU[i] = (U[i] - dU[i]) / R.dt;
     cross3(U, dU, Tq);
     for (int i = 0; i < 3; i++) Tq[i] *= sp.rfc;

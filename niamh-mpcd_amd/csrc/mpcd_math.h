@@ -26,6 +26,7 @@ MPCD_HD bool fneq(double x, double y) { return fabs(x - y) >= kTOL; }  // mtools
 // The geometric helpers are templates on the arithmetic type F: double for
 // the step, mpcd_ad::Dual (mpcd_dual.h) for its tangent.
 MPCD_HD double val(double x) { return x; }
+MPCD_HD float val(float x) { return x; }
 
 template <typename F>
 MPCD_HD F dotn(const F *x, const F *y, int n) {                        // mtools.c:111
