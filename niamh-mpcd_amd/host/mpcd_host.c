@@ -103,6 +103,8 @@ static void read_wall(const json *o, int nspec, mpcd_gpu_wall *w, int index) {
     if (json_nums(o, "V", v3, 3) == 3) memcpy(w->V, v3, sizeof v3);
     if (json_nums(o, "L", v3, 3) == 3) memcpy(w->L, v3, sizeof v3);
     if (json_nums(o, "DVxyz", v3, 3) == 3) memcpy(w->DVxyz, v3, sizeof v3);
+    /* orientation factors per axis default to 1 (read.c:1344-1348) */
+    w->MUxyz[0] = w->MUxyz[1] = w->MUxyz[2] = 1.0;
     if (json_nums(o, "MUxyz", v3, 3) == 3) memcpy(w->MUxyz, v3, sizeof v3);
     if (json_nums(o, "DUxyz", v3, 3) == 3) memcpy(w->DUxyz, v3, sizeof v3);
     w->R = json_num(o, "R", 2.0);
@@ -142,6 +144,7 @@ static int domain_walls(int dim, const int L[3], int mode, int nspec, mpcd_gpu_w
             w->Q[ax] = side ? (double)L[ax] : 0.0;
             w->R = 0.0;
             w->MUN = w->MUT = 1.0;
+            w->MUxyz[0] = w->MUxyz[1] = w->MUxyz[2] = 1.0; /* read.c:1558-1559 */
             w->COLL_TYPE = 1;
             w->inter_mask = (1 << nspec) - 1;
             if (mode == 1) {
@@ -278,10 +281,11 @@ int main(int argc, char **argv) {
         for (int w = 0; w < p.nwall; w++) {
             const mpcd_gpu_wall *W = &p.wall[w];
             printf("%s{\"A\": [%.17g, %.17g, %.17g], \"Q\": [%.17g, %.17g, %.17g], \"R\": %.17g, \"DN\": %.17g, "
-                   "\"MVN\": %.17g, \"MVT\": %.17g, \"MUN\": %.17g, \"MUT\": %.17g, \"V\": [%.17g, %.17g, %.17g], "
+                   "\"MVN\": %.17g, \"MVT\": %.17g, \"MUN\": %.17g, \"MUT\": %.17g, \"MUxyz\": [%.17g, %.17g, %.17g], "
+                   "\"V\": [%.17g, %.17g, %.17g], "
                    "\"INV\": %d, \"PHANTOM\": %d, \"inter_mask\": %d}",
                    w ? ", " : "", W->A[0], W->A[1], W->A[2], W->Q[0], W->Q[1], W->Q[2], W->R, W->DN, W->MVN, W->MVT,
-                   W->MUN, W->MUT, W->V[0], W->V[1], W->V[2], W->INV, W->PHANTOM, W->inter_mask);
+                   W->MUN, W->MUT, W->MUxyz[0], W->MUxyz[1], W->MUxyz[2], W->V[0], W->V[1], W->V[2], W->INV, W->PHANTOM, W->inter_mask);
         }
         printf("]}\n");
         json_free(J);

@@ -188,3 +188,34 @@ def test_gpu_active_species_vs_oracle(rtech):
     _assert_close(Qg, Qo, "Q", rtol=1e-11, atol=1e-11)
     _assert_close(Vg, Vo, "V", rtol=1e-11, atol=1e-11)
     _assert_close(Ug, Uor, "U", rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rtech,mfpot", [(3, 12.0), (21, 0.6), (17, 3.0)])
+def test_gpu_nematic_3d_channel_vs_oracle(rtech, mfpot):
+    """3D nematic channel: periodic x and y, bounce-back phantom walls in z
+    with planar strong anchoring (MUN 0, MUT 1), gravity; the Gaussian,
+    cubic-root and Metropolis Maier-Saupe samplers by the mean-field strength"""
+    import pyoracle
+    M = U.package()
+    L, ppc = (6, 5, 6), 10
+    walls = CS.pbc_walls(0, L[0]) + CS.pbc_walls(1, L[1]) + CS.solid_walls(2, L[2], MUN=0.0, MUT=1.0)
+    c, Q, V, Uo = _random_box(3, L, ppc, rtech, 1, seed=29, walls=walls, grav=[1e-3, 0, 0],
+                              sp=dict(mfpot=mfpot, tumble=0.7, act=0.04))
+    gnd = float(ppc)
+    osys = pyoracle.System(CS.oracle_cfg(c, gnd, np.zeros(3)), Q, V, Uo, None)
+    d = pyoracle.Draws(pyoracle.DRAW_PHILOX, 5, 0)
+    ctx = M.Context(U.gpu_cfg(c, gnd, np.zeros(3)), Q.shape[0], seed=5)
+    ctx.upload(0, Q, V, Uo, None)
+    print(f"[gpu philox] 3D nematic channel rtech={rtech} U={mfpot}", flush=True)
+    for _ in range(3):
+        osys.step(d)
+        ctx.step(None)
+    Qo, Vo, Uor = osys.state()
+    Qg, Vg, Ug = ctx.download(0)
+    o_ord, _ = osys.order()
+    g_ord, _ = ctx.order(0)
+    np.testing.assert_array_equal(g_ord, o_ord)
+    _assert_close(Qg, Qo, "Q", rtol=1e-11, atol=1e-11)
+    _assert_close(Vg, Vo, "V", rtol=1e-11, atol=1e-11)
+    _assert_close(Ug, Uor, "U", rtol=1e-10, atol=1e-10)

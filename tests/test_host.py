@@ -68,6 +68,7 @@ def test_density_solid_domain_walls_and_collop(tmp_path):
     assert len(d["walls"]) == 4
     for w in d["walls"]:
         assert w["DN"] == 0.0 and w["MVN"] == -1.0 and w["MVT"] == -1.0 and w["PHANTOM"] == 1
+        assert w["MUxyz"] == [1.0, 1.0, 1.0]  # read.c:1558-1559
 
 
 def test_bc_entries(tmp_path):
@@ -81,6 +82,10 @@ def test_bc_entries(tmp_path):
     # absent keys take the reference's defaults: R 2, DN 1, MVN = MVT = 1
     assert w1["A"] == [0.0, -1.0, 0.0] and w1["R"] == 2.0 and w1["DN"] == 1.0 and w1["MVN"] == 1.0
     assert w1["MUN"] == 0.0 and w1["MUT"] == 1.0 and w1["INV"] == 1 and w1["inter_mask"] == 3
+    # an absent MUxyz is 1 on every axis (read.c:1344-1348), a given one is taken as it is
+    assert w0["MUxyz"] == [1.0, 1.0, 1.0] and w1["MUxyz"] == [1.0, 1.0, 1.0]
+    bc[1]["MUxyz"] = [0.5, -1.0, 1.0]
+    assert dump(tmp_path, dict(domain=[9, 9], BC=bc))["walls"][1]["MUxyz"] == [0.5, -1.0, 1.0]
 
 
 def test_rtech_alone(tmp_path):
@@ -160,3 +165,17 @@ def test_host_runs_a_channel(tmp_path):
     for step, kbt, ke, px, py, pz in rows:
         assert 0.9 < kbt < 1.1
         assert pz == 0.0
+
+
+@pytest.mark.gpu
+def test_host_runs_an_active_nematic(tmp_path):
+    """the host on a 2D active nematic input (lc 1, collOp 16, random
+    orientations): the run completes and the temperature stays near kT"""
+    cfg = {"domain": [24, 24], "simSteps": 99, "lc": 1, "collOp": 16, "seed": 8,
+           "species": [{"dens": 20, "vDist": 4, "mfpot": 10, "tumble": 0.5, "rfc": 0.01, "act": 0.05}]}
+    print("[host] 2D active nematic 24x24x20, 100 steps", flush=True)
+    info, rows = run_host(tmp_path, cfg, "--every", "25")
+    assert info["steps"] == 100 and info["particles_per_replica"] == 24 * 24 * 20
+    assert [int(x[0]) for x in rows] == [24, 49, 74, 99]
+    for step, kbt, ke, px, py, pz in rows:
+        assert 0.8 < kbt < 1.2
