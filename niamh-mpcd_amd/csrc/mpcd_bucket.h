@@ -195,22 +195,37 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   __shared__ int32_t s_nwl, s_wlbase;  // wall-list entries of the workgroup, their first index
   __shared__ double s_ke[NT / 64];
 
-  // the first loads are independent: issue them together.  The slot base
-  // of the bucket is the sum of the counts of all earlier buckets: the
-  // block sums of the preparation kernel plus the rest of this block
+  // The first loads are independent, and all are issued before any is
+  // used, so a tile waits for one memory latency rather than one per load:
+  //  - every record of the thread.  These loads are unconditional: slot
+  //    tid + j * NT < CAP lies in the bucket's own allocation, and what a
+  //    slot at or past the bucket's count holds is never used (rk = -1);
+  //  - the bucket's count, and the terms of its slot base: the counts of the
+  //    earlier buckets of its block of kBlock (one a thread) plus the block
+  //    sums of the preparation kernel;
+  //  - the poison flag, read once for the whole workgroup.
+  static_assert(NT == kBlock, "one count of the bucket's block a thread");
+  const size_t off = (size_t)blockIdx.x * CAP;
+  Real rq[PPT][3], rv[PPT][3];
+  int rid[PPT], rsp[PPT], rk[PPT], ra[PPT];
+  uint64_t rkey[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; j++) rec_load(I, off + tid + j * NT, ONESP, rq[j], rv[j], rid[j], rsp[j], rkey[j]);
   const int n_raw = I.cnt[blockIdx.x];
   __shared__ int s_base[NT / 64];
   {
-    const int bb = blockIdx.x / kBlock;
+    const int bb = blockIdx.x / kBlock, k1 = bb * kBlock + tid;
+    const int c1 = I.cnt[min(k1, (int)blockIdx.x)];
+    int flag = 0;
+    if (tid == 0) flag = C.flags[0];
     int part = 0;
-    for (int k = bb * kBlock + tid; k < (int)blockIdx.x; k += NT) part += I.cnt[k];
     for (int k = tid; k < bb; k += NT) part += bsum[k];
+    part += (k1 < (int)blockIdx.x) ? c1 : 0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
     if ((tid & 63) == 0) s_base[tid >> 6] = part;
+    if (tid == 0) s_flag = flag;
   }
-  // one read of the poison flag for the whole workgroup
-  if (tid == 0) s_flag = C.flags[0];
   const TileDev T = bkts[blockIdx.x];
   const int ncl = T.c1 - T.c0;
   for (int k = tid; k <= ncl; k += NT) s_cnt[k] = 0;
@@ -228,7 +243,6 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   const RepDev &R = reps[T.rep];
   const BucketRep BR = brep[T.rep];
   const int n = min(n_raw, CAP);
-  const size_t off = (size_t)blockIdx.x * CAP;
   // the replica values of the hot path, loaded once
   const double Ld[3] = {(double)R.L[0], (double)R.L[1], (double)R.L[2]};
   const int Li[3] = {R.L[0], R.L[1], R.L[2]}, pbc[3] = {R.pbc[0], R.pbc[1], R.pbc[2]};
@@ -241,27 +255,17 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   const double dt = R.dt;
 
   // 1. the bucket: grid shift, escape repair, body force, cell in the tile
-  Real rq[PPT][3], rv[PPT][3];
-  int rid[PPT], rsp[PPT], rk[PPT], ra[PPT];
-  uint64_t rkey[PPT];
 #pragma unroll
   for (int j = 0; j < PPT; j++) {
     const int i = tid + j * NT;
     rk[j] = -1;
     ra[j] = 0;
-    rid[j] = 0;
-    rsp[j] = 0;
-    rkey[j] = 0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) rq[j][d] = rv[j][d] = (Real)0;
     if (i < n) {
-      Real q_[3], v_[3];
-      int pid, spid;
-      uint64_t key;
-      rec_load(I, off + i, ONESP, q_, v_, pid, spid, key);
-      if (ONESP) spid = 0;
-      double q[3] = {(double)q_[0], (double)q_[1], (double)q_[2]};
-      double v[3] = {(double)v_[0], (double)v_[1], (double)v_[2]};
+      const int pid = rid[j];
+      const int spid = ONESP ? 0 : rsp[j];
+      const uint64_t key = rkey[j];
+      double q[3] = {(double)rq[j][0], (double)rq[j][1], (double)rq[j][2]};
+      double v[3] = {(double)rv[j][0], (double)rv[j][1], (double)rv[j][2]};
       if (R.galinv) shift_fwd_bf<DIM>(Ld, shc, q);
       int cn = cell_bf<DIM>(Li, pbc, cell_off, ny, nz, q, true);
       if (cn < 0) {
@@ -286,9 +290,7 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
         rq[j][d] = (Real)q[d];
         rv[j][d] = vv[d];
       }
-      rid[j] = pid;
       rsp[j] = spid;
-      rkey[j] = key;
       const int k = cn - T.c0;
       if ((unsigned)k < (unsigned)ncl) {
         rk[j] = k;
