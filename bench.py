@@ -425,6 +425,15 @@ def main():
         r32 = timed_run(M, ctx, args.steps, args.warmup, B * n1)
         r32["graph"] = graph_run(ctx, args.steps, B * n1)
         r32["tile_fill"] = ctx.tile_fill()
+        # the same steps with the running flow field on (the reference's flow
+        # output, localFLOW + sumFLOW every step): what it adds to a step
+        kf = max(10, args.steps // 4)
+        ctx.flow(True)
+        rf = timed_run(M, ctx, kf, 2, B * n1)
+        ctx.flow(False)
+        r32["flow_field"] = dict(steps=kf, ms_per_step=rf["ms_per_step"],
+                                 flow_kernels_ms_per_step=rf["timing"]["flow"][0] / kf,
+                                 summed_steps=ctx.flow_read(0)[1])
         result["fp32"] = (r32, B * n1, B)
         ctx.close()
     if args.only in ("fp64", "both"):
@@ -512,6 +521,7 @@ def main():
         "adjoint": adjoint,
         "tangent_active_nematic": tangent,
         "lc_active_nematic": lc,
+        "flow_field": r.get("flow_field"),
     }
     if "fp64" in result and head_key != "fp64":
         r64, n64, _ = result["fp64"]

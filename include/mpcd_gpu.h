@@ -250,17 +250,35 @@ mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *ctx, int replica, int32_t *pop, dou
 /* list order of replica `replica` after the last phase: order [n] particle
  * indices, cstart [ncell+1] cell offsets */
 mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *ctx, int replica, int32_t *order, int32_t *cstart);
+/* Running flow field of the reference's flow output.  While on, every
+ * completed step adds to each cell of the unshifted binning its
+ * centre-of-mass velocity, sum m v / sum m (0 in an empty cell), as
+ * localFLOW + sumFLOW do at the end of timestep() when flowOut is set
+ * (mpc.c:4946-4951, 226-246, 3889-3910).  Replaces those two calls; the
+ * caller writes the output, as flowout does (pout.c:1681-1700).  The sums
+ * persist while off.  Cost: one pass over the particles with FP64 atomics
+ * per step; the atomics add in no fixed order, so a sum equals the
+ * reference's list-order sum to round-off.  Not rewound by
+ * mpcd_gpu_restore. */
+mpcd_status mpcd_gpu_flow(mpcd_gpu_ctx *ctx, int on);
+/* the sums of replica `replica` (flow: ncell x 3 doubles, cells in the order
+ * of mpcd_gpu_get_cells, components >= dim zero) and the number of steps
+ * summed; reset != 0 then clears both, as flowout clears FLOW after writing
+ * it (pout.c:1696).  Either pointer may be NULL.  Before the first
+ * mpcd_gpu_flow(ctx, 1) the sums and the count read as zero. */
+mpcd_status mpcd_gpu_flow_read(mpcd_gpu_ctx *ctx, int replica, double *flow, int64_t *nsteps, int reset);
 /* number of cells of replica `replica` and the keyed LC stride */
 int         mpcd_gpu_ncells(const mpcd_gpu_ctx *ctx, int replica);
 int         mpcd_gpu_lc_stride(const mpcd_gpu_ctx *ctx);
 /* kernel timing with HIP events recorded on the library's stream around
  * every kernel family while enabled (enable != 0 also clears the records) */
 mpcd_status mpcd_gpu_profile(mpcd_gpu_ctx *ctx, int enable);
-/* out[f] = total device ms of family f, out[13 + f] = number of timed
+/* out[f] = total device ms of family f, out[14 + f] = number of timed
  * launches, for the families 0 count (histogram + keys), 1 scan, 2 scatter,
  * 3 in-cell sort, 4 gather, 5 cell properties, 6 ghost particles, 7 LC,
  * 8 collision, 9 temperature/thermostat, 10 stream + walls, 11 fused
- * gather + collision + stream + next binning, 12 bucketed step; n >= 26 */
+ * gather + collision + stream + next binning, 12 bucketed step, 13 flow
+ * field; n >= 28 */
 mpcd_status mpcd_gpu_timing(mpcd_gpu_ctx *ctx, double *out, int n);
 
 #ifdef __cplusplus

@@ -117,7 +117,7 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_tile_fill", "mpcd_gpu_checkpoint", "mpcd_gpu_restore", "mpcd_gpu_record",
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
            "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
-           "mpcd_gpu_download_aos"]
+           "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read"]
 
 _lib = None
 
@@ -164,6 +164,8 @@ def lib():
                                       C.c_int64]
     L.mpcd_gpu_download_aos.argtypes = [vp, C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64]
     L.mpcd_gpu_tangent_download.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.mpcd_gpu_flow.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_flow_read.argtypes = [vp, C.c_int, dp, i64p, C.c_int]
     _lib = L
     return L
 
@@ -400,8 +402,22 @@ class Context:
         self._check(self.L.mpcd_gpu_get_cells(self.h, r, _ip(pop), _dp(vcm), _dp(S), _dp(d)), "get_cells")
         return dict(pop=pop, vcm=vcm, S=S, dir=d)
 
+    def flow(self, on=True):
+        """start (or stop) the running flow field: every later step adds each
+        unshifted cell's centre-of-mass velocity (localFLOW + sumFLOW,
+        mpc.c:226-246)"""
+        self._check(self.L.mpcd_gpu_flow(self.h, int(bool(on))), "flow")
+
+    def flow_read(self, r=0, reset=False):
+        """(sums [ncell, 3], steps summed) of replica r; reset clears both
+        afterwards, as flowout does (pout.c:1696)"""
+        out, n = np.zeros((self.ncells[r], 3)), np.zeros(1, np.int64)
+        self._check(self.L.mpcd_gpu_flow_read(self.h, r, _dp(out), n.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              int(bool(reset))), "flow_read")
+        return out, int(n[0])
+
     FAMILIES = ["count", "scan", "scatter", "sort", "gather", "props", "ghost", "lc", "collide", "temp", "stream",
-                "fused", "bucket"]
+                "fused", "bucket", "flow"]
 
     def profile(self, enable=True):
         self._check(self.L.mpcd_gpu_profile(self.h, int(bool(enable))), "profile")

@@ -17,6 +17,7 @@
 
 #include "mpcd_adjoint.h"
 #include "mpcd_bucket.h"
+#include "mpcd_flow.h"
 #include "mpcd_fused.h"
 #include "mpcd_kernels.h"
 
@@ -119,6 +120,12 @@ struct mpcd_gpu_ctx {
   WallHot *wh2 = nullptr;                                         // per step parity
   double *box2 = nullptr;                                         // per step parity: wall-free boxes
   double *d_celldraw = nullptr;  // SRD cell draws of the current step (fused path, internal generator)
+  // running flow field (mpcd_gpu_flow): sums per cell [3 * nctot], the
+  // step's accumulator [4 * nctot], steps summed per replica [nrep]
+  bool flow_on = false;
+  double *d_flow = nullptr, *d_flow_acc = nullptr;
+  double *d_flow_part = nullptr;  // bucketed runs: partials per offset class and cell [kFlowCls * 4 * nctot]
+  int64_t *d_flow_n = nullptr;
   int32_t *hist2[2] = {nullptr, nullptr};
   State<double> sd[2]{};
   State<float> sf[2]{};
@@ -148,7 +155,7 @@ constexpr double kFusedMaxPPC = 40.0;
 
 // kernel families reported by mpcd_gpu_timing
 enum { F_COUNT = 0, F_SCAN, F_SCATTER, F_SORT, F_GATHER, F_PROPS, F_GHOST, F_LC, F_COLLIDE, F_TEMP, F_STREAM, F_FUSED,
-       F_BUCKET, F_N };
+       F_BUCKET, F_FLOW, F_N };
 
 static hipEvent_t prof_event(mpcd_gpu_ctx *c) {
   if (c->ev_used == c->ev_pool.size()) {
@@ -818,6 +825,30 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
   if (c->any_therm) k_scale<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk);
 }
 
+// the running flow field after a completed step (mpc.c:4946-4951): the
+// sums of the unshifted cells, then FLOW += their quotient
+static void flow_fold(mpcd_gpu_ctx *c) {
+  const int nb = nblocks(std::max(c->nctot, c->nrep), kBlock);
+  if (c->dim == 3)
+    k_flow_fold<3><<<nb, kBlock, 0, c->stream>>>(c->C.flags, c->nctot, c->nrep, c->d_flow_acc, c->d_flow, c->d_flow_n);
+  else
+    k_flow_fold<2><<<nb, kBlock, 0, c->stream>>>(c->C.flags, c->nctot, c->nrep, c->d_flow_acc, c->d_flow, c->d_flow_n);
+}
+
+template <typename Real>
+static void flow_after_dense(mpcd_gpu_ctx *c) {
+  if (!c->flow_on) return;
+  ProfScope ps(c, F_FLOW);
+  State<Real> *S = states<Real>(c);
+  if (c->dim == 3)
+    k_flow_dense<Real, 3><<<c->pblk.size(), kBlock, 0, c->stream>>>(c->C, S[c->cur], c->d_pblk, c->cu_src,
+                                                                   c->d_flow_acc);
+  else
+    k_flow_dense<Real, 2><<<c->pblk.size(), kBlock, 0, c->stream>>>(c->C, S[c->cur], c->d_pblk, c->cu_src,
+                                                                   c->d_flow_acc);
+  flow_fold(c);
+}
+
 template <typename Real>
 static void run_stream(mpcd_gpu_ctx *c, int injected) {
   State<Real> *S = states<Real>(c);
@@ -839,6 +870,7 @@ static void run_stream(mpcd_gpu_ctx *c, int injected) {
   c->pending = true;
   c->cu_src = 0;
   c->hist_ready = false;
+  flow_after_dense<Real>(c);
   c->step++;
 }
 
@@ -936,6 +968,7 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
   c->pending = true;
   c->cu_src = do_next ? 1 : 0;
   c->hist_ready = do_next != 0;
+  flow_after_dense<Real>(c);
   c->step++;
 }
 
@@ -1467,6 +1500,28 @@ static void run_step_bucket(mpcd_gpu_ctx *c) {
           c->nwtot);
   }
   c->bcur ^= 1;
+  if (c->flow_on) {
+    // the running flow field from the records just written: partials per
+    // tile cell and offset class, then the fold per cell
+    ProfScope ps(c, F_FLOW);
+    BState<Real> *B = bstates<Real>(c);
+    const int cap = bkt_cap(fused_op(c->rtech)), one = c->nspec == 1 ? 1 : 0;
+    const double *shn = c->shift2 + ((c->step & 1) ^ 1) * 3 * c->nrep;
+    const int ncb = (int)c->cblk.size();
+    if (c->dim == 3) {
+      k_flow_tiles<Real, 3><<<c->nbkt, kBlock, 0, c->stream>>>(c->C.flags, B[c->bcur], c->d_bkts, c->d_rep, c->d_sp,
+                                                                shn, cap, one, c->nctot, c->d_flow_part,
+                                                                c->d_flow_acc);
+      k_flow_fold_tiles<3><<<ncb, kBlock, 0, c->stream>>>(c->C.flags, c->d_rep, c->d_cblk, c->nrep, c->nctot,
+                                                           c->d_flow_part, c->d_flow_acc, c->d_flow, c->d_flow_n);
+    } else {
+      k_flow_tiles<Real, 2><<<c->nbkt, kBlock, 0, c->stream>>>(c->C.flags, B[c->bcur], c->d_bkts, c->d_rep, c->d_sp,
+                                                                shn, cap, one, c->nctot, c->d_flow_part,
+                                                                c->d_flow_acc);
+      k_flow_fold_tiles<2><<<ncb, kBlock, 0, c->stream>>>(c->C.flags, c->d_rep, c->d_cblk, c->nrep, c->nctot,
+                                                           c->d_flow_part, c->d_flow_acc, c->d_flow, c->d_flow_n);
+    }
+  }
   c->step++;
 }
 
@@ -2056,6 +2111,48 @@ extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, 
   if (vcm) HIPCHK(c, hipMemcpyAsync(vcm, c->C.cell.vcm + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   if (S) HIPCHK(c, hipMemcpyAsync(S, c->C.cell.S + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   if (dir) HIPCHK(c, hipMemcpyAsync(dir, c->C.cell.dir + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_flow(mpcd_gpu_ctx *c, int on) {
+  if (!c) return MPCD_EINVAL;
+  if ((on != 0) == c->flow_on) return MPCD_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (on && !c->d_flow) {
+    c->d_flow = dalloc<double>(c, 3 * (size_t)c->nctot);
+    c->d_flow_acc = dalloc<double>(c, 4 * (size_t)c->nctot);
+    c->d_flow_n = dalloc<int64_t>(c, c->nrep);
+    c->d_flow_part = dalloc<double>(c, kFlowCls * 4 * (size_t)c->nctot);
+    if (!c->d_flow || !c->d_flow_acc || !c->d_flow_n || !c->d_flow_part) return fail(c, MPCD_ENOMEM, "flow field");
+  }
+  c->flow_on = on != 0;
+  // recorded graphs hold the step's launches, with or without the flow pair
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  drop_bucket_graph(c);
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_flow_read(mpcd_gpu_ctx *c, int r, double *flow, int64_t *nsteps, int reset) {
+  if (!c || r < 0 || r >= c->nrep) return MPCD_EINVAL;
+  const RepDev &R = c->reps[r];
+  const size_t off = R.cell_off, n = R.ncell;
+  if (!c->d_flow) {
+    // never switched on: nothing summed
+    if (flow) memset(flow, 0, sizeof(double) * 3 * n);
+    if (nsteps) *nsteps = 0;
+    return MPCD_OK;
+  }
+  if (flow)
+    HIPCHK(c, hipMemcpyAsync(flow, c->d_flow + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  if (nsteps) HIPCHK(c, hipMemcpyAsync(nsteps, c->d_flow_n + r, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  if (reset) {
+    HIPCHK(c, hipMemsetAsync(c->d_flow + 3 * off, 0, sizeof(double) * 3 * n, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_flow_n + r, 0, sizeof(int64_t), c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c);
 }

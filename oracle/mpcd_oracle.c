@@ -60,6 +60,10 @@ struct orc_sys {
     long step;
     long n_escape, n_bcfail, n_rewind, n_retherm;
     int lc_stride;
+    /* flow output (outFlags.FLOWOUT): FLOW per cell and the steps summed */
+    int flow_on;
+    double *flow;
+    long flow_n;
     /* scratch */
     int *tmp_order, *newcell, *tmp_cnt, *tmp_seen;
     int *slot_of; /* particle -> slot in the collision order */
@@ -517,6 +521,7 @@ orc_sys *orc_create(const orc_params *p, int n)
     s->spcnt = (int *)calloc((size_t)s->nc * ORC_MAXSPEC, sizeof(int));
     s->mass = (double *)calloc((size_t)s->nc, sizeof(double));
     s->vcm = (double *)calloc((size_t)s->nc * 3, sizeof(double));
+    s->flow = (double *)calloc((size_t)s->nc * 3, sizeof(double));
     s->cm = (double *)calloc((size_t)s->nc * 3, sizeof(double));
     s->I = (double *)calloc((size_t)s->nc * 9, sizeof(double));
     s->E = (double *)calloc((size_t)s->nc * 9, sizeof(double));
@@ -537,7 +542,7 @@ void orc_destroy(orc_sys *s)
     free(s->Q); free(s->V); free(s->U); free(s->T); free(s->spid);
     free(s->order); free(s->tmp_order); free(s->newcell); free(s->cellof); free(s->slot_of);
     free(s->cstart); free(s->tmp_cnt); free(s->tmp_seen);
-    free(s->pop); free(s->spcnt); free(s->mass); free(s->vcm); free(s->cm); free(s->I); free(s->E);
+    free(s->pop); free(s->spcnt); free(s->mass); free(s->vcm); free(s->flow); free(s->cm); free(s->I); free(s->E);
     free(s->S); free(s->dir);
     free(s);
 }
@@ -1604,6 +1609,28 @@ int orc_phase_collide(orc_sys *s, orc_draws *d)
     return 0;
 }
 
+/* localFLOW + sumFLOW (mpc.c:226-246), as timestep() ends with them when
+ * the flow output is on (mpc.c:4946-4951): each cell's centre-of-mass
+ * velocity by localMPCVCM (mpc.c:3889-3910, list order, divided by 1 in an
+ * empty cell) is added to FLOW, components < DIM */
+static void sum_flow(orc_sys *s)
+{
+    const orc_params *p = &s->p;
+    int c, t, d;
+    for (c = 0; c < s->nc; c++) {
+        double vcm[3] = {0., 0., 0.}, summ = 0.0;
+        for (t = s->cstart[c]; t < s->cstart[c + 1]; t++) {
+            int pi = s->order[t];
+            double mass = p->sp[s->spid[pi]].mass;
+            summ += mass;
+            for (d = 0; d < p->dim; d++) vcm[d] += s->V[pi * 3 + d] * mass;
+        }
+        for (d = 0; d < p->dim; d++) vcm[d] /= (summ ? summ : 1.);
+        for (d = 0; d < p->dim; d++) s->flow[c * 3 + d] += vcm[d];
+    }
+    s->flow_n++;
+}
+
 int orc_phase_stream(orc_sys *s, orc_draws *d)
 {
     int i, j;
@@ -1612,8 +1639,21 @@ int orc_phase_stream(orc_sys *s, orc_draws *d)
     for (i = 0; i < s->N; i++) bc_collision(s, d, i);
     bin(s, d, 0);
     local_prop(s);
+    if (s->flow_on) sum_flow(s);
     s->step++;
     return 0;
+}
+
+void orc_flow(orc_sys *s, int on) { s->flow_on = on != 0; }
+
+void orc_get_flow(orc_sys *s, double *flow, long *nsteps, int reset)
+{
+    if (flow) memcpy(flow, s->flow, sizeof(double) * 3 * (size_t)s->nc);
+    if (nsteps) *nsteps = s->flow_n;
+    if (reset) {
+        memset(s->flow, 0, sizeof(double) * 3 * (size_t)s->nc);
+        s->flow_n = 0;
+    }
 }
 
 int orc_step(orc_sys *s, orc_draws *d)

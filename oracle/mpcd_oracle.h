@@ -104,6 +104,10 @@ int      orc_phase_bin(orc_sys *s, orc_draws *d);
 int      orc_phase_collide(orc_sys *s, orc_draws *d);
 int      orc_phase_stream(orc_sys *s, orc_draws *d);
 void     orc_last_shift(const orc_sys *s, double *shift);
+/* flow output: with on, every step ends with localFLOW + sumFLOW
+ * (mpc.c:226-246, 4946-4951); get returns FLOW [ncell*3] and the steps summed */
+void     orc_flow(orc_sys *s, int on);
+void     orc_get_flow(orc_sys *s, double *flow, long *nsteps, int reset);
 
 /* draw-source helpers for Python (ctypes) callers */
 orc_draws *orc_draws_new(int mode, uint64_t seed, uint32_t replica);

@@ -82,6 +82,8 @@ def lib():
         for f in ("orc_step", "orc_phase_bin", "orc_phase_collide", "orc_phase_stream"):
             getattr(L, f).argtypes = [vp, vp]
         L.orc_last_shift.argtypes = [vp, dp]
+        L.orc_flow.argtypes = [vp, C.c_int]
+        L.orc_get_flow.argtypes = [vp, dp, C.POINTER(C.c_long), C.c_int]
         L.orc_draws_new.restype = vp
         L.orc_draws_new.argtypes = [C.c_int, C.c_uint64, C.c_uint32]
         L.orc_draws_free.argtypes = [vp]
@@ -256,6 +258,16 @@ class System:
         s = np.zeros(3)
         self.L.orc_last_shift(self.h, _dp(s))
         return s
+
+    def flow(self, on=True):
+        """with on, every later step ends with localFLOW + sumFLOW (mpc.c:226-246)"""
+        self.L.orc_flow(self.h, int(bool(on)))
+
+    def get_flow(self, reset=False):
+        """(FLOW [ncell, 3], steps summed); reset clears both afterwards"""
+        f, n = np.zeros((self.L.orc_ncells(self.h), 3)), C.c_long(0)
+        self.L.orc_get_flow(self.h, _dp(f), C.byref(n), int(bool(reset)))
+        return f, int(n.value)
 
 
 def eigen(m: np.ndarray):

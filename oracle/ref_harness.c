@@ -13,7 +13,7 @@
  * the reference's timestep() (mpc.c:4496) once per step with every output
  * switched off.  Random draws come from ref_rand.c (sequential streams).
  *
- * Usage:  ref_harness CASE.txt STATE.bin OUT.bin [timing]
+ * Usage:  ref_harness CASE.txt STATE.bin OUT.bin [timing | flow]
  *
  * CASE.txt is a list of "key value..." lines (see tests/golden/make_golden.py).
  * STATE.bin holds N records of 9 doubles (Q, V, U) followed by N int32 SPIDs.
@@ -24,7 +24,10 @@
  *   N int32 particle indices in list order (cells x-major, lists head->tail),
  *   ncell*(1+3+3) doubles: S, DIR[3], VCM[3] per cell.
  * With a 4th argument "timing" nothing is dumped and the wall-clock time of
- * the step loop is printed (used as the CPU baseline).
+ * the step loop is printed (used as the CPU baseline).  With "flow" the
+ * reference's flow output is switched on (outFlags.FLOWOUT = OUT, so every
+ * timestep() ends with localFLOW + sumFLOW, mpc.c:4946-4951), and after the
+ * last step OUT.bin receives int32 tag -2 and ncell*3 doubles: FLOW per cell.
  */
 #define _POSIX_C_SOURCE 199309L
 #include <math.h>
@@ -158,10 +161,11 @@ int main(int argc, char **argv)
     outputFilesList ofl;
     specSwimmer SS;
     double AVNOW[3] = {0, 0, 0}, AVV[3], avDIR[3] = {0, 0, 0}, KBTNOW = 0.0, AVS = 0.0;
-    int timing;
+    int timing, flow;
 
     if (argc < 4) die("usage: ref_harness CASE STATE OUT [timing]");
     timing = (argc > 4 && !strcmp(argv[4], "timing"));
+    flow = (argc > 4 && !strcmp(argv[4], "flow"));
     read_case(argv[1], &hc);
 
     DBUG = -1; /* below DBGRUN: silence the reference's diagnostic printing */
@@ -272,6 +276,7 @@ int main(int argc, char **argv)
     in.FRICCO = 1.0;
     in.stepsMD = 20;
     memset(&of, 0, sizeof(of));
+    if (flow) of.FLOWOUT = OUT;
     memset(&ofl, 0, sizeof(ofl));
     memset(&SS, 0, sizeof(SS));
     for (j = 0; j < 3; j++) AVV[j] = hc.avv[j];
@@ -301,6 +306,12 @@ int main(int argc, char **argv)
             double sec = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
             printf("{\"steps\": %d, \"particles\": %d, \"seconds\": %.6f}\n", hc.steps, GPOP, sec);
         }
+    }
+    if (o && flow) {
+        int a, b, k, tag = -2;
+        fwrite(&tag, sizeof(int), 1, o);
+        for (a = 0; a < XYZ_P1[0]; a++) for (b = 0; b < XYZ_P1[1]; b++) for (k = 0; k < XYZ_P1[2]; k++)
+            fwrite(CL[a][b][k].FLOW, sizeof(double), 3, o);
     }
     if (o) fclose(o);
     return 0;

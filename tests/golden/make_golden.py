@@ -14,6 +14,10 @@ case of cases.py through it, and stores what the reference computed:
     draws     [steps, 3]     cumulative (uniform, gauss, unit-vector) draws consumed
     S, DIR, VCM              per-cell values after every step
 
+  tests/golden/flow.npz: FLOW_<case> [nc, 3], the reference's running flow
+    sum (localFLOW + sumFLOW, mpc.c:226-246) after the case's steps, from a
+    second run of the same case with flow output on (compact cell layout)
+
 The compact cell layout drops the reference's always-empty z = 1 slab in 2D.
 Run:  python tests/golden/make_golden.py   (needs /root/reference; the
 fixtures it writes are committed, the reference is not needed afterwards).
@@ -98,6 +102,9 @@ def parse(path, c, N):
     recs = []
     while off < len(b):
         tag = take("<i4", 1)[0]
+        if tag == -2:
+            recs.append(dict(tag=tag, FLOW=compact_cells(take("<f8", nref * 3).reshape(nref, 3).copy(), c)))
+            break
         kbt = take("<f8", 1)[0]
         cnt = take("<u8", 3).copy()
         st = take("<f8", N * 9).reshape(N, 9)
@@ -132,6 +139,18 @@ def record(c, outdir):
     return N
 
 
+def record_flow(c):
+    """FLOW after the case's steps, from a run with the flow output on"""
+    Q, V, U, spid, avv = C.initial_state(c)
+    with tempfile.TemporaryDirectory() as d:
+        cp, sp_ = write_case(c, Q, V, U, spid, avv, d)
+        out = os.path.join(d, "out.bin")
+        subprocess.run([pyoracle.REF_HARNESS, cp, sp_, out, "flow"], check=True)
+        _, _, recs = parse(out, c, Q.shape[0])
+    assert recs[-1]["tag"] == -2 and len(recs) == c["steps"] + 2
+    return recs[-1]["FLOW"]
+
+
 def main():
     pyoracle.build_ref()
     names = sys.argv[1:]
@@ -140,6 +159,9 @@ def main():
             continue
         N = record(c, HERE)
         print(f"{c['name']}: N={N} steps={c['steps']}")
+    if not names or "flow" in names:
+        np.savez_compressed(os.path.join(HERE, "flow.npz"), **{"FLOW_" + c["name"]: record_flow(c) for c in C.CASES})
+        print("flow: FLOW of every case")
 
 
 if __name__ == "__main__":

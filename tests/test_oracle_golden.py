@@ -10,7 +10,10 @@ its sequential draw mode on the same initial state, must reproduce:
   * the number of draws of each kind consumed (bit-exact),
   * positions, velocities and orientations (bit-exact; the test tolerates
     1e-13 relative only to survive a different libm on another host),
-  * KBTNOW and the per-cell S, director and centre-of-mass velocity.
+  * KBTNOW and the per-cell S, director and centre-of-mass velocity,
+  * with the flow output on, the running flow sum FLOW after the case's steps
+    (localFLOW + sumFLOW, mpc.c:226-246; tests/golden/flow.npz, recorded from
+    a second reference run with outFlags.FLOWOUT set).
 
 Parts of the oracle that no fixture exercises, and how they are pinned
 instead (tests/test_oracle_invariants.py):
@@ -83,3 +86,23 @@ def test_oracle_bitexact_on_this_host(name):
     Q, V, U = s.state()
     diff = max(np.abs(Q - g["Q"][-1]).max(), np.abs(V - g["V"][-1]).max(), np.abs(U - g["U"][-1]).max())
     assert diff == 0.0, f"{name}: oracle differs from the reference by {diff:.3e} (expected bit-exact)"
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_flow_matches_reference(name):
+    g = load(name)
+    c, s = oracle_for(g)
+    ref = np.load(os.path.join(GOLD, "flow.npz"), allow_pickle=False)["FLOW_" + name]
+    s.flow(True)
+    d = pyoracle.Draws(pyoracle.DRAW_SEQ, c["seed"])
+    for k in range(c["steps"]):
+        s.step(d)
+    f, n = s.get_flow()
+    assert n == c["steps"]
+    np.testing.assert_allclose(f, ref, rtol=1e-13, atol=1e-13, err_msg=f"{name} FLOW")
+    assert np.all(f[:, c["dim"]:] == 0.0)
+    # a read with reset clears the sums and the count, as flowout clears FLOW (pout.c:1696)
+    f2, n2 = s.get_flow(reset=True)
+    assert n2 == n and np.array_equal(f2, f)
+    f3, n3 = s.get_flow()
+    assert n3 == 0 and not f3.any()
