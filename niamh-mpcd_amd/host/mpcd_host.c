@@ -21,6 +21,15 @@
  * the kinetic energy and the total momentum; a timing line on stdout.
  * Like the reference, each loop runs its step count plus one
  * (mpcd.c:264, 294).
+ *
+ * flowOut = F > 0 (read.c:1732): after the warm-up, every step adds each
+ * cell's centre-of-mass velocity to a running sum on the GPU (localFLOW +
+ * sumFLOW, mpc.c:4946-4951), and at every runtime with runtime % F == 0
+ * (pout.c:2690) DIR/flowfield.dat receives the sum divided by F for every
+ * cell of the domain, in flowout's line format (pout.c:1681-1700), after
+ * which the sum starts again.  The file opens with 13 header lines, as the
+ * reference's does (a preamble, a blank line and flowheader's column
+ * names, pout.c:355-357).  Replica r > 0 writes DIR/flowfield_r<r>.dat.
  */
 #include <math.h>
 #include <stdint.h>
@@ -61,6 +70,45 @@ static void check(mpcd_gpu_ctx *ctx, mpcd_status st, const char *what) {
 }
 
 /* keys this host rejects: they select parts of the reference outside the solvent path */
+/* flowout's lines (pout.c:1681-1700): time, cell indices and the mean
+ * centre-of-mass velocity over the interval of every cell of the domain;
+ * flow holds the compact cell array (x-major, (L+1) cells per axis, one z
+ * layer in 2D) */
+static void flow_write(FILE *f, const double *flow, const int L[3], int dim, double t, int interval) {
+    const int ny = L[1] + 1, nz = dim == 3 ? L[2] + 1 : 1, kz = dim == 3 ? L[2] : 1;
+    for (int i = 0; i < L[0]; i++)
+        for (int j = 0; j < L[1]; j++)
+            for (int k = 0; k < kz; k++) {
+                const double *F = flow + 3 * ((size_t)(i * ny + j) * nz + k);
+                double av[3] = {0.0, 0.0, 0.0};
+                for (int h = 0; h < dim; h++) av[h] = F[h] / (double)interval;
+                fprintf(f, "%12.5e\t%5d\t%5d\t%5d\t%12.5e\t%12.5e\t%12.5e\n", t, i, j, k, av[0], av[1], av[2]);
+            }
+}
+
+static FILE *flow_open(const char *outdir, int r, int nrep, int interval, const int L[3], int dim, int fp32) {
+    char path[4096];
+    if (r == 0) snprintf(path, sizeof path, "%s/flowfield.dat", outdir);
+    else snprintf(path, sizeof path, "%s/flowfield_r%d.dat", outdir, r);
+    FILE *f = fopen(path, "w");
+    if (!f) die("cannot write %s", path);
+    /* 11 preamble lines and a blank line, then the column names: readers of
+     * the reference's files skip 13 lines */
+    fprintf(f, "# flow field: per cell, the centre-of-mass velocity averaged over\n");
+    fprintf(f, "# the steps of each output interval (localFLOW + sumFLOW, flowout)\n");
+    fprintf(f, "# written by mpcd_host through the C-ABI of mpcd_gpu.h\n");
+    fprintf(f, "# replica %d of %d\n", r, nrep);
+    fprintf(f, "# domain %d x %d x %d, %dD\n", L[0], L[1], dim == 3 ? L[2] : 1, dim);
+    fprintf(f, "# interval %d steps (flowOut)\n", interval);
+    fprintf(f, "# particle state %s\n", fp32 ? "FP32, cell sums FP64" : "FP64");
+    fprintf(f, "# columns: time, cell indices QX QY QZ, mean VCM x y z\n");
+    fprintf(f, "# one block of cells per output, x-major\n");
+    fprintf(f, "# the first output (runtime 0) holds one step divided by the interval\n");
+    fprintf(f, "#\n\n");
+    fprintf(f, "   t\t   QX\t   QY\t   QZ\tVcmX\t\tVcmY\t\tVcmZ\n");
+    return f;
+}
+
 static void reject_unsupported(const json *J) {
     const char *strs[] = {"checkpointIn", "mdIn"};
     for (int k = 0; k < 2; k++) {
@@ -200,6 +248,7 @@ int main(int argc, char **argv) {
     p.dt = json_num(J, "dt", 0.1);
     int sim_steps = json_int(J, "simSteps", 2000);
     const int warm = json_int(J, "warmUp", 0);
+    const int flow_out = json_int(J, "flowOut", 0);
     if (steps_override >= 0) sim_steps = steps_override;
     p.galinv = json_int(J, "galInv", 1);
     p.tstech = json_int(J, "tsTech", 0);
@@ -266,9 +315,9 @@ int main(int argc, char **argv) {
         /* the parameters as read, without touching the GPU */
         printf("{\"dim\": %d, \"L\": [%d, %d, %d], \"dt\": %.17g, \"kbt\": %.17g, \"rotang\": %.17g, "
                "\"tau\": %.17g, \"grav\": [%.17g, %.17g, %.17g], \"rtech\": %d, \"lc\": %d, \"tstech\": %d, "
-               "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"seed\": %ld, \"species\": [",
+               "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"flowOut\": %d, \"seed\": %ld, \"species\": [",
                p.dim, p.L[0], p.L[1], p.L[2], p.dt, p.kbt, p.rotang, p.tau, p.grav[0], p.grav[1], p.grav[2], p.rtech,
-               p.lc, p.tstech, p.galinv, sim_steps, warm, seed);
+               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, seed);
         for (int s = 0; s < p.nspec; s++) {
             const mpcd_gpu_species *sp = &p.sp[s];
             printf("%s{\"pop\": %lld, \"mass\": %.17g, \"vdist\": %d, \"rfc\": %.17g, \"tumble\": %.17g, "
@@ -366,12 +415,28 @@ int main(int argc, char **argv) {
     const int total = sim_steps + 1;
     if (every <= 0) every = total > 10 ? total / 10 : 1;
     if (warm > 0) check(ctx, mpcd_gpu_run(ctx, warm + 1, use_graph), "warm-up");
+    /* the flow sums start after the warm-up (mpc.c:4947: !in.warmupSteps) */
+    FILE *ff[64] = {NULL};
+    double *flow = NULL;
+    if (flow_out > 0) {
+        if (nrep > 64) die("%s", "flowOut writes one file per replica: at most 64 replicas");
+        check(ctx, mpcd_gpu_flow(ctx, 1), "mpcd_gpu_flow");
+        flow = (double *)malloc(sizeof(double) * 3 * (size_t)mpcd_gpu_ncells(ctx, 0));
+        if (!flow) die("%s", "out of host memory");
+        for (int r = 0; r < nrep; r++) ff[r] = flow_open(outdir, r, nrep, flow_out, p.L, dim, fp32);
+    }
     check(ctx, mpcd_gpu_synchronize(ctx), "synchronize");
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     double busy = 0.0;
+    int next_out = every < total ? every : total;
     for (int done = 0; done < total;) {
-        const int k = (total - done) < every ? (total - done) : every;
+        int k = next_out - done;
+        if (flow_out > 0) {
+            /* the next runtime r >= done with r % flowOut == 0 ends after step r + 1 */
+            const int r = ((done + flow_out - 1) / flow_out) * flow_out;
+            if (r + 1 - done < k) k = r + 1 - done;
+        }
         struct timespec a, b;
         clock_gettime(CLOCK_MONOTONIC, &a);
         check(ctx, mpcd_gpu_run(ctx, k, use_graph), "mpcd_gpu_run");
@@ -379,6 +444,16 @@ int main(int argc, char **argv) {
         clock_gettime(CLOCK_MONOTONIC, &b);
         busy += (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
         done += k;
+        if (flow_out > 0 && (done - 1) % flow_out == 0) {
+            /* flowout at runtime done - 1 (pout.c:2690), then the sums restart */
+            for (int r = 0; r < nrep; r++) {
+                int64_t nsum = 0;
+                check(ctx, mpcd_gpu_flow_read(ctx, r, flow, &nsum, 1), "mpcd_gpu_flow_read");
+                flow_write(ff[r], flow, p.L, dim, (double)(done - 1) * p.dt, flow_out);
+            }
+        }
+        if (done < next_out) continue;
+        next_out = (next_out + every < total) ? next_out + every : total;
         check(ctx, mpcd_gpu_download(ctx, 0, Q, V, NULL), "mpcd_gpu_download");
         double ke = 0.0, P[3] = {0, 0, 0};
         for (size_t i = 0; i < n; i++) {
@@ -395,6 +470,9 @@ int main(int argc, char **argv) {
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
     fclose(fo);
+    for (int r = 0; r < nrep; r++)
+        if (ff[r]) fclose(ff[r]);
+    free(flow);
     printf("{\"steps\": %d, \"replicas\": %d, \"particles_per_replica\": %lld, \"precision\": \"%s\", "
            "\"wall_s\": %.3f, \"step_s\": %.3f, \"particle_steps_per_s\": %.4g, \"output\": \"%s\"}\n",
            total, nrep, (long long)ntot, fp32 ? "fp32" : "fp64",

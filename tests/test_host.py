@@ -11,6 +11,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import gpu_util as U
@@ -48,6 +49,7 @@ def test_defaults_and_duplicate_keys(tmp_path):
     assert d["dim"] == 3 and d["L"] == [16, 12, 8]
     assert (d["kbt"], d["dt"], d["tau"], d["rotang"]) == (1.0, 0.1, 0.5, 1.570796)
     assert (d["rtech"], d["lc"], d["tstech"], d["galinv"], d["simSteps"], d["warmUp"]) == (2, 0, 0, 1, 2000, 0)
+    assert d["flowOut"] == 0  # read.c:1732
     sp = d["species"][0]
     assert sp["pop"] == 1000 and sp["mass"] == 1.0 and sp["vdist"] == 0
     assert (sp["rfc"], sp["tumble"], sp["chihi"], sp["act"], sp["mfpot"]) == (0.01, 2.0, 0.5, 0.05, 10.0)
@@ -179,3 +181,34 @@ def test_host_runs_an_active_nematic(tmp_path):
     assert [int(x[0]) for x in rows] == [24, 49, 74, 99]
     for step, kbt, ke, px, py, pz in rows:
         assert 0.8 < kbt < 1.2
+
+
+@pytest.mark.gpu
+def test_host_writes_the_flow_field(tmp_path):
+    """flowOut: 13 header lines, then one block of L0 x L1 cell lines at every
+    runtime divisible by flowOut (pout.c:2690), each holding the sum over the
+    interval divided by flowOut (pout.c:1691).  In a periodic box the body
+    force gives every particle the same drift g dt s after s steps, so the
+    mean over cells of a block is g dt times the mean step count of its
+    interval, up to thermal noise (about 2e-3 here; a missing division or a
+    sum that is not restarted after a block is off by 0.05 or more)."""
+    g, dt, F = 0.02, 0.1, 50
+    cfg = {"domain": [30, 20], "simSteps": 199, "rTech": 0, "grav": [g, 0, 0], "seed": 5, "dt": dt,
+           "flowOut": F, "species": [{"dens": 50, "vDist": 4}]}
+    print("[host] periodic 2D box 30x20x50 with flowOut 50, 200 steps", flush=True)
+    info, rows = run_host(tmp_path, cfg, "--every", "100")
+    assert info["steps"] == 200
+    lines = open(tmp_path / "out" / "flowfield.dat").read().splitlines()
+    assert lines[13 - 1].split() == ["t", "QX", "QY", "QZ", "VcmX", "VcmY", "VcmZ"]
+    data = np.array([[float(x) for x in l.split()] for l in lines[13:]])
+    assert data.shape == (4 * 600, 7)
+    blocks = data.reshape(4, 600, 7)
+    for b, r in enumerate((0, 50, 100, 150)):
+        blk = blocks[b]
+        assert np.allclose(blk[:, 0], r * dt)
+        assert sorted(set(blk[:, 1].astype(int))) == list(range(30)) and sorted(set(blk[:, 2].astype(int))) == list(range(20))
+        assert np.all(blk[:, 3] == 0) and np.all(blk[:, 6] == 0.0)
+        # steps done at runtime s: s + 1; the block at r sums runtimes r - F + 1 .. r (just r = 0 for the first)
+        expect = g * dt * 1.0 / F if r == 0 else g * dt * (r - (F - 1) / 2.0 + 1.0)
+        assert abs(blk[:, 4].mean() - expect) < 1e-2, (r, blk[:, 4].mean(), expect)
+        assert abs(blk[:, 5].mean()) < 1e-2
