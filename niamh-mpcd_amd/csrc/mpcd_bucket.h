@@ -195,23 +195,26 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   __shared__ int32_t s_nwl, s_wlbase;  // wall-list entries of the workgroup, their first index
   __shared__ double s_ke[NT / 64];
 
-  // The first loads are independent, and all are issued before any is
-  // used, so a tile waits for one memory latency rather than one per load:
-  //  - every record of the thread.  These loads are unconditional: slot
-  //    tid + j * NT < CAP lies in the bucket's own allocation, and what a
-  //    slot at or past the bucket's count holds is never used (rk = -1);
-  //  - the bucket's count, and the terms of its slot base: the counts of the
-  //    earlier buckets of its block of kBlock (one a thread) plus the block
-  //    sums of the preparation kernel;
+  // The loads of a tile are issued before any of their values is used, so
+  // a tile waits for the bucket's count and then for one memory latency,
+  // not one per particle:
+  //  - every record of the thread.  These loads are unconditional: a slot at
+  //    or past the bucket's count reads the last filled slot again (a cached
+  //    line, no HBM traffic), and its values are never used (rk = -1);
+  //  - the terms of the bucket's slot base: the counts of the earlier
+  //    buckets of its block of kBlock (one a thread) plus the block sums of
+  //    the preparation kernel;
   //  - the poison flag, read once for the whole workgroup.
   static_assert(NT == kBlock, "one count of the bucket's block a thread");
   const size_t off = (size_t)blockIdx.x * CAP;
+  const int n_raw = I.cnt[blockIdx.x];
+  const int last = max(min(n_raw, CAP) - 1, 0);
   Real rq[PPT][3], rv[PPT][3];
   int rid[PPT], rsp[PPT], rk[PPT], ra[PPT];
   uint64_t rkey[PPT];
 #pragma unroll
-  for (int j = 0; j < PPT; j++) rec_load(I, off + tid + j * NT, ONESP, rq[j], rv[j], rid[j], rsp[j], rkey[j]);
-  const int n_raw = I.cnt[blockIdx.x];
+  for (int j = 0; j < PPT; j++)
+    rec_load(I, off + min(tid + j * NT, last), ONESP, rq[j], rv[j], rid[j], rsp[j], rkey[j]);
   __shared__ int s_base[NT / 64];
   {
     const int bb = blockIdx.x / kBlock, k1 = bb * kBlock + tid;
