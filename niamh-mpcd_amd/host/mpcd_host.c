@@ -27,9 +27,16 @@
  * sumFLOW, mpc.c:4946-4951), and at every runtime with runtime % F == 0
  * (pout.c:2690) DIR/flowfield.dat receives the sum divided by F for every
  * cell of the domain, in flowout's line format (pout.c:1681-1700), after
- * which the sum starts again.  The file opens with 13 header lines, as the
- * reference's does (a preamble, a blank line and flowheader's column
- * names, pout.c:355-357).  Replica r > 0 writes DIR/flowfield_r<r>.dat.
+ * which the sum starts again.
+ * velOut = F > 0: DIR/velfield.dat, every cell's centre-of-mass velocity at
+ * every runtime divisible by F (velout, pout.c:1715-1731, 2691).
+ * dirSOut = F > 0 with lc: DIR/directorfield.dat, every cell's director and
+ * scalar order parameter, zeros in a cell without particles (orderout,
+ * pout.c:2021-2033, 2695).
+ * Each field file opens with 13 header lines, as the reference's do (a
+ * preamble, a blank line and the reference's column names: flowheader,
+ * pout.c:355-357, for both velocity fields, orderheader, pout.c:98-100).
+ * Replica r > 0 writes DIR/<name>_r<r>.dat.
  */
 #include <math.h>
 #include <stdint.h>
@@ -70,6 +77,9 @@ static void check(mpcd_gpu_ctx *ctx, mpcd_status st, const char *what) {
 }
 
 /* keys this host rejects: they select parts of the reference outside the solvent path */
+/* flowheader's column names (pout.c:355-357), for the flow and velocity fields */
+#define FLOW_COLUMNS "   t\t   QX\t   QY\t   QZ\tVcmX\t\tVcmY\t\tVcmZ"
+
 /* flowout's lines (pout.c:1681-1700): time, cell indices and the mean
  * centre-of-mass velocity over the interval of every cell of the domain;
  * flow holds the compact cell array (x-major, (L+1) cells per axis, one z
@@ -86,27 +96,63 @@ static void flow_write(FILE *f, const double *flow, const int L[3], int dim, dou
             }
 }
 
-static FILE *flow_open(const char *outdir, int r, int nrep, int interval, const int L[3], int dim, int fp32) {
+/* velout's lines (pout.c:1715-1731): the instantaneous centre-of-mass
+ * velocity of every cell of the domain */
+static void vel_write(FILE *f, const double *vcm, const int L[3], int dim, double t) {
+    const int ny = L[1] + 1, nz = dim == 3 ? L[2] + 1 : 1, kz = dim == 3 ? L[2] : 1;
+    for (int i = 0; i < L[0]; i++)
+        for (int j = 0; j < L[1]; j++)
+            for (int k = 0; k < kz; k++) {
+                const double *v = vcm + 3 * ((size_t)(i * ny + j) * nz + k);
+                double vel[3] = {0.0, 0.0, 0.0};
+                for (int h = 0; h < dim; h++) vel[h] = v[h];
+                fprintf(f, "%12.5e\t%5d\t%5d\t%5d\t%12.5e\t%12.5e\t%12.5e\n", t, i, j, k, vel[0], vel[1], vel[2]);
+            }
+}
+
+/* orderout's lines (pout.c:2021-2033): director and S of every cell of the
+ * domain, zeros where the cell holds no particle */
+static void order_write(FILE *f, const int32_t *pop, const double *dir, const double *S, const int L[3], int dim,
+                        double t) {
+    const int ny = L[1] + 1, nz = dim == 3 ? L[2] + 1 : 1, kz = dim == 3 ? L[2] : 1;
+    for (int i = 0; i < L[0]; i++)
+        for (int j = 0; j < L[1]; j++)
+            for (int k = 0; k < kz; k++) {
+                const size_t c = (size_t)(i * ny + j) * nz + k;
+                fprintf(f, "%.2f\t%5d\t%5d\t%5d\t", t, i, j, k);
+                if (pop[c] == 0) fprintf(f, "%12.5e\t%12.5e\t%12.5e\t%12.5e\n", 0.0, 0.0, 0.0, 0.0);
+                else fprintf(f, "%12.5e\t%12.5e\t%12.5e\t%12.5e\n", dir[3 * c], dir[3 * c + 1], dir[3 * c + 2], S[c]);
+            }
+}
+
+/* a field file of replica r: DIR/<name>.dat (r = 0) or DIR/<name>_r<r>.dat,
+ * with 11 preamble lines, a blank line and the column names: readers of the
+ * reference's files skip 13 lines */
+static FILE *field_open(const char *outdir, const char *name, const char *what, const char *columns, int r,
+                        int nrep, int interval, const int L[3], int dim, int fp32) {
     char path[4096];
-    if (r == 0) snprintf(path, sizeof path, "%s/flowfield.dat", outdir);
-    else snprintf(path, sizeof path, "%s/flowfield_r%d.dat", outdir, r);
+    if (r == 0) snprintf(path, sizeof path, "%s/%s.dat", outdir, name);
+    else snprintf(path, sizeof path, "%s/%s_r%d.dat", outdir, name, r);
     FILE *f = fopen(path, "w");
     if (!f) die("cannot write %s", path);
-    /* 11 preamble lines and a blank line, then the column names: readers of
-     * the reference's files skip 13 lines */
-    fprintf(f, "# flow field: per cell, the centre-of-mass velocity averaged over\n");
-    fprintf(f, "# the steps of each output interval (localFLOW + sumFLOW, flowout)\n");
+    fprintf(f, "# %s\n", what);
+    fprintf(f, "# per cell of the domain, one block of cells per output, x-major\n");
     fprintf(f, "# written by mpcd_host through the C-ABI of mpcd_gpu.h\n");
     fprintf(f, "# replica %d of %d\n", r, nrep);
     fprintf(f, "# domain %d x %d x %d, %dD\n", L[0], L[1], dim == 3 ? L[2] : 1, dim);
-    fprintf(f, "# interval %d steps (flowOut)\n", interval);
+    fprintf(f, "# output every %d steps\n", interval);
     fprintf(f, "# particle state %s\n", fp32 ? "FP32, cell sums FP64" : "FP64");
-    fprintf(f, "# columns: time, cell indices QX QY QZ, mean VCM x y z\n");
-    fprintf(f, "# one block of cells per output, x-major\n");
-    fprintf(f, "# the first output (runtime 0) holds one step divided by the interval\n");
-    fprintf(f, "#\n\n");
-    fprintf(f, "   t\t   QX\t   QY\t   QZ\tVcmX\t\tVcmY\t\tVcmZ\n");
+    fprintf(f, "# columns: time, cell indices QX QY QZ, then the values\n");
+    fprintf(f, "#\n#\n#\n\n");
+    fprintf(f, "%s\n", columns);
     return f;
+}
+
+/* the runtimes divisible by F end after step runtime + 1: the number of
+ * steps to run from `done` completed steps to the next of them */
+static int steps_to_next(int done, int F) {
+    const int r = ((done + F - 1) / F) * F;
+    return r + 1 - done;
 }
 
 static void reject_unsupported(const json *J) {
@@ -249,6 +295,8 @@ int main(int argc, char **argv) {
     int sim_steps = json_int(J, "simSteps", 2000);
     const int warm = json_int(J, "warmUp", 0);
     const int flow_out = json_int(J, "flowOut", 0);
+    const int vel_out = json_int(J, "velOut", 0);
+    const int dir_out = json_int(J, "dirSOut", 0);
     if (steps_override >= 0) sim_steps = steps_override;
     p.galinv = json_int(J, "galInv", 1);
     p.tstech = json_int(J, "tsTech", 0);
@@ -315,9 +363,10 @@ int main(int argc, char **argv) {
         /* the parameters as read, without touching the GPU */
         printf("{\"dim\": %d, \"L\": [%d, %d, %d], \"dt\": %.17g, \"kbt\": %.17g, \"rotang\": %.17g, "
                "\"tau\": %.17g, \"grav\": [%.17g, %.17g, %.17g], \"rtech\": %d, \"lc\": %d, \"tstech\": %d, "
-               "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"flowOut\": %d, \"seed\": %ld, \"species\": [",
+               "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"flowOut\": %d, \"velOut\": %d, \"dirSOut\": %d, \"seed\": %ld, "
+               "\"species\": [",
                p.dim, p.L[0], p.L[1], p.L[2], p.dt, p.kbt, p.rotang, p.tau, p.grav[0], p.grav[1], p.grav[2], p.rtech,
-               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, seed);
+               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, vel_out, dir_out, seed);
         for (int s = 0; s < p.nspec; s++) {
             const mpcd_gpu_species *sp = &p.sp[s];
             printf("%s{\"pop\": %lld, \"mass\": %.17g, \"vdist\": %d, \"rfc\": %.17g, \"tumble\": %.17g, "
@@ -415,16 +464,36 @@ int main(int argc, char **argv) {
     const int total = sim_steps + 1;
     if (every <= 0) every = total > 10 ? total / 10 : 1;
     if (warm > 0) check(ctx, mpcd_gpu_run(ctx, warm + 1, use_graph), "warm-up");
-    /* the flow sums start after the warm-up (mpc.c:4947: !in.warmupSteps) */
-    FILE *ff[64] = {NULL};
-    double *flow = NULL;
+    /* field outputs; the flow sums start after the warm-up (mpc.c:4947:
+     * !in.warmupSteps); orderout needs LC (pout.c:2695) */
+    const int do_dir = p.lc && dir_out > 0;
+    FILE *ff[64] = {NULL}, *fv[64] = {NULL}, *fd[64] = {NULL};
+    const size_t nc = (size_t)mpcd_gpu_ncells(ctx, 0);
+    double *flow = NULL, *cvcm = NULL, *cS = NULL, *cdir = NULL;
+    int32_t *cpop = NULL;
+    if ((flow_out > 0 || vel_out > 0 || do_dir) && nrep > 64) die("%s", "field outputs: at most 64 replicas");
     if (flow_out > 0) {
-        if (nrep > 64) die("%s", "flowOut writes one file per replica: at most 64 replicas");
         check(ctx, mpcd_gpu_flow(ctx, 1), "mpcd_gpu_flow");
-        flow = (double *)malloc(sizeof(double) * 3 * (size_t)mpcd_gpu_ncells(ctx, 0));
+        flow = (double *)malloc(sizeof(double) * 3 * nc);
         if (!flow) die("%s", "out of host memory");
-        for (int r = 0; r < nrep; r++) ff[r] = flow_open(outdir, r, nrep, flow_out, p.L, dim, fp32);
+        for (int r = 0; r < nrep; r++)
+            ff[r] = field_open(outdir, "flowfield",
+                               "flow field: centre-of-mass velocity averaged over each output interval", FLOW_COLUMNS,
+                               r, nrep, flow_out, p.L, dim, fp32);
     }
+    if (vel_out > 0 || do_dir) {
+        cpop = (int32_t *)malloc(sizeof(int32_t) * nc);
+        cvcm = (double *)malloc(sizeof(double) * 3 * nc);
+        cS = (double *)malloc(sizeof(double) * nc);
+        cdir = (double *)malloc(sizeof(double) * 3 * nc);
+        if (!cpop || !cvcm || !cS || !cdir) die("%s", "out of host memory");
+    }
+    for (int r = 0; r < nrep && vel_out > 0; r++)
+        fv[r] = field_open(outdir, "velfield", "velocity field: instantaneous centre-of-mass velocity", FLOW_COLUMNS, r,
+                           nrep, vel_out, p.L, dim, fp32);
+    for (int r = 0; r < nrep && do_dir; r++)
+        fd[r] = field_open(outdir, "directorfield", "director field: director and scalar order parameter",
+                           "t\t\tQX\t\tQY\t\tQZ\t\tNX\t\tNY\t\tNZ\t\tS", r, nrep, dir_out, p.L, dim, fp32);
     check(ctx, mpcd_gpu_synchronize(ctx), "synchronize");
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
@@ -432,11 +501,9 @@ int main(int argc, char **argv) {
     int next_out = every < total ? every : total;
     for (int done = 0; done < total;) {
         int k = next_out - done;
-        if (flow_out > 0) {
-            /* the next runtime r >= done with r % flowOut == 0 ends after step r + 1 */
-            const int r = ((done + flow_out - 1) / flow_out) * flow_out;
-            if (r + 1 - done < k) k = r + 1 - done;
-        }
+        const int cad[3] = {flow_out, vel_out, do_dir ? dir_out : 0};
+        for (int q = 0; q < 3; q++)
+            if (cad[q] > 0 && steps_to_next(done, cad[q]) < k) k = steps_to_next(done, cad[q]);
         struct timespec a, b;
         clock_gettime(CLOCK_MONOTONIC, &a);
         check(ctx, mpcd_gpu_run(ctx, k, use_graph), "mpcd_gpu_run");
@@ -444,14 +511,24 @@ int main(int argc, char **argv) {
         clock_gettime(CLOCK_MONOTONIC, &b);
         busy += (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
         done += k;
-        if (flow_out > 0 && (done - 1) % flow_out == 0) {
-            /* flowout at runtime done - 1 (pout.c:2690), then the sums restart */
+        const int rt = done - 1;
+        const double tnow = (double)rt * p.dt;
+        if (flow_out > 0 && rt % flow_out == 0) {
+            /* flowout at this runtime (pout.c:2690), then the sums restart */
             for (int r = 0; r < nrep; r++) {
                 int64_t nsum = 0;
                 check(ctx, mpcd_gpu_flow_read(ctx, r, flow, &nsum, 1), "mpcd_gpu_flow_read");
-                flow_write(ff[r], flow, p.L, dim, (double)(done - 1) * p.dt, flow_out);
+                flow_write(ff[r], flow, p.L, dim, tnow, flow_out);
             }
         }
+        const int vel_due = vel_out > 0 && rt % vel_out == 0, dir_due = do_dir && rt % dir_out == 0;
+        if (vel_due || dir_due)
+            for (int r = 0; r < nrep; r++) {
+                /* the cells of the re-bin that ends the step (localPROP, mpc.c:4942) */
+                check(ctx, mpcd_gpu_get_cells(ctx, r, cpop, cvcm, cS, cdir), "mpcd_gpu_get_cells");
+                if (vel_due) vel_write(fv[r], cvcm, p.L, dim, tnow);
+                if (dir_due) order_write(fd[r], cpop, cdir, cS, p.L, dim, tnow);
+            }
         if (done < next_out) continue;
         next_out = (next_out + every < total) ? next_out + every : total;
         check(ctx, mpcd_gpu_download(ctx, 0, Q, V, NULL), "mpcd_gpu_download");
@@ -470,9 +547,16 @@ int main(int argc, char **argv) {
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
     fclose(fo);
-    for (int r = 0; r < nrep; r++)
+    for (int r = 0; r < nrep; r++) {
         if (ff[r]) fclose(ff[r]);
+        if (fv[r]) fclose(fv[r]);
+        if (fd[r]) fclose(fd[r]);
+    }
     free(flow);
+    free(cpop);
+    free(cvcm);
+    free(cS);
+    free(cdir);
     printf("{\"steps\": %d, \"replicas\": %d, \"particles_per_replica\": %lld, \"precision\": \"%s\", "
            "\"wall_s\": %.3f, \"step_s\": %.3f, \"particle_steps_per_s\": %.4g, \"output\": \"%s\"}\n",
            total, nrep, (long long)ntot, fp32 ? "fp32" : "fp64",

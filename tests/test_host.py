@@ -49,7 +49,7 @@ def test_defaults_and_duplicate_keys(tmp_path):
     assert d["dim"] == 3 and d["L"] == [16, 12, 8]
     assert (d["kbt"], d["dt"], d["tau"], d["rotang"]) == (1.0, 0.1, 0.5, 1.570796)
     assert (d["rtech"], d["lc"], d["tstech"], d["galinv"], d["simSteps"], d["warmUp"]) == (2, 0, 0, 1, 2000, 0)
-    assert d["flowOut"] == 0  # read.c:1732
+    assert (d["flowOut"], d["velOut"], d["dirSOut"]) == (0, 0, 0)  # read.c:1732, 1735, 1738
     sp = d["species"][0]
     assert sp["pop"] == 1000 and sp["mass"] == 1.0 and sp["vdist"] == 0
     assert (sp["rfc"], sp["tumble"], sp["chihi"], sp["act"], sp["mfpot"]) == (0.01, 2.0, 0.5, 0.05, 10.0)
@@ -246,3 +246,39 @@ def test_host_flow_field_resolves_a_channel(tmp_path):
     assert middle - walls > 0.08, rows
     assert abs(rows[0] - rows[19]) < 0.05, rows
     assert np.all(np.diff(rows[:10]) > -0.03) and np.all(np.diff(rows[10:]) < 0.03), rows
+
+
+@pytest.mark.gpu
+def test_host_velocity_and_director_fields(tmp_path):
+    """velOut and dirSOut on a 2D active nematic.  With flowOut = 1 the flow
+    field at a runtime holds that one step's cell velocities, summed by the
+    flow kernels, and the velocity field holds the same velocities from the
+    list-order cell sums (localPROP): the two files agree cell by cell to the
+    printed precision.  The director field holds a unit director and S in
+    [0, 1] in every occupied cell (orderout, pout.c:2021-2033)."""
+    cfg = {"domain": [16, 12], "simSteps": 9, "lc": 1, "collOp": 16, "seed": 4, "flowOut": 1, "velOut": 1,
+           "dirSOut": 5, "species": [{"dens": 20, "vDist": 4, "mfpot": 10, "tumble": 0.5, "act": 0.05}]}
+    print("[host] 2D active nematic 16x12x20, 10 steps, flowOut 1, velOut 1, dirSOut 5", flush=True)
+    run_host(tmp_path, cfg, "--every", "10")
+    out = tmp_path / "out"
+
+    def table(name):
+        lines = open(out / name).read().splitlines()
+        return lines[12], np.array([[float(x) for x in l.split()] for l in lines[13:]])
+
+    hf, flow = table("flowfield.dat")
+    hv, vel = table("velfield.dat")
+    hd, dirs = table("directorfield.dat")
+    assert hf.split() == hv.split() == ["t", "QX", "QY", "QZ", "VcmX", "VcmY", "VcmZ"]
+    assert hd.split() == ["t", "QX", "QY", "QZ", "NX", "NY", "NZ", "S"]
+    assert flow.shape == vel.shape == (10 * 16 * 12, 7)
+    np.testing.assert_array_equal(flow[:, :4], vel[:, :4])
+    np.testing.assert_allclose(flow[:, 4:], vel[:, 4:], rtol=2e-5, atol=2e-6)
+    assert np.abs(vel[:, 4:6]).max() > 0.05 and np.all(vel[:, 6] == 0.0)
+    assert dirs.shape == (2 * 16 * 12, 8)
+    assert sorted(set(np.round(dirs[:, 0], 2))) == [0.0, 0.5]
+    occ = np.abs(dirs[:, 4:7]).sum(axis=1) > 0
+    assert occ.mean() > 0.95
+    np.testing.assert_allclose(np.linalg.norm(dirs[occ, 4:7], axis=1), 1.0, atol=1e-4)
+    assert np.all(dirs[:, 6] == 0.0)
+    assert np.all((dirs[occ, 7] >= 0.0) & (dirs[occ, 7] <= 1.0))
