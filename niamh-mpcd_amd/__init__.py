@@ -72,7 +72,7 @@ def build_host(verbose: bool = False) -> str:
     cc = shutil.which("gcc") or "cc"
     host = os.path.join(HERE, "host")
     cmd = [cc, "-O2", "-std=c11", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-I", INCLUDE,
-           os.path.join(host, "mpcd_host.c"), os.path.join(host, "json.c"),
+           os.path.join(host, "mpcd_host.c"), os.path.join(host, "json.c"), os.path.join(host, "checkpoint.c"),
            "-L", os.path.dirname(LIB_PATH), "-lmpcd_gpu", "-Wl,-rpath,$ORIGIN", "-lm", "-o", HOST_PATH + ".tmp"]
     if verbose:
         print(" ".join(cmd))

@@ -1,0 +1,271 @@
+/*
+ * checkpoint.c -- reads and writes the reference's checkpoint file (the
+ * layout of checkpoint(), mpcd/subroutines/pout.c:2325-2404).
+ *
+ * The reader follows the order in which checkpoint() writes; the reference's
+ * own reader takes the output-flag record in a different order
+ * (read.c:722), so the writer's order is the file's.  Every value is read as
+ * a double, which also parses the integers.
+ */
+#include "checkpoint.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    FILE *f;
+    int ok;
+} rd_t;
+
+static double rd(rd_t *r) {
+    double v = 0.0;
+    if (r->ok && fscanf(r->f, "%lf", &v) != 1) r->ok = 0;
+    return v;
+}
+
+static void rdn(rd_t *r, double *v, int n) {
+    for (int k = 0; k < n; k++) v[k] = rd(r);
+}
+
+static void skip(rd_t *r, int n) {
+    for (int k = 0; k < n; k++) (void)rd(r);
+}
+
+#define FAIL(...)                              \
+    do {                                       \
+        snprintf(err, errlen, __VA_ARGS__);    \
+        if (f) fclose(f);                      \
+        ckpt_free(ck);                         \
+        return -1;                             \
+    } while (0)
+
+int ckpt_read(const char *path, mpcd_gpu_params *p, mpcd_ckpt *ck, char *err, size_t errlen) {
+    memset(ck, 0, sizeof *ck);
+    FILE *f = fopen(path, "r");
+    if (!f) FAIL("cannot open the checkpoint %s", path);
+    rd_t R = {f, 1};
+    double v[32];
+    memset(p, 0, sizeof *p);
+    ck->sim_steps = (int)rd(&R);
+    ck->warm_steps = (int)rd(&R);
+    p->dt = rd(&R);
+    ck->seed = (long)rd(&R);
+    rdn(&R, v, 6);  /* DIM, XYZ[3], KBT, KBTNOW */
+    p->dim = (int)v[0];
+    for (int d = 0; d < 3; d++) p->L[d] = (int)v[1 + d];
+    p->kbt = v[4];
+    ck->kbtnow = v[5];
+    rdn(&R, v, 6);  /* RFRAME, zeroNetMom, GALINV, TSTECH, RTECH, LC */
+    /* rFrame defaults to 1 (read.c:1062) but resets only every zeroNetMom > 0 steps */
+    if (R.ok && v[0] != 0.0 && v[1] > 0.0) FAIL("checkpoint: periodic zero-net-momentum resets are outside this path");
+    p->galinv = (int)v[2];
+    p->tstech = (int)v[3];
+    p->rtech = (int)v[4];
+    p->lc = (int)v[5];
+    rdn(&R, v, 3);  /* TAU, RA, FRICCO */
+    p->tau = v[0];
+    p->rotang = v[1];
+    rdn(&R, v, 4);  /* noHI, inCOMP, MULTIPHASE, MFPLAYERH */
+    if (R.ok && (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0)) FAIL("checkpoint: noHI / inCOMP / multiphase runs are outside this path");
+    rdn(&R, p->grav, 3);
+    skip(&R, 3);  /* MAG */
+    rdn(&R, v, 2);  /* MD mode, stepsMD */
+    if (R.ok && v[0] != 0.0) FAIL("checkpoint: MD coupling is outside this path");
+    rdn(&R, v, 2);  /* GPOP, NSPECI */
+    ck->n = (int64_t)v[0];
+    p->nspec = (int)v[1];
+    if (R.ok && (p->nspec < 1 || p->nspec > MPCD_MAXSPEC)) FAIL("checkpoint: %d species", p->nspec);
+    if (R.ok && (p->dim < 2 || p->dim > 3 || ck->n <= 0)) FAIL("checkpoint: dimension %d, %lld particles", p->dim, (long long)ck->n);
+    rdn(&R, v, 7);  /* runtime, warmtime, C, S, GRAV_FLAG, MAG_FLAG, tolD */
+    ck->runtime = (int)v[0];
+    ck->warmtime = (int)v[1];
+    skip(&R, 8);  /* AVVEL, AVS, avDIR[3], S4, stdN, VOL */
+    rdn(&R, p->avv, 3);
+    rdn(&R, ck->avnow, 3);
+    skip(&R, 6);  /* theory (global) */
+    skip(&R, 2);  /* chckpntIn, MDmode */
+    rdn(&R, v, 32);  /* output flags, in checkpoint()'s order */
+    ck->flow_out = (int)v[4];
+    ck->vel_out = (int)v[6];
+    ck->dir_out = (int)v[10];
+    ck->ckpt_out = (int)v[29];
+    skip(&R, 2 + 7 + 5 + 3);  /* SPOUT PRESOUT, histograms, spectra/topology, swimmer outputs */
+    for (int s = 0; s < p->nspec && R.ok; s++) {
+        mpcd_gpu_species *sp = &p->sp[s];
+        rdn(&R, v, 5);  /* MASS, POP, QDIST, VDIST, ODIST */
+        sp->mass = v[0];
+        sp->vdist = (int)v[3];
+        rdn(&R, v, 10);  /* RFC, LEN, TUMBLE, CHIHI, CHIA, ACT, MFPOT, SIGWIDTH, SIGPOS, DAMP */
+        sp->rfc = v[0];
+        sp->tumble = v[2];
+        sp->chihi = v[3];
+        sp->act = v[5];
+        sp->mfpot = v[6];
+        sp->sigwidth = v[7];
+        sp->sigpos = v[8];
+        sp->damp = v[9];
+        rdn(&R, v, 5);  /* VOL, nDNST, mDNST, MINACTRATIO, BS */
+        sp->minactratio = v[3];
+        skip(&R, p->nspec + 6);  /* binary-fluid M[], species theory */
+    }
+    rdn(&R, v, 4);  /* GnDNST, GmDNST, GMASS, maxXYZ */
+    p->gndnst = v[0];
+    p->nwall = (int)rd(&R);
+    if (R.ok && (p->nwall < 0 || p->nwall > MPCD_MAXWALL)) FAIL("checkpoint: %d boundaries", p->nwall);
+    for (int w = 0; w < p->nwall && R.ok; w++) {
+        mpcd_gpu_wall *W = &p->wall[w];
+        rdn(&R, v, 12);  /* COLL_TYPE, PHANTOM, E, Q[3], V[3], O[3] */
+        W->COLL_TYPE = (int)v[0];
+        W->PHANTOM = (int)v[1];
+        memcpy(W->Q, v + 3, sizeof W->Q);
+        memcpy(W->V, v + 6, sizeof W->V);
+        const double O[3] = {v[9], v[10], v[11]};
+        rdn(&R, v, 20);  /* L[3], G[3], A[3], AINV[3], P[4], R, B[3] */
+        memcpy(W->L, v, sizeof W->L);
+        memcpy(W->A, v + 6, sizeof W->A);
+        W->R = v[16];
+        int naxis = 0;
+        for (int d = 0; d < 3; d++) naxis += fabs(W->A[d]) >= 1e-8;
+        const int planar = naxis == 1 && fabs(v[17]) <= 1e-8 && fabs(v[12] - 1) <= 1e-8 && fabs(v[13] - 1) <= 1e-8 &&
+                           fabs(v[14] - 1) <= 1e-8;
+        const int turned = fabs(O[0]) + fabs(O[1]) + fabs(O[2]) > 0.0;
+        rdn(&R, v, 14);  /* DN, DT, DVN, DVT, DVxyz[3], MVN, MVT, MUN, MUT, MUxyz[3] */
+        W->DN = v[0];
+        W->DT = v[1];
+        W->DVN = v[2];
+        W->DVT = v[3];
+        memcpy(W->DVxyz, v + 4, sizeof W->DVxyz);
+        W->MVN = v[7];
+        W->MVT = v[8];
+        W->MUN = v[9];
+        W->MUT = v[10];
+        memcpy(W->MUxyz, v + 11, sizeof W->MUxyz);
+        rdn(&R, v, 7);  /* DUxyz[3], KBT, DSPLC, INV, MASS */
+        memcpy(W->DUxyz, v, sizeof W->DUxyz);
+        const int dsplc = (int)v[4];
+        W->INV = (int)v[5];
+        skip(&R, 17 + 5 + 6);  /* W, VOL, Q_old, O_old, I; PLANAR, REORIENT, ABS, ROTSYMM; dV, dL */
+        rdn(&R, v, CKPT_MAXSPECI + 2);
+        W->inter_mask = 0;
+        for (int s = 0; s < p->nspec; s++)
+            if (v[s] == 1.0) W->inter_mask |= 1 << s;  /* BCON = 1: interacts (bc.c:1263) */
+        if (R.ok && (!planar || turned || dsplc)) FAIL("checkpoint: boundary %d is not a fixed axis-aligned plane", w);
+    }
+    if (!R.ok) FAIL("checkpoint %s: the file ends or breaks before the particles", path);
+    ck->Q = (double *)calloc(3 * (size_t)ck->n, sizeof(double));
+    ck->V = (double *)calloc(3 * (size_t)ck->n, sizeof(double));
+    ck->U = (double *)calloc(3 * (size_t)ck->n, sizeof(double));
+    ck->spid = (int32_t *)calloc((size_t)ck->n, sizeof(int32_t));
+    if (!ck->Q || !ck->V || !ck->U || !ck->spid) FAIL("checkpoint: out of host memory");
+    for (int64_t i = 0; i < ck->n && R.ok; i++) {
+        rdn(&R, v, 15);  /* S_flag, SPID, q, Q[3], V[3], U[3], T[3] */
+        ck->spid[i] = (int32_t)v[1];
+        memcpy(ck->Q + 3 * i, v + 3, 3 * sizeof(double));
+        memcpy(ck->V + 3 * i, v + 6, 3 * sizeof(double));
+        memcpy(ck->U + 3 * i, v + 9, 3 * sizeof(double));
+        if (ck->spid[i] < 0 || ck->spid[i] >= p->nspec) FAIL("checkpoint: particle %lld of species %d", (long long)i, ck->spid[i]);
+    }
+    const int ns = (int)rd(&R);
+    if (!R.ok) FAIL("checkpoint %s: the file ends or breaks in the particles", path);
+    if (ns != 0) FAIL("checkpoint: swimmers are outside this path");
+    fclose(f);
+    return 0;
+}
+
+void ckpt_write(FILE *f, const mpcd_gpu_params *p, const mpcd_ckpt *ck) {
+    const int dim = p->dim;
+    const int X = p->L[0], Y = p->L[1], Z = dim == 3 ? p->L[2] : 1;
+    const double vol = (double)X * Y * Z;
+    int64_t pop[MPCD_MAXSPEC] = {0};
+    double gmass = 0.0;
+    for (int64_t i = 0; i < ck->n; i++) {
+        pop[ck->spid[i]]++;
+        gmass += p->sp[ck->spid[i]].mass;
+    }
+    int grav_flag = 0;
+    for (int d = 0; d < 3; d++) grav_flag |= p->grav[d] != 0.0;
+    fprintf(f, "%d\n", ck->sim_steps);
+    fprintf(f, "%d %lf\n", ck->warm_steps, p->dt);
+    fprintf(f, "%ld\n", ck->seed);
+    fprintf(f, "%d %d %d %d %lf %lf\n", dim, X, Y, Z, p->kbt, ck->kbtnow);
+    fprintf(f, "%d %d %d %d %d %d\n", 0, 0, p->galinv, p->tstech, p->rtech, p->lc);
+    fprintf(f, "%lf %lf %lf\n", p->tau, p->rotang, 1.0);
+    fprintf(f, "%d %d %d %d\n", 0, 0, 0, 0);
+    fprintf(f, "%lf %lf %lf\n", p->grav[0], p->grav[1], p->grav[2]);
+    fprintf(f, "%lf %lf %lf\n", 0.0, 0.0, 0.0);
+    fprintf(f, "%d %d\n", 0, 20);
+    fprintf(f, "%lld %d\n", (long long)ck->n, p->nspec);
+    fprintf(f, "%d %d %lf %lf %d %d %lf\n", ck->runtime, ck->warmtime, cos(p->rotang), sin(p->rotang), grav_flag, 0, 0.0);
+    fprintf(f, "%lf %lf %lf %lf %lf %lf %lf %lf\n", 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, vol);
+    fprintf(f, "%lf %lf %lf %lf %lf %lf\n", p->avv[0], p->avv[1], p->avv[2], ck->avnow[0], ck->avnow[1], ck->avnow[2]);
+    fprintf(f, "%lf %lf %lf %lf %lf %lf\n", 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    fprintf(f, "%d %d\n", 0, 0);
+    /* output flags in checkpoint()'s order: DBUG, TRAJOUT, printSP, COAROUT,
+     * FLOWOUT, SWFLOWOUT, VELOUT, DENSITYOUT, AVVELOUT, AVORIOUT, ORDEROUT,
+     * ..., CHCKPNT (30th), CHCKPNTrcvr, CHCKPNTTIMER */
+    int fl[31] = {0};
+    fl[0] = -1;
+    fl[4] = ck->flow_out;
+    fl[6] = ck->vel_out;
+    fl[10] = ck->dir_out;
+    fl[29] = ck->ckpt_out;
+    for (int k = 0; k < 31; k++) fprintf(f, "%d ", fl[k]);
+    fprintf(f, "%f\n", 0.0);
+    fprintf(f, "%d %d\n", 0, 0);
+    fprintf(f, "%d %d %d %d %d %d %d\n", 0, 0, 0, 0, 0, 0, 0);
+    fprintf(f, "%d %d %d %d %d\n", 0, 0, 0, 0, 0);
+    fprintf(f, "%d %d %d\n", 0, 0, 0);
+    for (int s = 0; s < p->nspec; s++) {
+        const mpcd_gpu_species *sp = &p->sp[s];
+        const double n_dens = (double)pop[s] / vol;
+        fprintf(f, "%lf %i %i %i %i ", sp->mass, (int)pop[s], 0, sp->vdist, 0);
+        fprintf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf ", sp->rfc, 0.007, sp->tumble, sp->chihi, 0.0, sp->act,
+                sp->mfpot, sp->sigwidth, sp->sigpos, sp->damp);
+        fprintf(f, "%lf %lf %lf %lf %lf\n", vol, n_dens, n_dens * sp->mass, sp->minactratio, 0.0);
+        for (int j = 0; j < p->nspec; j++) fprintf(f, "%lf ", 0.0);
+        fprintf(f, "\n");
+        fprintf(f, "%lf %lf %lf %lf %lf %lf\n", 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    }
+    const int maxxyz = X > Y ? (X > Z ? X : Z) : (Y > Z ? Y : Z);
+    fprintf(f, "%lf %lf %lf %d\n", p->gndnst, ck->n > 0 ? p->gndnst * gmass / (double)ck->n : 0.0, gmass, maxxyz);
+    fprintf(f, "%d\n", p->nwall);
+    for (int w = 0; w < p->nwall; w++) {
+        const mpcd_gpu_wall *W = &p->wall[w];
+        double ainv[3];
+        for (int d = 0; d < 3; d++) ainv[d] = W->A[d] != 0.0 ? 1.0 / W->A[d] : 0.0;
+        fprintf(f, "%d %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", W->COLL_TYPE, W->PHANTOM, -1.0, W->Q[0], W->Q[1],
+                W->Q[2], W->V[0], W->V[1], W->V[2], 0.0, 0.0, 0.0);
+        fprintf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", W->L[0], W->L[1],
+                W->L[2], 0.0, 0.0, 0.0, W->A[0], W->A[1], W->A[2], ainv[0], ainv[1], ainv[2], 1.0, 1.0, 1.0, 1.0, W->R,
+                0.0, 0.0, 0.0);
+        fprintf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", W->DN, W->DT, W->DVN, W->DVT,
+                W->DVxyz[0], W->DVxyz[1], W->DVxyz[2], W->MVN, W->MVT, W->MUN, W->MUT, W->MUxyz[0], W->MUxyz[1],
+                W->MUxyz[2]);
+        fprintf(f, "%lf %lf %lf %lf %d %d %lf\n", W->DUxyz[0], W->DUxyz[1], W->DUxyz[2], 1.0, 0, W->INV, 1.0);
+        for (int k = 0; k < 17; k++) fprintf(f, k ? " %lf" : "%lf", 0.0);  /* W, VOL, Q_old, O_old, I */
+        fprintf(f, "\n");
+        fprintf(f, "%d %d %d %lf %lf\n", 1, 0, 0, 4.0, 4.0);
+        fprintf(f, "%lf %lf %lf %lf %lf %lf\n", 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+        /* species slots past nspec, MD and swimmers keep the default BCON = 1 (read.c:1375) */
+        for (int j = 0; j < CKPT_MAXSPECI + 2; j++) fprintf(f, "%i ", j < p->nspec ? (W->inter_mask >> j) & 1 : 1);
+        fprintf(f, "\n");
+    }
+    for (int64_t i = 0; i < ck->n; i++) {
+        const double *q = ck->Q + 3 * i, *v = ck->V + 3 * i, *u = ck->U + 3 * i;
+        fprintf(f, "%d %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", 1, ck->spid[i], 0.0, q[0], q[1], q[2],
+                v[0], v[1], v[2], u[0], u[1], u[2], 0.0, 0.0, 0.0);
+    }
+    fprintf(f, "%d %d %d %d %d %d %lf %lf %d %d\n", 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0, 0);
+    fprintf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %d %lf %lf %lf %lf %d %lf\n", 0.0, 0.0, 0.0, 0.0, 0.0,
+            0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0.0);
+}
+
+void ckpt_free(mpcd_ckpt *ck) {
+    free(ck->Q);
+    free(ck->V);
+    free(ck->U);
+    free(ck->spid);
+    ck->Q = ck->V = ck->U = NULL;
+    ck->spid = NULL;
+}

@@ -4,6 +4,7 @@
  * temporal loops (mpcd/mpcd.c:255-300) through the C-ABI of mpcd_gpu.h.
  *
  *   mpcd_host input.json [-o DIR] [--fp32] [--replicas N] [--every K] [--steps N] [--seed S] [--no-graph]
+ *             [--dump-params] [--write-checkpoint FILE]
  *
  * JSON keys and defaults follow readJson (mpcd/subroutines/read.c:973-1804)
  * for the options of this path: domain, kbt, dt, simSteps, warmUp, galInv,
@@ -12,8 +13,15 @@
  * sigPos, minActRatio, damp) and planar BC entries (aInv, Q, P, R, DN, DT,
  * DVN, DVT, DVxyz, MVN, MVT, MUN, MUT, MUxyz, DUxyz, V, L, inv, phantom,
  * colType, interSRD, homeotropic, planar).  Options outside the path (MD,
- * swimmers, checkpoints, curved or wavy walls) stop with a message, as the
- * reference stops on an unknown option.
+ * swimmers, curved or wavy walls) stop with a message, as the reference
+ * stops on an unknown option.
+ *
+ * Checkpoints in the reference's file layout (checkpoint.h): checkpointIn
+ * recovers one as readchckpnt does (its inputs, particles, loop counters and
+ * output flags; no warm-up; the temporal loop from its runtime), and
+ * checkpointOut = F writes DIR/checkpoint.dat after the outputs of every
+ * runtime divisible by F (mpcd.c:310-312; replica 0).  --write-checkpoint
+ * FILE writes the state the run would start from and exits without the GPU.
  *
  * Output: DIR/solvent.dat with one line per output step: the temporal
  * loop's step counter (runtime, 0..simSteps, mpcd.c:291), the
@@ -46,6 +54,7 @@
 #include <sys/stat.h>
 #include <time.h>
 
+#include "checkpoint.h"
 #include "json.h"
 #include "mpcd_gpu.h"
 
@@ -155,9 +164,46 @@ static int steps_to_next(int done, int F) {
     return r + 1 - done;
 }
 
+/* a checkpoint of one replica's state (Q, V, U in particle-index order):
+ * the current temperature and mean velocity are computed here, as KBTNOW
+ * and AVNOW of the reference's file */
+static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int warm_steps, int runtime, int warmtime,
+                           long seed, int flow_out, int vel_out, int dir_out, int ckpt_out, size_t n, const double *Q,
+                           const double *V, const double *U, const int32_t *spid) {
+    mpcd_ckpt ck;
+    memset(&ck, 0, sizeof ck);
+    ck.sim_steps = sim_steps;
+    ck.warm_steps = warm_steps;
+    ck.runtime = runtime;
+    ck.warmtime = warmtime;
+    ck.seed = seed;
+    ck.flow_out = flow_out;
+    ck.vel_out = vel_out;
+    ck.dir_out = dir_out;
+    ck.ckpt_out = ckpt_out;
+    ck.n = (int64_t)n;
+    ck.Q = (double *)Q;
+    ck.V = (double *)V;
+    ck.U = (double *)U;
+    ck.spid = (int32_t *)spid;
+    double ke = 0.0, mom[3] = {0, 0, 0}, mtot = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const double m = p->sp[spid[i]].mass;
+        for (int d = 0; d < 3; d++) {
+            const double dv = V[3 * i + d] - p->avv[d];
+            ke += m * dv * dv;
+            mom[d] += m * V[3 * i + d];
+        }
+        mtot += m;
+    }
+    ck.kbtnow = n ? ke / (p->dim * (double)n) : 0.0;  /* TEMP, therm.c:209-235 */
+    for (int d = 0; d < 3; d++) ck.avnow[d] = mtot > 0.0 ? mom[d] / mtot : 0.0;
+    ckpt_write(f, p, &ck);
+}
+
 static void reject_unsupported(const json *J) {
-    const char *strs[] = {"checkpointIn", "mdIn"};
-    for (int k = 0; k < 2; k++) {
+    const char *strs[] = {"mdIn"};
+    for (int k = 0; k < 1; k++) {
         const char *v = json_str(J, strs[k], "");
         if (v[0]) die("\"%s\" (outside the solvent path) is not supported", strs[k]);
     }
@@ -256,6 +302,7 @@ static int domain_walls(int dim, const int L[3], int mode, int nspec, mpcd_gpu_w
 int main(int argc, char **argv) {
     const char *input = NULL, *outdir = ".";
     int fp32 = 0, nrep = 1, every = 0, steps_override = -1, use_graph = 1, dump = 0;
+    const char *ckpt_now = NULL;
     long seed_override = -1;
     for (int a = 1; a < argc; a++) {
         if (!strcmp(argv[a], "-o") && a + 1 < argc) outdir = argv[++a];
@@ -266,6 +313,7 @@ int main(int argc, char **argv) {
         else if (!strcmp(argv[a], "--seed") && a + 1 < argc) seed_override = atol(argv[++a]);
         else if (!strcmp(argv[a], "--no-graph")) use_graph = 0;
         else if (!strcmp(argv[a], "--dump-params")) dump = 1;
+        else if (!strcmp(argv[a], "--write-checkpoint") && a + 1 < argc) ckpt_now = argv[++a];
         else if (argv[a][0] != '-' && !input) input = argv[a];
         else die("unknown argument %s", argv[a]);
     }
@@ -286,18 +334,22 @@ int main(int argc, char **argv) {
     mpcd_gpu_params p;
     memset(&p, 0, sizeof p);
     double dom[3] = {0, 0, 0};
-    const int dim = json_nums(J, "domain", dom, 3);
-    if (dim != 2 && dim != 3) die("%s", "domain must have 2 or 3 entries");
+    /* a checkpoint (read.c:1116-1121) replaces the inputs, the particles and
+     * the loop counters after readJson, as readchckpnt does (mpcd.c:175-180) */
+    const char *ckpt_in = json_str(J, "checkpointIn", "");
+    int dim = json_nums(J, "domain", dom, 3);
+    if (!ckpt_in[0] && dim != 2 && dim != 3) die("%s", "domain must have 2 or 3 entries");
+    if (dim != 2 && dim != 3) dim = 2;
     p.dim = dim;
     for (int d = 0; d < 3; d++) p.L[d] = d < dim ? (int)dom[d] : 1;
     p.kbt = json_num(J, "kbt", 1.0);
     p.dt = json_num(J, "dt", 0.1);
     int sim_steps = json_int(J, "simSteps", 2000);
-    const int warm = json_int(J, "warmUp", 0);
-    const int flow_out = json_int(J, "flowOut", 0);
-    const int vel_out = json_int(J, "velOut", 0);
-    const int dir_out = json_int(J, "dirSOut", 0);
-    if (steps_override >= 0) sim_steps = steps_override;
+    int warm = json_int(J, "warmUp", 0);
+    int flow_out = json_int(J, "flowOut", 0);
+    int vel_out = json_int(J, "velOut", 0);
+    int dir_out = json_int(J, "dirSOut", 0);
+    int ckpt_out = json_int(J, "checkpointOut", 0);
     p.galinv = json_int(J, "galInv", 1);
     p.tstech = json_int(J, "tsTech", 0);
     /* getJObjIntMultiple (cJson.c:265-279): each tag that differs from the
@@ -346,7 +398,7 @@ int main(int argc, char **argv) {
         sp->damp = json_num(o, "damp", 0.0);
         ntot += pop[s];
     }
-    if (ntot <= 0) die("%s", "no particles");
+    if (ntot <= 0 && !ckpt_in[0]) die("%s", "no particles");
 
     /* boundaries: the BC list, or the domainWalls override */
     const json *bca = json_get(J, "BC");
@@ -359,14 +411,44 @@ int main(int argc, char **argv) {
         p.nwall = domain_walls(dim, p.L, dwalls == 2 ? 2 : 1, p.nspec, p.wall);
     }
 
+    /* recovering a checkpoint: its inputs, particles, loop counters and
+     * output flags replace the input file's; the warm-up is skipped and the
+     * temporal loop starts at the checkpoint's runtime (mpcd.c:257, 290) */
+    mpcd_ckpt ck;
+    memset(&ck, 0, sizeof ck);
+    int run0 = 0, warmtime = warm > 0 ? warm + 1 : 0;
+    if (ckpt_in[0]) {
+        char cerr[512];
+        mpcd_gpu_params q;
+        if (ckpt_read(ckpt_in, &q, &ck, cerr, sizeof cerr)) die("%s", cerr);
+        q.precision = p.precision;
+        p = q;
+        dim = p.dim;
+        sim_steps = ck.sim_steps;
+        warm = 0;
+        run0 = ck.runtime;
+        warmtime = ck.warmtime;
+        flow_out = ck.flow_out;
+        vel_out = ck.vel_out;
+        dir_out = ck.dir_out;
+        ckpt_out = ck.ckpt_out;
+        if (seed_override < 0 && ck.seed != 0) seed = ck.seed;
+        ntot = ck.n;
+        for (int s = 0; s < MPCD_MAXSPEC; s++) pop[s] = 0;
+        for (int64_t i = 0; i < ck.n; i++) pop[ck.spid[i]]++;
+    }
+    if (steps_override >= 0) sim_steps = steps_override;
+
     if (dump) {
         /* the parameters as read, without touching the GPU */
         printf("{\"dim\": %d, \"L\": [%d, %d, %d], \"dt\": %.17g, \"kbt\": %.17g, \"rotang\": %.17g, "
                "\"tau\": %.17g, \"grav\": [%.17g, %.17g, %.17g], \"rtech\": %d, \"lc\": %d, \"tstech\": %d, "
-               "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"flowOut\": %d, \"velOut\": %d, \"dirSOut\": %d, \"seed\": %ld, "
-               "\"species\": [",
+               "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"flowOut\": %d, \"velOut\": %d, \"dirSOut\": %d, "
+               "\"checkpointOut\": %d, \"runtime\": %d, \"gndnst\": %.17g, \"avv\": [%.17g, %.17g, %.17g], "
+               "\"seed\": %ld, \"species\": [",
                p.dim, p.L[0], p.L[1], p.L[2], p.dt, p.kbt, p.rotang, p.tau, p.grav[0], p.grav[1], p.grav[2], p.rtech,
-               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, vel_out, dir_out, seed);
+               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, vel_out, dir_out, ckpt_out, run0, p.gndnst, p.avv[0],
+               p.avv[1], p.avv[2], seed);
         for (int s = 0; s < p.nspec; s++) {
             const mpcd_gpu_species *sp = &p.sp[s];
             printf("%s{\"pop\": %lld, \"mass\": %.17g, \"vdist\": %d, \"rfc\": %.17g, \"tumble\": %.17g, "
@@ -404,7 +486,16 @@ int main(int argc, char **argv) {
     mpcd_gpu_params *pr = (mpcd_gpu_params *)calloc((size_t)nrep, sizeof(mpcd_gpu_params));
     int64_t *nr = (int64_t *)calloc((size_t)nrep, sizeof(int64_t));
     if (!Q || !V || !U || !spid || !ccount || !pr || !nr) die("%s", "out of host memory");
-    for (int r = 0; r < nrep; r++) {
+    for (int r = 0; r < nrep && ckpt_in[0]; r++) {
+        /* every replica starts from the checkpoint's particles */
+        memcpy(Q + 3 * n * r, ck.Q, sizeof(double) * 3 * n);
+        memcpy(V + 3 * n * r, ck.V, sizeof(double) * 3 * n);
+        memcpy(U + 3 * n * r, ck.U, sizeof(double) * 3 * n);
+        memcpy(spid, ck.spid, sizeof(int32_t) * n);
+        pr[r] = p;
+        nr[r] = ntot;
+    }
+    for (int r = 0; r < nrep && !ckpt_in[0]; r++) {
         double *Qr = Q + 3 * n * r, *Vr = V + 3 * n * r, *Ur = U + 3 * n * r;
         size_t i = 0;
         double mom[3] = {0, 0, 0}, mtot = 0.0;
@@ -446,6 +537,17 @@ int main(int argc, char **argv) {
         pr[r].gndnst = (double)n / occ;
         nr[r] = ntot;
     }
+    p.gndnst = pr[0].gndnst;
+    if (ckpt_now) {
+        /* the state the run would start from, as a checkpoint; no GPU */
+        FILE *fc = fopen(ckpt_now, "w");
+        if (!fc) die("cannot write %s", ckpt_now);
+        checkpoint_out(fc, &p, sim_steps, warm, run0, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, n, Q, V, U,
+                       spid);
+        fclose(fc);
+        ckpt_free(&ck);
+        return 0;
+    }
     mpcd_gpu_ctx *ctx = NULL;
     check(NULL, mpcd_gpu_create(pr, nrep, nr, (uint64_t)seed, &ctx), "mpcd_gpu_create");
     for (int r = 0; r < nrep; r++)
@@ -461,7 +563,8 @@ int main(int argc, char **argv) {
     if (!fo) die("cannot write %s", path);
     fprintf(fo, "# step  KBT(replica 0)  KE  Px  Py  Pz   (%s state, %d replica%s of %lld particles)\n",
             fp32 ? "FP32" : "FP64", nrep, nrep > 1 ? "s" : "", (long long)ntot);
-    const int total = sim_steps + 1;
+    /* steps of the temporal loop: runtime run0 .. simSteps */
+    const int total = sim_steps + 1 - run0 > 0 ? sim_steps + 1 - run0 : 0;
     if (every <= 0) every = total > 10 ? total / 10 : 1;
     if (warm > 0) check(ctx, mpcd_gpu_run(ctx, warm + 1, use_graph), "warm-up");
     /* field outputs; the flow sums start after the warm-up (mpc.c:4947:
@@ -501,9 +604,9 @@ int main(int argc, char **argv) {
     int next_out = every < total ? every : total;
     for (int done = 0; done < total;) {
         int k = next_out - done;
-        const int cad[3] = {flow_out, vel_out, do_dir ? dir_out : 0};
-        for (int q = 0; q < 3; q++)
-            if (cad[q] > 0 && steps_to_next(done, cad[q]) < k) k = steps_to_next(done, cad[q]);
+        const int cad[4] = {flow_out, vel_out, do_dir ? dir_out : 0, ckpt_out};
+        for (int q = 0; q < 4; q++)
+            if (cad[q] > 0 && steps_to_next(run0 + done, cad[q]) < k) k = steps_to_next(run0 + done, cad[q]);
         struct timespec a, b;
         clock_gettime(CLOCK_MONOTONIC, &a);
         check(ctx, mpcd_gpu_run(ctx, k, use_graph), "mpcd_gpu_run");
@@ -511,7 +614,7 @@ int main(int argc, char **argv) {
         clock_gettime(CLOCK_MONOTONIC, &b);
         busy += (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
         done += k;
-        const int rt = done - 1;
+        const int rt = run0 + done - 1;
         const double tnow = (double)rt * p.dt;
         if (flow_out > 0 && rt % flow_out == 0) {
             /* flowout at this runtime (pout.c:2690), then the sums restart */
@@ -529,6 +632,18 @@ int main(int argc, char **argv) {
                 if (vel_due) vel_write(fv[r], cvcm, p.L, dim, tnow);
                 if (dir_due) order_write(fd[r], cpop, cdir, cS, p.L, dim, tnow);
             }
+        if (ckpt_out > 0 && rt % ckpt_out == 0) {
+            /* runCheckpoint after the outputs of this runtime (mpcd.c:310-312): DIR/checkpoint.dat, replica 0;
+             * the warm-up count is 0 once its loop has run (mpcd.c:278) */
+            check(ctx, mpcd_gpu_download(ctx, 0, Q, V, p.lc ? U : NULL), "mpcd_gpu_download");
+            char cpath[4096];
+            snprintf(cpath, sizeof cpath, "%s/checkpoint.dat", outdir);
+            FILE *fc = fopen(cpath, "w");
+            if (!fc) die("cannot write %s", cpath);
+            checkpoint_out(fc, &p, sim_steps, 0, rt, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, n, Q, V, U,
+                           spid);
+            fclose(fc);
+        }
         if (done < next_out) continue;
         next_out = (next_out + every < total) ? next_out + every : total;
         check(ctx, mpcd_gpu_download(ctx, 0, Q, V, NULL), "mpcd_gpu_download");
@@ -542,8 +657,7 @@ int main(int argc, char **argv) {
             }
             ke += m * v2;
         }
-        fprintf(fo, "%d %.12g %.12g %.6e %.6e %.6e\n", done - 1, ke / (dim * (double)n), 0.5 * ke, P[0], P[1],
-                P[2]);
+        fprintf(fo, "%d %.12g %.12g %.6e %.6e %.6e\n", rt, ke / (dim * (double)n), 0.5 * ke, P[0], P[1], P[2]);
     }
     clock_gettime(CLOCK_MONOTONIC, &t1);
     fclose(fo);
@@ -552,6 +666,7 @@ int main(int argc, char **argv) {
         if (fv[r]) fclose(fv[r]);
         if (fd[r]) fclose(fd[r]);
     }
+    ckpt_free(&ck);
     free(flow);
     free(cpop);
     free(cvcm);

@@ -13,7 +13,7 @@
  * the reference's timestep() (mpc.c:4496) once per step with every output
  * switched off.  Random draws come from ref_rand.c (sequential streams).
  *
- * Usage:  ref_harness CASE.txt STATE.bin OUT.bin [timing | flow]
+ * Usage:  ref_harness CASE.txt STATE.bin OUT.bin [timing | flow | ckpt]
  *
  * CASE.txt is a list of "key value..." lines (see tests/golden/make_golden.py).
  * STATE.bin holds N records of 9 doubles (Q, V, U) followed by N int32 SPIDs.
@@ -28,6 +28,9 @@
  * reference's flow output is switched on (outFlags.FLOWOUT = OUT, so every
  * timestep() ends with localFLOW + sumFLOW, mpc.c:4946-4951), and after the
  * last step OUT.bin receives int32 tag -2 and ncell*3 doubles: FLOW per cell.
+ * With "ckpt" the reference's own checkpoint() (pout.c:2325-2404) writes
+ * OUT.bin.ckpt after the last step (runtime = the last step, warmtime 0,
+ * simSteps and warmupSteps as given by the case's "simsteps"/"warmup" keys).
  */
 #define _POSIX_C_SOURCE 199309L
 #include <math.h>
@@ -45,6 +48,7 @@
 #include "read.h"
 #include "therm.h"
 #include "lc.h"
+#include "pout.h"
 
 void ref_rand_seed(uint64_t seed);
 void ref_rand_counts(uint64_t out[3]);
@@ -54,7 +58,7 @@ void globalDensities(cell ***CL, spec SP[]);
 #define MAXW 16
 
 typedef struct {
-    int dim, L[3], N, steps, rtech, lc, tstech, galinv, nspec, nwall;
+    int dim, L[3], N, steps, rtech, lc, tstech, galinv, nspec, nwall, simsteps, warmup;
     double dt, kbt, rotang, tau, grav[3], avv[3];
     unsigned long long seed;
     double spv[MAXSPECI][12];
@@ -79,6 +83,8 @@ static void read_case(const char *path, hcase *c)
         else if (!strcmp(key, "L")) { if (fscanf(f, "%d %d %d", &c->L[0], &c->L[1], &c->L[2]) != 3) die("L"); }
         else if (!strcmp(key, "N")) { if (fscanf(f, "%d", &c->N) != 1) die("N"); }
         else if (!strcmp(key, "steps")) { if (fscanf(f, "%d", &c->steps) != 1) die("steps"); }
+        else if (!strcmp(key, "simsteps")) { if (fscanf(f, "%d", &c->simsteps) != 1) die("simsteps"); }
+        else if (!strcmp(key, "warmup")) { if (fscanf(f, "%d", &c->warmup) != 1) die("warmup"); }
         else if (!strcmp(key, "rtech")) { if (fscanf(f, "%d", &c->rtech) != 1) die("rtech"); }
         else if (!strcmp(key, "lc")) { if (fscanf(f, "%d", &c->lc) != 1) die("lc"); }
         else if (!strcmp(key, "tstech")) { if (fscanf(f, "%d", &c->tstech) != 1) die("tstech"); }
@@ -161,11 +167,12 @@ int main(int argc, char **argv)
     outputFilesList ofl;
     specSwimmer SS;
     double AVNOW[3] = {0, 0, 0}, AVV[3], avDIR[3] = {0, 0, 0}, KBTNOW = 0.0, AVS = 0.0;
-    int timing, flow;
+    int timing, flow, ckpt;
 
     if (argc < 4) die("usage: ref_harness CASE STATE OUT [timing]");
     timing = (argc > 4 && !strcmp(argv[4], "timing"));
     flow = (argc > 4 && !strcmp(argv[4], "flow"));
+    ckpt = (argc > 4 && !strcmp(argv[4], "ckpt"));
     read_case(argv[1], &hc);
 
     DBUG = -1; /* below DBGRUN: silence the reference's diagnostic printing */
@@ -306,6 +313,22 @@ int main(int argc, char **argv)
             double sec = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
             printf("{\"steps\": %d, \"particles\": %d, \"seconds\": %.6f}\n", hc.steps, GPOP, sec);
         }
+    }
+    if (ckpt) {
+        char path[4096];
+        FILE *fc;
+        kinTheory *th = (kinTheory *)calloc(NSPECI, sizeof(kinTheory)), thg;
+        memset(&thg, 0, sizeof thg);
+        snprintf(path, sizeof path, "%s.ckpt", argv[3]);
+        fc = fopen(path, "w");
+        if (!fc) die("cannot open ckpt");
+        in.simSteps = hc.simsteps;
+        in.warmupSteps = hc.warmup;
+        in.seed = hc.seed;
+        checkpoint(fc, in, SP, pp, 0, WALL, of, hc.steps - 1, 0, 0.0, 0.0, avDIR, 0.0, 0.0, KBTNOW, AVV, AVNOW, th, thg,
+                   SS, NULL);
+        fclose(fc);
+        free(th);
     }
     if (o && flow) {
         int a, b, k, tag = -2;

@@ -14,6 +14,9 @@ case of cases.py through it, and stores what the reference computed:
     draws     [steps, 3]     cumulative (uniform, gauss, unit-vector) draws consumed
     S, DIR, VCM              per-cell values after every step
 
+  tests/golden/ckpt_<case>.dat: the reference's own checkpoint file
+    (checkpoint(), pout.c:2325-2404) after the case's steps, for CKPT_CASES,
+    with simSteps 50 and warmupSteps 5
   tests/golden/flow.npz: FLOW_<case> [nc, 3], the reference's running flow
     sum (localFLOW + sumFLOW, mpc.c:226-246) after the case's steps, from a
     second run of the same case with flow output on (compact cell layout)
@@ -151,6 +154,21 @@ def record_flow(c):
     return recs[-1]["FLOW"]
 
 
+CKPT_CASES = ["rat3d_channel", "lc2d_channel", "rat2d_2sp"]
+
+
+def record_ckpt(c):
+    """the reference's checkpoint file after the case's steps"""
+    Q, V, U, spid, avv = C.initial_state(c)
+    with tempfile.TemporaryDirectory() as d:
+        cp, sp_ = write_case(c, Q, V, U, spid, avv, d)
+        with open(cp, "a") as f:
+            f.write("simsteps 50\nwarmup 5\n")
+        out = os.path.join(d, "out.bin")
+        subprocess.run([pyoracle.REF_HARNESS, cp, sp_, out, "ckpt"], check=True)
+        return open(out + ".ckpt").read()
+
+
 def main():
     pyoracle.build_ref()
     names = sys.argv[1:]
@@ -159,6 +177,12 @@ def main():
             continue
         N = record(c, HERE)
         print(f"{c['name']}: N={N} steps={c['steps']}")
+    if not names or "ckpt" in names:
+        for name in CKPT_CASES:
+            c = next(x for x in C.CASES if x["name"] == name)
+            with open(os.path.join(HERE, f"ckpt_{name}.dat"), "w") as f:
+                f.write(record_ckpt(c))
+        print("ckpt: " + ", ".join(CKPT_CASES))
     if not names or "flow" in names:
         np.savez_compressed(os.path.join(HERE, "flow.npz"), **{"FLOW_" + c["name"]: record_flow(c) for c in C.CASES})
         print("flow: FLOW of every case")

@@ -97,7 +97,7 @@ def test_rtech_alone(tmp_path):
 
 @pytest.mark.parametrize("cfg,msg", [
     (dict(domain=[8, 8], mdIn="md.inp"), "mdIn"),
-    (dict(domain=[8, 8], checkpointIn="c.dat"), "checkpointIn"),
+    (dict(domain=[8, 8], checkpointIn="c.dat"), "cannot open the checkpoint"),
     (dict(domain=[8, 8], BC=[dict(aInv=[0.25, 0.25, 0], Q=[4, 4, 0], P=[2, 2, 1, 1], R=4)]), "not an axis-aligned"),
     (dict(domain=[8, 8], BC=[dict(aInv=[1, 0, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], wavy=[0.5, 1, 0])]),
      "not an axis-aligned"),
@@ -282,3 +282,131 @@ def test_host_velocity_and_director_fields(tmp_path):
     np.testing.assert_allclose(np.linalg.norm(dirs[occ, 4:7], axis=1), 1.0, atol=1e-4)
     assert np.all(dirs[:, 6] == 0.0)
     assert np.all((dirs[occ, 7] >= 0.0) & (dirs[occ, 7] <= 1.0))
+
+
+# --- checkpoints (checkpoint(), pout.c:2325-2404; readchckpnt(), read.c:669) ---------------------------------------
+
+CKPT_CASES = ["rat3d_channel", "lc2d_channel", "rat2d_2sp"]
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def ckpt_fields(text):
+    """(name, token) pairs of a checkpoint file, in the order checkpoint() writes them"""
+    tok = iter(text.split())
+    out = []
+
+    def take(*names):
+        for nm in names:
+            out.append((nm, next(tok)))
+
+    take("simSteps", "warmupSteps", "dt", "seed", "DIM", "X", "Y", "Z", "KBT", "KBTNOW", "RFRAME", "zeroNetMom",
+         "GALINV", "TSTECH", "RTECH", "LC", "TAU", "RA", "FRICCO", "noHI", "inCOMP", "MULTIPHASE", "MFPLAYERH",
+         "G0", "G1", "G2", "MAG0", "MAG1", "MAG2", "MD_mode", "stepsMD", "GPOP", "NSPECI", "runtime", "warmtime", "C",
+         "S", "GRAV_FLAG", "MAG_FLAG", "tolD", "AVVEL", "AVS", "avDIR0", "avDIR1", "avDIR2", "S4", "stdN", "VOL",
+         "AVV0", "AVV1", "AVV2", "AVNOW0", "AVNOW1", "AVNOW2", *[f"theory{k}" for k in range(6)], "chckpntIn",
+         "MDmode", *[f"flag{k}" for k in range(32)], *[f"more{k}" for k in range(17)])
+    ns = int(dict(out)["NSPECI"])
+    for s in range(ns):
+        take(*[f"sp{s}.{k}" for k in ("MASS", "POP", "QDIST", "VDIST", "ODIST", "RFC", "LEN", "TUMBLE", "CHIHI", "CHIA",
+                                       "ACT", "MFPOT", "SIGWIDTH", "SIGPOS", "DAMP", "VOL", "nDNST", "mDNST",
+                                       "MINACTRATIO", "BS")], *[f"sp{s}.M{j}" for j in range(ns)],
+             *[f"sp{s}.theory{k}" for k in range(6)])
+    take("GnDNST", "GmDNST", "GMASS", "maxXYZ", "NBC")
+    for w in range(int(dict(out)["NBC"])):
+        take(*[f"w{w}.{k}" for k in ("COLL_TYPE", "PHANTOM", "E", "Q0", "Q1", "Q2", "V0", "V1", "V2", "O0", "O1", "O2",
+                                       "L0", "L1", "L2", "G0", "G1", "G2", "A0", "A1", "A2", "AINV0", "AINV1", "AINV2",
+                                       "P0", "P1", "P2", "P3", "R", "B0", "B1", "B2", "DN", "DT", "DVN", "DVT", "DVx",
+                                       "DVy", "DVz", "MVN", "MVT", "MUN", "MUT", "MUx", "MUy", "MUz", "DUx", "DUy",
+                                       "DUz", "KBT", "DSPLC", "INV", "MASS")], *[f"w{w}.hist{k}" for k in range(28)],
+             *[f"w{w}.INTER{j}" for j in range(12)])
+    for i in range(int(dict(out)["GPOP"])):
+        take(*[f"p{i}.{k}" for k in ("S_flag", "SPID", "q", "Q0", "Q1", "Q2", "V0", "V1", "V2", "U0", "U1", "U2",
+                                       "T0", "T1", "T2")])
+    take("NS", *[f"swimmer{k}" for k in range(9 + 19)])
+    assert next(tok, None) is None, "tokens after the swimmer records"
+    return out
+
+
+# fields this path does not carry (the reference fills them from state it alone has)
+NOT_CARRIED = ("warmupSteps", "KBTNOW", "AVNOW", "VOL", "AVVEL", "AVS", "avDIR", "S4", "stdN", "theory", "flag",
+               ".POP", ".LEN", ".CHIA", ".QDIST", ".ODIST", ".VOL", ".nDNST", ".mDNST", ".BS", ".M", "GmDNST", "GMASS",
+               "maxXYZ", ".hist", ".S_flag", ".q", ".T0", ".T1", ".T2")
+
+
+@pytest.mark.parametrize("name", CKPT_CASES)
+def test_checkpoint_layout_and_values(tmp_path, name):
+    """The reference's own checkpoint file (tests/golden/ckpt_<case>.dat) is
+    read, and written back by --write-checkpoint: the two files hold the same
+    fields in the same order, and every field this path carries (inputs,
+    species, boundaries, particles) is the same token."""
+    ref = open(os.path.join(GOLDEN, f"ckpt_{name}.dat")).read()
+    f = tmp_path / "input.json"
+    f.write_text(json.dumps({"checkpointIn": os.path.join(GOLDEN, f"ckpt_{name}.dat")}))
+    out = tmp_path / "mine.dat"
+    r = subprocess.run([host(), str(f), "--write-checkpoint", str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    a, b = ckpt_fields(ref), ckpt_fields(open(out).read())
+    assert [k for k, _ in a] == [k for k, _ in b]
+    diff = [(k, x, y) for (k, x), (_, y) in zip(a, b) if x != y and not any(s in k for s in NOT_CARRIED)]
+    assert not diff, diff[:10]
+
+
+@pytest.mark.parametrize("name", CKPT_CASES)
+def test_checkpoint_read(tmp_path, name):
+    """The parameters and particles the host takes from the reference's
+    checkpoint are the recorded case's (to the file's six decimals)."""
+    g = U.load_golden(name)
+    c = g["case_d"]
+    d = dump(tmp_path, dict(checkpointIn=os.path.join(GOLDEN, f"ckpt_{name}.dat")))
+    assert (d["dim"], d["rtech"], d["lc"], d["tstech"], d["galinv"]) == (c["dim"], c["rtech"], c["lc"], c["tstech"],
+                                                                         c["galinv"])
+    assert d["L"][:c["dim"]] == c["L"][:c["dim"]]
+    assert d["simSteps"] == 50 and d["warmUp"] == 0 and d["runtime"] == c["steps"] - 1
+    for k in ("dt", "kbt", "rotang", "tau"):
+        assert abs(d[k] - c[k]) < 1e-6, k
+    assert np.allclose(d["grav"], c["grav"], atol=1e-6) and np.allclose(d["avv"], g["avv"], atol=1e-6)
+    assert abs(d["gndnst"] - float(g["gndnst"])) < 1e-6
+    pops = np.bincount(g["spid"], minlength=len(c["species"]))
+    assert [s["pop"] for s in d["species"]] == list(pops)
+    for s, sp in zip(d["species"], c["species"]):
+        for k in ("mass", "rfc", "tumble", "chihi", "act", "mfpot", "damp", "sigwidth", "sigpos", "minactratio"):
+            assert abs(s[k] - sp[k]) < 1e-6, k
+    assert len(d["walls"]) == len(c["walls"])
+    for w, cw in zip(d["walls"], c["walls"]):
+        assert np.allclose(w["A"], cw["A"]) and np.allclose(w["Q"], cw.get("Q", (0, 0, 0)))
+        for k, dflt in (("R", 0.0), ("DN", 0.0), ("MVN", 1.0), ("MVT", 1.0), ("MUN", 1.0), ("MUT", 1.0)):
+            assert abs(w[k] - cw.get(k, dflt)) < 1e-6, k
+        assert w["PHANTOM"] == cw.get("PHANTOM", 0) and w["INV"] == cw.get("INV", 0)
+        assert w["inter_mask"] == (1 << len(c["species"])) - 1
+
+
+@pytest.mark.gpu
+def test_host_checkpoint_and_resume(tmp_path):
+    """checkpointOut writes DIR/checkpoint.dat after the outputs of every
+    runtime divisible by it (mpcd.c:310-312).  A run recovered from it skips
+    the warm-up and starts its temporal loop at the checkpoint's runtime
+    (mpcd.c:257, 290), so that runtime's step runs again.  In a periodic box
+    SRD keeps the momentum the body force gives, N m g dt per step from zero:
+    after the resume it counts 5 warm-up steps, 51 steps to runtime 50 and
+    the resumed ones (to the file's six decimals per velocity)."""
+    g, dt, N = 0.001, 0.1, 20 * 12 * 10
+    cfg = {"domain": [20, 12], "simSteps": 59, "warmUp": 4, "collOp": 0, "seed": 3, "checkpointOut": 25, "dt": dt,
+           "grav": [g, 0, 0], "species": [{"dens": 10, "vDist": 4}]}
+    print("[host] periodic 2D box 20x12x10, checkpointOut 25, then a resume", flush=True)
+    run_host(tmp_path, cfg, "--every", "20")
+    ck = tmp_path / "out" / "checkpoint.dat"
+    f = dict(ckpt_fields(open(ck).read()))
+    assert (int(f["runtime"]), int(f["warmtime"]), int(f["simSteps"]), int(f["warmupSteps"])) == (50, 5, 59, 0)
+    assert int(f["flag29"]) == 25 and int(f["GPOP"]) == N
+    res = tmp_path / "resume"
+    res.mkdir()
+    info, rows = run_host(res, {"checkpointIn": str(ck)}, "--every", "5")
+    assert info["steps"] == 10
+    assert [int(x[0]) for x in rows] == [54, 59]
+    for rt, kbt, ke, px, py, pz in rows:
+        expect = N * g * dt * (5 + 51 + (rt - 50 + 1))
+        assert abs(px - expect) < 2e-3, (rt, px, expect)
+        assert 0.9 < kbt < 1.1
+    # the resumed run wrote its own checkpoint at runtime 50 again (the flag came from the file)
+    f2 = dict(ckpt_fields(open(res / "out" / "checkpoint.dat").read()))
+    assert int(f2["runtime"]) == 50
