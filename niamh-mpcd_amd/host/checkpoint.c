@@ -88,6 +88,7 @@ int ckpt_read(const char *path, mpcd_gpu_params *p, mpcd_ckpt *ck, char *err, si
     rdn(&R, v, 32);  /* output flags, in checkpoint()'s order */
     ck->flow_out = (int)v[4];
     ck->vel_out = (int)v[6];
+    ck->dens_out = (int)v[7];
     ck->dir_out = (int)v[10];
     ck->ckpt_out = (int)v[29];
     skip(&R, 2 + 7 + 5 + 3);  /* SPOUT PRESOUT, histograms, spectra/topology, swimmer outputs */
@@ -208,6 +209,7 @@ void ckpt_write(FILE *f, const mpcd_gpu_params *p, const mpcd_ckpt *ck) {
     fl[0] = -1;
     fl[4] = ck->flow_out;
     fl[6] = ck->vel_out;
+    fl[7] = ck->dens_out;
     fl[10] = ck->dir_out;
     fl[29] = ck->ckpt_out;
     for (int k = 0; k < 31; k++) fprintf(f, "%d ", fl[k]);

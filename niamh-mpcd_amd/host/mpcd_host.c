@@ -38,6 +38,10 @@
  * which the sum starts again.
  * velOut = F > 0: DIR/velfield.dat, every cell's centre-of-mass velocity at
  * every runtime divisible by F (velout, pout.c:1715-1731, 2691).
+ * densOut = F > 0: DIR/densityfield.dat, every cell's population, mass and
+ * population per species (densityout, pout.c:1744-1757, 2692), counted on
+ * the host from the particles (the cell of a particle is (int)Q, as bin()
+ * finds it, mpc.c:958-960).
  * dirSOut = F > 0 with lc: DIR/directorfield.dat, every cell's director and
  * scalar order parameter, zeros in a cell without particles (orderout,
  * pout.c:2021-2033, 2695).
@@ -134,6 +138,25 @@ static void order_write(FILE *f, const int32_t *pop, const double *dir, const do
             }
 }
 
+/* densityout's lines (pout.c:1744-1757): population, mass, MPCD / MD /
+ * swimmer populations (MD and swimmers: none here) and the population of
+ * every species, per cell of the domain; counts and masses per compact cell */
+static void density_write(FILE *f, const int32_t *cnt, const double *mass, int nspec, const int L[3], int dim,
+                          double t) {
+    const int ny = L[1] + 1, nz = dim == 3 ? L[2] + 1 : 1, kz = dim == 3 ? L[2] : 1;
+    for (int i = 0; i < L[0]; i++)
+        for (int j = 0; j < L[1]; j++)
+            for (int k = 0; k < kz; k++) {
+                const size_t c = (size_t)(i * ny + j) * nz + k;
+                int pop = 0;
+                for (int s = 0; s < nspec; s++) pop += cnt[c * nspec + s];
+                fprintf(f, "%12.5e\t%5d\t%5d\t%5d\t", t, i, j, k);
+                fprintf(f, "%5d\t%12.5e\t%5d\t%5d\t%5d\t", pop, mass[c], pop, 0, 0);
+                for (int s = 0; s < nspec; s++) fprintf(f, "\t%5d", cnt[c * nspec + s]);
+                fprintf(f, "\n");
+            }
+}
+
 /* a field file of replica r: DIR/<name>.dat (r = 0) or DIR/<name>_r<r>.dat,
  * with 11 preamble lines, a blank line and the column names: readers of the
  * reference's files skip 13 lines */
@@ -168,7 +191,8 @@ static int steps_to_next(int done, int F) {
  * the current temperature and mean velocity are computed here, as KBTNOW
  * and AVNOW of the reference's file */
 static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int warm_steps, int runtime, int warmtime,
-                           long seed, int flow_out, int vel_out, int dir_out, int ckpt_out, size_t n, const double *Q,
+                           long seed, int flow_out, int vel_out, int dir_out, int ckpt_out, int dens_out, size_t n,
+                           const double *Q,
                            const double *V, const double *U, const int32_t *spid) {
     mpcd_ckpt ck;
     memset(&ck, 0, sizeof ck);
@@ -181,6 +205,7 @@ static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int
     ck.vel_out = vel_out;
     ck.dir_out = dir_out;
     ck.ckpt_out = ckpt_out;
+    ck.dens_out = dens_out;
     ck.n = (int64_t)n;
     ck.Q = (double *)Q;
     ck.V = (double *)V;
@@ -349,6 +374,7 @@ int main(int argc, char **argv) {
     int flow_out = json_int(J, "flowOut", 0);
     int vel_out = json_int(J, "velOut", 0);
     int dir_out = json_int(J, "dirSOut", 0);
+    int dens_out = json_int(J, "densOut", 0);
     int ckpt_out = json_int(J, "checkpointOut", 0);
     p.galinv = json_int(J, "galInv", 1);
     p.tstech = json_int(J, "tsTech", 0);
@@ -432,6 +458,7 @@ int main(int argc, char **argv) {
         vel_out = ck.vel_out;
         dir_out = ck.dir_out;
         ckpt_out = ck.ckpt_out;
+        dens_out = ck.dens_out;
         if (seed_override < 0 && ck.seed != 0) seed = ck.seed;
         ntot = ck.n;
         for (int s = 0; s < MPCD_MAXSPEC; s++) pop[s] = 0;
@@ -444,10 +471,10 @@ int main(int argc, char **argv) {
         printf("{\"dim\": %d, \"L\": [%d, %d, %d], \"dt\": %.17g, \"kbt\": %.17g, \"rotang\": %.17g, "
                "\"tau\": %.17g, \"grav\": [%.17g, %.17g, %.17g], \"rtech\": %d, \"lc\": %d, \"tstech\": %d, "
                "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"flowOut\": %d, \"velOut\": %d, \"dirSOut\": %d, "
-               "\"checkpointOut\": %d, \"runtime\": %d, \"gndnst\": %.17g, \"avv\": [%.17g, %.17g, %.17g], "
+               "\"densOut\": %d, \"checkpointOut\": %d, \"runtime\": %d, \"gndnst\": %.17g, \"avv\": [%.17g, %.17g, %.17g], "
                "\"seed\": %ld, \"species\": [",
                p.dim, p.L[0], p.L[1], p.L[2], p.dt, p.kbt, p.rotang, p.tau, p.grav[0], p.grav[1], p.grav[2], p.rtech,
-               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, vel_out, dir_out, ckpt_out, run0, p.gndnst, p.avv[0],
+               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, vel_out, dir_out, dens_out, ckpt_out, run0, p.gndnst, p.avv[0],
                p.avv[1], p.avv[2], seed);
         for (int s = 0; s < p.nspec; s++) {
             const mpcd_gpu_species *sp = &p.sp[s];
@@ -542,7 +569,7 @@ int main(int argc, char **argv) {
         /* the state the run would start from, as a checkpoint; no GPU */
         FILE *fc = fopen(ckpt_now, "w");
         if (!fc) die("cannot write %s", ckpt_now);
-        checkpoint_out(fc, &p, sim_steps, warm, run0, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, n, Q, V, U,
+        checkpoint_out(fc, &p, sim_steps, warm, run0, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, n, Q, V, U,
                        spid);
         fclose(fc);
         ckpt_free(&ck);
@@ -570,11 +597,26 @@ int main(int argc, char **argv) {
     /* field outputs; the flow sums start after the warm-up (mpc.c:4947:
      * !in.warmupSteps); orderout needs LC (pout.c:2695) */
     const int do_dir = p.lc && dir_out > 0;
-    FILE *ff[64] = {NULL}, *fv[64] = {NULL}, *fd[64] = {NULL};
+    FILE *ff[64] = {NULL}, *fv[64] = {NULL}, *fd[64] = {NULL}, *fn[64] = {NULL};
     const size_t nc = (size_t)mpcd_gpu_ncells(ctx, 0);
     double *flow = NULL, *cvcm = NULL, *cS = NULL, *cdir = NULL;
     int32_t *cpop = NULL;
-    if ((flow_out > 0 || vel_out > 0 || do_dir) && nrep > 64) die("%s", "field outputs: at most 64 replicas");
+    if ((flow_out > 0 || vel_out > 0 || do_dir || dens_out > 0) && nrep > 64)
+        die("%s", "field outputs: at most 64 replicas");
+    int32_t *dcnt = NULL;
+    double *dmass = NULL;
+    if (dens_out > 0) {
+        dcnt = (int32_t *)malloc(sizeof(int32_t) * nc * p.nspec);
+        dmass = (double *)malloc(sizeof(double) * nc);
+        if (!dcnt || !dmass) die("%s", "out of host memory");
+        char cols[512];
+        int off = snprintf(cols, sizeof cols, "   t\t   QX\t   QY\t   QZ\tpop\t\tmass\t\tpopSRD\t\tpopMD\t\tpopSW");
+        for (int s = 0; s < p.nspec && off < (int)sizeof cols; s++)
+            off += snprintf(cols + off, sizeof cols - off, "\t\tSP%d", s);
+        for (int r = 0; r < nrep; r++)
+            fn[r] = field_open(outdir, "densityfield", "density field: population and mass per cell and species", cols, r,
+                               nrep, dens_out, p.L, dim, fp32);
+    }
     if (flow_out > 0) {
         check(ctx, mpcd_gpu_flow(ctx, 1), "mpcd_gpu_flow");
         flow = (double *)malloc(sizeof(double) * 3 * nc);
@@ -604,8 +646,8 @@ int main(int argc, char **argv) {
     int next_out = every < total ? every : total;
     for (int done = 0; done < total;) {
         int k = next_out - done;
-        const int cad[4] = {flow_out, vel_out, do_dir ? dir_out : 0, ckpt_out};
-        for (int q = 0; q < 4; q++)
+        const int cad[5] = {flow_out, vel_out, do_dir ? dir_out : 0, ckpt_out, dens_out};
+        for (int q = 0; q < 5; q++)
             if (cad[q] > 0 && steps_to_next(run0 + done, cad[q]) < k) k = steps_to_next(run0 + done, cad[q]);
         struct timespec a, b;
         clock_gettime(CLOCK_MONOTONIC, &a);
@@ -632,6 +674,23 @@ int main(int argc, char **argv) {
                 if (vel_due) vel_write(fv[r], cvcm, p.L, dim, tnow);
                 if (dir_due) order_write(fd[r], cpop, cdir, cS, p.L, dim, tnow);
             }
+        if (dens_out > 0 && rt % dens_out == 0)
+            for (int r = 0; r < nrep; r++) {
+                double *Qr = Q + 3 * n * r, *Vr = V + 3 * n * r;
+                check(ctx, mpcd_gpu_download(ctx, r, Qr, Vr, NULL), "mpcd_gpu_download");
+                memset(dcnt, 0, sizeof(int32_t) * nc * p.nspec);
+                memset(dmass, 0, sizeof(double) * nc);
+                const int ny = p.L[1] + 1, nz = dim == 3 ? p.L[2] + 1 : 1;
+                for (size_t i = 0; i < n; i++) {
+                    const int a = (int)Qr[3 * i], b = (int)Qr[3 * i + 1], cz = dim == 3 ? (int)Qr[3 * i + 2] : 0;
+                    if (a < 0 || b < 0 || cz < 0 || a > p.L[0] || b > p.L[1] || cz >= nz) continue;
+                    const size_t c = (size_t)(a * ny + b) * nz + cz;
+                    const int s = p.nspec > 1 ? spid[i] : 0;
+                    dcnt[c * p.nspec + s]++;
+                    dmass[c] += p.sp[s].mass;
+                }
+                density_write(fn[r], dcnt, dmass, p.nspec, p.L, dim, tnow);
+            }
         if (ckpt_out > 0 && rt % ckpt_out == 0) {
             /* runCheckpoint after the outputs of this runtime (mpcd.c:310-312): DIR/checkpoint.dat, replica 0;
              * the warm-up count is 0 once its loop has run (mpcd.c:278) */
@@ -640,7 +699,7 @@ int main(int argc, char **argv) {
             snprintf(cpath, sizeof cpath, "%s/checkpoint.dat", outdir);
             FILE *fc = fopen(cpath, "w");
             if (!fc) die("cannot write %s", cpath);
-            checkpoint_out(fc, &p, sim_steps, 0, rt, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, n, Q, V, U,
+            checkpoint_out(fc, &p, sim_steps, 0, rt, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, n, Q, V, U,
                            spid);
             fclose(fc);
         }
@@ -665,7 +724,10 @@ int main(int argc, char **argv) {
         if (ff[r]) fclose(ff[r]);
         if (fv[r]) fclose(fv[r]);
         if (fd[r]) fclose(fd[r]);
+        if (fn[r]) fclose(fn[r]);
     }
+    free(dcnt);
+    free(dmass);
     ckpt_free(&ck);
     free(flow);
     free(cpop);

@@ -49,7 +49,7 @@ def test_defaults_and_duplicate_keys(tmp_path):
     assert d["dim"] == 3 and d["L"] == [16, 12, 8]
     assert (d["kbt"], d["dt"], d["tau"], d["rotang"]) == (1.0, 0.1, 0.5, 1.570796)
     assert (d["rtech"], d["lc"], d["tstech"], d["galinv"], d["simSteps"], d["warmUp"]) == (2, 0, 0, 1, 2000, 0)
-    assert (d["flowOut"], d["velOut"], d["dirSOut"]) == (0, 0, 0)  # read.c:1732, 1735, 1738
+    assert (d["flowOut"], d["velOut"], d["dirSOut"], d["densOut"]) == (0, 0, 0, 0)  # read.c:1732-1738
     sp = d["species"][0]
     assert sp["pop"] == 1000 and sp["mass"] == 1.0 and sp["vdist"] == 0
     assert (sp["rfc"], sp["tumble"], sp["chihi"], sp["act"], sp["mfpot"]) == (0.01, 2.0, 0.5, 0.05, 10.0)
@@ -410,3 +410,25 @@ def test_host_checkpoint_and_resume(tmp_path):
     # the resumed run wrote its own checkpoint at runtime 50 again (the flag came from the file)
     f2 = dict(ckpt_fields(open(res / "out" / "checkpoint.dat").read()))
     assert int(f2["runtime"]) == 50
+
+
+@pytest.mark.gpu
+def test_host_density_field(tmp_path):
+    """densOut: per cell the population, the mass and the population of each
+    species (densityout, pout.c:1744-1757); every block holds each species'
+    whole population, the mass column is the species-weighted sum and the
+    MPCD population is the population (no MD, no swimmers)."""
+    cfg = {"domain": [12, 10], "simSteps": 19, "rTech": 1, "seed": 6, "densOut": 10,
+           "species": [{"pop": 700, "mass": 1.0, "vDist": 4}, {"pop": 500, "mass": 2.5, "vDist": 4}]}
+    print("[host] 2D box 12x10, two species, densOut 10", flush=True)
+    run_host(tmp_path, cfg, "--every", "20")
+    lines = open(tmp_path / "out" / "densityfield.dat").read().splitlines()
+    assert lines[12].split() == ["t", "QX", "QY", "QZ", "pop", "mass", "popSRD", "popMD", "popSW", "SP0", "SP1"]
+    data = np.array([[float(x) for x in l.split()] for l in lines[13:]])
+    assert data.shape == (2 * 120, 11)
+    for blk, t in zip(data.reshape(2, 120, 11), (0.0, 1.0)):
+        assert np.allclose(blk[:, 0], t)
+        assert blk[:, 9].sum() == 700 and blk[:, 10].sum() == 500
+        assert np.array_equal(blk[:, 4], blk[:, 9] + blk[:, 10]) and np.array_equal(blk[:, 6], blk[:, 4])
+        assert np.allclose(blk[:, 5], blk[:, 9] * 1.0 + blk[:, 10] * 2.5, atol=1e-4)
+        assert not blk[:, 7].any() and not blk[:, 8].any()
