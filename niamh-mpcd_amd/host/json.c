@@ -268,13 +268,9 @@ json *json_parse(const char *text, char *err, size_t errlen) {
         free(j);
         return NULL;
     }
-    skip_ws(&p);
-    if (p.s[p.i] != '\0') {
-        fail(&p, "trailing characters");
-        json_free(j);
-        free(j);
-        return NULL;
-    }
+    /* text after the first value is ignored, as cJSON_Parse does
+     * (cJSON_ParseWithOpts with require_null_terminated = 0; the reference's
+     * readJson, read.c:993) */
     return j;
 }
 

@@ -113,6 +113,15 @@ def test_malformed_json(tmp_path):
     assert r.returncode != 0 and "JSON line 1" in r.stderr
 
 
+def test_text_after_the_first_object_is_ignored(tmp_path):
+    """cJSON_Parse (the reference's readJson, read.c:993) reads the first
+    value and ignores what follows it; so does the host: a second object
+    after the first changes nothing."""
+    text = '{"domain": [8, 6], "kbt": 2.0}\n\n{"checkpointIn": "./checkpoint.dat", "kbt": 3.0}\n'
+    d = dump(tmp_path, text=text)
+    assert d["L"][:2] == [8, 6] and d["kbt"] == 2.0 and d["runtime"] == 0
+
+
 def run_host(tmp_path, cfg, *args):
     f = tmp_path / "input.json"
     f.write_text(json.dumps(cfg))
