@@ -238,6 +238,11 @@ static void reject_unsupported(const json *J) {
     const json *sw = json_get(J, "swimmers");
     if (!sw) sw = json_get(J, "swimmer");
     if (sw && sw->type == JSON_ARR && sw->n > 0) die("%s are not supported", "swimmers");
+    /* swimmers (read.c:1781) and periodic zero-net-momentum resets (rFrame
+     * defaults to 1, read.c:1062, but resets only every zeroNetMom > 0 steps) */
+    if (json_int(J, "nSwim", 0) > 0) die("%s", "\"nSwim\": swimmers are outside the solvent path");
+    if (json_int(J, "rFrame", 1) != 0 && json_int(J, "zeroNetMom", 0) > 0)
+        die("%s", "\"zeroNetMom\": periodic zero-net-momentum resets are not supported");
 }
 
 /* BC entry of the JSON (read.c:1136-1440) as a planar wall */
@@ -426,15 +431,22 @@ int main(int argc, char **argv) {
     }
     if (ntot <= 0 && !ckpt_in[0]) die("%s", "no particles");
 
-    /* boundaries: the BC list, or the domainWalls override */
+    /* boundaries: the BC list, then the domain walls of domainWalls = 1
+     * (periodic) or 2 (solid) appended after it; without a BC list the
+     * domain walls are periodic unless domainWalls = 2 (read.c:1136-1441) */
     const json *bca = json_get(J, "BC");
-    const int dwalls = json_int(J, "domainWalls", 0);
-    if (bca && bca->type == JSON_ARR && bca->n > 0 && dwalls == 0) {
+    int dwalls = json_int(J, "domainWalls", 0);
+    p.nwall = 0;
+    if (bca && bca->type == JSON_ARR) {
         if (bca->n > MPCD_MAXWALL) die("%s", "too many BC entries");
         for (int w = 0; w < bca->n; w++) read_wall(&bca->items[w], p.nspec, &p.wall[w], w);
         p.nwall = bca->n;
-    } else {
-        p.nwall = domain_walls(dim, p.L, dwalls == 2 ? 2 : 1, p.nspec, p.wall);
+    } else if (dwalls != 2) {
+        dwalls = 1;
+    }
+    if (dwalls == 1 || dwalls == 2) {
+        if (p.nwall + 2 * dim > MPCD_MAXWALL) die("%s", "too many BC entries with the domain walls");
+        p.nwall += domain_walls(dim, p.L, dwalls, p.nspec, p.wall + p.nwall);
     }
 
     /* recovering a checkpoint: its inputs, particles, loop counters and

@@ -90,6 +90,20 @@ def test_bc_entries(tmp_path):
     assert dump(tmp_path, dict(domain=[9, 9], BC=bc))["walls"][1]["MUxyz"] == [0.5, -1.0, 1.0]
 
 
+def test_domain_walls_follow_the_bc_list(tmp_path):
+    """domainWalls = 1 appends the periodic domain walls after the BC entries
+    (read.c:1441-1520): a solid wall pair in y plus four periodic walls"""
+    bc = [dict(aInv=[0, 1, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], R=0, DN=0, MVN=-1, MVT=-1),
+          dict(aInv=[0, -1, 0], Q=[0, 6, 0], P=[1, 1, 1, 1], R=0, DN=0, MVN=-1, MVT=-1)]
+    d = dump(tmp_path, dict(domain=[10, 6], BC=bc, domainWalls=1))
+    assert len(d["walls"]) == 6
+    assert d["walls"][0]["A"] == [0.0, 1.0, 0.0] and d["walls"][0]["MVN"] == -1.0
+    assert [w["DN"] for w in d["walls"][2:]] == [10.0, 10.0, 6.0, 6.0]
+    # a BC list alone adds no domain walls; no BC list gives periodic ones
+    assert len(dump(tmp_path, dict(domain=[10, 6], BC=bc))["walls"]) == 2
+    assert len(dump(tmp_path, dict(domain=[10, 6]))["walls"]) == 4
+
+
 def test_rtech_alone(tmp_path):
     assert dump(tmp_path, dict(domain=[8, 8], rTech=1))["rtech"] == 1
     assert dump(tmp_path, dict(domain=[8, 8], collOp=0))["rtech"] == 0
@@ -98,6 +112,8 @@ def test_rtech_alone(tmp_path):
 @pytest.mark.parametrize("cfg,msg", [
     (dict(domain=[8, 8], mdIn="md.inp"), "mdIn"),
     (dict(domain=[8, 8], checkpointIn="c.dat"), "cannot open the checkpoint"),
+    (dict(domain=[8, 8], nSwim=2), "nSwim"),
+    (dict(domain=[8, 8], zeroNetMom=100), "zeroNetMom"),
     (dict(domain=[8, 8], BC=[dict(aInv=[0.25, 0.25, 0], Q=[4, 4, 0], P=[2, 2, 1, 1], R=4)]), "not an axis-aligned"),
     (dict(domain=[8, 8], BC=[dict(aInv=[1, 0, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], wavy=[0.5, 1, 0])]),
      "not an axis-aligned"),
