@@ -349,8 +349,19 @@ def lc_throughput(M, L=256, ppc=20, steps=40, warmup=5):
         ctx.upload(0, Q, V, U)
         r = timed_run(M, ctx, steps, warmup, N)
         ctx.close()
+        # the dominant kernel against HBM, with 2D algorithmic bytes per particle: the
+        # nematic collision reads r, v, u (2 components each) and the species and writes v;
+        # the orientation step reads and writes u and reads the species
+        pb = PREC[name]
+        lc_bytes = {"collide": 8 * pb + 4, "lc": 4 * pb + 4}
+        fam, (ms, cnt) = max(((k, v) for k, v in r["timing"].items() if k in lc_bytes), key=lambda kv: kv[1][0])
+        avg_s = ms / max(cnt, 1) * 1e-3
+        ach = lc_bytes[fam] * N / avg_s / 1e9
         out[name] = dict(value=r["value"], ms_per_step=r["ms_per_step"],
-                         kernels_ms_per_step={k: v[0] / steps for k, v in r["timing"].items() if v[1]})
+                         kernels_ms_per_step={k: v[0] / steps for k, v in r["timing"].items() if v[1]},
+                         roofline=dict(bound="hbm", kernel_family=fam, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                                       frac=ach / HBM_PEAK_GBS, avg_launch_ms=ms / max(cnt, 1),
+                                       alg_bytes_per_launch=lc_bytes[fam] * N))
     out["workload"] = (f"2D active nematic {L}x{L} x {ppc} ppc (lc=1, collision 16 DIPOLE_DIR_SUM, U=10, tumble 0.5, "
                        f"rfc 0.01, activity 0.05), {steps} timed steps after {warmup}; split path")
     out["unit"] = "particle-steps/s"
