@@ -478,7 +478,13 @@ def main():
         by_ppc = {}
         for x in e["per_replica"]:
             by_ppc.setdefault(x["ppc"], []).append(x["nu"])
-        ensemble = {k: e[k] for k in ("particles", "replicas", "value", "ms_per_step", "steps", "samples")}
+        ensemble = {k: e[k] for k in ("particles", "replicas", "value", "ms_per_step", "steps", "samples",
+                                      "kernels_ms_per_step")}
+        # the bucketed MPC-AT+a kernel against HBM: the same record moved as in the headline's bucketed step
+        ach = fam_bytes("bucket", "fp32") * e["particles"] / (e["bucket_avg_launch_ms"] * 1e-3) / 1e9
+        ensemble["roofline"] = dict(bound="hbm", kernel_family="bucket", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                                    frac=ach / HBM_PEAK_GBS, avg_launch_ms=e["bucket_avg_launch_ms"],
+                                    alg_bytes_per_launch=fam_bytes("bucket", "fp32") * e["particles"])
         ensemble.update(
             workload="64 replicas of 3D MPC-AT+a channels: L^3 with L ~ U{16..48}, ppc ~ U{5..20}, dt 0.05, kT 1, "
                      "bounce-back phantom walls in z, g = 1e-3 along x, FP32 state, autotuned tiles, hipGraph replays",

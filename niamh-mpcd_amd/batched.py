@@ -97,7 +97,13 @@ def run(M, precision, reps=None, steps=3000, sample_from=600, every=100, g=1e-3,
     ctx.run(time_steps, use_graph=True)
     ctx.synchronize()
     sec = time.perf_counter() - t0
-    done = 5 * (1 + trial) + 2 + time_steps  # steps so far: autotune trials, graph warm-up, timed stretch
+    # kernel times of a few eager steps (HIP events on the library stream)
+    ctx.profile(True)
+    ctx.run(4, use_graph=False)
+    ctx.synchronize()
+    kern = ctx.timing()
+    ctx.profile(False)
+    done = 5 * (1 + trial) + 2 + time_steps + 4  # autotune trials, graph warm-up, timed stretch, profiled steps
     prof = [[] for _ in reps]
     times = []
     nz = [L + 1 for L, _ in reps]
@@ -127,4 +133,6 @@ def run(M, precision, reps=None, steps=3000, sample_from=600, every=100, g=1e-3,
     ctx.close()
     N = int(sum(ns))
     return dict(particles=N, replicas=len(reps), value=N * time_steps / sec, ms_per_step=1e3 * sec / time_steps,
-                tile_ms=tile_ms, steps=steps, samples=samples, per_replica=out)
+                tile_ms=tile_ms, steps=steps, samples=samples, per_replica=out,
+                kernels_ms_per_step={k: v[0] / 4 for k, v in kern.items() if v[1]},
+                bucket_avg_launch_ms=kern["bucket"][0] / max(kern["bucket"][1], 1))

@@ -343,7 +343,9 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
     if (OP != OP_SRD) {
       // thermal velocities of the collision (mpc.c:1382, 1506), keyed by slot
       const double sig = sqrt(R.kbt / sps[rsp[j]].mass);
-      for (int d = 0; d < DIM; d++) s_rv[d][pos] = sig * Dr.part_n(T.rep, gbase + pos, d);
+      double nrm[3];
+      Dr.part_n_all(T.rep, gbase + pos, DIM, nrm);
+      for (int d = 0; d < DIM; d++) s_rv[d][pos] = sig * nrm[d];
     }
   }
   __syncthreads();

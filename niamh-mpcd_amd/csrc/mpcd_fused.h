@@ -537,7 +537,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
       if (OP != OP_SRD) {
         // thermal velocities of the collision (mpc.c:1382, 1506), keyed by slot
         const double sig = sqrt(R.kbt / sps[spid].mass);
-        for (int j = 0; j < DIM; j++) s_rv[j][p] = sig * Dr.part_n(T.rep, base + p, j);
+        double nrm[3];
+        Dr.part_n_all(T.rep, base + p, DIM, nrm);
+        for (int j = 0; j < DIM; j++) s_rv[j][p] = sig * nrm[j];
       }
     }
     __syncthreads();

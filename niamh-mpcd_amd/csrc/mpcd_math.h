@@ -256,6 +256,17 @@ struct Rng {
     double r = sqrt(-2.0 * log(u1));
     return (k & 1) ? r * sin(6.283185307179586 * u2) : r * cos(6.283185307179586 * u2);
   }
+  // the two normals of one block, normal(.., 2 * pair) and normal(.., 2 * pair + 1),
+  // by the same expressions: one counter block, one logarithm and one root for both
+  MPCD_HD void normal_pair(uint32_t rep, int purpose, uint32_t key, uint32_t pair, double out[2]) const {
+    uint32_t w[4];
+    block(rep, purpose, key, pair, w);
+    double u1 = 1.0 - u53(w[0], w[1]);
+    double u2 = u53(w[2], w[3]);
+    double r = sqrt(-2.0 * log(u1));
+    out[0] = r * cos(6.283185307179586 * u2);
+    out[1] = r * sin(6.283185307179586 * u2);
+  }
   MPCD_HD void unitvec(uint32_t rep, int purpose, uint32_t key, uint32_t k, double v[3]) const {
     uint32_t w[4];
     block(rep, purpose, key, k, w);
@@ -298,6 +309,19 @@ struct Draws {
   }
   MPCD_HD double part_n(uint32_t rep, uint32_t slot, int k) const {
     return injected ? part[(size_t)slot * 3 + k] : rng.normal(rep, P_NORMAL, slot, k);
+  }
+  // part_n(rep, slot, k) for k < dim, the normals of a block computed once
+  MPCD_HD void part_n_all(uint32_t rep, uint32_t slot, int dim, double out[3]) const {
+    if (injected) {
+      for (int k = 0; k < dim; k++) out[k] = part[(size_t)slot * 3 + k];
+      return;
+    }
+    rng.normal_pair(rep, P_NORMAL, slot, 0, out);
+    if (dim == 3) {
+      double t[2];
+      rng.normal_pair(rep, P_NORMAL, slot, 1, t);
+      out[2] = t[0];
+    }
   }
   MPCD_HD double lc_u(uint32_t rep, uint32_t slot, int k) const {
     return injected ? lc[(size_t)slot * lc_stride + k] : rng.uniform(rep, P_LC, slot, k);
