@@ -1242,15 +1242,21 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
         double RS[3] = {0, 0, 0};
         for (int t = b; t < e; t++) {
           double m = C.sp[S.sp[t]].mass, sig = sqrt(R.kbt / m);
-          for (int j = 0; j < dim; j++) RS[j] += m * (sig * Dr.part_n(rep, t, j));
+          {
+            double nrm[3];
+            Dr.part_n_all(rep, t, dim, nrm);
+            for (int j = 0; j < dim; j++) RS[j] += m * (sig * nrm[j]);
+          }
         }
         for (int j = 0; j < dim; j++) RS[j] /= C.cell.mass[c];
         for (int t = b; t < e; t++) {
           const mpcd_gpu_species &sp = C.sp[S.sp[t]];
           double sig = sqrt(R.kbt / sp.mass);
           F v[3];
+          double nrm[3];
+          Dr.part_n_all(rep, t, dim, nrm);
           for (int j = 0; j < dim; j++) {
-            double rv = sig * Dr.part_n(rep, t, j);
+            double rv = sig * nrm[j];
             F dv = (sp.damp) * (vcm[j]) / ((double)n);
             v[j] = vcm[j] + rv - RS[j] - dv;
           }
@@ -1276,7 +1282,11 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
         for (int t = b; t < e; t++) {
           const mpcd_gpu_species &sp = C.sp[S.sp[t]];
           double sig = sqrt(R.kbt / sp.mass);
-          for (int j = 0; j < dim; j++) RS[j] += sp.mass * (sig * Dr.part_n(rep, t, j));
+          {
+            double nrm[3];
+            Dr.part_n_all(rep, t, dim, nrm);
+            for (int j = 0; j < dim; j++) RS[j] += sp.mass * (sig * nrm[j]);
+          }
           if (act) {
             F q[3], pW = 0.0;
             load_qt(S, T, t, q);
@@ -1307,8 +1317,10 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
           F v[3] = {A::ld(S.vx, T.v[0], t), A::ld(S.vy, T.v[1], t), A::ld(S.vz, T.v[2], t)};
           load_qt(S, T, t, q);
           for (int j = 0; j < dim; j++) rq[j] = q[j] - cm[j];
+          double nrm[3];
+          Dr.part_n_all(rep, t, dim, nrm);
           for (int j = 0; j < dim; j++) {
-            double rv = sig * Dr.part_n(rep, t, j);
+            double rv = sig * nrm[j];
             if (act) {
               F pW = 0.0;
               for (int jj = 0; jj < dim; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
@@ -1348,8 +1360,10 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
             pW -= 0.0;
             pm = (pW <= 0) ? -1. : 1.;
           }
+          double nrm[3];
+          Dr.part_n_all(rep, t, dim, nrm);
           for (int j = 0; j < dim; j++) {
-            double rv = sig * Dr.part_n(rep, t, j);
+            double rv = sig * nrm[j];
             if (mode == 0) v[j] = vcm[j] + rv - RS[j] - (sp.damp) * (vcm[j]) + at[j];
             else if (mode == 1) v[j] = vcm[j] + rv - RS[j] + at[j];
             else v[j] = vcm[j] + rv - RS[j] + at[j] + (act ? pm * ACT * PA[j] : F(0.0)) - AS[j] - (sp.damp) * (vcm[j]);
@@ -1476,9 +1490,11 @@ __global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S,
         pW -= 0.0;
         pm = (pW <= 0) ? -1. : 1.;
       }
+      double nrm[3];
+      Dr.part_n_all(Tl.rep, t, DIM, nrm);
 #pragma unroll
       for (int d = 0; d < DIM; d++) {
-        const double rv = sig * Dr.part_n(Tl.rep, t, d);
+        const double rv = sig * nrm[d];
         rvk[j][d] = rv;
         s_term[d][i] = sp.mass * rv;
         s_term[DIM + d][i] = act ? sp.mass * (pm * ACT * PA[d]) : 0.0;
