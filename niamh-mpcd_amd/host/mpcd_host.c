@@ -42,6 +42,11 @@
  * population per species (densityout, pout.c:1744-1757, 2692), counted on
  * the host from the particles (the cell of a particle is (int)Q, as bin()
  * finds it, mpc.c:958-960).
+ * energyOut = F > 0: DIR/energy.dat, one line per runtime divisible by F
+ * (enout, pout.c:1389-1422, 2609-2612): time, MPCD kinetic energy, the
+ * nematic mean-field energy of calcE_LC (therm.c:330-351; 0 without lc),
+ * the boundaries' kinetic and rotational energies (0: no displaceable
+ * walls here), the total and the temperature of the step's velocities.
  * dirSOut = F > 0 with lc: DIR/directorfield.dat, every cell's director and
  * scalar order parameter, zeros in a cell without particles (orderout,
  * pout.c:2021-2033, 2695).
@@ -155,6 +160,34 @@ static void density_write(FILE *f, const int32_t *cnt, const double *mass, int n
                 for (int s = 0; s < nspec; s++) fprintf(f, "\t%5d", cnt[c * nspec + s]);
                 fprintf(f, "\n");
             }
+}
+
+/* enout's line (pout.c:1389-1422): MPC_K = sum 1/2 m v^2 over the DIM
+ * components; wmf = calcE_LC (therm.c:330-351): sum over cells with more than
+ * one particle of S (u . n)^2 MFPOT over their particles; KBT = TEMP */
+static void energy_write(FILE *f, const mpcd_gpu_params *p, size_t n, const double *Q, const double *V,
+                         const double *U, const int32_t *spid, const int32_t *pop, const double *S, const double *dir,
+                         double t) {
+    const int dim = p->dim, ny = p->L[1] + 1, nz = dim == 3 ? p->L[2] + 1 : 1;
+    double mpc_k = 0.0, wmf = 0.0, ke = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const int s = p->nspec > 1 ? spid[i] : 0;
+        const double m = p->sp[s].mass;
+        double e = 0.0;
+        for (int j = 0; j < dim; j++) e += V[3 * i + j] * V[3 * i + j];
+        mpc_k += 0.5 * m * e;
+        for (int j = 0; j < 3; j++) ke += m * (V[3 * i + j] - p->avv[j]) * (V[3 * i + j] - p->avv[j]);
+        if (!p->lc) continue;
+        const int a = (int)Q[3 * i], b = (int)Q[3 * i + 1], c = dim == 3 ? (int)Q[3 * i + 2] : 0;
+        if (a < 0 || b < 0 || c < 0 || a >= p->L[0] || b >= p->L[1] || (dim == 3 && c >= p->L[2])) continue;
+        const size_t k = (size_t)(a * ny + b) * nz + c;
+        if (pop[k] <= 1) continue;
+        double un = 0.0;
+        for (int j = 0; j < dim; j++) un += U[3 * i + j] * dir[3 * k + j];
+        wmf += S[k] * un * un * p->sp[s].mfpot;
+    }
+    const double kbt = n ? ke / (dim * (double)n) : 0.0;
+    fprintf(f, "%lf\t%lf\t%lf\t%lf\t%lf\t%lf\t%lf\n", t, mpc_k, wmf, 0.0, 0.0, mpc_k + wmf, kbt);
 }
 
 /* a field file of replica r: DIR/<name>.dat (r = 0) or DIR/<name>_r<r>.dat,
@@ -380,6 +413,7 @@ int main(int argc, char **argv) {
     int vel_out = json_int(J, "velOut", 0);
     int dir_out = json_int(J, "dirSOut", 0);
     int dens_out = json_int(J, "densOut", 0);
+    const int en_out = json_int(J, "energyOut", 0);
     int ckpt_out = json_int(J, "checkpointOut", 0);
     p.galinv = json_int(J, "galInv", 1);
     p.tstech = json_int(J, "tsTech", 0);
@@ -609,11 +643,11 @@ int main(int argc, char **argv) {
     /* field outputs; the flow sums start after the warm-up (mpc.c:4947:
      * !in.warmupSteps); orderout needs LC (pout.c:2695) */
     const int do_dir = p.lc && dir_out > 0;
-    FILE *ff[64] = {NULL}, *fv[64] = {NULL}, *fd[64] = {NULL}, *fn[64] = {NULL};
+    FILE *ff[64] = {NULL}, *fv[64] = {NULL}, *fd[64] = {NULL}, *fn[64] = {NULL}, *fe[64] = {NULL};
     const size_t nc = (size_t)mpcd_gpu_ncells(ctx, 0);
     double *flow = NULL, *cvcm = NULL, *cS = NULL, *cdir = NULL;
     int32_t *cpop = NULL;
-    if ((flow_out > 0 || vel_out > 0 || do_dir || dens_out > 0) && nrep > 64)
+    if ((flow_out > 0 || vel_out > 0 || do_dir || dens_out > 0 || en_out > 0) && nrep > 64)
         die("%s", "field outputs: at most 64 replicas");
     int32_t *dcnt = NULL;
     double *dmass = NULL;
@@ -638,7 +672,11 @@ int main(int argc, char **argv) {
                                "flow field: centre-of-mass velocity averaged over each output interval", FLOW_COLUMNS,
                                r, nrep, flow_out, p.L, dim, fp32);
     }
-    if (vel_out > 0 || do_dir) {
+    for (int r = 0; r < nrep && en_out > 0; r++)
+        fe[r] = field_open(outdir, "energy", "energy: kinetic and nematic mean-field energies, temperature",
+                           "t\t\tMPCD_kin\t\tMPCD_nem\t\tBC_kin\t\tBC_rot\t\tTotal\t\tKBT", r, nrep, en_out, p.L, dim,
+                           fp32);
+    if (vel_out > 0 || do_dir || (en_out > 0 && p.lc)) {
         cpop = (int32_t *)malloc(sizeof(int32_t) * nc);
         cvcm = (double *)malloc(sizeof(double) * 3 * nc);
         cS = (double *)malloc(sizeof(double) * nc);
@@ -658,8 +696,8 @@ int main(int argc, char **argv) {
     int next_out = every < total ? every : total;
     for (int done = 0; done < total;) {
         int k = next_out - done;
-        const int cad[5] = {flow_out, vel_out, do_dir ? dir_out : 0, ckpt_out, dens_out};
-        for (int q = 0; q < 5; q++)
+        const int cad[6] = {flow_out, vel_out, do_dir ? dir_out : 0, ckpt_out, dens_out, en_out};
+        for (int q = 0; q < 6; q++)
             if (cad[q] > 0 && steps_to_next(run0 + done, cad[q]) < k) k = steps_to_next(run0 + done, cad[q]);
         struct timespec a, b;
         clock_gettime(CLOCK_MONOTONIC, &a);
@@ -703,6 +741,13 @@ int main(int argc, char **argv) {
                 }
                 density_write(fn[r], dcnt, dmass, p.nspec, p.L, dim, tnow);
             }
+        if (en_out > 0 && rt % en_out == 0)
+            for (int r = 0; r < nrep; r++) {
+                double *Qr = Q + 3 * n * r, *Vr = V + 3 * n * r, *Ur = U + 3 * n * r;
+                check(ctx, mpcd_gpu_download(ctx, r, Qr, Vr, p.lc ? Ur : NULL), "mpcd_gpu_download");
+                if (p.lc) check(ctx, mpcd_gpu_get_cells(ctx, r, cpop, cvcm, cS, cdir), "mpcd_gpu_get_cells");
+                energy_write(fe[r], &p, n, Qr, Vr, Ur, spid, cpop, cS, cdir, tnow);
+            }
         if (ckpt_out > 0 && rt % ckpt_out == 0) {
             /* runCheckpoint after the outputs of this runtime (mpcd.c:310-312): DIR/checkpoint.dat, replica 0;
              * the warm-up count is 0 once its loop has run (mpcd.c:278) */
@@ -737,6 +782,7 @@ int main(int argc, char **argv) {
         if (fv[r]) fclose(fv[r]);
         if (fd[r]) fclose(fd[r]);
         if (fn[r]) fclose(fn[r]);
+        if (fe[r]) fclose(fe[r]);
     }
     free(dcnt);
     free(dmass);

@@ -457,3 +457,31 @@ def test_host_density_field(tmp_path):
         assert np.array_equal(blk[:, 4], blk[:, 9] + blk[:, 10]) and np.array_equal(blk[:, 6], blk[:, 4])
         assert np.allclose(blk[:, 5], blk[:, 9] * 1.0 + blk[:, 10] * 2.5, atol=1e-4)
         assert not blk[:, 7].any() and not blk[:, 8].any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lc", [0, 1])
+def test_host_energy_output(tmp_path, lc):
+    """energyOut (enout, pout.c:1389-1422): in a periodic box SRD keeps the
+    kinetic energy exactly, so every line shows the same MPCD_kin, equal to
+    solvent.dat's kinetic energy (the mean velocity AVV is zero); without LC
+    the nematic term is 0 and Total = MPCD_kin.  With LC the nematic
+    mean-field energy (calcE_LC) is positive and Total is the sum."""
+    cfg = {"domain": [16, 12], "simSteps": 19, "seed": 2, "energyOut": 5, "rTech": 16 if lc else 0, "lc": lc,
+           "species": [{"dens": 20, "vDist": 4, "mfpot": 10, "tumble": 0.5, "act": 0.0}]}
+    print(f"[host] energy output, lc={lc}", flush=True)
+    info, rows = run_host(tmp_path, cfg, "--every", "5")
+    lines = open(tmp_path / "out" / "energy.dat").read().splitlines()
+    assert lines[12].split() == ["t", "MPCD_kin", "MPCD_nem", "BC_kin", "BC_rot", "Total", "KBT"]
+    e = np.array([[float(x) for x in l.split()] for l in lines[13:]])
+    assert e.shape == (4, 7) and np.allclose(e[:, 0], [0.0, 0.5, 1.0, 1.5])
+    assert not e[:, 3].any() and not e[:, 4].any()
+    assert np.allclose(e[:, 5], e[:, 1] + e[:, 2], atol=2e-6)
+    assert np.all((0.8 < e[:, 6]) & (e[:, 6] < 1.2))
+    if lc:
+        assert np.all(e[:, 2] > 0)
+    else:
+        assert not e[:, 2].any()
+        assert np.allclose(e[:, 1], e[0, 1], rtol=1e-9)
+        ke = [r[2] for r in rows]
+        assert np.allclose(e[1:, 1], ke[:3], rtol=1e-6)
