@@ -322,7 +322,7 @@ def adjoint_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, angle=2.0, h=1e-11):
 
 
 def lc_throughput(M, L=256, ppc=20, steps=40, warmup=5):
-    """the active-nematic configuration of the scope table (2D 256^2 x 20 ppc,
+    """the 2D active-nematic workload (256^2 x 20 ppc,
     lc = 1, DIPOLE_DIR_SUM, U = 10, tumble 0.5, rfc 0.01, activity 0.05,
     random isotropic orientations): particle-steps/s of the split path in FP64
     and in the FP32-state form, with the per-kernel times"""
@@ -503,7 +503,7 @@ def main():
     if not args.no_adjoint and args.only == "both":
         print("[bench] forward-mode tangent of the active nematic channel vs finite differences", file=sys.stderr,
               flush=True)
-        tangent = dict(c5=tangent_check(M, hg=(1e-11, 1e-12), ha=(1e-12, 3e-13, 1e-13)),
+        tangent = dict(channel_128x64=tangent_check(M, hg=(1e-11, 1e-12), ha=(1e-12, 3e-13, 1e-13)),
                        box_32x16=tangent_check(M, Lx=32, Ly=16, K=100, hg=(1e-10,), ha=(1e-10,)))
     cpu = None if args.no_cpu else cpu_baseline()
     head_key = "fp32" if "fp32" in result else "fp64"
