@@ -485,3 +485,29 @@ def test_host_energy_output(tmp_path, lc):
         assert np.allclose(e[:, 1], e[0, 1], rtol=1e-9)
         ke = [r[2] for r in rows]
         assert np.allclose(e[1:, 1], ke[:3], rtol=1e-6)
+
+
+def test_checkpoint_outside_the_path_is_refused(tmp_path):
+    """A checkpoint with MD coupling, swimmers, a wavy wall or a truncated
+    particle list stops the host with a message naming the cause."""
+    text = open(os.path.join(GOLDEN, "ckpt_lc2d_channel.dat")).read()
+    fields = ckpt_fields(text)
+    names = [k for k, _ in fields]
+
+    def variant(edits, cut=None):
+        tok = [v for _, v in fields]
+        for k, v in edits.items():
+            tok[names.index(k)] = v
+        if cut is not None:
+            tok = tok[:cut]
+        return " ".join(tok)
+
+    cases = [(variant({"MD_mode": "1"}), "MD coupling"),
+             (variant({"NS": "2"}), "swimmers"),
+             (variant({"w0.B0": "0.500000"}), "not a fixed axis-aligned plane"),
+             (variant({}, cut=names.index("p5.Q0")), "ends or breaks")]
+    for k, (body, msg) in enumerate(cases):
+        ck = tmp_path / f"ck{k}.dat"
+        ck.write_text(body)
+        r = run_dump(tmp_path, {"checkpointIn": str(ck)})
+        assert r.returncode != 0 and msg in r.stderr, (msg, r.stderr)
