@@ -41,14 +41,53 @@ __device__ __forceinline__ void flow_add(double *acc, int c, double m, double vx
 
 // the cell-sorted state after a step: the unshifted cell that streaming
 // left for slot s is in cu[] or in bits 32..62 of the slot's key (cu_src)
+// (slots are in shifted-cell order, so neighbouring lanes often share an
+// unshifted cell: each run of equal cells in a wavefront is summed with
+// shuffles first, and only the run's last lane issues the atomics)
 template <typename Real, int DIM>
 __global__ __launch_bounds__(kBlock) void k_flow_dense(Ctx C, State<Real> S, const BlkDev *__restrict__ blk,
                                                         int cu_src, double *acc) {
   const BlkDev B = blk[blockIdx.x];
   const int s = B.beg + threadIdx.x;
-  if (s >= B.end) return;
-  const int cu = cu_src ? (int)((C.key[s] >> 32) & 0x7FFFFFFFull) : C.cu[s];
-  flow_add<DIM>(acc, cu, C.sp[S.sp[s]].mass, (double)S.vx[s], (double)S.vy[s], (double)S.vz[s]);
+  const bool live = s < B.end;
+  int cu = -1;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  if (live) {
+    cu = cu_src ? (int)((C.key[s] >> 32) & 0x7FFFFFFFull) : C.cu[s];
+    const double m = C.sp[S.sp[s]].mass;
+    a[0] = (double)S.vx[s] * m;
+    a[1] = (double)S.vy[s] * m;
+    a[2] = DIM == 3 ? (double)S.vz[s] * m : 0.0;
+    a[3] = m;
+  }
+  // segmented inclusive scan over runs of equal cu in lane order; a cell can
+  // occur in several runs of a wavefront, so a run starts wherever cu changes
+  const int lane = threadIdx.x & 63;
+  const int cu_prev = __shfl_up(cu, 1, 64);
+  int head = (lane == 0 || cu_prev != cu) ? lane : 0;  // first lane of this lane's run: a max-scan of the starts
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int hu = __shfl_up(head, o, 64);
+    if (lane >= o && hu > head) head = hu;
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    double t[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) t[k] = __shfl_up(a[k], o, 64);
+    if (lane - o >= head)
+#pragma unroll
+      for (int k = 0; k < 4; k++) a[k] += t[k];
+  }
+  const int cu_next = __shfl_down(cu, 1, 64);
+  const bool last = lane == 63 || cu_next != cu;
+  if (live && last) {
+    double *p = acc + (size_t)cu * 4;
+    unsafeAtomicAdd(p, a[0]);
+    unsafeAtomicAdd(p + 1, a[1]);
+    if (DIM == 3) unsafeAtomicAdd(p + 2, a[2]);
+    unsafeAtomicAdd(p + 3, a[3]);
+  }
 }
 
 // the next cell along an axis under the grid shift: u + 1, or 0 past the
