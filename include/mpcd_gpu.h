@@ -256,9 +256,11 @@ mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *ctx, int replica, int32_t *order, i
  * localFLOW + sumFLOW do at the end of timestep() when flowOut is set
  * (mpc.c:4946-4951, 226-246, 3889-3910).  Replaces those two calls; the
  * caller writes the output, as flowout does (pout.c:1681-1700).  The sums
- * persist while off.  Cost: one pass over the particles with FP64 atomics
- * per step; the atomics add in no fixed order, so a sum equals the
- * reference's list-order sum to round-off.  Not rewound by
+ * persist while off.  Cost: one pass over the particles and one over the
+ * cells per step (FP64 atomics per run of equal cells in the cell-sorted
+ * layout, LDS partials per tile in the bucketed one); the atomics add in no
+ * fixed order, so a sum equals the reference's list-order sum to round-off.
+ * Not rewound by
  * mpcd_gpu_restore. */
 mpcd_status mpcd_gpu_flow(mpcd_gpu_ctx *ctx, int on);
 /* the sums of replica `replica` (flow: ncell x 3 doubles, cells in the order

@@ -9,7 +9,7 @@
 //
 // Cell-sorted layout: one launch over the particles gathers the sums of m v
 // and m of every unshifted cell into an FP64 accumulator with hardware
-// atomics, and one launch over the cells adds the quotient to FLOW and clears
+// atomics (one per value and run of equal cells in a wavefront), and one launch over the cells adds the quotient to FLOW and clears
 // the accumulator with plain stores (so the pair can sit in a captured
 // graph).  The atomics add in no fixed order: a cell's sum equals the
 // reference's list-order sum to round-off, not bit for bit.
