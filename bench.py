@@ -445,6 +445,15 @@ def main():
         r32["flow_field"] = dict(steps=kf, ms_per_step=rf["ms_per_step"],
                                  flow_kernels_ms_per_step=rf["timing"]["flow"][0] / kf,
                                  summed_steps=ctx.flow_read(0)[1])
+        # the same on the cell-sorted layout (the path of parity steps,
+        # thermostats and the LC split path): FP64 atomics per run of equal
+        # cells in a wavefront instead of tile partials
+        ctx.set_layout(False)
+        ctx.flow(True)
+        rd = timed_run(M, ctx, kf, 2, B * n1)
+        ctx.flow(False)
+        r32["flow_field"]["cell_sorted"] = dict(ms_per_step=rd["ms_per_step"],
+                                                flow_kernels_ms_per_step=rd["timing"]["flow"][0] / kf)
         result["fp32"] = (r32, B * n1, B)
         ctx.close()
     if args.only in ("fp64", "both"):
