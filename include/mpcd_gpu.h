@@ -39,11 +39,31 @@
  *   lc      [lc_stride per slot]       Maier-Saupe sampler variates, slot 0 = 2D angle
  *                                      redraw, 1+2i / 2+2i = trial / acceptance of
  *                                      anneal step i, then phi and sign (3D)  (lc.c:52-411)
+ *   escape  [MPCD_ESC_STRIDE per particle and binning]   bin()'s escape repair
+ *                                      (mpc.c:970-982); binning b = 0 is the step's shifted
+ *                                      bin (mpc.c:4596), b = 1 the unshifted bin after
+ *                                      streaming (mpc.c:4936); entry (b * n_total + p) for
+ *                                      particle p (replica offset + particle index):
+ *                                      [0..2] Maxwell normals (genrand_gaussMB), [3..4]
+ *                                      orientation uniforms of genrand_coneNP (2D: the
+ *                                      angle; 3D: z, phi), [5 + DIM * t + j] uniform j of
+ *                                      position try t of replacePos_WithCheck (init.c:1462)
+ *   retherm [MPCD_RETHERM_STRIDE per particle]   wall re-thermalisation after
+ *                                      NBOUNCE bounces (push(), bc.c:1476, init.c:1261):
+ *                                      [3r + j] variate j of the particle's r-th push of
+ *                                      the step (uniforms for VDIST 0 and 3, normals for
+ *                                      2 and 4)
  * Cells are numbered x-major over (L0+1) x (L1+1) x (L2+1) (z extent 1 in 2D),
  * concatenated over replicas; particle slots are positions in the collision
  * order (cells x-major, each cell in the reference's linked-list order),
  * concatenated over replicas.  Normals are standard normals (the library
- * multiplies by sqrt(kT/m)); uniforms lie in [0,1).
+ * multiplies by sqrt(kT/m)); uniforms lie in [0,1).  escape and retherm may
+ * be NULL: those rare draws then come from the counter-based generator.
+ * The escape and retherm arrays given to a step's mpcd_gpu_bin(shifted = 1)
+ * or mpcd_gpu_collide also serve its mpcd_gpu_stream, which takes no draws.
+ * A particle that needs more position tries or pushes in one step than its
+ * injected entry holds fails the call with MPCD_EINVAL; with the generator
+ * the tries are unbounded up to 65536 (a box the walls leave no room in).
  */
 #ifndef MPCD_GPU_H
 #define MPCD_GPU_H
@@ -56,6 +76,8 @@ extern "C" {
 
 #define MPCD_MAXSPEC 10
 #define MPCD_MAXWALL 16
+#define MPCD_ESC_STRIDE 32      /* keyed escape draws per particle and binning */
+#define MPCD_RETHERM_STRIDE 24  /* keyed re-thermalisation draws per particle (8 pushes) */
 
 typedef enum {
     MPCD_OK = 0,
@@ -117,6 +139,8 @@ typedef struct {
     const double *part;    /* [3 * n_total] */
     const double *lc;      /* [lc_stride * n_total] */
     int lc_stride;
+    const double *escape;  /* [2 * n_total * MPCD_ESC_STRIDE] or NULL */
+    const double *retherm; /* [n_total * MPCD_RETHERM_STRIDE] or NULL */
 } mpcd_draws;
 
 typedef struct mpcd_gpu_ctx mpcd_gpu_ctx;
@@ -247,6 +271,25 @@ mpcd_status mpcd_gpu_tangent_download(mpcd_gpu_ctx *ctx, int replica, double *dQ
  * results, mpc.c:64-216); any pointer may be NULL */
 mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *ctx, int replica, int32_t *pop, double *vcm, double *S,
                                double *dir);
+/* Parity diagnostics of the nematic path (no counterpart in the reference):
+ * with on != 0 every later step records its discrete decisions, so that a
+ * comparison with the reference can tell a flipped decision from rounding.
+ * mpcd_gpu_lc_decisions returns those of the last step's collision phase for
+ * replica `replica`: part [n] one word per particle (particle-index order):
+ * bit 0 its cell held more than one particle, bits 1-2 the Maier-Saupe
+ * sampler (0 2D Metropolis lc.c:341, 1 3D Gaussian lc.c:85, 2 3D cube root
+ * lc.c:192, 3 3D Metropolis lc.c:257), bit 3 the 2D angle redrawn at
+ * +-pi/2 (lc.c:352), bit 4 the sign draw gave -1, bits 5-12 annealNum,
+ * bits 13-20 accepted moves, bit 21 the dipole force points along +DIR
+ * (lc.c:1859-1863); cell [ncell] one word per cell: bit 5 S and the
+ * director were computed (more than one particle, mpc.c:184), bits 0-1 the
+ * branch of eigenvectors2x2 (mtools.c:1247-1305: 0 m10 != 0, 1 m01 != 0,
+ * 2 equal diagonal, 3 other) or the column of eigenvectors3x3's product
+ * (mtools.c:1373-1430; 3 none), bit 2 the diagonal branch and bit 3 the
+ * clamp of r in eigenvalues3x3 (mtools.c:1307-1371), bit 4 S clamped to 1
+ * (mpc.c:207).  Either pointer may be NULL. */
+mpcd_status mpcd_gpu_lc_record(mpcd_gpu_ctx *ctx, int on);
+mpcd_status mpcd_gpu_lc_decisions(mpcd_gpu_ctx *ctx, int replica, uint32_t *part, uint32_t *cell);
 /* list order of replica `replica` after the last phase: order [n] particle
  * indices, cstart [ncell+1] cell offsets */
 mpcd_status mpcd_gpu_get_order(mpcd_gpu_ctx *ctx, int replica, int32_t *order, int32_t *cstart);

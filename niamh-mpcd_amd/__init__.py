@@ -107,7 +107,11 @@ class Params(C.Structure):
 class DrawsC(C.Structure):
     _fields_ = [("shift", C.POINTER(C.c_double)), ("cell", C.POINTER(C.c_double)),
                 ("ghost", C.POINTER(C.c_double)), ("part", C.POINTER(C.c_double)),
-                ("lc", C.POINTER(C.c_double)), ("lc_stride", C.c_int)]
+                ("lc", C.POINTER(C.c_double)), ("lc_stride", C.c_int),
+                ("escape", C.POINTER(C.c_double)), ("retherm", C.POINTER(C.c_double))]
+
+
+ESC_STRIDE, RETHERM_STRIDE = 32, 24  # MPCD_ESC_STRIDE, MPCD_RETHERM_STRIDE
 
 
 EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_gpu_upload", "mpcd_gpu_download",
@@ -117,7 +121,8 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_tile_fill", "mpcd_gpu_checkpoint", "mpcd_gpu_restore", "mpcd_gpu_record",
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
            "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
-           "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read"]
+           "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read", "mpcd_gpu_lc_record",
+           "mpcd_gpu_lc_decisions"]
 
 _lib = None
 
@@ -144,6 +149,8 @@ def lib():
     L.mpcd_gpu_run.argtypes = [vp, C.c_int, C.c_int]
     L.mpcd_gpu_synchronize.argtypes = [vp]
     L.mpcd_gpu_get_cells.argtypes = [vp, C.c_int, ip, dp, dp, dp]
+    L.mpcd_gpu_lc_record.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_lc_decisions.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.mpcd_gpu_get_order.argtypes = [vp, C.c_int, ip, ip]
     L.mpcd_gpu_ncells.argtypes = [vp, C.c_int]
     L.mpcd_gpu_lc_stride.argtypes = [vp]
@@ -297,9 +304,9 @@ class Context:
         if d is None:
             return None, None
         keep = {k: (None if d.get(k) is None else np.ascontiguousarray(d[k], np.float64))
-                for k in ("shift", "cell", "ghost", "part", "lc")}
+                for k in ("shift", "cell", "ghost", "part", "lc", "escape", "retherm")}
         s = DrawsC(_dp(keep["shift"]), _dp(keep["cell"]), _dp(keep["ghost"]), _dp(keep["part"]), _dp(keep["lc"]),
-                   int(d.get("lc_stride", 0)))
+                   int(d.get("lc_stride", 0)), _dp(keep["escape"]), _dp(keep["retherm"]))
         return s, keep
 
     def bin(self, draws=None, shifted=1, outputs=False):
@@ -395,6 +402,18 @@ class Context:
         dQ, dV, dU = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
         self._check(self.L.mpcd_gpu_tangent_download(self.h, r, _dp(dQ), _dp(dV), _dp(dU)), "tangent_download")
         return dQ, dV, dU
+
+    def lc_record(self, on=True):
+        """record the nematic path's discrete decisions of every later step"""
+        self._check(self.L.mpcd_gpu_lc_record(self.h, int(bool(on))), "lc_record")
+
+    def lc_decisions(self, r=0):
+        """(particle words [n], cell words [ncell]) of the last step (uint32)"""
+        part, cell = np.zeros(int(self.n[r]), np.uint32), np.zeros(self.ncells[r], np.uint32)
+        u32 = C.POINTER(C.c_uint32)
+        self._check(self.L.mpcd_gpu_lc_decisions(self.h, r, part.ctypes.data_as(u32), cell.ctypes.data_as(u32)),
+                    "lc_decisions")
+        return part, cell
 
     def cells(self, r=0):
         nc = self.ncells[r]

@@ -128,6 +128,27 @@ struct WallList {
   int32_t *cnt;  // [1]
 };
 
+// Slots (bucket * cap + position) of the records whose particle escapes in
+// the coming step's shifted bin (bin(shifted=1), mpc.c:970-982).  The step
+// that writes a record knows the next grid shift, so it lists such records;
+// the next step's preparation kernel re-places them (escape_repair) before
+// k_bucket runs and marks them (sign bit of the id: already in the shifted
+// frame).  The step kernel itself never runs the repair, which would raise
+// its register count for every particle.
+struct EscList {
+  int32_t *idx;
+  int32_t *cnt;  // [1]
+  int cap;
+};
+
+__device__ __forceinline__ void esc_push(const Ctx &C, const EscList &E, size_t o) {
+  const int e = atomicAdd(E.cnt, 1);
+  if (e < E.cap) E.idx[e] = (int32_t)o;
+  else atomicOr(&C.flags[0], kFlagPoison);  // the cell-sorted path finishes the run
+}
+
+constexpr int32_t kIdRepaired = (int32_t)0x80000000;
+
 __device__ __forceinline__ int bucket_of(const BucketRep &B, const RepDev &R, int cell) {
   const int x = cell - R.cell_off;
   int q = (int)((double)x * B.inv_tc);
@@ -180,7 +201,7 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
     const RepDev *__restrict__ reps, const mpcd_gpu_species *__restrict__ sps, const mpcd_gpu_wall *__restrict__ walls,
     const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr,
     const double *__restrict__ shift_cur, const double *__restrict__ wq_b, const double *__restrict__ shift_next,
-    const int32_t *__restrict__ bsum, int32_t *start_out, int cap_lim, WallList<Real> WL) {
+    const int32_t *__restrict__ bsum, int32_t *start_out, int cap_lim, WallList<Real> WL, EscList EN) {
   constexpr int NT = kBktThreads, CAP = NT * PPT;
   const int tid = threadIdx.x;
   __shared__ int s_flag;
@@ -264,25 +285,18 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
     rk[j] = -1;
     ra[j] = 0;
     if (i < n) {
-      const int pid = rid[j];
+      const bool repaired = rid[j] < 0;
+      rid[j] &= 0x7FFFFFFF;
       const int spid = ONESP ? 0 : rsp[j];
       const uint64_t key = rkey[j];
       double q[3] = {(double)rq[j][0], (double)rq[j][1], (double)rq[j][2]};
       double v[3] = {(double)rv[j][0], (double)rv[j][1], (double)rv[j][2]};
-      if (R.galinv) shift_fwd_bf<DIM>(Ld, shc, q);
+      if (R.galinv && !repaired) shift_fwd_bf<DIM>(Ld, shc, q);
       int cn = cell_bf<DIM>(Li, pbc, cell_off, ny, nz, q, true);
-      if (cn < 0) {
-        // escaped: fresh position and velocity, and the particle stays in
-        // the list it was in (its unshifted cell, bits 32..62 of the key)
-        atomicAdd(&C.events[0], 1ull);
-        const double m = sps[spid].mass;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-          q[d] = Dr.rng.uniform(T.rep, P_ESCAPE, (uint32_t)pid, d) * R.L[d];
-          v[d] = sqrt(R.kbt / m) * Dr.rng.normal(T.rep, P_ESCAPE, (uint32_t)pid, 16 + d);
-        }
-        cn = (int)((key >> 32) & 0x7FFFFFFFull);
-      }
+      // an escape of this shifted bin was re-placed by the preparation
+      // kernel and stays in the list it was in (its unshifted cell, bits
+      // 32..62 of the key); an unlisted escape would be an internal error
+      if (repaired) cn = (int)((key >> 32) & 0x7FFFFFFFull);
       Real vv[3] = {(Real)v[0], (Real)v[1], (Real)v[2]};
       if (R.gravflag) {
         // acc_all (mpc.c:697), in the state precision
@@ -363,6 +377,7 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
   // next cell and key, destination bucket.  A particle that leaves the
   // wall-free box (or its cells) goes to the wall list instead.
   double ke = 0.0;
+  uint32_t escm = 0;  // bit j: particle j escapes in the next shifted bin
 #pragma unroll
   for (int j = 0; j < PPT; j++) {
     if (rk[j] < 0) continue;
@@ -425,6 +440,7 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
     shift_fwd_bf<DIM>(Ld, shn, qn);
     const int cn0 = cell_bf<DIM>(Li, pbc, cell_off, ny, nz, qn, true);
     const int cn = cn0 < 0 ? cu : cn0;
+    if (cn0 < 0) escm |= 1u << j;
     rkey[j] = bin_key(cn, cu, c, gbase + pos);
     const int b2 = bucket_of(BR, R, cn);
     rk[j] = b2;
@@ -470,6 +486,7 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
     const size_t o = (size_t)b2 * CAP + p2;
     const Real q3[3] = {rq[j][0], rq[j][1], rq[j][2]}, v3[3] = {rv[j][0], rv[j][1], rv[j][2]};
     rec_store(O, o, ONESP, q3, v3, rid[j], rsp[j], rkey[j]);
+    if ((escm >> j) & 1u) esc_push(C, EN, o);
   }
   // kinetic-energy partial of the tile
 #pragma unroll
@@ -490,17 +507,47 @@ __global__ __launch_bounds__(kBktThreads) void k_bucket(
 //   block nbs: the next step's grid shift and wall positions (shift_replica);
 //   blocks above: the SRD cell draws of this step (as k_celldraws).
 // Nothing happens after a bucket overflow.
+template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_bucket_prep(
     Ctx C, Draws D1, const double *wq_prev_b, double *shift_out, double *wqs_out, double *wqb_out, WallHot *wh_out,
     double *box_out, const int32_t *__restrict__ cnt_in, int32_t *bsum, int32_t *cnt_zero, int32_t *wl_cnt, int nbkt,
-    int nbs, Draws D, const BlkDev *__restrict__ cblk, double *celldraw) {
+    int nbs, Draws D, const BlkDev *__restrict__ cblk, double *celldraw, int ncb, BState<Real> I, EscList EC,
+    EscList EN, const TileDev *__restrict__ bkts, int cap, int onesp, const WallHot *__restrict__ wh_cur, int nesc) {
   if (C.flags[0] & kFlagPoison) return;  // the flag does not change during this kernel
   const int tid = threadIdx.x, blk = blockIdx.x;
+  if (blk >= nbs + 1 + ncb) {
+    // the escapes of this step's shifted bin, listed by the step that wrote
+    // their records: grid shift, escape repair, record marked as shifted
+    D.resolve();
+    const int n = min(*EC.cnt, EC.cap);
+    for (int e = (blk - nbs - 1 - ncb) * kBlock + tid; e < n; e += nesc * kBlock) {
+      const size_t o = (size_t)EC.idx[e];
+      const TileDev T = bkts[o / (size_t)cap];
+      const RepDev &R = C.rep[T.rep];
+      Real q_[3], v_[3];
+      int id, sp;
+      uint64_t key;
+      rec_load(I, o, onesp != 0, q_, v_, id, sp, key);
+      double q[3] = {(double)q_[0], (double)q_[1], (double)q_[2]}, v[3] = {(double)v_[0], (double)v_[1], (double)v_[2]};
+      if (R.galinv) shift_fwd(R, &C.shift[T.rep * 3], q);
+      if (cell_of(R, q, 1) >= 0) continue;  // listed only when it escapes
+      atomicAdd(&C.events[0], 1ull);
+      double u0[3] = {0.0, 0.0, 0.0};
+      if (!escape_repair(R, wh_cur, 0, D, T.rep, (uint32_t)id, C.sp[sp].mass, false, q, v, u0))
+        atomicOr(&C.flags[0], kFlagEscape);
+      const Real q3[3] = {(Real)q[0], (Real)q[1], (Real)q[2]}, v3[3] = {(Real)v[0], (Real)v[1], (Real)v[2]};
+      rec_store(I, o, onesp != 0, q3, v3, id | kIdRepaired, sp, key);
+    }
+    return;
+  }
   if (blk < nbs) {
     const int b = blk * kBlock + tid;
     int v = (b < nbkt) ? cnt_in[b] : 0;
     if (b <= nbkt) cnt_zero[b] = 0;
-    if (b == 0) *wl_cnt = 0;
+    if (b == 0) {
+      *wl_cnt = 0;
+      *EN.cnt = 0;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     __shared__ int s_w[kBlock / 64];
@@ -547,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_walls(
     Ctx C, WallList<Real> WL, BState<Real> O, const BucketRep *__restrict__ brep, const RepDev *__restrict__ reps,
     const mpcd_gpu_species *__restrict__ sps, const mpcd_gpu_wall *__restrict__ walls,
     const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr, const double *__restrict__ wq_b,
-    const double *__restrict__ shift_next, int cap, int cap_lim, int nwall) {
+    const double *__restrict__ shift_next, int cap, int cap_lim, int nwall, EscList EN) {
   if (C.flags[0] & kFlagPoison) return;
   const int n = *WL.cnt;
   if ((int)(blockIdx.x * blockDim.x) >= n) return;  // the whole block has no entry
@@ -584,7 +631,10 @@ __global__ __launch_bounds__(kBlock) void k_bucket_walls(
       rv[d] = (Real)v[d];
       q[d] = (double)rq[d];
     }
-    const int cn = next_cell<DIM>(R, &shift_next[rep * 3], cu, q);
+    double qn[3] = {q[0], q[1], q[2]};
+    shift_fwd_d<DIM>(R, &shift_next[rep * 3], qn);
+    const int cn0 = cell_of_d<DIM>(R, qn, 1);
+    const int cn = cn0 < 0 ? cu : cn0;  // next_cell: an escape stays in its list
     const int b2 = bucket_of(brep[rep], R, cn);
     const int p2 = atomicAdd(&O.cnt[b2], 1);
     if (p2 >= cap_lim) {
@@ -594,6 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_walls(
     }
     const size_t o = (size_t)b2 * cap + p2;
     rec_store(O, o, false, rq, rv, pid, spid, bin_key(cn, cu, c, slot));
+    if (cn0 < 0) esc_push(C, EN, o);
   }
 }
 
@@ -601,7 +652,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_walls(
 // shifted bin of the coming step (cnew, key), goes to the bucket of its cell
 template <typename Real>
 __global__ void k_bucketize(Ctx C, State<Real> S, BState<Real> O, const BucketRep *__restrict__ brep, int cap,
-                            int cap_lim) {
+                            int cap_lim, EscList EN) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= C.ntot) return;
   const int cn = C.cnew[s];
@@ -618,6 +669,11 @@ __global__ void k_bucketize(Ctx C, State<Real> S, BState<Real> O, const BucketRe
   const size_t o = (size_t)b * cap + p;
   const Real q[3] = {S.x[s], S.y[s], S.z[s]}, v[3] = {S.vx[s], S.vy[s], S.vz[s]};
   rec_store(O, o, false, q, v, S.id[s], S.sp[s], C.key[s]);
+  // an escape of the coming shifted bin (C.shift) goes to the escape list
+  const RepDev &R = C.rep[r];
+  double qs[3] = {(double)q[0], (double)q[1], (double)q[2]};
+  if (R.galinv) shift_fwd(R, &C.shift[r * 3], qs);
+  if (cell_of(R, qs, 1) < 0) esc_push(C, EN, o);
 }
 
 // bucketed -> dense: each particle goes back to its slot of the last step's
@@ -638,6 +694,13 @@ __global__ void k_densify(Ctx C, BState<Real> I, State<Real> S, const TileDev *_
     rec_load(I, o, onesp != 0, q_, v_, id, sp, key);
     const int slot = (int)(key & 0x7FFFFFFFull);
     const int cu = (int)((key >> 32) & 0x7FFFFFFFull);
+    if (id < 0) {
+      // re-placed by a step that did not complete (bucket overflow): the
+      // cell-sorted step repeats the repair with the same draws; any position
+      // that escapes its shifted bin stands for the one it left
+      id &= 0x7FFFFFFF;
+      q_[0] = (Real)(-2.0 * (double)R.L[0] - 2.0);
+    }
     const double q[3] = {(double)q_[0], (double)q_[1], (double)q_[2]};
     S.x[slot] = q_[0];
     S.y[slot] = q_[1];

@@ -152,18 +152,21 @@ __device__ __forceinline__ void wall_loop(const Ctx &C, const RepDev &R, const m
         }
       } else {
         atomicAdd(&C.events[3], 1ull);
-#pragma unroll
-        for (int j = 0; j < DIM; j++) {
-          if (sp.vdist == 0) V[j] = sqrt(R.kbt / sp.mass) * (2. * Dr.rng.uniform(rep, P_RETHERM, pid, rk + j) - 1.);
-          else V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(rep, P_RETHERM, pid, rk + j);
+        if (Dr.reth_keyed() && rk + DIM > kRethStride) {
+          atomicOr(&C.flags[0], kFlagPushes);
+          break;
         }
+        push_velocity(sp, R.kbt, DIM, Dr, rep, (size_t)R.part_off + pid, pid, rk, V);
         rk += 3;
         t_left = t_step;
         t_coll = 0.0;
         cnt2++;
         cnt = 0;
       }
-      if (cnt2 > 1000) break;  // a particle that can never leave the walls
+      if (cnt2 > 1000) {  // a particle the walls leave no room for: the reference would loop for ever
+        atomicOr(&C.flags[0], kFlagStuck);
+        break;
+      }
     }
   }
   if (flagCNT) atomicAdd(&C.events[1], 1ull);
@@ -173,16 +176,12 @@ __device__ __forceinline__ void wall_loop(const Ctx &C, const RepDev &R, const m
 // re-placed and stay in the list they were in (mpc.c:970-982)
 template <int DIM>
 __device__ __forceinline__ int unshifted_cell(const Ctx &C, const RepDev &R, const mpcd_gpu_species &sp,
-                                              const Draws &Dr, int rep, int pid, int cell_this, double Q[3],
-                                              double V[3]) {
+                                              const WallHot *wh, const Draws &Dr, int rep, int pid, int cell_this,
+                                              double Q[3], double V[3], double U[3], bool lc) {
   int cu = cell_of_d<DIM>(R, Q, 0);
   if (cu < 0) {
     atomicAdd(&C.events[0], 1ull);
-#pragma unroll
-    for (int j = 0; j < DIM; j++) {
-      Q[j] = Dr.rng.uniform(rep, P_ESCAPE, pid, 8 + j) * R.L[j];
-      V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(rep, P_ESCAPE, pid, 24 + j);
-    }
+    if (!escape_repair(R, wh, 1, Dr, rep, pid, sp.mass, lc, Q, V, U)) atomicOr(&C.flags[0], kFlagEscape);
     cu = cell_this;
   }
   return cu;
@@ -204,7 +203,7 @@ __device__ __forceinline__ int stream_walls(const Ctx &C, const RepDev &R, const
 #pragma unroll
   for (int i = 0; i < DIM; i++) inside = inside && (Q[i] > box[2 * i]) && (Q[i] < box[2 * i + 1]);
   if (!inside) wall_loop<DIM>(C, R, sp, walls, wh, wq_b, Dr, rep, spid, pid, Q, V, U, lc);
-  return unshifted_cell<DIM>(C, R, sp, Dr, rep, pid, cell_this, Q, V);
+  return unshifted_cell<DIM>(C, R, sp, wh, Dr, rep, pid, cell_this, Q, V, U, lc);
 }
 
 // next-step shifted cell of a streamed particle (bin(shifted=1), mpc.c:948);
@@ -505,13 +504,11 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
       const int spid = ONESP ? 0 : So.sp[src];
       if (R.galinv) shift_fwd_d<DIM>(R, shc, q);
       if (cell_of_d<DIM>(R, q, 1) < 0) {
+        // escape repair (mpc.c:970-982); the particle stays in its list
         atomicAdd(&C.events[0], 1ull);
-        double m = sps[spid].mass;
-#pragma unroll
-        for (int j = 0; j < DIM; j++) {
-          q[j] = Dr.rng.uniform(T.rep, P_ESCAPE, (uint32_t)pid, j) * R.L[j];
-          v[j] = sqrt(R.kbt / m) * Dr.rng.normal(T.rep, P_ESCAPE, (uint32_t)pid, 16 + j);
-        }
+        double u0[3] = {0.0, 0.0, 0.0};
+        if (!escape_repair(R, wh, 0, Dr, T.rep, (uint32_t)pid, sps[spid].mass, false, q, v, u0))
+          atomicOr(&C.flags[0], kFlagEscape);
       }
       s_q[0][p] = (Real)q[0];
       s_q[1][p] = (Real)q[1];
@@ -642,15 +639,25 @@ __global__ __launch_bounds__(kBlock) void k_stream_next(Ctx C, State<Real> S, co
   Dr.resolve();
   const BlkDev B = blk[blockIdx.x];
   int t = B.beg + threadIdx.x;
-  if (t >= B.end) return;
   const RepDev &R = reps[B.rep];
+  // the block's term of the temperature after the scaling (mpc.c:4760)
+  double ke = 0.0;
   const double g = C.gscale[B.rep];
-  const bool lc = R.lc != MPCD_ISOF;
-  double q[3], v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]}, U[3] = {0, 0, 0};
-  load_q(S, t, q);
-  if (g != 1.0) {
-    for (int d = 0; d < DIM; d++) v[d] = (double)(Real)(v[d] * g);
+  double v[3] = {0.0, 0.0, 0.0};
+  if (t < B.end) {
+    v[0] = (double)S.vx[t];
+    v[1] = (double)S.vy[t];
+    v[2] = (double)S.vz[t];
+    if (g != 1.0) {
+      for (int d = 0; d < DIM; d++) v[d] = (double)(Real)(v[d] * g);
+    }
+    ke = temp_term(sps[S.sp[t]].mass, v, R.avv);
   }
+  block_sum_to(ke, &C.tpart[blockIdx.x]);
+  if (t >= B.end) return;
+  const bool lc = R.lc != MPCD_ISOF;
+  double q[3], U[3] = {0, 0, 0};
+  load_q(S, t, q);
   if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
   const int c = S.cst[t];
   int cu = stream_walls<DIM>(C, R, sps, walls, wh, wq_b, &boxes[B.rep * 6], Dr, B.rep, S.sp[t], S.id[t], c, q, v, U,

@@ -38,7 +38,7 @@ struct mpcd_gpu_ctx {
   TileDev *d_lct = nullptr;
   int2 *d_lct_rng = nullptr;
   int32_t *d_lct_over = nullptr;  // [1 + tiles]: count, then the tiles above k_collide_lc's capacity
-  std::vector<int2> cblk_rng, tile_rng;
+  std::vector<int2> cblk_rng, tile_rng, pblk_rng;
   std::vector<int> uploaded;
   bool need_init = true;
   // binning state: the logical list order is the storage order, or (pending)
@@ -80,6 +80,9 @@ struct mpcd_gpu_ctx {
   WallList<double> wld{};
   WallList<float> wlf{};
   int32_t *bcnt[2] = {nullptr, nullptr}, *bbase = nullptr, *bstart[2] = {nullptr, nullptr};
+  // records whose particle escapes in the next shifted bin, by step parity
+  int32_t *esc_idx[2] = {nullptr, nullptr}, *esc_cnt = nullptr;  // esc_cnt [2]
+  int esc_cap = 0;
   hipGraph_t bgraph = nullptr;
   hipGraphExec_t bgexec = nullptr;
   int bgraph_cur = -1;
@@ -112,7 +115,7 @@ struct mpcd_gpu_ctx {
   RepDev *d_rep = nullptr;
   BlkDev *d_pblk = nullptr, *d_cblk = nullptr;
   TileDev *d_tiles = nullptr;
-  int2 *d_cblk_rng = nullptr, *d_tile_rng = nullptr;
+  int2 *d_cblk_rng = nullptr, *d_tile_rng = nullptr, *d_pblk_rng = nullptr;
   mpcd_gpu_species *d_sp = nullptr;
   mpcd_gpu_wall *d_wall = nullptr;
   Ctx C{};
@@ -131,6 +134,10 @@ struct mpcd_gpu_ctx {
   State<float> sf[2]{};
   double *d_avpart = nullptr;
   double *d_dshift = nullptr, *d_dcell = nullptr, *d_dghost = nullptr, *d_dpart = nullptr, *d_dlc = nullptr;
+  // injected escape / re-thermalisation draws of the current step (null
+  // arrays: the generator), and whether the step's streaming uses them
+  double *d_desc = nullptr, *d_dreth = nullptr;
+  bool esc_inj = false, reth_inj = false, stream_inj = false;
   int dlc_stride = 0;
   size_t dlc_cap = 0;
   int lc_stride = 0;
@@ -320,8 +327,12 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
       return bad(MPCD_EINVAL, "species or wall count out of range");
     if (memcmp(p.sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC) != 0 || p.nspec != params[0].nspec)
       return bad(MPCD_EINVAL, "the species table must agree across replicas");
-    for (int i = 0; i < p.nspec; i++)
+    for (int i = 0; i < p.nspec; i++) {
       if (!(p.sp[i].mass > 0.0)) return bad(MPCD_EINVAL, "species mass must be positive");
+      // push() (init.c:1261-1293): READ (1) needs a file, anything else exits
+      if (!(p.sp[i].vdist == 0 || p.sp[i].vdist == 2 || p.sp[i].vdist == 3 || p.sp[i].vdist == 4))
+        return bad(MPCD_EUNSUPPORTED, "species velocity distribution (vDist) outside this path");
+    }
     if (p.lc == MPCD_ISOF) {
       if (!(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT || p.rtech == MPCD_RAT))
         return bad(MPCD_EUNSUPPORTED, "collision operator outside this path");
@@ -390,9 +401,10 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
     if (p.tstech != MPCD_NOTHERM || (p.rtech >= 7 && p.rtech <= 17)) c->any_therm = true;
     if (p.lc == MPCD_LCG) c->any_lcg = true;
     c->reps.push_back(R);
-    int cb0 = (int)c->cblk.size();
+    int cb0 = (int)c->cblk.size(), pb0 = (int)c->pblk.size();
     for (int b = 0; b < R.npart; b += kBlock)
       c->pblk.push_back(BlkDev{r, poff + b, poff + std::min(b + kBlock, R.npart), 0});
+    c->pblk_rng.push_back(make_int2(pb0, (int)c->pblk.size()));
     for (int b = 0; b < R.ncell; b += kBlock)
       c->cblk.push_back(BlkDev{r, coff + b, coff + std::min(b + kBlock, R.ncell), 0});
     c->cblk_rng.push_back(make_int2(cb0, (int)c->cblk.size()));
@@ -426,6 +438,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->d_cblk = dalloc<BlkDev>(c, c->cblk.size());
   c->d_tiles = dalloc<TileDev>(c, c->nctot);  // room for the finest tiling
   c->d_cblk_rng = dalloc<int2>(c, nrep);
+  c->d_pblk_rng = dalloc<int2>(c, nrep);
   c->d_tile_rng = dalloc<int2>(c, nrep);
   c->d_lct = dalloc<TileDev>(c, c->lct.size());
   c->d_lct_rng = dalloc<int2>(c, nrep);
@@ -445,7 +458,8 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   }
   c->hist2[0] = dalloc<int32_t>(c, c->nctot + 1);
   c->hist2[1] = dalloc<int32_t>(c, c->nctot + 1);
-  ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_tile_rng && c->d_sp &&
+  ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_pblk_rng && c->d_tile_rng &&
+       c->d_sp &&
        c->d_lct && c->d_lct_rng && c->d_lct_over &&
        c->d_wall && c->shift2 && c->wqs2 && c->wqb2 && c->wh2 && c->box2 && c->hist2[0] && c->hist2[1];
   Ctx &C = c->C;
@@ -468,7 +482,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   C.cell.E = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 9 * (size_t)c->nctot) : nullptr;
   C.cell.avm = (c->lc != MPCD_ISOF) ? dalloc<double>(c, (size_t)c->nctot) : nullptr;
   C.T = (c->lc != MPCD_ISOF) ? dalloc<double>(c, 3 * (size_t)c->ntot) : nullptr;
-  C.tpart = dalloc<double>(c, std::max<size_t>(c->cblk.size(), c->nctot));
+  C.tpart = dalloc<double>(c, std::max<size_t>(std::max<size_t>(c->cblk.size(), c->nctot), c->pblk.size()));
   C.kbtnow = dalloc<double>(c, nrep);
   C.gscale = dalloc<double>(c, nrep);
   C.sg = dalloc<double>(c, nrep);
@@ -510,6 +524,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   up(c->d_cblk, c->cblk.data(), sizeof(BlkDev) * c->cblk.size());
   up(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size());
   up(c->d_cblk_rng, c->cblk_rng.data(), sizeof(int2) * nrep);
+  up(c->d_pblk_rng, c->pblk_rng.data(), sizeof(int2) * nrep);
   up(c->d_tile_rng, c->tile_rng.data(), sizeof(int2) * nrep);
   up(c->d_lct, c->lct.data(), sizeof(TileDev) * c->lct.size());
   up(c->d_lct_rng, c->lct_rng.data(), sizeof(int2) * nrep);
@@ -564,6 +579,9 @@ static Draws make_draws(mpcd_gpu_ctx *c, int injected, uint32_t step) {
   D.part = c->d_dpart;
   D.lc = c->d_dlc;
   D.lc_stride = c->dlc_stride;
+  D.esc = (injected && c->esc_inj) ? c->d_desc : nullptr;
+  D.reth = (injected && c->reth_inj) ? c->d_dreth : nullptr;
+  D.ntot = c->ntot;
   if (c->capturing) {
     D.step_dev = c->d_step;
     D.step_off = step - c->cap_base;
@@ -616,6 +634,7 @@ static Ctx view(mpcd_gpu_ctx *c, int par) {
   C.shift = c->shift2 + par * 3 * c->nrep;
   C.wq_s = c->wqs2 + par * nw3;
   C.wq_b = c->wqb2 + par * nw3;
+  C.wh = c->wh2 + par * std::max(1, c->nwtot);
   C.hist = c->hist2[par];
   return C;
 }
@@ -643,6 +662,18 @@ static mpcd_status stage_draws(mpcd_gpu_ctx *c, const mpcd_draws *d) {
     }
     c->dlc_stride = d->lc_stride;
     HIPCHK(c, hipMemcpyAsync(c->d_dlc, d->lc, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+  }
+  c->esc_inj = d->escape != nullptr;
+  if (d->escape) {
+    const size_t need = 2 * (size_t)c->ntot * kEscStride;
+    if (!c->d_desc && !(c->d_desc = dalloc<double>(c, need))) return fail(c, MPCD_ENOMEM, "escape draw staging");
+    HIPCHK(c, hipMemcpyAsync(c->d_desc, d->escape, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+  }
+  c->reth_inj = d->retherm != nullptr;
+  if (d->retherm) {
+    const size_t need = (size_t)c->ntot * kRethStride;
+    if (!c->d_dreth && !(c->d_dreth = dalloc<double>(c, need))) return fail(c, MPCD_ENOMEM, "retherm draw staging");
+    HIPCHK(c, hipMemcpyAsync(c->d_dreth, d->retherm, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));  // host arrays are borrowed only for the call
   return MPCD_OK;
@@ -738,6 +769,9 @@ template <typename Real>
 static void run_props(mpcd_gpu_ctx *c, int do_acc) {
   State<Real> *S = states<Real>(c);
   Ctx C = view(c, c->step & 1);
+  // the cell decisions are those of the collision's localPROP
+  if (!do_acc) C.cdec = nullptr;
+  else if (C.cdec) k_fill<<<nblocks(c->nctot, kBlock), kBlock, 0, c->stream>>>((int32_t *)C.cdec, 0, c->nctot);
   ProfScope ps(c, F_PROPS);
   const bool tan = std::is_same<Real, double>::value && c->tan_on;
   const TanDev T = tan ? tan_view(c, c->cur) : TanDev{};
@@ -822,7 +856,10 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
   }
   ProfScope ps(c, F_TEMP);
   k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(C, lc_tiled ? c->d_lct_rng : c->d_cblk_rng);
-  if (c->any_therm) k_scale<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk);
+  if (c->any_therm) {
+    k_scale<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk);
+    k_temp_final<<<c->nrep, kBlock, 0, c->stream>>>(C, c->d_pblk_rng);
+  }
 }
 
 // the running flow field after a completed step (mpc.c:4946-4951): the
@@ -964,6 +1001,7 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
       k_stream_next<Real, 2><<<c->pblk.size(), kBlock, 0, c->stream>>>(
           C, S[c->cur], c->d_pblk, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
           c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, c->hist2[par ^ 1], do_next);
+    k_temp_final<<<c->nrep, kBlock, 0, c->stream>>>(C, c->d_pblk_rng);
   }
   c->pending = true;
   c->cu_src = do_next ? 1 : 0;
@@ -993,6 +1031,20 @@ static mpcd_status check_flags(mpcd_gpu_ctx *c) {
   int flags[4];
   HIPCHK(c, hipMemcpyAsync(flags, c->C.flags, sizeof flags, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flags[0] & (kFlagEscape | kFlagPushes | kFlagStuck)) {
+    // reported once: the bits are cleared
+    const int keep = flags[0] & ~(kFlagEscape | kFlagPushes | kFlagStuck);
+    k_fill<<<1, 1, 0, c->stream>>>(c->C.flags, keep, 1);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (flags[0] & kFlagEscape)
+      return fail(c, MPCD_EINVAL,
+                  "escape repair found no position clear of the walls (%d tries with injected draws, %d with the "
+                  "generator)", (kEscStride - kEscPos0) / c->dim, kEscMaxTries);
+    if (flags[0] & kFlagPushes)
+      return fail(c, MPCD_EINVAL, "a particle needed more than %d wall re-thermalisations in one step (injected draws)",
+                  kRethStride / 3);
+    return fail(c, MPCD_EINVAL, "a particle was re-thermalised 1000 times in one step: the walls leave it no room");
+  }
   if (flags[0] & 2) return fail(c, MPCD_EINVAL, "initial positions outside the box");
   if (flags[0] & 1)
     return fail(c, MPCD_ECAPACITY, "a cell holds more than %d particles (fused-kernel LDS capacity)",
@@ -1152,6 +1204,7 @@ extern "C" mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *c, const mpcd_draws *draws, in
   if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
+  if (shifted) c->stream_inj = draws != nullptr;
   if (shifted == 0 && !c->pending) {
     // already materialised
   } else if (c->precision == MPCD_FP64) {
@@ -1197,6 +1250,7 @@ extern "C" mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *c, const mpcd_draws *draws
   if (st) return st;
   st = stage_draws(c, draws);
   if (st) return st;
+  if (draws) c->stream_inj = true;
   if (c->precision == MPCD_FP64) run_collide<double>(c, draws != nullptr);
   else run_collide<float>(c, draws != nullptr);
   st = check_launch(c);
@@ -1219,9 +1273,13 @@ extern "C" mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *c, int64_t *bc_event_counts
   if (st) return st;
   st = ensure_dense(c);
   if (st) return st;
-  if (c->precision == MPCD_FP64) run_stream<double>(c, 0);
-  else run_stream<float>(c, 0);
+  const int inj = c->stream_inj ? 1 : 0;
+  c->stream_inj = false;
+  if (c->precision == MPCD_FP64) run_stream<double>(c, inj);
+  else run_stream<float>(c, inj);
   st = check_launch(c);
+  if (st) return st;
+  st = check_flags(c);
   if (st) return st;
   if (bc_event_counts_out) return read_events(c, bc_event_counts_out);
   return MPCD_OK;
@@ -1383,6 +1441,12 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
     // the wall list holds at most every particle once per step
     if (!(c->precision == MPCD_FP64 ? alloc_wall_list<double>(c, c->wld) : alloc_wall_list<float>(c, c->wlf)))
       return fail(c, MPCD_ENOMEM, "wall list");
+    // escape lists: a step with more escapes finishes on the cell-sorted path
+    c->esc_cap = std::max(4096, c->ntot / 8);
+    c->esc_idx[0] = dalloc<int32_t>(c, c->esc_cap);
+    c->esc_idx[1] = dalloc<int32_t>(c, c->esc_cap);
+    c->esc_cnt = dalloc<int32_t>(c, 2);
+    if (!c->esc_idx[0] || !c->esc_idx[1] || !c->esc_cnt) return fail(c, MPCD_ENOMEM, "escape lists");
   }
   const int cap = bkt_cap(fused_op(c->rtech));
   const size_t slots = (size_t)c->nbkt * cap;
@@ -1414,6 +1478,9 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
   return MPCD_OK;
 }
 
+// the escape list of the step with parity par
+static EscList esc_list(mpcd_gpu_ctx *c, int par) { return EscList{c->esc_idx[par], c->esc_cnt + par, c->esc_cap}; }
+
 static int bkt_limit(const mpcd_gpu_ctx *c) {
   const int cap = bkt_cap(fused_op(c->rtech));
   return (c->bkt_caplim > 0 && c->bkt_caplim < cap) ? c->bkt_caplim : cap;
@@ -1426,7 +1493,8 @@ static void launch_bucket_g(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int p
   k_bucket<Real, OP, ONESP, DIM, bkt_ppt(OP), GHOST><<<c->nbkt, kBktThreads, 0, c->stream>>>(
       C, B[c->bcur], B[1 - c->bcur], c->d_bkts, c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
       c->box2 + par * 6 * c->nrep, D, c->shift2 + par * 3 * c->nrep, c->wqb2 + par * nw3,
-      c->shift2 + (par ^ 1) * 3 * c->nrep, c->bbase, c->bstart[par], bkt_limit(c), wall_list<Real>(c));
+      c->shift2 + (par ^ 1) * 3 * c->nrep, c->bbase, c->bstart[par], bkt_limit(c), wall_list<Real>(c),
+      esc_list(c, par ^ 1));
 }
 
 // the ghost-particle code only where a replica has phantom walls (it sets
@@ -1473,10 +1541,13 @@ static void run_step_bucket(mpcd_gpu_ctx *c) {
     const int nbs = nblocks(c->nbkt + 1, kBlock);
     const int ncb = c->d_celldraw ? (int)c->cblk.size() : 0;
     Draws D1 = make_draws(c, 0, c->step + 1);
-    k_bucket_prep<<<nbs + 1 + ncb, kBlock, 0, c->stream>>>(
+    const int nesc = 4;  // blocks re-placing this step's listed escapes
+    k_bucket_prep<Real><<<nbs + 1 + ncb + nesc, kBlock, 0, c->stream>>>(
         C, D1, c->wqb2 + (p1 ^ 1) * nw3, c->shift2 + p1 * 3 * c->nrep, c->wqs2 + p1 * nw3, c->wqb2 + p1 * nw3,
         c->wh2 + p1 * std::max(1, c->nwtot), c->box2 + p1 * 6 * c->nrep, c->bcnt[c->bcur], c->bbase,
-        c->bcnt[1 - c->bcur], wall_list<Real>(c).cnt, c->nbkt, nbs, D, c->d_cblk, c->d_celldraw);
+        c->bcnt[1 - c->bcur], wall_list<Real>(c).cnt, c->nbkt, nbs, D, c->d_cblk, c->d_celldraw, ncb,
+        bstates<Real>(c)[c->bcur], esc_list(c, par), esc_list(c, p1), c->d_bkts, bkt_cap(fused_op(c->rtech)),
+        c->nspec == 1 ? 1 : 0, c->wh2 + par * std::max(1, c->nwtot), nesc);
     if (c->d_celldraw) D.cell = c->d_celldraw;
   }
   {
@@ -1492,12 +1563,12 @@ static void run_step_bucket(mpcd_gpu_ctx *c) {
       k_bucket_walls<Real, 3><<<nb, kBlock, 0, c->stream>>>(
           C, wall_list<Real>(c), B[1 - c->bcur], c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
           c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c),
-          c->nwtot);
+          c->nwtot, esc_list(c, par ^ 1));
     else
       k_bucket_walls<Real, 2><<<nb, kBlock, 0, c->stream>>>(
           C, wall_list<Real>(c), B[1 - c->bcur], c->d_brep, c->d_rep, c->d_sp, c->d_wall, c->wh2 + par * nwm,
           c->box2 + par * 6 * c->nrep, D, c->wqb2 + par * nw3, c->shift2 + (par ^ 1) * 3 * c->nrep, cap, bkt_limit(c),
-          c->nwtot);
+          c->nwtot, esc_list(c, par ^ 1));
   }
   c->bcur ^= 1;
   if (c->flow_on) {
@@ -1535,8 +1606,10 @@ static mpcd_status bucket_enter(mpcd_gpu_ctx *c) {
   Ctx C = view(c, c->step & 1);
   k_fill<<<nblocks(c->nbkt + 1, kBlock), kBlock, 0, c->stream>>>(c->bcnt[0], 0, c->nbkt + 1);
   k_fill<<<1, 1, 0, c->stream>>>(c->C.flags + 1, 0x7FFFFFFF, 1);
+  k_fill<<<1, 1, 0, c->stream>>>(c->esc_cnt + (c->step & 1), 0, 1);
   k_bucketize<Real><<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C, S[c->cur], B[0], c->d_brep,
-                                                                          bkt_cap(fused_op(c->rtech)), bkt_limit(c));
+                                                                          bkt_cap(fused_op(c->rtech)), bkt_limit(c),
+                                                                          esc_list(c, c->step & 1));
   c->bcur = 0;
   c->bkt_step0 = c->step;
   return check_launch(c);
@@ -2087,6 +2160,37 @@ extern "C" mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *c) {
   mpcd_status st = check_launch(c);
   if (st) return st;
   return check_flags(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_lc_record(mpcd_gpu_ctx *c, int on) {
+  if (!c) return MPCD_EINVAL;
+  if (c->lc == MPCD_ISOF) return fail(c, MPCD_EINVAL, "lc_record: the context has no nematic");
+  if (on && !c->C.pdec) {
+    c->C.pdec = (uint32_t *)dalloc<int32_t>(c, c->ntot);
+    c->C.cdec = (uint32_t *)dalloc<int32_t>(c, c->nctot);
+    if (!c->C.pdec || !c->C.cdec) return fail(c, MPCD_ENOMEM, "decision records");
+    k_fill<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>((int32_t *)c->C.pdec, 0, c->ntot);
+    k_fill<<<nblocks(c->nctot, kBlock), kBlock, 0, c->stream>>>((int32_t *)c->C.cdec, 0, c->nctot);
+  }
+  if (!on) {
+    c->C.pdec = nullptr;  // the buffers stay with the context
+    c->C.cdec = nullptr;
+  }
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_lc_decisions(mpcd_gpu_ctx *c, int r, uint32_t *part, uint32_t *cell) {
+  if (!c || r < 0 || r >= c->nrep) return c ? fail(c, MPCD_EINVAL, "lc_decisions: bad replica") : MPCD_EINVAL;
+  if (!c->C.pdec) return fail(c, MPCD_EINVAL, "lc_decisions: recording is off (mpcd_gpu_lc_record)");
+  const RepDev &R = c->reps[r];
+  if (part)
+    HIPCHK(c, hipMemcpyAsync(part, c->C.pdec + R.part_off, sizeof(uint32_t) * R.npart, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (cell)
+    HIPCHK(c, hipMemcpyAsync(cell, c->C.cdec + R.cell_off, sizeof(uint32_t) * R.ncell, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MPCD_OK;
 }
 
 extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, double *vcm, double *S, double *dir) {

@@ -66,11 +66,14 @@ struct CellDev {
   double *avm;  // nematic cells: mean mean-field potential (lc.c:495), cells of two or more
 };
 
+struct WallHot;
+
 struct Ctx {
   const RepDev *rep;
   const mpcd_gpu_species *sp;
   const mpcd_gpu_wall *wall;
   double *wq_s, *wq_b;       // wall positions of this step: shifted, shifted back
+  const WallHot *wh;         // the walls' hot fields of this step (escape repair)
   double *shift;             // [3 * nrep]
   int32_t *hist, *start, *cnew, *loc, *cu;
   uint64_t *key, *skey;
@@ -81,7 +84,26 @@ struct Ctx {
   unsigned long long *events;   // [4]
   int *flags;                   // [0] error bits (kFlag*), [1] first overflowing step of a bucketed run
   int ntot, nctot, nrep;
+  // nematic parity diagnostics (null unless recording): the discrete
+  // decisions of the last step, per particle (particle-index order,
+  // replicas concatenated) and per cell (lc_decision_word, cell_decision_word)
+  uint32_t *pdec, *cdec;
 };
+
+// the decision words of the nematic path (also written by the oracle):
+// particle: bit 0 collided (its cell holds more than one particle), bits 1-2
+// sampler (0 2D Metropolis, 1 3D Gaussian, 2 3D cube root, 3 3D
+// Metropolis), bit 3 the 2D angle redrawn at +-pi/2, bit 4 the sign pm = -1,
+// bits 5-12 annealNum, bits 13-20 accepted moves, bit 21 the dipole force
+// on the positive side;
+// cell: bit 5 S and director computed (more than one particle), bits 0-3
+// the eigen-solver's branches (eig2_top / eig3_top), bit 4 S clamped to 1
+constexpr uint32_t kDecSide = 1u << 21;
+MPCD_HD uint32_t lc_decision_word(int sampler, bool redraw, bool neg, int anneal, int accepted) {
+  const uint32_t a = (uint32_t)min(max(anneal, 0), 255), k = (uint32_t)min(max(accepted, 0), 255);
+  return 1u | ((uint32_t)sampler << 1) | (redraw ? 8u : 0u) | (neg ? 16u : 0u) | (a << 5) | (k << 13);
+}
+MPCD_HD uint32_t cell_decision_word(int br, bool clamped) { return 32u | (uint32_t)(br & 15) | (clamped ? 16u : 0u); }
 
 // Tangent of the FP64 state along one direction in parameter space (forward
 // mode, mpcd_gpu_tangent): per slot of the current buffer the derivatives of
@@ -134,6 +156,9 @@ constexpr int kFlagCellCap = 1;  // a cell above the fused kernel's LDS capacity
 constexpr int kFlagOutside = 2;  // initial positions outside the box
 constexpr int kFlagPoison = 4;   // a bucket overflowed: the bucketed steps from then on are no-ops
 constexpr int kFlagBadCell = 8;  // a particle outside its bucket's tile (internal error)
+constexpr int kFlagEscape = 16;  // escape repair ran out of position tries
+constexpr int kFlagPushes = 32;  // a particle needed more re-thermalisations than its injected entry holds
+constexpr int kFlagStuck = 64;   // a particle was re-thermalised 1000 times in one step (walls leave no room)
 
 // hot fields of an axis-aligned planar wall for the per-particle wall test:
 // the normal's one nonzero component A (0 when its axis is not a box axis),
@@ -203,6 +228,73 @@ __device__ __forceinline__ void shift_back(const RepDev &R, const double *s, F q
     q[j] += sg;
     if (q[j] < 0.0 && q[j] >= sg) q[j] += R.L[j];
   }
+}
+
+// push() (init.c:1261-1293): a fresh velocity from the species' distribution
+// (RANDVEL 0, HOMO 2, HOMOAXIS 3, GAUSS 4; READ is refused at creation),
+// from variates k0 .. k0 + dim - 1 of the particle's re-thermalisation draws
+__device__ __forceinline__ void push_velocity(const mpcd_gpu_species &sp, double kbt, int dim, const Draws &Dr,
+                                              uint32_t rep, size_t gp, uint32_t pid, int k0, double V[3]) {
+  if (sp.vdist == 0) {
+    for (int d = 0; d < dim; d++) V[d] = sqrt(kbt / sp.mass) * (2. * Dr.reth_u(rep, gp, pid, k0 + d) - 1.);
+  } else if (sp.vdist == 3) {
+    for (int d = 0; d < dim; d++) {
+      V[d] = sqrt((double)dim * kbt / sp.mass);
+      V[d] *= (Dr.reth_u(rep, gp, pid, k0 + d) <= 0.5) ? -1.0 : 1.0;  // genrand_pmOne, rand.c:378
+    }
+  } else {
+    for (int d = 0; d < dim; d++) V[d] = sqrt(kbt / sp.mass) * Dr.reth_n(rep, gp, pid, k0 + d);
+    if (sp.vdist == 2) {
+      normn(V, dim);
+      const double nrm = sqrt((double)dim * kbt / sp.mass);
+      for (int d = 0; d < dim; d++) V[d] *= nrm;
+    }
+  }
+}
+
+// bin()'s escape repair (mpc.c:970-982) of particle pid of replica R in
+// binning b (0: the shifted bin, walls at their shifted positions wqs;
+// 1: the unshifted bin after streaming, walls at wqb):
+//  - replacePos_WithCheck (init.c:1462-1483): a uniform position in the box
+//    (replace, init.c:1240), redrawn while any wall of the replica has
+//    calcW <= TOL there;
+//  - a Maxwell velocity, genrand_gaussMB whatever the species' VDIST;
+//  - for nematics, an orientation genrand_coneNP(U, pi, DIM) (rand.c:397-425).
+// The particle stays in the list it was in; the caller keeps its cell.
+// Returns false when the tries ran out (the injected entry's, or
+// kEscMaxTries: walls that leave no room in the box).  The bucketed step
+// kernel does not call it (its preparation kernel re-places listed escapes).
+__device__ __forceinline__ bool escape_repair(const RepDev &R, const WallHot *wh, int b, const Draws &Dr, uint32_t rep,
+                                              uint32_t pid, double mass, bool lc, double q[3], double v[3],
+                                              double u[3]) {
+  const size_t gp = (size_t)R.part_off + pid;
+  const int dim = R.dim;
+  const int tries = Dr.esc_keyed() ? (kEscStride - kEscPos0) / dim : kEscMaxTries;
+  bool ok = false;
+  for (int t = 0; t < tries && !ok; t++) {
+    for (int j = 0; j < dim; j++) q[j] = (double)R.L[j] * Dr.esc_u(rep, b, gp, pid, kEscPos0 + dim * t + j);
+    ok = true;
+    for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+      const WallHot &H = wh[w];
+      if (wall_W_hot(H, q, b ? H.wqb : H.wqs) <= kTOL) ok = false;
+    }
+  }
+  for (int j = 0; j < dim; j++) v[j] = sqrt(R.kbt / mass) * Dr.esc_n(rep, b, gp, pid, j);
+  if (lc) {
+    if (dim == 3) {
+      const double ct = cos(kPI);
+      const double z = Dr.esc_u(rep, b, gp, pid, 3) * (1.0 - ct) + ct;
+      const double phi = Dr.esc_u(rep, b, gp, pid, 4) * 2.0 * kPI;
+      u[0] = z;
+      u[1] = sqrt(1.0 - z * z) * cos(phi);
+      u[2] = sqrt(1.0 - z * z) * sin(phi);
+    } else {
+      const double phi = kPI * (1. - 2. * Dr.esc_u(rep, b, gp, pid, 3));
+      u[0] = cos(phi);
+      u[1] = sin(phi);
+    }
+  }
+  return ok;
 }
 
 // ---------------------------------------------------------------------------
@@ -451,16 +543,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(Ctx C, State<Real> S, State<R
   load_q(S, src, q);
   double v[3] = {(double)S.vx[src], (double)S.vy[src], (double)S.vz[src]};
   int id = S.id[src], sp = S.sp[src];
+  bool esc = false;
+  double u[3] = {0.0, 0.0, 0.0};
   if (shifted) {
     shift_fwd(R, &C.shift[B.rep * 3], q);
     if (cell_of(R, q, 1) < 0) {
-      // escape repair (mpc.c:970-982): fresh uniform position and Maxwell velocity
+      // escape repair (mpc.c:970-982); the particle stays in its list
       atomicAdd(&C.events[0], 1ull);
-      double m = C.sp[sp].mass;
-      for (int j = 0; j < R.dim; j++) {
-        q[j] = Dr.rng.uniform(B.rep, P_ESCAPE, (uint32_t)id, j) * R.L[j];
-        v[j] = sqrt(R.kbt / m) * Dr.rng.normal(B.rep, P_ESCAPE, (uint32_t)id, 16 + j);
-      }
+      esc = true;
+      if (!escape_repair(R, C.wh, 0, Dr, B.rep, (uint32_t)id, C.sp[sp].mass, lc != 0, q, v, u))
+        atomicOr(&C.flags[0], kFlagEscape);
     }
   }
   D.x[t] = (Real)q[0];
@@ -470,9 +562,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(Ctx C, State<Real> S, State<R
   D.vy[t] = (Real)v[1];
   D.vz[t] = (Real)v[2];
   if (lc) {
-    D.ux[t] = S.ux[src];
-    D.uy[t] = S.uy[src];
-    D.uz[t] = S.uz[src];
+    D.ux[t] = esc ? (Real)u[0] : S.ux[src];
+    D.uy[t] = esc ? (Real)u[1] : S.uy[src];
+    D.uz[t] = esc ? (Real)u[2] : S.uz[src];
   }
   D.id[t] = id;
   D.sp[t] = sp;
@@ -587,15 +679,18 @@ __device__ void cellprops_cell(const Ctx &C, const State<Real> &S, int rep, int 
     for (int i = 0; i < dim; i++)
       for (int j = i + 1; j < dim; j++) Sm[j][i] = Sm[i][j];
     F Sv, dir[3] = {0., 0., 0.};
+    int br = 0;
     if (dim == 3) {
       F ev[3];
-      eig3_top(Sm, ev, dir);
+      eig3_top(Sm, ev, dir, &br);
       Sv = -1. * (ev[1] + ev[2]);
     } else {
       F m2[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
-      Sv = eig2_top(m2, dir);
+      Sv = eig2_top(m2, dir, &br);
     }
+    const bool clamped = Sv > 1.0;
     if (Sv > 1.0) Sv = 1.0;
+    if (C.cdec) C.cdec[c] = cell_decision_word(br, clamped);
     A::st(C.cell.S, T.S, c, Sv);
     for (int i = 0; i < dim; i++) A::st(C.cell.dir, T.dir, c * 3 + i, dir[i]);
     double avM = 0.0;
@@ -623,15 +718,18 @@ __device__ __forceinline__ void order_of_cell(Ctx &C, int c, double Sm[3][3], in
   for (int i = 0; i < DIM; i++)
     for (int j = i + 1; j < DIM; j++) Sm[j][i] = Sm[i][j];
   double Sv, dir[3] = {0., 0., 0.};
+  int br = 0;
   if (DIM == 3) {
     double ev[3];
-    eig3_top(Sm, ev, dir);
+    eig3_top(Sm, ev, dir, &br);
     Sv = -1. * (ev[1] + ev[2]);
   } else {
     double m2[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
-    Sv = eig2_top(m2, dir);
+    Sv = eig2_top(m2, dir, &br);
   }
+  const bool clamped = Sv > 1.0;
   if (Sv > 1.0) Sv = 1.0;
+  if (C.cdec) C.cdec[c] = cell_decision_word(br, clamped);
   C.cell.S[c] = Sv;
   for (int i = 0; i < DIM; i++) C.cell.dir[c * 3 + i] = dir[i];
 }
@@ -972,13 +1070,15 @@ __global__ void k_avorder_final(Ctx C, const BlkDev *blk, int nblk, const double
 }
 
 // Maier-Saupe samplers (lc.c:52-411); `slot` keys the draws
+// `dec` receives the decision word of the draw (lc_decision_word)
 template <typename F>
 __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32_t slot, const F *DIR, F *rotAx,
-                          const F &rotAngle, F *U, const F &KBT, const F &S, const F &effM) {
+                          const F &rotAngle, F *U, const F &KBT, const F &S, const F &effM, uint32_t &dec) {
   if (R.dim == 2) {
     F newU[3] = {U[0], U[1], 0.0};
     F ang0 = signed_angle(DIR, U, 2);
-    if (feq(ang0, 0.5 * kPI) || feq(ang0, -0.5 * kPI)) ang0 = kPI * (0.5 - Dr.lc_u(rep, slot, 0));
+    const bool redraw = feq(ang0, 0.5 * kPI) || feq(ang0, -0.5 * kPI);
+    if (redraw) ang0 = kPI * (0.5 - Dr.lc_u(rep, slot, 0));
     F uz0 = dotn(DIR, U, 2);
     int annealNum = (int)val(kMCINT + kMCSLOPE * effM * S / KBT);
     int cnt = 0;
@@ -1000,6 +1100,7 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
       U[0] = newU[0];
       U[1] = newU[1];
     }
+    dec = lc_decision_word(0, redraw, false, annealNum, cnt);
     return;
   }
   F BUS = effM * S / KBT;
@@ -1015,6 +1116,7 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
     rodrigues(newU, rotAx, rotAngle);
     for (int d = 0; d < 3; d++) U[d] = newU[d];
+    dec = lc_decision_word(1, false, val(pm) < 0.0, 0, 0);
   } else if (BUS < kBUSMIN) {
     F B2 = BUS;
     if (B2 < kTOL) B2 = kTOL;
@@ -1030,6 +1132,7 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
     newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
     rodrigues(newU, rotAx, rotAngle);
     for (int d = 0; d < 3; d++) U[d] = newU[d];
+    dec = lc_decision_word(2, false, val(pm) < 0.0, 0, 0);
   } else {
     F uz0 = dotn(DIR, U, 3);
     int annealNum = (int)val(kMCINT + kMCSLOPE * effM * S / KBT);
@@ -1053,6 +1156,7 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
       rodrigues(newU, rotAx, rotAngle);
       for (int d = 0; d < 3; d++) U[d] = newU[d];
     }
+    dec = lc_decision_word(3, false, val(pm) < 0.0, annealNum, cnt);
   }
 }
 
@@ -1072,7 +1176,10 @@ __global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDe
   const RepDev &R = C.rep[B.rep];
   const int c = S.cst[t];
   const int b = C.start[c], e = C.start[c + 1];
-  if (e - b <= 1) return;
+  if (e - b <= 1) {
+    if (C.pdec) C.pdec[R.part_off + S.id[t]] = 0u;
+    return;
+  }
   const int dim = R.dim;
   F DIR[3] = {0., 0., 0.}, rotAx[3], xaxis[3] = {1.0, 0.0, 0.0};
   F Sv = (R.lc == MPCD_LCL) ? A::ld(C.cell.S, T.S, c) : F(C.sg[B.rep]);
@@ -1086,7 +1193,9 @@ __global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDe
   {
     F U[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)}, Tq[3], dU[3] = {0., 0., 0.};
     for (int i = 0; i < dim; i++) dU[i] = U[i];
-    ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, F(R.kbt), Sv, F(effM));
+    uint32_t dec = 0;
+    ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, F(R.kbt), Sv, F(effM), dec);
+    if (C.pdec) C.pdec[R.part_off + S.id[t]] = dec;
     for (int i = 0; i < dim; i++) dU[i] = (U[i] - dU[i]) / R.dt;
     cross3(U, dU, Tq);
     for (int i = 0; i < 3; i++) Tq[i] *= sp.rfc;
@@ -1293,6 +1402,7 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
             for (int j = 0; j < dim; j++) pW += PA[j] * (q[j] - cm[j]);
             pW -= 0.0;
             double pm = (pW <= 0) ? -1. : 1.;
+            if (C.pdec && pm > 0.0) C.pdec[R.part_off + S.id[t]] |= kDecSide;
             for (int j = 0; j < dim; j++) AS[j] += sp.mass * (pm * ACT * PA[j]);
           }
         }
@@ -1489,6 +1599,7 @@ __global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S,
         for (int jj = 0; jj < DIM; jj++) pW += PA[jj] * (q[jj] - cm[jj]);
         pW -= 0.0;
         pm = (pW <= 0) ? -1. : 1.;
+        if (C.pdec && pm > 0.0) C.pdec[R.part_off + S.id[t]] |= kDecSide;
       }
       double nrm[3];
       Dr.part_n_all(Tl.rep, t, DIM, nrm);
@@ -1659,17 +1770,63 @@ __global__ __launch_bounds__(kLcThreads) void k_collide_lc_over(Ctx C, State<Rea
 // 11. thermostat velocity scaling (scaleT, therm.c:52-143); the factor comes
 //     from k_temp_reduce
 // ---------------------------------------------------------------------------
+// one particle's term of TEMP (therm.c:209-235): m |v - AVV|^2 over all three
+// components, in the reference's order
+__device__ __forceinline__ double temp_term(double m, const double v[3], const double *avv) {
+  double tmp = (v[0] - avv[0]) * (v[0] - avv[0]);
+  tmp += (v[1] - avv[1]) * (v[1] - avv[1]);
+  tmp += (v[2] - avv[2]) * (v[2] - avv[2]);
+  return m * tmp;
+}
+
+// the sum of x over a workgroup of kBlock threads, written by thread 0 to out
+__device__ __forceinline__ void block_sum_to(double x, double *out) {
+  __shared__ double s_w[kBlock / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; w++) t += s_w[w];
+    *out = t;
+  }
+}
+
+// The thermostat's velocity scaling (scaleT, therm.c:52-143), with the
+// factor from k_temp_reduce, and the block's term of the temperature the
+// reference recomputes after it (mpc.c:4760, 4773) into C.tpart.
 template <typename Real>
-__global__ void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
+__global__ __launch_bounds__(kBlock) void k_scale(Ctx C, State<Real> S, const BlkDev *blk) {
   const BlkDev B = blk[blockIdx.x];
-  int t = B.beg + threadIdx.x;
-  if (t >= B.end) return;
-  double g = C.gscale[B.rep];
-  if (g == 1.0) return;
+  const int t = B.beg + threadIdx.x;
   const RepDev &R = C.rep[B.rep];
-  S.vx[t] = (Real)((double)S.vx[t] * g);
-  S.vy[t] = (Real)((double)S.vy[t] * g);
-  if (R.dim == 3) S.vz[t] = (Real)((double)S.vz[t] * g);
+  double ke = 0.0;
+  if (t < B.end) {
+    const double g = C.gscale[B.rep];
+    double v[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]};
+    if (g != 1.0) {
+      for (int d = 0; d < R.dim; d++) v[d] = (double)(Real)(v[d] * g);
+      S.vx[t] = (Real)v[0];
+      S.vy[t] = (Real)v[1];
+      if (R.dim == 3) S.vz[t] = (Real)v[2];
+    }
+    ke = temp_term(C.sp[S.sp[t]].mass, v, R.avv);
+  }
+  block_sum_to(ke, &C.tpart[blockIdx.x]);
+}
+
+// KBTNOW after the velocity scaling: the sum of k_scale's (or
+// k_stream_next's) block terms of each replica, blocks [rg.x, rg.y)
+__global__ __launch_bounds__(kBlock) void k_temp_final(Ctx C, const int2 *ranges) {
+  const int r = blockIdx.x;
+  const RepDev &R = C.rep[r];
+  const int2 rg = ranges[r];
+  double acc = 0.0;
+  for (int b = rg.x + threadIdx.x; b < rg.y; b += kBlock) acc += C.tpart[b];
+  __shared__ double s_out;
+  block_sum_to(acc, &s_out);
+  if (threadIdx.x == 0) C.kbtnow[r] = s_out / ((double)(R.npart * R.dim));
 }
 
 // ---------------------------------------------------------------------------
@@ -1781,18 +1938,24 @@ __global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const B
       } else {
         atomicAdd(&C.events[3], 1ull);
         mul[0] = mul[1] = mul[2] = __builtin_nanf("");
-        const uint32_t pid = (uint32_t)S.id[t];
-        for (int j = 0; j < dim; j++) {
-          if (sp.vdist == 0) V[j] = sqrt(R.kbt / sp.mass) * (2. * Dr.rng.uniform(B.rep, P_RETHERM, pid, rk + j) - 1.);
-          else V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(B.rep, P_RETHERM, pid, rk + j);
+        if (Dr.reth_keyed() && rk + dim > kRethStride) {
+          atomicOr(&C.flags[0], kFlagPushes);
+          break;
         }
+        const uint32_t pid = (uint32_t)S.id[t];
+        double Vn[3] = {0.0, 0.0, 0.0};
+        push_velocity(sp, R.kbt, dim, Dr, B.rep, (size_t)R.part_off + pid, pid, rk, Vn);
+        for (int j = 0; j < dim; j++) V[j] = Vn[j];
         rk += 3;
         t_left = t_step;
         t_coll = 0.0;
         cnt2++;
         cnt = 0;
       }
-      if (cnt2 > 1000) break;  // a particle that can never leave the walls
+      if (cnt2 > 1000) {  // a particle the walls leave no room for: the reference would loop for ever
+        atomicOr(&C.flags[0], kFlagStuck);
+        break;
+      }
     }
   }
   if (flagCNT) atomicAdd(&C.events[1], 1ull);
@@ -1803,9 +1966,12 @@ __global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const B
     atomicAdd(&C.events[0], 1ull);
     mul[0] = mul[1] = mul[2] = __builtin_nanf("");
     const uint32_t pid = (uint32_t)S.id[t];
-    for (int j = 0; j < dim; j++) {
-      Q[j] = Dr.rng.uniform(B.rep, P_ESCAPE, pid, 8 + j) * R.L[j];
-      V[j] = sqrt(R.kbt / sp.mass) * Dr.rng.normal(B.rep, P_ESCAPE, pid, 24 + j);
+    double q[3] = {val(Q[0]), val(Q[1]), val(Q[2])}, v[3], u[3] = {val(U[0]), val(U[1]), val(U[2])};
+    if (!escape_repair(R, C.wh, 1, Dr, B.rep, pid, sp.mass, lc, q, v, u)) atomicOr(&C.flags[0], kFlagEscape);
+    for (int j = 0; j < 3; j++) {
+      Q[j] = q[j];
+      if (j < dim) V[j] = v[j];
+      U[j] = u[j];
     }
     cu = S.cst[t];
   }

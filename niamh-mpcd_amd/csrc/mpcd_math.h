@@ -127,14 +127,17 @@ MPCD_HD void inv3(F inv[3][3], const F m[3][3]) {
 
 // eigenvalues2x2 + eigenvectors2x2, mtools.c:1230-1305.  Returns the largest
 // eigenvalue and writes the first eigenvector (the cell director).
+// *br (when given) receives the branch taken: 0 m10 != 0, 1 m01 != 0,
+// 2 diagonal with equal entries, 3 other diagonal
 template <typename F>
-MPCD_HD F eig2_top(const F m[2][2], F v0[2]) {
+MPCD_HD F eig2_top(const F m[2][2], F v0[2], int *br = nullptr) {
   F det = m[0][0] * m[1][1] - m[1][0] * m[0][1];
   F tr = m[0][0] + m[1][1];
   F sq = sqrt(tr * tr * 0.25 - det);
   F e0 = tr * 0.5 + sq;
   v0[0] = 0.0;
   v0[1] = 0.0;
+  if (br) *br = fneq(m[1][0], 0.0) ? 0 : fneq(m[0][1], 0.0) ? 1 : feq(m[0][0], m[1][1]) ? 2 : 3;
   if (fneq(m[1][0], 0.0)) {
     v0[0] = e0 - m[1][1];
     v0[1] = m[1][0];
@@ -161,10 +164,15 @@ MPCD_HD F eig2_top(const F m[2][2], F v0[2]) {
 // eigenvalues3x3 (Smith's trigonometric method) + eigenvectors3x3
 // (Cayley-Hamilton column products), mtools.c:1307-1430.  Writes all three
 // eigenvalues (descending by construction) and the first eigenvector.
+// *br (when given) receives the branches taken: bits 0-1 the column of the
+// eigenvector product used (3: none), bit 2 the diagonal branch, bit 3 the
+// clamp of r
 template <typename F>
-MPCD_HD void eig3_top(const F m[3][3], F ev[3], F v0[3]) {
+MPCD_HD void eig3_top(const F m[3][3], F ev[3], F v0[3], int *br = nullptr) {
   F p1 = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+  int b = 0;
   if (feq(p1, 0.0)) {
+    b |= 4;
     F q;
     for (int i = 0; i < 3; i++) ev[i] = m[i][i];
     if (ev[1] > ev[0]) { q = ev[0]; ev[0] = ev[1]; ev[1] = q; }
@@ -184,6 +192,7 @@ MPCD_HD void eig3_top(const F m[3][3], F ev[3], F v0[3]) {
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) B[i][j] *= ip;
     F r = det3(B) * 0.5, phi;
+    if (r <= -1. || r >= 1.) b |= 8;
     if (r <= -1.) phi = kPI / 3.;
     else if (r >= 1.) phi = 0.;
     else phi = acos(r) / 3.;
@@ -192,19 +201,24 @@ MPCD_HD void eig3_top(const F m[3][3], F ev[3], F v0[3]) {
     ev[1] = 3. * q - ev[0] - ev[2];
   }
   // first eigenvector: product (m - e1)(m - e2), first non-zero column
+  int used = 3;
   for (int col = 0; col < 3; col++) {
     for (int row = 0; row < 3; row++) v0[row] = 0.;
     for (int row = 0; row < 3; row++)
       for (int i = 0; i < 3; i++) {
         F a = m[row][i];
         if (i == row) a -= ev[1];
-        F b = m[i][col];
-        if (i == col) b -= ev[2];
-        v0[row] += a * b;
+        F bb = m[i][col];
+        if (i == col) bb -= ev[2];
+        v0[row] += a * bb;
       }
-    if (!(feq(v0[0], 0.0) && feq(v0[1], 0.0) && feq(v0[2], 0.0))) break;
+    if (!(feq(v0[0], 0.0) && feq(v0[1], 0.0) && feq(v0[2], 0.0))) {
+      used = col;
+      break;
+    }
   }
   normn(v0, 3);
+  if (br) *br = b | used;
 }
 
 // ---------------------------------------------------------------------------
@@ -279,7 +293,19 @@ struct Rng {
   }
 };
 
-enum Purpose { P_SHIFT = 1, P_CELL = 2, P_GHOST = 3, P_NORMAL = 4, P_LC = 5, P_ESCAPE = 6, P_RETHERM = 7 };
+// P_ESCAPE: escape repair in the step's shifted bin; P_ESCAPE2: in its
+// unshifted bin after streaming
+enum Purpose { P_SHIFT = 1, P_CELL = 2, P_GHOST = 3, P_NORMAL = 4, P_LC = 5, P_ESCAPE = 6, P_RETHERM = 7,
+               P_ESCAPE2 = 8 };
+
+// Keyed layout of the rare draws (the same numbers as MPCD_ESC_STRIDE and
+// MPCD_RETHERM_STRIDE of the C-ABI).  Escape repair, per particle and
+// binning: [0..2] Maxwell normals, [3..4] orientation uniforms, then DIM
+// position uniforms per try from kEscPos0.  The generator serves index k of
+// a normal as normal(k) (counter blocks 0 and 1) and of a uniform as
+// uniform(k), k >= 3, so no counter block feeds two variates.
+constexpr int kEscStride = 32, kEscPos0 = 5, kEscMaxTries = 65536;
+constexpr int kRethStride = 24;
 
 // Draw source seen by the kernels: the counter-based generator or injected
 // arrays in the keyed layout of include/mpcd_gpu.h.
@@ -288,6 +314,10 @@ struct Draws {
   int injected;
   const double *shift, *cell, *ghost, *part, *lc;
   int lc_stride;
+  // escape and re-thermalisation arrays (null: the generator serves them
+  // even when the rest is injected) and the particles of the context
+  const double *esc, *reth;
+  int ntot;
   // graph replays: the step number lives in device memory (step_dev) and a
   // launch is step_off steps after it; eager launches carry rng.step
   const uint32_t *step_dev;
@@ -328,6 +358,25 @@ struct Draws {
   }
   MPCD_HD double lc_n(uint32_t rep, uint32_t slot, int k) const {
     return injected ? lc[(size_t)slot * lc_stride + k] : rng.normal(rep, P_LC, slot, k);
+  }
+  // escape draws of particle pid (global index gp = replica offset + pid) in
+  // binning b (0 shifted, 1 unshifted); k < 3 normals, k >= 3 uniforms
+  MPCD_HD bool esc_keyed() const { return injected && esc; }
+  MPCD_HD double esc_n(uint32_t rep, int b, size_t gp, uint32_t pid, int k) const {
+    return esc_keyed() ? esc[((size_t)b * ntot + gp) * kEscStride + k]
+                       : rng.normal(rep, b ? P_ESCAPE2 : P_ESCAPE, pid, k);
+  }
+  MPCD_HD double esc_u(uint32_t rep, int b, size_t gp, uint32_t pid, int k) const {
+    return esc_keyed() ? esc[((size_t)b * ntot + gp) * kEscStride + k]
+                       : rng.uniform(rep, b ? P_ESCAPE2 : P_ESCAPE, pid, k);
+  }
+  // variate k = 3r + j of the particle's r-th wall re-thermalisation
+  MPCD_HD bool reth_keyed() const { return injected && reth; }
+  MPCD_HD double reth_n(uint32_t rep, size_t gp, uint32_t pid, int k) const {
+    return reth_keyed() ? reth[gp * kRethStride + k] : rng.normal(rep, P_RETHERM, pid, k);
+  }
+  MPCD_HD double reth_u(uint32_t rep, size_t gp, uint32_t pid, int k) const {
+    return reth_keyed() ? reth[gp * kRethStride + k] : rng.uniform(rep, P_RETHERM, pid, k);
   }
 };
 

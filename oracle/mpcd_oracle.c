@@ -67,6 +67,10 @@ struct orc_sys {
     /* scratch */
     int *tmp_order, *newcell, *tmp_cnt, *tmp_seen;
     int *slot_of; /* particle -> slot in the collision order */
+    /* discrete decisions of the last collision phase (nematics; the words of
+     * orc_lc_decisions) and whether local_prop records the cell words */
+    uint32_t *pdec, *cdec;
+    int rec_cdec;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -200,9 +204,10 @@ static void matvec3(double M[3][3], const double *v, double *r)        /* mtools
 
 /* eigen-solvers, mtools.c:1230-1476.  m is row-major; on return the rows of
  * m are the (normalised) eigenvectors, eigval[0] the largest eigenvalue. */
-static void eig2(double m[2][2], double ev[2])
+static int eig2(double m[2][2], double ev[2])
 {
     double vec[2][2];
+    int br;
     double det = m[0][0] * m[1][1] - m[1][0] * m[0][1];   /* determinant(), mtools.c:900 */
     double tr = m[0][0] + m[1][1];
     double sq = sqrt(tr * tr * 0.25 - det);
@@ -210,6 +215,7 @@ static void eig2(double m[2][2], double ev[2])
     ev[0] = tr * 0.5 + sq;
     ev[1] = tr * 0.5 - sq;
     vec[0][0] = vec[0][1] = vec[1][0] = vec[1][1] = 0.0;
+    br = fneq(m[1][0], 0.0) ? 0 : fneq(m[0][1], 0.0) ? 1 : feq(m[0][0], m[1][1]) ? 2 : 3;
     if (fneq(m[1][0], 0.0)) {
         vec[0][0] = ev[0] - m[1][1];
         vec[0][1] = m[1][0];
@@ -239,15 +245,17 @@ static void eig2(double m[2][2], double ev[2])
     }
     for (i = 0; i < 2; i++)
         for (j = 0; j < 2; j++) m[i][j] = vec[i][j];
+    return br;
 }
 
-static void eig3(double m[3][3], double ev[3])
+static int eig3(double m[3][3], double ev[3])
 {
     double vec[3][3];
     double q, p1, p2, p, ip, r, phi, B[3][3];
-    int i, j, k, row, col;
+    int i, j, k, row, col, br = 0;
     p1 = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
     if (feq(p1, 0.0)) {
+        br |= 4;
         for (i = 0; i < 3; i++) ev[i] = m[i][i];
         if (ev[1] > ev[0]) { q = ev[0]; ev[0] = ev[1]; ev[1] = q; }
         if (ev[2] > ev[0]) { q = ev[0]; ev[0] = ev[2]; ev[2] = q; }
@@ -265,6 +273,7 @@ static void eig3(double m[3][3], double ev[3])
         for (i = 0; i < 3; i++)
             for (j = 0; j < 3; j++) B[i][j] *= ip;
         r = det3x3(B) * 0.5;
+        if (r <= -1. || r >= 1.) br |= 8;
         if (r <= -1.) phi = ORC_PI / 3.;
         else if (r >= 1.) phi = 0.;
         else phi = acos(r) / 3.;
@@ -276,6 +285,7 @@ static void eig3(double m[3][3], double ev[3])
     for (k = 0; k < 3; k++) {
         double ea = (k == 0) ? ev[1] : ev[0];
         double eb = (k == 2) ? ev[1] : ev[2];
+        if (k == 0) br |= 3; /* the column of the first eigenvector; 3: none */
         for (col = 0; col < 3; col++) {
             for (row = 0; row < 3; row++) vec[k][row] = 0.;
             for (row = 0; row < 3; row++)
@@ -286,12 +296,16 @@ static void eig3(double m[3][3], double ev[3])
                     if (i == col) b -= eb;
                     vec[k][row] += a * b;
                 }
-            if (!(feq(vec[k][0], 0.0) && feq(vec[k][1], 0.0) && feq(vec[k][2], 0.0))) break;
+            if (!(feq(vec[k][0], 0.0) && feq(vec[k][1], 0.0) && feq(vec[k][2], 0.0))) {
+                if (k == 0) br = (br & ~3) | col;
+                break;
+            }
         }
     }
     for (k = 0; k < 3; k++) vnorm(vec[k], 3);
     for (i = 0; i < 3; i++)
         for (j = 0; j < 3; j++) m[i][j] = vec[i][j];
+    return br;
 }
 
 void orc_eigen(double *m, int dim, double *eigval)
@@ -393,11 +407,47 @@ static double *rec_base(const orc_draws *d, int purpose, int *stride)
     }
 }
 
-static void record(const orc_draws *d, int purpose, long key, int k, double x)
+/* the rare draws' keyed entry: escape repair of binning 0 (ORC_P_ESCAPE) or
+ * 1 (ORC_P_ESCAPE2), re-thermalisation (ORC_P_RETHERM); NULL when absent */
+static double *rare_base(const orc_draws *d, int purpose, int *stride)
+{
+    if ((purpose == ORC_P_ESCAPE || purpose == ORC_P_ESCAPE2) && d->escape) {
+        *stride = ORC_ESC_STRIDE;
+        return d->escape + (purpose == ORC_P_ESCAPE2 ? (size_t)d->rare_n * ORC_ESC_STRIDE : 0);
+    }
+    if (purpose == ORC_P_RETHERM && d->retherm) {
+        *stride = ORC_RETH_STRIDE;
+        return d->retherm;
+    }
+    *stride = 0;
+    return NULL;
+}
+
+static void record(orc_draws *d, int purpose, long key, int k, double x)
 {
     int stride;
     double *b = rec_base(d, purpose, &stride);
-    if (b && k < stride) b[key * stride + k] = x;
+    if (!b) b = rare_base(d, purpose, &stride);
+    if (!b) return;
+    if (k < stride) b[key * stride + k] = x;
+    else if (purpose >= ORC_P_ESCAPE) d->rare_overflow = 1;
+}
+
+/* keyed read of a rare draw; returns 0 when the generator serves it */
+static int rare_keyed(orc_draws *d, int purpose, long key, int k, double *x)
+{
+    int stride;
+    const double *b;
+    if (d->mode != ORC_DRAW_KEYED || purpose < ORC_P_ESCAPE) return 0;
+    b = rare_base(d, purpose, &stride);
+    if (!b) return 0;
+    if (k >= stride) {
+        d->rare_overflow = 1;
+        *x = 0.5;
+        return 1;
+    }
+    *x = b[key * stride + k];
+    return 1;
 }
 
 /* uniform on [0,1) -- the reference's genrand_real() (rand.c:365) */
@@ -414,6 +464,7 @@ static double dr_uniform(orc_sys *s, orc_draws *d, int purpose, long key, int k)
         const double *b = keyed_base(d, purpose, &stride);
         return b[key * stride + k];
     }
+    if (rare_keyed(d, purpose, key, k, &x)) return x;
     return orc_philox_uniform(d->seed, d->replica, (uint32_t)s->step, purpose, (uint32_t)key, (uint32_t)k);
 }
 
@@ -431,6 +482,7 @@ static double dr_normal(orc_sys *s, orc_draws *d, int purpose, long key, int k)
         const double *b = keyed_base(d, purpose, &stride);
         return b[key * stride + k];
     }
+    if (rare_keyed(d, purpose, key, k, &x)) return x;
     return orc_philox_normal(d->seed, d->replica, (uint32_t)s->step, purpose, (uint32_t)key, (uint32_t)k);
 }
 
@@ -514,6 +566,8 @@ orc_sys *orc_create(const orc_params *p, int n)
     s->newcell = (int *)calloc((size_t)n, sizeof(int));
     s->cellof = (int *)calloc((size_t)n, sizeof(int));
     s->slot_of = (int *)calloc((size_t)n, sizeof(int));
+    s->pdec = (uint32_t *)calloc((size_t)n, sizeof(uint32_t));
+    s->cdec = (uint32_t *)calloc((size_t)s->nc, sizeof(uint32_t));
     s->cstart = (int *)calloc((size_t)s->nc + 1, sizeof(int));
     s->tmp_cnt = (int *)calloc((size_t)s->nc + 1, sizeof(int));
     s->tmp_seen = (int *)calloc((size_t)s->nc + 1, sizeof(int));
@@ -543,7 +597,7 @@ void orc_destroy(orc_sys *s)
     free(s->order); free(s->tmp_order); free(s->newcell); free(s->cellof); free(s->slot_of);
     free(s->cstart); free(s->tmp_cnt); free(s->tmp_seen);
     free(s->pop); free(s->spcnt); free(s->mass); free(s->vcm); free(s->flow); free(s->cm); free(s->I); free(s->E);
-    free(s->S); free(s->dir);
+    free(s->S); free(s->dir); free(s->pdec); free(s->cdec);
     free(s);
 }
 
@@ -603,21 +657,47 @@ static int binin(orc_sys *s)
 
 static void push_velocity(orc_sys *s, orc_draws *d, int i, int purpose, int kbase);
 
-/* replacePos_WithCheck (init.c:1462) for the axis-aligned planar walls here:
- * redraw a uniform position in the box until no wall has W <= TOL. */
-static void replace_escaped(orc_sys *s, orc_draws *d, int i, int *k)
+/* bin()'s escape repair, mpc.c:970-982, of particle i in binning b (0: the
+ * shifted bin, 1: the unshifted bin after streaming): replacePos_WithCheck
+ * (init.c:1462-1483) redraws a uniform position in the box (replace,
+ * init.c:1240-1243) while any wall has calcW_PLANE <= TOL at the walls'
+ * current positions; then DIM Maxwell velocities (genrand_gaussMB,
+ * rand.c:510, for every VDIST) and, for nematics, genrand_coneNP(U, pi, DIM)
+ * (rand.c:397-425).  Draws are consumed in that order. */
+static void escape_repair(orc_sys *s, orc_draws *d, int i, int b)
 {
-    int flag = 1, w, j;
+    const int purpose = b ? ORC_P_ESCAPE2 : ORC_P_ESCAPE, dim = s->p.dim;
+    const double m = s->p.sp[s->spid[i]].mass;
+    int flag = 1, w, j, t = 0;
     while (flag) {
-        for (j = 0; j < s->p.dim; j++) s->Q[i * 3 + j] = dr_uniform(s, d, ORC_P_ESCAPE, i, (*k)++) * s->p.L[j];
+        for (j = 0; j < dim; j++)
+            s->Q[i * 3 + j] = (double)s->p.L[j] * dr_uniform(s, d, purpose, i, ORC_ESC_POS0 + dim * t + j);
+        t++;
         flag = 0;
         for (w = 0; w < s->p.nwall; w++) {
             const orc_wall *W = &s->p.wall[w];
             double Wv = 0.0;
-            for (j = 0; j < s->p.dim; j++) Wv += W->A[j] * (s->Q[i * 3 + j] - s->wq[w][j]);
+            for (j = 0; j < dim; j++) Wv += W->A[j] * (s->Q[i * 3 + j] - s->wq[w][j]);
             Wv -= W->R;
             if (W->INV) Wv *= -1.0;
             if (Wv <= ORC_TOL) flag = 1;
+        }
+        if (d->rare_overflow) break; /* a keyed entry ran out of tries */
+    }
+    for (j = 0; j < dim; j++) s->V[i * 3 + j] = dr_gaussMB(s, d, s->p.kbt, m, purpose, i, j);
+    if (s->p.lc > LC_ISOF) {
+        double *u = &s->U[i * 3];
+        if (dim == 3) {
+            const double ct = cos(ORC_PI);
+            const double z = dr_uniform(s, d, purpose, i, 3) * (1.0 - ct) + ct;
+            const double phi = dr_uniform(s, d, purpose, i, 4) * 2.0 * ORC_PI;
+            u[0] = z;
+            u[1] = sqrt(1.0 - z * z) * cos(phi);
+            u[2] = sqrt(1.0 - z * z) * sin(phi);
+        } else {
+            const double phi = ORC_PI * (1. - 2. * dr_uniform(s, d, purpose, i, 3));
+            u[0] = cos(phi);
+            u[1] = sin(phi);
         }
     }
 }
@@ -631,26 +711,9 @@ static void bin(orc_sys *s, orc_draws *d, int shifted)
         i = s->order[t];
         c = cell_of_pos(s, &s->Q[i * 3], shifted, &esc);
         if (esc) {
-            /* mpc.c:970-982: re-place, fresh Maxwell velocity (and orientation);
-             * the particle stays in the list it was in */
-            int k = 0, j;
-            double m = s->p.sp[s->spid[i]].mass;
+            /* mpc.c:970-982: the particle stays in the list it was in */
             s->n_escape++;
-            replace_escaped(s, d, i, &k);
-            for (j = 0; j < s->p.dim; j++) s->V[i * 3 + j] = dr_gaussMB(s, d, s->p.kbt, m, ORC_P_ESCAPE, i, 16 + j);
-            if (s->p.lc > LC_ISOF) {
-                double phi = ORC_PI * (1. - 2. * dr_uniform(s, d, ORC_P_ESCAPE, i, 20));
-                if (s->p.dim == 3) {
-                    double z = dr_uniform(s, d, ORC_P_ESCAPE, i, 21) * 2.0 - 1.0;
-                    double ph = dr_uniform(s, d, ORC_P_ESCAPE, i, 22) * 2.0 * ORC_PI;
-                    s->U[i * 3 + 0] = z;
-                    s->U[i * 3 + 1] = sqrt(1.0 - z * z) * cos(ph);
-                    s->U[i * 3 + 2] = sqrt(1.0 - z * z) * sin(ph);
-                } else {
-                    s->U[i * 3 + 0] = cos(phi);
-                    s->U[i * 3 + 1] = sin(phi);
-                }
-            }
+            escape_repair(s, d, i, shifted ? 0 : 1);
             c = s->cellof[i];
         }
         s->newcell[t] = c;
@@ -711,19 +774,21 @@ static void tens_order(orc_sys *s, int c, double Sm[3][3])
 /* S and director from an order tensor, mpc.c:184-207 */
 static void order_from_tensor(orc_sys *s, int c, double Sm[3][3])
 {
-    int dim = s->p.dim, i;
+    int dim = s->p.dim, i, br, clamped;
     double ev[3] = {0, 0, 0};
     if (dim == 3) {
-        eig3(Sm, ev);
+        br = eig3(Sm, ev);
         s->S[c] = -1. * (ev[1] + ev[2]);
     } else {
         double a[2][2] = {{Sm[0][0], Sm[0][1]}, {Sm[1][0], Sm[1][1]}};
-        eig2(a, ev);
+        br = eig2(a, ev);
         Sm[0][0] = a[0][0]; Sm[0][1] = a[0][1];
         s->S[c] = ev[0];
     }
     for (i = 0; i < dim; i++) s->dir[c * 3 + i] = Sm[0][i];
+    clamped = s->S[c] > 1.0;
     if (s->S[c] > 1.0) s->S[c] = 1.0;
+    if (s->rec_cdec) s->cdec[c] = 32u | (uint32_t)(br & 15) | (clamped ? 16u : 0u);
 }
 
 /* velGradD2Q9 (lc.c:2130-2162) and velGradD3Q15 (lc.c:2070-2119), including
@@ -958,14 +1023,25 @@ static void ghost_part(orc_sys *s, orc_draws *d)
 /* ------------------------------------------------------------------------ */
 
 /* genrand_maierSaupeMetropolis_2D, lc.c:341-411 */
-static void ms_metropolis_2d(orc_sys *s, orc_draws *d, long slot, const double *DIR, double *rotAx,
-                             double rotAngle, double *U, double KBT, double S, double effM)
+/* the decision word of a Maier-Saupe draw: bit 0 set, bits 1-2 the sampler,
+ * bit 3 the 2D redraw, bit 4 pm = -1, bits 5-12 annealNum, bits 13-20 the
+ * accepted moves (the GPU library's lc_decision_word) */
+static uint32_t dec_word(int sampler, int redraw, int neg, int anneal, int accepted)
+{
+    uint32_t a = (uint32_t)(anneal < 0 ? 0 : anneal > 255 ? 255 : anneal);
+    uint32_t k = (uint32_t)(accepted < 0 ? 0 : accepted > 255 ? 255 : accepted);
+    return 1u | ((uint32_t)sampler << 1) | (redraw ? 8u : 0u) | (neg ? 16u : 0u) | (a << 5) | (k << 13);
+}
+
+static uint32_t ms_metropolis_2d(orc_sys *s, orc_draws *d, long slot, const double *DIR, double *rotAx,
+                                 double rotAngle, double *U, double KBT, double S, double effM)
 {
     double newU[3] = {0, 0, 0}, R, ang0, uz0, w0, ang1, uz1, w1, dw;
-    int i, dd, annealNum, accept, cnt = 0;
+    int i, dd, annealNum, accept, cnt = 0, redraw;
     for (dd = 0; dd < 2; dd++) newU[dd] = U[dd];
     ang0 = signedAngle(DIR, U, 2);
-    if (feq(ang0, 0.5 * ORC_PI) || feq(ang0, -0.5 * ORC_PI)) ang0 = ORC_PI * (0.5 - dr_uniform(s, d, ORC_P_LC, slot, 0));
+    redraw = feq(ang0, 0.5 * ORC_PI) || feq(ang0, -0.5 * ORC_PI);
+    if (redraw) ang0 = ORC_PI * (0.5 - dr_uniform(s, d, ORC_P_LC, slot, 0));
     uz0 = dotprod(DIR, U, 2);
     annealNum = (int)(ORC_MCINT + ORC_MCSLOPE * effM * S / KBT);
     for (i = 0; i < annealNum; i++) {
@@ -992,11 +1068,12 @@ static void ms_metropolis_2d(orc_sys *s, orc_draws *d, long slot, const double *
         orc_rodrigues(newU, rotAx, rotAngle);
         for (dd = 0; dd < 2; dd++) U[dd] = newU[dd];
     }
+    return dec_word(0, redraw, 0, annealNum, cnt);
 }
 
 /* genrand_maierSaupeMetropolis_3D, lc.c:257-324 */
-static void ms_metropolis_3d(orc_sys *s, orc_draws *d, long slot, const double *DIR, double *rotAx,
-                             double rotAngle, double *U, double KBT, double S, double effM)
+static uint32_t ms_metropolis_3d(orc_sys *s, orc_draws *d, long slot, const double *DIR, double *rotAx,
+                                 double rotAngle, double *U, double KBT, double S, double effM)
 {
     double newU[3], phi, R, uz0, w0, uz1, w1, dw, pm;
     int i, dd, annealNum, accept, cnt = 0;
@@ -1028,11 +1105,12 @@ static void ms_metropolis_3d(orc_sys *s, orc_draws *d, long slot, const double *
         orc_rodrigues(newU, rotAx, rotAngle);
         for (dd = 0; dd < 3; dd++) U[dd] = newU[dd];
     }
+    return dec_word(3, 0, pm < 0.0, annealNum, cnt);
 }
 
 /* genrand_maierSaupeGAUSS_3D, lc.c:85-126 */
-static void ms_gauss_3d(orc_sys *s, orc_draws *d, long slot, double *rotAx, double rotAngle, double *U, double KBT,
-                        double S, double effM)
+static uint32_t ms_gauss_3d(orc_sys *s, orc_draws *d, long slot, double *rotAx, double rotAngle, double *U,
+                            double KBT, double S, double effM)
 {
     double std, phi, theta, uz, pm, newU[3];
     int dd;
@@ -1047,11 +1125,12 @@ static void ms_gauss_3d(orc_sys *s, orc_draws *d, long slot, double *rotAx, doub
     newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
     orc_rodrigues(newU, rotAx, rotAngle);
     for (dd = 0; dd < 3; dd++) U[dd] = newU[dd];
+    return dec_word(1, 0, pm < 0.0, 0, 0);
 }
 
 /* genrand_maierSaupeEXP_3D, lc.c:192-241 */
-static void ms_exp_3d(orc_sys *s, orc_draws *d, long slot, double *rotAx, double rotAngle, double *U, double KBT,
-                      double S, double effM)
+static uint32_t ms_exp_3d(orc_sys *s, orc_draws *d, long slot, double *rotAx, double rotAngle, double *U,
+                          double KBT, double S, double effM)
 {
     double BUS, c, R, num, denom, phi, uz, pm, newU[3];
     int dd;
@@ -1070,6 +1149,7 @@ static void ms_exp_3d(orc_sys *s, orc_draws *d, long slot, double *rotAx, double
     newU[2] = sqrt(1.0 - uz * uz) * sin(phi);
     orc_rodrigues(newU, rotAx, rotAngle);
     for (dd = 0; dd < 3; dd++) U[dd] = newU[dd];
+    return dec_word(2, 0, pm < 0.0, 0, 0);
 }
 
 /* LCcollision, lc.c:427-544 */
@@ -1095,11 +1175,11 @@ static void lc_collision(orc_sys *s, orc_draws *d, int c, double SG)
         for (i = 0; i < dim; i++) dU[i] = U[i];
         if (dim == 3) {
             double BUS = effM * S / p->kbt;
-            if (BUS > ORC_BUSMAX) ms_gauss_3d(s, d, t, rotAx, rotAngle, U, p->kbt, S, effM);
-            else if (BUS < ORC_BUSMIN) ms_exp_3d(s, d, t, rotAx, rotAngle, U, p->kbt, S, effM);
-            else ms_metropolis_3d(s, d, t, DIR, rotAx, rotAngle, U, p->kbt, S, effM);
+            if (BUS > ORC_BUSMAX) s->pdec[pi] = ms_gauss_3d(s, d, t, rotAx, rotAngle, U, p->kbt, S, effM);
+            else if (BUS < ORC_BUSMIN) s->pdec[pi] = ms_exp_3d(s, d, t, rotAx, rotAngle, U, p->kbt, S, effM);
+            else s->pdec[pi] = ms_metropolis_3d(s, d, t, DIR, rotAx, rotAngle, U, p->kbt, S, effM);
         } else {
-            ms_metropolis_2d(s, d, t, DIR, rotAx, rotAngle, U, p->kbt, S, effM);
+            s->pdec[pi] = ms_metropolis_2d(s, d, t, DIR, rotAx, rotAngle, U, p->kbt, S, effM);
         }
         for (i = 0; i < dim; i++) dU[i] = (U[i] - dU[i]) / p->dt;
         crossprod(U, dU, T);
@@ -1324,6 +1404,7 @@ static void rat_collide(orc_sys *s, orc_draws *d, int c, int mode)
             for (j = 0; j < dim; j++) pW += PA[j] * (s->Q[pi * 3 + j] - PQ[j]);
             pW -= 0.0;
             pm = (pW <= 0) ? -1. : 1.;
+            if (pm > 0.0) s->pdec[pi] |= 1u << 21;
             for (j = 0; j < dim; j++) AV[i * 3 + j] = pm * ACT * PA[j];
             for (j = 0; j < dim; j++) AS[j] += sp->mass * AV[i * 3 + j];
         }
@@ -1419,16 +1500,31 @@ static void scale_t(orc_sys *s, int tstech)                            /* therm.
 /* streaming and walls                                                       */
 /* ------------------------------------------------------------------------ */
 
-/* push(), init.c:1261-1293, for the velocity distributions used here */
+/* push(), init.c:1261-1293: RANDVEL (0), HOMO (2), HOMOAXIS (3), GAUSS (4);
+ * READ (1) needs a file and is not on this path */
 static void push_velocity(orc_sys *s, orc_draws *d, int i, int purpose, int kbase)
 {
     const orc_species *sp = &s->p.sp[s->spid[i]];
+    const int dim = s->p.dim;
+    double *V = &s->V[i * 3];
     int j;
     if (sp->vdist == 0) {
-        for (j = 0; j < s->p.dim; j++)
-            s->V[i * 3 + j] = sqrt(s->p.kbt / sp->mass) * (2. * dr_uniform(s, d, purpose, i, kbase + j) - 1.);
+        for (j = 0; j < dim; j++) V[j] = sqrt(s->p.kbt / sp->mass) * (2. * dr_uniform(s, d, purpose, i, kbase + j) - 1.);
+    } else if (sp->vdist == 3) {
+        for (j = 0; j < dim; j++) {
+            V[j] = sqrt(dim * s->p.kbt / sp->mass);
+            V[j] *= (dr_uniform(s, d, purpose, i, kbase + j) <= 0.5) ? -1.0 : 1.0; /* genrand_pmOne, rand.c:378 */
+        }
     } else {
-        for (j = 0; j < s->p.dim; j++) s->V[i * 3 + j] = dr_gaussMB(s, d, s->p.kbt, sp->mass, purpose, i, kbase + j);
+        for (j = 0; j < dim; j++) V[j] = dr_gaussMB(s, d, s->p.kbt, sp->mass, purpose, i, kbase + j);
+        if (sp->vdist == 2) {
+            double l = 0.0, nrm;
+            for (j = 0; j < dim; j++) l += V[j] * V[j];
+            l = sqrt(l);
+            if (fabs(l - 0.0) >= ORC_TOL) for (j = 0; j < dim; j++) V[j] = V[j] / l;   /* norm, mtools.c:246 */
+            nrm = sqrt((double)dim * s->p.kbt / sp->mass);
+            for (j = 0; j < dim; j++) V[j] *= nrm;
+        }
     }
 }
 
@@ -1581,7 +1677,11 @@ int orc_phase_collide(orc_sys *s, orc_draws *d)
     if (fneq(p->grav[0], 0.0) || fneq(p->grav[1], 0.0) || fneq(p->grav[2], 0.0))
         for (i = 0; i < s->N; i++)
             for (j = 0; j < p->dim; j++) s->V[i * 3 + j] = s->V[i * 3 + j] + p->dt * p->grav[j]; /* acc_all */
+    memset(s->pdec, 0, sizeof(uint32_t) * (size_t)s->N);
+    memset(s->cdec, 0, sizeof(uint32_t) * (size_t)s->nc);
+    s->rec_cdec = 1;
     local_prop(s);
+    s->rec_cdec = 0;
     ghost_part(s, d);
     if (p->lc != LC_ISOF) {
         if (p->lc == LC_LCG) s->avs = av_order_param(s);
@@ -1753,6 +1853,22 @@ void orc_draws_set_record(orc_draws *d, double *shift, double *cell, double *gho
     d->rec_shift = shift; d->rec_cell = cell; d->rec_ghost = ghost; d->rec_part = part; d->rec_lc = lc;
     d->rec_lc_stride = lc_stride;
 }
+
+void orc_lc_decisions(const orc_sys *s, uint32_t *part, uint32_t *cell)
+{
+    if (part) memcpy(part, s->pdec, sizeof(uint32_t) * (size_t)s->N);
+    if (cell) memcpy(cell, s->cdec, sizeof(uint32_t) * (size_t)s->nc);
+}
+
+void orc_draws_set_rare(orc_draws *d, double *escape, double *retherm, int n)
+{
+    d->escape = escape;
+    d->retherm = retherm;
+    d->rare_n = n;
+    d->rare_overflow = 0;
+}
+
+int orc_draws_rare_overflow(const orc_draws *d) { return d->rare_overflow; }
 
 void orc_draws_counts(const orc_draws *d, uint64_t *out3)
 {

@@ -37,9 +37,19 @@ enum {
     ORC_P_GHOST = 3,   /* key = cell, stride 3: ghost-particle normals    */
     ORC_P_NORMAL = 4,  /* key = slot, stride 3: thermal normals (AT)      */
     ORC_P_LC = 5,      /* key = slot, stride lc_stride: Maier-Saupe draws */
-    ORC_P_ESCAPE = 6,  /* key = particle, escape repair (rare)            */
-    ORC_P_RETHERM = 7  /* key = particle, wall re-thermalisation (rare)   */
+    ORC_P_ESCAPE = 6,  /* key = particle, escape repair in the shifted bin */
+    ORC_P_RETHERM = 7, /* key = particle, wall re-thermalisation (rare)    */
+    ORC_P_ESCAPE2 = 8  /* key = particle, escape repair in the unshifted bin after streaming */
 };
+
+/* keyed layout of the rare draws (MPCD_ESC_STRIDE / MPCD_RETHERM_STRIDE of
+ * include/mpcd_gpu.h): escape entries per particle and binning hold [0..2]
+ * Maxwell normals, [3..4] orientation uniforms, [5 + DIM*t + j] position
+ * uniform j of try t; re-thermalisation entries hold [3r + j], variate j of
+ * the particle's r-th push of the step */
+#define ORC_ESC_STRIDE 32
+#define ORC_ESC_POS0 5
+#define ORC_RETH_STRIDE 24
 
 typedef struct {
     double mass, rfc, tumble, chihi, act, mfpot, damp, sigwidth, sigpos, minactratio;
@@ -73,6 +83,14 @@ typedef struct {
     /* optional recording of consumed sequential variates into keyed arrays */
     double *rec_shift, *rec_cell, *rec_ghost, *rec_part, *rec_lc;
     int rec_lc_stride;
+    /* rare draws (escape repair [2 * n * ORC_ESC_STRIDE], re-thermalisation
+     * [n * ORC_RETH_STRIDE]): read in keyed mode, written in sequential mode;
+     * NULL in keyed mode means the Philox generator serves them */
+    double *escape, *retherm;
+    int rare_n;
+    /* set when a particle needed more escape tries or pushes than a keyed
+     * entry holds (the step is then not representable in the keyed layout) */
+    int rare_overflow;
 } orc_draws;
 
 typedef struct orc_sys orc_sys;
@@ -117,6 +135,13 @@ void       orc_draws_set_keyed(orc_draws *d, const double *shift, const double *
 void       orc_draws_set_record(orc_draws *d, double *shift, double *cell, double *ghost, double *part,
                                 double *lc, int lc_stride);
 void       orc_draws_counts(const orc_draws *d, uint64_t *out3);
+/* the nematic path's discrete decisions of the last collision phase: one
+ * word per particle (particle index) and per cell, in the layout of the GPU
+ * library's mpcd_gpu_lc_decisions */
+void       orc_lc_decisions(const orc_sys *s, uint32_t *part, uint32_t *cell);
+/* escape / re-thermalisation arrays (see orc_draws); n = particles */
+void       orc_draws_set_rare(orc_draws *d, double *escape, double *retherm, int n);
+int        orc_draws_rare_overflow(const orc_draws *d);
 
 /* exposed pieces for closed-form tests */
 void     orc_eigen(double *m, int dim, double *eigval); /* m row-major dim x dim -> eigvecs as rows */

@@ -21,6 +21,7 @@ MAXWALL = 16
 
 DRAW_SEQ, DRAW_KEYED, DRAW_PHILOX = 0, 1, 2
 P_SHIFT, P_CELL, P_GHOST, P_NORMAL, P_LC = 1, 2, 3, 4, 5
+ESC_STRIDE, RETH_STRIDE = 32, 24
 
 
 class Species(C.Structure):
@@ -90,6 +91,9 @@ def lib():
         L.orc_draws_set_keyed.argtypes = [vp, dp, dp, dp, dp, dp, C.c_int]
         L.orc_draws_set_record.argtypes = [vp, dp, dp, dp, dp, dp, C.c_int]
         L.orc_draws_counts.argtypes = [vp, C.POINTER(C.c_uint64)]
+        L.orc_draws_set_rare.argtypes = [vp, dp, dp, C.c_int]
+        L.orc_lc_decisions.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_draws_rare_overflow.argtypes = [vp]
         L.orc_eigen.argtypes = [dp, C.c_int, dp]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_philox_uniform.restype = C.c_double
@@ -169,6 +173,15 @@ class Draws:
     def set_record(self, shift, cell, ghost, part, lc, lc_stride):
         self._keep = [shift, cell, ghost, part, lc]
         self.L.orc_draws_set_record(self.h, *[_dp(a) for a in (shift, cell, ghost, part, lc)], int(lc_stride))
+
+    def set_rare(self, escape, retherm, n):
+        """escape [2 * n * ESC_STRIDE] / retherm [n * RETH_STRIDE] arrays: read in
+        keyed mode (None: Philox), written in sequential mode"""
+        self._rare = [escape, retherm]
+        self.L.orc_draws_set_rare(self.h, _dp(escape), _dp(retherm), int(n))
+
+    def rare_overflow(self):
+        return bool(self.L.orc_draws_rare_overflow(self.h))
 
     def counts(self):
         out = (C.c_uint64 * 3)()
@@ -250,6 +263,13 @@ class System:
         out = (C.c_long * 4)()
         self.L.orc_counters(self.h, out)
         return tuple(out)
+
+    def lc_decisions(self):
+        """(particle words [N], cell words [ncell]) of the last collision phase"""
+        part, cell = np.zeros(self.N, np.uint32), np.zeros(self.nc, np.uint32)
+        u32 = C.POINTER(C.c_uint32)
+        self.L.orc_lc_decisions(self.h, part.ctypes.data_as(u32), cell.ctypes.data_as(u32))
+        return part, cell
 
     def lc_stride(self):
         return self.L.orc_lc_stride(self.h)

@@ -54,11 +54,16 @@ def periodic(dim, L):
 
 
 def case(name, dim, L, ppc, rtech, steps, seed, lc=0, tstech=0, rotang=math.pi / 2, dt=0.1, kbt=1.0,
-         grav=(0, 0, 0), walls=None, sp=None, galinv=1, tau=0.5):
-    return dict(name=name, dim=dim, L=list(L) + [1] * (3 - len(L)), ppc=ppc, rtech=rtech, steps=steps,
-                seed=seed, lc=lc, tstech=tstech, rotang=rotang, dt=dt, kbt=kbt, grav=list(grav),
-                walls=walls if walls is not None else periodic(dim, L), species=[sp or species()],
-                galinv=galinv, tau=tau)
+         grav=(0, 0, 0), walls=None, sp=None, galinv=1, tau=0.5, qbox=None, kick=None):
+    c = dict(name=name, dim=dim, L=list(L) + [1] * (3 - len(L)), ppc=ppc, rtech=rtech, steps=steps,
+             seed=seed, lc=lc, tstech=tstech, rotang=rotang, dt=dt, kbt=kbt, grav=list(grav),
+             walls=walls if walls is not None else periodic(dim, L), species=[sp or species()],
+             galinv=galinv, tau=tau)
+    if qbox is not None:
+        c["qbox"] = [list(map(float, b)) for b in qbox]  # initial positions: [lo, hi) per axis
+    if kick is not None:
+        c["kick"] = [[int(i), [float(x) for x in v]] for i, v in kick]  # particle i starts with velocity v
+    return c
 
 
 CASES = [
@@ -97,8 +102,36 @@ CASES = [
                                        V=[0.2, 0, 0])]),
 ]
 
+# bin()'s escape repair (mpc.c:970-982) and the wall re-thermalisation after
+# NBOUNCE bounces (bc.c:1461-1488), which the periodic and channel cases above
+# never reach:
+CASES += [
+    # 2D SRD with an open y axis (no wall there): particles drift out and are
+    # re-placed, in the shifted bin and in the unshifted bin after streaming
+    case("srd2d_open", 2, (8, 8), 10, rtech=0, steps=4, seed=24, dt=1.0, walls=pbc_walls(0, 8)),
+    # the same for a 2D active nematic: escapes also redraw the orientation
+    case("lc2d_open", 2, (8, 8), 12, rtech=16, lc=1, steps=3, seed=25, dt=1.0, walls=pbc_walls(0, 8),
+         sp=species(mfpot=10.0, tumble=0.5, rfc=0.01, act=0.05)),
+    # 3D MPC-AT, open z, y walls inside the box (at y = 1 and y = 5 of 6): a
+    # re-placed position beyond them is redrawn (replacePos_WithCheck)
+    case("at3d_open_inset", 3, (6, 6, 6), 6, rtech=2, steps=3, seed=26, dt=1.0,
+         walls=pbc_walls(0, 6) + [dict(A=[0, 1, 0], R=1.0, DN=0.0, MVN=-1.0, MVT=-1.0),
+                                  dict(A=[0, -1, 0], R=-5.0, DN=0.0, MVN=-1.0, MVT=-1.0)],
+         qbox=[(0, 6), (1.2, 4.8), (0, 6)]),
+    # 3D nematic with an open z axis: the 3D orientation redraw
+    case("lc3d_open", 3, (4, 4, 4), 8, rtech=3, lc=1, steps=3, seed=27, dt=1.0,
+         walls=pbc_walls(0, 4) + pbc_walls(1, 4), sp=species(mfpot=0.6)),
+    # 2D MPC-AT channel, four species (vDist 0, 2, 3, 4): four particles start
+    # so fast that their cells bounce more than 2 NBOUNCE times in one step,
+    # which rewinds and then re-thermalises them with push()
+    case("at2d_nbounce", 2, (8, 4), 10, rtech=2, steps=3, seed=28,
+         walls=pbc_walls(0, 8) + solid_walls(1, 4), kick=[(0, (0.3, 2e5)), (1, (-0.2, -2e5)), (2, (0.1, 1.5e5)),
+                                                          (3, (0.0, -1.8e5))]),
+]
+
 # rat2d_2sp uses two species
 CASES[6]["species"] = [species(mass=1.0), species(mass=2.5)]
+next(c for c in CASES if c["name"] == "at2d_nbounce")["species"] = [species(vdist=v) for v in (0, 2, 3, 4)]
 
 
 def initial_state(c: dict):
@@ -113,8 +146,9 @@ def initial_state(c: dict):
     U = np.zeros((N, 3))
     nsp = len(c["species"])
     spid = (np.arange(N) % nsp).astype(np.int32)
+    qbox = c.get("qbox") or [(0.0, L[d]) for d in range(dim)]
     for d in range(dim):
-        Q[:, d] = rng.uniform(0.0, L[d], N)
+        Q[:, d] = rng.uniform(qbox[d][0], qbox[d][1], N)
     masses = np.array([s["mass"] for s in c["species"]])[spid]
     V[:, :dim] = rng.normal(size=(N, dim)) * np.sqrt(c["kbt"] / masses)[:, None]
     if dim == 3:
@@ -126,6 +160,8 @@ def initial_state(c: dict):
         U[:, 1] = np.sin(th)
     avv = np.zeros(3)
     avv[:dim] = V[:, :dim].mean(axis=0)
+    for i, v in c.get("kick", []):
+        V[i, :dim] = v[:dim]
     return Q, V, U, spid, avv
 
 

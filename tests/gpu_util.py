@@ -55,6 +55,11 @@ class KeyedRecorder:
         ghost = np.zeros(self.nc * 3)
         part = np.zeros(self.n * 3)
         lc = np.zeros(self.n * self.stride)
+        escape = np.zeros(2 * self.n * self.po.ESC_STRIDE)
+        retherm = np.zeros(self.n * self.po.RETH_STRIDE)
         self.d.set_record(shift, cell, ghost, part, lc, self.stride)
+        self.d.set_rare(escape, retherm, self.n)
         self.s.step(self.d)
-        return dict(shift=shift, cell=cell, ghost=ghost, part=part, lc=lc, lc_stride=self.stride)
+        assert not self.d.rare_overflow(), "a step needed more escape tries or pushes than a keyed entry holds"
+        return dict(shift=shift, cell=cell, ghost=ghost, part=part, lc=lc, lc_stride=self.stride, escape=escape,
+                    retherm=retherm)

@@ -36,10 +36,10 @@ def test_library_exports_every_declared_symbol():
         assert re.search(rf"\sT {name}$", out, re.M), f"{name} is not an exported text symbol"
 
 
-def test_struct_sizes_match_c():
+def test_struct_sizes_match_c(tmp_path):
     """compile a tiny C program against the header and compare sizeof()"""
     M = U.package()
-    src = os.path.join(ROOT, "_work", "sizes.c") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp/mpcd_sizes.c"
+    src = str(tmp_path / "sizes.c")
     exe = src[:-2]
     with open(src, "w") as f:
         f.write('#include <stdio.h>\n#include "mpcd_gpu.h"\nint main(void){printf("%zu %zu %zu %zu\\n",'
