@@ -190,6 +190,35 @@ mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *ctx, const mpcd_draws *draws, double 
  * re-thermalisations (bc.c:1364-1368), may be NULL */
 mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *ctx, int64_t *bc_event_counts_out);
 
+/* For callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc); no counterpart in
+ * the reference.  The binning of mpcd_gpu_bin without a context: n particles
+ * of one replica (params: dim, L and the walls that make an axis periodic)
+ * whose positions q_dev [3n] (FP64, particle-index order) lie in device
+ * memory.  shift == NULL: binin (mpc.c:924-934), every list in particle-index
+ * order.  shift [3] (host): the positions shifted by it (gridShift_all,
+ * mpc.c:728-736) and binned with bin(shifted=1)'s periodic wrap
+ * (mpc.c:964-968); given a previous binning (prev_perm_dev [n] its list order,
+ * prev_cell_dev [n] each particle's cell), its lists are relinked as bin()
+ * does (mpc.c:948-992): in every cell the particles that stayed keep their
+ * order, then the arrivals in the previous order; an escaped particle stays
+ * in its list (its re-placement is the caller's).  Without a previous binning
+ * an escaped particle gets cell -1 and is listed after every cell.  Outputs
+ * (device): cell_dev [n] cell of every particle (x-major over
+ * (L0+1)(L1+1)(L2+1), z extent 1 in 2D), count_dev [ncell] populations (may
+ * be NULL), perm_dev [n] particle indices in list order.  It allocates
+ * nothing: its working memory is a static device array of about 42 MB that
+ * each GPU keeps until the process ends, shared by every call on that GPU,
+ * so calls that overlap on one GPU race; it only launches kernels, on
+ * `stream` (a hipStream_t, NULL for the null stream).  n above
+ * MPCD_NOALLOC_MAX_PARTICLES or more cells than MPCD_NOALLOC_MAX_CELLS fail
+ * with MPCD_EINVAL; errors read through mpcd_gpu_last_error(NULL). */
+#define MPCD_NOALLOC_MAX_PARTICLES 2097152
+#define MPCD_NOALLOC_MAX_CELLS 1000000
+mpcd_status mpcd_gpu_bin_noalloc(const mpcd_gpu_params *params, int64_t n, const double *q_dev, const double *shift,
+                                 const int32_t *prev_perm_dev, const int32_t *prev_cell_dev, int32_t *cell_dev,
+                                 int32_t *count_dev, int32_t *perm_dev, void *stream);
+
 /* --- whole steps ------------------------------------------------------------ */
 /* one full step (bin + collide + stream); draws may be NULL (internal RNG) */
 mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *ctx, const mpcd_draws *draws, double *kbt_now_out);

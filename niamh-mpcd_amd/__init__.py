@@ -122,7 +122,7 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
            "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
            "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read", "mpcd_gpu_lc_record",
-           "mpcd_gpu_lc_decisions"]
+           "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc"]
 
 _lib = None
 
@@ -150,6 +150,7 @@ def lib():
     L.mpcd_gpu_synchronize.argtypes = [vp]
     L.mpcd_gpu_get_cells.argtypes = [vp, C.c_int, ip, dp, dp, dp]
     L.mpcd_gpu_lc_record.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_bin_noalloc.argtypes = [C.POINTER(Params), C.c_int64, vp, dp, vp, vp, vp, vp, vp, vp]
     L.mpcd_gpu_lc_decisions.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.mpcd_gpu_get_order.argtypes = [vp, C.c_int, ip, ip]
     L.mpcd_gpu_ncells.argtypes = [vp, C.c_int]
@@ -226,6 +227,22 @@ def make_params(cfg: dict, precision: int = FP64) -> Params:
         W.COLL_TYPE = int(w.get("COLL_TYPE", 1))
         W.inter_mask = int(w.get("inter_mask", (1 << MAXSPEC) - 1))
     return p
+
+
+def bin_noalloc(cfg, n, q_dev, shift=None, prev_perm_dev=None, prev_cell_dev=None, cell_dev=None, count_dev=None,
+                perm_dev=None, stream=None, precision=FP64):
+    """mpcd_gpu_bin_noalloc: binning without a context or allocation; every
+    *_dev argument is a device address (int, e.g. a tensor's data_ptr()),
+    stream a hipStream_t address or None (null stream)"""
+    L = lib()
+    p = make_params(cfg, precision)
+    sh = None if shift is None else (C.c_double * 3)(*[float(x) for x in shift])
+    st = L.mpcd_gpu_bin_noalloc(C.byref(p), int(n), q_dev, sh, prev_perm_dev, prev_cell_dev, cell_dev, count_dev,
+                                perm_dev, stream)
+    if st != 0:
+        L.mpcd_gpu_last_error.restype = C.c_char_p
+        msg = L.mpcd_gpu_last_error(None)
+        raise MPCDError(f"bin_noalloc failed: {STATUS.get(st, st)}: {msg.decode() if msg else ''}")
 
 
 class MPCDError(RuntimeError):

@@ -257,18 +257,6 @@ static bool state_ok(const State<Real> &s, bool lc) {
   return s.x && s.y && s.z && s.vx && s.vy && s.vz && s.id && s.sp && s.cst && (!lc || (s.ux && s.uy && s.uz));
 }
 
-static void set_pbc(const mpcd_gpu_params &p, int pbc[3]) {
-  // setPBC, read.c:584-603
-  for (int i = 0; i < 3; i++) pbc[i] = 0;
-  for (int w = 0; w < p.nwall; w++) {
-    const mpcd_gpu_wall &W = p.wall[w];
-    for (int i = 0; i < 3; i++) {
-      double ainv = (W.A[i] != 0.0) ? 1.0 / W.A[i] : 0.0;
-      if (feq(fabs(ainv), 1.0) && W.MVN > 0.0 && W.MVT > 0.0 && (fneq(W.DN, 0.0) || fneq(W.DT, 0.0))) pbc[i] = 1;
-    }
-  }
-}
-
 static double mean_ppc(const RepDev &R) {
   double vol = (double)R.L[0] * R.L[1] * (R.dim == 3 ? R.L[2] : 1);
   return R.npart / vol;
@@ -554,7 +542,7 @@ extern "C" void mpcd_gpu_destroy(mpcd_gpu_ctx *c) {
   delete c;
 }
 
-extern "C" const char *mpcd_gpu_last_error(const mpcd_gpu_ctx *c) { return c ? c->err.c_str() : "no context"; }
+extern "C" const char *mpcd_gpu_last_error(const mpcd_gpu_ctx *c) { return c ? c->err.c_str() : noalloc_error(); }
 
 extern "C" int mpcd_gpu_ncells(const mpcd_gpu_ctx *c, int r) {
   if (!c || r < 0 || r >= c->nrep) return -1;

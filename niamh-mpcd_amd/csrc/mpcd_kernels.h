@@ -23,7 +23,7 @@
 
 #include "mpcd_math.h"
 #include "mpcd_dual.h"
-#include "../../include/mpcd_gpu.h"
+#include "mpcd_params.h"
 
 namespace mpcd {
 

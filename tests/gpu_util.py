@@ -63,3 +63,54 @@ class KeyedRecorder:
         assert not self.d.rare_overflow(), "a step needed more escape tries or pushes than a keyed entry holds"
         return dict(shift=shift, cell=cell, ghost=ghost, part=part, lc=lc, lc_stride=self.stride, escape=escape,
                     retherm=retherm)
+
+
+def hip_runtime():
+    """the HIP runtime the library itself runs on (the libamdhip64 it mapped
+    into this process), for tests that hand it device memory of their own:
+    a second runtime, such as the one bundled with torch, would hold its own
+    device state"""
+    import ctypes as C
+    M = package()
+    M.lib()
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64" in line and "/torch/" not in line:
+                path = line.split()[-1]
+                break
+    if path is None:
+        raise RuntimeError("the library's HIP runtime is not mapped")
+    hip = C.CDLL(path)
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipDeviceSynchronize.argtypes = []
+    return hip
+
+
+class DevBuf:
+    """device memory from the library's HIP runtime, filled from and read
+    into numpy arrays"""
+
+    def __init__(self, hip, a):
+        import ctypes as C
+        self.hip, self.a = hip, np.ascontiguousarray(a)
+        self.ptr = C.c_void_p()
+        assert hip.hipMalloc(C.byref(self.ptr), max(1, self.a.nbytes)) == 0, "hipMalloc"
+        assert hip.hipMemcpy(self.ptr, self.a.ctypes.data, self.a.nbytes, 1) == 0, "hipMemcpy H2D"
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def read(self):
+        out = np.empty_like(self.a)
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0, "hipMemcpy D2H"
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = None

@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -47,3 +48,20 @@ def test_struct_sizes_match_c(tmp_path):
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
     sizes = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     assert sizes == [C.sizeof(M.Species), C.sizeof(M.Wall), C.sizeof(M.Params), C.sizeof(M.DrawsC)]
+
+
+def test_bin_noalloc_rejects_sizes_above_its_maximum():
+    """the size checks of the entry point that allocates nothing are host
+    logic: they fail before any device call (none is made here)"""
+    M = U.package()
+    if not os.path.exists(M.LIB_PATH):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_gpu_parity import _random_box
+    import numpy as np
+    c, _, _, _ = _random_box(3, (4, 4, 4), 2, 0)
+    cfg = U.gpu_cfg(c, 2.0, np.zeros(3))
+    with pytest.raises(M.MPCDError, match="MPCD_NOALLOC_MAX_PARTICLES"):
+        M.bin_noalloc(cfg, 2097153, None, None, None, None, None, None, None)
+    with pytest.raises(M.MPCDError, match="MPCD_NOALLOC_MAX_CELLS"):
+        M.bin_noalloc(dict(cfg, L=[128, 128, 64]), 100, None, None, None, None, None, None, None)
