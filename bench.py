@@ -33,27 +33,35 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "MPCD particle-steps/s on 1 MI355X (plain FP64 vs FP32 batched) + % HBM peak"
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per particle-step of the whole step: one read and one
-# write of r and v (3D)
+# write of the persistent particle state, r and v (3D: 6 words), r and v in
+# 2D (4 words), r, v and u for 2D nematics (6 words)
 B_ALG = {"fp64": 96.0, "fp32": 48.0}
+B_ALG_2D = {"fp64": 64.0, "fp32": 32.0}
+B_ALG_LC2D = {"fp64": 96.0, "fp32": 48.0}
 PREC = {"fp64": 8, "fp32": 4}
 
 
-def fam_bytes(fam, form):
+def fam_bytes(fam, form, dim=3, lc=False):
     """algorithmic bytes per particle that one launch of a kernel family must
-    move (each input read once, each output written once); p = bytes per real"""
-    p = PREC[form]
-    return {"count": 3 * p + 24,        # read r, slot cell, unshifted cell; write cell, rank, 64-bit key
-            "scatter": 24,              # read cell, rank, key; write key
-            "sort": 20,                 # read + write keys; write slot cells
-            "gather": 12 * p + 24,      # read key, r, v, id, species; write r, v, id, species
-            "props": 6 * p + 4,         # read r, v, species
-            "collide": 12 * p + 4,      # read r, v, species; write r, v
-            "stream": 12 * p + 8,       # read r, v, species; write r, v, unshifted cell
+    move (each input read once, each output written once); p = bytes per
+    real, c = components (dim), u = orientation words (nematics)"""
+    p, c = PREC[form], dim
+    u = c if lc else 0
+    return {"count": c * p + 24,               # read r, slot cell, unshifted cell; write cell, rank, 64-bit key
+            "scatter": 24,                     # read cell, rank, key; write key
+            "sort": 20,                        # read + write keys; write slot cells
+            "gather": 2 * (2 * c + u) * p + 24,  # read key, state, id, species; write state, id, species
+            "props": (2 * c + u) * p + 4,      # read r, v (u), species
+            "collide": (2 * c + u + 2 * c) * p + 4,  # read r, v (u), species; write r, v
+            "lc": 2 * u * p + 4,               # read and write u, read species
+            "stream": 2 * (2 * c + u) * p + 8,  # read state, species; write state, unshifted cell
             # fused: read key, r, v, id; write r, v, id, slot cell, next cell, rank, key
-            "fused": 12 * p + 44,
+            "fused": 4 * c * p + 44,
             # bucketed step: read r, v, id, key from the bucket; write r, v, id, key to the next bucket
-            "bucket": 12 * p + 24,
-            "ghost": 0, "lc": 6 * p + 4, "temp": 0, "scan": 0}[fam]
+            "bucket": 4 * c * p + 24,
+            "ghost": 0, "temp": 0, "scan": 0, "flow": 0}[fam]
+
+
 ROTANG_130 = 2.2689280275926285
 
 
@@ -69,7 +77,7 @@ def package():
 
 def periodic_walls(L):
     ws = []
-    for ax in range(3):
+    for ax in range(len(L)):
         a1 = [0.0, 0.0, 0.0]
         a1[ax] = 1.0
         a2 = [0.0, 0.0, 0.0]
@@ -78,26 +86,56 @@ def periodic_walls(L):
     return ws
 
 
-def srd_cfg(L, ppc):
+def srd_cfg(L, ppc, dim=3, rtech=0, rotang=ROTANG_130):
     sp = dict(mass=1.0, rfc=0.01, tumble=2.0, chihi=0.5, act=0.05, mfpot=10.0, damp=0.0, sigwidth=1.0, sigpos=1.0,
               minactratio=0.0, vdist=4)
-    return dict(dim=3, L=list(L), dt=0.1, kbt=1.0, rotang=ROTANG_130, tau=0.5, grav=[0, 0, 0], avv=[0, 0, 0],
-                gndnst=float(ppc), rtech=0, lc=0, tstech=0, galinv=1, species=[sp], walls=periodic_walls(L))
+    return dict(dim=dim, L=list(L) + [1] * (3 - len(L)), dt=0.1, kbt=1.0, rotang=rotang, tau=0.5, grav=[0, 0, 0],
+                avv=[0, 0, 0], gndnst=float(ppc), rtech=rtech, lc=0, tstech=0, galinv=1, species=[sp],
+                walls=periodic_walls(L))
 
 
-def initial_state(L, ppc, seed):
+def initial_state(L, ppc, seed, dim=3):
     rng = np.random.default_rng(seed)
-    N = int(np.prod(L)) * ppc
-    Q = np.empty((N, 3))
-    for d in range(3):
+    N = int(np.prod(L[:dim])) * ppc
+    Q = np.zeros((N, 3))
+    for d in range(dim):
         Q[:, d] = rng.uniform(0.0, L[d], N)
-    V = rng.normal(size=(N, 3))
+    V = np.zeros((N, 3))
+    V[:, :dim] = rng.normal(size=(N, dim))
     V -= V.mean(axis=0)
     return Q, V
 
 
+def small_box(M, steps=1000):
+    """the reference's CPU-runnable configuration on the GPU: 2D SRD 30x30
+    cells x 20 ppc (18 000 particles), ORTHAXIS 90 degrees, dt 0.1, periodic,
+    FP64, 1000 steps -- eager launches against hipGraph replays, where the
+    launch overhead of a step dominates -- beside the reference's own step
+    on one host core"""
+    L, ppc = (30, 30), 20
+    cfg = srd_cfg(L, ppc, dim=2, rtech=1, rotang=math.pi / 2)
+    Q, V = initial_state(L, ppc, seed=3, dim=2)
+    N = Q.shape[0]
+    out = {}
+    for name, graph in (("eager", False), ("hipgraph", True)):
+        ctx = M.Context(cfg, N, seed=17, precision=M.FP64)
+        ctx.upload(0, Q, V)
+        ctx.run(10, use_graph=graph)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(steps, use_graph=graph)
+        ctx.synchronize()
+        sec = time.perf_counter() - t0
+        ctx.close()
+        out[name] = dict(value=N * steps / sec, ms_per_step=1e3 * sec / steps)
+    out["workload"] = (f"2D SRD {L[0]}x{L[1]} cells x {ppc} ppc ({N} particles), ORTHAXIS 90 deg, dt 0.1, kT 1, "
+                       f"periodic, FP64, {steps} steps")
+    out["unit"] = "particle-steps/s"
+    return out
+
+
 def graph_run(ctx, steps, n_total):
-    """the same steps replayed as hipGraphs (pairs of fused steps), no profiling"""
+    """the same steps replayed as hipGraphs (pairs of steps), no profiling"""
     ctx.run(2, use_graph=True)  # records the graph
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -107,58 +145,73 @@ def graph_run(ctx, steps, n_total):
     return dict(value=n_total * steps / sec, ms_per_step=1e3 * sec / steps)
 
 
-def timed_run(M, ctx, steps, warmup, n_total):
+def timed_run(M, ctx, steps, warmup, n_total, prof_steps=20):
+    """warmup untimed steps, then exactly `steps` timed eager steps with
+    nothing else on the stream; then a separate run of prof_steps steps with
+    HIP events around every kernel family for the kernel times (the events
+    add a few microseconds between kernels, so they stay out of the timed
+    run)"""
     ctx.run(warmup, use_graph=False)
     ctx.synchronize()
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
-    ctx.profile(True)
     t0 = time.perf_counter()
     ctx.run(steps, use_graph=False)
     ctx.synchronize()
     t1 = time.perf_counter()
+    sec = t1 - t0
+    ctx.profile(True)
+    ctx.run(prof_steps, use_graph=False)
+    ctx.synchronize()
     timing = ctx.timing()
     ctx.profile(False)
-    sec = t1 - t0
-    return dict(sec=sec, ms_per_step=1e3 * sec / steps, value=n_total * steps / sec, timing=timing)
+    return dict(sec=sec, ms_per_step=1e3 * sec / steps, value=n_total * steps / sec, timing=timing,
+                prof_steps=prof_steps)
 
 
-def roofline(timing, n_per_launch, form):
-    fam, (ms, cnt) = max(timing.items(), key=lambda kv: kv[1][0])
+def roofline(timing, n_per_launch, form, b_alg=None, families=None, traffic_tag=None, dim=3, lc=False):
+    """the dominant kernel against HBM.  `achieved` counts the algorithmic
+    bytes of a particle-step (one read and one write of the particle state,
+    b_alg per particle; 48 / 96 B in 3D) for every particle of one launch,
+    over the kernel's mean launch time (HIP events on the library stream).
+    What the kernel itself must move -- its records, ids and binning keys --
+    is kernel_bytes, with kernel_frac beside it; `traffic` is the HBM bytes
+    per launch that the committed rocprofv3 counter summary measured."""
+    items = [kv for kv in timing.items() if families is None or kv[0] in families]
+    fam, (ms, cnt) = max(items, key=lambda kv: kv[1][0])
     avg_s = (ms / max(cnt, 1)) * 1e-3
-    bytes_alg = fam_bytes(fam, form) * n_per_launch
+    b = B_ALG[form] if b_alg is None else b_alg
+    bytes_alg = b * n_per_launch
+    kbytes = fam_bytes(fam, form, dim, lc) * n_per_launch
     achieved = bytes_alg / avg_s / 1e9
     return dict(bound="hbm", kernel_family=fam, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(fam, form), avg_launch_ms=ms / max(cnt, 1),
-                alg_bytes_per_launch=bytes_alg)
+                frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(fam, traffic_tag or form),
+                avg_launch_ms=ms / max(cnt, 1), alg_bytes_per_particle=b, alg_bytes_per_launch=bytes_alg,
+                kernel_bytes_per_particle=fam_bytes(fam, form, dim, lc), kernel_bytes=kbytes,
+                kernel_frac=kbytes / avg_s / 1e9 / HBM_PEAK_GBS)
 
 
-def measured_traffic(fam, form):
+def measured_traffic(fam, tag):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any"""
-    path = os.path.join(ROOT, "profiles", f"traffic_{fam}_{form}.md")
+    path = os.path.join(ROOT, "profiles", f"traffic_{fam}_{tag}.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(L=(64, 64, 64), ppc=10, steps=4):
+def cpu_baseline(L=(64, 64, 64), ppc=10, steps=4, dim=3, rtech=0, rotang=ROTANG_130):
     """the reference's own C step (oracle/_ref/ref_harness, built from the
     reference's sources) timed on one host core on a bounded sample of the
     same workload; falls back to the oracle port when the build is absent"""
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    Q, V = initial_state(L, ppc, seed=99)
+    Q, V = initial_state(L, ppc, seed=99, dim=dim)
     N = Q.shape[0]
     if os.path.exists(harness):
         with tempfile.TemporaryDirectory() as d:
             case = os.path.join(d, "case.txt")
             state = os.path.join(d, "state.bin")
-            lines = ["dim 3", "L {} {} {}".format(*L), f"N {N}", f"steps {steps}", "rtech 0", "lc 0", "tstech 0",
-                     "galinv 1", "dt 0.1", "kbt 1.0", f"rotang {ROTANG_130!r}", "tau 0.5", "grav 0 0 0",
+            Lc = list(L) + [1] * (3 - len(L))
+            lines = [f"dim {dim}", "L {} {} {}".format(*Lc), f"N {N}", f"steps {steps}", f"rtech {rtech}", "lc 0",
+                     "tstech 0", "galinv 1", "dt 0.1", "kbt 1.0", f"rotang {rotang!r}", "tau 0.5", "grav 0 0 0",
                      "avv 0 0 0", "seed 5", "species 1 0.01 2 0.5 0.05 10 0 1 1 0 4 0.007"]
             for w in periodic_walls(L):
                 v = list(w["A"]) + [0, 0, 0] + [w["R"], w["DN"], 0, 0, 0] + [0, 0, 0] + [1, 1, 1, 1] + \
@@ -178,8 +231,8 @@ def cpu_baseline(L=(64, 64, 64), ppc=10, steps=4):
         sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
         import cases as CS
         import pyoracle
-        cfg = srd_cfg(L, ppc)
-        cfg["walls"] = CS.periodic(3, L)
+        cfg = srd_cfg(L, ppc, dim=dim, rtech=rtech, rotang=rotang)
+        cfg["walls"] = CS.periodic(dim, L)
         s = pyoracle.System(cfg, Q, V, np.zeros_like(Q), None)
         d = pyoracle.Draws(pyoracle.DRAW_PHILOX, 5)
         t0 = time.perf_counter()
@@ -188,7 +241,8 @@ def cpu_baseline(L=(64, 64, 64), ppc=10, steps=4):
         sec = time.perf_counter() - t0
         kind = "port"
     return dict(value=N * steps / sec, unit="particle-steps/s", cores=1, kind=kind,
-                sample=f"3D SRD {L[0]}x{L[1]}x{L[2]} cells x {ppc} ppc ({N} particles), {steps} steps, one core")
+                sample=f"{dim}D SRD {'x'.join(str(x) for x in L)} cells x {ppc} ppc ({N} particles), {steps} steps, "
+                       f"one core")
 
 
 def tangent_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, act=0.03, hg=(1e-11,), ha=(1e-13,)):
@@ -349,19 +403,12 @@ def lc_throughput(M, L=256, ppc=20, steps=40, warmup=5):
         ctx.upload(0, Q, V, U)
         r = timed_run(M, ctx, steps, warmup, N)
         ctx.close()
-        # the dominant kernel against HBM, with 2D algorithmic bytes per particle: the
-        # nematic collision reads r, v, u (2 components each) and the species and writes v;
-        # the orientation step reads and writes u and reads the species
-        pb = PREC[name]
-        lc_bytes = {"collide": 8 * pb + 4, "lc": 4 * pb + 4}
-        fam, (ms, cnt) = max(((k, v) for k, v in r["timing"].items() if k in lc_bytes), key=lambda kv: kv[1][0])
-        avg_s = ms / max(cnt, 1) * 1e-3
-        ach = lc_bytes[fam] * N / avg_s / 1e9
+        # the dominant kernel against HBM with the 2D nematic step's algorithmic bytes
+        # (r, v, u read and written once: 96 / 48 B per particle-step)
+        rl = roofline(r["timing"], N, name, b_alg=B_ALG_LC2D[name], traffic_tag=f"lc2d_{name}", dim=2, lc=True)
         out[name] = dict(value=r["value"], ms_per_step=r["ms_per_step"],
-                         kernels_ms_per_step={k: v[0] / steps for k, v in r["timing"].items() if v[1]},
-                         roofline=dict(bound="hbm", kernel_family=fam, achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                                       frac=ach / HBM_PEAK_GBS, avg_launch_ms=ms / max(cnt, 1),
-                                       alg_bytes_per_launch=lc_bytes[fam] * N))
+                         kernels_ms_per_step={k: v[0] / r["prof_steps"] for k, v in r["timing"].items() if v[1]},
+                         roofline=rl, step_hbm_frac=r["value"] * B_ALG_LC2D[name] / 1e9 / HBM_PEAK_GBS)
     out["workload"] = (f"2D active nematic {L}x{L} x {ppc} ppc (lc=1, collision 16 DIPOLE_DIR_SUM, U=10, tumble 0.5, "
                        f"rfc 0.01, activity 0.05), {steps} timed steps after {warmup}; split path")
     out["unit"] = "particle-steps/s"
@@ -398,8 +445,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
     ap.add_argument("--no-lc", action="store_true", help="skip the 2D active-nematic throughput line")
-    ap.add_argument("--only", choices=["fp32", "fp64", "both", "lc"], default="both",
-                    help="lc: only the 2D active nematic throughput (its JSON instead of the bench line)")
+    ap.add_argument("--only", choices=["fp32", "fp64", "both", "lc", "ensemble"], default="both",
+                    help="lc / ensemble: only the 2D active nematic or the 64-replica MPC-AT+a ensemble throughput "
+                         "(--steps timed steps; its JSON instead of the bench line)")
     ap.add_argument("--fill", type=float, default=0.0,
                     help="fix the bucket fill of the run layout instead of autotuning it (reproducible profiles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -415,6 +463,14 @@ def main():
     M = package()
     if args.only == "lc":
         print(json.dumps(lc_throughput(M, steps=args.steps, warmup=args.warmup)))
+        return
+    if args.only == "ensemble":
+        import importlib
+        e = importlib.import_module("niamh_mpcd_amd.batched").run(M, M.FP32, steps=0, time_steps=args.steps)
+        e["roofline"] = roofline({"bucket": (e["bucket_avg_launch_ms"], 1)}, e["particles"], "fp32",
+                                 traffic_tag="ensemble_fp32")
+        e.pop("per_replica")
+        print(json.dumps(e))
         return
     L = (args.L,) * 3
     cfg = srd_cfg(L, args.ppc)
@@ -443,7 +499,7 @@ def main():
         rf = timed_run(M, ctx, kf, 2, B * n1)
         ctx.flow(False)
         r32["flow_field"] = dict(steps=kf, ms_per_step=rf["ms_per_step"],
-                                 flow_kernels_ms_per_step=rf["timing"]["flow"][0] / kf,
+                                 flow_kernels_ms_per_step=rf["timing"]["flow"][0] / rf["prof_steps"],
                                  summed_steps=ctx.flow_read(0)[1])
         # the same on the cell-sorted layout (the path of parity steps,
         # thermostats and the LC split path): FP64 atomics per run of equal
@@ -453,7 +509,7 @@ def main():
         rd = timed_run(M, ctx, kf, 2, B * n1)
         ctx.flow(False)
         r32["flow_field"]["cell_sorted"] = dict(ms_per_step=rd["ms_per_step"],
-                                                flow_kernels_ms_per_step=rd["timing"]["flow"][0] / kf)
+                                                flow_kernels_ms_per_step=rd["timing"]["flow"][0] / rd["prof_steps"])
         result["fp32"] = (r32, B * n1, B)
         ctx.close()
     if args.only in ("fp64", "both"):
@@ -487,19 +543,21 @@ def main():
         by_ppc = {}
         for x in e["per_replica"]:
             by_ppc.setdefault(x["ppc"], []).append(x["nu"])
+        th = {x["ppc"]: x["nu_theory"] for x in e["per_replica"]}
         ensemble = {k: e[k] for k in ("particles", "replicas", "value", "ms_per_step", "steps", "samples",
                                       "kernels_ms_per_step")}
-        # the bucketed MPC-AT+a kernel against HBM: the same record moved as in the headline's bucketed step
-        ach = fam_bytes("bucket", "fp32") * e["particles"] / (e["bucket_avg_launch_ms"] * 1e-3) / 1e9
-        ensemble["roofline"] = dict(bound="hbm", kernel_family="bucket", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                                    frac=ach / HBM_PEAK_GBS, avg_launch_ms=e["bucket_avg_launch_ms"],
-                                    alg_bytes_per_launch=fam_bytes("bucket", "fp32") * e["particles"])
+        # the bucketed MPC-AT+a kernel against HBM (48 B per particle-step, 3D FP32 state)
+        ensemble["roofline"] = roofline({"bucket": (e["bucket_avg_launch_ms"], 1)}, e["particles"], "fp32",
+                                        traffic_tag="ensemble_fp32")
+        ensemble["step_hbm_frac"] = e["value"] * B_ALG["fp32"] / 1e9 / HBM_PEAK_GBS
         ensemble.update(
             workload="64 replicas of 3D MPC-AT+a channels: L^3 with L ~ U{16..48}, ppc ~ U{5..20}, dt 0.05, kT 1, "
                      "bounce-back phantom walls in z, g = 1e-3 along x, FP32 state, autotuned tiles, hipGraph replays",
             unit="particle-steps/s",
-            viscosity_by_ppc={str(k): dict(mean=float(np.mean(v)), n=len(v)) for k, v in sorted(by_ppc.items())},
-            per_replica=[[x["L"], x["ppc"], round(x["nu"], 4)] for x in e["per_replica"]])
+            viscosity_by_ppc={str(k): dict(mean=float(np.mean(v)), n=len(v), theory=th[k],
+                                           ratio=float(np.mean(v)) / th[k]) for k, v in sorted(by_ppc.items())},
+            viscosity_theory="MPC-AT+a, Noguchi & Gompper 2008 (init.c:849-872), Poisson cell populations of mean ppc",
+            per_replica=[[x["L"], x["ppc"], round(x["nu"], 4), round(x["nu_theory"], 4)] for x in e["per_replica"]])
     lc = None
     if not args.no_lc and args.only == "both":
         print("[bench] 2D active nematic (LC) throughput", file=sys.stderr, flush=True)
@@ -515,6 +573,13 @@ def main():
         tangent = dict(channel_128x64=tangent_check(M, hg=(1e-11, 1e-12), ha=(1e-12, 3e-13, 1e-13)),
                        box_32x16=tangent_check(M, Lx=32, Ly=16, K=100, hg=(1e-10,), ha=(1e-10,)))
     cpu = None if args.no_cpu else cpu_baseline()
+    small = None
+    if args.only == "both":
+        print("[bench] 2D SRD 30x30 x 20 ppc, eager and hipGraph", file=sys.stderr, flush=True)
+        small = small_box(M)
+        if not args.no_cpu:
+            small["cpu_reference"] = cpu_baseline(L=(30, 30), ppc=20, steps=1000, dim=2, rtech=1,
+                                                  rotang=math.pi / 2)
     head_key = "fp32" if "fp32" in result else "fp64"
     r, n, B = result[head_key]
     line = {
@@ -537,7 +602,7 @@ def main():
         },
         "roofline": roofline(r["timing"], n, head_key),
         "cpu_baseline": cpu,
-        "kernels_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in r["timing"].items() if v[1]},
+        "kernels_ms_per_step": {k: v[0] / r["prof_steps"] for k, v in r["timing"].items() if v[1]},
         # the same workload replayed as hipGraphs of two fused steps (no per-kernel event timing)
         "hipgraph": r["graph"],
         "tile_fill": r["tile_fill"],
@@ -548,6 +613,7 @@ def main():
         "tangent_active_nematic": tangent,
         "lc_active_nematic": lc,
         "flow_field": r.get("flow_field"),
+        "small_box_2d": small,
     }
     if "fp64" in result and head_key != "fp64":
         r64, n64, _ = result["fp64"]

@@ -21,6 +21,8 @@ import time
 
 import numpy as np
 
+from .validate import mpcat_a_viscosity
+
 
 def channel_walls(L):
     ws = []
@@ -126,10 +128,15 @@ def run(M, precision, reps=None, steps=3000, sample_from=600, every=100, g=1e-3,
             prof[r].append(mom / np.maximum(w, 1.0))
     out = []
     for r, (L, p) in enumerate(reps):
+        nu_th = mpcat_a_viscosity(float(p), dt)["nu"]
+        if not times:
+            out.append(dict(L=L, ppc=p, nu=None, nu_theory=nu_th))
+            continue
         prof_r = np.asarray([pr for pr in prof[r]])
         nu = fit_startup(prof_r, np.asarray(times) * dt, L, g)
         tau = L * L / (math.pi ** 2 * max(nu, 1e-9)) / dt  # slowest mode, in steps
-        out.append(dict(L=L, ppc=p, nu=nu, vmax=float(prof_r[-1].max()), steps_over_tau=float(steps / tau)))
+        out.append(dict(L=L, ppc=p, nu=nu, nu_theory=nu_th, vmax=float(prof_r[-1].max()),
+                        steps_over_tau=float(steps / tau)))
     ctx.close()
     N = int(sum(ns))
     return dict(particles=N, replicas=len(reps), value=N * time_steps / sec, ms_per_step=1e3 * sec / time_steps,

@@ -35,6 +35,20 @@ def srd_viscosity_3d(n: float, alpha: float, dt: float, kbt: float = 1.0, m: flo
     return dict(nu=nu_kin + nu_coll, nu_kin=nu_kin, nu_coll=nu_coll)
 
 
+def mpcat_a_viscosity(n: float, dt: float, kbt: float = 1.0, m: float = 1.0, dim: int = 3) -> dict:
+    """kinematic viscosity of MPC-AT+a (Andersen thermostat with angular-momentum
+    conservation), cell size 1, Poisson-distributed cell populations of mean n:
+    the kinetic and collisional parts of Noguchi & Gompper, Phys. Rev. E 78,
+    016706 (2008), as theory_trans evaluates them with A = B = 1
+    (init.c:849-872; the smrtPow quirk of init.c:812 does not enter here)"""
+    e = math.exp(-n)
+    c = (1.0 - e * (1.0 + n)) + (1.0 + 1.0 / dim) * n * e / (dim + 2.0)
+    c += 0.5 * (dim - 0.5 * (3.0 * dim + 2.0)) * (1.0 - e * (1.0 + n + 0.5 * n * n)) / n
+    eta_kin = n * kbt * dt * (1.0 / c - 0.5)
+    eta_coll = m / (24.0 * dt) * (n - 7.0 / 5.0 + e * (7.0 / 5.0 + 2.0 * n / 5.0 + (1.0 / dim - 0.3) * n * n))
+    return dict(nu=(eta_kin + eta_coll) / (n * m), nu_kin=eta_kin / (n * m), nu_coll=eta_coll / (n * m))
+
+
 def periodic_walls(dim, L):
     ws = []
     for ax in range(dim):
@@ -53,9 +67,9 @@ def species(**kw):
     return sp
 
 
-def srd_cfg(L, ppc, dt=0.1, rotang=ROT130, kbt=1.0):
+def srd_cfg(L, ppc, dt=0.1, rotang=ROT130, kbt=1.0, rtech=0):
     return dict(dim=3, L=list(L), dt=dt, kbt=kbt, rotang=rotang, tau=0.5, grav=[0, 0, 0], avv=[0, 0, 0],
-                gndnst=float(ppc), rtech=0, lc=0, tstech=0, galinv=1, species=[species()],
+                gndnst=float(ppc), rtech=rtech, lc=0, tstech=0, galinv=1, species=[species()],
                 walls=periodic_walls(3, L))
 
 
@@ -66,11 +80,12 @@ def _cell_z(ctx, r, L):
 
 
 def shear_wave(M, precision, L=(64, 64, 32), ppc=10, nrep=8, steps=400, every=10, amp=0.2, seed=5,
-               dt=0.1, rotang=ROT130):
-    """decay of a transverse shear wave in nrep replicas; returns the fitted
-    kinematic viscosity (mean and standard error over replicas), the final
-    temperature and the total-momentum drift per particle"""
-    cfg = srd_cfg(L, ppc, dt=dt, rotang=rotang)
+               dt=0.1, rotang=ROT130, rtech=0):
+    """decay of a transverse shear wave in nrep replicas (collision operator
+    rtech: SRD ARBAXIS by default); returns the fitted kinematic viscosity
+    (mean and standard error over replicas), the final temperature and the
+    total-momentum drift per particle"""
+    cfg = srd_cfg(L, ppc, dt=dt, rotang=rotang, rtech=rtech)
     N = int(np.prod(L)) * ppc
     ctx = M.Context([cfg] * nrep, [N] * nrep, seed=seed, precision=precision)
     k = 2.0 * math.pi / L[2]
@@ -172,7 +187,10 @@ def nematic_order(M, precision, L=(64, 64), ppc=20, steps=300, every=20, start=1
 
 def fp32_report(M, quick: bool = True) -> dict:
     """the FP32-state observables with their relative errors against FP64"""
-    kw = dict(L=(64, 64, 32), nrep=64, steps=400) if quick else dict(L=(64, 64, 32), nrep=128, steps=400)
+    # the headline's box (64^3, 10 per cell); the wave's amplitude and the run's
+    # length set the statistical error of the decay rate
+    kw = dict(L=(64, 64, 64), nrep=32, steps=1200, every=40, amp=0.5) if quick else \
+        dict(L=(64, 64, 64), nrep=96, steps=1200, every=40, amp=0.5)
     w64 = shear_wave(M, M.FP64, **kw)
     w32 = shear_wave(M, M.FP32, **kw)
     th = srd_viscosity_3d(10.0, ROT130, 0.1)
@@ -185,8 +203,8 @@ def fp32_report(M, quick: bool = True) -> dict:
     paired = paired_rel_err(w32, w64)
 
     return dict(
-        workload=f"3D SRD {kw['L'][0]}x{kw['L'][1]}x{kw['L'][2]} x 10 ppc x {kw['nrep']} replicas, shear wave A=0.2 "
-                 f"along z, {kw['steps']} steps; "
+        workload=f"3D SRD {kw['L'][0]}x{kw['L'][1]}x{kw['L'][2]} x 10 ppc x {kw['nrep']} replicas, shear wave "
+                 f"A={kw['amp']} along z, {kw['steps']} steps; "
                  "2D active nematic 64^2 x 20 ppc (collision 16, U=10), 300 steps",
         temperature_fp64=w64["T"], temperature_fp32=w32["T"], temperature_rel_err=rel(w32["T"], w64["T"]),
         momentum_drift_per_particle_fp64=w64["momentum_drift"],
