@@ -295,6 +295,12 @@ mpcd_status mpcd_gpu_tangent(mpcd_gpu_ctx *ctx, int on, const double *dgrav, con
 mpcd_status mpcd_gpu_tangent_download(mpcd_gpu_ctx *ctx, int replica, double *dQ, double *dV, double *dU);
 
 /* --- observables ------------------------------------------------------------ */
+/* kbt_now [nrep]: KBTNOW of the last completed step of every replica, the
+ * temperature TEMP computes after the collision and after the thermostat's
+ * rescaling, before streaming (mpc.c:4752-4760) -- what the reference's
+ * energy output and checkpoint write (pout.c:2610, 2325).  0 before the first
+ * step. */
+mpcd_status mpcd_gpu_kbtnow(mpcd_gpu_ctx *ctx, double *kbt_now);
 /* per-cell populations, centre-of-mass velocities [3 per cell], scalar order
  * parameter and director [3 per cell] of replica `replica` (localPROP
  * results, mpc.c:64-216); any pointer may be NULL */

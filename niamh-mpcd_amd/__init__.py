@@ -122,7 +122,7 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
            "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
            "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read", "mpcd_gpu_lc_record",
-           "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc"]
+           "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc", "mpcd_gpu_kbtnow"]
 
 _lib = None
 
@@ -150,6 +150,7 @@ def lib():
     L.mpcd_gpu_synchronize.argtypes = [vp]
     L.mpcd_gpu_get_cells.argtypes = [vp, C.c_int, ip, dp, dp, dp]
     L.mpcd_gpu_lc_record.argtypes = [vp, C.c_int]
+    L.mpcd_gpu_kbtnow.argtypes = [vp, dp]
     L.mpcd_gpu_bin_noalloc.argtypes = [C.POINTER(Params), C.c_int64, vp, dp, vp, vp, vp, vp, vp, vp]
     L.mpcd_gpu_lc_decisions.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.mpcd_gpu_get_order.argtypes = [vp, C.c_int, ip, ip]
@@ -419,6 +420,12 @@ class Context:
         dQ, dV, dU = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
         self._check(self.L.mpcd_gpu_tangent_download(self.h, r, _dp(dQ), _dp(dV), _dp(dU)), "tangent_download")
         return dQ, dV, dU
+
+    def kbtnow(self):
+        """KBTNOW of the last completed step, per replica"""
+        k = np.zeros(self.nrep)
+        self._check(self.L.mpcd_gpu_kbtnow(self.h, _dp(k)), "kbtnow")
+        return k
 
     def lc_record(self, on=True):
         """record the nematic path's discrete decisions of every later step"""

@@ -2150,6 +2150,11 @@ extern "C" mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *c) {
   return check_flags(c);
 }
 
+extern "C" mpcd_status mpcd_gpu_kbtnow(mpcd_gpu_ctx *c, double *kbt_now) {
+  if (!c || !kbt_now) return c ? fail(c, MPCD_EINVAL, "kbtnow: null argument") : MPCD_EINVAL;
+  return read_kbt(c, kbt_now);
+}
+
 extern "C" mpcd_status mpcd_gpu_lc_record(mpcd_gpu_ctx *c, int on) {
   if (!c) return MPCD_EINVAL;
   if (c->lc == MPCD_ISOF) return fail(c, MPCD_EINVAL, "lc_record: the context has no nematic");

@@ -167,16 +167,15 @@ static void density_write(FILE *f, const int32_t *cnt, const double *mass, int n
  * one particle of S (u . n)^2 MFPOT over their particles; KBT = TEMP */
 static void energy_write(FILE *f, const mpcd_gpu_params *p, size_t n, const double *Q, const double *V,
                          const double *U, const int32_t *spid, const int32_t *pop, const double *S, const double *dir,
-                         double t) {
+                         double t, double kbt) {
     const int dim = p->dim, ny = p->L[1] + 1, nz = dim == 3 ? p->L[2] + 1 : 1;
-    double mpc_k = 0.0, wmf = 0.0, ke = 0.0;
+    double mpc_k = 0.0, wmf = 0.0;
     for (size_t i = 0; i < n; i++) {
         const int s = p->nspec > 1 ? spid[i] : 0;
         const double m = p->sp[s].mass;
         double e = 0.0;
         for (int j = 0; j < dim; j++) e += V[3 * i + j] * V[3 * i + j];
         mpc_k += 0.5 * m * e;
-        for (int j = 0; j < 3; j++) ke += m * (V[3 * i + j] - p->avv[j]) * (V[3 * i + j] - p->avv[j]);
         if (!p->lc) continue;
         const int a = (int)Q[3 * i], b = (int)Q[3 * i + 1], c = dim == 3 ? (int)Q[3 * i + 2] : 0;
         if (a < 0 || b < 0 || c < 0 || a >= p->L[0] || b >= p->L[1] || (dim == 3 && c >= p->L[2])) continue;
@@ -186,7 +185,6 @@ static void energy_write(FILE *f, const mpcd_gpu_params *p, size_t n, const doub
         for (int j = 0; j < dim; j++) un += U[3 * i + j] * dir[3 * k + j];
         wmf += S[k] * un * un * p->sp[s].mfpot;
     }
-    const double kbt = n ? ke / (dim * (double)n) : 0.0;
     fprintf(f, "%lf\t%lf\t%lf\t%lf\t%lf\t%lf\t%lf\n", t, mpc_k, wmf, 0.0, 0.0, mpc_k + wmf, kbt);
 }
 
@@ -220,13 +218,23 @@ static int steps_to_next(int done, int F) {
     return r + 1 - done;
 }
 
+/* TEMP (therm.c:209-235) of a state: sum m |v - AVV|^2 / (DIM n) */
+static double temp_of(const mpcd_gpu_params *p, size_t n, const double *V, const int32_t *spid) {
+    double ke = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const double m = p->sp[spid[i]].mass;
+        for (int d = 0; d < 3; d++) ke += m * (V[3 * i + d] - p->avv[d]) * (V[3 * i + d] - p->avv[d]);
+    }
+    return n ? ke / (p->dim * (double)n) : 0.0;
+}
+
 /* a checkpoint of one replica's state (Q, V, U in particle-index order):
- * the current temperature and mean velocity are computed here, as KBTNOW
- * and AVNOW of the reference's file */
+ * KBTNOW is the library's (the last step's temperature before streaming),
+ * AVNOW the mean velocity of the state */
 static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int warm_steps, int runtime, int warmtime,
                            long seed, int flow_out, int vel_out, int dir_out, int ckpt_out, int dens_out, size_t n,
                            const double *Q,
-                           const double *V, const double *U, const int32_t *spid) {
+                           const double *V, const double *U, const int32_t *spid, double kbtnow) {
     mpcd_ckpt ck;
     memset(&ck, 0, sizeof ck);
     ck.sim_steps = sim_steps;
@@ -244,17 +252,13 @@ static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int
     ck.V = (double *)V;
     ck.U = (double *)U;
     ck.spid = (int32_t *)spid;
-    double ke = 0.0, mom[3] = {0, 0, 0}, mtot = 0.0;
+    double mom[3] = {0, 0, 0}, mtot = 0.0;
     for (size_t i = 0; i < n; i++) {
         const double m = p->sp[spid[i]].mass;
-        for (int d = 0; d < 3; d++) {
-            const double dv = V[3 * i + d] - p->avv[d];
-            ke += m * dv * dv;
-            mom[d] += m * V[3 * i + d];
-        }
+        for (int d = 0; d < 3; d++) mom[d] += m * V[3 * i + d];
         mtot += m;
     }
-    ck.kbtnow = n ? ke / (p->dim * (double)n) : 0.0;  /* TEMP, therm.c:209-235 */
+    ck.kbtnow = kbtnow;  /* TEMP after the thermostat, mpc.c:4752-4760 */
     for (int d = 0; d < 3; d++) ck.avnow[d] = mtot > 0.0 ? mom[d] / mtot : 0.0;
     ckpt_write(f, p, &ck);
 }
@@ -616,7 +620,7 @@ int main(int argc, char **argv) {
         FILE *fc = fopen(ckpt_now, "w");
         if (!fc) die("cannot write %s", ckpt_now);
         checkpoint_out(fc, &p, sim_steps, warm, run0, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, n, Q, V, U,
-                       spid);
+                       spid, temp_of(&p, n, V, spid));
         fclose(fc);
         ckpt_free(&ck);
         return 0;
@@ -647,6 +651,8 @@ int main(int argc, char **argv) {
     const size_t nc = (size_t)mpcd_gpu_ncells(ctx, 0);
     double *flow = NULL, *cvcm = NULL, *cS = NULL, *cdir = NULL;
     int32_t *cpop = NULL;
+    double *kbt_now = (double *)calloc((size_t)nrep, sizeof(double));
+    if (!kbt_now) die("out of memory (%s)", "kbt_now");
     if ((flow_out > 0 || vel_out > 0 || do_dir || dens_out > 0 || en_out > 0) && nrep > 64)
         die("%s", "field outputs: at most 64 replicas");
     int32_t *dcnt = NULL;
@@ -741,12 +747,14 @@ int main(int argc, char **argv) {
                 }
                 density_write(fn[r], dcnt, dmass, p.nspec, p.L, dim, tnow);
             }
+        if ((en_out > 0 && rt % en_out == 0) || (ckpt_out > 0 && rt % ckpt_out == 0))
+            check(ctx, mpcd_gpu_kbtnow(ctx, kbt_now), "mpcd_gpu_kbtnow");
         if (en_out > 0 && rt % en_out == 0)
             for (int r = 0; r < nrep; r++) {
                 double *Qr = Q + 3 * n * r, *Vr = V + 3 * n * r, *Ur = U + 3 * n * r;
                 check(ctx, mpcd_gpu_download(ctx, r, Qr, Vr, p.lc ? Ur : NULL), "mpcd_gpu_download");
                 if (p.lc) check(ctx, mpcd_gpu_get_cells(ctx, r, cpop, cvcm, cS, cdir), "mpcd_gpu_get_cells");
-                energy_write(fe[r], &p, n, Qr, Vr, Ur, spid, cpop, cS, cdir, tnow);
+                energy_write(fe[r], &p, n, Qr, Vr, Ur, spid, cpop, cS, cdir, tnow, kbt_now[r]);
             }
         if (ckpt_out > 0 && rt % ckpt_out == 0) {
             /* runCheckpoint after the outputs of this runtime (mpcd.c:310-312): DIR/checkpoint.dat, replica 0;
@@ -757,7 +765,7 @@ int main(int argc, char **argv) {
             FILE *fc = fopen(cpath, "w");
             if (!fc) die("cannot write %s", cpath);
             checkpoint_out(fc, &p, sim_steps, 0, rt, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, n, Q, V, U,
-                           spid);
+                           spid, kbt_now[0]);
             fclose(fc);
         }
         if (done < next_out) continue;
@@ -789,6 +797,7 @@ int main(int argc, char **argv) {
     ckpt_free(&ck);
     free(flow);
     free(cpop);
+    free(kbt_now);
     free(cvcm);
     free(cS);
     free(cdir);

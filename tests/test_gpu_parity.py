@@ -116,6 +116,7 @@ def test_gpu_matches_reference(name):
         # KBTNOW and the cell values of the step's last localPROP (mpc.c:4942),
         # which the C host's outputs read through mpcd_gpu_get_cells
         close(kbt[0], g["kbt"][k], f"{name} step {k} KBTNOW")
+        close(ctx.kbtnow()[0], g["kbt"][k], f"{name} step {k} mpcd_gpu_kbtnow")
         cells = ctx.cells(0)
         np.testing.assert_array_equal(cells["pop"], g["pop"][k], err_msg=f"{name} step {k} get_cells pop")
         close(cells["vcm"], g["VCM"][k], f"{name} step {k} VCM")
