@@ -294,6 +294,21 @@ mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *ctx, const double *adj_in, doub
 mpcd_status mpcd_gpu_tangent(mpcd_gpu_ctx *ctx, int on, const double *dgrav, const double *dact);
 mpcd_status mpcd_gpu_tangent_download(mpcd_gpu_ctx *ctx, int replica, double *dQ, double *dV, double *dU);
 
+/* --- rest frame ------------------------------------------------------------- */
+/* The periodic Galilean transformation to the rest frame that the output
+ * step applies when runtime % zeroNetMom == 0 (outputResults pout.c:2557-2562:
+ * galileantrans mtools.c:673-718, then zeroExtraDims mtools.c:727-744).  For
+ * every replica the mass-weighted mean velocity of its particles less the
+ * replica's avv (in the DIM first components) is subtracted from every
+ * particle's velocity; in 2D the third components of Q and V are set to 0.
+ * Mobile walls and MD atoms are outside this library.  The caller applies
+ * the reference's conditions (rFrame on, zeroNetMom > 0, no gravity, more
+ * than 2 particles: init.c:2141-2156) and calls it after the step whose
+ * runtime is a multiple of zeroNetMom.  The cell values of the step
+ * (mpcd_gpu_get_cells) are those computed before it, as in outputResults.
+ * MPCD_EUNSUPPORTED while an adjoint tape or a tangent is recording. */
+mpcd_status mpcd_gpu_zero_net_momentum(mpcd_gpu_ctx *ctx);
+
 /* --- observables ------------------------------------------------------------ */
 /* kbt_now [nrep]: KBTNOW of the last completed step of every replica, the
  * temperature TEMP computes after the collision and after the thermostat's

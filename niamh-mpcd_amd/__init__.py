@@ -122,7 +122,8 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
            "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
            "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read", "mpcd_gpu_lc_record",
-           "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc", "mpcd_gpu_kbtnow"]
+           "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc", "mpcd_gpu_kbtnow",
+           "mpcd_gpu_zero_net_momentum"]
 
 _lib = None
 
@@ -151,6 +152,7 @@ def lib():
     L.mpcd_gpu_get_cells.argtypes = [vp, C.c_int, ip, dp, dp, dp]
     L.mpcd_gpu_lc_record.argtypes = [vp, C.c_int]
     L.mpcd_gpu_kbtnow.argtypes = [vp, dp]
+    L.mpcd_gpu_zero_net_momentum.argtypes = [vp]
     L.mpcd_gpu_bin_noalloc.argtypes = [C.POINTER(Params), C.c_int64, vp, dp, vp, vp, vp, vp, vp, vp]
     L.mpcd_gpu_lc_decisions.argtypes = [vp, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.mpcd_gpu_get_order.argtypes = [vp, C.c_int, ip, ip]
@@ -426,6 +428,11 @@ class Context:
         k = np.zeros(self.nrep)
         self._check(self.L.mpcd_gpu_kbtnow(self.h, _dp(k)), "kbtnow")
         return k
+
+    def zero_net_momentum(self):
+        """the rest-frame transformation of an output step (galileantrans,
+        mtools.c:673-718, + zeroExtraDims), on every replica"""
+        self._check(self.L.mpcd_gpu_zero_net_momentum(self.h), "zero_net_momentum")
 
     def lc_record(self, on=True):
         """record the nematic path's discrete decisions of every later step"""

@@ -133,6 +133,7 @@ struct mpcd_gpu_ctx {
   State<double> sd[2]{};
   State<float> sf[2]{};
   double *d_avpart = nullptr;
+  double *d_netmom = nullptr;  // mpcd_gpu_zero_net_momentum: 4 sums per slot block, then 4 per replica
   double *d_dshift = nullptr, *d_dcell = nullptr, *d_dghost = nullptr, *d_dpart = nullptr, *d_dlc = nullptr;
   // injected escape / re-thermalisation draws of the current step (null
   // arrays: the generator), and whether the step's streaming uses them
@@ -1244,6 +1245,44 @@ extern "C" mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *c, const mpcd_draws *draws
   st = check_launch(c);
   if (st) return st;
   return read_kbt(c, kbt_now_out);
+}
+
+// the periodic rest-frame transformation of the output step
+// (galileantrans mtools.c:673-718 + zeroExtraDims mtools.c:727-744, called
+// by outputResults pout.c:2557-2562 when runtime % zeroNetMom == 0)
+template <typename Real>
+static void run_netmom(mpcd_gpu_ctx *c) {
+  if (c->pending) {
+    // outputResults bins and runs localPROP before the transformation
+    // (pout.c:2540-2545): the cell values stay those of the step's velocities
+    run_bin<Real>(c, 2, true, 0);
+    run_props<Real>(c, 0);
+  }
+  State<Real> S = states<Real>(c)[c->cur];
+  double *part = c->d_netmom, *net = part + 4 * c->pblk.size();
+  k_netmom_part<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(c->C, S, c->d_pblk, part);
+  k_netmom_final<<<c->nrep, kBlock, 0, c->stream>>>(c->C, c->d_pblk_rng, part, net);
+  k_netmom_apply<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(c->C, S, c->d_pblk, net);
+}
+
+extern "C" mpcd_status mpcd_gpu_zero_net_momentum(mpcd_gpu_ctx *c) {
+  if (!c) return MPCD_EINVAL;
+  if (c->recording || c->tan_on)
+    return fail(c, MPCD_EUNSUPPORTED, "zero net momentum: not while an adjoint tape or a tangent is running");
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
+  if (!c->d_netmom) {
+    c->d_netmom = dalloc<double>(c, 4 * (c->pblk.size() + c->nrep));
+    if (!c->d_netmom) return fail(c, MPCD_ENOMEM, "net momentum sums");
+  }
+  if (c->precision == MPCD_FP64) run_netmom<double>(c);
+  else run_netmom<float>(c);
+  st = check_launch(c);
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MPCD_OK;
 }
 
 static mpcd_status read_events(mpcd_gpu_ctx *c, int64_t *out) {

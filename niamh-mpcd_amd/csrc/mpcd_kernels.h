@@ -1830,6 +1830,84 @@ __global__ __launch_bounds__(kBlock) void k_temp_final(Ctx C, const int2 *ranges
 }
 
 // ---------------------------------------------------------------------------
+// 11b. the periodic Galilean transformation to the rest frame (galileantrans,
+//      mtools.c:673-718, with zeroExtraDims, mtools.c:727-744): the
+//      mass-weighted mean velocity less AVV is taken off every particle
+// ---------------------------------------------------------------------------
+// per slot block: the sums of m and m v[d], d < 3, to part[4 * block]
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_netmom_part(Ctx C, State<Real> S, const BlkDev *blk, double *part) {
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
+  if (t < B.end) {
+    const double m = C.sp[S.sp[t]].mass;
+    x[0] = m;
+    x[1] = m * (double)S.vx[t];
+    x[2] = m * (double)S.vy[t];
+    if (C.rep[B.rep].dim == 3) x[3] = m * (double)S.vz[t];
+  }
+  __shared__ double s_w[4][kBlock / 64];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x[k] += __shfl_xor(x[k], o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[k][threadIdx.x >> 6] = x[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; w++) s += s_w[threadIdx.x][w];
+    part[4 * blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+// per replica (one workgroup each): NET[d] = sum m v[d] / sum m - AVV[d]
+// for d < DIM, to net[4 * r + d]
+__global__ __launch_bounds__(kBlock) void k_netmom_final(Ctx C, const int2 *ranges, const double *part, double *net) {
+  const int r = blockIdx.x;
+  const RepDev &R = C.rep[r];
+  const int2 rg = ranges[r];
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = rg.x + threadIdx.x; b < rg.y; b += kBlock)
+    for (int k = 0; k < 4; k++) x[k] += part[4 * b + k];
+  __shared__ double s_w[4][kBlock / 64];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x[k] += __shfl_xor(x[k], o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[k][threadIdx.x >> 6] = x[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s[4];
+    for (int k = 0; k < 4; k++) {
+      s[k] = 0.0;
+      for (int w = 0; w < kBlock / 64; w++) s[k] += s_w[k][w];
+    }
+    for (int d = 0; d < 3; d++) net[4 * r + d] = (d < R.dim) ? s[1 + d] / s[0] - R.avv[d] : 0.0;
+  }
+}
+
+// v[d] -= NET[d] for d < DIM; in 2D the third components of Q and V are set
+// to zero (zeroExtraDims)
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_netmom_apply(Ctx C, State<Real> S, const BlkDev *blk, const double *net) {
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const double *n = net + 4 * B.rep;
+  S.vx[t] = (Real)((double)S.vx[t] - n[0]);
+  S.vy[t] = (Real)((double)S.vy[t] - n[1]);
+  if (C.rep[B.rep].dim == 3) {
+    S.vz[t] = (Real)((double)S.vz[t] - n[2]);
+  } else {
+    S.z[t] = (Real)0;
+    S.vz[t] = (Real)0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 12. streaming and planar walls (stream_all mpc.c:680; MPC_BCcollision
 //     bc.c:1363-1496), then the unshifted cell of every particle
 // ---------------------------------------------------------------------------

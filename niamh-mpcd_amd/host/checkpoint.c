@@ -57,8 +57,8 @@ int ckpt_read(const char *path, mpcd_gpu_params *p, mpcd_ckpt *ck, char *err, si
     p->kbt = v[4];
     ck->kbtnow = v[5];
     rdn(&R, v, 6);  /* RFRAME, zeroNetMom, GALINV, TSTECH, RTECH, LC */
-    /* rFrame defaults to 1 (read.c:1062) but resets only every zeroNetMom > 0 steps */
-    if (R.ok && v[0] != 0.0 && v[1] > 0.0) FAIL("checkpoint: periodic zero-net-momentum resets are outside this path");
+    ck->rframe = (int)v[0];
+    ck->zero_net_mom = (int)v[1];
     p->galinv = (int)v[2];
     p->tstech = (int)v[3];
     p->rtech = (int)v[4];
@@ -190,7 +190,7 @@ void ckpt_write(FILE *f, const mpcd_gpu_params *p, const mpcd_ckpt *ck) {
     fprintf(f, "%d %lf\n", ck->warm_steps, p->dt);
     fprintf(f, "%ld\n", ck->seed);
     fprintf(f, "%d %d %d %d %lf %lf\n", dim, X, Y, Z, p->kbt, ck->kbtnow);
-    fprintf(f, "%d %d %d %d %d %d\n", 0, 0, p->galinv, p->tstech, p->rtech, p->lc);
+    fprintf(f, "%d %d %d %d %d %d\n", ck->rframe, ck->zero_net_mom, p->galinv, p->tstech, p->rtech, p->lc);
     fprintf(f, "%lf %lf %lf\n", p->tau, p->rotang, 1.0);
     fprintf(f, "%d %d %d %d\n", 0, 0, 0, 0);
     fprintf(f, "%lf %lf %lf\n", p->grav[0], p->grav[1], p->grav[2]);

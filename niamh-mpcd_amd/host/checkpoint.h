@@ -28,6 +28,7 @@ typedef struct {
     double kbtnow;               /* KBTNOW */
     double avnow[3];             /* AVNOW: mean velocity now */
     int flow_out, vel_out, dir_out, ckpt_out, dens_out;  /* FLOWOUT, VELOUT, ORDEROUT, CHCKPNT, DENSITYOUT */
+    int rframe, zero_net_mom;    /* RFRAME (after initializeSIM: the periodic reset is on), zeroNetMom */
     int64_t n;                   /* particles (GPOP) */
     double *Q, *V, *U;           /* n x 3 each, particle-index order */
     int32_t *spid;               /* n */

@@ -16,6 +16,12 @@
  * swimmers, curved or wavy walls) stop with a message, as the reference
  * stops on an unknown option.
  *
+ * rFrame (default 1) removes the net momentum of the initial state
+ * (galileantrans, init.c:2152); with zeroNetMom = Z > 0, no body force and
+ * more than 2 particles the library does it again after the step of every
+ * runtime divisible by Z, before that runtime's outputs (outputResults,
+ * pout.c:2557-2562; mpcd_gpu_zero_net_momentum).
+ *
  * Checkpoints in the reference's file layout (checkpoint.h): checkpointIn
  * recovers one as readchckpnt does (its inputs, particles, loop counters and
  * output flags; no warm-up; the temporal loop from its runtime), and
@@ -232,8 +238,8 @@ static double temp_of(const mpcd_gpu_params *p, size_t n, const double *V, const
  * KBTNOW is the library's (the last step's temperature before streaming),
  * AVNOW the mean velocity of the state */
 static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int warm_steps, int runtime, int warmtime,
-                           long seed, int flow_out, int vel_out, int dir_out, int ckpt_out, int dens_out, size_t n,
-                           const double *Q,
+                           long seed, int flow_out, int vel_out, int dir_out, int ckpt_out, int dens_out, int rframe,
+                           int znm, size_t n, const double *Q,
                            const double *V, const double *U, const int32_t *spid, double kbtnow) {
     mpcd_ckpt ck;
     memset(&ck, 0, sizeof ck);
@@ -247,6 +253,8 @@ static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int
     ck.dir_out = dir_out;
     ck.ckpt_out = ckpt_out;
     ck.dens_out = dens_out;
+    ck.rframe = rframe;
+    ck.zero_net_mom = znm;
     ck.n = (int64_t)n;
     ck.Q = (double *)Q;
     ck.V = (double *)V;
@@ -275,11 +283,8 @@ static void reject_unsupported(const json *J) {
     const json *sw = json_get(J, "swimmers");
     if (!sw) sw = json_get(J, "swimmer");
     if (sw && sw->type == JSON_ARR && sw->n > 0) die("%s are not supported", "swimmers");
-    /* swimmers (read.c:1781) and periodic zero-net-momentum resets (rFrame
-     * defaults to 1, read.c:1062, but resets only every zeroNetMom > 0 steps) */
+    /* swimmers (read.c:1781) */
     if (json_int(J, "nSwim", 0) > 0) die("%s", "\"nSwim\": swimmers are outside the solvent path");
-    if (json_int(J, "rFrame", 1) != 0 && json_int(J, "zeroNetMom", 0) > 0)
-        die("%s", "\"zeroNetMom\": periodic zero-net-momentum resets are not supported");
 }
 
 /* BC entry of the JSON (read.c:1136-1440) as a planar wall */
@@ -436,6 +441,11 @@ int main(int argc, char **argv) {
     json_nums(J, "grav", g, 3);
     memcpy(p.grav, g, sizeof g);
     p.precision = fp32 ? MPCD_FP32 : MPCD_FP64;
+    /* rest frame (read.c:1062-1063): rFrame (default 1) removes the net
+     * momentum of the initial state; with zeroNetMom = Z > 0 the output step
+     * of every runtime divisible by Z does it again */
+    int rframe = json_int(J, "rFrame", 1);
+    int znm = json_int(J, "zeroNetMom", 0);
     long seed = json_int(J, "seed", 0);
     if (seed_override >= 0) seed = seed_override;
     if (seed == 0) seed = (long)time(NULL);
@@ -509,22 +519,32 @@ int main(int argc, char **argv) {
         dir_out = ck.dir_out;
         ckpt_out = ck.ckpt_out;
         dens_out = ck.dens_out;
+        rframe = ck.rframe;
+        znm = ck.zero_net_mom;
         if (seed_override < 0 && ck.seed != 0) seed = ck.seed;
         ntot = ck.n;
         for (int s = 0; s < MPCD_MAXSPEC; s++) pop[s] = 0;
         for (int64_t i = 0; i < ck.n; i++) pop[ck.spid[i]]++;
     }
     if (steps_override >= 0) sim_steps = steps_override;
+    /* the initial transformation uses rFrame as read; from then on rFrame
+     * says whether the periodic one runs: not without zeroNetMom, not with a
+     * body force, not for 2 particles or fewer (init.c:2141-2156; a
+     * checkpoint holds the flag already reduced) */
+    const int rframe_init = rframe;
+    if (znm <= 0 || (!ckpt_in[0] && (ntot <= 2 || fabs(p.grav[0]) >= 1e-8 || fabs(p.grav[1]) >= 1e-8 ||
+                                      fabs(p.grav[2]) >= 1e-8)))
+        rframe = 0;
 
     if (dump) {
         /* the parameters as read, without touching the GPU */
         printf("{\"dim\": %d, \"L\": [%d, %d, %d], \"dt\": %.17g, \"kbt\": %.17g, \"rotang\": %.17g, "
                "\"tau\": %.17g, \"grav\": [%.17g, %.17g, %.17g], \"rtech\": %d, \"lc\": %d, \"tstech\": %d, "
                "\"galinv\": %d, \"simSteps\": %d, \"warmUp\": %d, \"flowOut\": %d, \"velOut\": %d, \"dirSOut\": %d, "
-               "\"densOut\": %d, \"checkpointOut\": %d, \"runtime\": %d, \"gndnst\": %.17g, \"avv\": [%.17g, %.17g, %.17g], "
+               "\"densOut\": %d, \"checkpointOut\": %d, \"rFrame\": %d, \"zeroNetMom\": %d, \"runtime\": %d, \"gndnst\": %.17g, \"avv\": [%.17g, %.17g, %.17g], "
                "\"seed\": %ld, \"species\": [",
                p.dim, p.L[0], p.L[1], p.L[2], p.dt, p.kbt, p.rotang, p.tau, p.grav[0], p.grav[1], p.grav[2], p.rtech,
-               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, vel_out, dir_out, dens_out, ckpt_out, run0, p.gndnst, p.avv[0],
+               p.lc, p.tstech, p.galinv, sim_steps, warm, flow_out, vel_out, dir_out, dens_out, ckpt_out, rframe, znm, run0, p.gndnst, p.avv[0],
                p.avv[1], p.avv[2], seed);
         for (int s = 0; s < p.nspec; s++) {
             const mpcd_gpu_species *sp = &p.sp[s];
@@ -605,8 +625,11 @@ int main(int argc, char **argv) {
                 const int a = (int)Qr[3 * i], b = (int)Qr[3 * i + 1], c = dim == 3 ? (int)Qr[3 * i + 2] : 0;
                 ccount[(a * p.L[1] + b) * p.L[2] + c]++;
             }
-        for (size_t k = 0; k < n; k++)
-            for (int d = 0; d < 3; d++) Vr[3 * k + d] -= mom[d] / mtot;
+        /* galileantrans of the initial state when rFrame is on and there are
+         * more than 2 particles (init.c:2141-2152) */
+        if (rframe_init && n > 2)
+            for (size_t k = 0; k < n; k++)
+                for (int d = 0; d < 3; d++) Vr[3 * k + d] -= mom[d] / mtot;
         /* GnDNST: mean population of the occupied cells (globalDensities, init.c:995-1018) */
         double occ = 0.0;
         for (int k = 0; k < p.L[0] * p.L[1] * p.L[2]; k++) occ += ccount[k] > 0;
@@ -619,7 +642,8 @@ int main(int argc, char **argv) {
         /* the state the run would start from, as a checkpoint; no GPU */
         FILE *fc = fopen(ckpt_now, "w");
         if (!fc) die("cannot write %s", ckpt_now);
-        checkpoint_out(fc, &p, sim_steps, warm, run0, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, n, Q, V, U,
+        checkpoint_out(fc, &p, sim_steps, warm, run0, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, rframe,
+                       znm, n, Q, V, U,
                        spid, temp_of(&p, n, V, spid));
         fclose(fc);
         ckpt_free(&ck);
@@ -702,8 +726,8 @@ int main(int argc, char **argv) {
     int next_out = every < total ? every : total;
     for (int done = 0; done < total;) {
         int k = next_out - done;
-        const int cad[6] = {flow_out, vel_out, do_dir ? dir_out : 0, ckpt_out, dens_out, en_out};
-        for (int q = 0; q < 6; q++)
+        const int cad[7] = {flow_out, vel_out, do_dir ? dir_out : 0, ckpt_out, dens_out, en_out, rframe ? znm : 0};
+        for (int q = 0; q < 7; q++)
             if (cad[q] > 0 && steps_to_next(run0 + done, cad[q]) < k) k = steps_to_next(run0 + done, cad[q]);
         struct timespec a, b;
         clock_gettime(CLOCK_MONOTONIC, &a);
@@ -714,6 +738,9 @@ int main(int argc, char **argv) {
         done += k;
         const int rt = run0 + done - 1;
         const double tnow = (double)rt * p.dt;
+        /* outputResults removes the net momentum before its outputs
+         * (pout.c:2557-2562); the cell values stay the step's */
+        if (rframe && rt % znm == 0) check(ctx, mpcd_gpu_zero_net_momentum(ctx), "mpcd_gpu_zero_net_momentum");
         if (flow_out > 0 && rt % flow_out == 0) {
             /* flowout at this runtime (pout.c:2690), then the sums restart */
             for (int r = 0; r < nrep; r++) {
@@ -764,7 +791,8 @@ int main(int argc, char **argv) {
             snprintf(cpath, sizeof cpath, "%s/checkpoint.dat", outdir);
             FILE *fc = fopen(cpath, "w");
             if (!fc) die("cannot write %s", cpath);
-            checkpoint_out(fc, &p, sim_steps, 0, rt, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, n, Q, V, U,
+            checkpoint_out(fc, &p, sim_steps, 0, rt, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, rframe,
+                           znm, n, Q, V, U,
                            spid, kbt_now[0]);
             fclose(fc);
         }

@@ -1816,6 +1816,28 @@ void orc_get_walls(const orc_sys *s, double *wq)
     }
 }
 
+/* galileantrans (mtools.c:673-718) on the particles, then zeroExtraDims
+ * (mtools.c:727-744): the mass-weighted mean velocity less AVV, summed in
+ * particle-index order, is subtracted in the DIM first components; in 2D the
+ * third components of Q and V are zeroed.  Called by outputResults
+ * (pout.c:2557-2562) after the timestep of a runtime divisible by zeroNetMom. */
+void orc_zero_net_momentum(orc_sys *s)
+{
+    double net[3] = {0., 0., 0.}, totm = 0.;
+    int i, j, dim = s->p.dim;
+    for (i = 0; i < s->N; i++) {
+        double m = s->p.sp[s->spid[i]].mass;
+        totm += m;
+        for (j = 0; j < dim; j++) net[j] += m * s->V[3 * i + j];
+    }
+    for (j = 0; j < dim; j++) net[j] /= totm;
+    for (j = 0; j < dim; j++) net[j] -= s->p.avv[j];
+    for (i = 0; i < s->N; i++)
+        for (j = 0; j < dim; j++) s->V[3 * i + j] -= net[j];
+    if (dim < 3)
+        for (i = 0; i < s->N; i++) s->Q[3 * i + 2] = s->V[3 * i + 2] = 0.;
+}
+
 double orc_kbtnow(const orc_sys *s) { return s->kbtnow; }
 double orc_avs(const orc_sys *s) { return s->avs; }
 int orc_lc_stride(const orc_sys *s) { return s->lc_stride; }

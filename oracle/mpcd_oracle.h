@@ -122,6 +122,9 @@ int      orc_phase_bin(orc_sys *s, orc_draws *d);
 int      orc_phase_collide(orc_sys *s, orc_draws *d);
 int      orc_phase_stream(orc_sys *s, orc_draws *d);
 void     orc_last_shift(const orc_sys *s, double *shift);
+/* the rest-frame transformation of an output step (galileantrans +
+ * zeroExtraDims, mtools.c:673-744) */
+void     orc_zero_net_momentum(orc_sys *s);
 /* flow output: with on, every step ends with localFLOW + sumFLOW
  * (mpc.c:226-246, 4946-4951); get returns FLOW [ncell*3] and the steps summed */
 void     orc_flow(orc_sys *s, int on);

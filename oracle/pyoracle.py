@@ -80,6 +80,7 @@ def lib():
         L.orc_avs.argtypes = [vp]
         L.orc_counters.argtypes = [vp, C.POINTER(C.c_long)]
         L.orc_lc_stride.argtypes = [vp]
+        L.orc_zero_net_momentum.argtypes = [vp]
         for f in ("orc_step", "orc_phase_bin", "orc_phase_collide", "orc_phase_stream"):
             getattr(L, f).argtypes = [vp, vp]
         L.orc_last_shift.argtypes = [vp, dp]
@@ -258,6 +259,10 @@ class System:
 
     def avs(self):
         return self.L.orc_avs(self.h)
+
+    def zero_net_momentum(self):
+        """galileantrans + zeroExtraDims (mtools.c:673-744)"""
+        self.L.orc_zero_net_momentum(self.h)
 
     def counters(self):
         out = (C.c_long * 4)()

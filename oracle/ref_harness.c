@@ -31,6 +31,9 @@
  * With "ckpt" the reference's own checkpoint() (pout.c:2325-2404) writes
  * OUT.bin.ckpt after the last step (runtime = the last step, warmtime 0,
  * simSteps and warmupSteps as given by the case's "simsteps"/"warmup" keys).
+ * A case key "zeronetmom Z" (Z > 0) follows every step s with s % Z == 0 by
+ * what outputResults does for rFrame (pout.c:2540-2562): bin(unshifted),
+ * localPROP, galileantrans and zeroExtraDims.
  */
 #define _POSIX_C_SOURCE 199309L
 #include <math.h>
@@ -49,6 +52,7 @@
 #include "therm.h"
 #include "lc.h"
 #include "pout.h"
+#include "mtools.h"
 
 void ref_rand_seed(uint64_t seed);
 void ref_rand_counts(uint64_t out[3]);
@@ -58,7 +62,7 @@ void globalDensities(cell ***CL, spec SP[]);
 #define MAXW 16
 
 typedef struct {
-    int dim, L[3], N, steps, rtech, lc, tstech, galinv, nspec, nwall, simsteps, warmup;
+    int dim, L[3], N, steps, rtech, lc, tstech, galinv, nspec, nwall, simsteps, warmup, zeronetmom;
     double dt, kbt, rotang, tau, grav[3], avv[3];
     unsigned long long seed;
     double spv[MAXSPECI][12];
@@ -85,6 +89,7 @@ static void read_case(const char *path, hcase *c)
         else if (!strcmp(key, "steps")) { if (fscanf(f, "%d", &c->steps) != 1) die("steps"); }
         else if (!strcmp(key, "simsteps")) { if (fscanf(f, "%d", &c->simsteps) != 1) die("simsteps"); }
         else if (!strcmp(key, "warmup")) { if (fscanf(f, "%d", &c->warmup) != 1) die("warmup"); }
+        else if (!strcmp(key, "zeronetmom")) { if (fscanf(f, "%d", &c->zeronetmom) != 1) die("zeronetmom"); }
         else if (!strcmp(key, "rtech")) { if (fscanf(f, "%d", &c->rtech) != 1) die("rtech"); }
         else if (!strcmp(key, "lc")) { if (fscanf(f, "%d", &c->lc) != 1) die("lc"); }
         else if (!strcmp(key, "tstech")) { if (fscanf(f, "%d", &c->tstech) != 1) die("tstech"); }
@@ -306,6 +311,13 @@ int main(int argc, char **argv)
         clock_gettime(CLOCK_MONOTONIC, &t0);
         for (s = 0; s < hc.steps; s++) {
             timestep(CL, pp, SP, WALL, NULL, &SS, NULL, AVNOW, AVV, avDIR, in, &KBTNOW, &AVS, s, 0, of, ofl);
+            if (hc.zeronetmom > 0 && s % hc.zeronetmom == 0) {
+                /* the rest-frame part of outputResults (pout.c:2540-2562) at runtime s */
+                bin(CL, SP, WALL, in.KBT, in.LC, 0);
+                localPROP(CL, SP, SS, in.RTECH, in.LC);
+                galileantrans(pp, WALL, NULL, SP, in.KBT, AVV, GPOP, NBC, 0, DIM);
+                zeroExtraDims(pp, WALL, NULL, GPOP, NBC, 0, DIM);
+            }
             if (!timing) dump(o, s, KBTNOW, pp, CL);
         }
         clock_gettime(CLOCK_MONOTONIC, &t1);

@@ -54,7 +54,7 @@ def periodic(dim, L):
 
 
 def case(name, dim, L, ppc, rtech, steps, seed, lc=0, tstech=0, rotang=math.pi / 2, dt=0.1, kbt=1.0,
-         grav=(0, 0, 0), walls=None, sp=None, galinv=1, tau=0.5, qbox=None, kick=None):
+         grav=(0, 0, 0), walls=None, sp=None, galinv=1, tau=0.5, qbox=None, kick=None, zeronetmom=0, avv=None):
     c = dict(name=name, dim=dim, L=list(L) + [1] * (3 - len(L)), ppc=ppc, rtech=rtech, steps=steps,
              seed=seed, lc=lc, tstech=tstech, rotang=rotang, dt=dt, kbt=kbt, grav=list(grav),
              walls=walls if walls is not None else periodic(dim, L), species=[sp or species()],
@@ -63,6 +63,12 @@ def case(name, dim, L, ppc, rtech, steps, seed, lc=0, tstech=0, rotang=math.pi /
         c["qbox"] = [list(map(float, b)) for b in qbox]  # initial positions: [lo, hi) per axis
     if kick is not None:
         c["kick"] = [[int(i), [float(x) for x in v]] for i, v in kick]  # particle i starts with velocity v
+    if zeronetmom:
+        # after every step s with s % zeronetmom == 0: the rest-frame transformation of outputResults
+        # (galileantrans + zeroExtraDims, pout.c:2557-2562)
+        c["zeronetmom"] = int(zeronetmom)
+    if avv is not None:
+        c["avv"] = [float(x) for x in avv]  # the AVV given to the reference instead of the initial mean velocity
     return c
 
 
@@ -129,9 +135,31 @@ CASES += [
                                                           (3, (0.0, -1.8e5))]),
 ]
 
+# the periodic rest-frame transformation (galileantrans, mtools.c:673-718,
+# with zeroExtraDims) of the output step, every zeronetmom steps; the walls
+# exchange momentum with the fluid, so the net momentum drifts between resets
+CASES += [
+    # 2D SRD Couette, two species of different mass, a given AVV: reset after steps 0, 2 and 4
+    case("srd2d_couette_znm", 2, (8, 6), 10, rtech=0, steps=5, seed=29, zeronetmom=2, avv=(0.05, -0.02, 0.0),
+         walls=pbc_walls(0, 8) + [dict(A=[0, 1, 0], R=0.0, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1),
+                                  dict(A=[0, -1, 0], R=-6.0, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1,
+                                       V=[0.3, 0, 0])]),
+    # 3D MPC-AT channel with bounce-back walls in z, two species: reset after every step
+    case("at3d_channel_znm", 3, (5, 4, 5), 7, rtech=2, steps=3, seed=30, zeronetmom=1, avv=(0.0, 0.01, 0.0),
+         walls=pbc_walls(0, 5) + pbc_walls(1, 4) + solid_walls(2, 5)),
+]
+
 # rat2d_2sp uses two species
 CASES[6]["species"] = [species(mass=1.0), species(mass=2.5)]
 next(c for c in CASES if c["name"] == "at2d_nbounce")["species"] = [species(vdist=v) for v in (0, 2, 3, 4)]
+for _n in ("srd2d_couette_znm", "at3d_channel_znm"):
+    next(c for c in CASES if c["name"] == _n)["species"] = [species(mass=1.0), species(mass=1.7)]
+
+
+def rest_frame_due(c: dict, k: int) -> bool:
+    """whether the output step after step k (runtime k) applies the rest-frame transformation"""
+    z = c.get("zeronetmom", 0)
+    return z > 0 and k % z == 0
 
 
 def initial_state(c: dict):
@@ -160,6 +188,8 @@ def initial_state(c: dict):
         U[:, 1] = np.sin(th)
     avv = np.zeros(3)
     avv[:dim] = V[:, :dim].mean(axis=0)
+    if "avv" in c:
+        avv = np.array(c["avv"], dtype=float)
     for i, v in c.get("kick", []):
         V[i, :dim] = v[:dim]
     return Q, V, U, spid, avv

@@ -62,6 +62,8 @@ def write_case(c, Q, V, U, spid, avv, d):
              f"dt {c['dt']!r}", f"kbt {c['kbt']!r}", f"rotang {c['rotang']!r}", f"tau {c['tau']!r}",
              "grav {!r} {!r} {!r}".format(*[float(x) for x in c["grav"]]),
              "avv {!r} {!r} {!r}".format(*[float(x) for x in avv]), f"seed {c['seed']}"]
+    if c.get("zeronetmom"):
+        lines.append(f"zeronetmom {c['zeronetmom']}")
     for sp in c["species"]:
         lines.append("species " + " ".join(repr(float(sp[k])) for k in (
             "mass", "rfc", "tumble", "chihi", "act", "mfpot", "damp", "sigwidth", "sigpos", "minactratio",

@@ -104,6 +104,45 @@ def test_domain_walls_follow_the_bc_list(tmp_path):
     assert len(dump(tmp_path, dict(domain=[10, 6]))["walls"]) == 4
 
 
+def test_rest_frame_flag(tmp_path):
+    """rFrame (default 1) stays on for the periodic reset only with
+    zeroNetMom > 0 and no body force (init.c:2155-2156)"""
+    d = dump(tmp_path, dict(domain=[8, 8], zeroNetMom=100))
+    assert (d["rFrame"], d["zeroNetMom"]) == (1, 100)
+    assert dump(tmp_path, dict(domain=[8, 8]))["rFrame"] == 0
+    assert dump(tmp_path, dict(domain=[8, 8], zeroNetMom=100, rFrame=0))["rFrame"] == 0
+    assert dump(tmp_path, dict(domain=[8, 8], zeroNetMom=100, grav=[1e-3, 0, 0]))["rFrame"] == 0
+
+
+@pytest.mark.gpu
+def test_host_zero_net_momentum(tmp_path):
+    """A channel between bounce-back walls: the walls change the total
+    momentum every step.  With zeroNetMom = 3 the output step of every
+    runtime divisible by 3 takes it back to zero (galileantrans,
+    pout.c:2557-2562), before the solvent.dat line of that runtime."""
+    cfg = {
+        "domain": [24, 16], "simSteps": 99, "collOp": 2, "seed": 5, "zeroNetMom": 3,
+        "species": [{"dens": 10, "vDist": 4, "mass": 1.0}, {"dens": 4, "vDist": 4, "mass": 2.5}],
+        "BC": [
+            {"aInv": [1, 0, 0], "Q": [0, 0, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 24},
+            {"aInv": [-1, 0, 0], "Q": [24, 0, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 24},
+            {"aInv": [0, 1, 0], "Q": [0, 0, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 0, "MVN": -1, "MVT": -1,
+             "phantom": 1},
+            {"aInv": [0, -1, 0], "Q": [0, 16, 0], "P": [1, 1, 1, 1], "R": 0, "DN": 0, "MVN": -1, "MVT": -1,
+             "phantom": 1},
+        ],
+    }
+    print("[host] 2D channel 24x16, two species, zeroNetMom 3, 100 steps", flush=True)
+    info, rows = run_host(tmp_path, cfg, "--every", "10")
+    assert [int(x[0]) for x in rows] == list(range(9, 100, 10))
+    reset = [r for r in rows if int(r[0]) % 3 == 0]
+    other = [r for r in rows if int(r[0]) % 3 != 0]
+    assert len(reset) == 4
+    for r in reset:
+        assert abs(r[3]) < 1e-9 and abs(r[4]) < 1e-9, r
+    assert max(abs(r[3]) + abs(r[4]) for r in other) > 1e-3
+
+
 def test_rtech_alone(tmp_path):
     assert dump(tmp_path, dict(domain=[8, 8], rTech=1))["rtech"] == 1
     assert dump(tmp_path, dict(domain=[8, 8], collOp=0))["rtech"] == 0
@@ -113,7 +152,6 @@ def test_rtech_alone(tmp_path):
     (dict(domain=[8, 8], mdIn="md.inp"), "mdIn"),
     (dict(domain=[8, 8], checkpointIn="c.dat"), "cannot open the checkpoint"),
     (dict(domain=[8, 8], nSwim=2), "nSwim"),
-    (dict(domain=[8, 8], zeroNetMom=100), "zeroNetMom"),
     (dict(domain=[8, 8], BC=[dict(aInv=[0.25, 0.25, 0], Q=[4, 4, 0], P=[2, 2, 1, 1], R=4)]), "not an axis-aligned"),
     (dict(domain=[8, 8], BC=[dict(aInv=[1, 0, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], wavy=[0.5, 1, 0])]),
      "not an axis-aligned"),
