@@ -263,20 +263,51 @@ double      mpcd_gpu_tile_fill(const mpcd_gpu_ctx *ctx);
  * counterpart in the reference (its checkpoints are files, pout.c:2325). */
 mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *ctx, int slot);
 mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *ctx, int slot);
-/* Pathwise adjoint of streaming plus collision (SRD and MPC-AT, tsTech 0,
- * resting walls) with cell assignments, draws and wall events held fixed.
- * mpcd_gpu_record(ctx, k) starts a tape of the next k steps (the steps run on
- * the unfused path while recording; k = 0 stops recording).
- * mpcd_gpu_step_backward pops the last taped step: adj_in [3 * n_total]
- * (particle-index order, replicas concatenated; may be NULL) is dJ/dv of the
- * observable at the end of that step and is added to the carried lambda
- * = dJ/dv; adj_out [3 * n_total] receives lambda at the start of the step
- * (may be NULL); dJ_dtheta [4 * nrep] receives the running sums of
- * dJ/dGRAV[0..2] and dJ/dRA per replica since mpcd_gpu_adjoint_reset (may be
- * NULL).  No counterpart in the reference. */
+/* Pathwise adjoint of the step with cell assignments, draws, wall events and
+ * every other discrete decision held fixed (FP64 state, tsTech 0, lc 0 or 1).
+ * mpcd_gpu_record(ctx, k) starts a tape of the next k steps taken with the
+ * library's generator (the steps run on the unfused path while recording;
+ * k = 0 stops recording).  Two backward entry points pop the last taped step:
+ *
+ * mpcd_gpu_step_backward_state: every operator of the unfused path (SRD,
+ * MPC-AT, MPC-AT+a, the nematic collision with Jeffery rotation and dipole
+ * activity, ghost particles and anchoring at phantom walls, planar walls).
+ * adj_in [9 * n_total] (particle-index order, replicas concatenated; per
+ * particle dJ/dQ[3], dJ/dV[3], dJ/dU[3]; may be NULL) is the observable's
+ * gradient at the end of that step and is added to the carried adjoint;
+ * adj_out [9 * n_total] receives the adjoint at the start of the step (may be
+ * NULL); dJ_dtheta [MPCD_NTHETA * nrep] receives per replica the running sums
+ * since mpcd_gpu_adjoint_reset of dJ/dGRAV[0..2], 0 (the rotation angle is
+ * not a variable here), and dJ/dACT of each species (may be NULL).  The step
+ * is replayed from its recorded start state, one kernel at a time, and each
+ * kernel's reverse-mode instance (hand-written reverse arithmetic, mpcd_rev.h)
+ * turns the adjoints of its outputs into those of its inputs; the context's
+ * own state is left as it was.
+ *
+ * mpcd_gpu_step_backward: SRD and MPC-AT between resting walls, velocities
+ * only: adj_in / adj_out [3 * n_total] are dJ/dV; dJ_dtheta [4 * nrep]
+ * receives dJ/dGRAV[0..2] and dJ/dRA per replica.
+ * No counterpart in the reference. */
+#define MPCD_NTHETA (4 + MPCD_MAXSPEC)
 mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *ctx, int max_steps);
 mpcd_status mpcd_gpu_adjoint_reset(mpcd_gpu_ctx *ctx);
 mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *ctx, const double *adj_in, double *adj_out, double *dJ_dtheta);
+mpcd_status mpcd_gpu_step_backward_state(mpcd_gpu_ctx *ctx, const double *adj_in, double *adj_out,
+                                         double *dJ_dtheta);
+/* Observables of the current state for the time-averaged objectives of the
+ * adjoint, per replica (value [nrep]) with their gradient (grad [9 * n_total]
+ * in the layout of adj_in above, FP64 state; may be NULL):
+ *   MPCD_OBS_BAND_VELOCITY, arg = {component, axis, lo, hi}: the mean of
+ *     V[component] over the particles whose cell index (int)Q[axis] lies in
+ *     [lo, hi) -- a channel's centreline velocity;
+ *   MPCD_OBS_MEAN_S (lc 1): the mean scalar order parameter S of the cells
+ *     holding more than one particle (localPROP, mpc.c:183-207) in the
+ *     unshifted binning that ends the step.
+ * The step's pending re-bin is done first, as mpcd_gpu_get_cells does.  No
+ * counterpart in the reference. */
+#define MPCD_OBS_BAND_VELOCITY 0
+#define MPCD_OBS_MEAN_S 1
+mpcd_status mpcd_gpu_observable(mpcd_gpu_ctx *ctx, int kind, const double *arg, double *value, double *grad);
 /* Forward-mode tangent of the whole step (every operator of the unfused path:
  * SRD, MPC-AT, MPC-AT+a, the nematic LC collision with Jeffery rotation and
  * dipole activity, ghost particles and anchoring at phantom walls, planar

@@ -123,7 +123,10 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
            "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read", "mpcd_gpu_lc_record",
            "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc", "mpcd_gpu_kbtnow",
-           "mpcd_gpu_zero_net_momentum"]
+           "mpcd_gpu_zero_net_momentum", "mpcd_gpu_step_backward_state", "mpcd_gpu_observable"]
+
+NTHETA = 4 + MAXSPEC  # MPCD_NTHETA: parameter adjoints per replica
+OBS_BAND_VELOCITY, OBS_MEAN_S = 0, 1
 
 _lib = None
 
@@ -170,6 +173,8 @@ def lib():
     L.mpcd_gpu_record.argtypes = [vp, C.c_int]
     L.mpcd_gpu_adjoint_reset.argtypes = [vp]
     L.mpcd_gpu_step_backward.argtypes = [vp, dp, dp, dp]
+    L.mpcd_gpu_step_backward_state.argtypes = [vp, dp, dp, dp]
+    L.mpcd_gpu_observable.argtypes = [vp, C.c_int, dp, dp, dp]
     L.mpcd_gpu_tangent.argtypes = [vp, C.c_int, dp, dp]
     L.mpcd_gpu_upload_aos.argtypes = [vp, C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                                       C.c_int64]
@@ -398,6 +403,29 @@ class Context:
 
     def adjoint_reset(self):
         self._check(self.L.mpcd_gpu_adjoint_reset(self.h), "adjoint_reset")
+
+    def step_backward_state(self, adj_in=None, want_lambda=False):
+        """pops one taped step (every operator); adj_in [n, 9] = dJ/d(Q, V, U) at
+        the end of that step; returns (the adjoint [n, 9] at its start or None,
+        dJ/dtheta [nrep, NTHETA]: GRAV[0..2], 0, ACT of each species)"""
+        n = int(self.n.sum())
+        a = None if adj_in is None else np.ascontiguousarray(adj_in, np.float64).reshape(-1)
+        if a is not None and a.size != 9 * n:
+            raise ValueError("adj_in must hold 9 values per particle")
+        lam = np.zeros(9 * n) if want_lambda else None
+        th = np.zeros(NTHETA * self.nrep)
+        self._check(self.L.mpcd_gpu_step_backward_state(self.h, _dp(a), _dp(lam), _dp(th)), "step_backward_state")
+        return (None if lam is None else lam.reshape(n, 9)), th.reshape(self.nrep, NTHETA)
+
+    def observable(self, kind, arg=None, grad=False):
+        """(value [nrep], gradient [n, 9] or None) of OBS_BAND_VELOCITY (arg =
+        component, axis, lo, hi) or OBS_MEAN_S for the current state"""
+        n = int(self.n.sum())
+        a = None if arg is None else np.ascontiguousarray(arg, np.float64)
+        val = np.zeros(self.nrep)
+        g = np.zeros(9 * n) if grad else None
+        self._check(self.L.mpcd_gpu_observable(self.h, int(kind), _dp(a), _dp(val), _dp(g)), "observable")
+        return val, (None if g is None else g.reshape(n, 9))
 
     def step_backward(self, adj_in=None, want_lambda=False):
         """pops one taped step; returns (lambda or None, dJ/dtheta [nrep, 4])"""

@@ -15,6 +15,15 @@ those of the recorded run (pathwise gradient).
 `CentrelineVelocity` wraps the two passes as a torch.autograd.Function of the
 parameter vector (g_x, RA).
 
+`ReverseRun` is the same checkpoint-and-recompute scheme for every operator
+of the step, the nematic LC collision with Jeffery rotation and dipole
+activity included, through mpcd_gpu_step_backward_state (the adjoint of
+positions, velocities and orientations): observables are the band velocity
+or the mean scalar order parameter of the cells (mpcd_gpu_observable), and
+the gradient is taken with respect to the body force and each species'
+activity.  `ActiveChannelReverse` wraps it as a torch.autograd.Function of
+(g_x, activity).
+
 The forward-mode tangent (mpcd_gpu_tangent) gives the same pathwise
 derivative for every operator of the step, including the nematic LC
 collision with activity: `tangent_value_and_grad` runs the K steps once per
@@ -77,6 +86,66 @@ class AdjointRun:
 
     def close(self):
         self.ctx.close()
+
+
+class ReverseRun:
+    """J = (1/K) sum_t obs(state after step t) and its pathwise gradient in
+    (GRAV, ACT) by the reverse mode of the whole step, checkpointing every
+    `ckpt` steps.  obs = (kind, arg) of mpcd_gpu_observable."""
+
+    def __init__(self, M, cfg, Q0, V0, K, seed, obs, U0=None, spid=None, ckpt=20):
+        self.M, self.K, self.ckpt, self.obs = M, K, ckpt, obs
+        self.N = int(Q0.shape[0])
+        self.ctx = M.Context(cfg, self.N, seed=seed, precision=M.FP64)
+        self.ctx.upload(0, Q0, V0, U=U0, spid=spid)
+        self.J = None
+
+    def forward(self) -> float:
+        ctx = self.ctx
+        ctx.record(1)  # recording keeps every step on the path the replay uses
+        J = 0.0
+        for t in range(self.K):
+            if t % self.ckpt == 0:
+                ctx.checkpoint(t // self.ckpt)
+            ctx.step()
+            v, _ = ctx.observable(*self.obs)
+            J += v[0] / self.K
+        self.J = J
+        return J
+
+    def backward(self):
+        """(dJ/dGRAV [3], dJ/dACT [per species]) of replica 0"""
+        ctx, K = self.ctx, self.K
+        ctx.adjoint_reset()
+        nseg = (K + self.ckpt - 1) // self.ckpt
+        th = np.zeros((1, self.M.NTHETA))
+        for s in reversed(range(nseg)):
+            t0, t1 = s * self.ckpt, min(K, (s + 1) * self.ckpt)
+            ctx.restore(s)
+            ctx.record(t1 - t0)
+            grads = []
+            for _ in range(t0, t1):
+                ctx.step()
+                _, g = ctx.observable(*self.obs, grad=True)
+                grads.append(g / K)
+            for g in reversed(grads):
+                _, th = ctx.step_backward_state(g)
+        ctx.record(0)
+        return th[0, 0:3].copy(), th[0, 4:].copy()
+
+    def close(self):
+        self.ctx.close()
+
+
+def reverse_value_and_grad(M, cfg, Q0, V0, K, seed, obs, U0=None, spid=None, ckpt=20):
+    """J, dJ/dGRAV [3], dJ/dACT [per species] by the reverse mode"""
+    run = ReverseRun(M, cfg, Q0, V0, K, seed, obs, U0=U0, spid=spid, ckpt=ckpt)
+    try:
+        J = run.forward()
+        dg, dact = run.backward()
+    finally:
+        run.close()
+    return J, dg, dact
 
 
 def value_and_grad(M, cfg, Q0, V0, K, seed, band, **kw):
@@ -170,3 +239,35 @@ def _torch_active():
 def active_channel_velocity(theta, problem):
     """torch entry point: J = active_channel_velocity(theta, problem); J.backward()"""
     return _torch_active().apply(theta, problem)
+
+
+def _torch_active_reverse():
+    import torch
+
+    class ActiveChannelReverse(torch.autograd.Function):
+        """J(theta) with theta = (g_x, activity of every species) by the reverse
+        mode; problem(g, act) returns (M, cfg, Q0, V0, U0, K, seed, obs)"""
+
+        @staticmethod
+        def forward(ctx, theta, problem):
+            M, cfg, Q0, V0, U0, K, seed, obs = problem(float(theta[0]), float(theta[1]))
+            run = ReverseRun(M, cfg, Q0, V0, K, seed, obs, U0=U0)
+            ctx.run = run
+            ctx.nsp = len(cfg["species"])
+            ctx.like = theta
+            return theta.new_tensor(run.forward())
+
+        @staticmethod
+        def backward(ctx, gJ):
+            try:
+                dg, dact = ctx.run.backward()
+            finally:
+                ctx.run.close()
+            return gJ * ctx.like.new_tensor([dg[0], float(np.sum(dact[:ctx.nsp]))]), None
+
+    return ActiveChannelReverse
+
+
+def active_channel_reverse(theta, problem):
+    """torch entry point (reverse mode): J = active_channel_reverse(theta, problem); J.backward()"""
+    return _torch_active_reverse().apply(theta, problem)

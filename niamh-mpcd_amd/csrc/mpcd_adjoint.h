@@ -220,3 +220,305 @@ __global__ void k_adj_scatter(TapeDev T, const double *lam1_s, double *lam, int 
 }
 
 }  // namespace mpcd
+
+// ---------------------------------------------------------------------------
+// The reverse mode of every operator of the unfused step (mpcd_rev.h): the
+// step's kernels instantiated with Rv, one launch per kernel, each on the
+// state that kernel saw in the forward run.  The adjoint arrays A follow the
+// layout of the tangent's (TanDev); A.dth receives the parameters' adjoints.
+// ---------------------------------------------------------------------------
+namespace mpcd {
+
+using mpcd_ad::RevTape;
+static_assert(kBlock == mpcd_ad::kRevBlock, "reverse launches use the step's workgroup size");
+static_assert(kNTheta == MPCD_NTHETA, "parameter adjoints per replica");
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_rev_cellprops(Ctx C, State<Real> S, const BlkDev *cblk, int do_acc,
+                                                          TanDev A, RevTape tape) {
+  mpcd_ad::rev_begin(tape);
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  if (c < B.end) cellprops_cell<Real, Rv>(C, S, B.rep, c, do_acc, A);
+  mpcd_ad::rev_finish();
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_rev_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev A,
+                                                      RevTape tape) {
+  Dr.resolve();
+  mpcd_ad::rev_begin(tape);
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  if (c < B.end) ghost_cell<Real, Rv>(C, S, B.rep, c, Dr, A);
+  mpcd_ad::rev_finish();
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_rev_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev A,
+                                                   RevTape tape) {
+  Dr.resolve();
+  mpcd_ad::rev_begin(tape);
+  const BlkDev B = pblk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t < B.end) lc_particle<Real, Rv>(C, S, B.rep, t, Dr, A);
+  mpcd_ad::rev_finish();
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_rev_collide(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev A,
+                                                        RevTape tape) {
+  Dr.resolve();
+  mpcd_ad::rev_begin(tape);
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  if (c < B.end) (void)collide_cell<Real, Rv>(C, S, B.rep, c, Dr, A);
+  mpcd_ad::rev_finish();
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_rev_stream(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev A,
+                                                       const double *boxes, RevTape tape) {
+  Dr.resolve();
+  mpcd_ad::rev_begin(tape);
+  const BlkDev B = pblk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t < B.end) stream_particle<Real, Rv>(C, S, B.rep, t, Dr, nullptr, A, boxes);
+  mpcd_ad::rev_finish();
+}
+
+// the transposed velocity-gradient stencil (velGradD2Q9 / velGradD3Q15,
+// lc.c:2070-2162): E_c[i][j] = 3 sum_k w_k e_k[j] VCM_{c + e_k}[i] gives
+// lambda_VCM[c + e_k][i] += 3 w_k e_k[j] lambda_E_c[i][j]; lambda_E_c is
+// then spent
+__global__ void k_velgrad_bulk_adj(Ctx C, const BlkDev *cblk, double *lvcm, double *lE) {
+  const BlkDev B = cblk[blockIdx.x];
+  int c = B.beg + threadIdx.x;
+  if (c >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  int lc_ = c - R.cell_off;
+  int a = lc_ / (R.ny * R.nz), b = (lc_ / R.nz) % R.ny, cz = lc_ % R.nz;
+  double *E = &lE[(size_t)c * 9];
+  auto idx = [&](int aa, int bb, int cc) { return R.cell_off + (aa * R.ny + bb) * R.nz + cc; };
+  if (R.dim == 2) {
+    if (!(a >= 1 && a < R.L[0] && b >= 1 && b < R.L[1])) return;
+    const int uv[8][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}, {1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
+    const double w1 = 1. / 9., w5 = 1. / 36.;
+    for (int k = 0; k < 8; k++) {
+      const double w = 3. * ((k < 4) ? w1 : w5);
+      const int nb = idx(a + uv[k][0], b + uv[k][1], 0);
+      for (int i = 0; i < 2; i++) {
+        double s = 0.0;
+        for (int j = 0; j < 2; j++) s += uv[k][j] * E[i * 3 + j];
+        if (s != 0.0) atomicAdd(&lvcm[(size_t)nb * 3 + i], w * s);
+      }
+    }
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) E[i * 3 + j] = 0.0;
+  } else {
+    if (!(a >= 1 && a < R.L[0] && b >= 1 && b < R.L[1] && cz >= 1 && cz < R.L[2])) return;
+    const int uv[14][3] = {{1, 0, 0},  {0, 1, 0},  {-1, 0, 0}, {0, -1, 0},  {0, 0, 1},
+                           {0, 0, -1}, {1, 1, 1},  {-1, 1, 1}, {1, -1, 1},  {-1, -1, 1},
+                           {1, 1, -1}, {-1, 1, -1}, {1, -1, -1}, {-1, -1, -1}};
+    const double w1 = 1. / 9., w7 = 1. / 72.;
+    for (int k = 0; k < 14; k++) {
+      const double w = 3. * ((k < 6) ? w1 : w7);
+      const int nb = idx(a + uv[k][0], b + uv[k][1], cz + uv[k][2]);
+      for (int i = 0; i < 3; i++) {
+        double s = 0.0;
+        for (int j = 0; j < 3; j++) s += uv[k][j] * E[i * 3 + j];
+        if (s != 0.0) atomicAdd(&lvcm[(size_t)nb * 3 + i], w * s);
+      }
+    }
+    for (int k = 0; k < 9; k++) E[k] = 0.0;
+  }
+}
+
+// the edge copies of k_velgrad_edges in reverse order (z faces, y faces, x
+// faces): each copy dst = src adds lambda_dst to lambda_src and spends it
+__global__ void k_velgrad_edges_adj(Ctx C, double *lE) {
+  const RepDev &R = C.rep[blockIdx.x];
+  const int X = R.L[0], Y = R.L[1], dim = R.dim;
+  const int Z = (dim == 3) ? R.L[2] : 1;
+  auto idx = [&](int aa, int bb, int cc) { return R.cell_off + (aa * R.ny + bb) * R.nz + cc; };
+  const int zr = (dim == 3) ? Z : 1;
+  auto back = [&](int dst, int src, int ij) {
+    lE[(size_t)src * 9 + ij] += lE[(size_t)dst * 9 + ij];
+    lE[(size_t)dst * 9 + ij] = 0.0;
+  };
+  if (dim == 3) {
+    for (int k = threadIdx.x; k < X * Y; k += blockDim.x) {
+      int a = k / Y, b = k % Y;
+      for (int ij = 0; ij < 9; ij++) {
+        back(idx(a, b, Z), idx(a, b, Z - 1), ij);
+        back(idx(a, b, 0), idx(a, b, 1), ij);
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < X * zr; k += blockDim.x) {
+    int a = k / zr, c = k % zr;
+    for (int ij = 0; ij < 9; ij++) {
+      if ((ij / 3) >= dim || (ij % 3) >= dim) continue;
+      back(idx(a, Y, c), idx(a, Y - 1, c), ij);
+      back(idx(a, 0, c), idx(a, 1, c), ij);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < Y * zr; k += blockDim.x) {
+    int b = k / zr, c = k % zr;
+    for (int ij = 0; ij < 9; ij++) {
+      if ((ij / 3) >= dim || (ij % 3) >= dim) continue;
+      back(idx(X, b, c), idx(X - 1, b, c), ij);
+      back(idx(0, b, c), idx(1, b, c), ij);
+    }
+  }
+}
+
+// adjoints in particle order (replicas concatenated, 9 per particle: q, v,
+// u) <-> slot order of the state S (the A arrays)
+template <typename Real>
+__global__ void k_rev_to_slots(State<Real> S, const BlkDev *blk, const RepDev *reps, const double *lam, TanDev A) {
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const size_t i = (size_t)reps[B.rep].part_off + S.id[t];
+  for (int k = 0; k < 3; k++) {
+    A.q[k][t] = lam[9 * i + k];
+    A.v[k][t] = lam[9 * i + 3 + k];
+    A.u[k][t] = lam[9 * i + 6 + k];
+  }
+}
+
+// after the reverse of a step: slot t of the step's shifted bin came from
+// slot skey[t] of the state before it, re-placed if it had escaped (then
+// nothing before the bin reaches it)
+template <typename Real>
+__global__ void k_rev_from_slots(Ctx C, State<Real> Sold, const BlkDev *blk, TanDev A, double *lam) {
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  const int src = (int)(C.skey[t] & 0x7FFFFFFFull);
+  double q[3];
+  load_q(Sold, src, q);
+  shift_fwd(R, &C.shift[B.rep * 3], q);
+  const bool fresh = cell_of(R, q, 1) < 0;
+  const size_t i = (size_t)R.part_off + Sold.id[src];
+  for (int k = 0; k < 3; k++) {
+    lam[9 * i + k] = fresh ? 0.0 : A.q[k][t];
+    lam[9 * i + 3 + k] = fresh ? 0.0 : A.v[k][t];
+    lam[9 * i + 6 + k] = (fresh && R.lc != MPCD_ISOF) ? 0.0 : A.u[k][t];
+  }
+}
+
+// the state's slots back to particle order (no bin in between)
+template <typename Real>
+__global__ void k_rev_unslot(State<Real> S, const BlkDev *blk, const RepDev *reps, TanDev A, double *lam) {
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const size_t i = (size_t)reps[B.rep].part_off + S.id[t];
+  for (int k = 0; k < 3; k++) {
+    lam[9 * i + k] = A.q[k][t];
+    lam[9 * i + 3 + k] = A.v[k][t];
+    lam[9 * i + 6 + k] = A.u[k][t];
+  }
+}
+
+__global__ void k_add_f64(double *a, const double *b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] += b[i];
+}
+
+__global__ void k_fill_f64(double *a, double v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// observables of a state in its unshifted binning, per replica
+// band velocity: the mean of v[comp] over the particles with
+// lo <= (int)q[axis] < hi; sums[2 r] = count, sums[2 r + 1] = sum
+template <typename Real>
+__global__ void k_obs_band(Ctx C, State<Real> S, const BlkDev *blk, int comp, int axis, int lo, int hi, double *sums) {
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  double n = 0.0, s = 0.0;
+  if (t < B.end) {
+    const Real *qa = axis == 0 ? S.x : axis == 1 ? S.y : S.z;
+    const Real *va = comp == 0 ? S.vx : comp == 1 ? S.vy : S.vz;
+    const int y = (int)qa[t];
+    if (y >= lo && y < hi) {
+      n = 1.0;
+      s = (double)va[t];
+    }
+  }
+  __shared__ double sh[2][kBlock];
+  sh[0][threadIdx.x] = n;
+  sh[1][threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&sums[2 * B.rep], sh[0][0]);
+    atomicAdd(&sums[2 * B.rep + 1], sh[1][0]);
+  }
+}
+
+// its gradient: 1 / count at v[comp] of the particles in the band
+template <typename Real>
+__global__ void k_obs_band_adj(Ctx C, State<Real> S, const BlkDev *blk, int comp, int axis, int lo, int hi,
+                               const double *sums, double *lam) {
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t >= B.end) return;
+  const RepDev &R = C.rep[B.rep];
+  const size_t i = (size_t)R.part_off + S.id[t];
+  for (int k = 0; k < 9; k++) lam[9 * i + k] = 0.0;
+  const Real *qa = axis == 0 ? S.x : axis == 1 ? S.y : S.z;
+  const int y = (int)qa[t];
+  const double cnt = sums[2 * B.rep];
+  if (y >= lo && y < hi && cnt > 0.0) lam[9 * i + 3 + comp] = 1.0 / cnt;
+}
+
+// mean S over the cells holding more than one particle: sums[2 r] = count,
+// sums[2 r + 1] = sum of S; with w != null, w[c] = 1 / count for those cells
+__global__ void k_obs_meanS(Ctx C, const BlkDev *cblk, double *sums) {
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  double n = 0.0, s = 0.0;
+  if (c < B.end && C.cell.pop[c] > 1) {
+    n = 1.0;
+    s = C.cell.S[c];
+  }
+  __shared__ double sh[2][kBlock];
+  sh[0][threadIdx.x] = n;
+  sh[1][threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&sums[2 * B.rep], sh[0][0]);
+    atomicAdd(&sums[2 * B.rep + 1], sh[1][0]);
+  }
+}
+
+__global__ void k_obs_meanS_seed(Ctx C, const BlkDev *cblk, const double *sums, double *lS) {
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  if (c >= B.end) return;
+  const double cnt = sums[2 * B.rep];
+  lS[c] = (C.cell.pop[c] > 1 && cnt > 0.0) ? 1.0 / cnt : 0.0;
+}
+
+}  // namespace mpcd

@@ -110,6 +110,23 @@ struct mpcd_gpu_ctx {
     int cu_src;
   };
   std::vector<Checkpoint> ckpts;
+  // reverse mode of the whole step (mpcd_gpu_step_backward_state): the start
+  // state of every recorded step, the context's own state around a backward
+  // call, the adjoints in slot order (9 per slot, 25 per cell, the torque 3
+  // per slot), in particle order (9 per particle), of the parameters
+  // (kNTheta per replica), and the tape of one reverse launch
+  std::vector<Checkpoint> rev_steps;
+  Checkpoint rev_here;
+  int rev_len = 0, rev_cap = 0;
+  bool rev_replay = false, rev_bad = false, lin_ok = false;
+  double *ra_part = nullptr, *ra_cell = nullptr, *ra_T = nullptr, *ra_lam = nullptr, *ra_in = nullptr;
+  double *ra_th = nullptr, *ra_obs = nullptr;
+  int2 *rt_par = nullptr;
+  double2 *rt_dp = nullptr;
+  double *rt_adj = nullptr;
+  double **rt_optr = nullptr;
+  int *rt_ovar = nullptr, *rt_need = nullptr;
+  size_t rt_entries = 0, rt_outs = 0;  // capacities (entries x lanes, outputs x lanes)
 
   std::vector<void *> allocs;
   RepDev *d_rep = nullptr;
@@ -715,8 +732,12 @@ static void launch_shift(mpcd_gpu_ctx *c, uint32_t step, int injected) {
 
 // binning: mode 0 initial, 1 shifted (this step), 2 materialise the pending
 // unshifted bin.  gather = false only builds the sorted keys.
+static void rev_save_step(mpcd_gpu_ctx *c, int injected);
+
 template <typename Real>
 static void run_bin(mpcd_gpu_ctx *c, int mode, bool gather, int injected) {
+  // a recorded step starts with its shifted bin: keep its start state
+  if (mode == 1 && c->recording && !c->rev_replay) rev_save_step(c, injected);
   State<Real> *S = states<Real>(c);
   const int par = c->step & 1;
   Ctx C = view(c, par);
@@ -1985,16 +2006,10 @@ static void state_buffers(mpcd_gpu_ctx *c, std::vector<std::pair<void *, size_t>
   v.push_back({c->C.cell.S, nc * sizeof(double)}); v.push_back({c->C.cell.dir, 3 * nc * sizeof(double)});
 }
 
-extern "C" mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *c, int slot) {
-  if (!c || slot < 0) return MPCD_EINVAL;
-  mpcd_status st = ensure_init(c);
-  if (st) return st;
-  st = ensure_dense(c);
-  if (st) return st;
+// copies of the cell-sorted state (state_buffers) and the host step state
+static mpcd_status save_state(mpcd_gpu_ctx *c, mpcd_gpu_ctx::Checkpoint &k) {
   std::vector<std::pair<void *, size_t>> v;
   state_buffers(c, v);
-  if ((int)c->ckpts.size() <= slot) c->ckpts.resize(slot + 1);
-  mpcd_gpu_ctx::Checkpoint &k = c->ckpts[slot];
   if (k.bufs.empty()) {
     for (auto &b : v) {
       void *p = dalloc<uint8_t>(c, b.second);
@@ -2010,15 +2025,12 @@ extern "C" mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *c, int slot) {
   k.hist_ready = c->hist_ready;
   k.need_init = c->need_init;
   k.cu_src = c->cu_src;
-  return check_launch(c);
+  return MPCD_OK;
 }
 
-extern "C" mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *c, int slot) {
-  if (!c || slot < 0 || slot >= (int)c->ckpts.size() || c->ckpts[slot].bufs.empty())
-    return c ? fail(c, MPCD_EINVAL, "restore: no checkpoint in slot %d", slot) : MPCD_EINVAL;
+static mpcd_status load_state(mpcd_gpu_ctx *c, const mpcd_gpu_ctx::Checkpoint &k) {
   c->in_bucket = false;  // the restored state is cell-sorted
-  mpcd_gpu_ctx::Checkpoint &k = c->ckpts[slot];
-  c->cur = k.cur;  // the saved buffer set is the one that was current
+  c->cur = k.cur;        // the saved buffer set is the one that was current
   std::vector<std::pair<void *, size_t>> v;
   state_buffers(c, v);
   for (size_t i = 0; i < v.size(); i++)
@@ -2028,7 +2040,27 @@ extern "C" mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *c, int slot) {
   c->hist_ready = k.hist_ready;
   c->need_init = k.need_init;
   c->cu_src = k.cu_src;
-  mpcd_status st = zero_tangent(c);  // checkpoints hold no tangent: it restarts from zero
+  return MPCD_OK;
+}
+
+extern "C" mpcd_status mpcd_gpu_checkpoint(mpcd_gpu_ctx *c, int slot) {
+  if (!c || slot < 0) return MPCD_EINVAL;
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
+  if ((int)c->ckpts.size() <= slot) c->ckpts.resize(slot + 1);
+  st = save_state(c, c->ckpts[slot]);
+  if (st) return st;
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_restore(mpcd_gpu_ctx *c, int slot) {
+  if (!c || slot < 0 || slot >= (int)c->ckpts.size() || c->ckpts[slot].bufs.empty())
+    return c ? fail(c, MPCD_EINVAL, "restore: no checkpoint in slot %d", slot) : MPCD_EINVAL;
+  mpcd_status st = load_state(c, c->ckpts[slot]);
+  if (st) return st;
+  st = zero_tangent(c);  // checkpoints hold no tangent: it restarts from zero
   if (st) return st;
   return check_launch(c);
 }
@@ -2044,16 +2076,24 @@ extern "C" mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *c, int max_steps) {
     return MPCD_OK;
   }
   if (c->tan_on) return fail(c, MPCD_EINVAL, "the tangent and the adjoint tape cannot run together");
+  // the reverse mode of the whole step covers what the tangent covers; the
+  // velocity-only tape of mpcd_gpu_step_backward, SRD and MPC-AT between
+  // resting walls
+  c->lin_ok = true;
   for (const mpcd_gpu_params &p : c->params) {
-    if (p.lc != MPCD_ISOF || !(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT))
-      return fail(c, MPCD_EUNSUPPORTED, "the adjoint covers SRD and MPC-AT");
     if (p.tstech != MPCD_NOTHERM) return fail(c, MPCD_EUNSUPPORTED, "the adjoint needs tsTech = 0");
+    if (p.lc == MPCD_LCG) return fail(c, MPCD_EUNSUPPORTED, "the adjoint needs the local order parameter (lc 1)");
+    if (p.lc != MPCD_ISOF || !(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT))
+      c->lin_ok = false;
     for (int w = 0; w < p.nwall; w++)
       for (int j = 0; j < 3; j++)
-        if (p.wall[w].L[j] != 0.0 || p.wall[w].V[j] != 0.0)
-          return fail(c, MPCD_EUNSUPPORTED, "the adjoint needs resting walls");
+        if (p.wall[w].L[j] != 0.0 || p.wall[w].V[j] != 0.0) c->lin_ok = false;
   }
-  if (max_steps > c->tape_cap) {
+  if (c->precision != MPCD_FP64) return fail(c, MPCD_EUNSUPPORTED, "the adjoint needs the FP64 state");
+  c->rev_len = 0;
+  c->rev_cap = max_steps;
+  c->rev_bad = false;
+  if (c->lin_ok && max_steps > c->tape_cap) {
     const size_t n = c->ntot, nc = c->nctot, k = max_steps;
     c->tp_id = dalloc<int32_t>(c, k * n);
     c->tp_sp = dalloc<int32_t>(c, k * n);
@@ -2082,17 +2122,29 @@ extern "C" mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *c, int max_steps) {
   return MPCD_OK;
 }
 
+static mpcd_status rev_alloc(mpcd_gpu_ctx *c);
+
 extern "C" mpcd_status mpcd_gpu_adjoint_reset(mpcd_gpu_ctx *c) {
   if (!c) return MPCD_EINVAL;
-  if (!c->lam) return fail(c, MPCD_EINVAL, "no adjoint state (mpcd_gpu_record first)");
-  HIPCHK(c, hipMemsetAsync(c->lam, 0, sizeof(double) * 3 * c->ntot, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->dtheta, 0, sizeof(double) * 4 * c->nrep, c->stream));
+  if (!c->lam && c->rev_cap == 0) return fail(c, MPCD_EINVAL, "no adjoint state (mpcd_gpu_record first)");
+  if (c->lam) {
+    HIPCHK(c, hipMemsetAsync(c->lam, 0, sizeof(double) * 3 * c->ntot, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dtheta, 0, sizeof(double) * 4 * c->nrep, c->stream));
+  }
+  mpcd_status st = rev_alloc(c);
+  if (st) return st;
+  HIPCHK(c, hipMemsetAsync(c->ra_lam, 0, sizeof(double) * 9 * c->ntot, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->ra_th, 0, sizeof(double) * kNTheta * c->nrep, c->stream));
   return MPCD_OK;
 }
 
 extern "C" mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *c, const double *adj_in, double *adj_out,
                                               double *dJ_dtheta) {
   if (!c) return MPCD_EINVAL;
+  if (!c->lin_ok)
+    return fail(c, MPCD_EUNSUPPORTED,
+                "step_backward: the velocity-only adjoint covers SRD and MPC-AT between resting walls "
+                "(mpcd_gpu_step_backward_state covers every operator)");
   if (c->tape_len <= 0 || !c->lam) return fail(c, MPCD_EINVAL, "step_backward: the tape is empty");
   const TapeDev T = tape_at(c, --c->tape_len);
   const size_t n = c->ntot;
@@ -2104,6 +2156,320 @@ extern "C" mpcd_status mpcd_gpu_step_backward(mpcd_gpu_ctx *c, const double *adj
   if (adj_out) HIPCHK(c, hipMemcpyAsync(adj_out, c->lam, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   if (dJ_dtheta)
     HIPCHK(c, hipMemcpyAsync(dJ_dtheta, c->dtheta, sizeof(double) * 4 * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+// ---------------------------------------------------------------------------
+// reverse mode of the whole step: the step is replayed from its recorded
+// start state up to each kernel in turn, and that kernel's reverse launch
+// (mpcd_adjoint.h) turns the adjoints of its outputs into those of its inputs
+// ---------------------------------------------------------------------------
+enum { RS_PROPS = 0, RS_VELGRAD, RS_GHOST, RS_LC, RS_COLLIDE, RS_STREAM };
+
+static void rev_save_step(mpcd_gpu_ctx *c, int injected) {
+  if (c->rev_len >= c->rev_cap) return;
+  if (injected) c->rev_bad = true;  // the replay draws from the generator
+  if ((int)c->rev_steps.size() <= c->rev_len) c->rev_steps.resize(c->rev_len + 1);
+  if (save_state(c, c->rev_steps[c->rev_len]) != MPCD_OK) c->rev_bad = true;
+  c->rev_len++;
+}
+
+static mpcd_status rev_alloc(mpcd_gpu_ctx *c) {
+  if (c->ra_part) return MPCD_OK;
+  const size_t n = c->ntot, nc = c->nctot;
+  c->ra_part = dalloc<double>(c, 9 * n);
+  c->ra_cell = dalloc<double>(c, 25 * nc);
+  c->ra_T = dalloc<double>(c, 3 * n);
+  c->ra_lam = dalloc<double>(c, 9 * n);
+  c->ra_in = dalloc<double>(c, 9 * n);
+  c->ra_th = dalloc<double>(c, (size_t)kNTheta * c->nrep);
+  c->ra_obs = dalloc<double>(c, 2 * (size_t)c->nrep);
+  c->rt_need = dalloc<int>(c, 4);
+  if (!c->ra_part || !c->ra_cell || !c->ra_T || !c->ra_lam || !c->ra_in || !c->ra_th || !c->ra_obs || !c->rt_need) {
+    c->ra_part = nullptr;
+    return fail(c, MPCD_ENOMEM, "adjoint arrays");
+  }
+  HIPCHK(c, hipMemsetAsync(c->ra_lam, 0, sizeof(double) * 9 * n, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->ra_th, 0, sizeof(double) * kNTheta * c->nrep, c->stream));
+  return MPCD_OK;
+}
+
+// the kernels' view of the adjoint arrays (the tangent's layout)
+static TanDev rev_view(mpcd_gpu_ctx *c, bool with_theta) {
+  TanDev T{};
+  const size_t n = c->ntot, nc = c->nctot;
+  for (int k = 0; k < 3; k++) {
+    T.q[k] = c->ra_part + k * n;
+    T.v[k] = c->ra_part + (3 + k) * n;
+    T.u[k] = c->ra_part + (6 + k) * n;
+  }
+  T.vcm = c->ra_cell;
+  T.cm = c->ra_cell + 3 * nc;
+  T.I = c->ra_cell + 6 * nc;
+  T.S = c->ra_cell + 12 * nc;
+  T.dir = c->ra_cell + 13 * nc;
+  T.E = c->ra_cell + 16 * nc;
+  T.T = c->ra_T;
+  T.dth = with_theta ? c->ra_th : nullptr;
+  return T;
+}
+
+// one reverse launch over nblk workgroups: a counting launch sizes the tape,
+// then the recording launch sweeps it.  A reverse launch stores the forward
+// values as it goes, so prep() puts the kernel's inputs back in between.
+template <typename L, typename P>
+static mpcd_status rev_launch(mpcd_gpu_ctx *c, int nblk, L launch, P prep) {
+  if (nblk <= 0) return MPCD_OK;
+  RevTape t{};
+  t.stride = nblk * kBlock;
+  t.need = c->rt_need;
+  t.counting = 1;
+  k_fill<<<1, 4, 0, c->stream>>>(c->rt_need, 0, 4);
+  launch(t);
+  int need[4];
+  HIPCHK(c, hipMemcpyAsync(need, c->rt_need, sizeof need, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  mpcd_status st = check_launch(c);
+  if (st) return st;
+  const size_t ent = (size_t)std::max(need[0], 1), outs = (size_t)std::max(need[1], 1);
+  if (ent * t.stride > c->rt_entries) {
+    dfree(c, c->rt_par);
+    dfree(c, c->rt_dp);
+    dfree(c, c->rt_adj);
+    c->rt_entries = 0;
+    const size_t m = ent * t.stride;
+    c->rt_par = dalloc<int2>(c, m);
+    c->rt_dp = dalloc<double2>(c, m);
+    c->rt_adj = dalloc<double>(c, m);
+    if (!c->rt_par || !c->rt_dp || !c->rt_adj) return fail(c, MPCD_ENOMEM, "adjoint tape (%zu entries)", m);
+    c->rt_entries = m;
+  }
+  if (outs * t.stride > c->rt_outs) {
+    dfree(c, c->rt_optr);
+    dfree(c, c->rt_ovar);
+    c->rt_outs = 0;
+    const size_t m = outs * t.stride;
+    c->rt_optr = dalloc<double *>(c, m);
+    c->rt_ovar = dalloc<int>(c, m);
+    if (!c->rt_optr || !c->rt_ovar) return fail(c, MPCD_ENOMEM, "adjoint outputs (%zu)", m);
+    c->rt_outs = m;
+  }
+  prep();
+  t.counting = 0;
+  t.cap = (int)ent;
+  t.ocap = (int)outs;
+  t.par = c->rt_par;
+  t.dp = c->rt_dp;
+  t.adj = c->rt_adj;
+  t.optr = c->rt_optr;
+  t.ovar = c->rt_ovar;
+  launch(t);
+  HIPCHK(c, hipMemcpyAsync(need, c->rt_need, sizeof need, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  st = check_launch(c);
+  if (st) return st;
+  if (need[2]) return fail(c, MPCD_EDEVICE, "adjoint tape: a lane recorded more than its counting launch");
+  return MPCD_OK;
+}
+
+// step k of the tape replayed (FP64, generator draws) from its start state
+// up to, not including, the kernel of `stage`
+static void rev_replay(mpcd_gpu_ctx *c, int k, int stage) {
+  (void)load_state(c, c->rev_steps[k]);
+  run_bin<double>(c, 1, true, 0);
+  if (stage <= RS_VELGRAD) return;
+  run_props<double>(c, 1);
+  Ctx C = view(c, c->step & 1);
+  Draws D = make_draws(c, 0, c->step);
+  State<double> &S = c->sd[c->cur];
+  if (stage <= RS_GHOST) return;
+  if (c->any_phantom) k_ghost<double, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S, c->d_cblk, D, TanDev{});
+  if (stage <= RS_LC) return;
+  if (c->lc != MPCD_ISOF) k_lc<double, double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S, c->d_pblk, D, TanDev{});
+  if (stage <= RS_COLLIDE) return;
+  k_collide<double, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S, c->d_cblk, D, TanDev{});
+}
+
+static mpcd_status rev_stage(mpcd_gpu_ctx *c, int k, int stage) {
+  Ctx C = view(c, c->step & 1);
+  C.cdec = nullptr;
+  C.pdec = nullptr;
+  Draws D = make_draws(c, 0, c->step);
+  State<double> S = c->sd[c->cur];
+  const TanDev A = rev_view(c, true);
+  const int npb = (int)c->pblk.size(), ncb = (int)c->cblk.size();
+  auto prep = [&]() { rev_replay(c, k, stage); };
+  switch (stage) {
+    case RS_STREAM: {
+      const double *box = c->box2 + (c->step & 1) * 6 * c->nrep;
+      return rev_launch(c, npb, [&](const RevTape &t) {
+        k_rev_stream<double><<<npb, kBlock, 0, c->stream>>>(C, S, c->d_pblk, D, A, box, t);
+      }, prep);
+    }
+    case RS_COLLIDE:
+      return rev_launch(c, ncb, [&](const RevTape &t) {
+        k_rev_collide<double><<<ncb, kBlock, 0, c->stream>>>(C, S, c->d_cblk, D, A, t);
+      }, prep);
+    case RS_LC:
+      return rev_launch(c, npb, [&](const RevTape &t) {
+        k_rev_lc<double><<<npb, kBlock, 0, c->stream>>>(C, S, c->d_pblk, D, A, t);
+      }, prep);
+    case RS_GHOST:
+      return rev_launch(c, ncb, [&](const RevTape &t) {
+        k_rev_ghost<double><<<ncb, kBlock, 0, c->stream>>>(C, S, c->d_cblk, D, A, t);
+      }, prep);
+    case RS_VELGRAD:
+      k_velgrad_edges_adj<<<c->nrep, 256, 0, c->stream>>>(C, A.E);
+      k_velgrad_bulk_adj<<<ncb, kBlock, 0, c->stream>>>(C, c->d_cblk, A.vcm, A.E);
+      return check_launch(c);
+    default:
+      return rev_launch(c, ncb, [&](const RevTape &t) {
+        k_rev_cellprops<double><<<ncb, kBlock, 0, c->stream>>>(C, S, c->d_cblk, 1, A, t);
+      }, prep);
+  }
+}
+
+extern "C" mpcd_status mpcd_gpu_step_backward_state(mpcd_gpu_ctx *c, const double *adj_in, double *adj_out,
+                                                    double *dJ_dtheta) {
+  if (!c) return MPCD_EINVAL;
+  if (c->rev_len <= 0) return fail(c, MPCD_EINVAL, "step_backward_state: the tape is empty");
+  if (c->rev_bad)
+    return fail(c, MPCD_EUNSUPPORTED,
+                "step_backward_state: a recorded step used injected draws (the replay draws from the generator)");
+  mpcd_status st = rev_alloc(c);
+  if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
+  const size_t n = c->ntot, nc = c->nctot;
+  // the context's own state and event counts come back at the end
+  st = save_state(c, c->rev_here);
+  if (st) return st;
+  unsigned long long events[4];
+  HIPCHK(c, hipMemcpyAsync(events, c->C.events, sizeof events, hipMemcpyDeviceToHost, c->stream));
+  const bool rec = c->recording;
+  c->recording = false;
+  c->rev_replay = true;
+  const int k = --c->rev_len;
+  if (adj_in) {
+    HIPCHK(c, hipMemcpyAsync(c->ra_in, adj_in, sizeof(double) * 9 * n, hipMemcpyHostToDevice, c->stream));
+    k_add_f64<<<nblocks((int)(9 * n), kBlock), kBlock, 0, c->stream>>>(c->ra_lam, c->ra_in, 9 * n);
+  }
+  // the slot order of the step: its shifted bin
+  rev_replay(c, k, RS_PROPS);
+  {
+    const TanDev A = rev_view(c, true);
+    k_rev_to_slots<double><<<c->pblk.size(), kBlock, 0, c->stream>>>(c->sd[c->cur], c->d_pblk, c->d_rep, c->ra_lam,
+                                                                     A);
+    k_fill_f64<<<nblocks((int)(25 * nc), kBlock), kBlock, 0, c->stream>>>(c->ra_cell, 0.0, 25 * nc);
+    k_fill_f64<<<nblocks((int)(3 * n), kBlock), kBlock, 0, c->stream>>>(c->ra_T, 0.0, 3 * n);
+  }
+  for (int stage = RS_STREAM; stage >= RS_PROPS && !st; stage--) {
+    if ((stage == RS_LC || stage == RS_VELGRAD) && c->lc == MPCD_ISOF) continue;
+    if (stage == RS_GHOST && !c->any_phantom) continue;
+    if (stage != RS_VELGRAD) rev_replay(c, k, stage);
+    st = rev_stage(c, k, stage);
+  }
+  if (!st) {
+    // the cell-sorted state before the bin is the replay's other buffer
+    const TanDev A = rev_view(c, true);
+    k_rev_from_slots<double><<<c->pblk.size(), kBlock, 0, c->stream>>>(view(c, c->step & 1), c->sd[1 - c->cur],
+                                                                       c->d_pblk, A, c->ra_lam);
+  }
+  c->rev_replay = false;
+  c->recording = rec;
+  {
+    mpcd_status s2 = load_state(c, c->rev_here);
+    if (!st) st = s2;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->C.events, events, sizeof events, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (st) return st;
+  if (adj_out) HIPCHK(c, hipMemcpyAsync(adj_out, c->ra_lam, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->stream));
+  if (dJ_dtheta)
+    HIPCHK(c, hipMemcpyAsync(dJ_dtheta, c->ra_th, sizeof(double) * kNTheta * c->nrep, hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
+
+extern "C" mpcd_status mpcd_gpu_observable(mpcd_gpu_ctx *c, int kind, const double *arg, double *value,
+                                           double *grad) {
+  if (!c || !value) return c ? fail(c, MPCD_EINVAL, "observable: null argument") : MPCD_EINVAL;
+  if (kind != MPCD_OBS_BAND_VELOCITY && kind != MPCD_OBS_MEAN_S)
+    return fail(c, MPCD_EINVAL, "observable: unknown kind %d", kind);
+  if (kind == MPCD_OBS_MEAN_S && c->lc != MPCD_LCL)
+    return fail(c, MPCD_EINVAL, "observable: the mean order parameter needs the local nematic (lc 1)");
+  int comp = 0, axis = 1, lo = 0, hi = 0;
+  if (kind == MPCD_OBS_BAND_VELOCITY) {
+    if (!arg) return fail(c, MPCD_EINVAL, "observable: the band needs its arguments");
+    comp = (int)arg[0];
+    axis = (int)arg[1];
+    lo = (int)arg[2];
+    hi = (int)arg[3];
+    if (comp < 0 || comp > 2 || axis < 0 || axis > 2) return fail(c, MPCD_EINVAL, "observable: bad component or axis");
+  }
+  if (grad && c->precision != MPCD_FP64) return fail(c, MPCD_EUNSUPPORTED, "observable: the gradient needs FP64");
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
+  st = rev_alloc(c);
+  if (st) return st;
+  const bool fp64 = c->precision == MPCD_FP64;
+  if (c->pending) {
+    // the cells of the re-bin that ends the step (mpc.c:4936-4942)
+    if (fp64) {
+      run_bin<double>(c, 2, true, 0);
+      run_props<double>(c, 0);
+    } else {
+      run_bin<float>(c, 2, true, 0);
+      run_props<float>(c, 0);
+    }
+  }
+  Ctx C = view(c, c->step & 1);
+  const size_t n = c->ntot;
+  k_fill_f64<<<1, 64, 0, c->stream>>>(c->ra_obs, 0.0, 2 * (size_t)c->nrep);
+  if (kind == MPCD_OBS_BAND_VELOCITY) {
+    if (fp64)
+      k_obs_band<double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, c->sd[c->cur], c->d_pblk, comp, axis, lo, hi,
+                                                                   c->ra_obs);
+    else
+      k_obs_band<float><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, c->sf[c->cur], c->d_pblk, comp, axis, lo, hi,
+                                                                  c->ra_obs);
+  } else {
+    k_obs_meanS<<<c->cblk.size(), kBlock, 0, c->stream>>>(C, c->d_cblk, c->ra_obs);
+  }
+  std::vector<double> sums(2 * c->nrep);
+  HIPCHK(c, hipMemcpyAsync(sums.data(), c->ra_obs, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int r = 0; r < c->nrep; r++) value[r] = sums[2 * r] > 0.0 ? sums[2 * r + 1] / sums[2 * r] : 0.0;
+  if (!grad) return check_launch(c);
+  if (kind == MPCD_OBS_BAND_VELOCITY) {
+    k_obs_band_adj<double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, c->sd[c->cur], c->d_pblk, comp, axis, lo, hi,
+                                                                      c->ra_obs, c->ra_in);
+  } else {
+    // dS/d(orientations) of the cells' localPROP: its reverse launch with the
+    // adjoint 1 / count at every counted cell's S
+    const size_t nc = c->nctot;
+    k_fill_f64<<<nblocks((int)(25 * nc), kBlock), kBlock, 0, c->stream>>>(c->ra_cell, 0.0, 25 * nc);
+    k_fill_f64<<<nblocks((int)(9 * n), kBlock), kBlock, 0, c->stream>>>(c->ra_part, 0.0, 9 * n);
+    k_fill_f64<<<nblocks((int)(3 * n), kBlock), kBlock, 0, c->stream>>>(c->ra_T, 0.0, 3 * n);
+    const TanDev A = rev_view(c, false);
+    k_obs_meanS_seed<<<c->cblk.size(), kBlock, 0, c->stream>>>(C, c->d_cblk, c->ra_obs, A.S);
+    Ctx Cr = C;
+    Cr.cdec = nullptr;
+    State<double> S = c->sd[c->cur];
+    const int ncb = (int)c->cblk.size();
+    // without the body force the cell properties leave the particles as
+    // they are: the recording launch sees the counting launch's inputs
+    st = rev_launch(c, ncb, [&](const RevTape &t) {
+      k_rev_cellprops<double><<<ncb, kBlock, 0, c->stream>>>(Cr, S, c->d_cblk, 0, A, t);
+    }, []() {});
+    if (st) return st;
+    k_rev_unslot<double><<<c->pblk.size(), kBlock, 0, c->stream>>>(S, c->d_pblk, c->d_rep, A, c->ra_in);
+  }
+  HIPCHK(c, hipMemcpyAsync(grad, c->ra_in, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c);
 }

@@ -23,11 +23,13 @@
 
 #include "mpcd_math.h"
 #include "mpcd_dual.h"
+#include "mpcd_rev.h"
 #include "mpcd_params.h"
 
 namespace mpcd {
 
 using mpcd_ad::Dual;
+using mpcd_ad::Rv;
 
 constexpr int kBlock = 256;
 
@@ -109,14 +111,23 @@ MPCD_HD uint32_t cell_decision_word(int br, bool clamped) { return 32u | (uint32
 // mode, mpcd_gpu_tangent): per slot of the current buffer the derivatives of
 // position, velocity and orientation; per cell those of the cell properties;
 // and the direction itself (derivatives of the body force and of each
-// species' activity).
+// species' activity).  The reverse mode (TA<Rv>) uses the same arrays for the
+// adjoints of those values, and dth for the adjoints of the parameters.
 struct TanDev {
   double *q[3], *v[3], *u[3];
   double *vcm, *cm, *I, *S, *dir, *E;  // per cell: 3, 3, 6, 1, 3, 9
   double *T;                           // LC torque per slot [3 * ntot]
   double dgrav[3];
   double dact[MPCD_MAXSPEC];
+  double *dth;  // reverse mode: per replica kNTheta adjoints (MPCD_NTHETA layout), else null
 };
+
+// adjoints of the parameters per replica: GRAV[0..2], the rotation angle, ACT of each species
+constexpr int kNTheta = 4 + MPCD_MAXSPEC;
+constexpr int kThGrav = 0, kThAngle = 3, kThAct = 4;
+__device__ __forceinline__ double *th_slot(const TanDev &T, int rep, int k) {
+  return T.dth ? T.dth + (size_t)rep * kNTheta + k : nullptr;
+}
 
 // loads and stores of state values and their tangents: TA<double> touches
 // the values only, TA<Dual> the values and the tangent arrays
@@ -128,7 +139,7 @@ struct TA<double> {
   static MPCD_HD double ld(const R *a, const double *, size_t i) { return (double)a[i]; }
   template <typename R>
   static MPCD_HD void st(R *a, double *, size_t i, double x) { a[i] = (R)x; }
-  static MPCD_HD double par(double x, double) { return x; }
+  static MPCD_HD double par(double x, double, double *) { return x; }
   static MPCD_HD bool moves(double) { return false; }
 };
 // FP32 arithmetic on the FP32 state (the orientation step of k_lc): values only
@@ -138,7 +149,7 @@ struct TA<float> {
   static MPCD_HD float ld(const R *a, const double *, size_t i) { return (float)a[i]; }
   template <typename R>
   static MPCD_HD void st(R *a, double *, size_t i, float x) { a[i] = (R)x; }
-  static MPCD_HD float par(double x, double) { return (float)x; }
+  static MPCD_HD float par(double x, double, double *) { return (float)x; }
   static MPCD_HD bool moves(float) { return false; }
 };
 template <>
@@ -147,8 +158,28 @@ struct TA<Dual> {
   static MPCD_HD Dual ld(const R *a, const double *d, size_t i) { return Dual((double)a[i], d[i]); }
   template <typename R>
   static MPCD_HD void st(R *a, double *d, size_t i, const Dual &x) { a[i] = (R)x.v; d[i] = x.d; }
-  static MPCD_HD Dual par(double x, double dx) { return Dual(x, dx); }
+  static MPCD_HD Dual par(double x, double dx, double *) { return Dual(x, dx); }
   static MPCD_HD bool moves(const Dual &x) { return x.d != 0.0; }
+};
+// reverse mode: loads are leaves (or the entry last stored to the same
+// element), stores are outputs, parameters are leaves whose adjoints go to dth
+template <>
+struct TA<Rv> {
+  template <typename R>
+  static __device__ __forceinline__ Rv ld(const R *a, const double *d, size_t i) {
+    const double v = (double)a[i];
+    if (!d) return Rv(v);
+    int var;
+    if (mpcd_ad::rv_stored(d + i, var)) return Rv(v, var);
+    return mpcd_ad::rv_leaf(v, const_cast<double *>(d) + i);
+  }
+  template <typename R>
+  static __device__ __forceinline__ void st(R *a, double *d, size_t i, const Rv &x) {
+    a[i] = (R)x.v;
+    if (d) mpcd_ad::rv_store(d + i, x);
+  }
+  static __device__ __forceinline__ Rv par(double x, double, double *adj) { return mpcd_ad::rv_leaf(x, adj); }
+  static __device__ __forceinline__ bool moves(const Rv &x) { return x.i >= 0; }
 };
 
 // bits of Ctx::flags[0]
@@ -614,7 +645,7 @@ __device__ void cellprops_cell(const Ctx &C, const State<Real> &S, int rep, int 
   for (int d = 0; d < dim; d++) vcm[d] = 0.0;
   if (ccm) for (int d = 0; d < dim; d++) cm[d] = 0.0;
   F grav[3];
-  for (int d = 0; d < 3; d++) grav[d] = A::par(R.grav[d], T.dgrav[d]);
+  for (int d = 0; d < 3; d++) grav[d] = A::par(R.grav[d], T.dgrav[d], th_slot(T, rep, kThGrav + d));
   // a tangent along g applies acc_all even where the force itself is zero
   const bool acc = do_acc && (R.gravflag || A::moves(grav[0]) || A::moves(grav[1]) || A::moves(grav[2]));
   for (int t = b; t < e; t++) {
@@ -867,13 +898,10 @@ __device__ __forceinline__ void ori_bc(F *u, const mpcd_gpu_wall &W, const F *n,
 }
 
 template <typename Real, typename F>
-__global__ __launch_bounds__(kBlock) void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+__device__ __forceinline__ void ghost_cell(const Ctx &C, const State<Real> &S, int rep, int c, const Draws &Dr,
+                                           const TanDev &T) {
   using A = TA<F>;
-  Dr.resolve();
-  const BlkDev B = cblk[blockIdx.x];
-  int c = B.beg + threadIdx.x;
-  if (c >= B.end) return;
-  const RepDev &R = C.rep[B.rep];
+  const RepDev &R = C.rep[rep];
   if (!R.anyphantom) return;
   int pop = C.cell.pop[c];
   if (pop <= 0) return;
@@ -922,13 +950,21 @@ __global__ __launch_bounds__(kBlock) void k_ghost(Ctx C, State<Real> S, const Bl
     double invN = 1.0 / (R.gndnst - (double)pop);
     for (int k = 0; k < dim; k++) {
       double sigma = sqrt(R.kbt / invN);
-      double Rg = sigma * Dr.ghost_n(B.rep, c, k);
+      double Rg = sigma * Dr.ghost_n(rep, c, k);
       F v = A::ld(C.cell.vcm, T.vcm, c * 3 + k);
       v += Rg;
       v /= R.gndnst;
       A::st(C.cell.vcm, T.vcm, c * 3 + k, v);
     }
   }
+}
+
+template <typename Real, typename F>
+__global__ __launch_bounds__(kBlock) void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
+  Dr.resolve();
+  const BlkDev B = cblk[blockIdx.x];
+  const int c = B.beg + threadIdx.x;
+  if (c < B.end) ghost_cell<Real, F>(C, S, B.rep, c, Dr, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -1167,13 +1203,10 @@ __device__ void ms_sample(const RepDev &R, const Draws &Dr, uint32_t rep, uint32
 // in list order), and the Metropolis draws are keyed by slot, so the result
 // does not depend on the mapping
 template <typename Real, typename F>
-__global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev T) {
+__device__ __forceinline__ void lc_particle(const Ctx &C, const State<Real> &S, int rep, int t, const Draws &Dr,
+                                            const TanDev &T) {
   using A = TA<F>;
-  Dr.resolve();
-  const BlkDev B = pblk[blockIdx.x];
-  const int t = B.beg + threadIdx.x;
-  if (t >= B.end) return;
-  const RepDev &R = C.rep[B.rep];
+  const RepDev &R = C.rep[rep];
   const int c = S.cst[t];
   const int b = C.start[c], e = C.start[c + 1];
   if (e - b <= 1) {
@@ -1182,7 +1215,7 @@ __global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDe
   }
   const int dim = R.dim;
   F DIR[3] = {0., 0., 0.}, rotAx[3], xaxis[3] = {1.0, 0.0, 0.0};
-  F Sv = (R.lc == MPCD_LCL) ? A::ld(C.cell.S, T.S, c) : F(C.sg[B.rep]);
+  F Sv = (R.lc == MPCD_LCL) ? A::ld(C.cell.S, T.S, c) : F(C.sg[rep]);
   for (int i = 0; i < dim; i++) DIR[i] = A::ld(C.cell.dir, T.dir, c * 3 + i);
   cross3(DIR, xaxis, rotAx);
   F rotAngle = -1. * abs_angle(DIR, xaxis, dim);
@@ -1194,7 +1227,7 @@ __global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDe
     F U[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)}, Tq[3], dU[3] = {0., 0., 0.};
     for (int i = 0; i < dim; i++) dU[i] = U[i];
     uint32_t dec = 0;
-    ms_sample(R, Dr, B.rep, t, DIR, rotAx, rotAngle, U, F(R.kbt), Sv, F(effM), dec);
+    ms_sample(R, Dr, rep, t, DIR, rotAx, rotAngle, U, F(R.kbt), Sv, F(effM), dec);
     if (C.pdec) C.pdec[R.part_off + S.id[t]] = dec;
     for (int i = 0; i < dim; i++) dU[i] = (U[i] - dU[i]) / R.dt;
     cross3(U, dU, Tq);
@@ -1232,6 +1265,14 @@ __global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDe
   A::st(S.uz, T.u[2], t, u[2]);
 }
 
+template <typename Real, typename F>
+__global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDev *pblk, Draws Dr, TanDev T) {
+  Dr.resolve();
+  const BlkDev B = pblk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t < B.end) lc_particle<Real, F>(C, S, B.rep, t, Dr, T);
+}
+
 // ---------------------------------------------------------------------------
 // 10. collision operators (MPCcollision switch, mpc.c:2777-2812), then the
 //     kinetic-energy partial sums of TEMP (therm.c:209) and gridShift back
@@ -1242,8 +1283,8 @@ __global__ __launch_bounds__(kBlock) void k_lc(Ctx C, State<Real> S, const BlkDe
 // spof(t) is the species of slot t; the species populations of the cell are
 // counted only for a species with a MINACTRATIO
 template <typename F, typename SpOf>
-__device__ __forceinline__ F cell_activity(const mpcd_gpu_species *sps, SpOf spof, const RepDev &R, int b, int e,
-                                           const TanDev &T) {
+__device__ __forceinline__ F cell_activity(const mpcd_gpu_species *sps, SpOf spof, const RepDev &R, int rep, int b,
+                                           int e, const TanDev &T) {
   const int n = e - b;
   F ACT = 0.;
   double sigW = 0., sigP = 0.;
@@ -1257,7 +1298,7 @@ __device__ __forceinline__ F cell_activity(const mpcd_gpu_species *sps, SpOf spo
       pass = sp.minactratio * R.gndnst < (double)pop;
     }
     if (pass) {
-      ACT += TA<F>::par(sp.act, T.dact[k]) / sp.mass;
+      ACT += TA<F>::par(sp.act, T.dact[k], th_slot(T, rep, kThAct + k)) / sp.mass;
       sigW += sp.sigwidth;
       sigP += sp.sigpos;
     }
@@ -1380,7 +1421,7 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
         F cm[3] = {A::ld(C.cell.cm, T.cm, c * 3), A::ld(C.cell.cm, T.cm, c * 3 + 1), A::ld(C.cell.cm, T.cm, c * 3 + 2)};
         F ACT = 0., PA[3] = {0., 0., 0.};
         if (mode == 2) {
-          ACT = cell_activity<F>(C.sp, [&](int t) { return (int)S.sp[t]; }, R, b, e, T);
+          ACT = cell_activity<F>(C.sp, [&](int t) { return (int)S.sp[t]; }, R, rep, b, e, T);
           for (int j = 0; j < dim; j++) PA[j] = A::ld(C.cell.dir, T.dir, c * 3 + j);
         }
         // a tangent along the activity keeps the active terms where the
@@ -1563,7 +1604,7 @@ __global__ __launch_bounds__(kLcThreads) void k_collide_lc(Ctx C, State<Real> S,
     for (int k = tid; k < ncl; k += NT) {
       const int c = Tl.c0 + k, b = C.start[c], e = C.start[c + 1];
       double ACT = 0.;
-      if (mode == 2 && e - b > 1) ACT = cell_activity<double>(s_spc, [&](int t) { return s_sp[t - b0]; }, R, b, e, T0);
+      if (mode == 2 && e - b > 1) ACT = cell_activity<double>(s_spc, [&](int t) { return s_sp[t - b0]; }, R, Tl.rep, b, e, T0);
       s_act[k] = ACT;
       s_actf[k] = (mode == 2) && fneq(ACT, 0.0);
     }
@@ -1912,14 +1953,10 @@ __global__ __launch_bounds__(kBlock) void k_netmom_apply(Ctx C, State<Real> S, c
 //     bc.c:1363-1496), then the unshifted cell of every particle
 // ---------------------------------------------------------------------------
 template <typename Real, typename F>
-__global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul, TanDev T,
-                         const double *boxes) {
+__device__ __forceinline__ void stream_particle(const Ctx &C, const State<Real> &S, int rep, int t, const Draws &Dr,
+                                                float *wmul, const TanDev &T, const double *boxes) {
   using A = TA<F>;
-  Dr.resolve();
-  const BlkDev B = blk[blockIdx.x];
-  int t = B.beg + threadIdx.x;
-  if (t >= B.end) return;
-  const RepDev &R = C.rep[B.rep];
+  const RepDev &R = C.rep[rep];
   const int dim = R.dim;
   F Q[3], V[3] = {A::ld(S.vx, T.v[0], t), A::ld(S.vy, T.v[1], t), A::ld(S.vz, T.v[2], t)};
   F U[3] = {0., 0., 0.};
@@ -1931,7 +1968,7 @@ __global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const B
   for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
   // a particle strictly inside the step's wall-free box crosses no wall: the
   // search below would end at once
-  const double *box = &boxes[B.rep * 6];
+  const double *box = &boxes[rep * 6];
   bool inside = true;
   for (int i = 0; i < dim; i++) inside = inside && (Q[i] > box[2 * i]) && (Q[i] < box[2 * i + 1]);
   // MPC_BCcollision
@@ -2022,7 +2059,7 @@ __global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const B
         }
         const uint32_t pid = (uint32_t)S.id[t];
         double Vn[3] = {0.0, 0.0, 0.0};
-        push_velocity(sp, R.kbt, dim, Dr, B.rep, (size_t)R.part_off + pid, pid, rk, Vn);
+        push_velocity(sp, R.kbt, dim, Dr, rep, (size_t)R.part_off + pid, pid, rk, Vn);
         for (int j = 0; j < dim; j++) V[j] = Vn[j];
         rk += 3;
         t_left = t_step;
@@ -2045,7 +2082,7 @@ __global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const B
     mul[0] = mul[1] = mul[2] = __builtin_nanf("");
     const uint32_t pid = (uint32_t)S.id[t];
     double q[3] = {val(Q[0]), val(Q[1]), val(Q[2])}, v[3], u[3] = {val(U[0]), val(U[1]), val(U[2])};
-    if (!escape_repair(R, C.wh, 1, Dr, B.rep, pid, sp.mass, lc, q, v, u)) atomicOr(&C.flags[0], kFlagEscape);
+    if (!escape_repair(R, C.wh, 1, Dr, rep, pid, sp.mass, lc, q, v, u)) atomicOr(&C.flags[0], kFlagEscape);
     for (int j = 0; j < 3; j++) {
       Q[j] = q[j];
       if (j < dim) V[j] = v[j];
@@ -2067,6 +2104,15 @@ __global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const B
   C.cu[t] = cu;
   if (wmul)
     for (int i = 0; i < 3; i++) wmul[3 * t + i] = mul[i];
+}
+
+template <typename Real, typename F>
+__global__ __launch_bounds__(kBlock) void k_stream(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr, float *wmul, TanDev T,
+                         const double *boxes) {
+  Dr.resolve();
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t < B.end) stream_particle<Real, F>(C, S, B.rep, t, Dr, wmul, T, boxes);
 }
 
 // ---------------------------------------------------------------------------
