@@ -465,7 +465,9 @@ static double dr_uniform(orc_sys *s, orc_draws *d, int purpose, long key, int k)
         return b[key * stride + k];
     }
     if (rare_keyed(d, purpose, key, k, &x)) return x;
-    return orc_philox_uniform(d->seed, d->replica, (uint32_t)s->step, purpose, (uint32_t)key, (uint32_t)k);
+    x = orc_philox_uniform(d->seed, d->replica, (uint32_t)s->step, purpose, (uint32_t)key, (uint32_t)k);
+    record(d, purpose, key, k, x);
+    return x;
 }
 
 /* standard normal -- the reference's genrand_gauss() (rand.c:473), a float */
@@ -483,7 +485,9 @@ static double dr_normal(orc_sys *s, orc_draws *d, int purpose, long key, int k)
         return b[key * stride + k];
     }
     if (rare_keyed(d, purpose, key, k, &x)) return x;
-    return orc_philox_normal(d->seed, d->replica, (uint32_t)s->step, purpose, (uint32_t)key, (uint32_t)k);
+    x = orc_philox_normal(d->seed, d->replica, (uint32_t)s->step, purpose, (uint32_t)key, (uint32_t)k);
+    record(d, purpose, key, k, x);
+    return x;
 }
 
 /* unit vector on the sphere -- the reference's ranvec3D() (rand.c:708) */
@@ -505,6 +509,9 @@ static void dr_unitvec(orc_sys *s, orc_draws *d, int purpose, long key, int k, d
         return;
     }
     philox_unitvec(d->seed, d->replica, (uint32_t)s->step, purpose, (uint32_t)key, (uint32_t)k, v);
+    record(d, purpose, key, k, v[0]);
+    record(d, purpose, key, k + 1, v[1]);
+    record(d, purpose, key, k + 2, v[2]);
 }
 
 static double dr_gaussMB(orc_sys *s, orc_draws *d, double kbt, double m, int purpose, long key, int k)
