@@ -245,20 +245,26 @@ def cpu_baseline(L=(64, 64, 64), ppc=10, steps=4, dim=3, rtech=0, rotang=ROTANG_
                        f"one core")
 
 
-def tangent_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, act=0.03, hg=(1e-11,), ha=(1e-13,)):
-    """gradient of the time-averaged centreline velocity of a 2D active nematic
+def nematic_gradients(M, Lx=128, Ly=64, ppc=20, K=200, ckpt=20, g=5e-4, act=0.03, hg=(1e-11,), ha=(1e-12,)):
+    """gradients of two time-averaged observables of a 2D active nematic
     channel (lc 1, dipole activity DIPOLE_DIR_SUM, U = 10, tumbling 0.5,
-    rfc 0.01; bounce-back phantom walls in y with homeotropic strong anchoring;
-    body force g along x) with respect to g and the activity, by the
-    forward-mode tangent (one run per direction) against central finite
-    differences with common random numbers at the steps hg, ha.
+    rfc 0.01; bounce-back phantom walls in y with homeotropic strong
+    anchoring; body force g along x) with respect to g and the activity:
+    the x-velocity of the centreline band and the mean scalar order
+    parameter of the cells.  The reverse mode of the whole step
+    (mpcd_gpu_step_backward_state, checkpoint every `ckpt` steps) gives both
+    derivatives of one observable in one backward pass; the forward-mode
+    tangent (one run per direction) gives the same pathwise derivative
+    independently; central finite differences with common random numbers
+    at the steps hg, ha are reported beside them.
     The pathwise derivative holds every discrete decision fixed.  A difference
     step perturbs the orientations by h times their tangents, which grow to
     O(100) over 200 steps, and Metropolis acceptances (about 2e8 comparisons at
     128 x 64 x 20) flip once h exceeds about 1e-12; below about 1e-13 the
-    FP64 round-off of the band average dominates.  At that size finite
-    differences therefore resolve the derivative to about 1 %; a smaller box
-    has a clean window and checks it to 1e-3."""
+    FP64 round-off of the observables dominates.  At that size finite
+    differences therefore resolve the derivative to about 1 %, and the exact
+    check is reverse against tangent; a smaller box has a clean window and
+    checks it to 1e-3."""
     import importlib
     H = importlib.import_module("niamh_mpcd_amd.adjoint")
     rng = np.random.default_rng(4)
@@ -284,40 +290,87 @@ def tangent_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, act=0.03, hg=(1e-11,)
                     walls=ws)
 
     band = (Ly // 2 - 1, Ly // 2 + 1)
+    OBS = {"centreline_velocity": (M.OBS_BAND_VELOCITY, (0, 1, band[0], band[1])), "mean_S": (M.OBS_MEAN_S, None)}
+    seed = 13
+    out = {}
+    for name, obs in OBS.items():
+        t0 = time.perf_counter()
+        run = H.ReverseRun(M, cfg(g, act), Q, V, K, seed, obs, U0=U, ckpt=ckpt)
+        try:
+            J = run.forward()
+            t1 = time.perf_counter()
+            dg, dact = run.backward()
+            t2 = time.perf_counter()
+        finally:
+            run.close()
+        out[name] = dict(J=J, dJ_dg=float(dg[0]), dJ_dact=float(dact[0]), forward_s=t1 - t0, backward_s=t2 - t1)
+
+    def tangent(direction):
+        """both observables and their tangents along one direction, with the
+        reverse runs' call sequence (an observable after every step)"""
+        ctx = M.Context(cfg(g, act), N, seed=seed, precision=M.FP64)
+        try:
+            ctx.upload(0, Q, V, U=U)
+            ctx.tangent(True, **direction)
+            d = {k: 0.0 for k in OBS}
+            for _ in range(K):
+                ctx.step()
+                ctx.observable(*OBS["centreline_velocity"])
+                Qs, Vs, Us = ctx.download(0)
+                _, dV, dU = ctx.tangent_download(0)
+                y = np.trunc(Qs[:, 1])
+                m = (y >= band[0]) & (y < band[1])
+                d["centreline_velocity"] += dV[m, 0].mean() / K
+                # dS = v0^T dQ v0 of each counted cell's 2x2 order tensor
+                cell = np.trunc(Qs[:, 0]).astype(np.int64) * (Ly + 1) + y.astype(np.int64)
+                nc = (Lx + 1) * (Ly + 1)
+                pop = np.bincount(cell, minlength=nc)
+                sxx = np.bincount(cell, Us[:, 0] ** 2, nc) * 2 / np.maximum(pop, 1) - 1
+                sxy = np.bincount(cell, Us[:, 0] * Us[:, 1], nc) * 2 / np.maximum(pop, 1)
+                tr2 = np.sqrt(sxx ** 2 + sxy ** 2)  # the eigenvalue of the traceless tensor
+                v0 = np.stack([sxx + tr2, sxy], 1)
+                v0 /= np.maximum(np.linalg.norm(v0, axis=1), 1e-300)[:, None]
+                w = v0[cell]
+                term = (dU[:, 0] * w[:, 0] + dU[:, 1] * w[:, 1]) * (Us[:, 0] * w[:, 0] + Us[:, 1] * w[:, 1])
+                dS = 4.0 * np.bincount(cell, term, nc) / np.maximum(pop, 1)
+                d["mean_S"] += dS[pop > 1].mean() / K
+            return d
+        finally:
+            ctx.close()
+
     t0 = time.perf_counter()
-    J, (dg, da) = H.tangent_value_and_grad(M, cfg(g, act), Q, V, K, 13, band,
-                                           [dict(dgrav=[1.0, 0.0, 0.0]), dict(dact=[1.0])], U0=U)
-    t1 = time.perf_counter()
+    tg = tangent(dict(dgrav=[1.0, 0.0, 0.0]))
+    ta = tangent(dict(dact=[1.0]))
+    t_tan = (time.perf_counter() - t0) / 2
 
     def J_of(gg, aa):
-        ctx = M.Context(cfg(gg, aa), N, seed=13, precision=M.FP64)
+        ctx = M.Context(cfg(gg, aa), N, seed=seed, precision=M.FP64)
         ctx.upload(0, Q, V, U=U)
-        j = 0.0
+        j = {k: 0.0 for k in OBS}
         for _ in range(K):
             ctx.step()
-            Qs, Vs, _ = ctx.download(0)
-            y = np.trunc(Qs[:, 1])
-            m = (y >= band[0]) & (y < band[1])
-            j += Vs[m, 0].mean() / K
+            for k, o in OBS.items():
+                j[k] += ctx.observable(*o)[0][0] / K
         ctx.close()
         return j
 
-    t2 = time.perf_counter()
-    j0 = J_of(g, act)
-    t3 = time.perf_counter()
-    fd_g = {h: (J_of(g + h, act) - J_of(g - h, act)) / (2 * h) for h in hg}
-    fd_a = {h: (J_of(g, act + h) - J_of(g, act - h)) / (2 * h) for h in ha}
-    rel = lambda a, b: abs(a - b) / max(abs(b), 1e-300)
+    rel = lambda a, b: abs(a - b) / max(abs(b), 1e-300)  # noqa: E731
+    fd_g = {h: (J_of(g + h, act), J_of(g - h, act)) for h in hg}
+    fd_a = {h: (J_of(g, act + h), J_of(g, act - h)) for h in ha}
+    for name in OBS:
+        o = out[name]
+        o["tangent_dJ_dg"], o["tangent_dJ_dact"] = tg[name], ta[name]
+        o["rel_reverse_vs_tangent"] = max(rel(o["dJ_dg"], tg[name]), rel(o["dJ_dact"], ta[name]))
+        o["fd_dJ_dg"] = {str(h): (p[name] - m[name]) / (2 * h) for h, (p, m) in fd_g.items()}
+        o["fd_dJ_dact"] = {str(h): (p[name] - m[name]) / (2 * h) for h, (p, m) in fd_a.items()}
+        o["rel_err_fd_g"] = {k: rel(o["dJ_dg"], v) for k, v in o["fd_dJ_dg"].items()}
+        o["rel_err_fd_act"] = {k: rel(o["dJ_dact"], v) for k, v in o["fd_dJ_dact"].items()}
     return dict(
         workload=f"2D active nematic channel {Lx}x{Ly} cells x {ppc} ppc ({N} particles), lc 1, DIPOLE_DIR_SUM, "
                  f"U 10, tumbling 0.5, rfc 0.01, activity {act}, g {g}, homeotropic strong anchoring at bounce-back "
-                 f"phantom walls, K {K} steps, J = time-averaged x-velocity of the centreline band y in [{band[0]}, "
-                 f"{band[1]})",
-        J=J, J_plain=j0, dJ_dg=dg, fd_dg={str(h): v for h, v in fd_g.items()},
-        rel_err_g={str(h): rel(dg, v) for h, v in fd_g.items()},
-        dJ_dact=da, fd_dact={str(h): v for h, v in fd_a.items()},
-        rel_err_act={str(h): rel(da, v) for h, v in fd_a.items()},
-        tangent_s_per_direction=(t1 - t0) / 2, plain_s=t3 - t2)
+                 f"phantom walls, K {K} steps, checkpoint every {ckpt}; J = time average of the x-velocity of the "
+                 f"centreline band y in [{band[0]}, {band[1]}) or of the mean S of the cells",
+        tangent_s_per_direction=t_tan, **out)
 
 
 def adjoint_check(M, Lx=128, Ly=64, ppc=20, K=200, g=5e-4, angle=2.0, h=1e-11):
@@ -445,9 +498,10 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-replica MPC-AT+a channel ensemble (throughput + Poiseuille viscosities)")
     ap.add_argument("--no-lc", action="store_true", help="skip the 2D active-nematic throughput line")
-    ap.add_argument("--only", choices=["fp32", "fp64", "both", "lc", "ensemble"], default="both",
+    ap.add_argument("--only", choices=["fp32", "fp64", "both", "lc", "ensemble", "gradients"], default="both",
                     help="lc / ensemble: only the 2D active nematic or the 64-replica MPC-AT+a ensemble throughput "
-                         "(--steps timed steps; its JSON instead of the bench line)")
+                         "(--steps timed steps; its JSON instead of the bench line); gradients: only the reverse "
+                         "mode, tangent and finite differences of the active nematic channel")
     ap.add_argument("--fill", type=float, default=0.0,
                     help="fix the bucket fill of the run layout instead of autotuning it (reproducible profiles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -463,6 +517,10 @@ def main():
     M = package()
     if args.only == "lc":
         print(json.dumps(lc_throughput(M, steps=args.steps, warmup=args.warmup)))
+        return
+    if args.only == "gradients":
+        print(json.dumps(dict(channel_128x64=nematic_gradients(M, hg=(1e-11, 1e-12), ha=(1e-12, 1e-13)),
+                              box_32x16=nematic_gradients(M, Lx=32, Ly=16, K=100, hg=(1e-10,), ha=(1e-10,)))))
         return
     if args.only == "ensemble":
         import importlib
@@ -568,10 +626,10 @@ def main():
         adjoint = adjoint_check(M)
     tangent = None
     if not args.no_adjoint and args.only == "both":
-        print("[bench] forward-mode tangent of the active nematic channel vs finite differences", file=sys.stderr,
-              flush=True)
-        tangent = dict(channel_128x64=tangent_check(M, hg=(1e-11, 1e-12), ha=(1e-12, 3e-13, 1e-13)),
-                       box_32x16=tangent_check(M, Lx=32, Ly=16, K=100, hg=(1e-10,), ha=(1e-10,)))
+        print("[bench] reverse mode and tangent of the active nematic channel vs finite differences",
+              file=sys.stderr, flush=True)
+        tangent = dict(channel_128x64=nematic_gradients(M, hg=(1e-11, 1e-12), ha=(1e-12, 1e-13)),
+                       box_32x16=nematic_gradients(M, Lx=32, Ly=16, K=100, hg=(1e-10,), ha=(1e-10,)))
     cpu = None if args.no_cpu else cpu_baseline()
     small = None
     if args.only == "both":
@@ -610,7 +668,7 @@ def main():
         "fp32_validation": validation,
         "ensemble_mpcat_a": ensemble,
         "adjoint": adjoint,
-        "tangent_active_nematic": tangent,
+        "gradients_active_nematic": tangent,
         "lc_active_nematic": lc,
         "flow_field": r.get("flow_field"),
         "small_box_2d": small,
