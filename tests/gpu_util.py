@@ -73,14 +73,17 @@ def hip_runtime():
     import ctypes as C
     M = package()
     M.lib()
-    path = None
+    paths = []
     with open("/proc/self/maps") as f:
         for line in f:
-            if "libamdhip64" in line and "/torch/" not in line:
-                path = line.split()[-1]
-                break
+            if "libamdhip64" in line and line.split()[-1] not in paths:
+                paths.append(line.split()[-1])
+    # with torch loaded first, the library binds to torch's copy (same soname)
+    # and that is the only runtime mapped; otherwise the library's is the other
+    own = [p for p in paths if "/torch/" not in p]
+    path = own[0] if own else (paths[0] if len(paths) == 1 else None)
     if path is None:
-        raise RuntimeError("the library's HIP runtime is not mapped")
+        raise RuntimeError(f"cannot tell the library's HIP runtime among {paths}")
     hip = C.CDLL(path)
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     hip.hipFree.argtypes = [C.c_void_p]
