@@ -55,6 +55,7 @@ struct orc_sys {
     int *pop, *spcnt;
     double *mass, *vcm, *cm, *I, *E, *S, *dir;
     double wq[ORC_MAXWALL][3];
+    int planar[ORC_MAXWALL];  /* read.c:1423-1434 */
     double shift[3];
     double kbtnow, avs, avdir[3];
     long step;
@@ -72,6 +73,9 @@ struct orc_sys {
     uint32_t *pdec, *cdec;
     int rec_cdec;
 };
+
+static double wall_W(const orc_sys *s, int w, const double *q);
+static int wall_planar(const orc_wall *W);
 
 /* ------------------------------------------------------------------------ */
 /* small vector helpers, restating mtools.c                                  */
@@ -558,6 +562,13 @@ orc_sys *orc_create(const orc_params *p, int n)
     if (!s) return NULL;
     s->p = *p;
     s->N = n;
+    for (w = 0; w < p->nwall; w++) {
+        orc_wall *W = &s->p.wall[w];
+        /* P all zero: the plane (1 1 1 1) */
+        if (W->P[0] == 0.0 && W->P[1] == 0.0 && W->P[2] == 0.0 && W->P[3] == 0.0)
+            W->P[0] = W->P[1] = W->P[2] = W->P[3] = 1.0;
+        s->planar[w] = wall_planar(W);
+    }
     s->nx = p->L[0] + 1;
     s->ny = p->L[1] + 1;
     s->nz = (p->dim == 3) ? p->L[2] + 1 : 1;
@@ -667,8 +678,8 @@ static void push_velocity(orc_sys *s, orc_draws *d, int i, int purpose, int kbas
 /* bin()'s escape repair, mpc.c:970-982, of particle i in binning b (0: the
  * shifted bin, 1: the unshifted bin after streaming): replacePos_WithCheck
  * (init.c:1462-1483) redraws a uniform position in the box (replace,
- * init.c:1240-1243) while any wall has calcW_PLANE <= TOL at the walls'
- * current positions; then DIM Maxwell velocities (genrand_gaussMB,
+ * init.c:1240-1243) while any wall has calcW <= TOL at the walls' current
+ * positions; then DIM Maxwell velocities (genrand_gaussMB,
  * rand.c:510, for every VDIST) and, for nematics, genrand_coneNP(U, pi, DIM)
  * (rand.c:397-425).  Draws are consumed in that order. */
 static void escape_repair(orc_sys *s, orc_draws *d, int i, int b)
@@ -681,14 +692,8 @@ static void escape_repair(orc_sys *s, orc_draws *d, int i, int b)
             s->Q[i * 3 + j] = (double)s->p.L[j] * dr_uniform(s, d, purpose, i, ORC_ESC_POS0 + dim * t + j);
         t++;
         flag = 0;
-        for (w = 0; w < s->p.nwall; w++) {
-            const orc_wall *W = &s->p.wall[w];
-            double Wv = 0.0;
-            for (j = 0; j < dim; j++) Wv += W->A[j] * (s->Q[i * 3 + j] - s->wq[w][j]);
-            Wv -= W->R;
-            if (W->INV) Wv *= -1.0;
-            if (Wv <= ORC_TOL) flag = 1;
-        }
+        for (w = 0; w < s->p.nwall; w++)
+            if (wall_W(s, w, &s->Q[i * 3]) <= ORC_TOL) flag = 1;
         if (d->rare_overflow) break; /* a keyed entry ran out of tries */
     }
     for (j = 0; j < dim; j++) s->V[i * 3 + j] = dr_gaussMB(s, d, s->p.kbt, m, purpose, i, j);
@@ -930,15 +935,305 @@ static void local_prop(orc_sys *s)
 /* walls                                                                     */
 /* ------------------------------------------------------------------------ */
 
-static double wall_W(const orc_sys *s, int w, const double *q)   /* calcW_PLANE bc.c:306, calcW bc.c:55 */
+/* smrtPow, mtools.c:38-50: integer powers up to 10 by repeated products */
+static double smrt_pow(double x, double y)
 {
-    const orc_wall *W = &s->p.wall[w];
+    if ((y - (int)y == 0) && (y <= 10)) {
+        int i;
+        double r = 1;
+        for (i = 0; i < y; i++) r *= x;
+        return r;
+    }
+    return pow(x, y);
+}
+
+/* calcWavyW, bc.c:94-150, for the wall at wq */
+static double calc_wavy_w(const orc_wall *W, const double *wq, const double *pos, double Wv, int dim)
+{
+    double W1 = 0.0, W2 = 0.0;
+    int i, flag = 0;
+    for (i = 0; i < dim; i++) if (!feq(W->P[i], 1.0)) flag += 1;
+    if (!flag) {
+        if (!feq(W->B[1], 0.0))
+            W1 = (W->A[1] * pos[0] - W->A[0] * pos[1]) / sqrt(W->A[0] * W->A[0] + W->A[1] * W->A[1]);
+        if (dim > 2 && !feq(W->B[2], 0.0)) {
+            W2 = W->A[0] * W->A[2] * pos[0] + W->A[1] * W->A[2] * pos[1] -
+                 (W->A[0] * W->A[0] + W->A[1] * W->A[1]) * pos[2];
+            W2 /= sqrt(pow(W->A[0], 4) + pow(W->A[1], 4) + pow(W->A[0], 2) * pow(W->A[2], 2) +
+                       2.0 * pow(W->A[0], 2) * pow(W->A[1], 2) + pow(W->A[1], 2) * pow(W->A[2], 2));
+        }
+    } else {
+        flag = 0;
+        for (i = 0; i < dim; i++) if (!feq(W->P[i], 2.0) || feq(W->A[i], 0.0)) flag += 1;
+        if (!flag) {
+            if (!feq(W->B[1], 0.0)) W1 = ref_atan2(W->A[1] * (pos[1] - wq[1]), W->A[0] * (pos[0] - wq[0]));
+            if (dim > 2 && !feq(W->B[2], 0.0))
+                W2 = ref_atan2(sqrt(pow(W->A[0] * (pos[0] - wq[0]), 2) + pow(W->A[1] * (pos[1] - wq[1]), 2)),
+                               W->A[2] * (pos[2] - wq[2]));
+        } else if (dim > 2 && flag == 1) {
+            int f = -1;
+            for (i = 0; i < dim; i++) if (feq(W->A[i], 0.0)) f = i;
+            if (f != -1) {
+                int j, k, l;
+                if (f == 0) { j = 0; k = 1; l = 2; }
+                else if (f == 1) { j = 1; k = 2; l = 0; }
+                else { j = 2; k = 0; l = 1; }
+                if (!feq(W->B[1], 0.0)) W1 = ref_atan2(W->A[l] * (pos[l] - wq[l]), W->A[k] * (pos[k] - wq[k]));
+                if (!feq(W->B[2], 0.0)) W2 = ref_atan2(sqrt(Wv + pow(W->R, W->P[3])), sqrt(pos[j]));
+            }
+        }
+    }
+    return W->B[0] * cos(W->B[1] * W1) * cos(W->B[2] * W2);
+}
+
+/* calcW, bc.c:55-80 (= surf_func, mtools.c:1196-1220), four-fold symmetric
+ * surfaces, for the wall at wq */
+static double calc_w(const orc_wall *W, const double *wq, const double *pos, int dim)
+{
+    double terms, Wv = 0.0;
+    int i;
+    for (i = 0; i < dim; i++) {
+        terms = W->A[i] * (pos[i] - wq[i]);
+        if (W->ABS) terms = fabs(terms);
+        terms = smrt_pow(terms, W->P[i]);
+        Wv += terms;
+    }
+    terms = W->R;
+    if (W->ABS) terms = fabs(terms);
+    terms = smrt_pow(terms, W->P[3]);
+    Wv -= terms;
+    if (!feq(W->B[0], 0.0)) Wv += calc_wavy_w(W, wq, pos, Wv, dim);
+    if (W->INV) Wv *= -1.0;
+    return Wv;
+}
+
+/* calcW_PLANE, bc.c:306-317 */
+static double calc_w_plane(const orc_wall *W, const double *wq, const double *pos, int dim)
+{
     double Wv = 0.;
     int i;
-    for (i = 0; i < s->p.dim; i++) Wv += W->A[i] * (q[i] - s->wq[w][i]);
+    for (i = 0; i < dim; i++) Wv += W->A[i] * (pos[i] - wq[i]);
     Wv -= W->R;
     if (W->INV) Wv *= -1.;
     return Wv;
+}
+
+/* the wall's W at its current position: calcW (bc.c:55) */
+static double wall_W(const orc_sys *s, int w, const double *q)
+{
+    return calc_w(&s->p.wall[w], s->wq[w], q, s->p.dim);
+}
+
+/* read.c:1423-1434: an axis-aligned plane that is not wavy */
+static int wall_planar(const orc_wall *W)
+{
+    if (!feq(W->B[0], 0.0)) return 0;
+    if (!(feq(W->P[0], 1.0) && feq(W->P[1], 1.0) && feq(W->P[2], 1.0))) return 0;
+    if (fneq(W->A[0], 0.0) && feq(W->A[1], 0.0) && feq(W->A[2], 0.0)) return 1;
+    if (feq(W->A[0], 0.0) && fneq(W->A[1], 0.0) && feq(W->A[2], 0.0)) return 1;
+    if (feq(W->A[0], 0.0) && feq(W->A[1], 0.0) && fneq(W->A[2], 0.0)) return 1;
+    return 0;
+}
+
+/* secant_time, bc.c:329-373: the time back along -V at which the surface
+ * (wall at wq) was crossed */
+static double secant_time(const orc_wall *W, const double *wq, const double *Q, const double *V, int dim,
+                          double t_step)
+{
+    double Qi[3] = {0, 0, 0}, QiM1[3] = {0, 0, 0}, ti, tiM1, root, fi, fiM1;
+    int i, iter = 0;
+    for (i = 0; i < dim; i++) {
+        QiM1[i] = Q[i];
+        Qi[i] = Q[i] + t_step * (-V[i]);
+    }
+    tiM1 = 0.0;
+    ti = t_step;
+    root = ti;
+    do {
+        iter++;
+        fi = calc_w(W, wq, Qi, dim);
+        fiM1 = calc_w(W, wq, QiM1, dim);
+        if (!feq(fi, fiM1)) root = ti - fi * (ti - tiM1) / (fi - fiM1);
+        if (root < ORC_TOL / 100.) {
+            ti = tiM1;
+            tiM1 = root;
+        } else {
+            tiM1 = ti;
+            ti = root;
+        }
+        for (i = 0; i < dim; i++) {
+            QiM1[i] = Q[i] + tiM1 * (-V[i]);
+            Qi[i] = Q[i] + ti * (-V[i]);
+        }
+    } while (iter < 10 && fabs(ti - tiM1) > ORC_TOL / 100. && fabs(fi - fiM1) > ORC_TOL / 100.);
+    return root;
+}
+
+/* crosstimeReverse, bc.c:251-298 */
+static void crosstime_reverse(const orc_wall *W, int planar, const double *wq, const double *Q, const double *V,
+                              int dim, double t_step, double *tc_pos, double *tc_neg)
+{
+    double a = 0.0, b = 0.0, c = 0.0;
+    int i;
+    if (planar || (feq(W->P[0], 1.0) && feq(W->P[1], 1.0) && feq(W->P[2], 1.0) && feq(W->P[3], 1.0) &&
+                   feq(W->B[0], 0.0))) {
+        *tc_pos = W->R;
+        for (i = 0; i < dim; i++) *tc_pos += W->A[i] * (wq[i] - Q[i]);
+        *tc_pos /= (W->A[0] * (-V[0]) + W->A[1] * (-V[1]) + W->A[2] * (-V[2]));
+        *tc_neg = *tc_pos;
+    } else if ((dim == 2 && feq(W->P[0], 2.0) && feq(W->P[1], 2.0) && feq(W->B[0], 0.0)) ||
+               (dim > 2 && feq(W->P[0], 2.0) && feq(W->P[1], 2.0) && feq(W->P[2], 2.0) && feq(W->B[0], 0.0))) {
+        for (i = 0; i < dim; i++) {
+            a += W->A[i] * W->A[i] * (-V[i]) * (-V[i]);
+            b += W->A[i] * W->A[i] * (-V[i]) * (Q[i] - wq[i]);
+            c += W->A[i] * W->A[i] * (Q[i] * Q[i] - 2. * Q[i] * wq[i] + wq[i] * wq[i]);
+        }
+        b *= 2.0;
+        c -= smrt_pow(W->R, W->P[3]);
+        *tc_neg = -b - sqrt(b * b - 4. * a * c);
+        *tc_neg /= (2. * a);
+        *tc_pos = -b + sqrt(b * b - 4. * a * c);
+        *tc_pos /= (2. * a);
+    } else {
+        *tc_pos = secant_time(W, wq, Q, V, dim, t_step);
+        if (*tc_pos < 0 || *tc_pos > t_step) *tc_pos = t_step;
+        *tc_neg = *tc_pos;
+    }
+}
+
+/* chooseT, bc.c:849-880 */
+static double choose_t(double tstep, double tp, double tn)
+{
+    double tc = tstep, zero = -ORC_TOL, step = tstep + ORC_TOL;
+    if (tp > step && tn > step) tc = 0.;
+    else if (tp < zero && tn < zero) tc = 0.;
+    else if (tp < zero) tc = tn;
+    else if (tn < zero) tc = tp;
+    else if (tn < tp) tc = tn;
+    else tc = tp;
+    if (tc < zero) tc = 0.;
+    if (tc > step) tc = 0.;
+    return tc;
+}
+
+/* shiftBC, bc.c:389-405: the periodic image of a curved wall nearest the
+ * particle; the wall position is moved in place (and moved back by
+ * shift_back_bc), as the reference does */
+static void shift_bc(orc_sys *s, int w, const double *q, double *shift)
+{
+    const orc_wall *W = &s->p.wall[w];
+    int k, dim = s->p.dim;
+    for (k = 0; k < dim; k++) shift[k] = 0.0;
+    if (fneq(W->P[0], 1.0) && fneq(W->P[1], 1.0)) {
+        for (k = 0; k < dim; k++) {
+            if (q[k] - s->wq[w][k] >= 0.5 * (double)s->p.L[k]) shift[k] = (double)s->p.L[k];
+            else if (q[k] - s->wq[w][k] <= -0.5 * (double)s->p.L[k]) shift[k] = -1.0 * (double)s->p.L[k];
+            s->wq[w][k] += shift[k];
+        }
+    }
+}
+
+static void shift_back_bc(orc_sys *s, int w, const double *shift)   /* shiftbackBC, bc.c:416-422 */
+{
+    int i;
+    for (i = 0; i < s->p.dim; i++) s->wq[w][i] -= shift[i];
+}
+
+/* normalWavy, bc.c:1568-1640 */
+static void wall_normal_wavy(const orc_wall *W, const double *wq, const double *point, int dimension, double *n)
+{
+    double W1 = 0.0, W2 = 0.0, div, dw1[3] = {0, 0, 0}, dw2[3] = {0, 0, 0};
+    int i;
+    if (feq(W->P[0], 1.0) && feq(W->P[1], 1.0) && feq(W->P[2], 1.0)) {
+        if (!feq(W->B[1], 0.0)) {
+            W1 = (W->A[1] * point[0] - W->A[0] * point[1]) / sqrt(W->A[0] * W->A[0] + W->A[1] * W->A[1]);
+            dw1[0] = W->A[1] / sqrt(W->A[0] * W->A[0] + W->A[1] * W->A[1]);
+            dw1[1] = -W->A[0] / sqrt(W->A[0] * W->A[0] + W->A[1] * W->A[1]);
+            dw1[2] = 0;
+        }
+        if (dimension > 2 && !feq(W->B[2], 0.0)) {
+            div = sqrt(pow(W->A[0], 4) + pow(W->A[1], 4) + pow(W->A[0], 2) * pow(W->A[2], 2) +
+                       2.0 * pow(W->A[0], 2) * pow(W->A[1], 2) + pow(W->A[1], 2) * pow(W->A[2], 2));
+            W2 = (W->A[0] * W->A[2] * point[0] + W->A[1] * W->A[2] * point[1] -
+                  (W->A[0] * W->A[0] + W->A[1] * W->A[1]) * point[2]) / div;
+            dw2[0] = (W->A[0] * W->A[2]) / div;
+            dw2[1] = (W->A[1] * W->A[2]) / div;
+            dw2[2] = -(W->A[0] * W->A[0] + W->A[1] * W->A[1]) / div;
+        }
+    } else {
+        int flag = 0;
+        for (i = 0; i < dimension; i++) if (!feq(W->P[i], 2.0) || feq(W->A[i], 0.0)) flag += 1;
+        if (!flag) {
+            div = pow(W->A[0] * (point[0] - wq[0]), 2) + pow(W->A[1] * (point[1] - wq[1]), 2);
+            W1 = ref_atan2(W->A[1] * (-wq[1] + point[1]), W->A[0] * (-wq[0] + point[0]));
+            if (!feq(W->B[1], 0.0)) {
+                dw1[0] = W->A[0] * W->A[1] * (wq[1] - point[1]) / div;
+                dw1[1] = W->A[0] * W->A[1] * (point[0] - wq[0]) / div;
+                dw1[2] = 0;
+            }
+            if (dimension > 2 && !feq(W->B[2], 0.0)) {
+                dw2[0] = W->A[2] * W->A[0] * (point[0] - wq[0]) * (point[2] - wq[2]) /
+                         (sqrt(div) * (div + pow(W->A[2] * (point[2] - wq[2]), 2)));
+                dw2[1] = W->A[2] * W->A[1] * (point[1] - wq[1]) * (point[2] - wq[2]) /
+                         (sqrt(div) * (div + pow(W->A[2] * (point[2] - wq[2]), 2)));
+                dw2[2] = -W->A[2] * sqrt(div) / (div + pow(W->A[2] * (point[2] - wq[2]), 2));
+            }
+        } else if (dimension > 2 && flag == 1) {
+            int f = -1;
+            for (i = 0; i < dimension; i++) if (feq(W->A[i], 0.0)) f = i;
+            if (f != -1) {
+                int j, k, l;
+                if (f == 0) { j = 0; k = 1; l = 2; }
+                else if (f == 1) { j = 1; k = 2; l = 0; }
+                else { j = 2; k = 0; l = 1; }
+                div = pow(W->A[k] * (point[k] - wq[k]), 2) + pow(W->A[l] * (point[l] - wq[l]), 2);
+                if (!feq(W->B[1], 0.0)) {
+                    dw1[k] = W->A[k] * W->A[l] * (wq[l] - point[l]) / div;
+                    dw1[l] = W->A[k] * W->A[l] * (point[k] - wq[k]) / div;
+                    dw1[j] = 0;
+                }
+                if (!feq(W->B[2], 0.0)) {
+                    dw2[k] = W->A[k] * W->A[k] * (point[k] - wq[k]) * point[j] / (sqrt(div) * (div + point[k]));
+                    dw2[l] = W->A[l] * W->A[l] * (point[l] - wq[l]) * point[j] / (sqrt(div) * (div + point[l]));
+                    dw2[j] = -sqrt(div) / (div + point[j]);
+                }
+            }
+        }
+    }
+    for (i = 0; i < 3; i++)
+        n[i] -= W->B[0] * (W->B[1] * dw1[i] * sin(W->B[1] * W1) * cos(W->B[2] * W2) +
+                           W->B[2] * dw2[i] * cos(W->B[1] * W1) * sin(W->B[2] * W2));
+}
+
+/* normal, bc.c:1516-1556 (four-fold symmetric surfaces), at point, wall at wq */
+static void wall_normal(const orc_wall *W, const double *wq, const double *point, int dimension, double *n)
+{
+    int i;
+    if (feq(W->P[0], 1.0) && feq(W->P[1], 1.0) && feq(W->P[2], 1.0))
+        for (i = 0; i < dimension; i++) n[i] = W->A[i];
+    else if ((dimension == 2 && feq(W->P[0], 2.0) && feq(W->P[1], 2.0)) ||
+             (dimension > 2 && feq(W->P[0], 2.0) && feq(W->P[1], 2.0) && feq(W->P[2], 2.0)))
+        for (i = 0; i < dimension; i++) n[i] = 2.0 * W->A[i] * W->A[i] * (point[i] - wq[i]);
+    else if ((dimension == 2 && feq(W->P[0], 4.0) && feq(W->P[1], 4.0)) ||
+             (dimension > 2 && feq(W->P[0], 4.0) && feq(W->P[1], 4.0) && feq(W->P[2], 4.0)))
+        for (i = 0; i < dimension; i++)
+            n[i] = 4.0 * W->A[i] * W->A[i] * W->A[i] * W->A[i] * (point[i] - wq[i]) * (point[i] - wq[i]) *
+                   (point[i] - wq[i]);
+    else if ((dimension == 2 && feq(W->P[0], 6.0) && feq(W->P[1], 6.0)) ||
+             (dimension > 2 && feq(W->P[0], 6.0) && feq(W->P[1], 6.0) && feq(W->P[2], 6.0)))
+        for (i = 0; i < dimension; i++) {
+            n[i] = 4.0 * W->A[i] * W->A[i] * W->A[i] * W->A[i] * W->A[i] * W->A[i];
+            n[i] *= (point[i] - wq[i]) * (point[i] - wq[i]) * (point[i] - wq[i]) * (point[i] - wq[i]) *
+                    (point[i] - wq[i]);
+        }
+    else if (W->ABS)
+        for (i = 0; i < dimension; i++)
+            n[i] = W->A[i] * W->A[i] * W->P[i] * (point[i] - wq[i]) *
+                   smrt_pow(fabs(W->A[i] * (point[i] - wq[i])), W->P[i] - 2.0);
+    else
+        for (i = 0; i < dimension; i++)
+            n[i] = W->P[i] * smrt_pow(W->A[i], W->P[i]) * smrt_pow(point[i] - wq[i], W->P[i] - 1.0);
+    if (!feq(W->B[0], 0.0)) wall_normal_wavy(W, wq, point, dimension, n);
 }
 
 static int wall_anchoring(const orc_wall *W) { return feq(W->MUN, 0.0) || feq(W->MUT, 0.0); }
@@ -1003,15 +1298,26 @@ static void ghost_part(orc_sys *s, orc_draws *d)
                                 int pi = s->order[t];
                                 if (W->inter[s->spid[pi]] == 1) {
                                     double n[3] = {0, 0, 0};
-                                    for (k = 0; k < dim; k++) n[k] = W->A[k];
+                                    wall_normal(W, s->wq[w], &s->Q[pi * 3], dim, n);   /* normal, bc.c:1516 */
                                     vnorm(n, dim);
                                     ori_bc(s, pi, W, n);
                                 }
                             }
-                            /* planar wall: S = 1, director = first particle's orientation */
-                            s->S[ce] = 1.0;
-                            for (k = 0; k < dim; k++) s->dir[ce * 3 + k] = s->U[s->order[s->cstart[ce]] * 3 + k];
-                            vnorm(&s->dir[ce * 3], dim);
+                            if ((feq(W->P[0], 1.0) && feq(W->P[1], 1.0) && feq(W->P[2], 1.0)) || s->pop[ce] == 1) {
+                                /* a plane, or one particle: S = 1, director = first particle's orientation */
+                                s->S[ce] = 1.0;
+                                for (k = 0; k < dim; k++)
+                                    s->dir[ce * 3 + k] = s->U[s->order[s->cstart[ce]] * 3 + k];
+                                vnorm(&s->dir[ce * 3], dim);
+                            } else if (s->pop[ce] > 1) {
+                                /* a curved wall: the order tensor of the anchored orientations */
+                                double Sm[3][3];
+                                const int rec = s->rec_cdec;
+                                s->rec_cdec = 0;
+                                tens_order(s, ce, Sm);
+                                order_from_tensor(s, ce, Sm);
+                                s->rec_cdec = rec;
+                            }
                         }
                     }
                 if (flagW && s->pop[ce] < p->gndnst && s->pop[ce] > 0) {
@@ -1535,9 +1841,11 @@ static void push_velocity(orc_sys *s, orc_draws *d, int i, int purpose, int kbas
     }
 }
 
-/* MPC_BCcollision, bc.c:1363-1496, with chooseBC's planar branch (1263-1278),
- * crosstimeReverse (257-263), velBC's rule branch (656-672), posBC (760-774)
- * and oriBC for LC; immobile planar walls only */
+/* MPC_BCcollision, bc.c:1363-1496, with chooseBC (1252-1328: planes, wavy
+ * planes, and curved surfaces at their periodic image nearest the particle),
+ * crosstimeReverse (251-298), chooseT (849-880), normal (1516-1556),
+ * velBC's rule branch (656-672), posBC (760-774) and oriBC for LC;
+ * immobile walls with four-fold symmetric surfaces */
 static void bc_collision(orc_sys *s, orc_draws *d, int pi)
 {
     const orc_params *p = &s->p;
@@ -1551,15 +1859,31 @@ static void bc_collision(orc_sys *s, orc_draws *d, int pi)
         double tmin = t_left, chW = 1.0;
         for (w = 0; w < p->nwall; w++) {
             const orc_wall *Wl = &p->wall[w];
-            double tw;
+            double tw, t1, t2, tc, shift[3];
             if (Wl->inter[s->spid[pi]] != 1) continue;
-            tw = wall_W(s, w, Q);
-            if (tw < 0.0) {
-                double tc = Wl->R, t1;
-                for (i = 0; i < dim; i++) tc += Wl->A[i] * (s->wq[w][i] - Q[i]);
-                tc /= (Wl->A[0] * (-V[0]) + Wl->A[1] * (-V[1]) + Wl->A[2] * (-V[2]));
-                t1 = t_left - tc;
-                if (t1 < tmin) { tmin = t1; chosen = w; chW = tw; }
+            if (s->planar[w]) {
+                tw = calc_w_plane(Wl, s->wq[w], Q, dim);
+                if (tw < 0.0) {
+                    crosstime_reverse(Wl, 1, s->wq[w], Q, V, dim, t_left, &t1, &t2);
+                    tc = t_left - t1;
+                    if (tc < tmin) { tmin = tc; chosen = w; chW = tw; }
+                }
+            } else if (!feq(Wl->B[0], 0.0) && feq(Wl->P[0], 1.0) && feq(Wl->P[1], 1.0) && feq(Wl->P[2], 1.0)) {
+                tw = calc_w(Wl, s->wq[w], Q, dim);
+                if (tw < 0.0) {
+                    crosstime_reverse(Wl, 0, s->wq[w], Q, V, dim, t_left, &t1, &t2);
+                    tc = choose_t(t_left, t_left - t1, t_left - t2);
+                    if (tc < tmin) { tmin = tc; chosen = w; chW = tw; }
+                }
+            } else {
+                shift_bc(s, w, Q, shift);
+                tw = calc_w(Wl, s->wq[w], Q, dim);
+                if (tw < 0.0) {
+                    crosstime_reverse(Wl, 0, s->wq[w], Q, V, dim, t_left, &t1, &t2);
+                    tc = choose_t(t_left, t_left - t1, t_left - t2);
+                    if (tc < tmin) { tmin = tc; chosen = w; chW = tw; }
+                }
+                shift_back_bc(s, w, shift);
             }
         }
         t_coll = tmin;
@@ -1568,11 +1892,13 @@ static void bc_collision(orc_sys *s, orc_draws *d, int pi)
         cnt++;
         {
             const orc_wall *Wl = &p->wall[chosen];
+            double shift[3];
+            shift_bc(s, chosen, Q, shift);
             if (t_coll > -ORC_TOL) {
                 double R[3] = {0, 0, 0}, VR[3], Vr[3] = {0, 0, 0}, VN[3] = {0, 0, 0}, VT[3] = {0, 0, 0};
                 double IMpart, PN[3] = {0, 0, 0}, PT[3] = {0, 0, 0};
                 for (i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];      /* rewind_P */
-                for (i = 0; i < dim; i++) n[i] = Wl->A[i];                             /* normal, bc.c:1522 */
+                wall_normal(Wl, s->wq[chosen], Q, dim, n);                              /* normal, bc.c:1516 */
                 vnorm(n, dim);
                 /* velBC, rule-based collisions (bc.c:598-672) */
                 for (i = 0; i < dim; i++) R[i] = Q[i] - s->wq[chosen][i];
@@ -1611,6 +1937,7 @@ static void bc_collision(orc_sys *s, orc_draws *d, int pi)
             } else {
                 for (i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
             }
+            shift_back_bc(s, chosen, shift);
         }
         if (cnt > ORC_NBOUNCE) {
             flagCNT = 1;

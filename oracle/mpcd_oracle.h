@@ -61,6 +61,10 @@ typedef struct {
     double A[3], Q[3], R, DN, DT, DVN, DVT, DVxyz[3], MVN, MVT, MUN, MUT, MUxyz[3], DUxyz[3], V[3], L[3];
     int INV, PHANTOM, COLL_TYPE;
     int inter[ORC_MAXSPEC];
+    /* surface shape (read.c:1264-1300, 1394-1436): exponents P[0..3] (all 0:
+     * the plane, 1 1 1 1), wavy amplitudes/wavenumbers B, |.| of the terms */
+    double P[4], B[3];
+    int ABS;
 } orc_wall;
 
 typedef struct {

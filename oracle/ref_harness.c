@@ -67,6 +67,8 @@ typedef struct {
     unsigned long long seed;
     double spv[MAXSPECI][12];
     double wv[MAXW][32];
+    double shape[MAXW][8];  /* P[4] B[3] ABS of a curved or wavy wall ("shape" after its "wall" line) */
+    int has_shape[MAXW];
 } hcase;
 
 static void die(const char *m)
@@ -112,6 +114,12 @@ static void read_case(const char *path, hcase *c)
             if (nw >= MAXW) die("too many walls");
             for (i = 0; i < 30; i++) if (fscanf(f, "%lf", &c->wv[nw][i]) != 1) die("wall");
             nw++;
+        }
+        else if (!strcmp(key, "shape")) {
+            /* P[4] B[3] ABS of the wall on the line before */
+            if (nw < 1) die("shape before wall");
+            for (i = 0; i < 8; i++) if (fscanf(f, "%lf", &c->shape[nw - 1][i]) != 1) die("shape");
+            c->has_shape[nw - 1] = 1;
         }
         else die("unknown key");
     }
@@ -237,11 +245,18 @@ int main(int argc, char **argv)
         W->E = -1.0;
         W->KBT = 1.0;
         W->MASS = 1.0;
-        for (j = 0; j < 4; j++) W->P[j] = 1.0;
+        for (j = 0; j < 4; j++) W->P[j] = hc.has_shape[i] ? hc.shape[i][j] : 1.0;
+        for (j = 0; j < 3; j++) W->B[j] = hc.has_shape[i] ? hc.shape[i][4 + j] : 0.0;
+        W->ABS = hc.has_shape[i] ? (int)hc.shape[i][7] : 0;
         W->ROTSYMM[0] = W->ROTSYMM[1] = 4.0;
         for (j = 0; j < MAXSPECI + 2; j++) W->INTER[j] = BCON;
-        /* planar flag, read.c:1424-1433 (walls here are axis-aligned planes) */
-        W->PLANAR = 1;
+        /* planar flag as read.c:1423-1434 sets it; immobile, unrotated walls (init.c:2063-2065) */
+        W->PLANAR = 0;
+        if (feq(W->B[0], 0.0) && feq(W->P[0], 1.0) && feq(W->P[1], 1.0) && feq(W->P[2], 1.0)) {
+            if (fneq(W->A[0], 0.0) && feq(W->A[1], 0.0) && feq(W->A[2], 0.0)) W->PLANAR = 1;
+            else if (feq(W->A[0], 0.0) && fneq(W->A[1], 0.0) && feq(W->A[2], 0.0)) W->PLANAR = 1;
+            else if (feq(W->A[0], 0.0) && feq(W->A[1], 0.0) && fneq(W->A[2], 0.0)) W->PLANAR = 1;
+        }
         W->REORIENT = 0;
     }
     for (i = 0; i < 3; i++) XYZPBC[i] = 0;

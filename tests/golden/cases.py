@@ -53,6 +53,10 @@ def periodic(dim, L):
     return ws
 
 
+def CS_pbc_x(L):
+    return pbc_walls(0, L)
+
+
 def case(name, dim, L, ppc, rtech, steps, seed, lc=0, tstech=0, rotang=math.pi / 2, dt=0.1, kbt=1.0,
          grav=(0, 0, 0), walls=None, sp=None, galinv=1, tau=0.5, qbox=None, kick=None, zeronetmom=0, avv=None):
     c = dict(name=name, dim=dim, L=list(L) + [1] * (3 - len(L)), ppc=ppc, rtech=rtech, steps=steps,
@@ -149,11 +153,76 @@ CASES += [
          walls=pbc_walls(0, 5) + pbc_walls(1, 4) + solid_walls(2, 5)),
 ]
 
+# the general wall engine (bc.c:55-150, 251-405, 1252-1556): curved surfaces
+# (ellipses and circles with the quadratic crossing time, superellipses and
+# pipes with the secant method), their periodic images (shiftBC), inverted
+# surfaces, wavy planes, and strong anchoring at a curved wall
+def _disc(Q, R, P=(2, 2, 2, 2), A=(1.0, 1.0, 0.0), **kw):
+    return dict(A=list(A), Q=list(Q), R=R, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1, P=list(P), **kw)
+
+
+CASES += [
+    # 2D MPC-AT, a disc obstacle in a periodic box
+    case("circle2d_at", 2, (12, 10), 10, rtech=2, steps=4, seed=31, dt=0.5,
+         walls=periodic(2, (12, 10)) + [_disc((6.0, 5.0, 0.0), 3.0)]),
+    # 2D SRD, an elliptic obstacle across the periodic x boundary (its image is the wall
+    # shiftBC moves for the particles on the far side)
+    case("ellipse2d_pbc", 2, (12, 10), 10, rtech=0, steps=4, seed=32, dt=0.5,
+         walls=periodic(2, (12, 10)) + [_disc((0.7, 5.0, 0.0), 1.0, A=(1 / 3.0, 1 / 2.5, 0.0))]),
+    # 2D SRD, a square-ish superellipse obstacle (P = 8: the secant crossing time)
+    case("square2d_srd", 2, (12, 10), 10, rtech=0, steps=4, seed=33, dt=0.5,
+         walls=periodic(2, (12, 10)) + [_disc((6.0, 5.0, 0.0), 1.0, P=(8, 8, 1, 8), A=(1 / 3.0, 1 / 3.0, 0.0))]),
+    # 3D MPC-AT+a in a pipe along x (inverted cylinder, P = 1 2 2 2: the secant crossing time)
+    case("pipe3d_rat", 3, (8, 8, 8), 8, rtech=3, steps=3, seed=34, dt=0.3, grav=(2e-3, 0, 0),
+         walls=periodic(3, (8, 8, 8)) + [_disc((0.0, 4.0, 4.0), 3.4, P=(1, 2, 2, 2), A=(0.0, 1.0, 1.0), INV=1)]),
+    # 3D MPC-AT in a cylinder along x with P = 2 2 2 2 (the quadratic crossing time)
+    case("cyl3d_at", 3, (6, 8, 8), 8, rtech=2, steps=3, seed=35, dt=0.3,
+         walls=periodic(3, (6, 8, 8)) + [_disc((0.0, 4.0, 4.0), 3.4, A=(0.0, 1.0, 1.0), INV=1)]),
+    # 2D SRD channel between wavy walls (cos modulation along x) inside the flat domain walls
+    case("wavy2d_srd", 2, (20, 12), 10, rtech=0, steps=4, seed=36, dt=0.5,
+         walls=CS_pbc_x(20) + solid_walls(1, 12) +
+         [dict(A=[0, 1, 0], Q=[0, 2.5, 0], R=0.0, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1, P=[1, 1, 1, 1],
+               B=[0.8, 2 * math.pi / 10.0, 0.0]),
+          dict(A=[0, -1, 0], Q=[0, 9.5, 0], R=0.0, DN=0.0, MVN=-1.0, MVT=-1.0, PHANTOM=1, P=[1, 1, 1, 1],
+               B=[0.8, 2 * math.pi / 10.0, 0.0])],
+         qbox=[(0, 20), (3.4, 8.6)]),
+    # 2D active nematic around a disc with homeotropic strong anchoring
+    case("lc2d_disc", 2, (12, 10), 12, rtech=16, lc=1, steps=3, seed=37, dt=0.3,
+         sp=species(mfpot=10.0, tumble=0.5, rfc=0.01, act=0.05),
+         walls=periodic(2, (12, 10)) + [_disc((6.0, 5.0, 0.0), 2.6, MUN=1.0, MUT=0.0)]),
+]
+
 # rat2d_2sp uses two species
 CASES[6]["species"] = [species(mass=1.0), species(mass=2.5)]
 next(c for c in CASES if c["name"] == "at2d_nbounce")["species"] = [species(vdist=v) for v in (0, 2, 3, 4)]
 for _n in ("srd2d_couette_znm", "at3d_channel_znm"):
     next(c for c in CASES if c["name"] == _n)["species"] = [species(mass=1.0), species(mass=1.7)]
+
+
+def wall_w(w: dict, Q: np.ndarray, dim: int) -> np.ndarray:
+    """calcW (bc.c:55-150) of rows of Q for the wall dict w (four-fold symmetric
+    surfaces; the planar and the ellipsoidal wavy forms), the wall at its Q"""
+    A = np.asarray(w["A"], float)
+    q0 = np.asarray(w.get("Q", (0, 0, 0)), float)
+    P = list(w.get("P", (1, 1, 1, 1)))
+    B = list(w.get("B", (0, 0, 0)))
+    W = np.zeros(Q.shape[0])
+    for i in range(dim):
+        t = A[i] * (Q[:, i] - q0[i])
+        if w.get("ABS", 0):
+            t = np.abs(t)
+        W += t ** P[i]
+    r = abs(w.get("R", 0.0)) if w.get("ABS", 0) else w.get("R", 0.0)
+    W -= r ** P[3]
+    if B[0] != 0.0:
+        if all(abs(x - 1.0) < 1e-8 for x in P[:dim]):
+            W1 = (A[1] * Q[:, 0] - A[0] * Q[:, 1]) / np.sqrt(A[0] ** 2 + A[1] ** 2)
+        else:
+            W1 = np.arctan2(A[1] * (Q[:, 1] - q0[1]), A[0] * (Q[:, 0] - q0[0]))
+        W += B[0] * np.cos(B[1] * W1)
+    if w.get("INV", 0):
+        W = -W
+    return W
 
 
 def rest_frame_due(c: dict, k: int) -> bool:
@@ -177,6 +246,17 @@ def initial_state(c: dict):
     qbox = c.get("qbox") or [(0.0, L[d]) for d in range(dim)]
     for d in range(dim):
         Q[:, d] = rng.uniform(qbox[d][0], qbox[d][1], N)
+    # curved and wavy walls: no particle starts inside one (replace, init.c:1240-1243, with
+    # replacePos_WithCheck's test W <= TOL)
+    curved = [w for w in c["walls"] if "P" in w or "B" in w]
+    for _ in range(1000):
+        bad = np.zeros(N, bool)
+        for w in curved:
+            bad |= wall_w(w, Q, dim) <= 0.05
+        if not bad.any():
+            break
+        for d in range(dim):
+            Q[bad, d] = rng.uniform(qbox[d][0], qbox[d][1], int(bad.sum()))
     masses = np.array([s["mass"] for s in c["species"]])[spid]
     V[:, :dim] = rng.normal(size=(N, dim)) * np.sqrt(c["kbt"] / masses)[:, None]
     if dim == 3:

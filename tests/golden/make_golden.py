@@ -70,6 +70,10 @@ def write_case(c, Q, V, U, spid, avv, d):
             "vdist", "len")))
     for w in c["walls"]:
         lines.append(wall_line(w))
+        if "P" in w or "B" in w or "ABS" in w:
+            # a curved or wavy surface (bc.c:55-150): exponents, wavy amplitude and wavenumbers, |.| terms
+            v = list(w.get("P", (1, 1, 1, 1))) + list(w.get("B", (0, 0, 0))) + [w.get("ABS", 0)]
+            lines.append("shape " + " ".join(repr(float(x)) for x in v))
     cp = os.path.join(d, "case.txt")
     sp_ = os.path.join(d, "state.bin")
     with open(cp, "w") as f:
