@@ -105,13 +105,24 @@ typedef struct {
     int pad_;
 } mpcd_gpu_species;
 
-/* axis-aligned planar wall -- subset of the reference's bc (SRDclss.h:128-262);
- * periodic planes have DN = box length, MVN = MVT = 1; bounce-back planes
- * DN = 0, MVN = MVT = -1, PHANTOM = 1 (read.c:1441-1583) */
+/* wall -- subset of the reference's bc (SRDclss.h:128-262) for immobile walls
+ * with rule-based collisions (COLL_TYPE 1) and the four-fold rotational
+ * symmetry (ROTSYMM 4 4): the surface W(r) = sum_i (A_i (r_i - Q_i))^P_i -
+ * R^P_3 (+ the wavy term B, |.| terms if ABS, negated if INV) of calcW
+ * (bc.c:55-150).  Axis-aligned planes (P = 1 1 1, B[0] = 0, ABS = 0, one
+ * nonzero A) take the planar fast path; periodic planes have DN = box length,
+ * MVN = MVT = 1; bounce-back planes DN = 0, MVN = MVT = -1, PHANTOM = 1
+ * (read.c:1441-1583).  P, B and ABS are read.c's "P", "wavy" and "abs"
+ * (read.c:1286-1301,1394-1409); a P left all zero is taken as 1 1 1 1 (a
+ * plane), so zero-initialised records of planar walls need not set it. */
 typedef struct {
     double A[3], Q[3], R, DN, DT, DVN, DVT, DVxyz[3], MVN, MVT, MUN, MUT, MUxyz[3], DUxyz[3], V[3], L[3];
     int INV, PHANTOM, COLL_TYPE;
     int inter_mask; /* bit s set: species s interacts with this wall (bc.INTER, SRDclss.h:238) */
+    double P[4];    /* exponents of the surface (bc.P) */
+    double B[3];    /* wavy-wall amplitude and wave numbers (bc.B) */
+    int ABS;        /* absolute values in the surface terms (bc.ABS) */
+    int pad_;
 } mpcd_gpu_wall;
 
 /* one replica's parameters -- subset of inputList (SRDclss.h:395-424) */

@@ -93,7 +93,8 @@ class Wall(C.Structure):
                 ("MVN", C.c_double), ("MVT", C.c_double), ("MUN", C.c_double), ("MUT", C.c_double),
                 ("MUxyz", C.c_double * 3), ("DUxyz", C.c_double * 3), ("V", C.c_double * 3),
                 ("L", C.c_double * 3), ("INV", C.c_int), ("PHANTOM", C.c_int), ("COLL_TYPE", C.c_int),
-                ("inter_mask", C.c_int)]
+                ("inter_mask", C.c_int), ("P", C.c_double * 4), ("B", C.c_double * 3), ("ABS", C.c_int),
+                ("pad_", C.c_int)]
 
 
 class Params(C.Structure):
@@ -234,6 +235,11 @@ def make_params(cfg: dict, precision: int = FP64) -> Params:
         W.PHANTOM = int(w.get("PHANTOM", 0))
         W.COLL_TYPE = int(w.get("COLL_TYPE", 1))
         W.inter_mask = int(w.get("inter_mask", (1 << MAXSPEC) - 1))
+        for j, x in enumerate(w.get("P", (1, 1, 1, 1))):
+            W.P[j] = float(x)
+        for j, x in enumerate(w.get("B", (0, 0, 0))):
+            W.B[j] = float(x)
+        W.ABS = int(w.get("ABS", 0))
     return p
 
 

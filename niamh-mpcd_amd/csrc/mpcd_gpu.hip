@@ -49,6 +49,7 @@ struct mpcd_gpu_ctx {
   int cu_src = 0;
   int cur = 0;
   bool any_therm = false, any_phantom = false, any_lcg = false, fusable = false;
+  bool any_curved = false;  // a wall that is not an axis-aligned plane: the split path with the general walls
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -351,7 +352,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
       const mpcd_gpu_wall &W = p.wall[w];
       int nz = 0;
       for (int i = 0; i < 3; i++) nz += fneq(W.A[i], 0.0) ? 1 : 0;
-      if (nz != 1) return bad(MPCD_EUNSUPPORTED, "only axis-aligned planar walls are on this path");
+      if (nz == 0) return bad(MPCD_EINVAL, "a wall needs a nonzero A");
       if (W.COLL_TYPE != 1) return bad(MPCD_EUNSUPPORTED, "only rule-based wall collisions are on this path");
     }
     for (int i = 0; i < p.dim; i++)
@@ -399,11 +400,17 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
     R.gravflag = (fneq(p.grav[0], 0.0) || fneq(p.grav[1], 0.0) || fneq(p.grav[2], 0.0)) ? 1 : 0;
     R.gndnst = p.gndnst;
     R.anyphantom = 0;
+    R.anycurved = 0;
     for (int w = 0; w < p.nwall; w++) {
-      walls.push_back(p.wall[w]);
-      if (p.wall[w].PHANTOM) R.anyphantom = 1;
+      mpcd_gpu_wall W = p.wall[w];
+      if (W.P[0] == 0.0 && W.P[1] == 0.0 && W.P[2] == 0.0 && W.P[3] == 0.0)
+        for (int i = 0; i < 4; i++) W.P[i] = 1.0;
+      walls.push_back(W);
+      if (W.PHANTOM) R.anyphantom = 1;
+      if (wall_is_general(W)) R.anycurved = 1;
     }
     if (R.anyphantom) c->any_phantom = true;
+    if (R.anycurved) c->any_curved = true;
     if (p.tstech != MPCD_NOTHERM || (p.rtech >= 7 && p.rtech <= 17)) c->any_therm = true;
     if (p.lc == MPCD_LCG) c->any_lcg = true;
     c->reps.push_back(R);
@@ -432,7 +439,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->uploaded.assign(nrep, 0);
   c->lc_stride = anneal_stride(params[0]);
   // the fused kernel stages a cell in LDS: dense systems take the split path
-  c->fusable = (c->lc == MPCD_ISOF);
+  c->fusable = (c->lc == MPCD_ISOF) && !c->any_curved;
   for (const RepDev &R : c->reps)
     if (mean_ppc(R) > kFusedMaxPPC) c->fusable = false;
 
@@ -827,7 +834,10 @@ static void run_collide(mpcd_gpu_ctx *c, int injected) {
       if (tan) {
         k_ghost<Real, Dual><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
       }
-    if (!tan) k_ghost<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
+    if (!tan && c->any_curved)
+      k_ghost<Real, double, true><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
+    else if (!tan)
+      k_ghost<Real, double><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, T);
   }
   if (c->recording && c->tape_len < c->tape_cap)
     k_tape_cells<Real><<<c->cblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_cblk, D, tape_at(c, c->tape_len));
@@ -911,7 +921,9 @@ static void run_stream(mpcd_gpu_ctx *c, int injected) {
       k_stream<Real, Dual><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, wmul,
                                                                      tan_view(c, c->cur), box);
     }
-  if (!tan)
+  if (!tan && c->any_curved)
+    k_stream_walls<Real><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D);
+  else if (!tan)
     k_stream<Real, double><<<c->pblk.size(), kBlock, 0, c->stream>>>(C, S[c->cur], c->d_pblk, D, wmul, TanDev{}, box);
   if (tape) c->tape_len++;
   c->pending = true;
@@ -2076,6 +2088,7 @@ extern "C" mpcd_status mpcd_gpu_record(mpcd_gpu_ctx *c, int max_steps) {
     return MPCD_OK;
   }
   if (c->tan_on) return fail(c, MPCD_EINVAL, "the tangent and the adjoint tape cannot run together");
+  if (c->any_curved) return fail(c, MPCD_EUNSUPPORTED, "the adjoint covers axis-aligned planar walls only");
   // the reverse mode of the whole step covers what the tangent covers; the
   // velocity-only tape of mpcd_gpu_step_backward, SRD and MPC-AT between
   // resting walls
@@ -2491,6 +2504,7 @@ extern "C" mpcd_status mpcd_gpu_tangent(mpcd_gpu_ctx *c, int on, const double *d
   }
   if (c->precision != MPCD_FP64) return fail(c, MPCD_EUNSUPPORTED, "the tangent needs the FP64 state");
   if (c->recording) return fail(c, MPCD_EINVAL, "the tangent and the adjoint tape cannot run together");
+  if (c->any_curved) return fail(c, MPCD_EUNSUPPORTED, "the tangent covers axis-aligned planar walls only");
   for (const mpcd_gpu_params &p : c->params) {
     if (p.tstech != MPCD_NOTHERM) return fail(c, MPCD_EUNSUPPORTED, "the tangent needs tsTech = 0");
     if (p.lc == MPCD_LCG) return fail(c, MPCD_EUNSUPPORTED, "the tangent needs the local order parameter (lc 1)");

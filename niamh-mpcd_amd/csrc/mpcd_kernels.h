@@ -25,6 +25,7 @@
 #include "mpcd_dual.h"
 #include "mpcd_rev.h"
 #include "mpcd_params.h"
+#include "mpcd_walls.h"
 
 namespace mpcd {
 
@@ -38,6 +39,7 @@ struct RepDev {
   int cell_off, ncell, part_off, npart;
   int wall_off, nwall;
   int rtech, lc, tstech, galinv, true_sign, gravflag, anyphantom;
+  int anycurved;  // a wall that is not an axis-aligned plane (mpcd_walls.h)
   double dt, kbt, cosA, sinA, tau, grav[3], avv[3], gndnst;
 };
 
@@ -297,7 +299,8 @@ __device__ __forceinline__ void push_velocity(const mpcd_gpu_species &sp, double
 // kernel does not call it (its preparation kernel re-places listed escapes).
 __device__ __forceinline__ bool escape_repair(const RepDev &R, const WallHot *wh, int b, const Draws &Dr, uint32_t rep,
                                               uint32_t pid, double mass, bool lc, double q[3], double v[3],
-                                              double u[3]) {
+                                              double u[3], const mpcd_gpu_wall *walls = nullptr,
+                                              const double *wq3 = nullptr) {
   const size_t gp = (size_t)R.part_off + pid;
   const int dim = R.dim;
   const int tries = Dr.esc_keyed() ? (kEscStride - kEscPos0) / dim : kEscMaxTries;
@@ -306,8 +309,14 @@ __device__ __forceinline__ bool escape_repair(const RepDev &R, const WallHot *wh
     for (int j = 0; j < dim; j++) q[j] = (double)R.L[j] * Dr.esc_u(rep, b, gp, pid, kEscPos0 + dim * t + j);
     ok = true;
     for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
-      const WallHot &H = wh[w];
-      if (wall_W_hot(H, q, b ? H.wqb : H.wqs) <= kTOL) ok = false;
+      if (R.anycurved) {
+        // calcW of every wall at its current position (the general forms
+        // equal calcW_PLANE on a plane)
+        if (calc_w(walls[w], &wq3[w * 3], q, dim) <= kTOL) ok = false;
+      } else {
+        const WallHot &H = wh[w];
+        if (wall_W_hot(H, q, b ? H.wqb : H.wqs) <= kTOL) ok = false;
+      }
     }
   }
   for (int j = 0; j < dim; j++) v[j] = sqrt(R.kbt / mass) * Dr.esc_n(rep, b, gp, pid, j);
@@ -378,8 +387,11 @@ __device__ __forceinline__ void shift_replica(const Ctx &C, const Draws &D, int 
     H.phantom = W.PHANTOM;
     wh_out[w] = H;
     // wall-free box: a point strictly inside is on the positive side of
-    // every wall by far more than the rounding of calcW_PLANE
-    if (H.A == 0.0) {
+    // every wall by far more than the rounding of calcW_PLANE; with a
+    // curved wall the box is empty and every particle takes the search
+    if (R.anycurved) {
+      lo[0] = INFINITY;
+    } else if (H.A == 0.0) {
       const bool safe = H.inv ? (H.R > 0.0) : (H.R < 0.0);  // calcW is the constant -R (or R)
       if (!safe) lo[0] = INFINITY;
     } else {
@@ -582,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Ctx C, State<Real> S, State<R
       // escape repair (mpc.c:970-982); the particle stays in its list
       atomicAdd(&C.events[0], 1ull);
       esc = true;
-      if (!escape_repair(R, C.wh, 0, Dr, B.rep, (uint32_t)id, C.sp[sp].mass, lc != 0, q, v, u))
+      if (!escape_repair(R, C.wh, 0, Dr, B.rep, (uint32_t)id, C.sp[sp].mass, lc != 0, q, v, u, C.wall, C.wq_s))
         atomicOr(&C.flags[0], kFlagEscape);
     }
   }
@@ -897,7 +909,11 @@ __device__ __forceinline__ void ori_bc(F *u, const mpcd_gpu_wall &W, const F *n,
   normn(u, dim);
 }
 
-template <typename Real, typename F>
+// CURVED: the general surfaces of mpcd_walls.h (FP64 only): corners tested
+// with calcW, the anchoring normal at each particle (normal, bc.c:1516), and
+// at a curved wall the cell's S and director from the order tensor of the
+// anchored orientations (mpc.c:512-536)
+template <typename Real, typename F, bool CURVED = false>
 __device__ __forceinline__ void ghost_cell(const Ctx &C, const State<Real> &S, int rep, int c, const Draws &Dr,
                                            const TanDev &T) {
   using A = TA<F>;
@@ -919,31 +935,58 @@ __device__ __forceinline__ void ghost_cell(const Ctx &C, const State<Real> &S, i
     const mpcd_gpu_wall &W = C.wall[w];
     if (!(W.PHANTOM && anchoring(W))) continue;
     int fw = 0;
-    for (int j = 0; j < nco; j++)
-      if (wall_W(W, &C.wq_s[w * 3], tq[j], dim) < 0.0) { fw = 1; wallindex = w; }
+    for (int j = 0; j < nco; j++) {
+      const double Wc = CURVED ? calc_w(W, &C.wq_s[w * 3], tq[j], dim) : wall_W(W, &C.wq_s[w * 3], tq[j], dim);
+      if (Wc < 0.0) { fw = 1; wallindex = w; }
+    }
     if (fw) numBC++;
   }
   int b = C.start[c], e = C.start[c + 1];
   for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
     const mpcd_gpu_wall &W = C.wall[w];
     if (!W.PHANTOM) continue;
-    for (int j = 0; j < nco; j++)
-      if (wall_W(W, &C.wq_s[w * 3], tq[j], dim) < 0.0) flagW = 1;
+    for (int j = 0; j < nco; j++) {
+      const double Wc = CURVED ? calc_w(W, &C.wq_s[w * 3], tq[j], dim) : wall_W(W, &C.wq_s[w * 3], tq[j], dim);
+      if (Wc < 0.0) flagW = 1;
+    }
     if (R.lc != MPCD_ISOF && flagW && numBC == 1 && wallindex == w && anchoring(W)) {
       for (int t = b; t < e; t++) {
         if (!((W.inter_mask >> S.sp[t]) & 1)) continue;
         F n[3] = {0., 0., 0.}, u[3] = {A::ld(S.ux, T.u[0], t), A::ld(S.uy, T.u[1], t), A::ld(S.uz, T.u[2], t)};
-        for (int k = 0; k < dim; k++) n[k] = W.A[k];
+        if constexpr (CURVED) {
+          double q[3];
+          load_q(S, t, q);
+          wall_normal(W, &C.wq_s[w * 3], q, dim, n);
+        } else {
+          for (int k = 0; k < dim; k++) n[k] = W.A[k];
+        }
         normn(n, dim);
         ori_bc(u, W, n, dim);
         A::st(S.ux, T.u[0], t, u[0]);
         A::st(S.uy, T.u[1], t, u[1]);
         A::st(S.uz, T.u[2], t, u[2]);
       }
-      A::st(C.cell.S, T.S, c, F(1.0));
-      F d3[3] = {A::ld(S.ux, T.u[0], b), A::ld(S.uy, T.u[1], b), A::ld(S.uz, T.u[2], b)};
-      normn(d3, dim);
-      for (int k = 0; k < dim; k++) A::st(C.cell.dir, T.dir, c * 3 + k, d3[k]);
+      const bool plane = feq(W.P[0], 1.0) && feq(W.P[1], 1.0) && feq(W.P[2], 1.0);
+      if (!CURVED || plane || pop == 1) {
+        // a plane, or a single particle: S = 1 along the first orientation
+        A::st(C.cell.S, T.S, c, F(1.0));
+        F d3[3] = {A::ld(S.ux, T.u[0], b), A::ld(S.uy, T.u[1], b), A::ld(S.uz, T.u[2], b)};
+        normn(d3, dim);
+        for (int k = 0; k < dim; k++) A::st(C.cell.dir, T.dir, c * 3 + k, d3[k]);
+      } else if constexpr (CURVED) {
+        // tensOrderParam + solveEigensystem of the anchored orientations;
+        // the cell's recorded decisions stay those of localPROP
+        double Sm[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+        for (int t = b; t < e; t++) {
+          const double u[3] = {(double)S.ux[t], (double)S.uy[t], (double)S.uz[t]};
+          for (int i = 0; i < dim; i++)
+            for (int j = i; j < dim; j++) Sm[i][j] += u[i] * u[j];
+        }
+        Ctx Cn = C;
+        Cn.cdec = nullptr;
+        if (dim == 3) order_of_cell<3>(Cn, c, Sm, pop);
+        else order_of_cell<2>(Cn, c, Sm, pop);
+      }
     }
   }
   if (flagW && pop < R.gndnst && pop > 0) {
@@ -959,12 +1002,12 @@ __device__ __forceinline__ void ghost_cell(const Ctx &C, const State<Real> &S, i
   }
 }
 
-template <typename Real, typename F>
+template <typename Real, typename F, bool CURVED = false>
 __global__ __launch_bounds__(kBlock) void k_ghost(Ctx C, State<Real> S, const BlkDev *cblk, Draws Dr, TanDev T) {
   Dr.resolve();
   const BlkDev B = cblk[blockIdx.x];
   const int c = B.beg + threadIdx.x;
-  if (c < B.end) ghost_cell<Real, F>(C, S, B.rep, c, Dr, T);
+  if (c < B.end) ghost_cell<Real, F, CURVED>(C, S, B.rep, c, Dr, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -2082,7 +2125,8 @@ __device__ __forceinline__ void stream_particle(const Ctx &C, const State<Real> 
     mul[0] = mul[1] = mul[2] = __builtin_nanf("");
     const uint32_t pid = (uint32_t)S.id[t];
     double q[3] = {val(Q[0]), val(Q[1]), val(Q[2])}, v[3], u[3] = {val(U[0]), val(U[1]), val(U[2])};
-    if (!escape_repair(R, C.wh, 1, Dr, rep, pid, sp.mass, lc, q, v, u)) atomicOr(&C.flags[0], kFlagEscape);
+    if (!escape_repair(R, C.wh, 1, Dr, rep, pid, sp.mass, lc, q, v, u, C.wall, C.wq_b))
+      atomicOr(&C.flags[0], kFlagEscape);
     for (int j = 0; j < 3; j++) {
       Q[j] = q[j];
       if (j < dim) V[j] = v[j];
@@ -2104,6 +2148,168 @@ __device__ __forceinline__ void stream_particle(const Ctx &C, const State<Real> 
   C.cu[t] = cu;
   if (wmul)
     for (int i = 0; i < 3; i++) wmul[3 * t + i] = mul[i];
+}
+
+// streaming with the general wall engine (mpcd_walls.h): MPC_BCcollision
+// (bc.c:1363-1496) with chooseBC over every wall the species meets
+// (bc.c:1252-1328: planes by calcW_PLANE and the plane crossing time, wavy
+// planes and curved surfaces by calcW, crosstimeReverse and chooseT, the
+// curved ones at their periodic image nearest the particle), the bounce at
+// the surface normal of the crossing point, and the NBOUNCE rewinds and
+// re-thermalisations of the planar path.  FP64 arithmetic; the derivative
+// modes are refused for contexts with such walls.
+template <typename Real>
+__device__ __forceinline__ void stream_particle_walls(const Ctx &C, const State<Real> &S, int rep, int t,
+                                                      const Draws &Dr) {
+  const RepDev &R = C.rep[rep];
+  const int dim = R.dim;
+  double Q[3], V[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]}, U[3] = {0., 0., 0.};
+  load_q(S, t, Q);
+  const bool lc = R.lc != MPCD_ISOF;
+  if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
+  const int spid = S.sp[t];
+  const mpcd_gpu_species &sp = C.sp[spid];
+  for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
+  const double t_step = R.dt;
+  double t_left = t_step, t_coll = 0.0;
+  int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
+  for (;;) {
+    // chooseBC: the earliest crossing among the walls the particle is inside
+    double tmin = t_left, chW = 1.0;
+    for (int w = R.wall_off; w < R.wall_off + R.nwall; w++) {
+      const mpcd_gpu_wall &W = C.wall[w];
+      if (!((W.inter_mask >> spid) & 1)) continue;
+      const double *wq = &C.wq_b[w * 3];
+      double wqi[3], tw, t1, t2, tc;
+      if (wall_is_plane(W)) {
+        tw = calc_w_plane(W, wq, Q, dim);
+        if (tw < 0.0) {
+          crosstime_reverse(W, true, wq, Q, V, dim, t_left, &t1, &t2);
+          tc = t_left - t1;
+          if (tc < tmin) { tmin = tc; chosen = w; chW = tw; }
+        }
+        continue;
+      }
+      const bool wavy_plane = !feq(W.B[0], 0.0) && feq(W.P[0], 1.0) && feq(W.P[1], 1.0) && feq(W.P[2], 1.0);
+      if (wavy_plane) for (int k = 0; k < 3; k++) wqi[k] = wq[k];
+      else wall_image(W, R.L, dim, wq, Q, wqi);
+      tw = calc_w(W, wqi, Q, dim);
+      if (tw < 0.0) {
+        crosstime_reverse(W, false, wqi, Q, V, dim, t_left, &t1, &t2);
+        tc = choose_t(t_left, t_left - t1, t_left - t2);
+        if (tc < tmin) { tmin = tc; chosen = w; chW = tw; }
+      }
+    }
+    t_coll = tmin;
+    if (chW > -kTOL) break;
+    cnt++;
+    const mpcd_gpu_wall &W = C.wall[chosen];
+    double wq[3];
+    wall_image(W, R.L, dim, &C.wq_b[chosen * 3], Q, wq);
+    if (t_coll > -kTOL) {
+      double n[3] = {0., 0., 0.};
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];  // rewind_P
+      wall_normal(W, wq, Q, dim, n);
+      normn(n, dim);
+      // velBC, rule-based collisions (bc.c:598-672)
+      double Rr[3] = {0., 0., 0.}, VR[3], Vr[3] = {0., 0., 0.}, VN[3] = {0., 0., 0.}, VT[3] = {0., 0., 0.};
+      const double WL[3] = {W.L[0], W.L[1], W.L[2]};
+      for (int i = 0; i < dim; i++) Rr[i] = Q[i] - wq[i];
+      cross3(WL, Rr, VR);
+      for (int i = 0; i < dim; i++) Vr[i] = V[i] - W.V[i] - VR[i];
+      projn(Vr, n, VN, dim);
+      for (int i = 0; i < dim; i++) VT[i] = Vr[i] - VN[i];
+      const double IMpart = 1.0 / sp.mass;
+      for (int i = 0; i < dim; i++) {
+        VN[i] *= W.MVN;
+        VN[i] += W.DVN * n[i];
+        VT[i] *= W.MVT;
+        VT[i] += W.DVT;
+      }
+      for (int i = 0; i < dim; i++) Vr[i] = VN[i] + VT[i];
+      for (int i = 0; i < dim; i++) Vr[i] += W.DVxyz[i];
+      for (int i = 0; i < dim; i++) Vr[i] += W.V[i] + VR[i];
+      for (int i = 0; i < dim; i++) {
+        Vr[i] -= V[i];
+        Vr[i] *= sp.mass;
+      }
+      for (int i = 0; i < dim; i++) V[i] += Vr[i] * 1.0 * IMpart;
+      // posBC (bc.c:760-774)
+      double PN[3] = {0., 0., 0.}, PT[3] = {0., 0., 0.};
+      projn(Q, n, PN, dim);
+      for (int i = 0; i < dim; i++) PT[i] = Q[i] - PN[i];
+      for (int i = 0; i < dim; i++) {
+        PN[i] += W.DN * n[i] * (1. - 2. * kTOL);
+        PT[i] += W.DT;
+      }
+      for (int i = 0; i < dim; i++) Q[i] = PN[i] + PT[i];
+      if (lc) ori_bc(U, W, n, dim);
+      t_left -= t_coll;
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] + t_left * V[i];  // stream_P
+      if (calc_w(W, wq, Q, dim) < -kTOL)
+        for (int i = 0; i < dim; i++) Q[i] = Q[i] - t_left * V[i];
+    } else {
+      for (int i = 0; i < dim; i++) Q[i] = Q[i] - (t_left - t_coll) * V[i];
+    }
+    if (cnt > kNBOUNCE) {
+      flagCNT = 1;
+      if (cnt < 2 * kNBOUNCE && cnt2 == 0) {
+        atomicAdd(&C.events[2], 1ull);
+        if (t_left > kTOL) t_left *= 1.1;
+        else {
+          t_left = 0.9 * t_step;
+          t_coll = 0.1 * t_step;
+        }
+      } else {
+        atomicAdd(&C.events[3], 1ull);
+        if (Dr.reth_keyed() && rk + dim > kRethStride) {
+          atomicOr(&C.flags[0], kFlagPushes);
+          break;
+        }
+        const uint32_t pid = (uint32_t)S.id[t];
+        push_velocity(sp, R.kbt, dim, Dr, rep, (size_t)R.part_off + pid, pid, rk, V);
+        rk += 3;
+        t_left = t_step;
+        t_coll = 0.0;
+        cnt2++;
+        cnt = 0;
+      }
+      if (cnt2 > 1000) {  // walls that leave the particle no room: the reference would not return
+        atomicOr(&C.flags[0], kFlagStuck);
+        break;
+      }
+    }
+  }
+  if (flagCNT) atomicAdd(&C.events[1], 1ull);
+  // the unshifted bin (mpc.c:4936) with bin()'s escape repair (mpc.c:970-982)
+  int cu = cell_of(R, Q, 0);
+  if (cu < 0) {
+    atomicAdd(&C.events[0], 1ull);
+    const uint32_t pid = (uint32_t)S.id[t];
+    if (!escape_repair(R, C.wh, 1, Dr, rep, pid, sp.mass, lc, Q, V, U, C.wall, C.wq_b))
+      atomicOr(&C.flags[0], kFlagEscape);
+    cu = S.cst[t];
+  }
+  S.x[t] = (Real)Q[0];
+  S.y[t] = (Real)Q[1];
+  S.z[t] = (Real)Q[2];
+  S.vx[t] = (Real)V[0];
+  S.vy[t] = (Real)V[1];
+  S.vz[t] = (Real)V[2];
+  if (lc) {
+    S.ux[t] = (Real)U[0];
+    S.uy[t] = (Real)U[1];
+    S.uz[t] = (Real)U[2];
+  }
+  C.cu[t] = cu;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_stream_walls(Ctx C, State<Real> S, const BlkDev *blk, Draws Dr) {
+  Dr.resolve();
+  const BlkDev B = blk[blockIdx.x];
+  const int t = B.beg + threadIdx.x;
+  if (t < B.end) stream_particle_walls<Real>(C, S, B.rep, t, Dr);
 }
 
 template <typename Real, typename F>
