@@ -5,8 +5,12 @@ CPU step timed on this host.
 Workload (both forms): 3D SRD, periodic 64^3 cells x 10 particles per cell
 (2 621 440 particles per replica), ARBAXIS rotation 130 degrees, dt 0.1,
 kT 1, random grid shift every step, no thermostat.
-  * headline `value`: FP32 particle state with FP64 cell sums, B independent
-    replicas (default 8) advancing together in one context;
+  * headline `value`: the exact-sum FP32 form (FP32 particle state and
+    arithmetic, cell sums of quantised terms added exactly in FP64, so no
+    list order), B independent replicas (default 8) advancing together in
+    one context;
+  * `fp32_list_order`: the same replicas with the FP32 state stepped in the
+    reference's list order (FP64 cell sums in list order);
   * `fp64_plain`: FP64 state, one replica.
 A step is one full MPCD step (shift, bin, cell sums, collision, shift back,
 stream, walls) over every particle.  Inputs are resident in HBM before the
@@ -35,10 +39,10 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per particle-step of the whole step: one read and one
 # write of the persistent particle state, r and v (3D: 6 words), r and v in
 # 2D (4 words), r, v and u for 2D nematics (6 words)
-B_ALG = {"fp64": 96.0, "fp32": 48.0}
+B_ALG = {"fp64": 96.0, "fp32": 48.0, "fp32x": 48.0}
 B_ALG_2D = {"fp64": 64.0, "fp32": 32.0}
 B_ALG_LC2D = {"fp64": 96.0, "fp32": 48.0}
-PREC = {"fp64": 8, "fp32": 4}
+PREC = {"fp64": 8, "fp32": 4, "fp32x": 4}
 
 
 def fam_bytes(fam, form, dim=3, lc=False):
@@ -57,8 +61,9 @@ def fam_bytes(fam, form, dim=3, lc=False):
             "stream": 2 * (2 * c + u) * p + 8,  # read state, species; write state, unshifted cell
             # fused: read key, r, v, id; write r, v, id, slot cell, next cell, rank, key
             "fused": 4 * c * p + 44,
-            # bucketed step: read r, v, id, key from the bucket; write r, v, id, key to the next bucket
-            "bucket": 4 * c * p + 24,
+            # bucketed step: read r, v, id (and the list-order key, 8 B, but in the exact-sum form) from the
+            # bucket; write them to the next bucket
+            "bucket": 4 * c * p + (8 if form == "fp32x" else 24),
             "ghost": 0, "temp": 0, "scan": 0, "flow": 0}[fam]
 
 
@@ -536,19 +541,32 @@ def main():
     if args.only in ("fp32", "both"):
         B = args.replicas
         n1 = int(np.prod(L)) * args.ppc
-        ctx = M.Context([cfg] * B, [n1] * B, seed=2024, precision=M.FP32)
-        for r in range(B):
-            Q, V = initial_state(L, args.ppc, seed=100 + r)
-            ctx.upload(r, Q, V)
-        if args.fill > 0:
-            ctx.set_layout(True, fill=args.fill)
-            tuned_ms = 0.0
-        else:
-            tuned_ms = ctx.autotune(3)
-        print(f"[bench] fp32 batched: {B} x {n1} particles, tile fill {ctx.tile_fill():.2f} ({tuned_ms:.3f} ms/step "
-              f"in the autotuner), warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
-        r32 = timed_run(M, ctx, args.steps, args.warmup, B * n1)
-        r32["graph"] = graph_run(ctx, args.steps, B * n1)
+
+        def batched_ctx(prec, seed):
+            x = M.Context([cfg] * B, [n1] * B, seed=seed, precision=prec)
+            for r in range(B):
+                Q, V = initial_state(L, args.ppc, seed=100 + r)
+                x.upload(r, Q, V)
+            if args.fill > 0:
+                x.set_layout(True, fill=args.fill)
+                return x, 0.0
+            return x, x.autotune(3)
+
+        # the headline: the exact-sum FP32 form (order-free cell sums, 28-byte records)
+        ctx, tuned_ms = batched_ctx(M.FP32_EXACT, 2024)
+        print(f"[bench] fp32 exact-sum batched: {B} x {n1} particles, tile fill {ctx.tile_fill():.2f} "
+              f"({tuned_ms:.3f} ms/step in the autotuner), warmup {args.warmup}, steps {args.steps}",
+              file=sys.stderr, flush=True)
+        r32x = timed_run(M, ctx, args.steps, args.warmup, B * n1)
+        r32x["graph"] = graph_run(ctx, args.steps, B * n1)
+        r32x["tile_fill"] = ctx.tile_fill()
+        result["fp32x"] = (r32x, B * n1, B)
+        ctx.close()
+        # the FP32 state stepped in the reference's list order (FP64 cell sums in list order)
+        ctx, tuned_ms = batched_ctx(M.FP32, 2024)
+        print(f"[bench] fp32 list-order batched: tile fill {ctx.tile_fill():.2f}", file=sys.stderr, flush=True)
+        r32 = timed_run(M, ctx, max(20, args.steps // 2), args.warmup, B * n1)
+        r32["graph"] = graph_run(ctx, max(20, args.steps // 2), B * n1)
         r32["tile_fill"] = ctx.tile_fill()
         # the same steps with the running flow field on (the reference's flow
         # output, localFLOW + sumFLOW every step): what it adds to a step
@@ -638,25 +656,26 @@ def main():
         if not args.no_cpu:
             small["cpu_reference"] = cpu_baseline(L=(30, 30), ppc=20, steps=1000, dim=2, rtech=1,
                                                   rotang=math.pi / 2)
-    head_key = "fp32" if "fp32" in result else "fp64"
+    head_key = "fp32x" if "fp32x" in result else "fp64"
     r, n, B = result[head_key]
     line = {
         "metric": METRIC,
         "value": r["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps if head_key == "fp32" else (args.fp64_steps or args.steps),
+        "steps": args.steps if head_key == "fp32x" else (args.fp64_steps or args.steps),
         "warmup": args.warmup,
         "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 state / f64 cell sums" if head_key == "fp32" else "f64",
+        "dtype": "f32 state / exact f64 cell sums" if head_key == "fp32x" else "f64",
         "data": "synthetic (uniform positions, Maxwell-Boltzmann velocities)",
         "config": {
             "workload": f"3D SRD {L[0]}^3 cells x {args.ppc} ppc, ARBAXIS 130 deg, dt 0.1, kT 1, random grid shift, "
-                        f"periodic; {'FP32 state / FP64 cell sums, ' + str(B) + ' replicas' if head_key == 'fp32' else 'FP64, 1 replica'}",
-            "replicas": B, "particles": n, "collision": "SRD ARBAXIS", "precision": head_key,
+                        f"periodic; {'FP32 state / exact-sum FP64 cell sums, ' + str(B) + ' replicas' if head_key == 'fp32x' else 'FP64, 1 replica'}",
+            "replicas": B, "particles": n, "collision": "SRD ARBAXIS",
+            "precision": "fp32_exact" if head_key == "fp32x" else head_key,
         },
         "roofline": roofline(r["timing"], n, head_key),
         "cpu_baseline": cpu,
@@ -670,9 +689,17 @@ def main():
         "adjoint": adjoint,
         "gradients_active_nematic": tangent,
         "lc_active_nematic": lc,
-        "flow_field": r.get("flow_field"),
+        "flow_field": result["fp32"][0].get("flow_field") if "fp32" in result else None,
         "small_box_2d": small,
     }
+    if "fp32" in result:
+        r32, n32, _ = result["fp32"]
+        line["fp32_list_order"] = {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
+                                   "hipgraph": r32["graph"], "tile_fill": r32["tile_fill"],
+                                   "roofline": roofline(r32["timing"], n32, "fp32"),
+                                   "kernels_ms_per_step": {k: v[0] / r32["prof_steps"]
+                                                           for k, v in r32["timing"].items() if v[1]},
+                                   "step_hbm_frac": r32["value"] * B_ALG["fp32"] / 1e9 / HBM_PEAK_GBS}
     if "fp64" in result and head_key != "fp64":
         r64, n64, _ = result["fp64"]
         line["fp64_plain"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"], "hipgraph": r64["graph"],

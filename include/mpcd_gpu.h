@@ -95,8 +95,15 @@ enum { MPCD_ARBAXIS = 0, MPCD_ORTHAXIS = 1, MPCD_MPCAT = 2, MPCD_RAT = 3, MPCD_D
 /* liquid-crystal modes (definitions.h:194-200) and thermostats (definitions.h:238-246) */
 enum { MPCD_ISOF = 0, MPCD_LCL = 1, MPCD_LCG = 2 };
 enum { MPCD_NOTHERM = 0, MPCD_VSC = 1, MPCD_BEREND = 2 };
-/* state precision */
-enum { MPCD_FP64 = 0, MPCD_FP32 = 1 };
+/* state precision.  MPCD_FP64 and MPCD_FP32 step the reference's algorithm in
+ * its list order with FP64 cell sums (FP32: positions and velocities stored
+ * in FP32).  MPCD_FP32_EXACT (no counterpart in the reference) is the batched
+ * FP32 form of the isotropic operators without a thermostat: FP32 state and
+ * particle arithmetic, cell sums of terms quantised to 2^-28 and added
+ * exactly in FP64, so every sum -- and the step -- is independent of the
+ * particles' order; whole steps only (no phase entry points, no injected
+ * draws, no flow field). */
+enum { MPCD_FP64 = 0, MPCD_FP32 = 1, MPCD_FP32_EXACT = 2 };
 
 /* species -- subset of the reference's spec (SRDclss.h:54-77) */
 typedef struct {
@@ -137,7 +144,7 @@ typedef struct {
     double gndnst;         /* global number density GnDNST (init.c:1002) */
     int rtech, lc, tstech, galinv;
     int true_random_sign;  /* 0: the reference's always-negative SRD angle (mpc.c:1291) */
-    int precision;         /* MPCD_FP64 or MPCD_FP32 particle state */
+    int precision;         /* MPCD_FP64, MPCD_FP32 or MPCD_FP32_EXACT particle state */
     int nspec, nwall;
     mpcd_gpu_species sp[MPCD_MAXSPEC];
     mpcd_gpu_wall wall[MPCD_MAXWALL];

@@ -40,7 +40,7 @@ ARBAXIS, ORTHAXIS, MPCAT, RAT = 0, 1, 2, 3
 DIPOLE_DIR_SUM, DIPOLE_DIR_AV, DIPOLE_DIR_SIG, DIPOLE_DIR_SIG_SUM = 16, 17, 20, 21
 ISOF, LCL, LCG = 0, 1, 2
 NOTHERM, VSC, BEREND = 0, 1, 2
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_EXACT = 0, 1, 2
 
 STATUS = {0: "MPCD_OK", 1: "MPCD_EINVAL", 2: "MPCD_ENOMEM", 3: "MPCD_EDEVICE", 4: "MPCD_EUNSUPPORTED",
           5: "MPCD_EESCAPE", 6: "MPCD_ECAPACITY"}

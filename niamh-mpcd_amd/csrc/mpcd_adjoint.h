@@ -46,12 +46,8 @@ __device__ __forceinline__ void srd_matrix(int dim, int rtech, int CA, const dou
       M[6] = L[0] * L[2] * (1. - Cc) - L[1] * Sn;
       M[7] = L[1] * L[2] * (1. - Cc) + L[0] * Sn;
       M[8] = L[2] * L[2] + (1. - L[2] * L[2]) * Cc;
-    } else if (CA == 0) {
-      M[0] = 1.; M[1] = 0.; M[2] = 0.; M[3] = 0.; M[4] = Cc; M[5] = Sn; M[6] = 0.; M[7] = -.1 * Sn; M[8] = Cc;
-    } else if (CA == 1) {
-      M[0] = Cc; M[1] = 0.; M[2] = Sn; M[3] = 0.; M[4] = 1.; M[5] = 0.; M[6] = -1. * Sn; M[7] = 0.; M[8] = Cc;
     } else {
-      M[0] = Cc; M[1] = Sn; M[2] = 0.; M[3] = -1. * Sn; M[4] = Cc; M[5] = 0.; M[6] = 0.; M[7] = 0.; M[8] = 1.;
+      orth_matrix(CA, Cc, Sn, M);
     }
   } else {
     M[0] = Cc; M[1] = -1. * Sn; M[2] = 0.;

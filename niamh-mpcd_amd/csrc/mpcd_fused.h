@@ -318,12 +318,8 @@ __device__ __forceinline__ void cell_terms(const RepDev &R, const mpcd_gpu_speci
         M[6] = L[0] * L[2] * (1. - Cc) - L[1] * Sn_;
         M[7] = L[1] * L[2] * (1. - Cc) + L[0] * Sn_;
         M[8] = L[2] * L[2] + (1. - L[2] * L[2]) * Cc;
-      } else if (CA == 0) {
-        M[0] = 1.; M[1] = 0.; M[2] = 0.; M[3] = 0.; M[4] = Cc; M[5] = Sn_; M[6] = 0.; M[7] = -.1 * Sn_; M[8] = Cc;
-      } else if (CA == 1) {
-        M[0] = Cc; M[1] = 0.; M[2] = Sn_; M[3] = 0.; M[4] = 1.; M[5] = 0.; M[6] = -1. * Sn_; M[7] = 0.; M[8] = Cc;
       } else {
-        M[0] = Cc; M[1] = Sn_; M[2] = 0.; M[3] = -1. * Sn_; M[4] = Cc; M[5] = 0.; M[6] = 0.; M[7] = 0.; M[8] = 1.;
+        orth_matrix(CA, Cc, Sn_, M);
       }
     } else {
       M[0] = Cc; M[1] = -1. * Sn_; M[2] = 0.;

@@ -17,6 +17,7 @@
 
 #include "mpcd_adjoint.h"
 #include "mpcd_bucket.h"
+#include "mpcd_exact.h"
 #include "mpcd_flow.h"
 #include "mpcd_fused.h"
 #include "mpcd_kernels.h"
@@ -84,6 +85,12 @@ struct mpcd_gpu_ctx {
   // records whose particle escapes in the next shifted bin, by step parity
   int32_t *esc_idx[2] = {nullptr, nullptr}, *esc_cnt = nullptr;  // esc_cnt [2]
   int esc_cap = 0;
+  // the exact-sum FP32 form (mpcd_exact.h): its bucketed records, wall list,
+  // escape lists by step parity, and the cell arrays of its dense steps
+  bool exact = false;
+  XState bx[2]{};
+  XList xwl{}, xesc[2]{};
+  XDense xd{};
   hipGraph_t bgraph = nullptr;
   hipGraphExec_t bgexec = nullptr;
   int bgraph_cur = -1;
@@ -316,6 +323,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   c->lc = params[0].lc;
   c->rtech = params[0].rtech;
   c->precision = params[0].precision;
+  c->exact = c->precision == MPCD_FP32_EXACT;
   c->nspec = params[0].nspec;
   c->params.assign(params, params + nrep);
   auto bad = [&](mpcd_status st, const char *m) {
@@ -329,7 +337,22 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
     if (p.dim != c->dim || p.lc != c->lc || p.rtech != c->rtech || p.precision != c->precision)
       return bad(MPCD_EINVAL, "dim, lc, rtech and precision must agree across replicas");
     if (p.dim != 2 && p.dim != 3) return bad(MPCD_EINVAL, "dim must be 2 or 3");
-    if (p.precision != MPCD_FP64 && p.precision != MPCD_FP32) return bad(MPCD_EINVAL, "precision");
+    if (p.precision != MPCD_FP64 && p.precision != MPCD_FP32 && p.precision != MPCD_FP32_EXACT)
+      return bad(MPCD_EINVAL, "precision");
+    bool curved_phantom = false;
+    for (int w = 0; w < p.nwall; w++) {
+      mpcd_gpu_wall W = p.wall[w];
+      if (W.P[0] == 0.0 && W.P[1] == 0.0 && W.P[2] == 0.0 && W.P[3] == 0.0)
+        for (int i = 0; i < 4; i++) W.P[i] = 1.0;  // a P left zero is a plane
+      if (W.PHANTOM && wall_is_general(W)) curved_phantom = true;
+    }
+    if (p.precision == MPCD_FP32_EXACT && curved_phantom)
+      return bad(MPCD_EUNSUPPORTED, "the exact-sum FP32 form takes phantom walls that are axis-aligned planes");
+    if (p.precision == MPCD_FP32_EXACT &&
+        (p.lc != MPCD_ISOF || p.tstech != MPCD_NOTHERM ||
+         !(p.rtech == MPCD_ARBAXIS || p.rtech == MPCD_ORTHAXIS || p.rtech == MPCD_MPCAT || p.rtech == MPCD_RAT)))
+      return bad(MPCD_EUNSUPPORTED,
+                 "the exact-sum FP32 form covers SRD, MPC-AT and MPC-AT+a without a thermostat or nematic mode");
     if (p.nspec < 1 || p.nspec > MPCD_MAXSPEC || p.nwall < 0 || p.nwall > MPCD_MAXWALL)
       return bad(MPCD_EINVAL, "species or wall count out of range");
     if (memcmp(p.sp, params[0].sp, sizeof(mpcd_gpu_species) * MPCD_MAXSPEC) != 0 || p.nspec != params[0].nspec)
@@ -401,6 +424,7 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
     R.gndnst = p.gndnst;
     R.anyphantom = 0;
     R.anycurved = 0;
+    R.xwrap = (p.precision == MPCD_FP32_EXACT && walls_are_box_faces(p)) ? 1 : 0;
     for (int w = 0; w < p.nwall; w++) {
       mpcd_gpu_wall W = p.wall[w];
       if (W.P[0] == 0.0 && W.P[1] == 0.0 && W.P[2] == 0.0 && W.P[3] == 0.0)
@@ -1032,8 +1056,14 @@ static void run_step_fused(mpcd_gpu_ctx *c, int injected) {
   c->step++;
 }
 
+static void run_step_xdense(mpcd_gpu_ctx *c);
+
 template <typename Real>
 static void run_step(mpcd_gpu_ctx *c, int injected) {
+  if (c->exact) {
+    run_step_xdense(c);
+    return;
+  }
   if (c->fusable && !c->recording && !c->tan_on) {
     run_step_fused<Real>(c, injected);
     return;
@@ -1220,6 +1250,7 @@ extern "C" mpcd_status mpcd_gpu_download_aos(mpcd_gpu_ctx *c, int r, void *base,
 extern "C" mpcd_status mpcd_gpu_bin(mpcd_gpu_ctx *c, const mpcd_draws *draws, int shifted, int32_t *cell_out,
                                     int32_t *count_out, int32_t *perm_out) {
   if (!c) return MPCD_EINVAL;
+  if (c->exact) return fail(c, MPCD_EUNSUPPORTED, "the exact-sum FP32 form steps whole steps only");
   mpcd_status st = ensure_init(c);
   if (st) return st;
   st = ensure_dense(c);
@@ -1266,6 +1297,7 @@ static mpcd_status read_kbt(mpcd_gpu_ctx *c, double *kbt_now_out) {
 
 extern "C" mpcd_status mpcd_gpu_collide(mpcd_gpu_ctx *c, const mpcd_draws *draws, double *kbt_now_out) {
   if (!c) return MPCD_EINVAL;
+  if (c->exact) return fail(c, MPCD_EUNSUPPORTED, "the exact-sum FP32 form steps whole steps only");
   mpcd_status st = ensure_init(c);
   if (st) return st;
   st = ensure_dense(c);
@@ -1329,6 +1361,7 @@ static mpcd_status read_events(mpcd_gpu_ctx *c, int64_t *out) {
 
 extern "C" mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *c, int64_t *bc_event_counts_out) {
   if (!c) return MPCD_EINVAL;
+  if (c->exact) return fail(c, MPCD_EUNSUPPORTED, "the exact-sum FP32 form steps whole steps only");
   mpcd_status st = ensure_init(c);
   if (st) return st;
   st = ensure_dense(c);
@@ -1347,6 +1380,8 @@ extern "C" mpcd_status mpcd_gpu_stream(mpcd_gpu_ctx *c, int64_t *bc_event_counts
 
 extern "C" mpcd_status mpcd_gpu_step(mpcd_gpu_ctx *c, const mpcd_draws *draws, double *kbt_now_out) {
   if (!c) return MPCD_EINVAL;
+  if (c->exact && draws)
+    return fail(c, MPCD_EUNSUPPORTED, "the exact-sum FP32 form draws from its generator (no injected draws)");
   mpcd_status st = ensure_init(c);
   if (st) return st;
   st = ensure_dense(c);
@@ -1421,6 +1456,7 @@ static bool alloc_wall_list(mpcd_gpu_ctx *c, WallList<Real> &w) {
 }
 
 static bool bucket_usable(const mpcd_gpu_ctx *c) {
+  if (c->exact) return c->bkt_on;  // no thermostat, tape or tangent in this form
   return c->bkt_on && c->fusable && !c->recording && !c->tan_on && !c->any_therm;
 }
 
@@ -1461,6 +1497,25 @@ static void dfree(mpcd_gpu_ctx *c, void *p) {
   (void)hipFree(p);
 }
 
+static bool alloc_xstate(mpcd_gpu_ctx *c, XState &b, size_t n) {
+  void *old[] = {b.a, b.b, b.id, b.sp};
+  for (void *p : old) dfree(c, p);
+  b.a = dalloc<float4>(c, n);
+  b.b = dalloc<float2>(c, n);
+  b.id = dalloc<int32_t>(c, n);
+  b.sp = (c->nspec > 1) ? dalloc<int32_t>(c, n) : nullptr;
+  return b.a && b.b && b.id && (c->nspec == 1 || b.sp);
+}
+
+static bool alloc_xlist(mpcd_gpu_ctx *c, XList &L, int cap) {
+  L.a = dalloc<float4>(c, cap);
+  L.b = dalloc<float2>(c, cap);
+  L.m = dalloc<int4>(c, cap);
+  L.cnt = dalloc<int32_t>(c, 1);
+  L.cap = cap;
+  return L.a && L.b && L.m && L.cnt;
+}
+
 static bool alloc_bstate(mpcd_gpu_ctx *c, BState<float> &b, size_t n) {
   void *old[] = {b.a, b.b, b.c, b.sp};
   for (void *p : old) dfree(c, p);
@@ -1498,15 +1553,21 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
         !c->bstart[1])
       return fail(c, MPCD_ENOMEM, "bucket tables");
     if (!alloc_scan(c)) return fail(c, MPCD_ENOMEM, "scan scratch");
-    // the wall list holds at most every particle once per step
-    if (!(c->precision == MPCD_FP64 ? alloc_wall_list<double>(c, c->wld) : alloc_wall_list<float>(c, c->wlf)))
-      return fail(c, MPCD_ENOMEM, "wall list");
     // escape lists: a step with more escapes finishes on the cell-sorted path
     c->esc_cap = std::max(4096, c->ntot / 8);
-    c->esc_idx[0] = dalloc<int32_t>(c, c->esc_cap);
-    c->esc_idx[1] = dalloc<int32_t>(c, c->esc_cap);
-    c->esc_cnt = dalloc<int32_t>(c, 2);
-    if (!c->esc_idx[0] || !c->esc_idx[1] || !c->esc_cnt) return fail(c, MPCD_ENOMEM, "escape lists");
+    if (c->exact) {
+      // the wall list holds at most every particle once per step
+      if (!alloc_xlist(c, c->xwl, c->ntot) || !alloc_xlist(c, c->xesc[0], c->esc_cap) ||
+          !alloc_xlist(c, c->xesc[1], c->esc_cap))
+        return fail(c, MPCD_ENOMEM, "wall and escape lists");
+    } else {
+      if (!(c->precision == MPCD_FP64 ? alloc_wall_list<double>(c, c->wld) : alloc_wall_list<float>(c, c->wlf)))
+        return fail(c, MPCD_ENOMEM, "wall list");
+      c->esc_idx[0] = dalloc<int32_t>(c, c->esc_cap);
+      c->esc_idx[1] = dalloc<int32_t>(c, c->esc_cap);
+      c->esc_cnt = dalloc<int32_t>(c, 2);
+      if (!c->esc_idx[0] || !c->esc_idx[1] || !c->esc_cnt) return fail(c, MPCD_ENOMEM, "escape lists");
+    }
   }
   const int cap = bkt_cap(fused_op(c->rtech));
   const size_t slots = (size_t)c->nbkt * cap;
@@ -1514,7 +1575,9 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     bool ok = true;
     for (int k = 0; k < 2; k++)
-      ok = ok && (c->precision == MPCD_FP64 ? alloc_bstate(c, c->bd[k], slots) : alloc_bstate(c, c->bf[k], slots));
+      ok = ok && (c->exact                       ? alloc_xstate(c, c->bx[k], slots)
+                  : c->precision == MPCD_FP64 ? alloc_bstate(c, c->bd[k], slots)
+                                              : alloc_bstate(c, c->bf[k], slots));
     if (!ok) {
       c->bkt_slots = 0;
       return fail(c, MPCD_ENOMEM, "bucketed state (%zu slots)", slots);
@@ -1534,6 +1597,7 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
   for (int k = 0; k < 2; k++) {
     c->bd[k].cnt = c->bcnt[k];
     c->bf[k].cnt = c->bcnt[k];
+    c->bx[k].cnt = c->bcnt[k];
   }
   return MPCD_OK;
 }
@@ -1587,9 +1651,197 @@ static void dispatch_bucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int p
   }
 }
 
+// ---------------------------------------------------------------------------
+// the exact-sum FP32 form (mpcd_exact.h): bucketed steps and dense steps
+// ---------------------------------------------------------------------------
+template <int OP, bool ONESP, int DIM, bool GHOST>
+static void launch_xbucket_g(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  const int nwm = std::max(1, c->nwtot);
+  k_xbucket<OP, ONESP, DIM, GHOST><<<c->nbkt, kBktThreads, 0, c->stream>>>(
+      C, c->bx[c->bcur], c->bx[1 - c->bcur], c->d_bkts, c->d_brep, c->d_rep, c->d_sp, c->wh2 + par * nwm,
+      c->box2 + par * 6 * c->nrep, D, c->shift2 + par * 3 * c->nrep, c->shift2 + (par ^ 1) * 3 * c->nrep,
+      bkt_limit(c), c->xwl, c->xesc[par ^ 1], c->nspec);
+}
+
+template <int OP, bool ONESP, int DIM>
+static void launch_xbucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  if (c->any_phantom) launch_xbucket_g<OP, ONESP, DIM, true>(c, C, D, par);
+  else launch_xbucket_g<OP, ONESP, DIM, false>(c, C, D, par);
+}
+
+template <int OP, bool ONESP>
+static void launch_xbucket_dim(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  if (c->dim == 3) launch_xbucket<OP, ONESP, 3>(c, C, D, par);
+  else launch_xbucket<OP, ONESP, 2>(c, C, D, par);
+}
+
+static void dispatch_xbucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  const bool one = c->nspec == 1;
+  const int op = fused_op(c->rtech);
+  if (op == OP_SRD) {
+    if (one) launch_xbucket_dim<OP_SRD, true>(c, C, D, par);
+    else launch_xbucket_dim<OP_SRD, false>(c, C, D, par);
+  } else if (op == OP_AT) {
+    if (one) launch_xbucket_dim<OP_AT, true>(c, C, D, par);
+    else launch_xbucket_dim<OP_AT, false>(c, C, D, par);
+  } else {
+    if (one) launch_xbucket_dim<OP_RAT, true>(c, C, D, par);
+    else launch_xbucket_dim<OP_RAT, false>(c, C, D, par);
+  }
+}
+
+// does any particle of the context ever take the wall list?  Not when every
+// replica's walls are its box's periodic faces (they wrap in the step kernel)
+static bool x_needs_walls(const mpcd_gpu_ctx *c) {
+  for (const RepDev &R : c->reps)
+    if (!R.xwrap) return true;
+  return false;
+}
+
+static void run_step_xbucket(mpcd_gpu_ctx *c) {
+  const int par = c->step & 1, p1 = par ^ 1;
+  Ctx C = view(c, par);
+  Draws D = make_draws(c, 0, c->step);
+  const int cap = bkt_cap(fused_op(c->rtech)), one = c->nspec == 1 ? 1 : 0;
+  const int nw3 = 3 * std::max(1, c->nwtot), nwm = std::max(1, c->nwtot);
+  {
+    // zeroed output counts and lists, the next step's shift, this step's escapes
+    ProfScope ps(c, F_SCAN);
+    const int nbz = nblocks(c->nbkt, kBlock), nesc = 4;
+    Draws D1 = make_draws(c, 0, c->step + 1);
+    k_xprep<<<nbz + 1 + nesc, kBlock, 0, c->stream>>>(
+        C, D1, c->wqb2 + par * nw3, c->shift2 + p1 * 3 * c->nrep, c->wqs2 + p1 * nw3, c->wqb2 + p1 * nw3,
+        c->wh2 + p1 * nwm, c->box2 + p1 * 6 * c->nrep, c->bcnt[1 - c->bcur], c->nbkt, nbz, c->xwl, c->xesc[par],
+        c->xesc[p1], D, c->bx[c->bcur], c->d_brep, cap, bkt_limit(c), one, nesc);
+  }
+  {
+    ProfScope ps(c, F_BUCKET);
+    dispatch_xbucket(c, C, D, par);
+  }
+  if (x_needs_walls(c)) {
+    ProfScope ps(c, F_STREAM);
+    const int nb = 512;
+    if (c->dim == 3)
+      k_xwalls<3><<<nb, kBlock, 0, c->stream>>>(C, c->xwl, c->bx[1 - c->bcur], c->xesc[p1], c->d_brep, c->d_rep,
+                                                c->d_sp, c->d_wall, c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
+                                                c->wqb2 + par * nw3, c->shift2 + p1 * 3 * c->nrep, cap, bkt_limit(c),
+                                                one);
+    else
+      k_xwalls<2><<<nb, kBlock, 0, c->stream>>>(C, c->xwl, c->bx[1 - c->bcur], c->xesc[p1], c->d_brep, c->d_rep,
+                                                c->d_sp, c->d_wall, c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
+                                                c->wqb2 + par * nw3, c->shift2 + p1 * 3 * c->nrep, cap, bkt_limit(c),
+                                                one);
+  }
+  c->bcur ^= 1;
+  c->step++;
+}
+
+// dense (particle-order) state -> bucketed, at a step whose shift is drawn here
+static mpcd_status xbucket_enter(mpcd_gpu_ctx *c) {
+  mpcd_status st = ensure_buckets(c);
+  if (st) return st;
+  const int par = c->step & 1;
+  launch_shift(c, c->step, 0);
+  Ctx C = view(c, par);
+  k_fill<<<nblocks(c->nbkt + 1, kBlock), kBlock, 0, c->stream>>>(c->bcnt[0], 0, c->nbkt + 1);
+  k_fill<<<nblocks(c->nbkt + 1, kBlock), kBlock, 0, c->stream>>>(c->bcnt[1], 0, c->nbkt + 1);
+  k_fill<<<1, 1, 0, c->stream>>>(c->C.flags + 1, 0x7FFFFFFF, 1);
+  k_fill<<<1, 1, 0, c->stream>>>(c->xesc[par].cnt, 0, 1);
+  k_xbucketize<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C, c->sf[c->cur], c->bx[0], c->xesc[par],
+                                                                    c->d_brep, bkt_cap(fused_op(c->rtech)),
+                                                                    bkt_limit(c), c->nspec == 1 ? 1 : 0);
+  c->bcur = 0;
+  c->bkt_step0 = c->step;
+  return check_launch(c);
+}
+
+// bucketed -> dense (particle order) at step s from buffer bsrc; the binned
+// structures of the cell-sorted layout are rebuilt when something reads them
+static void xbucket_densify(mpcd_gpu_ctx *c, uint32_t s, int bsrc) {
+  Ctx C = view(c, s & 1);
+  k_xdensify<<<c->nbkt, kBlock, 0, c->stream>>>(C, c->bx[bsrc], c->xesc[s & 1], c->sf[c->cur], c->d_bkts,
+                                                bkt_cap(fused_op(c->rtech)), bkt_limit(c), c->nspec == 1 ? 1 : 0);
+  c->pending = false;
+  c->hist_ready = false;
+  c->need_init = true;
+}
+
+static mpcd_status ensure_xdense(mpcd_gpu_ctx *c) {
+  if (c->xd.n) return MPCD_OK;
+  const size_t nc = c->nctot, ns = xnsum(fused_op(c->rtech));
+  c->xd.n = dalloc<int32_t>(c, nc);
+  c->xd.nsp = (c->nspec > 1) ? dalloc<int32_t>(c, nc * MPCD_MAXSPEC) : nullptr;
+  c->xd.sum = dalloc<double>(c, nc * ns);
+  c->xd.cd = dalloc<float>(c, nc * 12);
+  c->xd.cell = dalloc<int32_t>(c, c->ntot);
+  c->xd.rt = (fused_op(c->rtech) != OP_SRD) ? dalloc<float>(c, 3 * (size_t)c->ntot) : nullptr;
+  if (!c->xd.n || !c->xd.sum || !c->xd.cd || !c->xd.cell || (c->nspec > 1 && !c->xd.nsp) ||
+      (fused_op(c->rtech) != OP_SRD && !c->xd.rt)) {
+    c->xd.n = nullptr;
+    return fail(c, MPCD_ENOMEM, "cell sums of the exact form");
+  }
+  return MPCD_OK;
+}
+
+template <int OP, bool ONESP, int DIM>
+static void launch_xdense(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  State<float> S = c->sf[c->cur];
+  const int npb = (int)c->pblk.size(), ncb = (int)c->cblk.size();
+  k_xd_sums<OP, ONESP, DIM><<<npb, kBlock, 0, c->stream>>>(C, S, c->xd, c->d_pblk, c->d_sp, D);
+  if (c->any_phantom)
+    k_xd_cells<OP, ONESP, DIM, true><<<ncb, kBlock, 0, c->stream>>>(C, c->xd, c->d_cblk, c->d_sp, D, c->nspec);
+  else
+    k_xd_cells<OP, ONESP, DIM, false><<<ncb, kBlock, 0, c->stream>>>(C, c->xd, c->d_cblk, c->d_sp, D, c->nspec);
+  k_xd_part<OP, ONESP, DIM><<<npb, kBlock, 0, c->stream>>>(C, S, c->xd, c->d_pblk, c->d_sp, c->d_wall,
+                                                           c->box2 + par * 6 * c->nrep, D);
+}
+
+template <int OP, bool ONESP>
+static void launch_xdense_dim(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+  if (c->dim == 3) launch_xdense<OP, ONESP, 3>(c, C, D, par);
+  else launch_xdense<OP, ONESP, 2>(c, C, D, par);
+}
+
+// one step of the exact form on the dense state (any particle order)
+static void run_step_xdense(mpcd_gpu_ctx *c) {
+  if (ensure_xdense(c)) return;
+  const int par = c->step & 1;
+  launch_shift(c, c->step, 0);
+  Ctx C = view(c, par);
+  Draws D = make_draws(c, 0, c->step);
+  const size_t nc = c->nctot, ns = xnsum(fused_op(c->rtech));
+  k_fill<<<nblocks((int)nc, kBlock), kBlock, 0, c->stream>>>(c->xd.n, 0, (int)nc);
+  if (c->xd.nsp)
+    k_fill<<<nblocks((int)(nc * MPCD_MAXSPEC), kBlock), kBlock, 0, c->stream>>>(c->xd.nsp, 0,
+                                                                                 (int)(nc * MPCD_MAXSPEC));
+  // zero doubles: all-zero words
+  k_fill<<<nblocks((int)(2 * nc * ns), kBlock), kBlock, 0, c->stream>>>((int32_t *)c->xd.sum, 0, (int)(2 * nc * ns));
+  const bool one = c->nspec == 1;
+  const int op = fused_op(c->rtech);
+  if (op == OP_SRD) {
+    if (one) launch_xdense_dim<OP_SRD, true>(c, C, D, par);
+    else launch_xdense_dim<OP_SRD, false>(c, C, D, par);
+  } else if (op == OP_AT) {
+    if (one) launch_xdense_dim<OP_AT, true>(c, C, D, par);
+    else launch_xdense_dim<OP_AT, false>(c, C, D, par);
+  } else {
+    if (one) launch_xdense_dim<OP_RAT, true>(c, C, D, par);
+    else launch_xdense_dim<OP_RAT, false>(c, C, D, par);
+  }
+  k_xtemp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(C, c->d_pblk_rng);
+  c->pending = false;
+  c->hist_ready = false;
+  c->need_init = true;
+  c->step++;
+}
+
 // one step in the bucketed layout (internal generator)
 template <typename Real>
 static void run_step_bucket(mpcd_gpu_ctx *c) {
+  if (c->exact) {
+    run_step_xbucket(c);
+    return;
+  }
   const int par = c->step & 1;
   Ctx C = view(c, par);
   Draws D = make_draws(c, 0, c->step);
@@ -1659,6 +1911,7 @@ static void run_step_bucket(mpcd_gpu_ctx *c) {
 // dense -> bucketed, from a dense state whose next shifted bin is built
 template <typename Real>
 static mpcd_status bucket_enter(mpcd_gpu_ctx *c) {
+  if (c->exact) return xbucket_enter(c);
   mpcd_status st = ensure_buckets(c);
   if (st) return st;
   State<Real> *S = states<Real>(c);
@@ -1679,6 +1932,10 @@ static mpcd_status bucket_enter(mpcd_gpu_ctx *c) {
 // step s's particles), with the pending shifted bin of step s
 template <typename Real>
 static void bucket_densify(mpcd_gpu_ctx *c, uint32_t s, int bsrc) {
+  if (c->exact) {
+    xbucket_densify(c, s, bsrc);
+    return;
+  }
   Ctx C = view(c, s & 1);
   State<Real> *S = states<Real>(c);
   BState<Real> *B = bstates<Real>(c);
@@ -1707,7 +1964,8 @@ static mpcd_status bucket_finish(mpcd_gpu_ctx *c, uint32_t run_step0, int *done)
   const uint32_t s0 = c->bkt_step0;
   if (c->step != run_step0) {
     ProfScope ps(c, F_TEMP);
-    k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(view(c, (c->step - 1) & 1), c->d_bkt_rng);
+    if (c->exact) k_xtemp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(view(c, (c->step - 1) & 1), c->d_bkt_rng);
+    else k_temp_reduce<<<c->nrep, kTempThreads, 0, c->stream>>>(view(c, (c->step - 1) & 1), c->d_bkt_rng);
   }
   int fl[2];
   HIPCHK(c, hipMemcpyAsync(fl, c->C.flags, sizeof fl, hipMemcpyDeviceToHost, c->stream));
@@ -1836,8 +2094,9 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
     else run_step<float>(c, 0);
   };
   if (bucket_usable(c) && nsteps > 0) {
-    // the bucketed layout starts from a cell-sorted state whose next bin is built
-    while (!c->in_bucket && nsteps > 0 && !c->hist_ready) {
+    // the bucketed layout starts from a cell-sorted state whose next bin is
+    // built (the exact form bins the dense state itself)
+    while (!c->exact && !c->in_bucket && nsteps > 0 && !c->hist_ready) {
       eager();
       nsteps--;
     }
@@ -1853,7 +2112,7 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
     st = ensure_dense(c);
     if (st) return st;
   }
-  if (use_graph && c->fusable && !c->recording && !c->tan_on) {
+  if (use_graph && c->fusable && !c->exact && !c->recording && !c->tan_on) {
     // graphs start from an even step whose histogram the previous step built
     while (nsteps > 0 && (!c->hist_ready || (c->step & 1))) {
       eager();
@@ -1892,7 +2151,7 @@ static mpcd_status autotune_buckets(mpcd_gpu_ctx *c, int steps, double *ms_out) 
     // when the next bin is not built yet
     build_buckets(c, f);
     int untimed = 0;
-    if (!c->hist_ready) run_step<Real>(c, 0);
+    if (!c->exact && !c->hist_ready) run_step<Real>(c, 0);
     else untimed = 1;
     st = bucket_enter<Real>(c);
     if (st) break;
@@ -1929,9 +2188,13 @@ extern "C" mpcd_status mpcd_gpu_autotune(mpcd_gpu_ctx *c, int steps_per_trial, d
   if (st) return st;
   st = ensure_dense(c);
   if (st) return st;
-  if (!c->fusable) {
+  if (!c->fusable && !c->exact) {
     if (ms_per_step_out) *ms_per_step_out = 0.0;
     return MPCD_OK;  // the split path has no tile parameter
+  }
+  if (c->exact && !bucket_usable(c)) {
+    if (ms_per_step_out) *ms_per_step_out = 0.0;
+    return MPCD_OK;  // dense steps of the exact form have no tile parameter
   }
   if (bucket_usable(c)) {
     st = c->precision == MPCD_FP64 ? autotune_buckets<double>(c, steps_per_trial, ms_per_step_out)
@@ -2633,6 +2896,7 @@ extern "C" mpcd_status mpcd_gpu_get_cells(mpcd_gpu_ctx *c, int r, int32_t *pop, 
 
 extern "C" mpcd_status mpcd_gpu_flow(mpcd_gpu_ctx *c, int on) {
   if (!c) return MPCD_EINVAL;
+  if (c->exact && on) return fail(c, MPCD_EUNSUPPORTED, "the flow field is not summed in the exact-sum FP32 form");
   if ((on != 0) == c->flow_on) return MPCD_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (on && !c->d_flow) {

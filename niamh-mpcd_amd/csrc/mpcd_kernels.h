@@ -40,6 +40,7 @@ struct RepDev {
   int wall_off, nwall;
   int rtech, lc, tstech, galinv, true_sign, gravflag, anyphantom;
   int anycurved;  // a wall that is not an axis-aligned plane (mpcd_walls.h)
+  int xwrap;      // exact-sum FP32 form: the walls are the box's periodic faces (mpcd_exact.h)
   double dt, kbt, cosA, sinA, tau, grav[3], avv[3], gndnst;
 };
 
@@ -1404,18 +1405,12 @@ __device__ double collide_cell(const Ctx &C, const State<Real> &S, int rep, int 
             M[2][0] = L[0] * L[2] * (1. - Cc) - L[1] * Sn;
             M[2][1] = L[1] * L[2] * (1. - Cc) + L[0] * Sn;
             M[2][2] = L[2] * L[2] + (1. - L[2] * L[2]) * Cc;
-          } else if (CA == 0) {
-            M[0][0] = 1.; M[0][1] = 0.; M[0][2] = 0.;
-            M[1][0] = 0.; M[1][1] = Cc; M[1][2] = Sn;
-            M[2][0] = 0.; M[2][1] = -.1 * Sn; M[2][2] = Cc;  // mpc.c:1215 as written
-          } else if (CA == 1) {
-            M[0][0] = Cc; M[0][1] = 0.; M[0][2] = Sn;
-            M[1][0] = 0.; M[1][1] = 1.; M[1][2] = 0.;
-            M[2][0] = -1. * Sn; M[2][1] = 0.; M[2][2] = Cc;
           } else {
-            M[0][0] = Cc; M[0][1] = Sn; M[0][2] = 0.;
-            M[1][0] = -1. * Sn; M[1][1] = Cc; M[1][2] = 0.;
-            M[2][0] = 0.; M[2][1] = 0.; M[2][2] = 1.;
+            // mpc.c:1206-1237, the x-axis entry -.1 S as written (mpc.c:1215)
+            double M9[9];
+            orth_matrix(CA, Cc, Sn, M9);
+            for (int i = 0; i < 3; i++)
+              for (int j = 0; j < 3; j++) M[i][j] = M9[3 * i + j];
           }
         } else {
           M[0][0] = Cc; M[0][1] = -1. * Sn;
@@ -2150,26 +2145,19 @@ __device__ __forceinline__ void stream_particle(const Ctx &C, const State<Real> 
     for (int i = 0; i < 3; i++) wmul[3 * t + i] = mul[i];
 }
 
-// streaming with the general wall engine (mpcd_walls.h): MPC_BCcollision
-// (bc.c:1363-1496) with chooseBC over every wall the species meets
-// (bc.c:1252-1328: planes by calcW_PLANE and the plane crossing time, wavy
-// planes and curved surfaces by calcW, crosstimeReverse and chooseT, the
+// the general wall engine (mpcd_walls.h) on a drifted particle (Q after the
+// drift; pid: its id, the key of its re-thermalisation draws):
+// MPC_BCcollision (bc.c:1363-1496) with chooseBC over every wall the species
+// meets (bc.c:1252-1328: planes by calcW_PLANE and the plane crossing time,
+// wavy planes and curved surfaces by calcW, crosstimeReverse and chooseT, the
 // curved ones at their periodic image nearest the particle), the bounce at
 // the surface normal of the crossing point, and the NBOUNCE rewinds and
 // re-thermalisations of the planar path.  FP64 arithmetic; the derivative
 // modes are refused for contexts with such walls.
-template <typename Real>
-__device__ __forceinline__ void stream_particle_walls(const Ctx &C, const State<Real> &S, int rep, int t,
-                                                      const Draws &Dr) {
-  const RepDev &R = C.rep[rep];
+__device__ __forceinline__ void general_wall_loop(const Ctx &C, const RepDev &R, int rep, int spid, uint32_t pid,
+                                                  const Draws &Dr, bool lc, double Q[3], double V[3], double U[3]) {
   const int dim = R.dim;
-  double Q[3], V[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]}, U[3] = {0., 0., 0.};
-  load_q(S, t, Q);
-  const bool lc = R.lc != MPCD_ISOF;
-  if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
-  const int spid = S.sp[t];
   const mpcd_gpu_species &sp = C.sp[spid];
-  for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
   const double t_step = R.dt;
   double t_left = t_step, t_coll = 0.0;
   int chosen = 0, cnt = 0, cnt2 = 0, flagCNT = 0, rk = 0;
@@ -2266,7 +2254,6 @@ __device__ __forceinline__ void stream_particle_walls(const Ctx &C, const State<
           atomicOr(&C.flags[0], kFlagPushes);
           break;
         }
-        const uint32_t pid = (uint32_t)S.id[t];
         push_velocity(sp, R.kbt, dim, Dr, rep, (size_t)R.part_off + pid, pid, rk, V);
         rk += 3;
         t_left = t_step;
@@ -2281,6 +2268,23 @@ __device__ __forceinline__ void stream_particle_walls(const Ctx &C, const State<
     }
   }
   if (flagCNT) atomicAdd(&C.events[1], 1ull);
+}
+
+// streaming with the general wall engine, the unshifted bin and its escape
+// repair, for one slot of the split path
+template <typename Real>
+__device__ __forceinline__ void stream_particle_walls(const Ctx &C, const State<Real> &S, int rep, int t,
+                                                      const Draws &Dr) {
+  const RepDev &R = C.rep[rep];
+  const int dim = R.dim;
+  double Q[3], V[3] = {(double)S.vx[t], (double)S.vy[t], (double)S.vz[t]}, U[3] = {0., 0., 0.};
+  load_q(S, t, Q);
+  const bool lc = R.lc != MPCD_ISOF;
+  if (lc) { U[0] = (double)S.ux[t]; U[1] = (double)S.uy[t]; U[2] = (double)S.uz[t]; }
+  const int spid = S.sp[t];
+  const mpcd_gpu_species &sp = C.sp[spid];
+  for (int i = 0; i < dim; i++) Q[i] = Q[i] + R.dt * V[i];
+  general_wall_loop(C, R, rep, spid, (uint32_t)S.id[t], Dr, lc, Q, V, U);
   // the unshifted bin (mpc.c:4936) with bin()'s escape repair (mpc.c:970-982)
   int cu = cell_of(R, Q, 0);
   if (cu < 0) {

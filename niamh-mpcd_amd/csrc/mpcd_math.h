@@ -111,6 +111,24 @@ MPCD_HD F det3(const F m[3][3]) {                                      // mtools
   return (c1 - c2 + c3);
 }
 
+// the ORTHAXIS rotation about axis CA (0 x, 1 y, 2 z) by the angle whose
+// cosine and sine are Cc and Sn (rotate, mpc.c:1206-1237, with the x-axis
+// entry R[0][2][1] = -.1 S as written there).  The entries are selected, not
+// branched on: an if-else chain over CA in a cell loop was compiled to code
+// that left the z-axis matrix's entry [1][0] at zero on gfx950.
+MPCD_HD void orth_matrix(int CA, double Cc, double Sn, double M[9]) {
+  const bool a0 = CA == 0, a1 = CA == 1, a2 = !a0 && !a1;
+  M[0] = a0 ? 1. : Cc;
+  M[1] = a2 ? Sn : 0.;
+  M[2] = a1 ? Sn : 0.;
+  M[3] = a2 ? -1. * Sn : 0.;
+  M[4] = a1 ? 1. : Cc;
+  M[5] = a0 ? Sn : 0.;
+  M[6] = a1 ? -1. * Sn : 0.;
+  M[7] = a0 ? -.1 * Sn : 0.;
+  M[8] = a2 ? 1. : Cc;
+}
+
 // invert3x3 via cofactors, mtools.c:972-1037
 template <typename F>
 MPCD_HD void inv3(F inv[3][3], const F m[3][3]) {

@@ -1,4 +1,5 @@
-"""Physics checks of the FP32-state form against the FP64 path.
+"""Physics checks of the FP32-state forms (MPCD_FP32 and the exact-sum
+MPCD_FP32_EXACT) against the FP64 path.
 
 Both precisions run the same replicas from the same initial states with the
 same counter-based draws (the generator is keyed by seed, replica, step,
@@ -193,6 +194,7 @@ def fp32_report(M, quick: bool = True) -> dict:
         dict(L=(64, 64, 64), nrep=96, steps=1200, every=40, amp=0.5)
     w64 = shear_wave(M, M.FP64, **kw)
     w32 = shear_wave(M, M.FP32, **kw)
+    w32x = shear_wave(M, M.FP32_EXACT, **kw)
     th = srd_viscosity_3d(10.0, ROT130, 0.1)
     n64 = nematic_order(M, M.FP64)
     n32 = nematic_order(M, M.FP32)
@@ -201,6 +203,7 @@ def fp32_report(M, quick: bool = True) -> dict:
         return abs(a - b) / abs(b)
 
     paired = paired_rel_err(w32, w64)
+    paired_x = paired_rel_err(w32x, w64)
 
     return dict(
         workload=f"3D SRD {kw['L'][0]}x{kw['L'][1]}x{kw['L'][2]} x 10 ppc x {kw['nrep']} replicas, shear wave "
@@ -214,4 +217,10 @@ def fp32_report(M, quick: bool = True) -> dict:
         viscosity_rel_err_stderr=paired[1], viscosity_signed_rel_err=paired[0],
         viscosity_rel_err_stderr_unpaired=math.hypot(w64["nu_err"], w32["nu_err"]) / w64["nu"],
         viscosity_theory=th["nu"], viscosity_fp64_vs_theory=rel(w64["nu"], th["nu"]),
-        S_fp64=n64["S"], S_fp32=n32["S"], S_rel_err=rel(n32["S"], n64["S"]))
+        S_fp64=n64["S"], S_fp32=n32["S"], S_rel_err=rel(n32["S"], n64["S"]),
+        # the exact-sum FP32 form (the batched headline form; SRD, no nematic mode)
+        fp32_exact=dict(temperature=w32x["T"], temperature_rel_err=rel(w32x["T"], w64["T"]),
+                        momentum_drift_per_particle=w32x["momentum_drift"], viscosity=w32x["nu"],
+                        viscosity_stderr=w32x["nu_err"], viscosity_rel_err=rel(w32x["nu"], w64["nu"]),
+                        viscosity_signed_rel_err=paired_x[0], viscosity_rel_err_stderr=paired_x[1],
+                        viscosity_vs_theory=rel(w32x["nu"], th["nu"])))

@@ -198,6 +198,13 @@ static void invert3x3(double inv[3][3], double m[3][3])                /* mtools
         for (j = 0; j < 3; j++) inv[j][i] = cofactor3x3(m, i, j) / det;
 }
 
+void orc_inv3(double inv[3][3], const double m[3][3])
+{
+    double a[3][3];
+    memcpy(a, m, sizeof a);
+    invert3x3(inv, a);
+}
+
 static void matvec3(double M[3][3], const double *v, double *r)        /* mtools.c:130 */
 {
     int i, j;
@@ -385,6 +392,12 @@ static void philox_unitvec(uint64_t seed, uint32_t replica, uint32_t step, int p
     v[0] = s * cos(phi);
     v[1] = s * sin(phi);
     v[2] = z;
+}
+
+void orc_philox_unitvec(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k,
+                        double v[3])
+{
+    philox_unitvec(seed, replica, step, purpose, key, k, v);
 }
 
 static const double *keyed_base(const orc_draws *d, int purpose, int *stride)

@@ -155,6 +155,9 @@ void     orc_eigen(double *m, int dim, double *eigval); /* m row-major dim x dim
 void     orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double   orc_philox_uniform(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k);
 double   orc_philox_normal(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k);
+void     orc_inv3(double inv[3][3], const double m[3][3]);
+void     orc_philox_unitvec(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k,
+                            double v[3]);
 void     orc_rodrigues(double *vec, double *axis, double theta);
 
 #endif

@@ -47,6 +47,14 @@ class Params(C.Structure):
                 ("wall", Wall * MAXWALL)]
 
 
+class XCfg(C.Structure):
+    """one replica of the exact-sum FP32 form (exact_step.c)"""
+    _fields_ = [("dim", C.c_int), ("L", C.c_int * 3), ("rtech", C.c_int), ("true_sign", C.c_int),
+                ("galinv", C.c_int), ("nspec", C.c_int), ("cell_off", C.c_int), ("dt", C.c_double),
+                ("kbt", C.c_double), ("cosA", C.c_double), ("sinA", C.c_double), ("grav", C.c_double * 3),
+                ("avv", C.c_double * 3), ("mass", C.c_double * 10), ("damp", C.c_double * 10)]
+
+
 _lib = None
 
 
@@ -97,6 +105,9 @@ def lib():
         L.orc_draws_rare_overflow.argtypes = [vp]
         L.orc_eigen.argtypes = [dp, C.c_int, dp]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_exact_step.restype = C.c_int
+        L.orc_exact_step.argtypes = [C.POINTER(XCfg), C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), ip, ip,
+                                     C.c_uint64, C.c_uint32, C.c_uint32, dp]
         L.orc_philox_uniform.restype = C.c_double
         L.orc_philox_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32]
         L.orc_philox_normal.restype = C.c_double
@@ -321,3 +332,45 @@ def rodrigues(vec, axis, theta):
     a = np.ascontiguousarray(axis, dtype=np.float64).copy()
     lib().orc_rodrigues(_dp(v), _dp(a), float(theta))
     return v
+
+
+def exact_step(cfg: dict, x, v, sp=None, seed=1, replica=0, step=0, cell_off=0, ids=None):
+    """One step of the exact-sum FP32 form on a box of periodic faces
+    (exact_step.c).  cfg: the keys of the library's mpcd_gpu_params dicts
+    (dim, L, dt, kbt, rotang, grav, avv, rtech, galinv, true_random_sign,
+    species).  x, v: (n, 3) float32 positions and velocities; ids: the keys of
+    the particles' draws (default: their index).  Returns the new x, v
+    (float32) and KBTNOW."""
+    import math
+    p = XCfg()
+    p.dim = int(cfg["dim"])
+    for i in range(3):
+        p.L[i] = int(cfg["L"][i]) if i < p.dim else 1
+        p.grav[i] = float(cfg.get("grav", (0, 0, 0))[i])
+        p.avv[i] = float(cfg.get("avv", (0, 0, 0))[i])
+    p.rtech = int(cfg.get("rtech", 0))
+    p.true_sign = int(cfg.get("true_random_sign", 0))
+    p.galinv = int(cfg.get("galinv", 1))
+    sps = cfg["species"]
+    p.nspec = len(sps)
+    p.cell_off = int(cell_off)
+    p.dt = float(cfg["dt"])
+    p.kbt = float(cfg["kbt"])
+    p.cosA = math.cos(float(cfg.get("rotang", 0.0)))
+    p.sinA = math.sin(float(cfg.get("rotang", 0.0)))
+    for k, s_ in enumerate(sps):
+        p.mass[k] = float(s_["mass"])
+        p.damp[k] = float(s_.get("damp", 0.0))
+    x = np.ascontiguousarray(x, dtype=np.float32).copy()
+    v = np.ascontiguousarray(v, dtype=np.float32).copy()
+    spa = None if sp is None else np.ascontiguousarray(sp, dtype=np.int32)
+    ida = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+    kbt = C.c_double()
+    fp = C.POINTER(C.c_float)
+    bad = lib().orc_exact_step(C.byref(p), int(x.shape[0]), x.ctypes.data_as(fp), v.ctypes.data_as(fp),
+                               None if spa is None else spa.ctypes.data_as(C.POINTER(C.c_int)),
+                               None if ida is None else ida.ctypes.data_as(C.POINTER(C.c_int)), int(seed),
+                               int(replica), int(step), C.byref(kbt))
+    if bad:
+        raise RuntimeError(f"exact_step: {bad} particles outside the box")
+    return x, v, kbt.value

@@ -1,0 +1,143 @@
+"""CPU: the oracle's restatement of the exact-sum FP32 form (oracle/exact_step.c).
+
+What pins it:
+  * order independence, the property the form is built on: the particles
+    shuffled, stepped and put back give the same bits (positions,
+    velocities, KBTNOW) for SRD, MPC-AT and MPC-AT+a with two species;
+  * one SRD step agrees with the FP64 oracle's step (pinned to the
+    reference's own fixtures) from the same state with the same counter-based
+    draws, to FP32 precision;
+  * SRD conserves the momentum of every cell and its kinetic energy to FP32
+    rounding; MPC-AT+a conserves momentum;
+  * with the rotation angle 0 an SRD step is a pure drift with the periodic
+    wrap, computed independently here in float32 numpy.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import pyoracle as P  # noqa: E402
+
+
+def box_walls(L):
+    ws = []
+    for ax in range(len(L)):
+        a1 = [0.0, 0.0, 0.0]
+        a1[ax] = 1.0
+        a2 = [0.0, 0.0, 0.0]
+        a2[ax] = -1.0
+        ws += [dict(A=a1, R=0.0, DN=float(L[ax])), dict(A=a2, R=-float(L[ax]), DN=float(L[ax]))]
+    return ws
+
+
+def cfg(dim=3, L=(6, 5, 4), rtech=0, nspec=1, grav=(0.0, 0.0, 0.0), rotang=2.268928, dt=0.1):
+    sp = [dict(mass=1.0 + 0.5 * k, rfc=0.01, tumble=2.0, chihi=0.5, act=0.0, mfpot=10.0, damp=0.0, sigwidth=1.0,
+               sigpos=1.0, minactratio=0.0, vdist=4) for k in range(nspec)]
+    L = list(L[:dim]) + [1] * (3 - dim)
+    return dict(dim=dim, L=L, dt=dt, kbt=1.0, rotang=rotang, tau=0.5, grav=list(grav), avv=[0.0, 0.0, 0.0],
+                gndnst=10.0, rtech=rtech, lc=0, tstech=0, galinv=1, species=sp, walls=box_walls(L[:dim]))
+
+
+def state(c, ppc=10, seed=3):
+    rng = np.random.default_rng(seed)
+    dim = c["dim"]
+    n = int(np.prod(c["L"][:dim])) * ppc
+    x = np.zeros((n, 3), np.float32)
+    v = np.zeros((n, 3), np.float32)
+    for d in range(dim):
+        x[:, d] = rng.uniform(0.0, c["L"][d], n).astype(np.float32)
+        v[:, d] = rng.normal(0.0, 1.0, n).astype(np.float32)
+    sp = rng.integers(0, len(c["species"]), n).astype(np.int32)
+    return x, v, sp
+
+
+CASES = {
+    "srd3d_arb": dict(),
+    "srd3d_orth": dict(rtech=1),
+    "srd2d": dict(dim=2, L=(9, 7)),
+    "at3d_grav": dict(rtech=2, grav=(1e-3, 0.0, 0.0)),
+    "rat3d_2sp": dict(rtech=3, nspec=2),
+    "rat2d": dict(dim=2, L=(9, 7), rtech=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_exact_step_is_order_independent(name):
+    c = cfg(**CASES[name])
+    x, v, sp = state(c)
+    ids = np.arange(x.shape[0], dtype=np.int32)
+    x1, v1, k1 = P.exact_step(c, x, v, sp, seed=11, step=5, ids=ids)
+    perm = np.random.default_rng(1).permutation(x.shape[0])
+    x2, v2, k2 = P.exact_step(c, x[perm], v[perm], sp[perm], seed=11, step=5, ids=ids[perm])
+    inv = np.argsort(perm)
+    assert np.array_equal(x2[inv], x1) and np.array_equal(v2[inv], v1), name
+    assert k1 == k2
+    assert abs(k1 - 1.0) < 0.2
+
+
+def test_srd_step_matches_fp64_oracle():
+    """One ARBAXIS SRD step: the exact FP32 form vs the FP64 oracle's Philox
+    mode (same shift and cell axes), from the same FP32-representable state"""
+    c = cfg(L=(8, 7, 6))
+    x, v, _ = state(c, seed=5)
+    n = x.shape[0]
+    sysd = P.System(c, x.astype(np.float64), v.astype(np.float64), np.zeros((n, 3)), np.zeros(n, np.int32))
+    sysd.step(P.Draws(P.DRAW_PHILOX, 21, 0))
+    Q, V, _ = sysd.state()
+    x1, v1, kbt = P.exact_step(c, x, v, seed=21, step=0)
+    dq = np.abs(x1 - Q)
+    dq = np.minimum(dq, np.abs(dq - np.array(c["L"], np.float32)))  # the same point across the periodic wrap
+    dv = np.abs(v1 - V)
+    # a particle within FP32 rounding of a cell face can bin differently
+    ok = (dq.max(axis=1) < 2e-5) & (dv.max(axis=1) < 2e-5)
+    assert ok.mean() > 0.995, (ok.mean(), dq.max(), dv.max())
+    assert abs(kbt - sysd.kbtnow()) < 1e-5 * max(1.0, kbt)
+
+
+def test_srd_conserves_cell_momentum_and_energy():
+    c = cfg(L=(6, 6, 6), rotang=1.3)
+    x, v, _ = state(c, seed=8)
+    x1, v1, _ = P.exact_step(c, x, v, seed=2, step=1)
+    # total momentum and kinetic energy (SRD rotates relative velocities)
+    p0, p1 = v.astype(np.float64).sum(axis=0), v1.astype(np.float64).sum(axis=0)
+    e0, e1 = (v.astype(np.float64) ** 2).sum(), (v1.astype(np.float64) ** 2).sum()
+    assert np.abs(p1 - p0).max() < 1e-6 * x.shape[0] ** 0.5 * 10
+    assert abs(e1 - e0) / e0 < 1e-5
+    # the velocities changed (collisions happened)
+    assert np.abs(v1 - v).max() > 0.1
+
+
+def test_rat_conserves_momentum():
+    c = cfg(rtech=3, nspec=2)
+    x, v, sp = state(c, seed=9)
+    m = np.array([s["mass"] for s in c["species"]])[sp][:, None]
+    x1, v1, _ = P.exact_step(c, x, v, sp, seed=4, step=2)
+    p0 = (m * v.astype(np.float64)).sum(axis=0)
+    p1 = (m * v1.astype(np.float64)).sum(axis=0)
+    assert np.abs(p1 - p0).max() < 1e-4
+
+
+def test_zero_angle_is_a_pure_drift():
+    """rotation angle 0 (cos 1, sin 0): SRD leaves v to rounding, and the
+    positions are the grid shift, the shift back and the drift with the
+    periodic wrap, in float32"""
+    c = cfg(dim=2, L=(9, 7), rotang=0.0)
+    x, v, _ = state(c, seed=12)
+    x1, v1, _ = P.exact_step(c, x, v, seed=6, step=3)
+    s = np.array([P.lib().orc_philox_uniform(6, 0, 3, P.P_SHIFT, 0, j) for j in range(2)]).astype(np.float32)
+    L = np.array(c["L"][:2], np.float32)
+    q = x[:, :2].copy()
+    t = q + s
+    q = np.where((t > L) & (t <= L + s), t - L, t).astype(np.float32)
+    t = q + (-s)
+    q = np.where((t < 0) & (t >= -s), t + L, t).astype(np.float32)
+    t = (q + np.float32(0.1) * v1[:, :2]).astype(np.float32)
+    q = np.where(t < 0, t + L, np.where(t >= L, t - L, t)).astype(np.float32)
+    assert np.abs(v1 - v).max() <= 4e-7 * max(1.0, float(np.abs(v).max()))
+    assert np.array_equal(x1[:, :2], q)
+    assert math.isfinite(float(x1.sum()))
