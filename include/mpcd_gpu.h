@@ -264,8 +264,8 @@ mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *ctx);
 
 /* Batched-form tuning (no counterpart in the reference, which has no tiles):
  * runs steps_per_trial real steps (plus one untimed step) for each candidate
- * fill of the layout mpcd_gpu_run uses (bucket fill of the bucketed layout,
- * else tile fill of the fused kernel), keeps the fastest and returns its ms
+ * fill of the layout mpcd_gpu_run uses (six bucket fills, 0.4 to 0.8, of the
+ * bucketed layout, else five tile fills of the fused kernel), keeps the fastest and returns its ms
  * per step (0 when the context uses the split path).  The trial steps advance
  * the simulation like mpcd_gpu_run.  Invalidates a graph recorded by
  * mpcd_gpu_run. */

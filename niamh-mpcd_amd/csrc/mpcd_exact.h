@@ -40,6 +40,12 @@
 // with the reference's escape repair and then binned in the cell of their
 // new position (the reference keeps them in their old list for that step).
 // Thermostats, nematic modes and injected draws are outside this form.
+//
+// A bucketed step is one launch of k_xbucket, plus k_xwalls when a replica
+// has walls other than its box's periodic faces.  The step kernel also
+// clears the bucket counts of the step after next (three count arrays turn
+// over) and, in workgroup 0, writes the next step's grid shift and wall
+// tables; every workgroup draws the next shift of its replica itself.
 #pragma once
 
 #include "mpcd_bucket.h"
@@ -69,15 +75,26 @@ struct XState {
   int32_t *cnt;
 };
 
-// records waiting for another kernel: the wall list (pre-drift state of the
-// particles that leave the wall-free box) and the escape lists (streamed
-// records whose next shifted bin they escape)
+// records waiting for the wall kernel: the pre-drift state of particles
+// that leave the wall-free box (m.w = 0), and streamed particles whose next
+// shifted bin they escape (m.w = 1)
 struct XList {
   float4 *a;
   float2 *b;
-  int4 *m;      // id, species, replica, 0
+  int4 *m;      // id, species, replica, kind
   int32_t *cnt;  // [1]
   int cap;
+};
+
+// what the step kernel writes for the next step: its grid shift and wall
+// tables (shift_replica, from this step's shifted-back walls), and the
+// cleared count of the next step's wall list
+struct XNext {
+  Draws D1;  // the next step's draws
+  const double *wq_prev_b;
+  double *shift, *wqs, *wqb, *box;
+  WallHot *wh;
+  int32_t *wl_cnt;
 };
 
 __device__ __forceinline__ void xrec_load(const XState &B, size_t o, bool onesp, float q[3], float v[3], int &id,
@@ -104,12 +121,12 @@ __device__ __forceinline__ void xrec_store(const XState &B, size_t o, bool onesp
 
 // appends a record to a list; false when the list is full
 __device__ __forceinline__ bool xlist_push(const XList &L, const float q[3], const float v[3], int id, int sp,
-                                           int rep) {
+                                           int rep, int kind) {
   const int e = atomicAdd(L.cnt, 1);
   if (e >= L.cap) return false;
   L.a[e] = make_float4(q[0], q[1], q[2], v[0]);
   L.b[e] = make_float2(v[1], v[2]);
-  L.m[e] = make_int4(id, sp, rep, 0);
+  L.m[e] = make_int4(id, sp, rep, kind);
   return true;
 }
 
@@ -148,6 +165,31 @@ __device__ __forceinline__ int xcell(const int Li[3], const int pbc[3], int cell
   bool bad = (a > Li[0]) | (b > Li[1]) | (a < 0) | (b < 0);
   if (DIM == 3) bad = bad | (c > Lz) | (c < 0);
   return bad ? -1 : cell_off + (a * ny + b) * nz + c;
+}
+
+// the escape repair of a record whose coming shifted bin it escapes (bin,
+// mpc.c:970-982; replacePos_WithCheck, init.c:1462) with the draws and wall
+// positions of that coming step (Cn, Dn): the re-placed position is in that
+// step's shifted frame.  Returns its shifted cell.
+__device__ __forceinline__ int xrepair(const Ctx &Cn, const RepDev &R, const Draws &Dn, int rep, int pid, double mass,
+                                       float q[3], float v[3]) {
+  double qd[3] = {(double)q[0], (double)q[1], (double)q[2]}, vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
+  double u0[3] = {0.0, 0.0, 0.0};
+  atomicAdd(&Cn.events[0], 1ull);
+  if (!escape_repair(R, Cn.wh, 0, Dn, rep, (uint32_t)pid, mass, false, qd, vd, u0, Cn.wall, Cn.wq_s))
+    atomicOr(&Cn.flags[0], kFlagEscape);
+  for (int d = 0; d < 3; d++) {
+    q[d] = (float)qd[d];
+    v[d] = (float)vd[d];
+  }
+  int cn = (R.dim == 3) ? xcell<3>(R.L, R.pbc, R.cell_off, R.ny, R.nz, q, true)
+                        : xcell<2>(R.L, R.pbc, R.cell_off, R.ny, R.nz, q, true);
+  if (cn < 0) {
+    // the FP32 rounding of a re-placed position can reach the box's edge
+    atomicOr(&Cn.flags[0], kFlagEscape);
+    cn = R.cell_off;
+  }
+  return cn;
 }
 
 // the wall-free box in FP32, rounded inwards, and cut to the positions
@@ -465,17 +507,16 @@ __device__ __forceinline__ double xke(double m, const float avv[3], const float 
 
 // per-replica values of a tile kernel in FP32
 struct XRep {
-  float L[3], sc[3], sn[3], box[6], dt, dtg[3], avv[3];
+  float L[3], sc[3], box[6], dt, dtg[3], avv[3];
   int Li[3], pbc[3], cell_off, ny, nz;
   bool wrap;
 };
 
-__device__ __forceinline__ XRep xrep(const RepDev &R, const double *shc, const double *shn, const double *box) {
+__device__ __forceinline__ XRep xrep(const RepDev &R, const double *shc, const double *box) {
   XRep X;
   for (int j = 0; j < 3; j++) {
     X.L[j] = (float)R.L[j];
     X.sc[j] = (float)shc[j];
-    X.sn[j] = (float)shn[j];
     X.dtg[j] = (float)(R.dt * R.grav[j]);
     X.avv[j] = (float)R.avv[j];
     X.Li[j] = R.L[j];
@@ -495,12 +536,19 @@ __device__ __forceinline__ XRep xrep(const RepDev &R, const double *shc, const d
 // ---------------------------------------------------------------------------
 constexpr int kXMaxCells = kBktMaxCells;
 
+// the bucketed step: one workgroup per bucket.  It also clears bucket count
+// cnt_zero[blockIdx.x] of the step after next, and workgroup 0 writes the
+// next step's tables (Nx) once its tile is done.  The SRD kernel asks for 5
+// waves per SIMD (at most 96 VGPRs) and spreads its cell phase over the four
+// waves (cell k on wave k mod 4); the MPC-AT kernels ask for 3 (at most 168
+// VGPRs, which the two-species ghost variant would exceed) and keep cell k on
+// lane k, since their heavier cell phase gains nothing from the spread.
 template <int OP, bool ONESP, int DIM, bool GHOST>
-__global__ __launch_bounds__(kBktThreads) void k_xbucket(
-    Ctx C, XState I, XState O, const TileDev *__restrict__ bkts, const BucketRep *__restrict__ brep,
-    const RepDev *__restrict__ reps, const mpcd_gpu_species *__restrict__ sps, const WallHot *__restrict__ wh,
-    const double *__restrict__ boxes, Draws Dr, const double *__restrict__ shift_cur,
-    const double *__restrict__ shift_next, int cap_lim, XList WL, XList EN, int nspec) {
+__global__ __launch_bounds__(kBktThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_SRD ? 5 : 3))) void k_xbucket(
+    Ctx C, XState I, XState O, int32_t *__restrict__ cnt_zero, const TileDev *__restrict__ bkts,
+    const BucketRep *__restrict__ brep, const RepDev *__restrict__ reps, const mpcd_gpu_species *__restrict__ sps,
+    const WallHot *__restrict__ wh, const double *__restrict__ boxes, Draws Dr, const double *__restrict__ shift_cur,
+    XNext Nx, int cap_lim, XList WL, int nspec) {
   constexpr int NT = kBktThreads, PPT = bkt_ppt(OP), CAP = NT * PPT, NS = xnsum(OP);
   constexpr int NSP = ONESP ? 1 : MPCD_MAXSPEC;
   const int tid = threadIdx.x;
@@ -511,6 +559,7 @@ __global__ __launch_bounds__(kBktThreads) void k_xbucket(
   __shared__ float s_cd[kXMaxCells][12];
   __shared__ int32_t s_hkey[kBktHash], s_hcnt[kBktHash], s_hbase[kBktHash];
   __shared__ double s_ke[NT / 64];
+  __shared__ double s_shn[3];
 
   // every record load of the thread first (a slot past the count reads the
   // last filled slot again; its values are not used)
@@ -537,10 +586,11 @@ __global__ __launch_bounds__(kBktThreads) void k_xbucket(
   }
   __syncthreads();
   if (s_flag & kFlagPoison) return;
+  if (tid == 0) cnt_zero[blockIdx.x] = 0;
   Dr.resolve();
   const RepDev &R = reps[T.rep];
   const BucketRep BR = brep[T.rep];
-  const XRep X = xrep(R, &shift_cur[T.rep * 3], &shift_next[T.rep * 3], &boxes[T.rep * 6]);
+  const XRep X = xrep(R, &shift_cur[T.rep * 3], &boxes[T.rep * 6]);
   const double sig1 = (OP == OP_SRD) ? 0.0 : sqrt(R.kbt / sps[0].mass);
   float rt[PPT][OP == OP_SRD ? 1 : 3];  // thermal velocities
 
@@ -582,8 +632,16 @@ __global__ __launch_bounds__(kBktThreads) void k_xbucket(
     }
   }
   __syncthreads();
-  // 2. one lane per cell: the collision data
-  for (int k = tid; k < ncl; k += NT) {
+  // 2. one lane per cell: the collision data; the last three lanes, which
+  //    hold no cell in either cell order, draw the next step's grid shift
+  //    of this replica as shift_replica draws it
+  static_assert(kXMaxCells <= NT - 3, "the last lanes hold no cell");
+  if (tid >= NT - 3) {
+    const int j = NT - 1 - tid;
+    Nx.D1.resolve();
+    s_shn[j] = (j < DIM && R.galinv) ? Nx.D1.rng.uniform(T.rep, P_SHIFT, 0, j) : 0.0;
+  }
+  for (int k = (OP == OP_SRD) ? (((tid & 63) << 2) | (tid >> 6)) : tid; k < ncl; k += NT) {
     const int np_ = s_n[k];
     if (np_ == 0) continue;
     double mass;
@@ -647,7 +705,8 @@ __global__ __launch_bounds__(kBktThreads) void k_xbucket(
       continue;
     }
     float qn[3] = {q[0], q[1], q[2]};
-    xshift_fwd<DIM>(X.L, X.sn, qn);
+    const float sn[3] = {(float)s_shn[0], (float)s_shn[1], (float)s_shn[2]};
+    xshift_fwd<DIM>(X.L, sn, qn);
     const int cn = xcell<DIM>(X.Li, X.pbc, X.cell_off, X.ny, X.nz, qn, true);
     if (cn < 0) {
       wl |= 1 << (8 + j);
@@ -676,7 +735,7 @@ __global__ __launch_bounds__(kBktThreads) void k_xbucket(
     const int b2 = rk[j];
     if (b2 == -2 || b2 == -3) {
       const int spid = ONESP ? 0 : rsp[j];
-      if (!xlist_push(b2 == -2 ? WL : EN, rq[j], rv[j], rid[j], spid, T.rep)) {
+      if (!xlist_push(WL, rq[j], rv[j], rid[j], spid, T.rep, b2 == -2 ? 0 : 1)) {
         atomicOr(&C.flags[0], kFlagPoison);
         atomicMin(&C.flags[1], (int)Dr.rng.step);
       }
@@ -702,110 +761,58 @@ __global__ __launch_bounds__(kBktThreads) void k_xbucket(
     for (int w = 0; w < NT / 64; w++) t += s_ke[w];
     C.tpart[blockIdx.x] = t;
   }
-}
-
-// the preparation of a bucketed step of the exact form, one launch:
-//   blocks [0, nbz): zero this step's output counts (the input counts of the
-//     previous step, complete when this kernel runs), the wall-list count and
-//     the next escape list's count;
-//   block nbz: the next step's grid shift and wall positions;
-//   blocks above: this step's escapes (records listed by the previous step):
-//     escape repair, then appended to the bucket of their new cell, marked
-//     as already in the shifted frame.
-// Nothing runs after a bucket overflow.
-__global__ __launch_bounds__(kBlock) void k_xprep(Ctx C, Draws D1, const double *wq_prev_b, double *shift_out,
-                                                  double *wqs_out, double *wqb_out, WallHot *wh_out, double *box_out,
-                                                  int32_t *cnt_zero, int nbkt, int nbz, XList WL, XList EC, XList EN,
-                                                  Draws D, XState I, const BucketRep *__restrict__ brep, int cap,
-                                                  int cap_lim, int onesp, int nesc) {
-  if (C.flags[0] & kFlagPoison) return;
-  const int tid = threadIdx.x, blk = blockIdx.x;
-  if (blk < nbz) {
-    const int b = blk * kBlock + tid;
-    if (b < nbkt) cnt_zero[b] = 0;
-    if (b == 0) {
-      *WL.cnt = 0;
-      *EN.cnt = 0;
-    }
-    return;
-  }
-  if (blk == nbz) {
-    D1.resolve();
-    for (int r = tid; r < C.nrep; r += kBlock)
-      shift_replica(C, D1, r, 1, wq_prev_b, shift_out, wqs_out, wqb_out, wh_out, box_out);
-    return;
-  }
-  D.resolve();
-  const int ne = min(*EC.cnt, EC.cap);
-  for (int e = (blk - nbz - 1) * kBlock + tid; e < ne; e += nesc * kBlock) {
-    const float4 a = EC.a[e];
-    const float2 b = EC.b[e];
-    const int4 m = EC.m[e];
-    const int pid = m.x, sp = m.y, rep = m.z;
-    const RepDev &R = C.rep[rep];
-    double q[3] = {(double)a.x, (double)a.y, (double)a.z}, v[3] = {(double)a.w, (double)b.x, (double)b.y};
-    double u0[3] = {0.0, 0.0, 0.0};
-    atomicAdd(&C.events[0], 1ull);
-    if (!escape_repair(R, C.wh, 0, D, rep, (uint32_t)pid, C.sp[sp].mass, false, q, v, u0, C.wall, C.wq_s))
-      atomicOr(&C.flags[0], kFlagEscape);
-    const float q3[3] = {(float)q[0], (float)q[1], (float)q[2]}, v3[3] = {(float)v[0], (float)v[1], (float)v[2]};
-    int cn = (R.dim == 3) ? xcell<3>(R.L, R.pbc, R.cell_off, R.ny, R.nz, q3, true)
-                          : xcell<2>(R.L, R.pbc, R.cell_off, R.ny, R.nz, q3, true);
-    if (cn < 0) {
-      // the FP32 rounding of a re-placed position can reach the box's edge
-      atomicOr(&C.flags[0], kFlagEscape);
-      cn = R.cell_off;
-    }
-    const int bk = bucket_of(brep[rep], R, cn);
-    const int p = atomicAdd(&I.cnt[bk], 1);
-    if (p >= cap_lim) {
-      atomicOr(&C.flags[0], kFlagPoison);
-      atomicMin(&C.flags[1], (int)D.rng.step);
-      continue;
-    }
-    xrec_store(I, (size_t)bk * cap + p, onesp != 0, q3, v3, pid | kIdRepaired, sp);
+  // workgroup 0, with the tile done (its registers free): the next step's
+  // shift and wall tables, in the other parity slot
+  if (blockIdx.x == 0) {
+    Nx.D1.resolve();
+    for (int r = tid; r < C.nrep; r += NT)
+      shift_replica(C, Nx.D1, r, 1, Nx.wq_prev_b, Nx.shift, Nx.wqs, Nx.wqb, Nx.wh, Nx.box);
+    if (tid == 0) *Nx.wl_cnt = 0;
   }
 }
 
 // the wall list of a bucketed step: the reference's FP64 bounce loop from
-// the pre-drift state, then the next shifted cell and the append
+// the pre-drift state (kind 0), then the next shifted cell; a record whose
+// next shifted bin it escapes (kind 1, or after the bounce loop) is re-placed
+// with the next step's draws and wall positions (Cn, Dn) and appended
+// already in that step's shifted frame
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_xwalls(Ctx C, XList WL, XState O, XList EN,
-                                                   const BucketRep *__restrict__ brep, const RepDev *__restrict__ reps,
+__global__ __launch_bounds__(kBlock) void k_xwalls(Ctx C, XList WL, XState O, const BucketRep *__restrict__ brep,
+                                                   const RepDev *__restrict__ reps,
                                                    const mpcd_gpu_species *__restrict__ sps,
                                                    const mpcd_gpu_wall *__restrict__ walls,
                                                    const WallHot *__restrict__ wh, const double *__restrict__ boxes,
-                                                   Draws Dr, const double *__restrict__ wq_b,
-                                                   const double *__restrict__ shift_next, int cap, int cap_lim,
-                                                   int onesp) {
+                                                   Draws Dr, const double *__restrict__ wq_b, Ctx Cn, Draws Dn,
+                                                   int cap, int cap_lim, int onesp) {
   if (C.flags[0] & kFlagPoison) return;
   const int n = min(*WL.cnt, WL.cap);
   Dr.resolve();
+  Dn.resolve();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float4 a = WL.a[i];
     const float2 b = WL.b[i];
     const int4 m = WL.m[i];
     const int pid = m.x, spid = m.y, rep = m.z;
     const RepDev &R = reps[rep];
-    double q[3] = {(double)a.x, (double)a.y, (double)a.z}, v[3] = {(double)a.w, (double)b.x, (double)b.y};
-    xwalls_fp64<DIM>(C, R, sps, walls, wh, wq_b, &boxes[rep * 6], Dr, rep, spid, pid, q, v);
-    float rq[3], rv[3];
+    float rq[3] = {a.x, a.y, a.z}, rv[3] = {a.w, b.x, b.y};
+    if (m.w == 0) {
+      double q[3] = {(double)a.x, (double)a.y, (double)a.z}, v[3] = {(double)a.w, (double)b.x, (double)b.y};
+      xwalls_fp64<DIM>(C, R, sps, walls, wh, wq_b, &boxes[rep * 6], Dr, rep, spid, pid, q, v);
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      rq[d] = (float)q[d];
-      rv[d] = (float)v[d];
+      for (int d = 0; d < 3; d++) {
+        rq[d] = (float)q[d];
+        rv[d] = (float)v[d];
+      }
     }
     float L[3] = {(float)R.L[0], (float)R.L[1], (float)R.L[2]}, sn[3];
-    for (int d = 0; d < 3; d++) sn[d] = (float)shift_next[rep * 3 + d];
+    for (int d = 0; d < 3; d++) sn[d] = (float)Cn.shift[rep * 3 + d];
     float qn[3] = {rq[0], rq[1], rq[2]};
     xshift_fwd<DIM>(L, sn, qn);
-    const int cn = xcell<DIM>(R.L, R.pbc, R.cell_off, R.ny, R.nz, qn, true);
+    int cn = xcell<DIM>(R.L, R.pbc, R.cell_off, R.ny, R.nz, qn, true);
+    int id = pid;
     if (cn < 0) {
-      if (!xlist_push(EN, rq, rv, pid, spid, rep)) {
-        atomicOr(&C.flags[0], kFlagPoison);
-        atomicMin(&C.flags[1], (int)Dr.rng.step);
-      }
-      continue;
+      cn = xrepair(Cn, R, Dn, rep, pid, sps[spid].mass, rq, rv);
+      id = pid | kIdRepaired;
     }
     const int b2 = bucket_of(brep[rep], R, cn);
     const int p2 = atomicAdd(&O.cnt[b2], 1);
@@ -814,15 +821,15 @@ __global__ __launch_bounds__(kBlock) void k_xwalls(Ctx C, XList WL, XState O, XL
       atomicMin(&C.flags[1], (int)Dr.rng.step);
       continue;
     }
-    xrec_store(O, (size_t)b2 * cap + p2, onesp != 0, rq, rv, pid, spid);
+    xrec_store(O, (size_t)b2 * cap + p2, onesp != 0, rq, rv, id, spid);
   }
 }
 
-// dense -> bucketed (step s, whose shift is drawn): every particle goes to
-// the bucket of its shifted cell; an escape of that bin goes to the escape
-// list, which the step's preparation kernel repairs
-__global__ void k_xbucketize(Ctx C, State<float> S, XState O, XList EC, const BucketRep *__restrict__ brep, int cap,
-                             int cap_lim, int onesp) {
+// dense -> bucketed (step s, whose shift and tables are drawn): every
+// particle goes to the bucket of its shifted cell; an escape of that bin is
+// re-placed first (with step s's draws) and marked as already shifted
+__global__ void k_xbucketize(Ctx C, State<float> S, XState O, Draws D, const BucketRep *__restrict__ brep, int cap,
+                             int cap_lim, int onesp, int32_t *shifted) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= C.ntot) return;
   int r = 0;
@@ -834,39 +841,41 @@ __global__ void k_xbucketize(Ctx C, State<float> S, XState O, XList EC, const Bu
   const float sc[3] = {(float)C.shift[r * 3], (float)C.shift[r * 3 + 1], (float)C.shift[r * 3 + 2]};
   float qs[3] = {q[0], q[1], q[2]};
   int cn;
+  // a particle re-placed by the step that left it: already in this step's frame
+  const bool pre = shifted[R.part_off + id] != 0;
+  if (pre) shifted[R.part_off + id] = 0;
   if (R.dim == 3) {
-    xshift_fwd<3>(L, sc, qs);
+    if (!pre) xshift_fwd<3>(L, sc, qs);
     cn = xcell<3>(R.L, R.pbc, R.cell_off, R.ny, R.nz, qs, true);
   } else {
-    xshift_fwd<2>(L, sc, qs);
+    if (!pre) xshift_fwd<2>(L, sc, qs);
     cn = xcell<2>(R.L, R.pbc, R.cell_off, R.ny, R.nz, qs, true);
   }
+  float rq[3] = {q[0], q[1], q[2]}, rv[3] = {v[0], v[1], v[2]};
+  int rid = pre ? (id | kIdRepaired) : id;
   if (cn < 0) {
-    if (!xlist_push(EC, q, v, id, sp, r)) {
-      atomicOr(&C.flags[0], kFlagPoison);
-      atomicMin(&C.flags[1], -1);
-    }
-    return;
+    D.resolve();
+    cn = xrepair(C, R, D, r, id, C.sp[sp].mass, rq, rv);
+    rid = id | kIdRepaired;
   }
   const int b = bucket_of(brep[r], R, cn);
-  const int p = hist_add(O.cnt, b);
+  const int p = atomicAdd(&O.cnt[b], 1);
   if (p >= cap_lim) {
     atomicOr(&C.flags[0], kFlagPoison);
     atomicMin(&C.flags[1], -1);
     return;
   }
-  xrec_store(O, (size_t)b * cap + p, onesp != 0, q, v, id, sp);
+  xrec_store(O, (size_t)b * cap + p, onesp != 0, rq, rv, rid, sp);
 }
 
 // bucketed -> dense: every record to the slot of its particle (replica offset
-// + id), so the dense state is in particle order.  A record re-placed by a
-// step that did not complete (bucket overflow) is given a position that
-// escapes its shifted bin, so the dense step repeats the repair with the same
-// draws.  The escapes listed for the step go back the same way (also those
-// whose re-placed record did not fit its bucket: only slots below cap_lim
-// hold records).
-__global__ void k_xdensify(Ctx C, XState I, XList EC, State<float> S, const TileDev *__restrict__ bkts, int cap,
-                           int cap_lim, int onesp) {
+// + id), so the dense state is in particle order.  A re-placed record (an
+// escape of the coming step's shifted bin, already in that step's shifted
+// frame) keeps its position and raises its particle's flag (by particle
+// index, so it survives any reordering of the dense state), which tells the
+// next step not to shift it.
+__global__ void k_xdensify(Ctx C, XState I, State<float> S, const TileDev *__restrict__ bkts, int cap,
+                           int cap_lim, int onesp, int32_t *shifted) {
   const TileDev T = bkts[blockIdx.x];
   const RepDev &R = C.rep[T.rep];
   const int n = min(I.cnt[blockIdx.x], min(cap, cap_lim));
@@ -877,7 +886,7 @@ __global__ void k_xdensify(Ctx C, XState I, XList EC, State<float> S, const Tile
     xrec_load(I, o, onesp != 0, q, v, id, sp);
     if (id < 0) {
       id &= 0x7FFFFFFF;
-      q[0] = (float)(-2.0 * (double)R.L[0] - 2.0);
+      shifted[R.part_off + id] = 1;
     }
     const int s = R.part_off + id;
     S.x[s] = q[0];
@@ -889,24 +898,6 @@ __global__ void k_xdensify(Ctx C, XState I, XList EC, State<float> S, const Tile
     S.id[s] = id;
     S.sp[s] = onesp ? 0 : sp;
     S.cst[s] = -1;
-  }
-  if (blockIdx.x == 0) {
-    const int ne = min(*EC.cnt, EC.cap);
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      const float4 a = EC.a[e];
-      const float2 b = EC.b[e];
-      const int4 m = EC.m[e];
-      const int s = C.rep[m.z].part_off + m.x;
-      S.x[s] = a.x;
-      S.y[s] = a.y;
-      S.z[s] = a.z;
-      S.vx[s] = a.w;
-      S.vy[s] = b.x;
-      S.vz[s] = b.y;
-      S.id[s] = m.x;
-      S.sp[s] = m.y;
-      S.cst[s] = -1;
-    }
   }
 }
 
@@ -925,7 +916,8 @@ struct XDense {
 
 template <int OP, bool ONESP, int DIM>
 __global__ __launch_bounds__(kBlock) void k_xd_sums(Ctx C, State<float> S, XDense X, const BlkDev *__restrict__ blk,
-                                                    const mpcd_gpu_species *__restrict__ sps, Draws Dr) {
+                                                    const mpcd_gpu_species *__restrict__ sps, Draws Dr,
+                                                    int32_t *shifted) {
   constexpr int NS = xnsum(OP);
   const BlkDev B = blk[blockIdx.x];
   const int s = B.beg + threadIdx.x;
@@ -937,7 +929,9 @@ __global__ __launch_bounds__(kBlock) void k_xd_sums(Ctx C, State<float> S, XDens
   float q[3] = {S.x[s], S.y[s], S.z[s]}, v[3] = {S.vx[s], S.vy[s], S.vz[s]};
   const int pid = S.id[s], spid = ONESP ? 0 : S.sp[s];
   const double m = sps[spid].mass;
-  xshift_fwd<DIM>(L, sc, q);
+  // a particle re-placed by the step that left it is already in this frame
+  if (shifted[R.part_off + pid]) shifted[R.part_off + pid] = 0;
+  else xshift_fwd<DIM>(L, sc, q);
   int cn = xcell<DIM>(R.L, R.pbc, R.cell_off, R.ny, R.nz, q, true);
   if (cn < 0) {
     // escape repair (mpc.c:970-982), then binned where it was placed
@@ -1002,19 +996,23 @@ __global__ __launch_bounds__(kBlock) void k_xd_cells(Ctx C, XDense X, const BlkD
   xcell_terms<OP, DIM, GHOST>(R, C.wh, Dr, B.rep, c, np_, mass, sm, &X.cd[(size_t)c * 12]);
 }
 
+// the particle phase of a dense step; a particle whose next shifted bin it
+// escapes is re-placed here with the next step's draws and wall positions
+// (Cn, Dn) and flagged as already in that step's frame, as the bucketed
+// layout does
 template <int OP, bool ONESP, int DIM>
 __global__ __launch_bounds__(kBlock) void k_xd_part(Ctx C, State<float> S, XDense X, const BlkDev *__restrict__ blk,
                                                     const mpcd_gpu_species *__restrict__ sps,
                                                     const mpcd_gpu_wall *__restrict__ walls,
-                                                    const double *__restrict__ boxes, Draws Dr) {
+                                                    const double *__restrict__ boxes, Draws Dr, Ctx Cn, Draws Dn,
+                                                    int32_t *shifted) {
   const BlkDev B = blk[blockIdx.x];
   const int s = B.beg + threadIdx.x;
   double ke = 0.0;
   if (s < B.end) {
     Dr.resolve();
     const RepDev &R = C.rep[B.rep];
-    const double shz[3] = {0.0, 0.0, 0.0};
-    const XRep XR = xrep(R, &C.shift[B.rep * 3], shz, &boxes[B.rep * 6]);
+    const XRep XR = xrep(R, &C.shift[B.rep * 3], &boxes[B.rep * 6]);
     float q[3] = {S.x[s], S.y[s], S.z[s]}, v[3] = {S.vx[s], S.vy[s], S.vz[s]};
     const int pid = S.id[s], spid = ONESP ? 0 : S.sp[s];
     const double m = sps[spid].mass;
@@ -1042,6 +1040,16 @@ __global__ __launch_bounds__(kBlock) void k_xd_part(Ctx C, State<float> S, XDens
       for (int d = 0; d < 3; d++) {
         q[d] = (float)qd[d];
         v[d] = (float)vd[d];
+      }
+    }
+    {
+      float sn[3] = {(float)Cn.shift[B.rep * 3], (float)Cn.shift[B.rep * 3 + 1], (float)Cn.shift[B.rep * 3 + 2]};
+      float qn[3] = {q[0], q[1], q[2]};
+      xshift_fwd<DIM>(XR.L, sn, qn);
+      if (xcell<DIM>(R.L, R.pbc, R.cell_off, R.ny, R.nz, qn, true) < 0) {
+        Dn.resolve();
+        (void)xrepair(Cn, R, Dn, B.rep, pid, m, q, v);
+        shifted[R.part_off + pid] = 1;
       }
     }
     S.x[s] = q[0];

@@ -89,7 +89,11 @@ struct mpcd_gpu_ctx {
   // escape lists by step parity, and the cell arrays of its dense steps
   bool exact = false;
   XState bx[2]{};
-  XList xwl{}, xesc[2]{};
+  XList xwl[2]{};                       // wall lists by step parity
+  int32_t *xcnt[3] = {nullptr, nullptr, nullptr};  // bucket counts, turning over
+  int xci0 = 0;                          // count array of the first bucketed step (bkt_step0)
+  int32_t *xshifted = nullptr;           // per particle: re-placed, already in the coming step's frame
+  uint32_t x_tables = 0xFFFFFFFFu;      // the step whose shift and wall tables are in its parity slot
   XDense xd{};
   hipGraph_t bgraph = nullptr;
   hipGraphExec_t bgexec = nullptr;
@@ -495,6 +499,10 @@ extern "C" mpcd_status mpcd_gpu_create(const mpcd_gpu_params *params, int nrep, 
   }
   c->hist2[0] = dalloc<int32_t>(c, c->nctot + 1);
   c->hist2[1] = dalloc<int32_t>(c, c->nctot + 1);
+  if (c->exact) {
+    c->xshifted = dalloc<int32_t>(c, c->ntot);
+    ok = ok && c->xshifted;
+  }
   ok = ok && c->d_rep && c->d_pblk && c->d_cblk && c->d_tiles && c->d_cblk_rng && c->d_pblk_rng && c->d_tile_rng &&
        c->d_sp &&
        c->d_lct && c->d_lct_rng && c->d_lct_over &&
@@ -1168,6 +1176,8 @@ extern "C" mpcd_status mpcd_gpu_upload(mpcd_gpu_ctx *c, int r, const double *Q, 
     mpcd_status zs = zero_tangent(c);  // a new state has no derivative yet
     if (zs) return zs;
   }
+  if (c->xshifted)  // a new state has no re-placed particle
+    k_fill<<<nblocks(R.npart, kBlock), kBlock, 0, c->stream>>>(c->xshifted + R.part_off, 0, R.npart);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->uploaded[r] = 1;
   c->need_init = true;
@@ -1556,10 +1566,11 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
     // escape lists: a step with more escapes finishes on the cell-sorted path
     c->esc_cap = std::max(4096, c->ntot / 8);
     if (c->exact) {
-      // the wall list holds at most every particle once per step
-      if (!alloc_xlist(c, c->xwl, c->ntot) || !alloc_xlist(c, c->xesc[0], c->esc_cap) ||
-          !alloc_xlist(c, c->xesc[1], c->esc_cap))
-        return fail(c, MPCD_ENOMEM, "wall and escape lists");
+      // a wall list holds at most every particle once per step
+      for (int k = 0; k < 3; k++) c->xcnt[k] = dalloc<int32_t>(c, nc1);
+      if (!alloc_xlist(c, c->xwl[0], c->ntot) || !alloc_xlist(c, c->xwl[1], c->ntot) || !c->xcnt[0] ||
+          !c->xcnt[1] || !c->xcnt[2])
+        return fail(c, MPCD_ENOMEM, "wall lists and bucket counts");
     } else {
       if (!(c->precision == MPCD_FP64 ? alloc_wall_list<double>(c, c->wld) : alloc_wall_list<float>(c, c->wlf)))
         return fail(c, MPCD_ENOMEM, "wall list");
@@ -1597,7 +1608,6 @@ static mpcd_status ensure_buckets(mpcd_gpu_ctx *c) {
   for (int k = 0; k < 2; k++) {
     c->bd[k].cnt = c->bcnt[k];
     c->bf[k].cnt = c->bcnt[k];
-    c->bx[k].cnt = c->bcnt[k];
   }
   return MPCD_OK;
 }
@@ -1654,39 +1664,50 @@ static void dispatch_bucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int p
 // ---------------------------------------------------------------------------
 // the exact-sum FP32 form (mpcd_exact.h): bucketed steps and dense steps
 // ---------------------------------------------------------------------------
+struct XLaunch {
+  Ctx C;
+  XState I, O;
+  int32_t *zero;
+  XNext Nx;
+  Draws D;
+  int par;
+};
+
 template <int OP, bool ONESP, int DIM, bool GHOST>
-static void launch_xbucket_g(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+static void launch_xbucket_g(mpcd_gpu_ctx *c, const XLaunch &a) {
   const int nwm = std::max(1, c->nwtot);
   k_xbucket<OP, ONESP, DIM, GHOST><<<c->nbkt, kBktThreads, 0, c->stream>>>(
-      C, c->bx[c->bcur], c->bx[1 - c->bcur], c->d_bkts, c->d_brep, c->d_rep, c->d_sp, c->wh2 + par * nwm,
-      c->box2 + par * 6 * c->nrep, D, c->shift2 + par * 3 * c->nrep, c->shift2 + (par ^ 1) * 3 * c->nrep,
-      bkt_limit(c), c->xwl, c->xesc[par ^ 1], c->nspec);
+      a.C, a.I, a.O, a.zero, c->d_bkts, c->d_brep, c->d_rep, c->d_sp, c->wh2 + a.par * nwm,
+      c->box2 + a.par * 6 * c->nrep, a.D, c->shift2 + a.par * 3 * c->nrep, a.Nx, bkt_limit(c), c->xwl[a.par],
+      c->nspec);
 }
 
 template <int OP, bool ONESP, int DIM>
-static void launch_xbucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
-  if (c->any_phantom) launch_xbucket_g<OP, ONESP, DIM, true>(c, C, D, par);
-  else launch_xbucket_g<OP, ONESP, DIM, false>(c, C, D, par);
+static void launch_xbucket(mpcd_gpu_ctx *c, const XLaunch &a) {
+  if (c->any_phantom) launch_xbucket_g<OP, ONESP, DIM, true>(c, a);
+  else launch_xbucket_g<OP, ONESP, DIM, false>(c, a);
 }
 
 template <int OP, bool ONESP>
-static void launch_xbucket_dim(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
-  if (c->dim == 3) launch_xbucket<OP, ONESP, 3>(c, C, D, par);
-  else launch_xbucket<OP, ONESP, 2>(c, C, D, par);
+static void launch_xbucket_dim(mpcd_gpu_ctx *c, const XLaunch &a) {
+  if (c->dim == 3) launch_xbucket<OP, ONESP, 3>(c, a);
+  else launch_xbucket<OP, ONESP, 2>(c, a);
 }
 
-static void dispatch_xbucket(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
+static void dispatch_xbucket(mpcd_gpu_ctx *c, const Ctx &C, const XState &I, const XState &O, int32_t *zero,
+                             const XNext &Nx, const Draws &D, int par) {
+  const XLaunch a{C, I, O, zero, Nx, D, par};
   const bool one = c->nspec == 1;
   const int op = fused_op(c->rtech);
   if (op == OP_SRD) {
-    if (one) launch_xbucket_dim<OP_SRD, true>(c, C, D, par);
-    else launch_xbucket_dim<OP_SRD, false>(c, C, D, par);
+    if (one) launch_xbucket_dim<OP_SRD, true>(c, a);
+    else launch_xbucket_dim<OP_SRD, false>(c, a);
   } else if (op == OP_AT) {
-    if (one) launch_xbucket_dim<OP_AT, true>(c, C, D, par);
-    else launch_xbucket_dim<OP_AT, false>(c, C, D, par);
+    if (one) launch_xbucket_dim<OP_AT, true>(c, a);
+    else launch_xbucket_dim<OP_AT, false>(c, a);
   } else {
-    if (one) launch_xbucket_dim<OP_RAT, true>(c, C, D, par);
-    else launch_xbucket_dim<OP_RAT, false>(c, C, D, par);
+    if (one) launch_xbucket_dim<OP_RAT, true>(c, a);
+    else launch_xbucket_dim<OP_RAT, false>(c, a);
   }
 }
 
@@ -1698,60 +1719,76 @@ static bool x_needs_walls(const mpcd_gpu_ctx *c) {
   return false;
 }
 
+// the count array of the input of bucketed step s (three turn over)
+static int x_ci(const mpcd_gpu_ctx *c, uint32_t s) { return (int)((c->xci0 + (s - c->bkt_step0)) % 3u); }
+
 static void run_step_xbucket(mpcd_gpu_ctx *c) {
   const int par = c->step & 1, p1 = par ^ 1;
   Ctx C = view(c, par);
   Draws D = make_draws(c, 0, c->step);
   const int cap = bkt_cap(fused_op(c->rtech)), one = c->nspec == 1 ? 1 : 0;
   const int nw3 = 3 * std::max(1, c->nwtot), nwm = std::max(1, c->nwtot);
-  {
-    // zeroed output counts and lists, the next step's shift, this step's escapes
-    ProfScope ps(c, F_SCAN);
-    const int nbz = nblocks(c->nbkt, kBlock), nesc = 4;
-    Draws D1 = make_draws(c, 0, c->step + 1);
-    k_xprep<<<nbz + 1 + nesc, kBlock, 0, c->stream>>>(
-        C, D1, c->wqb2 + par * nw3, c->shift2 + p1 * 3 * c->nrep, c->wqs2 + p1 * nw3, c->wqb2 + p1 * nw3,
-        c->wh2 + p1 * nwm, c->box2 + p1 * 6 * c->nrep, c->bcnt[1 - c->bcur], c->nbkt, nbz, c->xwl, c->xesc[par],
-        c->xesc[p1], D, c->bx[c->bcur], c->d_brep, cap, bkt_limit(c), one, nesc);
-  }
+  const int ci = x_ci(c, c->step);
+  XState I = c->bx[c->bcur], O = c->bx[1 - c->bcur];
+  I.cnt = c->xcnt[ci];
+  O.cnt = c->xcnt[(ci + 1) % 3];
+  XNext Nx{};
+  Nx.D1 = make_draws(c, 0, c->step + 1);
+  Nx.wq_prev_b = c->wqb2 + par * nw3;
+  Nx.shift = c->shift2 + p1 * 3 * c->nrep;
+  Nx.wqs = c->wqs2 + p1 * nw3;
+  Nx.wqb = c->wqb2 + p1 * nw3;
+  Nx.box = c->box2 + p1 * 6 * c->nrep;
+  Nx.wh = c->wh2 + p1 * nwm;
+  Nx.wl_cnt = c->xwl[p1].cnt;
   {
     ProfScope ps(c, F_BUCKET);
-    dispatch_xbucket(c, C, D, par);
+    dispatch_xbucket(c, C, I, O, c->xcnt[(ci + 2) % 3], Nx, D, par);
   }
   if (x_needs_walls(c)) {
     ProfScope ps(c, F_STREAM);
     const int nb = 512;
+    Ctx Cn = view(c, p1);
     if (c->dim == 3)
-      k_xwalls<3><<<nb, kBlock, 0, c->stream>>>(C, c->xwl, c->bx[1 - c->bcur], c->xesc[p1], c->d_brep, c->d_rep,
-                                                c->d_sp, c->d_wall, c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
-                                                c->wqb2 + par * nw3, c->shift2 + p1 * 3 * c->nrep, cap, bkt_limit(c),
-                                                one);
+      k_xwalls<3><<<nb, kBlock, 0, c->stream>>>(C, c->xwl[par], O, c->d_brep, c->d_rep, c->d_sp, c->d_wall,
+                                                c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
+                                                c->wqb2 + par * nw3, Cn, Nx.D1, cap, bkt_limit(c), one);
     else
-      k_xwalls<2><<<nb, kBlock, 0, c->stream>>>(C, c->xwl, c->bx[1 - c->bcur], c->xesc[p1], c->d_brep, c->d_rep,
-                                                c->d_sp, c->d_wall, c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
-                                                c->wqb2 + par * nw3, c->shift2 + p1 * 3 * c->nrep, cap, bkt_limit(c),
-                                                one);
+      k_xwalls<2><<<nb, kBlock, 0, c->stream>>>(C, c->xwl[par], O, c->d_brep, c->d_rep, c->d_sp, c->d_wall,
+                                                c->wh2 + par * nwm, c->box2 + par * 6 * c->nrep, D,
+                                                c->wqb2 + par * nw3, Cn, Nx.D1, cap, bkt_limit(c), one);
   }
   c->bcur ^= 1;
   c->step++;
+  c->x_tables = c->step;  // the step kernel wrote them
 }
 
-// dense (particle-order) state -> bucketed, at a step whose shift is drawn here
+// the shift and wall tables of step s in its parity slot (from step s - 1's)
+static void x_ensure_tables(mpcd_gpu_ctx *c, uint32_t s) {
+  if (c->x_tables == s) return;
+  launch_shift(c, s, 0);
+  c->x_tables = s;
+}
+
+// dense (any particle order) state -> bucketed, at the current step
 static mpcd_status xbucket_enter(mpcd_gpu_ctx *c) {
   mpcd_status st = ensure_buckets(c);
   if (st) return st;
   const int par = c->step & 1;
-  launch_shift(c, c->step, 0);
+  x_ensure_tables(c, c->step);
   Ctx C = view(c, par);
-  k_fill<<<nblocks(c->nbkt + 1, kBlock), kBlock, 0, c->stream>>>(c->bcnt[0], 0, c->nbkt + 1);
-  k_fill<<<nblocks(c->nbkt + 1, kBlock), kBlock, 0, c->stream>>>(c->bcnt[1], 0, c->nbkt + 1);
-  k_fill<<<1, 1, 0, c->stream>>>(c->C.flags + 1, 0x7FFFFFFF, 1);
-  k_fill<<<1, 1, 0, c->stream>>>(c->xesc[par].cnt, 0, 1);
-  k_xbucketize<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C, c->sf[c->cur], c->bx[0], c->xesc[par],
-                                                                    c->d_brep, bkt_cap(fused_op(c->rtech)),
-                                                                    bkt_limit(c), c->nspec == 1 ? 1 : 0);
-  c->bcur = 0;
+  Draws D = make_draws(c, 0, c->step);
   c->bkt_step0 = c->step;
+  c->xci0 = 0;
+  for (int k = 0; k < 3; k++) k_fill<<<nblocks(c->nbkt + 1, kBlock), kBlock, 0, c->stream>>>(c->xcnt[k], 0, c->nbkt + 1);
+  k_fill<<<1, 1, 0, c->stream>>>(c->C.flags + 1, 0x7FFFFFFF, 1);
+  k_fill<<<1, 1, 0, c->stream>>>(c->xwl[par].cnt, 0, 1);
+  XState O = c->bx[0];
+  O.cnt = c->xcnt[0];
+  k_xbucketize<<<nblocks(c->ntot, kBlock), kBlock, 0, c->stream>>>(C, c->sf[c->cur], O, D, c->d_brep,
+                                                                    bkt_cap(fused_op(c->rtech)), bkt_limit(c),
+                                                                    c->nspec == 1 ? 1 : 0, c->xshifted);
+  c->bcur = 0;
   return check_launch(c);
 }
 
@@ -1759,8 +1796,10 @@ static mpcd_status xbucket_enter(mpcd_gpu_ctx *c) {
 // structures of the cell-sorted layout are rebuilt when something reads them
 static void xbucket_densify(mpcd_gpu_ctx *c, uint32_t s, int bsrc) {
   Ctx C = view(c, s & 1);
-  k_xdensify<<<c->nbkt, kBlock, 0, c->stream>>>(C, c->bx[bsrc], c->xesc[s & 1], c->sf[c->cur], c->d_bkts,
-                                                bkt_cap(fused_op(c->rtech)), bkt_limit(c), c->nspec == 1 ? 1 : 0);
+  XState I = c->bx[bsrc];
+  I.cnt = c->xcnt[x_ci(c, s)];
+  k_xdensify<<<c->nbkt, kBlock, 0, c->stream>>>(C, I, c->sf[c->cur], c->d_bkts, bkt_cap(fused_op(c->rtech)),
+                                                bkt_limit(c), c->nspec == 1 ? 1 : 0, c->xshifted);
   c->pending = false;
   c->hist_ready = false;
   c->need_init = true;
@@ -1787,13 +1826,14 @@ template <int OP, bool ONESP, int DIM>
 static void launch_xdense(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int par) {
   State<float> S = c->sf[c->cur];
   const int npb = (int)c->pblk.size(), ncb = (int)c->cblk.size();
-  k_xd_sums<OP, ONESP, DIM><<<npb, kBlock, 0, c->stream>>>(C, S, c->xd, c->d_pblk, c->d_sp, D);
+  k_xd_sums<OP, ONESP, DIM><<<npb, kBlock, 0, c->stream>>>(C, S, c->xd, c->d_pblk, c->d_sp, D, c->xshifted);
   if (c->any_phantom)
     k_xd_cells<OP, ONESP, DIM, true><<<ncb, kBlock, 0, c->stream>>>(C, c->xd, c->d_cblk, c->d_sp, D, c->nspec);
   else
     k_xd_cells<OP, ONESP, DIM, false><<<ncb, kBlock, 0, c->stream>>>(C, c->xd, c->d_cblk, c->d_sp, D, c->nspec);
   k_xd_part<OP, ONESP, DIM><<<npb, kBlock, 0, c->stream>>>(C, S, c->xd, c->d_pblk, c->d_sp, c->d_wall,
-                                                           c->box2 + par * 6 * c->nrep, D);
+                                                           c->box2 + par * 6 * c->nrep, D, view(c, par ^ 1),
+                                                           make_draws(c, 0, c->step + 1), c->xshifted);
 }
 
 template <int OP, bool ONESP>
@@ -1806,7 +1846,7 @@ static void launch_xdense_dim(mpcd_gpu_ctx *c, const Ctx &C, const Draws &D, int
 static void run_step_xdense(mpcd_gpu_ctx *c) {
   if (ensure_xdense(c)) return;
   const int par = c->step & 1;
-  launch_shift(c, c->step, 0);
+  x_ensure_tables(c, c->step);
   Ctx C = view(c, par);
   Draws D = make_draws(c, 0, c->step);
   const size_t nc = c->nctot, ns = xnsum(fused_op(c->rtech));
@@ -1816,6 +1856,9 @@ static void run_step_xdense(mpcd_gpu_ctx *c) {
                                                                                  (int)(nc * MPCD_MAXSPEC));
   // zero doubles: all-zero words
   k_fill<<<nblocks((int)(2 * nc * ns), kBlock), kBlock, 0, c->stream>>>((int32_t *)c->xd.sum, 0, (int)(2 * nc * ns));
+  // the next step's tables: the particle phase re-places the particles that
+  // would escape its shifted bin
+  launch_shift(c, c->step + 1, 0);
   const bool one = c->nspec == 1;
   const int op = fused_op(c->rtech);
   if (op == OP_SRD) {
@@ -1833,6 +1876,7 @@ static void run_step_xdense(mpcd_gpu_ctx *c) {
   c->hist_ready = false;
   c->need_init = true;
   c->step++;
+  c->x_tables = c->step;
 }
 
 // one step in the bucketed layout (internal generator)
@@ -1984,6 +2028,7 @@ static mpcd_status bucket_finish(mpcd_gpu_ctx *c, uint32_t run_step0, int *done)
   // s == s0: the cell-sorted state the bucketed layout started from is intact
   if (s != s0) bucket_densify<Real>(c, s, (int)((s - s0) & 1u));
   c->in_bucket = false;
+  if (c->exact) c->x_tables = s;  // written by step s - 1 (or on entry)
   *done = (int)(s - run_step0);
   c->bkt_steps += *done;
   c->step = s;
@@ -1992,23 +2037,36 @@ static mpcd_status bucket_finish(mpcd_gpu_ctx *c, uint32_t run_step0, int *done)
 
 // records two bucketed steps (from an even step) and the advance of the
 // device step counter into c->bgexec; nothing runs while recording
+// the steps of one recorded group of the bucketed layout: after them the
+// buffers (two), the parity slots (two) and, in the exact-sum form, the
+// bucket count arrays (three) are where they were
+static int bucket_group(const mpcd_gpu_ctx *c) { return c->exact ? 6 : 2; }
+
+// what a recorded group depends on besides the buffers: the exact-sum form's
+// count arrays turn over with the step since the layout was entered
+static int bucket_graph_key(const mpcd_gpu_ctx *c) {
+  if (!c->exact) return c->bcur;
+  return (int)((c->step - c->bkt_step0) % 6u) + 6 * (int)(c->bkt_step0 & 1u);
+}
+
 template <typename Real>
 static mpcd_status capture_bucket_pair(mpcd_gpu_ctx *c) {
   drop_bucket_graph(c);
-  const uint32_t step0 = c->step;
-  const int bcur0 = c->bcur;
+  const uint32_t step0 = c->step, tables0 = c->x_tables;
+  const int bcur0 = c->bcur, key0 = bucket_graph_key(c), G = bucket_group(c);
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   c->capturing = true;
   c->cap_base = step0;
-  for (int k = 0; k < 2; k++) run_step_bucket<Real>(c);
-  k_addu32<<<1, 1, 0, c->stream>>>(c->d_step, 2u);
+  for (int k = 0; k < G; k++) run_step_bucket<Real>(c);
+  k_addu32<<<1, 1, 0, c->stream>>>(c->d_step, (uint32_t)G);
   c->capturing = false;
   hipError_t e = hipStreamEndCapture(c->stream, &c->bgraph);
   c->step = step0;
   c->bcur = bcur0;
+  c->x_tables = tables0;
   if (e != hipSuccess) return fail(c, MPCD_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
   HIPCHK(c, hipGraphInstantiate(&c->bgexec, c->bgraph, nullptr, nullptr, 0));
-  c->bgraph_cur = bcur0;
+  c->bgraph_cur = key0;
   return MPCD_OK;
 }
 
@@ -2025,22 +2083,26 @@ static mpcd_status bucket_run(mpcd_gpu_ctx *c, int nsteps, int use_graph, int *d
   const uint32_t run_step0 = c->step;
   int left = nsteps;
   if (use_graph) {
-    while (left > 0 && (c->step & 1)) {
+    // eager steps up to the start of a group, so that one recorded group
+    // serves every later call
+    const int G = bucket_group(c);
+    while (left > 0 && (c->exact ? (c->step - c->bkt_step0) % (uint32_t)G != 0u : (c->step & 1u) != 0u)) {
       run_step_bucket<Real>(c);
       left--;
     }
-    const int pairs = left / 2;
-    if (pairs > 0) {
-      if (!c->bgexec || c->bgraph_cur != c->bcur) {
+    const int groups = left / G;
+    if (groups > 0) {
+      if (!c->bgexec || c->bgraph_cur != bucket_graph_key(c)) {
         st = check_launch(c);
         if (st) return st;
         st = capture_bucket_pair<Real>(c);
         if (st) return st;
       }
       k_setu32<<<1, 1, 0, c->stream>>>(c->d_step, c->step);
-      for (int k = 0; k < pairs; k++) HIPCHK(c, hipGraphLaunch(c->bgexec, c->stream));
-      c->step += 2u * pairs;
-      left -= 2 * pairs;
+      for (int k = 0; k < groups; k++) HIPCHK(c, hipGraphLaunch(c->bgexec, c->stream));
+      c->step += (uint32_t)(G * groups);
+      left -= G * groups;
+      if (c->exact) c->x_tables = c->step;
     }
   }
   for (; left > 0; left--) run_step_bucket<Real>(c);
@@ -2140,7 +2202,7 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
 // the first allocates the most slots; a trial whose buckets overflow loses
 template <typename Real>
 static mpcd_status autotune_buckets(mpcd_gpu_ctx *c, int steps, double *ms_out) {
-  const double fills[] = {0.4, 0.5, 0.6, 0.7, 0.75};
+  const double fills[] = {0.4, 0.5, 0.6, 0.7, 0.75, 0.8};
   double best = 1e300, best_fill = c->bkt_fill;
   hipEvent_t a, b;
   HIPCHK(c, hipEventCreate(&a));
@@ -2279,6 +2341,7 @@ static void state_buffers(mpcd_gpu_ctx *c, std::vector<std::pair<void *, size_t>
   v.push_back({c->box2, 2 * 6 * c->nrep * sizeof(double)});
   v.push_back({c->C.kbtnow, c->nrep * sizeof(double)}); v.push_back({c->C.gscale, c->nrep * sizeof(double)});
   v.push_back({c->C.cell.S, nc * sizeof(double)}); v.push_back({c->C.cell.dir, 3 * nc * sizeof(double)});
+  if (c->xshifted) v.push_back({c->xshifted, n * sizeof(int32_t)});
 }
 
 // copies of the cell-sorted state (state_buffers) and the host step state
@@ -2315,6 +2378,7 @@ static mpcd_status load_state(mpcd_gpu_ctx *c, const mpcd_gpu_ctx::Checkpoint &k
   c->hist_ready = k.hist_ready;
   c->need_init = k.need_init;
   c->cu_src = k.cu_src;
+  c->x_tables = 0xFFFFFFFFu;  // redrawn from the restored wall positions
   return MPCD_OK;
 }
 

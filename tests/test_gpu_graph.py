@@ -84,10 +84,11 @@ def test_autotune_changes_no_result(prec):
     a = make_ctx(M, cfgs, P)
     b = make_ctx(M, cfgs, P)
     trial = 2
-    ms = a.autotune(trial)  # 5 candidates x (1 + trial) real steps
+    ms = a.autotune(trial)  # 6 bucket fills x (1 + trial) real steps
     assert ms > 0.0
     assert 0.2 <= a.tile_fill() <= 0.9
-    b.run(5 * (1 + trial), use_graph=False)
+    assert a.layout_stats()[0] > 0, "the contexts take the bucketed layout"
+    b.run(6 * (1 + trial), use_graph=False)
     a.run(6, use_graph=True)
     b.run(6, use_graph=True)
     assert same(snapshot(a, 2), snapshot(b, 2))
