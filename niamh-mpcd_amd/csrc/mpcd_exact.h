@@ -560,6 +560,7 @@ __global__ __launch_bounds__(kBktThreads) __attribute__((amdgpu_waves_per_eu(OP 
   __shared__ int32_t s_hkey[kBktHash], s_hcnt[kBktHash], s_hbase[kBktHash];
   __shared__ double s_ke[NT / 64];
   __shared__ double s_shn[3];
+  __shared__ int32_t s_id[CAP];  // particle ids, out of the registers during the cell phase
 
   // every record load of the thread first (a slot past the count reads the
   // last filled slot again; its values are not used)
@@ -601,6 +602,7 @@ __global__ __launch_bounds__(kBktThreads) __attribute__((amdgpu_waves_per_eu(OP 
     if (tid + j * NT < n) {
       const bool repaired = rid[j] < 0;
       rid[j] &= 0x7FFFFFFF;
+      s_id[tid + j * NT] = rid[j];
       if (!repaired) xshift_fwd<DIM>(X.L, X.sc, rq[j]);
       const int cn = xcell<DIM>(X.Li, X.pbc, X.cell_off, X.ny, X.nz, rq[j], true);
       if (R.gravflag)
@@ -735,7 +737,7 @@ __global__ __launch_bounds__(kBktThreads) __attribute__((amdgpu_waves_per_eu(OP 
     const int b2 = rk[j];
     if (b2 == -2 || b2 == -3) {
       const int spid = ONESP ? 0 : rsp[j];
-      if (!xlist_push(WL, rq[j], rv[j], rid[j], spid, T.rep, b2 == -2 ? 0 : 1)) {
+      if (!xlist_push(WL, rq[j], rv[j], s_id[tid + j * NT], spid, T.rep, b2 == -2 ? 0 : 1)) {
         atomicOr(&C.flags[0], kFlagPoison);
         atomicMin(&C.flags[1], (int)Dr.rng.step);
       }
@@ -748,7 +750,7 @@ __global__ __launch_bounds__(kBktThreads) __attribute__((amdgpu_waves_per_eu(OP 
       atomicMin(&C.flags[1], (int)Dr.rng.step);
       continue;
     }
-    xrec_store(O, (size_t)b2 * CAP + p2, ONESP, rq[j], rv[j], rid[j], ONESP ? 0 : rsp[j]);
+    xrec_store(O, (size_t)b2 * CAP + p2, ONESP, rq[j], rv[j], s_id[tid + j * NT], ONESP ? 0 : rsp[j]);
   }
   (void)wl;
   // exact kinetic-energy sum of the tile (integer-valued FP64)
@@ -841,9 +843,10 @@ __global__ void k_xbucketize(Ctx C, State<float> S, XState O, Draws D, const Buc
   const float sc[3] = {(float)C.shift[r * 3], (float)C.shift[r * 3 + 1], (float)C.shift[r * 3 + 2]};
   float qs[3] = {q[0], q[1], q[2]};
   int cn;
-  // a particle re-placed by the step that left it: already in this step's frame
+  // a particle re-placed by the step that left it: already in this step's
+  // frame.  The flag stays: after an overflow of this step the dense state,
+  // flags included, is where the step starts again.
   const bool pre = shifted[R.part_off + id] != 0;
-  if (pre) shifted[R.part_off + id] = 0;
   if (R.dim == 3) {
     if (!pre) xshift_fwd<3>(L, sc, qs);
     cn = xcell<3>(R.L, R.pbc, R.cell_off, R.ny, R.nz, qs, true);
@@ -873,7 +876,7 @@ __global__ void k_xbucketize(Ctx C, State<float> S, XState O, Draws D, const Buc
 // escape of the coming step's shifted bin, already in that step's shifted
 // frame) keeps its position and raises its particle's flag (by particle
 // index, so it survives any reordering of the dense state), which tells the
-// next step not to shift it.
+// next step not to shift it; every other record clears its particle's flag.
 __global__ void k_xdensify(Ctx C, XState I, State<float> S, const TileDev *__restrict__ bkts, int cap,
                            int cap_lim, int onesp, int32_t *shifted) {
   const TileDev T = bkts[blockIdx.x];
@@ -884,11 +887,10 @@ __global__ void k_xdensify(Ctx C, XState I, State<float> S, const TileDev *__res
     float q[3], v[3];
     int id, sp;
     xrec_load(I, o, onesp != 0, q, v, id, sp);
-    if (id < 0) {
-      id &= 0x7FFFFFFF;
-      shifted[R.part_off + id] = 1;
-    }
+    const int pre = id < 0 ? 1 : 0;
+    id &= 0x7FFFFFFF;
     const int s = R.part_off + id;
+    shifted[s] = pre;
     S.x[s] = q[0];
     S.y[s] = q[1];
     S.z[s] = q[2];

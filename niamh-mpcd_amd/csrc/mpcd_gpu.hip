@@ -2162,12 +2162,27 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
       eager();
       nsteps--;
     }
-    if (nsteps > 0) {
+    // after a bucket overflow the cell-sorted path takes the overflowed
+    // step; the run goes back to buckets when the stretch before held at
+    // least 16 steps (or it is the run's first overflow), else the
+    // cell-sorted path finishes it
+    bool first = true;
+    while (nsteps > 0) {
       int done = 0;
       st = c->precision == MPCD_FP64 ? bucket_run<double>(c, nsteps, use_graph, &done)
                                      : bucket_run<float>(c, nsteps, use_graph, &done);
       if (st) return st;
-      nsteps -= done;  // after a bucket overflow the dense path finishes the run
+      nsteps -= done;
+      if (nsteps == 0 || !(first || done >= 16)) break;
+      first = false;
+      st = ensure_dense(c);
+      if (st) return st;
+      eager();
+      nsteps--;
+      while (!c->exact && nsteps > 0 && !c->hist_ready) {
+        eager();
+        nsteps--;
+      }
     }
   }
   if (nsteps > 0) {
@@ -2198,8 +2213,23 @@ extern "C" mpcd_status mpcd_gpu_run(mpcd_gpu_ctx *c, int nsteps, int use_graph) 
   return check_launch(c);
 }
 
+// whether the buckets as built leave every replica's mean bucket population
+// mu at least 6 sqrt(mu) slots below the limit: a bucket's count fluctuates
+// like a Poisson count, and one overflow in a run costs a step on the
+// cell-sorted path and a re-bucketing
+static bool buckets_have_headroom(const mpcd_gpu_ctx *c) {
+  const int lim = bkt_limit(c);
+  for (int r = 0; r < c->nrep; r++) {
+    const double mu = c->brep[r].tc * mean_ppc(c->reps[r]);
+    if (lim - mu < 6.0 * sqrt(mu)) return false;
+  }
+  return true;
+}
+
 // the bucket fill of the bucketed layout: candidates in increasing order, so
-// the first allocates the most slots; a trial whose buckets overflow loses
+// the first allocates the most slots; a trial whose buckets overflow, or
+// (after the first) whose fill leaves too little headroom, loses.  Every
+// candidate runs its trial steps, so the number of steps taken is fixed.
 template <typename Real>
 static mpcd_status autotune_buckets(mpcd_gpu_ctx *c, int steps, double *ms_out) {
   const double fills[] = {0.4, 0.5, 0.6, 0.7, 0.75, 0.8};
@@ -2212,6 +2242,7 @@ static mpcd_status autotune_buckets(mpcd_gpu_ctx *c, int steps, double *ms_out) 
     // 1 + steps steps per candidate: the untimed one is a cell-sorted step
     // when the next bin is not built yet
     build_buckets(c, f);
+    const bool eligible = f == fills[0] || buckets_have_headroom(c);
     int untimed = 0;
     if (!c->exact && !c->hist_ready) run_step<Real>(c, 0);
     else untimed = 1;
@@ -2230,7 +2261,7 @@ static mpcd_status autotune_buckets(mpcd_gpu_ctx *c, int steps, double *ms_out) 
     if (st) break;
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, a, b));
-    if (done == steps + untimed && ms / steps < best) {
+    if (eligible && done == steps + untimed && ms / steps < best) {
       best = ms / steps;
       best_fill = f;
     }
