@@ -91,9 +91,11 @@ def run(M, precision, reps=None, steps=3000, sample_from=600, every=100, g=1e-3,
         V -= V.mean(axis=0)
         ctx.upload(r, Q, V)
     trial = 2
-    tile_ms = ctx.autotune(trial)  # 5 candidates x (1 + trial) real steps
-    # throughput: graph replays with nothing else in the timed region
-    ctx.run(2, use_graph=True)
+    tile_ms = ctx.autotune(trial)  # 6 bucket fills x (1 + trial) real steps
+    # throughput: graph replays with nothing else in the timed region (the
+    # warm-up records the graph: groups of six steps in the exact-sum form)
+    warm = 12
+    ctx.run(warm, use_graph=True)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.run(time_steps, use_graph=True)
@@ -105,7 +107,7 @@ def run(M, precision, reps=None, steps=3000, sample_from=600, every=100, g=1e-3,
     ctx.synchronize()
     kern = ctx.timing()
     ctx.profile(False)
-    done = 5 * (1 + trial) + 2 + time_steps + 4  # autotune trials, graph warm-up, timed stretch, profiled steps
+    done = 6 * (1 + trial) + warm + time_steps + 4  # autotune trials, graph warm-up, timed stretch, profiled steps
     prof = [[] for _ in reps]
     times = []
     nz = [L + 1 for L, _ in reps]

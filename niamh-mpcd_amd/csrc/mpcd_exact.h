@@ -298,8 +298,8 @@ __device__ __forceinline__ bool xdrift(const float boxf[6], const float L[3], bo
 template <int DIM>
 __device__ __forceinline__ void xthermal(const Draws &Dr, int rep, int pid, double sig, float rv[3]) {
   double n0[2], n1[2] = {0.0, 0.0};
-  Dr.rng.normal_pair(rep, P_NORMAL, (uint32_t)pid, 0, n0);
-  if (DIM == 3) Dr.rng.normal_pair(rep, P_NORMAL, (uint32_t)pid, 1, n1);
+  Dr.rng.normal_pair_turn(rep, P_NORMAL, (uint32_t)pid, 0, n0);
+  if (DIM == 3) Dr.rng.normal_pair_turn(rep, P_NORMAL, (uint32_t)pid, 1, n1);
   rv[0] = (float)(sig * n0[0]);
   rv[1] = (float)(sig * n0[1]);
   rv[2] = (DIM == 3) ? (float)(sig * n1[0]) : 0.0f;
@@ -348,9 +348,10 @@ __device__ __forceinline__ void xterms(double m, const float q[3], const float v
 template <int OP, int DIM, bool GHOST>
 __device__ __forceinline__ void xcell_terms(const RepDev &R, const WallHot *__restrict__ wh, const Draws &Dr, int rep,
                                             int c, int np_, double mass, const double *s, float cd[12]) {
+  // one division per cell: the sums' quantum over the cell mass
+  const double im = (mass > 0.0) ? kXQinv / mass : 0.0;
   double vcm[3] = {0.0, 0.0, 0.0};
-  if (mass > 0.0)
-    for (int d = 0; d < DIM; d++) vcm[d] = (s[d] * kXQinv) / mass;
+  for (int d = 0; d < DIM; d++) vcm[d] = s[d] * im;
   if (GHOST && R.anyphantom && np_ > 0) {
     // ghostPart (mpc.c:351-567): a cell cut by a phantom wall and holding
     // fewer than GnDNST particles gains the ghosts' momentum
@@ -383,7 +384,7 @@ __device__ __forceinline__ void xcell_terms(const RepDev &R, const WallHot *__re
     double RV[3] = {0.0, 0.0, 0.0};
     int CA = 0;
     if (R.rtech == MPCD_ARBAXIS) {
-      if (DIM == 3) Dr.rng.unitvec(rep, P_CELL, c, 0, RV);
+      if (DIM == 3) Dr.rng.unitvec_turn(rep, P_CELL, c, 0, RV);
     } else {
       CA = (int)((int)DIM * Dr.rng.uniform(rep, P_CELL, c, 0));
     }
@@ -416,14 +417,14 @@ __device__ __forceinline__ void xcell_terms(const RepDev &R, const WallHot *__re
     return;
   }
   double RS[3];
-  for (int d = 0; d < 3; d++) RS[d] = (d < DIM) ? (s[3 + d] * kXQinv) / mass : 0.0;
+  for (int d = 0; d < 3; d++) RS[d] = (d < DIM) ? s[3 + d] * im : 0.0;
   for (int d = 0; d < 3; d++) cd[3 + d] = (float)RS[d];
   if (OP != OP_RAT) return;
   // centre of mass, inertia about it (mpc.c:4209-4299, off-diagonals negated)
   // and angular momentum about it (mpc.c:1554-1618), from the moments about
   // the cell's corner
   double cm[3] = {0.0, 0.0, 0.0};
-  for (int d = 0; d < DIM; d++) cm[d] = (s[6 + d] * kXQinv) / mass;
+  for (int d = 0; d < DIM; d++) cm[d] = s[6 + d] * im;
   const double Sxx = s[9] * kXQinv - mass * cm[0] * cm[0], Syy = s[10] * kXQinv - mass * cm[1] * cm[1];
   const double Szz = s[11] * kXQinv - mass * cm[2] * cm[2];
   const double Sxy = s[12] * kXQinv - mass * cm[0] * cm[1], Sxz = s[13] * kXQinv - mass * cm[0] * cm[2];

@@ -265,6 +265,69 @@ MPCD_HD double u53(uint32_t a, uint32_t b) {
   return (double)(((uint64_t)a << 21) | (uint64_t)(b >> 11)) * (1.0 / 9007199254740992.0);
 }
 
+// sine and cosine of 2 pi u for u = n 2^-53 in [0, 1): the octant of u and
+// the position in it are exact in FP64 (8u, its fraction f and 1 - f), the
+// angle in the octant is g pi/4 with g = f or 1 - f, and Taylor polynomials of
+// degree 13 (sine) and 14 (cosine) on [0, pi/4] leave truncation errors below
+// 3e-14.  Each operation is a plain FP64 add or multiply in a fixed order, so
+// a C build without contraction gives the same bits (no library sin or cos,
+// whose last bits differ between implementations).
+MPCD_HD void sincos_turn(double u, double *sn, double *cs) {
+  const double t = 8.0 * u;
+  const int o = (int)t;
+  const double f = t - (double)o;
+  const double g = (o & 1) ? 1.0 - f : f;
+  const double th = g * 0.78539816339744830962;
+  const double x = th * th;
+  double p = 1.0 / 6227020800.0;
+  p = p * x + -1.0 / 39916800.0;
+  p = p * x + 1.0 / 362880.0;
+  p = p * x + -1.0 / 5040.0;
+  p = p * x + 1.0 / 120.0;
+  p = p * x + -1.0 / 6.0;
+  const double s = th + (th * x) * p;
+  double q = -1.0 / 87178291200.0;
+  q = q * x + 1.0 / 479001600.0;
+  q = q * x + -1.0 / 3628800.0;
+  q = q * x + 1.0 / 40320.0;
+  q = q * x + -1.0 / 720.0;
+  q = q * x + 1.0 / 24.0;
+  q = q * x + -0.5;
+  const double c = 1.0 + x * q;
+  // octants 1, 2, 5, 6 swap sine and cosine; 2-5 negate the cosine, 4-7 the sine
+  const bool swap = (((o + 1) >> 1) & 1) != 0;
+  const double a = swap ? s : c, b = swap ? c : s;
+  *cs = ((o + 2) & 4) ? -a : a;
+  *sn = (o & 4) ? -b : b;
+}
+
+// natural log of u in (0, 1] without a library log: u = m 2^e with m in
+// [1/sqrt 2, sqrt 2) (frexp and a doubling, both exact), log m = 2 atanh s
+// with s = (m - 1) / (m + 1), |s| < 0.172, from its series to s^19
+// (truncation below 2e-16 relative), plus e ln 2.  Plain FP64 operations in
+// a fixed order, as in sincos_turn.
+MPCD_HD double log_unit(double u) {
+  int e;
+  double m = frexp(u, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double x = s * s;
+  double p = 1.0 / 19.0;
+  p = p * x + 1.0 / 17.0;
+  p = p * x + 1.0 / 15.0;
+  p = p * x + 1.0 / 13.0;
+  p = p * x + 1.0 / 11.0;
+  p = p * x + 1.0 / 9.0;
+  p = p * x + 1.0 / 7.0;
+  p = p * x + 1.0 / 5.0;
+  p = p * x + 1.0 / 3.0;
+  p = p * x + 1.0;
+  return (double)e * 0.69314718055994530942 + 2.0 * (s * p);
+}
+
 struct Rng {
   uint64_t seed;
   uint32_t step;
@@ -299,6 +362,17 @@ struct Rng {
     out[0] = r * cos(6.283185307179586 * u2);
     out[1] = r * sin(6.283185307179586 * u2);
   }
+  // the same pair with log_unit and sincos_turn (the exact-sum form)
+  MPCD_HD void normal_pair_turn(uint32_t rep, int purpose, uint32_t key, uint32_t pair, double out[2]) const {
+    uint32_t w[4];
+    block(rep, purpose, key, pair, w);
+    double u1 = 1.0 - u53(w[0], w[1]);
+    double r = sqrt(-2.0 * log_unit(u1));
+    double sn, cs;
+    sincos_turn(u53(w[2], w[3]), &sn, &cs);
+    out[0] = r * cs;
+    out[1] = r * sn;
+  }
   MPCD_HD void unitvec(uint32_t rep, int purpose, uint32_t key, uint32_t k, double v[3]) const {
     uint32_t w[4];
     block(rep, purpose, key, k, w);
@@ -307,6 +381,18 @@ struct Rng {
     double s = sqrt((1.0 - z) * (1.0 + z));
     v[0] = s * cos(phi);
     v[1] = s * sin(phi);
+    v[2] = z;
+  }
+  // the same axis with the sine and cosine of sincos_turn (the exact-sum form)
+  MPCD_HD void unitvec_turn(uint32_t rep, int purpose, uint32_t key, uint32_t k, double v[3]) const {
+    uint32_t w[4];
+    block(rep, purpose, key, k, w);
+    double z = 1.0 - 2.0 * u53(w[0], w[1]);
+    double sn, cs;
+    sincos_turn(u53(w[2], w[3]), &sn, &cs);
+    double s = sqrt((1.0 - z) * (1.0 + z));
+    v[0] = s * cs;
+    v[1] = s * sn;
     v[2] = z;
   }
 };

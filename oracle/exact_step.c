@@ -101,9 +101,9 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
         if (op != 0) {
             const double sig = sqrt(p->kbt / m);
             const uint32_t key = (uint32_t)(ids ? ids[i] : i);
-            rv[0] = (float)(sig * orc_philox_normal(seed, replica, step, XP_NORMAL, key, 0));
-            rv[1] = (float)(sig * orc_philox_normal(seed, replica, step, XP_NORMAL, key, 1));
-            rv[2] = (dim == 3) ? (float)(sig * orc_philox_normal(seed, replica, step, XP_NORMAL, key, 2)) : 0.0f;
+            rv[0] = (float)(sig * orc_philox_normal_turn(seed, replica, step, XP_NORMAL, key, 0));
+            rv[1] = (float)(sig * orc_philox_normal_turn(seed, replica, step, XP_NORMAL, key, 1));
+            rv[2] = (dim == 3) ? (float)(sig * orc_philox_normal_turn(seed, replica, step, XP_NORMAL, key, 2)) : 0.0f;
             memcpy(&rt[3 * i], rv, sizeof rv);
         }
         const double mm = onesp ? 1.0 : m;
@@ -144,14 +144,15 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
         for (int j = 0; j < ns; j++) sm[j] = sum[(size_t)c * ns + j];
         if (onesp)
             for (int j = 0; j < (op == 0 ? 3 : ns); j++) sm[j] *= p->mass[0];
-        for (int d = 0; d < dim; d++) vcm[d] = (sm[d] * XQINV) / mass;
+        const double im = XQINV / mass; /* mass > 0: the cell holds a particle */
+        for (int d = 0; d < dim; d++) vcm[d] = sm[d] * im;
         for (int d = 0; d < 3; d++) o[d] = (float)vcm[d];
         if (np_ <= 1) continue;
         if (op == 0) {
             double RV[3] = {0.0, 0.0, 0.0}, M[9];
             int CA = 0;
             if (p->rtech == 0) {
-                if (dim == 3) orc_philox_unitvec(seed, replica, step, XP_CELL, (uint32_t)(c + p->cell_off), 0, RV);
+                if (dim == 3) orc_philox_unitvec_turn(seed, replica, step, XP_CELL, (uint32_t)(c + p->cell_off), 0, RV);
             } else {
                 CA = (int)((int)dim * orc_philox_uniform(seed, replica, step, XP_CELL, (uint32_t)(c + p->cell_off), 0));
             }
@@ -186,10 +187,10 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
             for (int k = 0; k < 9; k++) o[3 + k] = (float)M[k];
             continue;
         }
-        for (int d = 0; d < 3; d++) o[3 + d] = (float)((d < dim) ? (sm[3 + d] * XQINV) / mass : 0.0);
+        for (int d = 0; d < 3; d++) o[3 + d] = (float)((d < dim) ? sm[3 + d] * im : 0.0);
         if (op != 2) continue;
         double cm[3] = {0.0, 0.0, 0.0};
-        for (int d = 0; d < dim; d++) cm[d] = (sm[6 + d] * XQINV) / mass;
+        for (int d = 0; d < dim; d++) cm[d] = sm[6 + d] * im;
         const double Sxx = sm[9] * XQINV - mass * cm[0] * cm[0], Syy = sm[10] * XQINV - mass * cm[1] * cm[1];
         const double Szz = sm[11] * XQINV - mass * cm[2] * cm[2];
         const double Sxy = sm[12] * XQINV - mass * cm[0] * cm[1], Sxz = sm[13] * XQINV - mass * cm[0] * cm[2];

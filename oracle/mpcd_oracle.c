@@ -400,6 +400,94 @@ void orc_philox_unitvec(uint64_t seed, uint32_t replica, uint32_t step, int purp
     philox_unitvec(seed, replica, step, purpose, key, k, v);
 }
 
+/* sine and cosine of 2 pi u for u = n 2^-53 in [0, 1) without a library
+   sine: octant of u (8u, its fraction f and 1 - f are exact), angle g pi/4 in
+   the octant, Taylor polynomials of degree 13 and 14 on [0, pi/4].  Plain
+   FP64 adds and multiplies in a fixed order (no contraction under -std=c99),
+   so the exact-sum form's device code gives the same bits. */
+void orc_sincos_turn(double u, double *sn, double *cs)
+{
+    const double t = 8.0 * u;
+    const int o = (int)t;
+    const double f = t - (double)o;
+    const double g = (o & 1) ? 1.0 - f : f;
+    const double th = g * 0.78539816339744830962;
+    const double x = th * th;
+    double p = 1.0 / 6227020800.0, q = -1.0 / 87178291200.0, s, c, a, b;
+    int swap;
+    p = p * x + -1.0 / 39916800.0;
+    p = p * x + 1.0 / 362880.0;
+    p = p * x + -1.0 / 5040.0;
+    p = p * x + 1.0 / 120.0;
+    p = p * x + -1.0 / 6.0;
+    s = th + (th * x) * p;
+    q = q * x + 1.0 / 479001600.0;
+    q = q * x + -1.0 / 3628800.0;
+    q = q * x + 1.0 / 40320.0;
+    q = q * x + -1.0 / 720.0;
+    q = q * x + 1.0 / 24.0;
+    q = q * x + -0.5;
+    c = 1.0 + x * q;
+    swap = ((o + 1) >> 1) & 1;
+    a = swap ? s : c;
+    b = swap ? c : s;
+    *cs = ((o + 2) & 4) ? -a : a;
+    *sn = (o & 4) ? -b : b;
+}
+
+/* natural log of u in (0, 1] without a library log: u = m 2^e with m in
+   [1/sqrt 2, sqrt 2) (frexp and a doubling, exact), log m = 2 atanh s with
+   s = (m - 1) / (m + 1) from its series to s^19, plus e ln 2; the same
+   operations in the same order as the exact-sum form's device code */
+double orc_log_unit(double u)
+{
+    int e;
+    double m = frexp(u, &e), s, x, p;
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    s = (m - 1.0) / (m + 1.0);
+    x = s * s;
+    p = 1.0 / 19.0;
+    p = p * x + 1.0 / 17.0;
+    p = p * x + 1.0 / 15.0;
+    p = p * x + 1.0 / 13.0;
+    p = p * x + 1.0 / 11.0;
+    p = p * x + 1.0 / 9.0;
+    p = p * x + 1.0 / 7.0;
+    p = p * x + 1.0 / 5.0;
+    p = p * x + 1.0 / 3.0;
+    p = p * x + 1.0;
+    return (double)e * 0.69314718055994530942 + 2.0 * (s * p);
+}
+
+/* orc_philox_normal with orc_log_unit and orc_sincos_turn */
+double orc_philox_normal_turn(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k)
+{
+    uint32_t w[4];
+    double r, sn, cs;
+    philox_block(seed, replica, step, purpose, key, k >> 1, w);
+    r = sqrt(-2.0 * orc_log_unit(1.0 - u53(w[0], w[1])));
+    orc_sincos_turn(u53(w[2], w[3]), &sn, &cs);
+    return (k & 1) ? r * sn : r * cs;
+}
+
+/* philox_unitvec with the sine and cosine of orc_sincos_turn */
+void orc_philox_unitvec_turn(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k,
+                             double v[3])
+{
+    uint32_t w[4];
+    double z, s, sn, cs;
+    philox_block(seed, replica, step, purpose, key, k, w);
+    z = 1.0 - 2.0 * u53(w[0], w[1]);
+    orc_sincos_turn(u53(w[2], w[3]), &sn, &cs);
+    s = sqrt((1.0 - z) * (1.0 + z));
+    v[0] = s * cs;
+    v[1] = s * sn;
+    v[2] = z;
+}
+
 static const double *keyed_base(const orc_draws *d, int purpose, int *stride)
 {
     switch (purpose) {

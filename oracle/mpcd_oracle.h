@@ -158,6 +158,11 @@ double   orc_philox_normal(uint64_t seed, uint32_t replica, uint32_t step, int p
 void     orc_inv3(double inv[3][3], const double m[3][3]);
 void     orc_philox_unitvec(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k,
                             double v[3]);
+void     orc_philox_unitvec_turn(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key,
+                                 uint32_t k, double v[3]);
+void     orc_sincos_turn(double u, double *sn, double *cs);
+double   orc_log_unit(double u);
+double   orc_philox_normal_turn(uint64_t seed, uint32_t replica, uint32_t step, int purpose, uint32_t key, uint32_t k);
 void     orc_rodrigues(double *vec, double *axis, double theta);
 
 #endif

@@ -112,6 +112,13 @@ def lib():
         L.orc_philox_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32]
         L.orc_philox_normal.restype = C.c_double
         L.orc_philox_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32]
+        for f in ("orc_philox_unitvec", "orc_philox_unitvec_turn"):
+            getattr(L, f).argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, dp]
+        L.orc_sincos_turn.argtypes = [C.c_double, dp, dp]
+        L.orc_log_unit.restype = C.c_double
+        L.orc_log_unit.argtypes = [C.c_double]
+        L.orc_philox_normal_turn.restype = C.c_double
+        L.orc_philox_normal_turn.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32]
         L.orc_rodrigues.argtypes = [dp, dp, C.c_double]
         _lib = L
     return _lib
