@@ -125,11 +125,9 @@ int ckpt_read(const char *path, mpcd_gpu_params *p, mpcd_ckpt *ck, char *err, si
         rdn(&R, v, 20);  /* L[3], G[3], A[3], AINV[3], P[4], R, B[3] */
         memcpy(W->L, v, sizeof W->L);
         memcpy(W->A, v + 6, sizeof W->A);
+        memcpy(W->P, v + 12, sizeof W->P);
         W->R = v[16];
-        int naxis = 0;
-        for (int d = 0; d < 3; d++) naxis += fabs(W->A[d]) >= 1e-8;
-        const int planar = naxis == 1 && fabs(v[17]) <= 1e-8 && fabs(v[12] - 1) <= 1e-8 && fabs(v[13] - 1) <= 1e-8 &&
-                           fabs(v[14] - 1) <= 1e-8;
+        memcpy(W->B, v + 17, sizeof W->B);
         const int turned = fabs(O[0]) + fabs(O[1]) + fabs(O[2]) > 0.0;
         rdn(&R, v, 14);  /* DN, DT, DVN, DVT, DVxyz[3], MVN, MVT, MUN, MUT, MUxyz[3] */
         W->DN = v[0];
@@ -146,12 +144,17 @@ int ckpt_read(const char *path, mpcd_gpu_params *p, mpcd_ckpt *ck, char *err, si
         memcpy(W->DUxyz, v, sizeof W->DUxyz);
         const int dsplc = (int)v[4];
         W->INV = (int)v[5];
-        skip(&R, 17 + 5 + 6);  /* W, VOL, Q_old, O_old, I; PLANAR, REORIENT, ABS, ROTSYMM; dV, dL */
+        skip(&R, 17);  /* W, VOL, Q_old, O_old, I */
+        rdn(&R, v, 5); /* PLANAR, REORIENT, ABS, ROTSYMM */
+        W->ABS = (int)v[2];
+        const int sym4 = fabs(v[3] - 4.0) <= 1e-8 && fabs(v[4] - 4.0) <= 1e-8;
+        skip(&R, 6);   /* dV, dL */
         rdn(&R, v, CKPT_MAXSPECI + 2);
         W->inter_mask = 0;
         for (int s = 0; s < p->nspec; s++)
             if (v[s] == 1.0) W->inter_mask |= 1 << s;  /* BCON = 1: interacts (bc.c:1263) */
-        if (R.ok && (!planar || turned || dsplc)) FAIL("checkpoint: boundary %d is not a fixed axis-aligned plane", w);
+        if (R.ok && (turned || dsplc || !sym4))
+            FAIL("checkpoint: boundary %d is turned, displaceable or not four-fold symmetric (outside this path)", w);
     }
     if (!R.ok) FAIL("checkpoint %s: the file ends or breaks before the particles", path);
     ck->Q = (double *)calloc(3 * (size_t)ck->n, sizeof(double));
@@ -239,15 +242,20 @@ void ckpt_write(FILE *f, const mpcd_gpu_params *p, const mpcd_ckpt *ck) {
         fprintf(f, "%d %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", W->COLL_TYPE, W->PHANTOM, -1.0, W->Q[0], W->Q[1],
                 W->Q[2], W->V[0], W->V[1], W->V[2], 0.0, 0.0, 0.0);
         fprintf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", W->L[0], W->L[1],
-                W->L[2], 0.0, 0.0, 0.0, W->A[0], W->A[1], W->A[2], ainv[0], ainv[1], ainv[2], 1.0, 1.0, 1.0, 1.0, W->R,
-                0.0, 0.0, 0.0);
+                W->L[2], 0.0, 0.0, 0.0, W->A[0], W->A[1], W->A[2], ainv[0], ainv[1], ainv[2], W->P[0], W->P[1],
+                W->P[2], W->P[3], W->R, W->B[0], W->B[1], W->B[2]);
         fprintf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf\n", W->DN, W->DT, W->DVN, W->DVT,
                 W->DVxyz[0], W->DVxyz[1], W->DVxyz[2], W->MVN, W->MVT, W->MUN, W->MUT, W->MUxyz[0], W->MUxyz[1],
                 W->MUxyz[2]);
         fprintf(f, "%lf %lf %lf %lf %d %d %lf\n", W->DUxyz[0], W->DUxyz[1], W->DUxyz[2], 1.0, 0, W->INV, 1.0);
         for (int k = 0; k < 17; k++) fprintf(f, k ? " %lf" : "%lf", 0.0);  /* W, VOL, Q_old, O_old, I */
         fprintf(f, "\n");
-        fprintf(f, "%d %d %d %lf %lf\n", 1, 0, 0, 4.0, 4.0);
+        /* PLANAR (read.c:1423-1434), REORIENT, ABS, ROTSYMM */
+        int naxis = 0;
+        for (int d = 0; d < 3; d++) naxis += fabs(W->A[d]) >= 1e-8;
+        const int plane = naxis == 1 && fabs(W->B[0]) <= 1e-8 && fabs(W->P[0] - 1) <= 1e-8 &&
+                          fabs(W->P[1] - 1) <= 1e-8 && fabs(W->P[2] - 1) <= 1e-8;
+        fprintf(f, "%d %d %d %lf %lf\n", plane, 0, W->ABS, 4.0, 4.0);
         fprintf(f, "%lf %lf %lf %lf %lf %lf\n", 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
         /* species slots past nspec, MD and swimmers keep the default BCON = 1 (read.c:1375) */
         for (int j = 0; j < CKPT_MAXSPECI + 2; j++) fprintf(f, "%i ", j < p->nspec ? (W->inter_mask >> j) & 1 : 1);

@@ -10,10 +10,11 @@
  * for the options of this path: domain, kbt, dt, simSteps, warmUp, galInv,
  * tsTech, rTech/collOp, lc, tau, rotAng, grav, seed, domainWalls, species
  * (mass, pop/dens, vDist, rfc, tumble, shearSusc, act, mfpot, sigWidth,
- * sigPos, minActRatio, damp) and planar BC entries (aInv, Q, P, R, DN, DT,
- * DVN, DVT, DVxyz, MVN, MVT, MUN, MUT, MUxyz, DUxyz, V, L, inv, phantom,
- * colType, interSRD, homeotropic, planar).  Options outside the path (MD,
- * swimmers, curved or wavy walls) stop with a message, as the reference
+ * sigPos, minActRatio, damp) and BC entries, planar, curved or wavy (aInv,
+ * Q, P, wavy, abs, rotSym 4 4, R, DN, DT, DVN, DVT, DVxyz, MVN, MVT, MUN,
+ * MUT, MUxyz, DUxyz, V, L, inv, phantom, colType, interSRD, homeotropic,
+ * planar).  Options outside the path (MD,
+ * swimmers, mobile walls) stop with a message, as the reference
  * stops on an unknown option.
  *
  * rFrame (default 1) removes the net momentum of the initial state
@@ -85,6 +86,87 @@ static double rng_u(void) { return (double)(rng_next() >> 11) * (1.0 / 900719925
 static double rng_n(void) {
     double u1 = 1.0 - rng_u(), u2 = rng_u();
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/* integer powers by products, other exponents by pow (smrtPow, mtools.c:38-50) */
+static double host_pow(double x, double y) {
+    if (y - (int)y == 0 && y <= 10) {
+        double r = 1;
+        for (int i = 0; i < y; i++) r *= x;
+        return r;
+    }
+    return pow(x, y);
+}
+
+/* the surface function of a wall at pos (calcW with its wavy term,
+ * bc.c:55-150), W < 0 inside the wall; a curved wall acts through its
+ * periodic image nearest pos along the box's axes (shiftBC, bc.c:389-401).
+ * Used to place the initial particles and to estimate the accessible
+ * volume, which are the host's own; the step itself evaluates walls on the
+ * GPU. */
+static double host_wall_w(const mpcd_gpu_wall *w, const double *pos, int dim, const int L[3]) {
+    double q[3] = {w->Q[0], w->Q[1], w->Q[2]}, Wv = 0.0, W1 = 0.0, W2 = 0.0;
+    /* shiftBC moves a surface whose first two exponents both differ from 1 */
+    const int shifted = fabs(w->P[0] - 1) > 1e-8 && fabs(w->P[1] - 1) > 1e-8;
+    for (int d = 0; d < dim && shifted; d++) {
+        if (pos[d] - q[d] >= 0.5 * L[d]) q[d] += L[d];
+        else if (pos[d] - q[d] <= -0.5 * L[d]) q[d] -= L[d];
+    }
+    for (int d = 0; d < dim; d++) {
+        double t = w->A[d] * (pos[d] - q[d]);
+        Wv += host_pow(w->ABS ? fabs(t) : t, w->P[d]);
+    }
+    Wv -= host_pow(w->ABS ? fabs(w->R) : w->R, w->P[3]);
+    if (fabs(w->B[0]) > 1e-8) {
+        int curved = 0, ell = 1, f = -1;
+        for (int d = 0; d < dim; d++) {
+            curved += fabs(w->P[d] - 1) > 1e-8;
+            if (fabs(w->P[d] - 2) > 1e-8 || fabs(w->A[d]) <= 1e-8) ell = 0;
+            if (fabs(w->A[d]) <= 1e-8) f = d;
+        }
+        if (!curved) {
+            W1 = (w->A[1] * pos[0] - w->A[0] * pos[1]) / sqrt(w->A[0] * w->A[0] + w->A[1] * w->A[1]);
+        } else if (ell) {
+            W1 = atan2(w->A[1] * (pos[1] - q[1]), w->A[0] * (pos[0] - q[0]));
+            if (dim > 2)
+                W2 = atan2(sqrt(pow(w->A[0] * (pos[0] - q[0]), 2) + pow(w->A[1] * (pos[1] - q[1]), 2)),
+                           w->A[2] * (pos[2] - q[2]));
+        } else if (dim > 2 && f >= 0) {
+            const int k = (f + 1) % 3, l = (f + 2) % 3;
+            W1 = atan2(w->A[l] * (pos[l] - q[l]), w->A[k] * (pos[k] - q[k]));
+        }
+        Wv += w->B[0] * cos(w->B[1] * W1) * cos(w->B[2] * W2);
+    }
+    return w->INV ? -Wv : Wv;
+}
+
+/* a position no wall excludes (replacePos_WithCheck, init.c:1462: W <= TOL
+ * inside) */
+static int host_position_free(const mpcd_gpu_params *p, const double *pos) {
+    for (int k = 0; k < p->nwall; k++)
+        if (host_wall_w(&p->wall[k], pos, p->dim, p->L) <= 1e-7) return 0;
+    return 1;
+}
+
+/* accessible volume of species s (accessibleVolume, init.c:954-990): Monte
+ * Carlo over the box with the walls the species interacts with, NUMMC = 1000
+ * samples per unit volume as the reference takes, at most 2^22 */
+static double host_accessible_volume(const mpcd_gpu_params *p, int s) {
+    const double cv = (double)p->L[0] * p->L[1] * (p->dim == 3 ? p->L[2] : 1);
+    double nd = 1000.0 * cv;
+    if (nd > 4194304.0) nd = 4194304.0;
+    const long n = (long)nd;
+    long fails = 0;
+    for (long i = 0; i < n; i++) {
+        double pos[3] = {0, 0, 0};
+        for (int d = 0; d < p->dim; d++) pos[d] = rng_u() * p->L[d];
+        for (int k = 0; k < p->nwall; k++)
+            if ((p->wall[k].inter_mask >> s & 1) && host_wall_w(&p->wall[k], pos, p->dim, p->L) < 0.0) {
+                fails++;
+                break;
+            }
+    }
+    return cv * (1.0 - (double)fails / (double)n);
 }
 
 static void die(const char *fmt, const char *arg) {
@@ -287,7 +369,7 @@ static void reject_unsupported(const json *J) {
     if (json_int(J, "nSwim", 0) > 0) die("%s", "\"nSwim\": swimmers are outside the solvent path");
 }
 
-/* BC entry of the JSON (read.c:1136-1440) as a planar wall */
+/* BC entry of the JSON (read.c:1136-1440): planes, curved and wavy surfaces */
 static void read_wall(const json *o, int nspec, mpcd_gpu_wall *w, int index) {
     char msg[64];
     memset(w, 0, sizeof *w);
@@ -301,16 +383,22 @@ static void read_wall(const json *o, int nspec, mpcd_gpu_wall *w, int index) {
         snprintf(msg, sizeof msg, "%d", index);
         die("BC %s: P must have 4 elements", msg);
     }
-    double B[3] = {0, 0, 0};
+    double B[3] = {0, 0, 0}, rs[2] = {4, 4};
     json_nums(o, "wavy", B, 3);
-    int naxis = 0;
-    for (int j = 0; j < 3; j++) naxis += fabs(w->A[j]) >= 1e-8;
-    const int planar = fabs(B[0]) <= 1e-8 && fabs(P[0] - 1) <= 1e-8 && fabs(P[1] - 1) <= 1e-8 &&
-                       fabs(P[2] - 1) <= 1e-8 && naxis == 1;
-    if (!planar) {
+    /* the general wall engine takes immobile walls of the four-fold symmetry
+       (read.c:1268-1284,1370) */
+    json_nums(o, "rotSym", rs, 2);
+    if (fabs(rs[0] - 4) > 1e-8 || fabs(rs[1] - 4) > 1e-8) {
         snprintf(msg, sizeof msg, "%d", index);
-        die("BC %s is not an axis-aligned plane (curved and wavy walls are outside this path)", msg);
+        die("BC %s: rotSym other than 4 4 is outside this path", msg);
     }
+    if (json_int(o, "dsplc", 0)) {
+        snprintf(msg, sizeof msg, "%d", index);
+        die("BC %s: displaceable (mobile) walls are outside this path", msg);
+    }
+    memcpy(w->P, P, sizeof P);
+    memcpy(w->B, B, sizeof B);
+    w->ABS = json_int(o, "abs", 0);
     if (json_nums(o, "Q", v3, 3) == 3) memcpy(w->Q, v3, sizeof v3);
     if (json_nums(o, "V", v3, 3) == 3) memcpy(w->V, v3, sizeof v3);
     if (json_nums(o, "L", v3, 3) == 3) memcpy(w->L, v3, sizeof v3);
@@ -457,13 +545,14 @@ int main(int argc, char **argv) {
     if (p.nspec > MPCD_MAXSPEC) die("%s", "too many species");
     const double vol = (double)p.L[0] * p.L[1] * (dim == 3 ? p.L[2] : 1);
     int64_t pop[MPCD_MAXSPEC], ntot = 0;
+    double dens[MPCD_MAXSPEC];
     int vdist[MPCD_MAXSPEC];
     for (int s = 0; s < p.nspec; s++) {
         const json *o = (spa && spa->type == JSON_ARR && spa->n > 0) ? &spa->items[s] : NULL;
         mpcd_gpu_species *sp = &p.sp[s];
         sp->mass = json_num(o, "mass", 1.0);
-        const double dens = json_num(o, "dens", -1.0);
-        pop[s] = dens < 0 ? (int64_t)json_int(o, "pop", (int)(vol * 20)) : (int64_t)(vol * dens);
+        dens[s] = json_num(o, "dens", -1.0);
+        pop[s] = dens[s] < 0 ? (int64_t)json_int(o, "pop", (int)(vol * 20)) : 0;
         vdist[s] = json_int(o, "vDist", 0);
         sp->vdist = vdist[s];
         sp->rfc = json_num(o, "rfc", 0.01);
@@ -475,9 +564,7 @@ int main(int argc, char **argv) {
         sp->sigpos = json_num(o, "sigPos", sp->sigwidth);
         sp->minactratio = json_num(o, "minActRatio", 0.0);
         sp->damp = json_num(o, "damp", 0.0);
-        ntot += pop[s];
     }
-    if (ntot <= 0 && !ckpt_in[0]) die("%s", "no particles");
 
     /* boundaries: the BC list, then the domain walls of domainWalls = 1
      * (periodic) or 2 (solid) appended after it; without a BC list the
@@ -496,6 +583,12 @@ int main(int argc, char **argv) {
         if (p.nwall + 2 * dim > MPCD_MAXWALL) die("%s", "too many BC entries with the domain walls");
         p.nwall += domain_walls(dim, p.L, dwalls, p.nspec, p.wall + p.nwall);
     }
+    /* a density sets the population over the accessible volume (read.c:1708-1716) */
+    for (int s = 0; s < p.nspec; s++) {
+        if (dens[s] >= 0) pop[s] = (int64_t)(host_accessible_volume(&p, s) * dens[s]);
+        ntot += pop[s];
+    }
+    if (ntot <= 0 && !ckpt_in[0]) die("%s", "no particles");
 
     /* recovering a checkpoint: its inputs, particles, loop counters and
      * output flags replace the input file's; the warm-up is skipped and the
@@ -559,10 +652,11 @@ int main(int argc, char **argv) {
             const mpcd_gpu_wall *W = &p.wall[w];
             printf("%s{\"A\": [%.17g, %.17g, %.17g], \"Q\": [%.17g, %.17g, %.17g], \"R\": %.17g, \"DN\": %.17g, "
                    "\"MVN\": %.17g, \"MVT\": %.17g, \"MUN\": %.17g, \"MUT\": %.17g, \"MUxyz\": [%.17g, %.17g, %.17g], "
-                   "\"V\": [%.17g, %.17g, %.17g], "
-                   "\"INV\": %d, \"PHANTOM\": %d, \"inter_mask\": %d}",
+                   "\"V\": [%.17g, %.17g, %.17g], \"P\": [%.17g, %.17g, %.17g, %.17g], \"B\": [%.17g, %.17g, %.17g], "
+                   "\"ABS\": %d, \"INV\": %d, \"PHANTOM\": %d, \"inter_mask\": %d}",
                    w ? ", " : "", W->A[0], W->A[1], W->A[2], W->Q[0], W->Q[1], W->Q[2], W->R, W->DN, W->MVN, W->MVT,
-                   W->MUN, W->MUT, W->MUxyz[0], W->MUxyz[1], W->MUxyz[2], W->V[0], W->V[1], W->V[2], W->INV, W->PHANTOM, W->inter_mask);
+                   W->MUN, W->MUT, W->MUxyz[0], W->MUxyz[1], W->MUxyz[2], W->V[0], W->V[1], W->V[2], W->P[0], W->P[1],
+                   W->P[2], W->P[3], W->B[0], W->B[1], W->B[2], W->ABS, W->INV, W->PHANTOM, W->inter_mask);
         }
         printf("]}\n");
         json_free(J);
@@ -601,7 +695,12 @@ int main(int argc, char **argv) {
             for (int64_t k = 0; k < pop[s]; k++, i++) {
                 const double m = p.sp[s].mass, sig = sqrt(p.kbt / m);
                 spid[i] = s;
-                for (int d = 0; d < 3; d++) Qr[3 * i + d] = d < dim ? rng_u() * p.L[d] : 0.0;
+                /* uniform over the space no wall excludes (replacePos_WithCheck, init.c:1462) */
+                int tries = 0;
+                do {
+                    if (++tries > 1000000) die("%s", "no free position for a particle: the walls fill the box");
+                    for (int d = 0; d < 3; d++) Qr[3 * i + d] = d < dim ? rng_u() * p.L[d] : 0.0;
+                } while (!host_position_free(&p, Qr + 3 * i));
                 if (vdist[s] == 4) {
                     for (int d = 0; d < dim; d++) Vr[3 * i + d] = sig * rng_n();
                 } else {

@@ -8,6 +8,7 @@ interaction masks, densities, repeated "comment" keys and the options this
 path rejects.  GPU: the host runs a periodic box, whose momentum the body force sets in
 closed form, and a channel between bounce-back walls."""
 import json
+import math
 import os
 import subprocess
 
@@ -152,9 +153,9 @@ def test_rtech_alone(tmp_path):
     (dict(domain=[8, 8], mdIn="md.inp"), "mdIn"),
     (dict(domain=[8, 8], checkpointIn="c.dat"), "cannot open the checkpoint"),
     (dict(domain=[8, 8], nSwim=2), "nSwim"),
-    (dict(domain=[8, 8], BC=[dict(aInv=[0.25, 0.25, 0], Q=[4, 4, 0], P=[2, 2, 1, 1], R=4)]), "not an axis-aligned"),
-    (dict(domain=[8, 8], BC=[dict(aInv=[1, 0, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], wavy=[0.5, 1, 0])]),
-     "not an axis-aligned"),
+    (dict(domain=[8, 8], BC=[dict(aInv=[0.25, 0.25, 0], Q=[4, 4, 0], P=[2, 2, 1, 1], R=4, dsplc=1)]),
+     "displaceable"),
+    (dict(domain=[8, 8], BC=[dict(aInv=[1, 0, 0], Q=[0, 0, 0], P=[1, 1, 1, 1], rotSym=[2, 2])]), "rotSym"),
     (dict(domain=[8]), "domain"),
 ])
 def test_rejected_options(tmp_path, cfg, msg):
@@ -526,8 +527,9 @@ def test_host_energy_output(tmp_path, lc):
 
 
 def test_checkpoint_outside_the_path_is_refused(tmp_path):
-    """A checkpoint with MD coupling, swimmers, a wavy wall or a truncated
-    particle list stops the host with a message naming the cause."""
+    """A checkpoint with MD coupling, swimmers, a displaceable or turned wall,
+    or a truncated particle list stops the host with a message naming the
+    cause; a wavy wall is read with its wavy terms."""
     text = open(os.path.join(GOLDEN, "ckpt_lc2d_channel.dat")).read()
     fields = ckpt_fields(text)
     names = [k for k, _ in fields]
@@ -542,10 +544,78 @@ def test_checkpoint_outside_the_path_is_refused(tmp_path):
 
     cases = [(variant({"MD_mode": "1"}), "MD coupling"),
              (variant({"NS": "2"}), "swimmers"),
-             (variant({"w0.B0": "0.500000"}), "not a fixed axis-aligned plane"),
+             (variant({"w0.DSPLC": "1"}), "displaceable"),
+             (variant({"w0.O1": "0.500000"}), "turned"),
              (variant({}, cut=names.index("p5.Q0")), "ends or breaks")]
     for k, (body, msg) in enumerate(cases):
         ck = tmp_path / f"ck{k}.dat"
         ck.write_text(body)
         r = run_dump(tmp_path, {"checkpointIn": str(ck)})
         assert r.returncode != 0 and msg in r.stderr, (msg, r.stderr)
+    ck = tmp_path / "wavy.dat"
+    ck.write_text(variant({"w0.B0": "0.500000", "w0.B1": "0.250000", "w0.P2": "2.000000"}))
+    r = run_dump(tmp_path, {"checkpointIn": str(ck)})
+    assert r.returncode == 0, r.stderr
+    w0 = json.loads(r.stdout)["walls"][0]
+    assert w0["B"] == [0.5, 0.25, 0.0] and w0["P"][2] == 2.0
+
+
+def test_curved_walls_pass_through(tmp_path):
+    """a cylinder and a wavy plane reach the library with their exponents,
+    wavy terms and |.| flag; a density fills the accessible volume
+    (accessibleVolume, init.c:954-990): the box minus the disc"""
+    cyl = dict(aInv=[1, 1, 0], Q=[8, 8, 0], P=[2, 2, 1, 2], R=4, inv=1, abs=0)
+    wavy = dict(aInv=[0, 1, 0], Q=[0, 2, 0], P=[1, 1, 1, 1], R=0, wavy=[0.5, 0.25, 0])
+    d = dump(tmp_path, dict(domain=[16, 16], BC=[cyl, wavy], domainWalls=1, species=[dict(dens=10)]))
+    w0, w1 = d["walls"][0], d["walls"][1]
+    assert w0["P"] == [2, 2, 1, 2] and w0["INV"] == 1 and w0["ABS"] == 0 and w0["R"] == 4
+    assert w1["B"] == [0.5, 0.25, 0] and w1["P"] == [1, 1, 1, 1]
+    # inside the inverted cylinder of radius 4 only, so the accessible area is
+    # about pi 4^2 = 50.3 of 256 minus the part below the wavy plane near y = 2
+    pop = d["species"][0]["pop"]
+    assert 0 < pop < 10 * 16 * 16
+    ref = 10 * math.pi * 16.0
+    assert abs(pop - ref) < 0.05 * ref, (pop, ref)
+
+
+def _ckpt_particles(path):
+    f = ckpt_fields(open(path).read())
+    d = dict(f)
+    n = int(d["GPOP"])
+    return np.array([[float(d[f"p{i}.Q{k}"]) for k in range(3)] for i in range(n)])
+
+
+@pytest.mark.gpu
+def test_host_runs_curved_walls(tmp_path):
+    """pipeFlow- and wavyWalls-shaped inputs through the host and the
+    general wall engine: the particles are placed in the space the walls leave
+    (a density fills the accessible volume), stay there after 20 steps (every
+    checkpointed position inside the pipe, or between the wavy walls, to the
+    file's six decimals) and the body force drives a flow along the pipe."""
+    pipe = {"domain": [24, 12, 12], "domainWalls": 1, "simSteps": 20, "collOp": 2, "seed": 8, "checkpointOut": 20,
+            "grav": [0.01, 0, 0], "species": [{"dens": 10}],
+            "BC": [{"Q": [0, 6, 6], "aInv": [0, 1, 1], "P": [1, 2, 2, 2], "R": 5, "DN": 0, "phantom": 1,
+                    "MVN": -1, "MVT": -1, "inv": 1}]}
+    print("[host] pipe 24x12x12, R 5, MPC-AT, checkpoint at 20", flush=True)
+    (tmp_path / "pipe").mkdir()
+    info, rows = run_host(tmp_path / "pipe", pipe, "--every", "10")
+    Q = _ckpt_particles(tmp_path / "pipe" / "out" / "checkpoint.dat")
+    n = Q.shape[0]
+    assert abs(n - 10 * math.pi * 25 * 24) < 0.03 * 10 * math.pi * 25 * 24, n
+    r2 = (Q[:, 1] - 6) ** 2 + (Q[:, 2] - 6) ** 2
+    assert r2.max() <= 25 + 1e-4, r2.max()
+    assert rows[-1][3] > 0.0  # momentum along the pipe
+    k = 2 * math.pi / 20
+    wavy = {"domain": [40, 16], "domainWalls": 1, "simSteps": 20, "collOp": 0, "seed": 9, "checkpointOut": 20,
+            "species": [{"dens": 10}], "grav": [0.005, 0, 0],
+            "BC": [{"Q": [0, 2, 0], "aInv": [0, 1, 0], "P": [1, 1, 1, 1], "R": 0, "wavy": [0.5, k, 0], "DN": 0,
+                    "MVN": -1, "MVT": -1},
+                   {"Q": [0, 14, 0], "aInv": [0, -1, 0], "P": [1, 1, 1, 1], "R": 0, "wavy": [0.5, k, 0], "DN": 0,
+                    "MVN": -1, "MVT": -1}]}
+    print("[host] wavy channel 40x16, SRD, checkpoint at 20", flush=True)
+    (tmp_path / "wavy").mkdir()
+    run_host(tmp_path / "wavy", wavy, "--every", "10")
+    Q = _ckpt_particles(tmp_path / "wavy" / "out" / "checkpoint.dat")
+    lo = 2 - 0.5 * np.cos(k * Q[:, 0])
+    hi = 14 + 0.5 * np.cos(k * Q[:, 0])
+    assert (Q[:, 1] >= lo - 1e-4).all() and (Q[:, 1] <= hi + 1e-4).all()
