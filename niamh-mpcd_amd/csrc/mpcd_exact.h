@@ -339,7 +339,12 @@ __device__ __forceinline__ void xterms(double m, const float q[3], const float v
 }
 
 // the collision data of one cell from its exact sums, rounded to FP32:
-//   cd[0..2] centre-of-mass velocity (with ghost particles at phantom walls);
+//   cd[0..2] centre-of-mass velocity after the body force's kick g dt (with
+//   ghost particles at phantom walls); for SRD instead the offset
+//   b = vcm - M (vcm - g dt) of v' = M v + b.  The particles' velocities enter
+//   unkicked: the kick is added once per cell in FP64, not per particle in
+//   FP32, whose rounding of a fixed small increment would be biased by the
+//   binade of v;
 //   SRD: cd[3..11] the rotation matrix;
 //   MPC-AT: cd[3..5] the random-velocity sum RS = sum m v_rand / M;
 //   MPC-AT+a: also cd[6..8] the angular velocity W and cd[9..11] the centre
@@ -350,8 +355,10 @@ __device__ __forceinline__ void xcell_terms(const RepDev &R, const WallHot *__re
                                             int c, int np_, double mass, const double *s, float cd[12]) {
   // one division per cell: the sums' quantum over the cell mass
   const double im = (mass > 0.0) ? kXQinv / mass : 0.0;
-  double vcm[3] = {0.0, 0.0, 0.0};
-  for (int d = 0; d < DIM; d++) vcm[d] = s[d] * im;
+  double vcm[3] = {0.0, 0.0, 0.0}, dtg[3] = {0.0, 0.0, 0.0};
+  if (R.gravflag)
+    for (int d = 0; d < DIM; d++) dtg[d] = R.dt * R.grav[d];
+  for (int d = 0; d < DIM; d++) vcm[d] = s[d] * im + dtg[d];
   if (GHOST && R.anyphantom && np_ > 0) {
     // ghostPart (mpc.c:351-567): a cell cut by a phantom wall and holding
     // fewer than GnDNST particles gains the ghosts' momentum
@@ -414,6 +421,12 @@ __device__ __forceinline__ void xcell_terms(const RepDev &R, const WallHot *__re
       M[6] = 0.; M[7] = 0.; M[8] = 0.;
     }
     for (int k = 0; k < 9; k++) cd[3 + k] = (float)M[k];
+    // v' = vcm + M (v + g dt - vcm) = M v + b with the unkicked v
+    for (int d = 0; d < 3; d++) {
+      double b = vcm[d];
+      for (int k = 0; k < 3; k++) b -= M[3 * d + k] * (vcm[k] - dtg[k]);
+      cd[d] = (float)b;
+    }
     return;
   }
   double RS[3];
@@ -460,8 +473,9 @@ __device__ __forceinline__ void xcollide(const float cd[12], int np_, float damp
                                          float v[3]) {
   const float vc[3] = {cd[0], cd[1], cd[2]};
   if (OP == OP_SRD) {
+    // v' = M v + b (b in cd[0..2])
     const float *M = cd + 3;
-    const float d0 = v[0] - vc[0], d1 = v[1] - vc[1], d2 = v[2] - vc[2];
+    const float d0 = v[0], d1 = v[1], d2 = v[2];
     if (DIM == 3) {
       float t0 = M[0] * d0;
       t0 += M[1] * d1;
@@ -606,9 +620,6 @@ __global__ __launch_bounds__(kBktThreads) __attribute__((amdgpu_waves_per_eu(OP 
       s_id[tid + j * NT] = rid[j];
       if (!repaired) xshift_fwd<DIM>(X.L, X.sc, rq[j]);
       const int cn = xcell<DIM>(X.Li, X.pbc, X.cell_off, X.ny, X.nz, rq[j], true);
-      if (R.gravflag)
-#pragma unroll
-        for (int d = 0; d < DIM; d++) rv[j][d] = rv[j][d] + X.dtg[d];
       const int k = cn - T.c0;
       if ((unsigned)k < (unsigned)ncl) {
         rk[j] = k;
@@ -695,6 +706,10 @@ __global__ __launch_bounds__(kBktThreads) __attribute__((amdgpu_waves_per_eu(OP 
         r[2] = (DIM == 3) ? q[2] - (float)(lc_ % X.nz) : 0.0f;
       }
       xcollide<OP, DIM>(cd, np_, (float)sps[spid].damp, rvv, r, v);
+    } else if (R.gravflag) {
+      // a particle alone in its cell takes the kick itself
+#pragma unroll
+      for (int d = 0; d < DIM; d++) v[d] = v[d] + X.dtg[d];
     }
     ke += xke(m, X.avv, v);
     xshift_back<DIM>(X.L, X.sc, q);
@@ -952,8 +967,6 @@ __global__ __launch_bounds__(kBlock) void k_xd_sums(Ctx C, State<float> S, XDens
       cn = R.cell_off;
     }
   }
-  if (R.gravflag)
-    for (int d = 0; d < DIM; d++) v[d] = v[d] + (float)(R.dt * R.grav[d]);
   float rvv[3] = {0.0f, 0.0f, 0.0f};
   if (OP != OP_SRD) {
     xthermal<DIM>(Dr, B.rep, pid, sqrt(R.kbt / m), rvv);
@@ -1034,6 +1047,8 @@ __global__ __launch_bounds__(kBlock) void k_xd_part(Ctx C, State<float> S, XDens
         r[2] = (DIM == 3) ? q[2] - (float)(lc_ % R.nz) : 0.0f;
       }
       xcollide<OP, DIM>(cd, np_, (float)sps[spid].damp, rvv, r, v);
+    } else if (R.gravflag) {
+      for (int d = 0; d < DIM; d++) v[d] = v[d] + XR.dtg[d];
     }
     ke = xke(m, XR.avv, v);
     xshift_back<DIM>(XR.L, XR.sc, q);

@@ -3,7 +3,7 @@
  * input.json, sets up the particles and runs the reference's warm-up and
  * temporal loops (mpcd/mpcd.c:255-300) through the C-ABI of mpcd_gpu.h.
  *
- *   mpcd_host input.json [-o DIR] [--fp32] [--replicas N] [--every K] [--steps N] [--seed S] [--no-graph]
+ *   mpcd_host input.json [-o DIR] [--fp32 | --fp32x] [--replicas N] [--every K] [--steps N] [--seed S] [--no-graph]
  *             [--dump-params] [--write-checkpoint FILE]
  *
  * JSON keys and defaults follow readJson (mpcd/subroutines/read.c:973-1804)
@@ -292,7 +292,7 @@ static FILE *field_open(const char *outdir, const char *name, const char *what, 
     fprintf(f, "# replica %d of %d\n", r, nrep);
     fprintf(f, "# domain %d x %d x %d, %dD\n", L[0], L[1], dim == 3 ? L[2] : 1, dim);
     fprintf(f, "# output every %d steps\n", interval);
-    fprintf(f, "# particle state %s\n", fp32 ? "FP32, cell sums FP64" : "FP64");
+    fprintf(f, "# particle state %s\n", fp32 == 2 ? "FP32, exact FP64 cell sums" : fp32 ? "FP32, cell sums FP64" : "FP64");
     fprintf(f, "# columns: time, cell indices QX QY QZ, then the values\n");
     fprintf(f, "#\n#\n#\n\n");
     fprintf(f, "%s\n", columns);
@@ -467,6 +467,7 @@ int main(int argc, char **argv) {
     for (int a = 1; a < argc; a++) {
         if (!strcmp(argv[a], "-o") && a + 1 < argc) outdir = argv[++a];
         else if (!strcmp(argv[a], "--fp32")) fp32 = 1;
+        else if (!strcmp(argv[a], "--fp32x")) fp32 = 2; /* the exact-sum FP32 form */
         else if (!strcmp(argv[a], "--replicas") && a + 1 < argc) nrep = atoi(argv[++a]);
         else if (!strcmp(argv[a], "--every") && a + 1 < argc) every = atoi(argv[++a]);
         else if (!strcmp(argv[a], "--steps") && a + 1 < argc) steps_override = atoi(argv[++a]);
@@ -479,7 +480,7 @@ int main(int argc, char **argv) {
     }
     if (!input) {
         fprintf(stderr,
-                "usage: mpcd_host input.json [-o DIR] [--fp32] [--replicas N] [--every K] [--steps N] [--seed S] "
+                "usage: mpcd_host input.json [-o DIR] [--fp32 | --fp32x] [--replicas N] [--every K] [--steps N] [--seed S] "
                 "[--no-graph]\n");
         return 2;
     }
@@ -528,7 +529,7 @@ int main(int argc, char **argv) {
     double g[3] = {0, 0, 0};
     json_nums(J, "grav", g, 3);
     memcpy(p.grav, g, sizeof g);
-    p.precision = fp32 ? MPCD_FP32 : MPCD_FP64;
+    p.precision = fp32 == 2 ? MPCD_FP32_EXACT : fp32 ? MPCD_FP32 : MPCD_FP64;
     /* rest frame (read.c:1062-1063): rFrame (default 1) removes the net
      * momentum of the initial state; with zeroNetMom = Z > 0 the output step
      * of every runtime divisible by Z does it again */
@@ -762,7 +763,7 @@ int main(int argc, char **argv) {
     FILE *fo = fopen(path, "w");
     if (!fo) die("cannot write %s", path);
     fprintf(fo, "# step  KBT(replica 0)  KE  Px  Py  Pz   (%s state, %d replica%s of %lld particles)\n",
-            fp32 ? "FP32" : "FP64", nrep, nrep > 1 ? "s" : "", (long long)ntot);
+            fp32 == 2 ? "FP32 exact-sum" : fp32 ? "FP32" : "FP64", nrep, nrep > 1 ? "s" : "", (long long)ntot);
     /* steps of the temporal loop: runtime run0 .. simSteps */
     const int total = sim_steps + 1 - run0 > 0 ? sim_steps + 1 - run0 : 0;
     if (every <= 0) every = total > 10 ? total / 10 : 1;

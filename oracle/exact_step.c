@@ -95,8 +95,6 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
             bad++;
             continue;
         }
-        if (gravflag)
-            for (int d = 0; d < dim; d++) w[d] = w[d] + dtg[d];
         float rv[3] = {0.0f, 0.0f, 0.0f};
         if (op != 0) {
             const double sig = sqrt(p->kbt / m);
@@ -145,7 +143,12 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
         if (onesp)
             for (int j = 0; j < (op == 0 ? 3 : ns); j++) sm[j] *= p->mass[0];
         const double im = XQINV / mass; /* mass > 0: the cell holds a particle */
-        for (int d = 0; d < dim; d++) vcm[d] = sm[d] * im;
+        /* the body force's kick enters once per cell, in FP64: o[0..2] the
+           kicked centre-of-mass velocity (SRD: the offset b of v' = M v + b) */
+        double g[3] = {0.0, 0.0, 0.0};
+        if (gravflag)
+            for (int d = 0; d < dim; d++) g[d] = p->dt * p->grav[d];
+        for (int d = 0; d < dim; d++) vcm[d] = sm[d] * im + g[d];
         for (int d = 0; d < 3; d++) o[d] = (float)vcm[d];
         if (np_ <= 1) continue;
         if (op == 0) {
@@ -185,6 +188,12 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
                 memcpy(M, m3, sizeof m3);
             }
             for (int k = 0; k < 9; k++) o[3 + k] = (float)M[k];
+            /* v' = vcm + M (v + g dt - vcm) = M v + b with the unkicked v */
+            for (int d = 0; d < 3; d++) {
+                double b = vcm[d];
+                for (int k = 0; k < 3; k++) b -= M[3 * d + k] * (vcm[k] - g[k]);
+                o[d] = (float)b;
+            }
             continue;
         }
         for (int d = 0; d < 3; d++) o[3 + d] = (float)((d < dim) ? sm[3 + d] * im : 0.0);
@@ -231,8 +240,9 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
             const float *o = &cd[(size_t)c * 12];
             const float vc[3] = {o[0], o[1], o[2]};
             if (op == 0) {
+                /* v' = M v + b */
                 const float *M = o + 3;
-                const float d0 = w[0] - vc[0], d1 = w[1] - vc[1], d2 = w[2] - vc[2];
+                const float d0 = w[0], d1 = w[1], d2 = w[2];
                 if (dim == 3) {
                     float t0 = M[0] * d0;
                     t0 += M[1] * d1;
@@ -267,6 +277,9 @@ int orc_exact_step(const orc_xcfg *p, int n, float *x, float *v, const int *sp, 
                 const float damp = (float)p->damp[spid];
                 for (int d = 0; d < dim; d++) w[d] = vc[d] + rt[3 * i + d] - o[3 + d] - damp * vc[d] + at[d];
             }
+        } else if (gravflag) {
+            /* a particle alone in its cell takes the kick itself */
+            for (int d = 0; d < dim; d++) w[d] = w[d] + dtg[d];
         }
         {
             const float a = w[0] - avv[0], b = w[1] - avv[1], cz = w[2] - avv[2];

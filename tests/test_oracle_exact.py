@@ -209,3 +209,23 @@ def test_normal_turn_matches_library_normal():
             a = L.orc_philox_normal(5, 0, 7, P.P_NORMAL, key, k)
             b = L.orc_philox_normal_turn(5, 0, 7, P.P_NORMAL, key, k)
             assert abs(a - b) < 1e-13 * max(1.0, abs(a)), (key, k, a, b)
+
+
+def test_body_force_sets_the_momentum():
+    """in a periodic box the collisions conserve momentum, so after k steps
+    the body force has added N m g dt k, up to the unbiased FP32 rounding of
+    the collisions (a random walk: about 5e-4 after 240 steps here, the same
+    along y, where no force acts).  The kick enters once per cell in FP64;
+    added per particle in FP32 (a fixed small increment) its rounding would
+    depend only on the binade of v and fall 1.1e-2 short here."""
+    c = cfg(dim=2, L=(30, 20), grav=(0.0005, 0.0, 0.0))
+    x, v, _ = state(c, ppc=10, seed=3)
+    p0 = v.astype(np.float64).sum(axis=0)
+    ids = np.arange(x.shape[0], dtype=np.int32)
+    k = 240
+    for s in range(k):
+        x, v, _ = P.exact_step(c, x, v, seed=5, step=s, ids=ids)
+    dp = v.astype(np.float64).sum(axis=0) - p0
+    expect = x.shape[0] * 0.0005 * 0.1 * k
+    assert abs(dp[0] - expect) < 2.5e-3, (dp[0], expect)
+    assert abs(dp[1]) < 2.5e-3
