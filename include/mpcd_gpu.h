@@ -262,6 +262,17 @@ mpcd_status mpcd_gpu_set_layout(mpcd_gpu_ctx *ctx, int bucketed, double fill, in
 mpcd_status mpcd_gpu_layout_stats(const mpcd_gpu_ctx *ctx, int64_t *out);
 mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *ctx);
 
+/* The draw counter (no counterpart in the reference, whose generator state a
+ * checkpoint does not keep, so its resumes do not repeat a run): every draw
+ * is keyed by (seed, replica, step, ...), so a state saved with the step
+ * mpcd_gpu_get_step reports and uploaded into a context of the same seed
+ * followed by mpcd_gpu_set_step continues the same run.  In the exact-sum
+ * form (any particle order) the continuation is bit-identical; the list-order
+ * forms bin the uploaded order afresh.  set_step discards a pending binning
+ * and is refused while a tape or a tangent runs. */
+mpcd_status mpcd_gpu_get_step(const mpcd_gpu_ctx *ctx, uint32_t *step);
+mpcd_status mpcd_gpu_set_step(mpcd_gpu_ctx *ctx, uint32_t step);
+
 /* Batched-form tuning (no counterpart in the reference, which has no tiles):
  * runs steps_per_trial real steps (plus one untimed step) for each candidate
  * fill of the layout mpcd_gpu_run uses (six bucket fills, 0.4 to 0.8, of the

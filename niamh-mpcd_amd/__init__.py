@@ -123,7 +123,8 @@ EXPORTS = ["mpcd_gpu_create", "mpcd_gpu_destroy", "mpcd_gpu_last_error", "mpcd_g
            "mpcd_gpu_adjoint_reset", "mpcd_gpu_step_backward", "mpcd_gpu_set_layout",
            "mpcd_gpu_layout_stats", "mpcd_gpu_tangent", "mpcd_gpu_tangent_download", "mpcd_gpu_upload_aos",
            "mpcd_gpu_download_aos", "mpcd_gpu_flow", "mpcd_gpu_flow_read", "mpcd_gpu_lc_record",
-           "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc", "mpcd_gpu_kbtnow",
+           "mpcd_gpu_lc_decisions", "mpcd_gpu_bin_noalloc", "mpcd_gpu_kbtnow", "mpcd_gpu_get_step",
+           "mpcd_gpu_set_step",
            "mpcd_gpu_zero_net_momentum", "mpcd_gpu_step_backward_state", "mpcd_gpu_observable"]
 
 NTHETA = 4 + MAXSPEC  # MPCD_NTHETA: parameter adjoints per replica
@@ -153,6 +154,8 @@ def lib():
     L.mpcd_gpu_step.argtypes = [vp, C.POINTER(DrawsC), dp]
     L.mpcd_gpu_run.argtypes = [vp, C.c_int, C.c_int]
     L.mpcd_gpu_synchronize.argtypes = [vp]
+    L.mpcd_gpu_get_step.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.mpcd_gpu_set_step.argtypes = [vp, C.c_uint32]
     L.mpcd_gpu_get_cells.argtypes = [vp, C.c_int, ip, dp, dp, dp]
     L.mpcd_gpu_lc_record.argtypes = [vp, C.c_int]
     L.mpcd_gpu_kbtnow.argtypes = [vp, dp]
@@ -371,6 +374,16 @@ class Context:
 
     def run(self, nsteps, use_graph=True):
         self._check(self.L.mpcd_gpu_run(self.h, int(nsteps), int(bool(use_graph))), "run")
+
+    def get_step(self):
+        """the draw counter (steps taken since the context began, or since set_step)"""
+        out = C.c_uint32(0)
+        self._check(self.L.mpcd_gpu_get_step(self.h, C.byref(out)), "get_step")
+        return int(out.value)
+
+    def set_step(self, step):
+        """continue a saved run: the draws of the next step are those of `step`"""
+        self._check(self.L.mpcd_gpu_set_step(self.h, int(step)), "set_step")
 
     def synchronize(self):
         self._check(self.L.mpcd_gpu_synchronize(self.h), "synchronize")

@@ -2927,6 +2927,29 @@ extern "C" mpcd_status mpcd_gpu_synchronize(mpcd_gpu_ctx *c) {
   return check_flags(c);
 }
 
+extern "C" mpcd_status mpcd_gpu_get_step(const mpcd_gpu_ctx *c, uint32_t *step) {
+  if (!c || !step) return MPCD_EINVAL;
+  *step = c->step;
+  return MPCD_OK;
+}
+
+// the draw counter: every draw is keyed by (seed, replica, step, ...), so a
+// state saved with its step and restored with it continues the same run
+extern "C" mpcd_status mpcd_gpu_set_step(mpcd_gpu_ctx *c, uint32_t step) {
+  if (!c) return MPCD_EINVAL;
+  if (c->recording || c->tan_on) return fail(c, MPCD_EUNSUPPORTED, "set_step: not while a tape or tangent runs");
+  mpcd_status st = ensure_init(c);
+  if (st) return st;
+  st = ensure_dense(c);
+  if (st) return st;
+  c->step = step;
+  c->pending = false;  // the next step bins the state afresh at this step
+  c->hist_ready = false;
+  c->x_tables = step ^ 0x80000000u;
+  drop_bucket_graph(c);
+  return check_launch(c);
+}
+
 extern "C" mpcd_status mpcd_gpu_kbtnow(mpcd_gpu_ctx *c, double *kbt_now) {
   if (!c || !kbt_now) return c ? fail(c, MPCD_EINVAL, "kbtnow: null argument") : MPCD_EINVAL;
   return read_kbt(c, kbt_now);

@@ -40,10 +40,83 @@ static void skip(rd_t *r, int n) {
         return -1;                             \
     } while (0)
 
+static const char kBinMagic[8] = {'M', 'P', 'C', 'D', 'C', 'K', 'B', '1'};
+
+void ckpt_write_bin(FILE *f, const mpcd_gpu_params *p, const mpcd_ckpt *ck) {
+    const int32_t iv[11] = {ck->sim_steps, ck->warm_steps, ck->runtime, ck->warmtime, ck->flow_out, ck->vel_out,
+                            ck->dir_out,   ck->ckpt_out,   ck->dens_out, ck->rframe, ck->zero_net_mom};
+    const int64_t seed = ck->seed;
+    fwrite(kBinMagic, 1, sizeof kBinMagic, f);
+    fwrite(p, sizeof *p, 1, f);
+    fwrite(iv, sizeof iv, 1, f);
+    fwrite(&seed, sizeof seed, 1, f);
+    fwrite(&ck->kbtnow, sizeof(double), 1, f);
+    fwrite(ck->avnow, sizeof(double), 3, f);
+    fwrite(&ck->lib_step, sizeof(uint32_t), 1, f);
+    fwrite(&ck->n, sizeof(int64_t), 1, f);
+    fwrite(ck->Q, sizeof(double), 3 * (size_t)ck->n, f);
+    fwrite(ck->V, sizeof(double), 3 * (size_t)ck->n, f);
+    fwrite(ck->U, sizeof(double), 3 * (size_t)ck->n, f);
+    fwrite(ck->spid, sizeof(int32_t), (size_t)ck->n, f);
+}
+
+static int ckpt_read_bin(FILE *f, mpcd_gpu_params *p, mpcd_ckpt *ck, char *err, size_t errlen) {
+    int32_t iv[11];
+    int64_t seed;
+    int ok = fread(p, sizeof *p, 1, f) == 1 && fread(iv, sizeof iv, 1, f) == 1 && fread(&seed, sizeof seed, 1, f) == 1 &&
+             fread(&ck->kbtnow, sizeof(double), 1, f) == 1 && fread(ck->avnow, sizeof(double), 3, f) == 3 &&
+             fread(&ck->lib_step, sizeof(uint32_t), 1, f) == 1 && fread(&ck->n, sizeof(int64_t), 1, f) == 1;
+    if (!ok || ck->n <= 0 || ck->n >= (1ll << 31) || p->nspec < 1 || p->nspec > MPCD_MAXSPEC || p->nwall < 0 ||
+        p->nwall > MPCD_MAXWALL) {
+        snprintf(err, errlen, "binary checkpoint: the header is cut short or out of range");
+        return -1;
+    }
+    ck->sim_steps = iv[0];
+    ck->warm_steps = iv[1];
+    ck->runtime = iv[2];
+    ck->warmtime = iv[3];
+    ck->flow_out = iv[4];
+    ck->vel_out = iv[5];
+    ck->dir_out = iv[6];
+    ck->ckpt_out = iv[7];
+    ck->dens_out = iv[8];
+    ck->rframe = iv[9];
+    ck->zero_net_mom = iv[10];
+    ck->seed = (long)seed;
+    ck->binary = 1;
+    const size_t n = (size_t)ck->n;
+    ck->Q = (double *)calloc(3 * n, sizeof(double));
+    ck->V = (double *)calloc(3 * n, sizeof(double));
+    ck->U = (double *)calloc(3 * n, sizeof(double));
+    ck->spid = (int32_t *)calloc(n, sizeof(int32_t));
+    if (!ck->Q || !ck->V || !ck->U || !ck->spid) {
+        snprintf(err, errlen, "binary checkpoint: out of host memory");
+        return -1;
+    }
+    if (fread(ck->Q, sizeof(double), 3 * n, f) != 3 * n || fread(ck->V, sizeof(double), 3 * n, f) != 3 * n ||
+        fread(ck->U, sizeof(double), 3 * n, f) != 3 * n || fread(ck->spid, sizeof(int32_t), n, f) != n) {
+        snprintf(err, errlen, "binary checkpoint: the particle arrays are cut short");
+        return -1;
+    }
+    for (size_t i = 0; i < n; i++)
+        if (ck->spid[i] < 0 || ck->spid[i] >= p->nspec) {
+            snprintf(err, errlen, "binary checkpoint: particle %zu has species %d", i, (int)ck->spid[i]);
+            return -1;
+        }
+    return 0;
+}
+
 int ckpt_read(const char *path, mpcd_gpu_params *p, mpcd_ckpt *ck, char *err, size_t errlen) {
     memset(ck, 0, sizeof *ck);
-    FILE *f = fopen(path, "r");
+    FILE *f = fopen(path, "rb");
     if (!f) FAIL("cannot open the checkpoint %s", path);
+    char magic[8] = {0};
+    if (fread(magic, 1, sizeof magic, f) == sizeof magic && !memcmp(magic, kBinMagic, sizeof magic)) {
+        const int r = ckpt_read_bin(f, p, ck, err, errlen);
+        fclose(f);
+        return r;
+    }
+    rewind(f);
     rd_t R = {f, 1};
     double v[32];
     memset(p, 0, sizeof *p);

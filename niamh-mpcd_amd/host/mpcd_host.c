@@ -4,7 +4,7 @@
  * temporal loops (mpcd/mpcd.c:255-300) through the C-ABI of mpcd_gpu.h.
  *
  *   mpcd_host input.json [-o DIR] [--fp32 | --fp32x] [--replicas N] [--every K] [--steps N] [--seed S] [--no-graph]
- *             [--dump-params] [--write-checkpoint FILE]
+ *             [--dump-params] [--write-checkpoint FILE] [--binary-checkpoint]
  *
  * JSON keys and defaults follow readJson (mpcd/subroutines/read.c:973-1804)
  * for the options of this path: domain, kbt, dt, simSteps, warmUp, galInv,
@@ -29,6 +29,10 @@
  * checkpointOut = F writes DIR/checkpoint.dat after the outputs of every
  * runtime divisible by F (mpcd.c:310-312; replica 0).  --write-checkpoint
  * FILE writes the state the run would start from and exits without the GPU.
+ * --binary-checkpoint writes DIR/checkpoint.bin (and --write-checkpoint's
+ * FILE) in the lossless binary layout instead, with the library's draw
+ * counter; "checkpointIn" reads either layout, and a binary one continues
+ * the run's draws (bit for bit in the exact-sum form, --fp32x).
  *
  * Output: DIR/solvent.dat with one line per output step: the temporal
  * loop's step counter (runtime, 0..simSteps, mpcd.c:291), the
@@ -322,9 +326,11 @@ static double temp_of(const mpcd_gpu_params *p, size_t n, const double *V, const
 static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int warm_steps, int runtime, int warmtime,
                            long seed, int flow_out, int vel_out, int dir_out, int ckpt_out, int dens_out, int rframe,
                            int znm, size_t n, const double *Q,
-                           const double *V, const double *U, const int32_t *spid, double kbtnow) {
+                           const double *V, const double *U, const int32_t *spid, double kbtnow, int binary,
+                           uint32_t lib_step) {
     mpcd_ckpt ck;
     memset(&ck, 0, sizeof ck);
+    ck.lib_step = lib_step;
     ck.sim_steps = sim_steps;
     ck.warm_steps = warm_steps;
     ck.runtime = runtime;
@@ -350,7 +356,8 @@ static void checkpoint_out(FILE *f, const mpcd_gpu_params *p, int sim_steps, int
     }
     ck.kbtnow = kbtnow;  /* TEMP after the thermostat, mpc.c:4752-4760 */
     for (int d = 0; d < 3; d++) ck.avnow[d] = mtot > 0.0 ? mom[d] / mtot : 0.0;
-    ckpt_write(f, p, &ck);
+    if (binary) ckpt_write_bin(f, p, &ck);
+    else ckpt_write(f, p, &ck);
 }
 
 static void reject_unsupported(const json *J) {
@@ -463,11 +470,13 @@ int main(int argc, char **argv) {
     const char *input = NULL, *outdir = ".";
     int fp32 = 0, nrep = 1, every = 0, steps_override = -1, use_graph = 1, dump = 0;
     const char *ckpt_now = NULL;
+    int ckpt_bin = 0;
     long seed_override = -1;
     for (int a = 1; a < argc; a++) {
         if (!strcmp(argv[a], "-o") && a + 1 < argc) outdir = argv[++a];
         else if (!strcmp(argv[a], "--fp32")) fp32 = 1;
         else if (!strcmp(argv[a], "--fp32x")) fp32 = 2; /* the exact-sum FP32 form */
+        else if (!strcmp(argv[a], "--binary-checkpoint")) ckpt_bin = 1;
         else if (!strcmp(argv[a], "--replicas") && a + 1 < argc) nrep = atoi(argv[++a]);
         else if (!strcmp(argv[a], "--every") && a + 1 < argc) every = atoi(argv[++a]);
         else if (!strcmp(argv[a], "--steps") && a + 1 < argc) steps_override = atoi(argv[++a]);
@@ -606,7 +615,10 @@ int main(int argc, char **argv) {
         dim = p.dim;
         sim_steps = ck.sim_steps;
         warm = 0;
-        run0 = ck.runtime;
+        /* the reference's file re-runs its runtime's step (mpcd.c:257, 290);
+           a binary checkpoint, which carries the draw counter, continues
+           with the next one */
+        run0 = ck.binary ? ck.runtime + 1 : ck.runtime;
         warmtime = ck.warmtime;
         flow_out = ck.flow_out;
         vel_out = ck.vel_out;
@@ -740,21 +752,23 @@ int main(int argc, char **argv) {
     p.gndnst = pr[0].gndnst;
     if (ckpt_now) {
         /* the state the run would start from, as a checkpoint; no GPU */
-        FILE *fc = fopen(ckpt_now, "w");
+        FILE *fc = fopen(ckpt_now, ckpt_bin ? "wb" : "w");
         if (!fc) die("cannot write %s", ckpt_now);
         checkpoint_out(fc, &p, sim_steps, warm, run0, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, rframe,
                        znm, n, Q, V, U,
-                       spid, temp_of(&p, n, V, spid));
+                       spid, temp_of(&p, n, V, spid), ckpt_bin, ck.lib_step);
         fclose(fc);
         ckpt_free(&ck);
         return 0;
     }
     mpcd_gpu_ctx *ctx = NULL;
     check(NULL, mpcd_gpu_create(pr, nrep, nr, (uint64_t)seed, &ctx), "mpcd_gpu_create");
+    const int resume_step = ck.binary; /* a binary checkpoint carries the draw counter */
     for (int r = 0; r < nrep; r++)
         check(ctx, mpcd_gpu_upload(ctx, r, Q + 3 * n * r, V + 3 * n * r, p.lc ? U + 3 * n * r : NULL,
                                    p.nspec > 1 ? spid : NULL),
               "mpcd_gpu_upload");
+    if (resume_step) check(ctx, mpcd_gpu_set_step(ctx, ck.lib_step), "mpcd_gpu_set_step");
 
     /* the reference's loops: warmUp + 1 and simSteps + 1 steps, outputs in between */
     char path[4096];
@@ -888,12 +902,14 @@ int main(int argc, char **argv) {
              * the warm-up count is 0 once its loop has run (mpcd.c:278) */
             check(ctx, mpcd_gpu_download(ctx, 0, Q, V, p.lc ? U : NULL), "mpcd_gpu_download");
             char cpath[4096];
-            snprintf(cpath, sizeof cpath, "%s/checkpoint.dat", outdir);
-            FILE *fc = fopen(cpath, "w");
+            snprintf(cpath, sizeof cpath, "%s/checkpoint.%s", outdir, ckpt_bin ? "bin" : "dat");
+            FILE *fc = fopen(cpath, ckpt_bin ? "wb" : "w");
             if (!fc) die("cannot write %s", cpath);
+            uint32_t lstep = 0;
+            check(ctx, mpcd_gpu_get_step(ctx, &lstep), "mpcd_gpu_get_step");
             checkpoint_out(fc, &p, sim_steps, 0, rt, warmtime, seed, flow_out, vel_out, dir_out, ckpt_out, dens_out, rframe,
                            znm, n, Q, V, U,
-                           spid, kbt_now[0]);
+                           spid, kbt_now[0], ckpt_bin, lstep);
             fclose(fc);
         }
         if (done < next_out) continue;

@@ -621,3 +621,63 @@ def test_host_runs_curved_walls(tmp_path):
     lo = 2 - 0.5 * np.cos(k * Q[:, 0])
     hi = 14 + 0.5 * np.cos(k * Q[:, 0])
     assert (Q[:, 1] >= lo - 1e-4).all() and (Q[:, 1] <= hi + 1e-4).all()
+
+
+def _bin_particles(path, n):
+    """the particle arrays at the end of a binary checkpoint: Q, V, U (FP64) and the species"""
+    raw = open(path, "rb").read()
+    tail = n * (9 * 8 + 4)
+    return raw[-tail:]
+
+
+def test_binary_checkpoint_round_trip(tmp_path):
+    """--binary-checkpoint writes the lossless layout (magic, parameters, loop
+    state, draw counter, FP64 particles); reading it back and writing it again
+    gives the same particle bytes and the same parameters"""
+    cfg = {"domain": [10, 8], "collOp": 3, "seed": 4, "species": [{"pop": 400, "mass": 1.0}, {"pop": 300, "mass": 2.0}],
+           "BC": [{"Q": [5, 4, 0], "aInv": [1, 1, 0], "P": [2, 2, 1, 2], "R": 2, "DN": 0, "MVN": -1, "MVT": -1}]}
+    f = tmp_path / "in.json"
+    f.write_text(json.dumps(cfg))
+    a, b = tmp_path / "a.bin", tmp_path / "b.bin"
+    r = subprocess.run([host(), str(f), "--write-checkpoint", str(a), "--binary-checkpoint"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert open(a, "rb").read(8) == b"MPCDCKB1"
+    g = tmp_path / "in2.json"
+    g.write_text(json.dumps({"checkpointIn": str(a)}))
+    r = subprocess.run([host(), str(g), "--write-checkpoint", str(b), "--binary-checkpoint"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert _bin_particles(a, 700) == _bin_particles(b, 700)
+    d1 = dump(tmp_path, cfg)
+    d2 = json.loads(subprocess.run([host(), str(g), "--dump-params"], capture_output=True, text=True,
+                                   timeout=60).stdout)
+    assert d1["walls"] == d2["walls"] and [s["mass"] for s in d1["species"]] == [s["mass"] for s in d2["species"]]
+    assert [s["pop"] for s in d2["species"]] == [400, 300]
+    # a binary file cut short is refused with a message
+    c = tmp_path / "c.bin"
+    c.write_bytes(open(a, "rb").read()[:-100])
+    g.write_text(json.dumps({"checkpointIn": str(c)}))
+    r = subprocess.run([host(), str(g), "--dump-params"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "cut short" in r.stderr
+
+
+@pytest.mark.gpu
+def test_binary_checkpoint_resume_is_bit_exact(tmp_path):
+    """in the exact-sum form a run resumed from a binary checkpoint (state and
+    draw counter) ends on the same bits as the run that went straight
+    through: 21 steps, against 11 steps, a checkpoint and 10 more"""
+    cfg = {"domain": [12, 10, 8], "collOp": 2, "seed": 21, "grav": [0.002, 0, 0], "simSteps": 20, "checkpointOut": 10,
+           "species": [{"dens": 8}]}
+    print("[host] binary checkpoint resume, MPC-AT 12x10x8, exact-sum form", flush=True)
+    full, half, res = tmp_path / "full", tmp_path / "half", tmp_path / "res"
+    for d in (full, half, res):
+        d.mkdir()
+    run_host(full, cfg, "--fp32x", "--binary-checkpoint", "--every", "10")
+    run_host(half, dict(cfg, simSteps=10), "--fp32x", "--binary-checkpoint", "--every", "10")
+    run_host(res, {"checkpointIn": str(half / "out" / "checkpoint.bin")}, "--fp32x", "--binary-checkpoint",
+             "--steps", "20", "--every", "10")
+    n = 12 * 10 * 8 * 8
+    a = _bin_particles(full / "out" / "checkpoint.bin", n)
+    b = _bin_particles(res / "out" / "checkpoint.bin", n)
+    assert a == b
